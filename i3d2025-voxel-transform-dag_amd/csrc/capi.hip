@@ -1,0 +1,237 @@
+// C-ABI of the matching path (include/tdag.h): contexts, error reporting, matcher calls.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "errors.hpp"
+#include "matcher.hpp"
+#include "tdag.h"
+
+struct tdag_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    void* scratch = nullptr;
+    size_t scratchBytes = 0;
+    // Host-buffer entry points stage their inputs/outputs here.
+    void* io = nullptr;
+    size_t ioBytes = 0;
+    uint64_t* pinned = nullptr; // 4 words of pinned host memory for small readbacks
+    hipEvent_t ev[4] = { nullptr, nullptr, nullptr, nullptr };
+    tdag_match_stats last {};
+};
+
+namespace {
+
+thread_local std::string g_lastError;
+
+tdag_status fail(tdag_status s, const std::string& msg)
+{
+    g_lastError = msg;
+    return s;
+}
+
+tdag_status failHip(hipError_t e, const char* what)
+{
+    g_lastError = std::string(what) + ": " + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ")";
+    return e == hipErrorOutOfMemory ? TDAG_ENOMEM : TDAG_EDEVICE;
+}
+
+#define CAPI_HIP(expr, what)                 \
+    do {                                     \
+        const hipError_t e_ = (expr);        \
+        if (e_ != hipSuccess)                \
+            return failHip(e_, what);        \
+    } while (0)
+
+tdag_status ensureBuffer(void*& buf, size_t& have, size_t need, const char* what)
+{
+    if (have >= need && buf)
+        return TDAG_OK;
+    if (buf) {
+        (void)hipFree(buf);
+        buf = nullptr;
+        have = 0;
+    }
+    const size_t bytes = std::max<size_t>(need, 4096);
+    CAPI_HIP(hipMalloc(&buf, bytes), what);
+    have = bytes;
+    return TDAG_OK;
+}
+
+uint32_t cfgBitsOf(const tdag_config* cfg)
+{
+    return (cfg->symmetry ? tdag::kCfgSymmetry : 0u) | (cfg->axis_permutation ? tdag::kCfgAxis : 0u) | (cfg->translation ? tdag::kCfgTranslation : 0u);
+}
+
+tdag_status checkArgs(tdag_context* ctx, uint32_t level, uint64_t n, const tdag_config* cfg)
+{
+    if (!ctx)
+        return fail(TDAG_EINVAL, "context is null");
+    if (!cfg)
+        return fail(TDAG_EINVAL, "config is null");
+    if (level < 2 || level > 5)
+        return fail(TDAG_EINVAL, "level " + std::to_string(level) + " is outside the supported range [2, 5]");
+    if (n >= (1ull << 28))
+        return fail(TDAG_EINVAL, "item count " + std::to_string(n) + " exceeds the supported maximum 2^28 - 1");
+    return TDAG_OK;
+}
+
+} // namespace
+
+namespace tdag {
+tdag_status setError(tdag_status status, const std::string& message)
+{
+    g_lastError = message;
+    return status;
+}
+} // namespace tdag
+
+extern "C" {
+
+size_t tdag_last_error(char* buf, size_t cap)
+{
+    if (buf && cap > 0) {
+        const size_t k = std::min(cap - 1, g_lastError.size());
+        std::memcpy(buf, g_lastError.data(), k);
+        buf[k] = '\0';
+    }
+    return g_lastError.size();
+}
+
+const char* tdag_version(void) { return "tdag-mi355x 0.1 gfx950"; }
+
+tdag_status tdag_context_create(int device, tdag_context** out_ctx)
+{
+    if (!out_ctx)
+        return fail(TDAG_EINVAL, "out_ctx is null");
+    *out_ctx = nullptr;
+    int count = 0;
+    CAPI_HIP(hipGetDeviceCount(&count), "hipGetDeviceCount");
+    if (device < 0 || device >= count)
+        return fail(TDAG_EDEVICE, "HIP device " + std::to_string(device) + " not present (" + std::to_string(count) + " devices)");
+    CAPI_HIP(hipSetDevice(device), "hipSetDevice");
+    tdag_context* ctx = new tdag_context();
+    ctx->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess)
+        e = hipHostMalloc((void**)&ctx->pinned, 64, hipHostMallocDefault);
+    for (int i = 0; i < 4 && e == hipSuccess; ++i)
+        e = hipEventCreate(&ctx->ev[i]);
+    if (e != hipSuccess) {
+        tdag_context_destroy(ctx);
+        return failHip(e, "context creation");
+    }
+    *out_ctx = ctx;
+    return TDAG_OK;
+}
+
+tdag_status tdag_context_destroy(tdag_context* ctx)
+{
+    if (!ctx)
+        return TDAG_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream)
+        (void)hipStreamSynchronize(ctx->stream);
+    for (int i = 0; i < 4; ++i)
+        if (ctx->ev[i])
+            (void)hipEventDestroy(ctx->ev[i]);
+    if (ctx->scratch)
+        (void)hipFree(ctx->scratch);
+    if (ctx->io)
+        (void)hipFree(ctx->io);
+    if (ctx->pinned)
+        (void)hipHostFree(ctx->pinned);
+    if (ctx->stream)
+        (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return TDAG_OK;
+}
+
+tdag_status tdag_match_level_device(tdag_context* ctx, uint32_t level, const uint64_t* d_masks, uint64_t n, const tdag_config* cfg,
+    uint64_t* d_encoded, uint64_t* d_mapping, uint32_t* d_unique, uint64_t* out_num_unique, void* stream, tdag_match_stats* stats)
+{
+    if (tdag_status s = checkArgs(ctx, level, n, cfg); s != TDAG_OK)
+        return s;
+    if (n > 0 && !d_masks)
+        return fail(TDAG_EINVAL, "d_masks is null");
+    CAPI_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+    const tdag::MatchPlan plan = tdag::planMatch(level, n);
+    if (tdag_status s = ensureBuffer(ctx->scratch, ctx->scratchBytes, plan.scratchBytes, "matcher scratch allocation"); s != TDAG_OK)
+        return s;
+    CAPI_HIP(hipEventRecord(ctx->ev[0], st), "hipEventRecord");
+    CAPI_HIP(tdag::enqueueMatch(plan, d_masks, cfgBitsOf(cfg), ctx->scratch, st, d_encoded, d_mapping, d_unique, ctx->ev[1], ctx->ev[2]), "matcher launch");
+    CAPI_HIP(hipEventRecord(ctx->ev[3], st), "hipEventRecord");
+    const bool needSync = out_num_unique || stats;
+    if (needSync) {
+        CAPI_HIP(hipMemcpyAsync(ctx->pinned, (char*)ctx->scratch + plan.offCounters, 16, hipMemcpyDeviceToHost, st), "counter readback");
+        CAPI_HIP(hipStreamSynchronize(st), "hipStreamSynchronize");
+        tdag_match_stats s {};
+        s.num_items = n;
+        s.num_matched = ctx->pinned[0];
+        s.num_unique = n > 0 ? (uint32_t)(ctx->pinned[1] & 0xFFFFFFFFull) : 0;
+        s.table_entries = 0;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[3]) == hipSuccess)
+            s.ms_total = ms;
+        if (hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]) == hipSuccess)
+            s.ms_search = ms;
+        ctx->last = s;
+        if (out_num_unique)
+            *out_num_unique = s.num_unique;
+        if (stats)
+            *stats = s;
+    }
+    return TDAG_OK;
+}
+
+tdag_status tdag_match_level(tdag_context* ctx, uint32_t level, const uint64_t* masks, uint64_t n, const tdag_config* cfg,
+    uint64_t* out_encoded, uint64_t* out_mapping, uint32_t* out_unique, uint64_t* out_num_unique, tdag_match_stats* stats)
+{
+    if (tdag_status s = checkArgs(ctx, level, n, cfg); s != TDAG_OK)
+        return s;
+    if (n > 0 && (!masks || !out_mapping || !out_unique))
+        return fail(TDAG_EINVAL, "masks, out_mapping and out_unique are required");
+    CAPI_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    const tdag::MatchPlan plan = tdag::planMatch(level, n);
+    const size_t maskBytes = (size_t)n * plan.W * 8;
+    const size_t a = (maskBytes + 255) / 256 * 256;
+    const size_t b = ((size_t)n * 8 + 255) / 256 * 256;
+    const size_t c = ((size_t)n * 4 + 255) / 256 * 256;
+    if (tdag_status s = ensureBuffer(ctx->io, ctx->ioBytes, a + 2 * b + c, "io buffer allocation"); s != TDAG_OK)
+        return s;
+    uint64_t* dMasks = (uint64_t*)ctx->io;
+    uint64_t* dEncoded = (uint64_t*)((char*)ctx->io + a);
+    uint64_t* dMapping = (uint64_t*)((char*)ctx->io + a + b);
+    uint32_t* dUnique = (uint32_t*)((char*)ctx->io + a + 2 * b);
+    if (n > 0)
+        CAPI_HIP(hipMemcpyAsync(dMasks, masks, maskBytes, hipMemcpyHostToDevice, ctx->stream), "mask upload");
+    uint64_t numUnique = 0;
+    tdag_match_stats st {};
+    if (tdag_status s = tdag_match_level_device(ctx, level, dMasks, n, cfg, out_encoded ? dEncoded : nullptr, dMapping, dUnique, &numUnique, ctx->stream, &st); s != TDAG_OK)
+        return s;
+    if (n > 0) {
+        if (out_encoded)
+            CAPI_HIP(hipMemcpyAsync(out_encoded, dEncoded, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream), "encoded readback");
+        CAPI_HIP(hipMemcpyAsync(out_mapping, dMapping, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream), "mapping readback");
+        CAPI_HIP(hipMemcpyAsync(out_unique, dUnique, (size_t)numUnique * 4, hipMemcpyDeviceToHost, ctx->stream), "unique readback");
+        CAPI_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    }
+    if (out_num_unique)
+        *out_num_unique = numUnique;
+    if (stats)
+        *stats = st;
+    return TDAG_OK;
+}
+
+tdag_status tdag_last_stats(const tdag_context* ctx, tdag_match_stats* out)
+{
+    if (!ctx || !out)
+        return fail(TDAG_EINVAL, "null argument");
+    *out = ctx->last;
+    return TDAG_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,276 @@
+// constructTransformDAG driver and the pipeline entry points of the C-ABI (host code).
+//
+// Driver semantics (transform_dag.cpp:138-240): the hierarchical pass runs with
+// midLevel = maxTranslationLevel + 1; levels <= maxTranslationLevel are rebuilt top-down, each
+// from the original nodes referenced by the surviving parents, with the GPU matcher deciding
+// which items survive and how every parent pointer is rewritten.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <thread>
+
+#include "errors.hpp"
+#include "pipeline.hpp"
+#include "tdag.h"
+
+struct tdag_svdag {
+    tdag::Svdag s;
+};
+
+struct tdag_tdag {
+    tdag::TDag t;
+    std::vector<tdag::LevelRecord> records;
+};
+
+namespace {
+
+int hostThreads()
+{
+    const unsigned hc = std::thread::hardware_concurrency();
+    return (int)std::max(1u, std::min(16u, hc ? hc : 1u));
+}
+
+std::array<uint64_t, 8> toTransformNode(const std::array<uint32_t, 8>& n) // transform_dag.cpp:123-135
+{
+    std::array<uint64_t, 8> o;
+    for (int c = 0; c < 8; ++c)
+        o[c] = n[c] == tdag::kEmptyChild ? 0x82082400FFFFFFFFull : (0x8208240000000000ull | n[c]);
+    return o;
+}
+
+tdag_status setErr(tdag_status s, const std::string& m) { return tdag::setError(s, m); }
+
+} // namespace
+
+extern "C" {
+
+tdag_status tdag_scene_dense(uint32_t kind, uint32_t resolution, uint64_t seed, uint8_t* out_bits, uint64_t out_bytes)
+{
+    if (!out_bits)
+        return setErr(TDAG_EINVAL, "out_bits is null");
+    if (resolution < 4 || (resolution & (resolution - 1)) || resolution > 4096)
+        return setErr(TDAG_EINVAL, "resolution must be a power of two in [4, 4096]");
+    if (out_bytes < (uint64_t)resolution * resolution * resolution / 8)
+        return setErr(TDAG_EINVAL, "out_bytes is smaller than resolution^3 / 8");
+    if (!tdag::generateSceneDense(kind, resolution, seed, out_bits, hostThreads()))
+        return setErr(TDAG_EINVAL, "unknown scene kind " + std::to_string(kind));
+    return TDAG_OK;
+}
+
+tdag_status tdag_svdag_from_dense(const uint8_t* bits, uint32_t resolution, tdag_svdag** out)
+{
+    if (!bits || !out)
+        return setErr(TDAG_EINVAL, "null argument");
+    if (resolution < 4 || (resolution & (resolution - 1)) || resolution > 4096)
+        return setErr(TDAG_EINVAL, "resolution must be a power of two in [4, 4096]");
+    tdag_svdag* h = new tdag_svdag();
+    h->s = tdag::svdagFromDense(bits, resolution, hostThreads());
+    *out = h;
+    return TDAG_OK;
+}
+
+tdag_status tdag_svdag_info(const tdag_svdag* h, uint32_t* resolution, uint32_t* root_level, uint64_t* level_sizes, uint32_t cap)
+{
+    if (!h)
+        return setErr(TDAG_EINVAL, "null handle");
+    if (resolution)
+        *resolution = h->s.resolution;
+    if (root_level)
+        *root_level = h->s.rootLevel;
+    if (level_sizes)
+        for (uint32_t l = 0; l < cap; ++l)
+            level_sizes[l] = l == 2 ? h->s.leaves.size() : (l < h->s.nodes.size() ? h->s.nodes[l].size() : 0);
+    return TDAG_OK;
+}
+
+tdag_status tdag_svdag_copy_level(const tdag_svdag* h, uint32_t level, void* out)
+{
+    if (!h || !out)
+        return setErr(TDAG_EINVAL, "null argument");
+    if (level == 2) {
+        std::memcpy(out, h->s.leaves.data(), h->s.leaves.size() * 8);
+        return TDAG_OK;
+    }
+    if (level < 3 || level > h->s.rootLevel)
+        return setErr(TDAG_EINVAL, "level out of range");
+    const auto& n = h->s.nodes[level];
+    std::memcpy(out, n.data(), n.size() * 32);
+    return TDAG_OK;
+}
+
+tdag_status tdag_svdag_flatten(const tdag_svdag* h, uint32_t level, const uint32_t* nodes, uint64_t n, uint64_t* out_masks)
+{
+    if (!h || (n && (!nodes || !out_masks)))
+        return setErr(TDAG_EINVAL, "null argument");
+    if (level < 2 || level > h->s.rootLevel || level > 5)
+        return setErr(TDAG_EINVAL, "level must be in [2, min(5, rootLevel)]");
+    const uint64_t size = level == 2 ? h->s.leaves.size() : h->s.nodes[level].size();
+    for (uint64_t i = 0; i < n; ++i)
+        if (nodes[i] >= size)
+            return setErr(TDAG_EINVAL, "node index out of range");
+    tdag::flattenNodes(h->s, level, nodes, n, out_masks, hostThreads());
+    return TDAG_OK;
+}
+
+tdag_status tdag_svdag_free(tdag_svdag* h)
+{
+    delete h;
+    return TDAG_OK;
+}
+
+tdag_status tdag_construct_transform_dag(tdag_context* ctx, const tdag_svdag* in, const tdag_config* cfg, uint32_t flags, tdag_tdag** out)
+{
+    if (!in || !cfg || !out)
+        return setErr(TDAG_EINVAL, "null argument");
+    if (cfg->translation && !ctx)
+        return setErr(TDAG_EINVAL, "a context is required when translation is enabled");
+    const tdag::Svdag& dag = in->s;
+    if (dag.structureType != 1)
+        return setErr(TDAG_EINVAL, "input must be a DAG (transform_dag.cpp:141)");
+    const uint32_t maxT = cfg->max_translation_level;
+    if (maxT < 2 || maxT > 5)
+        return setErr(TDAG_EINVAL, "max_translation_level " + std::to_string(maxT) + " outside [2, 5]");
+    if (maxT + 1 > dag.rootLevel)
+        return setErr(TDAG_EINVAL, "max_translation_level + 1 exceeds the root level");
+    const int threads = hostThreads();
+    tdag_tdag* res = new tdag_tdag();
+    std::vector<uint32_t> translationParents;
+    res->t = tdag::hierarchicalDedup(dag, maxT + 1, translationParents, *cfg, threads);
+    if (cfg->translation) {
+        tdag::TDag& t = res->t;
+        t.leaves.clear();
+        for (uint32_t level = 3; level <= maxT; ++level)
+            t.nodes[level].clear();
+        if (t.nodes[maxT + 1].size() != translationParents.size()) {
+            delete res;
+            return setErr(TDAG_EINTERNAL, "mid level size mismatch");
+        }
+        for (size_t i = 0; i < translationParents.size(); ++i)
+            t.nodes[maxT + 1][i] = toTransformNode(dag.nodes[maxT + 1][translationParents[i]]);
+        for (uint32_t level = maxT; level >= 2; --level) {
+            const size_t numItems = level == 2 ? dag.leaves.size() : dag.nodes[level].size();
+            std::vector<uint8_t> referenced(numItems, 0);
+            auto& parents = t.nodes[level + 1];
+            for (const auto& node : parents)
+                for (uint64_t c : node)
+                    if (c != 0x82082400FFFFFFFFull)
+                        referenced[c & ((1ull << 37) - 1)] = 1;
+            std::vector<uint32_t> indices;
+            for (uint32_t i = 0; i < numItems; ++i)
+                if (referenced[i])
+                    indices.push_back(i);
+            const uint32_t R = 1u << level;
+            const uint64_t W = std::max<uint64_t>(1, (uint64_t)R * R * R / 64);
+            tdag::LevelRecord rec;
+            rec.level = level;
+            rec.items = indices.size();
+            rec.masks.resize(indices.size() * W);
+            tdag::flattenNodes(dag, level, indices.data(), indices.size(), rec.masks.data(), threads);
+            std::vector<uint64_t> mapping(indices.size());
+            std::vector<uint32_t> uniq(indices.size());
+            uint64_t nu = 0;
+            tdag_match_stats st {};
+            const tdag_status s = tdag_match_level(ctx, level, rec.masks.data(), indices.size(), cfg, nullptr, mapping.data(), uniq.data(), &nu, &st);
+            if (s != TDAG_OK) {
+                delete res;
+                char msg[512];
+                tdag_last_error(msg, sizeof(msg));
+                return setErr(s, "matcher failed at level " + std::to_string(level) + ": " + msg);
+            }
+            rec.unique = nu;
+            rec.matched = st.num_matched;
+            rec.msMatch = st.ms_total;
+            if (!(flags & TDAG_KEEP_LEVEL_INPUTS))
+                std::vector<uint64_t>().swap(rec.masks);
+            if (level == 2) {
+                t.leaves.resize(nu);
+                for (uint64_t k = 0; k < nu; ++k)
+                    t.leaves[k] = dag.leaves[indices[uniq[k]]];
+            } else {
+                t.nodes[level].resize(nu);
+                for (uint64_t k = 0; k < nu; ++k)
+                    t.nodes[level][k] = toTransformNode(dag.nodes[level][indices[uniq[k]]]);
+            }
+            for (auto& node : parents)
+                for (uint64_t& c : node) {
+                    if (c == 0x82082400FFFFFFFFull)
+                        continue;
+                    const uint32_t ptr = (uint32_t)(c & ((1ull << 37) - 1));
+                    const auto it = std::lower_bound(indices.begin(), indices.end(), ptr);
+                    c = mapping[it - indices.begin()];
+                }
+            res->records.push_back(std::move(rec));
+            if (level == 2)
+                break;
+        }
+    }
+    *out = res;
+    return TDAG_OK;
+}
+
+tdag_status tdag_tdag_serialize(const tdag_tdag* t, const tdag_config* cfg, uint8_t* buf, uint64_t cap, uint64_t* out_size)
+{
+    if (!t || !cfg || !out_size)
+        return setErr(TDAG_EINVAL, "null argument");
+    const std::vector<uint8_t> bytes = tdag::serializeTdag(t->t, *cfg);
+    *out_size = bytes.size();
+    if (buf) {
+        if (cap < bytes.size())
+            return setErr(TDAG_EINVAL, "buffer too small");
+        std::memcpy(buf, bytes.data(), bytes.size());
+    }
+    return TDAG_OK;
+}
+
+tdag_status tdag_tdag_level_record(const tdag_tdag* t, uint32_t level, tdag_level_record* out)
+{
+    if (!t || !out)
+        return setErr(TDAG_EINVAL, "null argument");
+    for (const auto& r : t->records)
+        if (r.level == level) {
+            out->level = r.level;
+            out->items = r.items;
+            out->unique = r.unique;
+            out->matched = r.matched;
+            out->ms_match = r.msMatch;
+            out->has_masks = r.masks.empty() ? 0u : 1u;
+            return TDAG_OK;
+        }
+    return setErr(TDAG_EINVAL, "no matcher record for level " + std::to_string(level));
+}
+
+tdag_status tdag_tdag_level_masks(const tdag_tdag* t, uint32_t level, uint64_t* out, uint64_t cap_words)
+{
+    if (!t || !out)
+        return setErr(TDAG_EINVAL, "null argument");
+    for (const auto& r : t->records)
+        if (r.level == level) {
+            if (r.masks.size() > cap_words)
+                return setErr(TDAG_EINVAL, "buffer too small");
+            std::memcpy(out, r.masks.data(), r.masks.size() * 8);
+            return TDAG_OK;
+        }
+    return setErr(TDAG_EINVAL, "no matcher record for level " + std::to_string(level));
+}
+
+tdag_status tdag_tdag_info(const tdag_tdag* t, uint32_t* root_level, uint64_t* level_sizes, uint32_t cap)
+{
+    if (!t)
+        return setErr(TDAG_EINVAL, "null handle");
+    if (root_level)
+        *root_level = t->t.rootLevel;
+    if (level_sizes)
+        for (uint32_t l = 0; l < cap; ++l)
+            level_sizes[l] = l == 2 ? t->t.leaves.size() : (l < t->t.nodes.size() ? t->t.nodes[l].size() : 0);
+    return TDAG_OK;
+}
+
+tdag_status tdag_tdag_free(tdag_tdag* t)
+{
+    delete t;
+    return TDAG_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,633 @@
+// Exhaustive Transform-Aware SVDAG matcher for gfx950.
+//
+// Problem (reference: find_translation.cpp:616-1000): for every item c of a level, find the
+// minimum 64-bit key (parent rank, perm, mirror, translation) over all exact matches
+//     c == translate(mirror(permute(P, perm), mirror), t),   t in [-(R-1), R-1]^3,
+// where P ranges over all items in stable popcount-descending order. The reference finds the
+// candidates with a nested corner hash and verifies each one; its result equals the
+// exhaustive minimum. This file computes the same minimum with its own GPU formulation:
+//
+//  * Rank order: stable LSD radix sort of (R^3 - popcount, index).
+//  * Child table: each item's content as seen from each of its 8 corners ("anchored content")
+//    goes into an open-addressing multi-map. At level 2 the anchored content is itself a
+//    64-bit mask and is the exact key; at levels 3-5 the key is a 32-bit nested sub-box hash.
+//  * Search: every parent enumerates 6 permutations x 8 directions x R^3 anchored sub-boxes.
+//    A table hit (child c, corner k) yields mirror = d ^ k and the translation the reference
+//    derives (find_translation.cpp:835-837); the candidate is verified bitwise and merged into
+//    best[c] with a 64-bit atomicMin. The self-match seeds best[c], so only ranks < c probe.
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+#include "matcher.hpp"
+#include "primitives.hpp"
+
+namespace tdag {
+
+namespace {
+
+inline size_t ceilDiv(size_t a, size_t b) { return (a + b - 1) / b; }
+inline size_t alignUp(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+__device__ __forceinline__ uint64_t loadRelaxed(const uint64_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ bool configAllows(uint32_t cfgBits, uint32_t permCode, uint32_t sym, int tx, int ty, int tz)
+{
+    if (!(cfgBits & kCfgAxis) && permCode != 0)
+        return false;
+    if (!(cfgBits & kCfgSymmetry) && sym != 0)
+        return false;
+    if (!(cfgBits & kCfgTranslation) && (tx | ty | tz) != 0)
+        return false;
+    return true;
+}
+
+// ---------------------------------------------------------------------------------------
+// Rank order and initialisation
+// ---------------------------------------------------------------------------------------
+
+// keys[i] = R^3 - popcount(mask i), vals[i] = i. G lanes cooperate on one item.
+template <int G>
+__global__ __launch_bounds__(256) void kPopKeys(const uint64_t* __restrict__ masks, uint32_t n, uint32_t W, uint32_t R3, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
+{
+    const uint32_t gid = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t item = gid / G;
+    const uint32_t sub = gid % G;
+    uint32_t pop = 0;
+    if (item < n) {
+        const uint64_t* m = masks + (size_t)item * W;
+        for (uint32_t w = sub; w < W; w += G)
+            pop += (uint32_t)__popcll(m[w]);
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1)
+        pop += __shfl_xor(pop, off, G);
+    if (item < n && sub == 0) {
+        keys[item] = R3 - pop;
+        vals[item] = item;
+    }
+}
+
+// sortedMasks[r] = masks[order[r]]; best[r] = self key (or the sentinel key for empty items).
+__global__ __launch_bounds__(256) void kGatherInit(const uint64_t* __restrict__ masks, const uint32_t* __restrict__ order, const uint32_t* __restrict__ sortedKeys,
+    uint32_t n, uint32_t W, uint32_t R3, uint32_t level, uint64_t* __restrict__ sortedMasks, uint64_t* __restrict__ best)
+{
+    const size_t total = (size_t)n * W;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256) {
+        const uint32_t r = (uint32_t)(i / W);
+        const uint32_t w = (uint32_t)(i % W);
+        sortedMasks[i] = masks[(size_t)order[r] * W + w];
+        if (w == 0)
+            best[r] = (sortedKeys[r] == R3) ? sentinelKey(level) : selfKey(r, level);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Level 2 (4^3 leaves, one uint64 per mask): exact anchored-content keys.
+// ---------------------------------------------------------------------------------------
+
+// Corner k (bit a set = Positive along axis a) sees the grid mirrored on its Negative axes.
+__global__ __launch_bounds__(256) void kBuildTable2(const uint64_t* __restrict__ sortedMasks, uint32_t n, uint64_t* __restrict__ tabKeys, uint32_t* __restrict__ tabVals, uint32_t logCap)
+{
+    const uint32_t gid = blockIdx.x * 256u + threadIdx.x;
+    if (gid >= n * 8u)
+        return;
+    const uint32_t r = gid >> 3;
+    const uint32_t corner = gid & 7u;
+    const uint64_t m = sortedMasks[r];
+    if (m == 0)
+        return;
+    const uint64_t key = mirror4(m, (~corner) & 7u);
+    const uint32_t capMask = (1u << logCap) - 1u;
+    uint32_t s = homeSlot64(key, logCap);
+    for (;;) {
+        const unsigned long long prev = atomicCAS((unsigned long long*)&tabKeys[s], 0ull, (unsigned long long)key);
+        if (prev == 0ull) {
+            tabVals[s] = (r << 3) | corner;
+            return;
+        }
+        s = (s + 1) & capMask;
+    }
+}
+
+// One wave per parent rank; lane = anchored voxel v' = (x, y, z) of the sub-box.
+__global__ __launch_bounds__(256) void kSearch2(const uint64_t* __restrict__ sortedMasks, uint32_t n, const uint64_t* __restrict__ tabKeys, const uint32_t* __restrict__ tabVals,
+    uint32_t logCap, uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t rankBase)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t r = rankBase + blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (r >= n)
+        return;
+    const uint64_t P = sortedMasks[r];
+    if (P == 0)
+        return;
+    const int vx = (int)(lane & 3u), vy = (int)((lane >> 2) & 3u), vz = (int)(lane >> 4);
+    const uint64_t clip = rangeMaskX4(0, 4 - vx) & rangeMaskY4(0, 4 - vy) & rangeMaskZ4(0, 4 - vz);
+    const uint32_t capMask = (1u << logCap) - 1u;
+    const uint32_t numPerms = (cfgBits & kCfgAxis) ? 6u : 1u;
+    for (uint32_t pc = 0; pc < numPerms; ++pc) {
+        // permute (find_translation.cpp:109-125): out(o) = P(v) with v[p[a]] = o[a].
+        int v[3];
+        v[permAxis(pc, 0)] = vx;
+        v[permAxis(pc, 1)] = vy;
+        v[permAxis(pc, 2)] = vz;
+        const uint32_t src = (uint32_t)(v[0] + 4 * v[1] + 16 * v[2]);
+        const uint64_t Q = (uint64_t)__ballot((P >> src) & 1ull);
+        for (uint32_t d = 0; d < 8; ++d) {
+            const uint64_t F = mirror4(Q, (~d) & 7u);
+            const uint64_t key = (F >> lane) & clip;
+            if (key == 0)
+                continue;
+            uint32_t s = homeSlot64(key, logCap);
+            for (;;) {
+                const uint64_t k = tabKeys[s];
+                if (k == 0)
+                    break;
+                if (k == key) {
+                    const uint32_t e = tabVals[s];
+                    const uint32_t c = e >> 3;
+                    const uint32_t kc = e & 7u;
+                    if (c > r) {
+                        const uint32_t sym = d ^ kc;
+                        const int tx = (kc & 1u) ? -vx : vx;
+                        const int ty = (kc & 2u) ? -vy : vy;
+                        const int tz = (kc & 4u) ? -vz : vz;
+                        if (configAllows(cfgBits, pc, sym, tx, ty, tz)) {
+                            const uint64_t k64 = encodeKey(r, pc, sym, tx, ty, tz, 2);
+                            if (k64 < loadRelaxed(&best[c])) {
+                                // Bitwise verification of the derived transform.
+                                const uint64_t T = translate4(mirror4(Q, sym), tx, ty, tz);
+                                if (T == sortedMasks[c])
+                                    atomicMin((unsigned long long*)&best[c], (unsigned long long)k64);
+                            }
+                        }
+                    }
+                }
+                s = (s + 1) & capMask;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Levels 3..5: 32-bit nested sub-box hashes computed in LDS, one line per thread per pass.
+// ---------------------------------------------------------------------------------------
+template <int L>
+struct Geo {
+    static constexpr int R = 1 << L;
+    static constexpr int R2 = R * R;
+    static constexpr int R3 = R2 * R;
+    static constexpr int W = R3 / 64;
+    static constexpr int Threads = R2;
+};
+
+template <int L>
+__device__ __forceinline__ uint32_t sidx(int x, int y, int z)
+{
+    constexpr int R = Geo<L>::R;
+    return (uint32_t)((x ^ (y & (R - 1))) + R * y + R * R * z);
+}
+
+template <int L>
+__device__ __forceinline__ uint32_t lbit(const uint64_t* s_bits, int x, int y, int z)
+{
+    constexpr int R = Geo<L>::R;
+    const uint32_t i = (uint32_t)(x + R * y + R * R * z);
+    return (uint32_t)((s_bits[i >> 6] >> (i & 63u)) & 1ull);
+}
+
+// Line coordinates of thread t for a pass along axis A: the other two axes (ascending) get
+// (t mod R, t div R).
+__device__ __forceinline__ void lineAxes(int A, int& B, int& C)
+{
+    B = (A == 0) ? 1 : 0;
+    C = (A == 2) ? 1 : 2;
+}
+
+// Passes 1 and 2 of the nested hash for (permCode, direction d) into s_h.
+template <int L>
+__device__ __forceinline__ void hashPasses12(const uint64_t* s_bits, uint32_t* s_h, uint32_t pc, uint32_t d)
+{
+    constexpr int R = Geo<L>::R;
+    const int t = (int)threadIdx.x;
+    const int u = t & (R - 1), w = t >> L;
+    {
+        const int A = (int)permAxis(pc, 0);
+        int B, C;
+        lineAxes(A, B, C);
+        const bool pos = d & 1u;
+        int c[3];
+        c[B] = u;
+        c[C] = w;
+        uint32_t s = 0;
+        for (int k = 0; k < R; ++k) {
+            c[A] = pos ? (R - 1 - k) : k;
+            s = hashStep(s, lbit<L>(s_bits, c[0], c[1], c[2]));
+            s_h[sidx<L>(c[0], c[1], c[2])] = s;
+        }
+    }
+    __syncthreads();
+    {
+        const int A = (int)permAxis(pc, 1);
+        int B, C;
+        lineAxes(A, B, C);
+        const bool pos = (d >> 1) & 1u;
+        int c[3];
+        c[B] = u;
+        c[C] = w;
+        uint32_t s = 0;
+        for (int k = 0; k < R; ++k) {
+            c[A] = pos ? (R - 1 - k) : k;
+            const uint32_t i = sidx<L>(c[0], c[1], c[2]);
+            s = hashStep(s, s_h[i]);
+            s_h[i] = s;
+        }
+    }
+    __syncthreads();
+}
+
+// Expected child word `wi` for the transform encoded in `key` applied to the parent bits.
+template <int L>
+__device__ __forceinline__ uint64_t transformedWord(const uint64_t* s_bits, uint64_t key, uint32_t wi)
+{
+    constexpr int R = Geo<L>::R;
+    constexpr uint32_t b = L + 1;
+    constexpr uint64_t m = (1ull << b) - 1ull;
+    const int tx = (int)(key & m) - R;
+    const int ty = (int)((key >> b) & m) - R;
+    const int tz = (int)((key >> (2 * b)) & m) - R;
+    const uint32_t sym = (uint32_t)((key >> (3 * b)) & 7u);
+    const uint32_t pc = (uint32_t)((key >> (3 * b + 3)) & 7u);
+    const int a0 = (int)permAxis(pc, 0), a1 = (int)permAxis(pc, 1), a2 = (int)permAxis(pc, 2);
+    uint64_t out = 0;
+    for (int j = 0; j < 64; ++j) {
+        const uint32_t i = wi * 64u + (uint32_t)j;
+        int o[3] = { (int)(i & (R - 1)), (int)((i >> L) & (R - 1)), (int)(i >> (2 * L)) };
+        int q[3] = { o[0] - tx, o[1] - ty, o[2] - tz }; // translate: out[o] = M[o - t]
+        if (q[0] < 0 || q[1] < 0 || q[2] < 0 || q[0] >= R || q[1] >= R || q[2] >= R)
+            continue;
+        if (sym & 1u)
+            q[0] = R - 1 - q[0];
+        if (sym & 2u)
+            q[1] = R - 1 - q[1];
+        if (sym & 4u)
+            q[2] = R - 1 - q[2];
+        int v[3];
+        v[a0] = q[0];
+        v[a1] = q[1];
+        v[a2] = q[2];
+        out |= (uint64_t)lbit<L>(s_bits, v[0], v[1], v[2]) << j;
+    }
+    return out;
+}
+
+// One block per item; the hash of the whole grid seen from each of its 8 corners goes into the
+// table (identity permutation, direction = corner).
+template <int L>
+__global__ __launch_bounds__(Geo<L>::Threads) void kCornerTable(const uint64_t* __restrict__ sortedMasks, uint32_t n, uint64_t* __restrict__ table, uint32_t logCap)
+{
+    constexpr int R = Geo<L>::R;
+    constexpr int W = Geo<L>::W;
+    __shared__ uint64_t s_bits[W];
+    __shared__ uint32_t s_h[Geo<L>::R3];
+    __shared__ uint32_t s_any;
+    const uint32_t r = blockIdx.x;
+    if (threadIdx.x == 0)
+        s_any = 0;
+    __syncthreads();
+    for (int i = (int)threadIdx.x; i < W; i += Geo<L>::Threads) {
+        const uint64_t v = sortedMasks[(size_t)r * W + i];
+        s_bits[i] = v;
+        if (v)
+            atomicOr(&s_any, 1u);
+    }
+    __syncthreads();
+    if (!s_any)
+        return;
+    const int t = (int)threadIdx.x;
+    const int x = t & (R - 1), y = t >> L;
+    const uint32_t capMask = (1u << logCap) - 1u;
+    for (uint32_t corner = 0; corner < 8; ++corner) {
+        hashPasses12<L>(s_bits, s_h, 0, corner);
+        const bool pos = (corner >> 2) & 1u;
+        uint32_t s = 0;
+        for (int k = 0; k < R; ++k) {
+            const int z = pos ? (R - 1 - k) : k;
+            s = hashStep(s, s_h[sidx<L>(x, y, z)]);
+        }
+        const int ax = (corner & 1u) ? 0 : R - 1;
+        const int ay = (corner & 2u) ? 0 : R - 1;
+        if (x == ax && y == ay && s != 0) {
+            const uint64_t packed = ((uint64_t)s << 32) | (uint64_t)((r << 3) | corner);
+            uint32_t slot = homeSlot32(s, logCap);
+            for (;;) {
+                const unsigned long long prev = atomicCAS((unsigned long long*)&table[slot], 0ull, (unsigned long long)packed);
+                if (prev == 0ull)
+                    break;
+                slot = (slot + 1) & capMask;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// One block per (parent rank, group of combos). Threads own one line per pass; pass 3 probes
+// the table at every voxel with a non-empty sub-box. Hits are verified cooperatively by the
+// wave that found them.
+template <int L>
+__global__ __launch_bounds__(Geo<L>::Threads) void kSearchNested(const uint64_t* __restrict__ sortedMasks, uint32_t n, const uint64_t* __restrict__ table, uint32_t logCap,
+    uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t combosPerBlock, uint32_t rankBase)
+{
+    constexpr int R = Geo<L>::R;
+    constexpr int W = Geo<L>::W;
+    __shared__ uint64_t s_bits[W];
+    __shared__ uint32_t s_h[Geo<L>::R3];
+    __shared__ uint32_t s_any;
+    const uint32_t r = rankBase + blockIdx.x;
+    if (r >= n)
+        return;
+    if (threadIdx.x == 0)
+        s_any = 0;
+    __syncthreads();
+    for (int i = (int)threadIdx.x; i < W; i += Geo<L>::Threads) {
+        const uint64_t v = sortedMasks[(size_t)r * W + i];
+        s_bits[i] = v;
+        if (v)
+            atomicOr(&s_any, 1u);
+    }
+    __syncthreads();
+    if (!s_any)
+        return;
+    const uint32_t lane = threadIdx.x & 63u;
+    const int t = (int)threadIdx.x;
+    const int u = t & (R - 1), w = t >> L;
+    const uint32_t capMask = (1u << logCap) - 1u;
+    const uint32_t comboBegin = blockIdx.y * combosPerBlock;
+    const uint32_t comboEnd = min(comboBegin + combosPerBlock, 48u);
+    for (uint32_t combo = comboBegin; combo < comboEnd; ++combo) {
+        const uint32_t pc = combo >> 3;
+        const uint32_t d = combo & 7u;
+        if (!(cfgBits & kCfgAxis) && pc != 0)
+            continue; // uniform across the block
+        hashPasses12<L>(s_bits, s_h, pc, d);
+        const int a0 = (int)permAxis(pc, 0), a1 = (int)permAxis(pc, 1), a2 = (int)permAxis(pc, 2);
+        int B, C;
+        lineAxes(a2, B, C);
+        const bool pos = (d >> 2) & 1u;
+        const int anchor[3] = { (d & 1u) ? 0 : R - 1, (d & 2u) ? 0 : R - 1, (d & 4u) ? 0 : R - 1 };
+        int c[3];
+        c[B] = u;
+        c[C] = w;
+        uint32_t s = 0;
+        for (int k = 0; k < R; ++k) {
+            c[a2] = pos ? (R - 1 - k) : k;
+            s = hashStep(s, s_h[sidx<L>(c[0], c[1], c[2])]);
+            // Voxel in the permuted frame (applyAxisPermutation, transform_dag.h:182-189).
+            const int voxel[3] = { c[a0], c[a1], c[a2] };
+            bool active = (s != 0);
+            uint32_t slot = active ? homeSlot32(s, logCap) : 0u;
+            while (__ballot(active)) {
+                bool found = false;
+                uint32_t hitC = 0;
+                uint64_t hitKey = 0;
+                while (active) {
+                    const uint64_t e = table[slot];
+                    if (e == 0) {
+                        active = false;
+                        break;
+                    }
+                    slot = (slot + 1) & capMask;
+                    if ((uint32_t)(e >> 32) != s)
+                        continue;
+                    const uint32_t val = (uint32_t)e;
+                    const uint32_t cc = val >> 3;
+                    const uint32_t kc = val & 7u;
+                    if (cc <= r)
+                        continue;
+                    const uint32_t sym = d ^ kc;
+                    int tt[3];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a)
+                        tt[a] = ((sym >> a) & 1u) ? voxel[a] - anchor[a] : anchor[a] - voxel[a];
+                    if (!configAllows(cfgBits, pc, sym, tt[0], tt[1], tt[2]))
+                        continue;
+                    const uint64_t k64 = encodeKey(r, pc, sym, tt[0], tt[1], tt[2], L);
+                    if (k64 >= loadRelaxed(&best[cc]))
+                        continue;
+                    found = true;
+                    hitC = cc;
+                    hitKey = k64;
+                    break;
+                }
+                uint64_t pending = __ballot(found);
+                while (pending) {
+                    const int leader = __ffsll((unsigned long long)pending) - 1;
+                    const uint32_t bc = (uint32_t)__shfl((int)hitC, leader, 64);
+                    const uint64_t bk = (uint64_t)__shfl((long long)hitKey, leader, 64);
+                    bool ok = true;
+                    for (uint32_t wi = lane; wi < (uint32_t)W; wi += 64u)
+                        ok = ok && (transformedWord<L>(s_bits, bk, wi) == sortedMasks[(size_t)bc * W + wi]);
+                    const bool allOk = (__ballot(!ok) == 0ull);
+                    if ((int)lane == leader && allOk)
+                        atomicMin((unsigned long long*)&best[bc], (unsigned long long)bk);
+                    pending &= pending - 1ull;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Post-processing (find_translation.cpp:923-997).
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void kFinalParents(const uint64_t* __restrict__ best, const uint32_t* __restrict__ order, uint32_t n, uint32_t level,
+    uint64_t* __restrict__ outEncoded, uint64_t* __restrict__ parentTp, uint32_t* __restrict__ isUnique, unsigned long long* __restrict__ counters)
+{
+    const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+    if (c >= n)
+        return;
+    const uint64_t key = best[c];
+    const uint32_t orig = order[c];
+    if (outEncoded)
+        outEncoded[orig] = key;
+    if (key == sentinelKey(level)) {
+        parentTp[orig] = tpSentinel();
+        isUnique[orig] = 1u;
+        return;
+    }
+    const uint64_t pr = keyPtr(key, level);
+    const uint32_t parentOrig = order[pr];
+    parentTp[orig] = keyToTp(key, level, parentOrig);
+    isUnique[parentOrig] = 1u;
+    if (pr != c)
+        atomicAdd(&counters[0], 1ull);
+}
+
+__global__ __launch_bounds__(256) void kFinalMapping(const uint64_t* __restrict__ parentTp, const uint32_t* __restrict__ isUnique, const uint32_t* __restrict__ newIdx, uint32_t n,
+    uint64_t* __restrict__ mapping, uint32_t* __restrict__ uniqueIdx)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n)
+        return;
+    if (isUnique[i]) {
+        if (mapping)
+            mapping[i] = tpIdentity(newIdx[i]);
+        if (uniqueIdx)
+            uniqueIdx[newIdx[i]] = i;
+    } else if (mapping) {
+        const uint64_t p = parentTp[i];
+        const uint32_t parentOrig = (uint32_t)(p & ((1ull << 37) - 1));
+        mapping[i] = (p & ~((1ull << 37) - 1)) | (uint64_t)newIdx[parentOrig];
+    }
+}
+
+#define TDAG_TRY(expr)                   \
+    do {                                 \
+        const hipError_t e_ = (expr);    \
+        if (e_ != hipSuccess)            \
+            return e_;                   \
+    } while (0)
+
+uint32_t log2Ceil(uint64_t v)
+{
+    uint32_t l = 0;
+    while ((1ull << l) < v)
+        ++l;
+    return l;
+}
+
+} // namespace
+
+MatchPlan planMatch(uint32_t level, uint64_t n)
+{
+    MatchPlan p {};
+    p.level = level;
+    p.n = n;
+    p.R = 1u << level;
+    p.R3 = p.R * p.R * p.R;
+    p.W = p.R3 >= 64 ? p.R3 / 64 : 1;
+    const uint64_t entries = 8 * (n > 0 ? n : 1);
+    p.logCap = std::max<uint32_t>(6, log2Ceil(2 * entries));
+    const uint64_t cap = 1ull << p.logCap;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off += alignUp(bytes > 0 ? bytes : 1, 256);
+        return o;
+    };
+    p.offKeys = take(n * 4);
+    p.offOrder = take(n * 4);
+    p.offSortScratch = take(radixScratchBytes(n, 4));
+    p.offSortedMasks = take(n * p.W * 8);
+    p.offBest = take(n * 8);
+    p.offTabKeys = take(cap * 8);
+    p.offTabVals = take(level == 2 ? cap * 4 : 0);
+    p.offParentTp = take(n * 8);
+    p.offIsUnique = take(n * 4);
+    p.offNewIdx = take(n * 4);
+    p.offScanScratch = take(scanScratchBytes(n));
+    p.offCounters = take(64);
+    p.scratchBytes = off;
+    return p;
+}
+
+hipError_t enqueueMatch(const MatchPlan& p, const uint64_t* d_masks, uint32_t cfgBits, void* d_scratch, hipStream_t stream,
+    uint64_t* d_encoded, uint64_t* d_mapping, uint32_t* d_unique, hipEvent_t evSearch0, hipEvent_t evSearch1)
+{
+    char* base = (char*)d_scratch;
+    uint32_t* keys = (uint32_t*)(base + p.offKeys);
+    uint32_t* order = (uint32_t*)(base + p.offOrder);
+    uint64_t* sortedMasks = (uint64_t*)(base + p.offSortedMasks);
+    uint64_t* best = (uint64_t*)(base + p.offBest);
+    uint64_t* tabKeys = (uint64_t*)(base + p.offTabKeys);
+    uint32_t* tabVals = (uint32_t*)(base + p.offTabVals);
+    uint64_t* parentTp = (uint64_t*)(base + p.offParentTp);
+    uint32_t* isUnique = (uint32_t*)(base + p.offIsUnique);
+    uint32_t* newIdx = (uint32_t*)(base + p.offNewIdx);
+    unsigned long long* counters = (unsigned long long*)(base + p.offCounters);
+    uint32_t* total = (uint32_t*)(counters + 1);
+    const uint32_t n = (uint32_t)p.n;
+    const uint32_t W = p.W;
+    const uint64_t cap = 1ull << p.logCap;
+
+    TDAG_TRY(fillU64((uint64_t*)counters, 2, 0ull, stream));
+    if (n == 0) {
+        if (evSearch0)
+            TDAG_TRY(hipEventRecord(evSearch0, stream));
+        if (evSearch1)
+            TDAG_TRY(hipEventRecord(evSearch1, stream));
+        return hipSuccess;
+    }
+
+    // 1. Popcount keys and the stable popcount-descending rank order.
+    {
+        const unsigned blocks256 = (unsigned)ceilDiv((size_t)n * std::min<uint32_t>(W, 64), 256);
+        if (W >= 64)
+            hipLaunchKernelGGL(kPopKeys<64>, dim3(blocks256), dim3(256), 0, stream, d_masks, n, W, p.R3, keys, order);
+        else if (W >= 8)
+            hipLaunchKernelGGL(kPopKeys<8>, dim3((unsigned)ceilDiv((size_t)n * 8, 256)), dim3(256), 0, stream, d_masks, n, W, p.R3, keys, order);
+        else
+            hipLaunchKernelGGL(kPopKeys<1>, dim3((unsigned)ceilDiv(n, 256)), dim3(256), 0, stream, d_masks, n, W, p.R3, keys, order);
+        TDAG_TRY(hipGetLastError());
+        TDAG_TRY(radixSortPairsU32(keys, order, n, 0, 16, base + p.offSortScratch, stream));
+    }
+    // 2. Masks in rank order; best[] seeded with the self-match.
+    {
+        const size_t total = (size_t)n * W;
+        const unsigned blocks = (unsigned)std::min<size_t>(ceilDiv(total, 256), 65536);
+        hipLaunchKernelGGL(kGatherInit, dim3(blocks), dim3(256), 0, stream, d_masks, order, keys, n, W, p.R3, p.level, sortedMasks, best);
+        TDAG_TRY(hipGetLastError());
+    }
+    // 3. Child table.
+    TDAG_TRY(fillU64(tabKeys, cap, 0ull, stream));
+    switch (p.level) {
+    case 2:
+        hipLaunchKernelGGL(kBuildTable2, dim3((unsigned)ceilDiv((size_t)n * 8, 256)), dim3(256), 0, stream, sortedMasks, n, tabKeys, tabVals, p.logCap);
+        break;
+    case 3:
+        hipLaunchKernelGGL(kCornerTable<3>, dim3(n), dim3(Geo<3>::Threads), 0, stream, sortedMasks, n, tabKeys, p.logCap);
+        break;
+    case 4:
+        hipLaunchKernelGGL(kCornerTable<4>, dim3(n), dim3(Geo<4>::Threads), 0, stream, sortedMasks, n, tabKeys, p.logCap);
+        break;
+    case 5:
+        hipLaunchKernelGGL(kCornerTable<5>, dim3(n), dim3(Geo<5>::Threads), 0, stream, sortedMasks, n, tabKeys, p.logCap);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    TDAG_TRY(hipGetLastError());
+    // 4. Search.
+    if (evSearch0)
+        TDAG_TRY(hipEventRecord(evSearch0, stream));
+    switch (p.level) {
+    case 2:
+        hipLaunchKernelGGL(kSearch2, dim3((unsigned)ceilDiv(n, 4)), dim3(256), 0, stream, sortedMasks, n, tabKeys, tabVals, p.logCap, best, cfgBits, 0u);
+        break;
+    case 3:
+        hipLaunchKernelGGL(kSearchNested<3>, dim3(n, 1), dim3(Geo<3>::Threads), 0, stream, sortedMasks, n, tabKeys, p.logCap, best, cfgBits, 48u, 0u);
+        break;
+    case 4:
+        hipLaunchKernelGGL(kSearchNested<4>, dim3(n, 6), dim3(Geo<4>::Threads), 0, stream, sortedMasks, n, tabKeys, p.logCap, best, cfgBits, 8u, 0u);
+        break;
+    case 5:
+        hipLaunchKernelGGL(kSearchNested<5>, dim3(n, 48), dim3(Geo<5>::Threads), 0, stream, sortedMasks, n, tabKeys, p.logCap, best, cfgBits, 1u, 0u);
+        break;
+    }
+    TDAG_TRY(hipGetLastError());
+    if (evSearch1)
+        TDAG_TRY(hipEventRecord(evSearch1, stream));
+    // 5. Post-processing.
+    TDAG_TRY(fillU32(isUnique, n, 0u, stream));
+    hipLaunchKernelGGL(kFinalParents, dim3((unsigned)ceilDiv(n, 256)), dim3(256), 0, stream, best, order, n, p.level, d_encoded, parentTp, isUnique, counters);
+    TDAG_TRY(hipGetLastError());
+    TDAG_TRY(exclusiveScanU32(isUnique, newIdx, n, total, base + p.offScanScratch, stream));
+    hipLaunchKernelGGL(kFinalMapping, dim3((unsigned)ceilDiv(n, 256)), dim3(256), 0, stream, parentTp, isUnique, newIdx, n, d_mapping, d_unique);
+    TDAG_TRY(hipGetLastError());
+    return hipSuccess;
+}
+
+} // namespace tdag

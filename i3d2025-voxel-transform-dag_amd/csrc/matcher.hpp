@@ -1,0 +1,32 @@
+// Host-side plan and launcher of the exhaustive matcher (one level per call).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <algorithm>
+
+namespace tdag {
+
+// TransformDAGConfig flags as one word (transform_dag.h:157-160).
+constexpr uint32_t kCfgSymmetry = 1u;
+constexpr uint32_t kCfgAxis = 2u;
+constexpr uint32_t kCfgTranslation = 4u;
+
+// Device scratch layout of one matcher call.
+struct MatchPlan {
+    uint32_t level, R, R3, W, logCap;
+    uint64_t n;
+    size_t offKeys, offOrder, offSortScratch, offSortedMasks, offBest;
+    size_t offTabKeys, offTabVals, offParentTp, offIsUnique, offNewIdx, offScanScratch, offCounters;
+    size_t scratchBytes;
+};
+
+MatchPlan planMatch(uint32_t level, uint64_t n);
+
+// Enqueues the whole matcher on `stream`. Outputs may be null. The device counters at
+// scratch + offCounters hold {num_matched (u64), num_unique (u32)} when the stream completes.
+hipError_t enqueueMatch(const MatchPlan& plan, const uint64_t* d_masks, uint32_t cfgBits, void* d_scratch, hipStream_t stream,
+    uint64_t* d_encoded, uint64_t* d_mapping, uint32_t* d_unique, hipEvent_t evSearch0, hipEvent_t evSearch1);
+
+} // namespace tdag

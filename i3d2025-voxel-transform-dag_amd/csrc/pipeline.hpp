@@ -1,0 +1,59 @@
+// Host-side data model and pipeline stages around the matcher (C++; compiled into libtdag.so).
+//
+//   Svdag  mirrors voxcom::EditStructure<void, uint32_t> after toDAG (structure.h:159-211)
+//   TDag   mirrors voxcom::EditStructure<void, TransformPointer> (transform_dag.h:161-162)
+#pragma once
+#include <array>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "tdag.h"
+
+namespace tdag {
+
+constexpr uint32_t kEmptyChild = 0xFFFFFFFFu;
+
+struct Svdag {
+    uint32_t resolution = 0;
+    uint32_t rootLevel = 0;
+    int32_t structureType = 1; // voxcom::StructureType::DAG
+    // nodes[level] holds 8 child indices per node (kEmptyChild for none); levels 0..2 empty.
+    std::vector<std::vector<std::array<uint32_t, 8>>> nodes;
+    // Leaves (level 2): 4x4x4 masks, bit = morton(x, y, z) (structure.h:116-125).
+    std::vector<uint64_t> leaves;
+};
+
+struct TDag {
+    uint32_t resolution = 0;
+    uint32_t rootLevel = 0;
+    int32_t structureType = 1;
+    std::vector<std::vector<std::array<uint64_t, 8>>> nodes; // TransformPointer bit patterns
+    std::vector<uint64_t> leaves;
+};
+
+// Per-level record of one constructTransformDAG run.
+struct LevelRecord {
+    uint32_t level = 0;
+    uint64_t items = 0; // matcher input size (referenced nodes)
+    uint64_t unique = 0;
+    uint64_t matched = 0; // items whose parent is another item
+    float msMatch = 0.f; // device time of the matcher call
+    std::vector<uint64_t> masks; // the matcher's input (kept when requested)
+};
+
+bool generateSceneDense(uint32_t kind, uint32_t R, uint64_t seed, uint8_t* bits, int threads);
+
+// EditStructure(VoxelGrid) + toDAG, built bottom-up in Morton order.
+Svdag svdagFromDense(const uint8_t* bits, uint32_t R, int threads);
+
+// createLargeSubGrid for level-`level` nodes (level 2 = leaves): linear masks, W words each.
+void flattenNodes(const Svdag& s, uint32_t level, const uint32_t* nodes, uint64_t n, uint64_t* out, int threads);
+
+// constructTransformDAGHierarchical with midLevel; fills outMidInv (transform_dag_hierarchical.cpp:304-523).
+TDag hierarchicalDedup(const Svdag& in, uint32_t midLevel, std::vector<uint32_t>& outMidInv, const tdag_config& cfg, int threads);
+
+// .tdag byte stream (voxelizer/src/main.cpp:147-150).
+std::vector<uint8_t> serializeTdag(const TDag& t, const tdag_config& cfg);
+
+} // namespace tdag

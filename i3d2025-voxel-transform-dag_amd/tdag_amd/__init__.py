@@ -1,0 +1,250 @@
+"""Python host mirror of the MI355X-native Transform-Aware SVDAG matching path.
+
+The compute lives in ``libtdag.so`` (hand-written HIP for gfx950 behind the C-ABI declared in
+``include/tdag.h``). This module only binds that ABI with ctypes, moves numpy/torch buffers
+across it, and mirrors the reference's entry points:
+
+* :func:`find_duplicates_among_flattened_items_gpu` mirrors
+  ``findDuplicatesAmongFlattenedItems_gpu<Level, Item>`` (transform_dag.h:310-311): it returns
+  the unique items and ``outPrevLevelMapping`` as TransformPointer bit patterns.
+* :meth:`Context.match_level_device` is the device-resident hot path used by the benchmark.
+
+There is no CPU fallback: if ``libtdag.so`` is missing or no GPU is present, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtdag.so")
+
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+
+TDAG_OK = 0
+_STATUS = {0: "OK", 1: "EINVAL", 2: "ENOMEM", 3: "EDEVICE", 4: "EINTERNAL"}
+
+
+class TdagConfig(ctypes.Structure):
+    """Byte layout of voxcom::TransformDAGConfig (transform_dag.h:157-160)."""
+
+    _fields_ = [
+        ("symmetry", ctypes.c_uint8),
+        ("axis_permutation", ctypes.c_uint8),
+        ("translation", ctypes.c_uint8),
+        ("pad", ctypes.c_uint8),
+        ("max_translation_level", ctypes.c_uint32),
+    ]
+
+
+class TdagMatchStats(ctypes.Structure):
+    _fields_ = [
+        ("num_items", ctypes.c_uint64),
+        ("num_unique", ctypes.c_uint64),
+        ("num_matched", ctypes.c_uint64),
+        ("table_entries", ctypes.c_uint64),
+        ("ms_total", ctypes.c_float),
+        ("ms_search", ctypes.c_float),
+    ]
+
+
+@dataclass
+class TransformDAGConfig:
+    """Mirror of voxcom::TransformDAGConfig with the reference defaults."""
+
+    symmetry: bool = True
+    axis_permutation: bool = True
+    translation: bool = True
+    max_translation_level: int = 4
+
+    def to_c(self) -> TdagConfig:
+        return TdagConfig(int(self.symmetry), int(self.axis_permutation), int(self.translation), 0, int(self.max_translation_level))
+
+
+class TdagError(RuntimeError):
+    """Raised when a C-ABI call returns a non-OK status (mirrors assert_always's throw)."""
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Loads libtdag.so. Raises if it was not built: there is no fallback path."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise TdagError(f"{LIB_PATH} is missing: build it with `python build.py` (hipcc, gfx950)")
+    L = ctypes.CDLL(LIB_PATH)
+    L.tdag_last_error.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+    L.tdag_last_error.restype = ctypes.c_size_t
+    L.tdag_version.restype = ctypes.c_char_p
+    L.tdag_context_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+    L.tdag_context_destroy.argtypes = [ctypes.c_void_p]
+    L.tdag_match_level_device.argtypes = [
+        ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(TdagConfig),
+        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, _u64p, ctypes.c_void_p, ctypes.POINTER(TdagMatchStats),
+    ]
+    L.tdag_match_level.argtypes = [
+        ctypes.c_void_p, ctypes.c_uint32, _u64p, ctypes.c_uint64, ctypes.POINTER(TdagConfig),
+        _u64p, _u64p, _u32p, _u64p, ctypes.POINTER(TdagMatchStats),
+    ]
+    L.tdag_last_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(TdagMatchStats)]
+    for name in ("tdag_context_create", "tdag_context_destroy", "tdag_match_level_device", "tdag_match_level", "tdag_last_stats"):
+        getattr(L, name).restype = ctypes.c_int
+    _extra_bindings(L)
+    _lib = L
+    return L
+
+
+def _extra_bindings(L):
+    """Bindings of the pipeline entry points (SVDAG build, driver, .tdag writer)."""
+    if hasattr(L, "tdag_scene_generate"):
+        from . import pipeline  # noqa: F401  (binds its own symbols)
+
+        pipeline.bind(L)
+
+
+def last_error() -> str:
+    buf = ctypes.create_string_buffer(4096)
+    lib().tdag_last_error(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def check(status: int, what: str) -> None:
+    if status != TDAG_OK:
+        raise TdagError(f"{what} failed with {_STATUS.get(status, status)}: {last_error()}")
+
+
+def version() -> str:
+    return lib().tdag_version().decode()
+
+
+def words_per_mask(level: int) -> int:
+    return max(1, (1 << (3 * level)) // 64)
+
+
+class Context:
+    """Owns a tdag_context (HIP stream + cached device scratch) on one device."""
+
+    def __init__(self, device: int = 0):
+        self._ptr = ctypes.c_void_p()
+        check(lib().tdag_context_create(int(device), ctypes.byref(self._ptr)), "tdag_context_create")
+        self.device = device
+
+    def close(self):
+        if self._ptr:
+            lib().tdag_context_destroy(self._ptr)
+            self._ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- device-resident hot path -----------------------------------------------------
+    def match_level_device(self, level: int, d_masks, n: int, config: TransformDAGConfig | None = None,
+                           d_encoded=None, d_mapping=None, d_unique=None, stream=None, want_stats=True):
+        """Runs the matcher on device buffers (torch tensors or raw device pointers).
+
+        Returns (num_unique, stats) when want_stats, else (None, None) without synchronising.
+        """
+        cfg = (config or TransformDAGConfig()).to_c()
+
+        def ptr(x):
+            if x is None:
+                return None
+            if isinstance(x, int):
+                return ctypes.c_void_p(x)
+            return ctypes.c_void_p(x.data_ptr())
+
+        nu = ctypes.c_uint64(0)
+        st = TdagMatchStats()
+        s = ptr(stream) if stream is not None and not hasattr(stream, "cuda_stream") else (
+            ctypes.c_void_p(stream.cuda_stream) if stream is not None else None)
+        check(lib().tdag_match_level_device(
+            self._ptr, int(level), ptr(d_masks), int(n), ctypes.byref(cfg), ptr(d_encoded), ptr(d_mapping), ptr(d_unique),
+            ctypes.byref(nu) if want_stats else None, s, ctypes.byref(st) if want_stats else None), "tdag_match_level_device")
+        if want_stats:
+            return int(nu.value), st
+        return None, None
+
+    # -- host-buffer drop-in ------------------------------------------------------------
+    def match_level(self, level: int, masks: np.ndarray, config: TransformDAGConfig | None = None, want_encoded=True):
+        """Host arrays in, host arrays out: (encoded, mapping, unique_indices, stats)."""
+        W = words_per_mask(level)
+        m = np.ascontiguousarray(masks, dtype=np.uint64).reshape(-1, W)
+        n = m.shape[0]
+        cfg = (config or TransformDAGConfig()).to_c()
+        enc = np.zeros(max(n, 1), dtype=np.uint64)
+        mapping = np.zeros(max(n, 1), dtype=np.uint64)
+        uniq = np.zeros(max(n, 1), dtype=np.uint32)
+        nu = ctypes.c_uint64(0)
+        st = TdagMatchStats()
+        check(lib().tdag_match_level(
+            self._ptr, int(level), m.ctypes.data_as(_u64p), n, ctypes.byref(cfg),
+            enc.ctypes.data_as(_u64p) if want_encoded else None, mapping.ctypes.data_as(_u64p), uniq.ctypes.data_as(_u32p),
+            ctypes.byref(nu), ctypes.byref(st)), "tdag_match_level")
+        return (enc[:n] if want_encoded else None), mapping[:n], uniq[: int(nu.value)], st
+
+
+_default_ctx: Context | None = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+def find_duplicates_among_flattened_items_gpu(level: int, items, grids: np.ndarray, config: TransformDAGConfig | None = None):
+    """Mirror of findDuplicatesAmongFlattenedItems_gpu (transform_dag.h:310-311).
+
+    ``items`` is any sequence of per-item payloads (indices, nodes...), ``grids`` the matching
+    LargeSubGrid masks (n x W uint64). Returns (unique_items, outPrevLevelMapping) where the
+    mapping holds TransformPointer bit patterns (uint64).
+    """
+    _, mapping, uniq, _ = default_context().match_level(level, grids, config, want_encoded=False)
+    items_arr = list(items) if not isinstance(items, np.ndarray) else items
+    if isinstance(items_arr, np.ndarray):
+        return items_arr[uniq], mapping
+    return [items_arr[i] for i in uniq], mapping
+
+
+# ---------------------------------------------------------------------------------------
+# TransformPointer helpers (transform_dag.h:19-54, 349-397), pure bit manipulation.
+# ---------------------------------------------------------------------------------------
+PERMS = ((0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0))
+
+
+def tp_unpack(bits: int) -> dict:
+    bits = int(bits)
+    return {
+        "ptr": bits & ((1 << 37) - 1),
+        "sym": (bits >> 37) & 7,
+        "perm": ((bits >> 40) & 3, (bits >> 42) & 3, (bits >> 44) & 3),
+        "t": (((bits >> 46) & 63) - 32, ((bits >> 52) & 63) - 32, ((bits >> 58) & 63) - 32),
+    }
+
+
+def tp_pack(ptr: int, sym: int = 0, perm=(0, 1, 2), t=(0, 0, 0)) -> int:
+    v = ptr & ((1 << 37) - 1)
+    v |= (sym & 7) << 37
+    v |= perm[0] << 40 | perm[1] << 42 | perm[2] << 44
+    v |= ((t[0] + 32) & 63) << 46 | ((t[1] + 32) & 63) << 52 | ((t[2] + 32) & 63) << 58
+    return v
+
+
+TP_SENTINEL = tp_pack(0xFFFFFFFF)

@@ -1,0 +1,161 @@
+/*
+ * tdag.h — C-ABI of the MI355X-native Transform-Aware SVDAG matching path.
+ *
+ * Plain pointers and sizes only. Every entry point returns a tdag_status; on failure a
+ * thread-local message is available through tdag_last_error(). No exceptions cross this
+ * boundary (the C++ wrappers in tdag.hpp rethrow them as std::runtime_error, matching the
+ * reference's assert_always behaviour).
+ *
+ * Masks ("grids") use the layout of voxcom::LargeSubGrid<R> (large_sub_grid.h:80-97):
+ * R = 2^level, bit i = x + R*y + R*R*z lives in word i >> 6, W = max(1, R^3/64) words per mask.
+ * TransformPointer values are the 64-bit bit patterns of voxcom::TransformPointer
+ * (transform_dag.h:19-54, GCC x86-64 bit-field layout): ptr bits 0-36, sx/sy/sz bits 37-39,
+ * p0/p1/p2 bits 40-45, tx/ty/tz (biased by 32) bits 46-63.
+ */
+#ifndef TDAG_H_
+#define TDAG_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum tdag_status {
+    TDAG_OK = 0,
+    TDAG_EINVAL = 1, /* level outside [2,5], null pointers, sizes out of range */
+    TDAG_ENOMEM = 2, /* device or host allocation failed */
+    TDAG_EDEVICE = 3, /* a HIP runtime call failed or no device is present */
+    TDAG_EINTERNAL = 4 /* an internal invariant did not hold */
+} tdag_status;
+
+/* Byte layout of voxcom::TransformDAGConfig (transform_dag.h:157-160): three bools, one
+ * padding byte, uint32 maxTranslationLevel. sizeof == 8. */
+typedef struct tdag_config {
+    uint8_t symmetry;
+    uint8_t axis_permutation;
+    uint8_t translation;
+    uint8_t pad;
+    uint32_t max_translation_level;
+} tdag_config;
+
+/* Library context: owns a HIP stream, cached device scratch and timing events. One call per
+ * context at a time (thread-compatible, not thread-safe). */
+typedef struct tdag_context tdag_context;
+
+/* Per-call statistics of the last matcher call on a context. */
+typedef struct tdag_match_stats {
+    uint64_t num_items;
+    uint64_t num_unique;
+    uint64_t num_matched; /* items whose minimum-key parent is another item ("verified matches") */
+    uint64_t table_entries;
+    float ms_total; /* device time of the whole call (hipEvents on the call's stream) */
+    float ms_search; /* device time of the search kernel(s) alone */
+} tdag_match_stats;
+
+/* Copies the calling thread's last error message into buf (NUL-terminated, truncated to
+ * cap). Returns the full message length. */
+size_t tdag_last_error(char* buf, size_t cap);
+
+/* Library version string, e.g. "tdag-mi355x 0.1 gfx950". */
+const char* tdag_version(void);
+
+/* Creates a context on HIP device `device`. Replaces the per-call thrust::device_vector
+ * ownership of find_translation.cu:1204-1205, 1286-1287 with cached scratch. */
+tdag_status tdag_context_create(int device, tdag_context** out_ctx);
+tdag_status tdag_context_destroy(tdag_context* ctx);
+
+/*
+ * Device-resident matcher — the hot path. Replaces the search of
+ * findDuplicatesAmongFlattenedItems_gpu / _cpp2 (transform_dag.h:307-311;
+ * find_translation.cpp:616-1000) for one level.
+ *
+ *   level      2..5 (R = 4, 8, 16, 32)
+ *   d_masks    device pointer, n * W uint64 words, items in their original order
+ *   cfg        host pointer, TransformDAGConfig layout (filters candidate transforms)
+ *   d_encoded  optional device output (n uint64): the reference's encodedParents array —
+ *              the minimum encodeFixed64 key per item, whose ptr field is the parent's rank in
+ *              the stable popcount-descending order (find_translation.cpp:631-643, 864)
+ *   d_mapping  optional device output (n uint64): outPrevLevelMapping, TransformPointer bits
+ *   d_unique   optional device output (n uint32): original indices of the unique items, in
+ *              original order; the first *out_num_unique entries are valid
+ *   out_num_unique  optional host output
+ *   stream     hipStream_t to enqueue on (NULL = the context's own stream). The call
+ *              synchronises this stream before returning when out_num_unique or stats is set.
+ */
+tdag_status tdag_match_level_device(tdag_context* ctx, uint32_t level, const uint64_t* d_masks, uint64_t n,
+    const tdag_config* cfg, uint64_t* d_encoded, uint64_t* d_mapping, uint32_t* d_unique,
+    uint64_t* out_num_unique, void* stream, tdag_match_stats* stats);
+
+/*
+ * Host-buffer matcher: findDuplicatesAmongFlattenedItems_gpu drop-in
+ * (transform_dag.h:310-311). Copies masks host->device, matches, copies results back.
+ *   out_encoded may be NULL; out_mapping (n), out_unique (n) required.
+ */
+tdag_status tdag_match_level(tdag_context* ctx, uint32_t level, const uint64_t* masks, uint64_t n,
+    const tdag_config* cfg, uint64_t* out_encoded, uint64_t* out_mapping, uint32_t* out_unique,
+    uint64_t* out_num_unique, tdag_match_stats* stats);
+
+/* Statistics of the last matcher call on ctx. */
+tdag_status tdag_last_stats(const tdag_context* ctx, tdag_match_stats* out);
+
+/* ------------------------------------------------------------------------------------------
+ * Pipeline around the matcher: scene synthesis, SVDAG producer, flattening, the
+ * constructTransformDAG driver and the .tdag writer.
+ * ---------------------------------------------------------------------------------------- */
+
+/* An SVDAG: voxcom::EditStructure<void, uint32_t> after toDAG (structure.h:159-211, 703-763). */
+typedef struct tdag_svdag tdag_svdag;
+/* A Transform-Aware SVDAG: voxcom::EditStructure<void, TransformPointer> plus per-level
+ * matcher records. */
+typedef struct tdag_tdag tdag_tdag;
+
+/* Per-level record of a tdag_construct_transform_dag run. */
+typedef struct tdag_level_record {
+    uint32_t level;
+    uint32_t has_masks; /* the matcher input masks were kept (TDAG_KEEP_LEVEL_INPUTS) */
+    uint64_t items; /* matcher input size: nodes referenced by surviving parents */
+    uint64_t unique;
+    uint64_t matched;
+    float ms_match;
+} tdag_level_record;
+
+/* Flag of tdag_construct_transform_dag: keep each level's matcher input masks. */
+#define TDAG_KEEP_LEVEL_INPUTS 1u
+
+/* Seeded synthetic scene as a dense bit grid (bit i = x + R*y + R*R*z, the VoxelGrid layout of
+ * voxel_grid.h:70-73). kind 0: 200 sphere shells + 100 box shells; kind 1: kind 0 plus a
+ * fractal value-noise heightfield shell. out_bytes >= resolution^3 / 8. */
+tdag_status tdag_scene_dense(uint32_t kind, uint32_t resolution, uint64_t seed, uint8_t* out_bits, uint64_t out_bytes);
+
+/* EditStructure<void, uint32_t>(VoxelGrid) followed by toDAG() (structure.h:423-429, 658-763). */
+tdag_status tdag_svdag_from_dense(const uint8_t* bits, uint32_t resolution, tdag_svdag** out);
+/* resolution, root level and the node count of levels [0, cap) (level 2 = leaves). */
+tdag_status tdag_svdag_info(const tdag_svdag* svdag, uint32_t* resolution, uint32_t* root_level, uint64_t* level_sizes, uint32_t cap);
+/* Copies level `level`: 8 uint32 children per node (0xFFFFFFFF = empty), or for level 2 the
+ * uint64 Morton leaf masks. */
+tdag_status tdag_svdag_copy_level(const tdag_svdag* svdag, uint32_t level, void* out);
+/* FlatGridGenerator / createLargeSubGrid (transform_dag.cpp:44-116, large_sub_grid.h:127-141):
+ * linear LargeSubGrid masks (W words each) of the given level-`level` nodes. */
+tdag_status tdag_svdag_flatten(const tdag_svdag* svdag, uint32_t level, const uint32_t* nodes, uint64_t n, uint64_t* out_masks);
+tdag_status tdag_svdag_free(tdag_svdag* svdag);
+
+/* constructTransformDAG (transform_dag.h:161-162, transform_dag.cpp:138-240): hierarchical
+ * sym+perm dedup above max_translation_level, then one GPU matcher call per level from
+ * max_translation_level down to 2. max_translation_level must lie in [2, 5]. ctx may be NULL
+ * when cfg->translation is 0 (no matcher call is made). */
+tdag_status tdag_construct_transform_dag(tdag_context* ctx, const tdag_svdag* in, const tdag_config* cfg, uint32_t flags, tdag_tdag** out);
+/* The .tdag byte stream (voxelizer/src/main.cpp:144-151): pass buf = NULL to query the size. */
+tdag_status tdag_tdag_serialize(const tdag_tdag* tdag, const tdag_config* cfg, uint8_t* buf, uint64_t cap, uint64_t* out_size);
+tdag_status tdag_tdag_info(const tdag_tdag* tdag, uint32_t* root_level, uint64_t* level_sizes, uint32_t cap);
+tdag_status tdag_tdag_level_record(const tdag_tdag* tdag, uint32_t level, tdag_level_record* out);
+/* Copies the kept matcher input masks of `level` (items * W words). */
+tdag_status tdag_tdag_level_masks(const tdag_tdag* tdag, uint32_t level, uint64_t* out, uint64_t cap_words);
+tdag_status tdag_tdag_free(tdag_tdag* tdag);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TDAG_H_ */

@@ -1,0 +1,31 @@
+// TEST INFRASTRUCTURE ONLY: C entry points of the CPU oracle (restatement of the reference's
+// Transform-Aware SVDAG path). Used by tests/, the smoke check and bench.py's cpu_baseline.
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+// Byte layout of voxcom::TransformDAGConfig (transform_dag.h:157-160) under GCC x86-64:
+// three bools, one padding byte, then uint32 maxTranslationLevel.
+typedef struct orc_config {
+    uint8_t symmetry;
+    uint8_t axisPermutation;
+    uint8_t translation;
+    uint8_t pad;
+    uint32_t maxTranslationLevel;
+} orc_config;
+
+uint64_t orc_tp_create(uint64_t ptr, uint32_t symBits, uint32_t permCode, int tx, int ty, int tz);
+uint64_t orc_tp_sentinel(void);
+uint64_t orc_encode_fixed64(uint64_t tpBits, uint32_t level);
+uint64_t orc_decode_fixed64(uint64_t key, uint32_t level);
+int orc_transform(uint32_t level, const uint64_t* in, uint32_t permCode, uint32_t symBits, int tx, int ty, int tz, uint64_t* out);
+int orc_corner_hash(uint32_t level, const uint64_t* in, uint32_t dirBits, uint64_t* outHash);
+int orc_match_encoded(uint32_t level, const uint64_t* masks, uint64_t n, const orc_config* cfg, uint64_t* outEncoded, int nthreads, int brute);
+int orc_postprocess(uint32_t level, const uint64_t* masks, uint64_t n, const uint64_t* encoded, uint64_t* outMapping, uint32_t* outUnique, uint64_t* outNumUnique);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,130 @@
+"""Pipeline rows on the CPU (no GPU): SVDAG producer, flattening, hierarchical dedup, .tdag.
+
+* The library's bottom-up SVDAG builder equals the oracle's restatement of the reference's
+  recursive constructor + toDAG, node for node (same indices, hence the same tie-breaks).
+* Flattened LargeSubGrid masks equal the oracle's structure.get-based createLargeSubGrid.
+* With translation off, constructTransformDAG is the hierarchical pass alone: its .tdag bytes
+  equal the oracle's, expand back to the original voxels, and the leaf table has the 22
+  canonical 2x2x2 classes under the 48 transforms.
+"""
+import ctypes
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+import tdag_decode
+
+SCENES = [(0, 64, 1), (0, 128, 2), (1, 128, 3), (0, 256, 0)]
+
+
+def _oracle_svdag(oracle, bits, R):
+    L = oracle.lib()
+    h = ctypes.c_void_p()
+    assert L.orc_svdag_build(bits.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), R, ctypes.byref(h)) == 0
+    return h
+
+
+@pytest.fixture(scope="module")
+def pl():
+    from tdag_amd import pipeline
+
+    import tdag_amd
+
+    tdag_amd.lib()
+    return pipeline
+
+
+@pytest.mark.parametrize("kind,R,seed", SCENES)
+def test_svdag_matches_oracle(oracle, pl, kind, R, seed):
+    bits = pl.scene_dense(kind, R, seed)
+    s = pl.Svdag(bits, R)
+    h = _oracle_svdag(oracle, bits, R)
+    L = oracle.lib()
+    assert L.orc_svdag_root_level(h) == s.root_level
+    for level in range(2, s.root_level + 1):
+        n = L.orc_svdag_level_size(h, level)
+        assert n == s.level_sizes[level]
+        if level == 2:
+            ref = np.zeros(max(1, n), dtype=np.uint64)
+            L.orc_svdag_copy_leaves(h, ref.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
+            assert np.array_equal(ref[:n], s.leaves())
+        else:
+            ref = np.zeros((max(1, n), 8), dtype=np.uint32)
+            L.orc_svdag_copy_level(h, level, ref.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+            assert np.array_equal(ref[:n], s.nodes(level))
+    L.orc_svdag_free(h)
+
+
+@pytest.mark.parametrize("level", [2, 3, 4, 5])
+def test_flatten_matches_oracle(oracle, pl, level):
+    bits = pl.scene_dense(1, 128, 5)
+    s = pl.Svdag(bits, 128)
+    h = _oracle_svdag(oracle, bits, 128)
+    rng = np.random.default_rng(level)
+    idx = rng.choice(s.level_sizes[level], size=min(200, s.level_sizes[level]), replace=False).astype(np.uint32)
+    mine = s.flatten(level, idx)
+    W = mine.shape[1]
+    ref = np.zeros((len(idx), W), dtype=np.uint64)
+    L = oracle.lib()
+    assert L.orc_flatten(h, level, idx.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), len(idx), ref.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))) == 0
+    assert np.array_equal(mine, ref)
+    L.orc_svdag_free(h)
+
+
+def _oracle_tdag_bytes(oracle, bits, R, cfg):
+    h = _oracle_svdag(oracle, bits, R)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "o.tdag")
+        n = oracle.lib().orc_construct_tdag(h, ctypes.byref(cfg), os.cpu_count() or 1, path.encode())
+        assert n > 0
+        data = open(path, "rb").read()
+    oracle.lib().orc_svdag_free(h)
+    return data
+
+
+@pytest.mark.parametrize("kind,R,seed", SCENES[:3])
+@pytest.mark.parametrize("sym,axis", [(True, True), (True, False), (False, True), (False, False)])
+def test_hierarchical_tdag_bytes(oracle, pl, kind, R, seed, sym, axis):
+    import tdag_amd
+
+    bits = pl.scene_dense(kind, R, seed)
+    cfg = tdag_amd.TransformDAGConfig(sym, axis, False, 3)
+    t = pl.construct_transform_dag(None, pl.Svdag(bits, R), cfg)
+    mine = t.tdag_bytes()
+    ref = _oracle_tdag_bytes(oracle, bits, R, oracle.config(sym, axis, False, 3))
+    assert mine == ref
+    d = tdag_decode.parse(mine)
+    assert np.array_equal(tdag_decode.expand(d), tdag_decode.dense_to_cube(bits, R))
+
+
+def test_canonical_leaf_classes():
+    """22 classes of 2x2x2 masks under the 48 mirror/permutation transforms
+    (transform_dag_hierarchical.cpp:200 log line, sym+perm on)."""
+    import itertools
+
+    def transform(m, perm, sym):
+        out = 0
+        for i in range(8):
+            if (m >> i) & 1:
+                v = (i & 1, (i >> 1) & 1, (i >> 2) & 1)
+                o = [v[perm[a]] ^ ((sym >> a) & 1) for a in range(3)]
+                out |= 1 << (o[0] | o[1] << 1 | o[2] << 2)
+        return out
+
+    orbits = set()
+    for m in range(256):
+        orbits.add(min(transform(m, p, s) for p in itertools.permutations(range(3)) for s in range(8)))
+    assert len(orbits) == 22
+
+
+def test_empty_scene(oracle, pl):
+    import tdag_amd
+
+    bits = np.zeros(64 ** 3 // 8, dtype=np.uint8)
+    s = pl.Svdag(bits, 64)
+    assert all(v == 0 for v in s.level_sizes)
+    cfg = tdag_amd.TransformDAGConfig(True, True, False, 3)
+    mine = pl.construct_transform_dag(None, s, cfg).tdag_bytes()
+    assert mine == _oracle_tdag_bytes(oracle, bits, 64, oracle.config(True, True, False, 3))
