@@ -112,7 +112,8 @@ __global__ __launch_bounds__(256) void kBuildTable2(const uint64_t* __restrict__
     }
 }
 
-// One wave per parent rank; lane = anchored voxel v' = (x, y, z) of the sub-box.
+// One wave per parent rank; lane = anchored voxel v' = (x, y, z) of the sub-box. For each
+// permutation the 8 direction keys are formed first and their table loads issued together.
 __global__ __launch_bounds__(256) void kSearch2(const uint64_t* __restrict__ sortedMasks, uint32_t n, const uint64_t* __restrict__ tabKeys, const uint32_t* __restrict__ tabVals,
     uint32_t logCap, uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t rankBase)
 {
@@ -135,17 +136,23 @@ __global__ __launch_bounds__(256) void kSearch2(const uint64_t* __restrict__ sor
         v[permAxis(pc, 2)] = vz;
         const uint32_t src = (uint32_t)(v[0] + 4 * v[1] + 16 * v[2]);
         const uint64_t Q = (uint64_t)__ballot((P >> src) & 1ull);
+        uint64_t key[8];
+        uint32_t slot[8];
+        uint64_t first[8];
+#pragma unroll
         for (uint32_t d = 0; d < 8; ++d) {
-            const uint64_t F = mirror4(Q, (~d) & 7u);
-            const uint64_t key = (F >> lane) & clip;
-            if (key == 0)
-                continue;
-            uint32_t s = homeSlot64(key, logCap);
-            for (;;) {
-                const uint64_t k = tabKeys[s];
-                if (k == 0)
-                    break;
-                if (k == key) {
+            key[d] = (mirror4(Q, (~d) & 7u) >> lane) & clip;
+            slot[d] = homeSlot64(key[d], logCap);
+        }
+#pragma unroll
+        for (uint32_t d = 0; d < 8; ++d)
+            first[d] = key[d] ? tabKeys[slot[d]] : 0ull;
+#pragma unroll
+        for (uint32_t d = 0; d < 8; ++d) {
+            uint64_t k = first[d];
+            uint32_t s = slot[d];
+            while (k != 0) {
+                if (k == key[d]) {
                     const uint32_t e = tabVals[s];
                     const uint32_t c = e >> 3;
                     const uint32_t kc = e & 7u;
@@ -166,6 +173,7 @@ __global__ __launch_bounds__(256) void kSearch2(const uint64_t* __restrict__ sor
                     }
                 }
                 s = (s + 1) & capMask;
+                k = tabKeys[s];
             }
         }
     }
@@ -206,9 +214,30 @@ __device__ __forceinline__ void lineAxes(int A, int& B, int& C)
     C = (A == 2) ? 1 : 2;
 }
 
-// Passes 1 and 2 of the nested hash for (permCode, direction d) into s_h.
+// Three bit layouts of one item in LDS: layout a has axis a fastest, so a line along axis a is
+// R consecutive bits starting at bit R * (u + R * w), where (u, w) are the line's coordinates
+// along the other two axes in ascending axis order (lineAxes).
+//   layout 0: x + R*y + R*R*z (the LargeSubGrid layout)
+//   layout 1: y + R*x + R*R*z
+//   layout 2: z + R*x + R*R*y
 template <int L>
-__device__ __forceinline__ void hashPasses12(const uint64_t* s_bits, uint32_t* s_h, uint32_t pc, uint32_t d)
+__device__ __forceinline__ uint32_t lineBits(const uint64_t* s_layouts, int A, int u, int w)
+{
+    constexpr int R = Geo<L>::R;
+    constexpr int W = Geo<L>::W;
+    const uint32_t start = (uint32_t)(R * (u + R * w));
+    const uint64_t word = s_layouts[A * W + (start >> 6)];
+    if constexpr (R == 32)
+        return (uint32_t)(word >> (start & 63u));
+    else
+        return (uint32_t)(word >> (start & 63u)) & ((1u << R) - 1u);
+}
+
+// Nested hash of the sub-box anchored at every voxel for (permCode, direction d), stored in s_h.
+// Pass k runs along axis permAxis(pc, k) in direction bit k of d ("Positive" scans from R-1
+// down to 0, so the value at v covers [v, R-1] on that axis).
+template <int L>
+__device__ __forceinline__ void hashGrid(const uint64_t* s_layouts, uint32_t* s_h, uint32_t pc, uint32_t d)
 {
     constexpr int R = Geo<L>::R;
     const int t = (int)threadIdx.x;
@@ -218,34 +247,38 @@ __device__ __forceinline__ void hashPasses12(const uint64_t* s_bits, uint32_t* s
         int B, C;
         lineAxes(A, B, C);
         const bool pos = d & 1u;
+        const uint32_t row = lineBits<L>(s_layouts, A, u, w);
         int c[3];
         c[B] = u;
         c[C] = w;
         uint32_t s = 0;
+#pragma unroll 8
         for (int k = 0; k < R; ++k) {
             c[A] = pos ? (R - 1 - k) : k;
-            s = hashStep(s, lbit<L>(s_bits, c[0], c[1], c[2]));
+            s = hashStep(s, (row >> c[A]) & 1u);
             s_h[sidx<L>(c[0], c[1], c[2])] = s;
         }
     }
     __syncthreads();
-    {
-        const int A = (int)permAxis(pc, 1);
+#pragma unroll
+    for (int pass = 1; pass < 3; ++pass) {
+        const int A = (int)permAxis(pc, pass);
         int B, C;
         lineAxes(A, B, C);
-        const bool pos = (d >> 1) & 1u;
+        const bool pos = (d >> pass) & 1u;
         int c[3];
         c[B] = u;
         c[C] = w;
         uint32_t s = 0;
+#pragma unroll 8
         for (int k = 0; k < R; ++k) {
             c[A] = pos ? (R - 1 - k) : k;
             const uint32_t i = sidx<L>(c[0], c[1], c[2]);
             s = hashStep(s, s_h[i]);
             s_h[i] = s;
         }
+        __syncthreads();
     }
-    __syncthreads();
 }
 
 // Expected child word `wi` for the transform encoded in `key` applied to the parent bits.
@@ -264,8 +297,7 @@ __device__ __forceinline__ uint64_t transformedWord(const uint64_t* s_bits, uint
     uint64_t out = 0;
     for (int j = 0; j < 64; ++j) {
         const uint32_t i = wi * 64u + (uint32_t)j;
-        int o[3] = { (int)(i & (R - 1)), (int)((i >> L) & (R - 1)), (int)(i >> (2 * L)) };
-        int q[3] = { o[0] - tx, o[1] - ty, o[2] - tz }; // translate: out[o] = M[o - t]
+        int q[3] = { (int)(i & (R - 1)) - tx, (int)((i >> L) & (R - 1)) - ty, (int)(i >> (2 * L)) - tz }; // out[o] = M[o - t]
         if (q[0] < 0 || q[1] < 0 || q[2] < 0 || q[0] >= R || q[1] >= R || q[2] >= R)
             continue;
         if (sym & 1u)
@@ -283,86 +315,107 @@ __device__ __forceinline__ uint64_t transformedWord(const uint64_t* s_bits, uint
     return out;
 }
 
-// One block per item; the hash of the whole grid seen from each of its 8 corners goes into the
-// table (identity permutation, direction = corner).
 template <int L>
-__global__ __launch_bounds__(Geo<L>::Threads) void kCornerTable(const uint64_t* __restrict__ sortedMasks, uint32_t n, uint64_t* __restrict__ table, uint32_t logCap)
+__device__ __forceinline__ bool loadLayouts(const uint64_t* __restrict__ layouts, uint32_t n, uint32_t r, uint64_t* s_layouts, uint32_t* s_any)
+{
+    constexpr int W = Geo<L>::W;
+    if (threadIdx.x == 0)
+        *s_any = 0;
+    __syncthreads();
+    for (int i = (int)threadIdx.x; i < 3 * W; i += Geo<L>::Threads) {
+        const int lay = i / W, wi = i % W;
+        const uint64_t v = layouts[((size_t)lay * n + r) * W + wi];
+        s_layouts[i] = v;
+        if (v)
+            atomicOr(s_any, 1u);
+    }
+    __syncthreads();
+    return *s_any != 0;
+}
+
+// Transposed bit layouts 1 and 2 of every item (layout 0 is the sorted masks themselves).
+template <int L>
+__global__ __launch_bounds__(256) void kLayouts(uint64_t* __restrict__ layouts, uint32_t n)
 {
     constexpr int R = Geo<L>::R;
     constexpr int W = Geo<L>::W;
     __shared__ uint64_t s_bits[W];
+    const uint32_t r = blockIdx.x;
+    for (int i = (int)threadIdx.x; i < W; i += 256)
+        s_bits[i] = layouts[(size_t)r * W + i];
+    __syncthreads();
+    for (int i = (int)threadIdx.x; i < 2 * W; i += 256) {
+        const int lay = 1 + i / W, wi = i % W;
+        uint64_t out = 0;
+        for (int j = 0; j < 64; ++j) {
+            const uint32_t k = (uint32_t)(wi * 64 + j);
+            const int f = (int)(k & (R - 1)), a = (int)((k >> L) & (R - 1)), b = (int)(k >> (2 * L));
+            // layout 1: (y, x, z) = (f, a, b); layout 2: (z, x, y) = (f, a, b)
+            const int x = a;
+            const int y = lay == 1 ? f : b;
+            const int z = lay == 1 ? b : f;
+            out |= (uint64_t)lbit<L>(s_bits, x, y, z) << j;
+        }
+        layouts[((size_t)lay * n + r) * W + wi] = out;
+    }
+}
+
+// One block per item; the hash of the whole grid seen from each of its 8 corners goes into the
+// table (identity permutation, direction = corner; the anchor is the corner itself).
+template <int L>
+__global__ __launch_bounds__(Geo<L>::Threads) void kCornerTable(const uint64_t* __restrict__ layouts, uint32_t n, uint64_t* __restrict__ table, uint32_t logCap)
+{
+    constexpr int R = Geo<L>::R;
+    constexpr int W = Geo<L>::W;
+    __shared__ uint64_t s_layouts[3 * W];
     __shared__ uint32_t s_h[Geo<L>::R3];
     __shared__ uint32_t s_any;
     const uint32_t r = blockIdx.x;
-    if (threadIdx.x == 0)
-        s_any = 0;
-    __syncthreads();
-    for (int i = (int)threadIdx.x; i < W; i += Geo<L>::Threads) {
-        const uint64_t v = sortedMasks[(size_t)r * W + i];
-        s_bits[i] = v;
-        if (v)
-            atomicOr(&s_any, 1u);
-    }
-    __syncthreads();
-    if (!s_any)
+    if (!loadLayouts<L>(layouts, n, r, s_layouts, &s_any))
         return;
-    const int t = (int)threadIdx.x;
-    const int x = t & (R - 1), y = t >> L;
     const uint32_t capMask = (1u << logCap) - 1u;
     for (uint32_t corner = 0; corner < 8; ++corner) {
-        hashPasses12<L>(s_bits, s_h, 0, corner);
-        const bool pos = (corner >> 2) & 1u;
-        uint32_t s = 0;
-        for (int k = 0; k < R; ++k) {
-            const int z = pos ? (R - 1 - k) : k;
-            s = hashStep(s, s_h[sidx<L>(x, y, z)]);
-        }
-        const int ax = (corner & 1u) ? 0 : R - 1;
-        const int ay = (corner & 2u) ? 0 : R - 1;
-        if (x == ax && y == ay && s != 0) {
-            const uint64_t packed = ((uint64_t)s << 32) | (uint64_t)((r << 3) | corner);
-            uint32_t slot = homeSlot32(s, logCap);
-            for (;;) {
-                const unsigned long long prev = atomicCAS((unsigned long long*)&table[slot], 0ull, (unsigned long long)packed);
-                if (prev == 0ull)
-                    break;
-                slot = (slot + 1) & capMask;
+        hashGrid<L>(s_layouts, s_h, 0, corner);
+        if (threadIdx.x == 0) {
+            const int ax = (corner & 1u) ? 0 : R - 1;
+            const int ay = (corner & 2u) ? 0 : R - 1;
+            const int az = (corner & 4u) ? 0 : R - 1;
+            const uint32_t s = s_h[sidx<L>(ax, ay, az)];
+            if (s != 0) {
+                const uint64_t packed = ((uint64_t)s << 32) | (uint64_t)((r << 3) | corner);
+                uint32_t slot = homeSlot32(s, logCap);
+                for (;;) {
+                    const unsigned long long prev = atomicCAS((unsigned long long*)&table[slot], 0ull, (unsigned long long)packed);
+                    if (prev == 0ull)
+                        break;
+                    slot = (slot + 1) & capMask;
+                }
             }
         }
         __syncthreads();
     }
 }
 
-// One block per (parent rank, group of combos). Threads own one line per pass; pass 3 probes
-// the table at every voxel with a non-empty sub-box. Hits are verified cooperatively by the
-// wave that found them.
+// One block per (parent rank, group of combos). For each combo the block builds the hash grid
+// in LDS, then every thread probes R voxels, kProbeBatch table loads in flight at a time. Hits
+// that pass the key checks are verified cooperatively by the wave that found them.
 template <int L>
-__global__ __launch_bounds__(Geo<L>::Threads) void kSearchNested(const uint64_t* __restrict__ sortedMasks, uint32_t n, const uint64_t* __restrict__ table, uint32_t logCap,
-    uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t combosPerBlock, uint32_t rankBase)
+__global__ __launch_bounds__(Geo<L>::Threads) void kSearchNested(const uint64_t* __restrict__ layouts, uint32_t n, const uint64_t* __restrict__ table, uint32_t logCap,
+    uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t combosPerBlock)
 {
     constexpr int R = Geo<L>::R;
     constexpr int W = Geo<L>::W;
-    __shared__ uint64_t s_bits[W];
+    constexpr int T = Geo<L>::Threads;
+    constexpr int PerThread = Geo<L>::R3 / T; // == R
+    constexpr int Batch = PerThread < 8 ? PerThread : 8;
+    __shared__ uint64_t s_layouts[3 * W];
     __shared__ uint32_t s_h[Geo<L>::R3];
     __shared__ uint32_t s_any;
-    const uint32_t r = rankBase + blockIdx.x;
-    if (r >= n)
+    const uint32_t r = blockIdx.x;
+    if (!loadLayouts<L>(layouts, n, r, s_layouts, &s_any))
         return;
-    if (threadIdx.x == 0)
-        s_any = 0;
-    __syncthreads();
-    for (int i = (int)threadIdx.x; i < W; i += Geo<L>::Threads) {
-        const uint64_t v = sortedMasks[(size_t)r * W + i];
-        s_bits[i] = v;
-        if (v)
-            atomicOr(&s_any, 1u);
-    }
-    __syncthreads();
-    if (!s_any)
-        return;
+    const uint64_t* sortedMasks = layouts; // layout 0
     const uint32_t lane = threadIdx.x & 63u;
-    const int t = (int)threadIdx.x;
-    const int u = t & (R - 1), w = t >> L;
     const uint32_t capMask = (1u << logCap) - 1u;
     const uint32_t comboBegin = blockIdx.y * combosPerBlock;
     const uint32_t comboEnd = min(comboBegin + combosPerBlock, 48u);
@@ -371,68 +424,83 @@ __global__ __launch_bounds__(Geo<L>::Threads) void kSearchNested(const uint64_t*
         const uint32_t d = combo & 7u;
         if (!(cfgBits & kCfgAxis) && pc != 0)
             continue; // uniform across the block
-        hashPasses12<L>(s_bits, s_h, pc, d);
+        hashGrid<L>(s_layouts, s_h, pc, d);
         const int a0 = (int)permAxis(pc, 0), a1 = (int)permAxis(pc, 1), a2 = (int)permAxis(pc, 2);
-        int B, C;
-        lineAxes(a2, B, C);
-        const bool pos = (d >> 2) & 1u;
-        const int anchor[3] = { (d & 1u) ? 0 : R - 1, (d & 2u) ? 0 : R - 1, (d & 4u) ? 0 : R - 1 };
-        int c[3];
-        c[B] = u;
-        c[C] = w;
-        uint32_t s = 0;
-        for (int k = 0; k < R; ++k) {
-            c[a2] = pos ? (R - 1 - k) : k;
-            s = hashStep(s, s_h[sidx<L>(c[0], c[1], c[2])]);
-            // Voxel in the permuted frame (applyAxisPermutation, transform_dag.h:182-189).
-            const int voxel[3] = { c[a0], c[a1], c[a2] };
-            bool active = (s != 0);
-            uint32_t slot = active ? homeSlot32(s, logCap) : 0u;
-            while (__ballot(active)) {
-                bool found = false;
-                uint32_t hitC = 0;
-                uint64_t hitKey = 0;
-                while (active) {
-                    const uint64_t e = table[slot];
-                    if (e == 0) {
-                        active = false;
-                        break;
-                    }
-                    slot = (slot + 1) & capMask;
-                    if ((uint32_t)(e >> 32) != s)
-                        continue;
-                    const uint32_t val = (uint32_t)e;
-                    const uint32_t cc = val >> 3;
-                    const uint32_t kc = val & 7u;
-                    if (cc <= r)
-                        continue;
-                    const uint32_t sym = d ^ kc;
-                    int tt[3];
+        for (int j0 = 0; j0 < PerThread; j0 += Batch) {
+            uint32_t hv[Batch];
+            uint32_t sl[Batch];
+            uint64_t ent[Batch];
 #pragma unroll
-                    for (int a = 0; a < 3; ++a)
-                        tt[a] = ((sym >> a) & 1u) ? voxel[a] - anchor[a] : anchor[a] - voxel[a];
-                    if (!configAllows(cfgBits, pc, sym, tt[0], tt[1], tt[2]))
-                        continue;
-                    const uint64_t k64 = encodeKey(r, pc, sym, tt[0], tt[1], tt[2], L);
-                    if (k64 >= loadRelaxed(&best[cc]))
-                        continue;
-                    found = true;
-                    hitC = cc;
-                    hitKey = k64;
-                    break;
-                }
-                uint64_t pending = __ballot(found);
-                while (pending) {
-                    const int leader = __ffsll((unsigned long long)pending) - 1;
-                    const uint32_t bc = (uint32_t)__shfl((int)hitC, leader, 64);
-                    const uint64_t bk = (uint64_t)__shfl((long long)hitKey, leader, 64);
-                    bool ok = true;
-                    for (uint32_t wi = lane; wi < (uint32_t)W; wi += 64u)
-                        ok = ok && (transformedWord<L>(s_bits, bk, wi) == sortedMasks[(size_t)bc * W + wi]);
-                    const bool allOk = (__ballot(!ok) == 0ull);
-                    if ((int)lane == leader && allOk)
-                        atomicMin((unsigned long long*)&best[bc], (unsigned long long)bk);
-                    pending &= pending - 1ull;
+            for (int b = 0; b < Batch; ++b) {
+                hv[b] = s_h[(j0 + b) * T + threadIdx.x];
+                sl[b] = homeSlot32(hv[b], logCap);
+            }
+#pragma unroll
+            for (int b = 0; b < Batch; ++b)
+                ent[b] = hv[b] ? table[sl[b]] : 0ull;
+#pragma unroll
+            for (int b = 0; b < Batch; ++b) {
+                const uint32_t h = hv[b];
+                bool active = h != 0;
+                uint32_t slot = sl[b];
+                uint64_t e = ent[b];
+                while (__ballot(active)) {
+                    bool found = false;
+                    uint32_t hitC = 0;
+                    uint64_t hitKey = 0;
+                    while (active) {
+                        if (e == 0) {
+                            active = false;
+                            break;
+                        }
+                        const uint64_t cur = e;
+                        slot = (slot + 1) & capMask;
+                        const bool same = (uint32_t)(cur >> 32) == h;
+                        if (same) {
+                            const uint32_t cc = (uint32_t)cur >> 3;
+                            const uint32_t kc = (uint32_t)cur & 7u;
+                            if (cc > r) {
+                                // Voxel of this hash (undo the LDS swizzle), in the permuted frame.
+                                const uint32_t idx = (uint32_t)((j0 + b) * T) + threadIdx.x;
+                                const int z = (int)(idx >> (2 * L));
+                                const int y = (int)((idx >> L) & (R - 1));
+                                const int x = (int)(idx & (R - 1)) ^ (y & (R - 1));
+                                const int c3[3] = { x, y, z };
+                                const int voxel[3] = { c3[a0], c3[a1], c3[a2] };
+                                const uint32_t sym = d ^ kc;
+                                int tt[3];
+#pragma unroll
+                                for (int a = 0; a < 3; ++a) {
+                                    const int anchor = ((d >> a) & 1u) ? 0 : R - 1;
+                                    tt[a] = ((sym >> a) & 1u) ? voxel[a] - anchor : anchor - voxel[a];
+                                }
+                                if (configAllows(cfgBits, pc, sym, tt[0], tt[1], tt[2])) {
+                                    const uint64_t k64 = encodeKey(r, pc, sym, tt[0], tt[1], tt[2], L);
+                                    if (k64 < loadRelaxed(&best[cc])) {
+                                        found = true;
+                                        hitC = cc;
+                                        hitKey = k64;
+                                    }
+                                }
+                            }
+                        }
+                        e = table[slot];
+                        if (found)
+                            break;
+                    }
+                    uint64_t pending = __ballot(found);
+                    while (pending) {
+                        const int leader = __ffsll((unsigned long long)pending) - 1;
+                        const uint32_t bc = (uint32_t)__shfl((int)hitC, leader, 64);
+                        const uint64_t bk = (uint64_t)__shfl((long long)hitKey, leader, 64);
+                        bool ok = true;
+                        for (uint32_t wi = lane; wi < (uint32_t)W; wi += 64u)
+                            ok = ok && (transformedWord<L>(s_layouts, bk, wi) == sortedMasks[(size_t)bc * W + wi]);
+                        const bool allOk = (__ballot(!ok) == 0ull);
+                        if ((int)lane == leader && allOk)
+                            atomicMin((unsigned long long*)&best[bc], (unsigned long long)bk);
+                        pending &= pending - 1ull;
+                    }
                 }
             }
         }
@@ -510,7 +578,7 @@ MatchPlan planMatch(uint32_t level, uint64_t n)
     p.R3 = p.R * p.R * p.R;
     p.W = p.R3 >= 64 ? p.R3 / 64 : 1;
     const uint64_t entries = 8 * (n > 0 ? n : 1);
-    p.logCap = std::max<uint32_t>(6, log2Ceil(2 * entries));
+    p.logCap = std::max<uint32_t>(6, log2Ceil(4 * entries)); // load factor <= 1/4
     const uint64_t cap = 1ull << p.logCap;
     size_t off = 0;
     auto take = [&](size_t bytes) {
@@ -521,7 +589,7 @@ MatchPlan planMatch(uint32_t level, uint64_t n)
     p.offKeys = take(n * 4);
     p.offOrder = take(n * 4);
     p.offSortScratch = take(radixScratchBytes(n, 4));
-    p.offSortedMasks = take(n * p.W * 8);
+    p.offSortedMasks = take((level == 2 ? 1 : 3) * n * p.W * 8); // layouts 0..2 for levels >= 3
     p.offBest = take(n * 8);
     p.offTabKeys = take(cap * 8);
     p.offTabVals = take(level == 2 ? cap * 4 : 0);
@@ -581,19 +649,22 @@ hipError_t enqueueMatch(const MatchPlan& p, const uint64_t* d_masks, uint32_t cf
         hipLaunchKernelGGL(kGatherInit, dim3(blocks), dim3(256), 0, stream, d_masks, order, keys, n, W, p.R3, p.level, sortedMasks, best);
         TDAG_TRY(hipGetLastError());
     }
-    // 3. Child table.
+    // 3. Child table (levels >= 3 first build the transposed bit layouts).
     TDAG_TRY(fillU64(tabKeys, cap, 0ull, stream));
     switch (p.level) {
     case 2:
         hipLaunchKernelGGL(kBuildTable2, dim3((unsigned)ceilDiv((size_t)n * 8, 256)), dim3(256), 0, stream, sortedMasks, n, tabKeys, tabVals, p.logCap);
         break;
     case 3:
+        hipLaunchKernelGGL(kLayouts<3>, dim3(n), dim3(256), 0, stream, sortedMasks, n);
         hipLaunchKernelGGL(kCornerTable<3>, dim3(n), dim3(Geo<3>::Threads), 0, stream, sortedMasks, n, tabKeys, p.logCap);
         break;
     case 4:
+        hipLaunchKernelGGL(kLayouts<4>, dim3(n), dim3(256), 0, stream, sortedMasks, n);
         hipLaunchKernelGGL(kCornerTable<4>, dim3(n), dim3(Geo<4>::Threads), 0, stream, sortedMasks, n, tabKeys, p.logCap);
         break;
     case 5:
+        hipLaunchKernelGGL(kLayouts<5>, dim3(n), dim3(256), 0, stream, sortedMasks, n);
         hipLaunchKernelGGL(kCornerTable<5>, dim3(n), dim3(Geo<5>::Threads), 0, stream, sortedMasks, n, tabKeys, p.logCap);
         break;
     default:
@@ -608,13 +679,13 @@ hipError_t enqueueMatch(const MatchPlan& p, const uint64_t* d_masks, uint32_t cf
         hipLaunchKernelGGL(kSearch2, dim3((unsigned)ceilDiv(n, 4)), dim3(256), 0, stream, sortedMasks, n, tabKeys, tabVals, p.logCap, best, cfgBits, 0u);
         break;
     case 3:
-        hipLaunchKernelGGL(kSearchNested<3>, dim3(n, 1), dim3(Geo<3>::Threads), 0, stream, sortedMasks, n, tabKeys, p.logCap, best, cfgBits, 48u, 0u);
+        hipLaunchKernelGGL(kSearchNested<3>, dim3(n, 1), dim3(Geo<3>::Threads), 0, stream, sortedMasks, n, tabKeys, p.logCap, best, cfgBits, 48u);
         break;
     case 4:
-        hipLaunchKernelGGL(kSearchNested<4>, dim3(n, 6), dim3(Geo<4>::Threads), 0, stream, sortedMasks, n, tabKeys, p.logCap, best, cfgBits, 8u, 0u);
+        hipLaunchKernelGGL(kSearchNested<4>, dim3(n, 6), dim3(Geo<4>::Threads), 0, stream, sortedMasks, n, tabKeys, p.logCap, best, cfgBits, 8u);
         break;
     case 5:
-        hipLaunchKernelGGL(kSearchNested<5>, dim3(n, 48), dim3(Geo<5>::Threads), 0, stream, sortedMasks, n, tabKeys, p.logCap, best, cfgBits, 1u, 0u);
+        hipLaunchKernelGGL(kSearchNested<5>, dim3(n, 48), dim3(Geo<5>::Threads), 0, stream, sortedMasks, n, tabKeys, p.logCap, best, cfgBits, 1u);
         break;
     }
     TDAG_TRY(hipGetLastError());

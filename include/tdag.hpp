@@ -1,0 +1,124 @@
+// tdag.hpp — C++ drop-ins over the C-ABI in tdag.h, mirroring the reference's entry points:
+//
+//   findDuplicatesAmongFlattenedItems_gpu<Level, Item>   transform_dag.h:310-311
+//   constructTransformDAG(structure, config)              transform_dag.h:161-162
+//
+// Header-only; link against libtdag.so. Errors are rethrown as std::runtime_error, which is
+// what the reference's assert_always does (error_handling.h:17-45).
+#pragma once
+
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <span>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "tdag.h"
+
+namespace tdag_cpp {
+
+inline void check(tdag_status s, const char* what)
+{
+    if (s != TDAG_OK) {
+        char buf[1024];
+        tdag_last_error(buf, sizeof(buf));
+        throw std::runtime_error(std::string(what) + ": " + buf);
+    }
+}
+
+// Owns a tdag_context (one HIP stream, cached device scratch). Not thread-safe.
+class Context {
+public:
+    explicit Context(int device = 0) { check(tdag_context_create(device, &m_ctx), "tdag_context_create"); }
+    ~Context() { tdag_context_destroy(m_ctx); }
+    Context(const Context&) = delete;
+    Context& operator=(const Context&) = delete;
+    tdag_context* get() const { return m_ctx; }
+
+    static Context& instance()
+    {
+        static Context ctx(0);
+        return ctx;
+    }
+
+private:
+    tdag_context* m_ctx = nullptr;
+};
+
+inline tdag_config toC(bool symmetry, bool axisPermutation, bool translation, uint32_t maxTranslationLevel)
+{
+    tdag_config c {};
+    c.symmetry = symmetry;
+    c.axis_permutation = axisPermutation;
+    c.translation = translation;
+    c.max_translation_level = maxTranslationLevel;
+    return c;
+}
+
+// findDuplicatesAmongFlattenedItems_gpu<Level, Item> (transform_dag.h:310-311).
+//   items    one payload per grid (node index, leaf, ...)
+//   grids    n * W uint64 words in the LargeSubGrid<2^Level> layout (large_sub_grid.h:80-97);
+//            a std::span<const LargeSubGrid<R>> can be passed as
+//            {reinterpret_cast<const uint64_t*>(grids.data()), grids.size() * W}
+//   outPrevLevelMapping  resized to n; TransformPointer bit patterns (transform_dag.h:24-33)
+// Returns the unique items in original order, as the reference does.
+template <uint32_t Level, typename Item>
+std::vector<Item> findDuplicatesAmongFlattenedItems_gpu(std::span<const Item> items, std::span<const uint64_t> grids,
+    std::vector<uint64_t>& outPrevLevelMapping, const tdag_config& config, Context& ctx = Context::instance())
+{
+    static_assert(Level >= 2 && Level <= 5, "levels 2..5 are supported");
+    constexpr size_t R = size_t(1) << Level;
+    constexpr size_t W = R * R * R / 64 > 0 ? R * R * R / 64 : 1;
+    if (grids.size() != items.size() * W)
+        throw std::runtime_error("findDuplicatesAmongFlattenedItems_gpu: grids.size() != items.size() * W");
+    const uint64_t n = items.size();
+    outPrevLevelMapping.resize(n);
+    std::vector<uint32_t> unique(n);
+    uint64_t numUnique = 0;
+    check(tdag_match_level(ctx.get(), Level, grids.data(), n, &config, nullptr, outPrevLevelMapping.data(), unique.data(), &numUnique, nullptr),
+        "tdag_match_level");
+    std::vector<Item> out;
+    out.reserve(numUnique);
+    for (uint64_t k = 0; k < numUnique; ++k)
+        out.push_back(items[unique[k]]);
+    return out;
+}
+
+// RAII owners of the pipeline handles.
+struct SvdagDeleter {
+    void operator()(tdag_svdag* p) const { tdag_svdag_free(p); }
+};
+struct TdagDeleter {
+    void operator()(tdag_tdag* p) const { tdag_tdag_free(p); }
+};
+using Svdag = std::unique_ptr<tdag_svdag, SvdagDeleter>;
+using TDag = std::unique_ptr<tdag_tdag, TdagDeleter>;
+
+inline Svdag svdagFromDense(const uint8_t* bits, uint32_t resolution)
+{
+    tdag_svdag* s = nullptr;
+    check(tdag_svdag_from_dense(bits, resolution, &s), "tdag_svdag_from_dense");
+    return Svdag(s);
+}
+
+// constructTransformDAG (transform_dag.h:161-162) on the GPU matcher.
+inline TDag constructTransformDAG(const tdag_svdag* inDag, const tdag_config& config, Context& ctx = Context::instance())
+{
+    tdag_tdag* t = nullptr;
+    check(tdag_construct_transform_dag(ctx.get(), inDag, &config, 0, &t), "tdag_construct_transform_dag");
+    return TDag(t);
+}
+
+// The .tdag byte stream the voxelizer writes (voxelizer/src/main.cpp:144-151).
+inline std::vector<uint8_t> tdagBytes(const tdag_tdag* t, const tdag_config& config)
+{
+    uint64_t size = 0;
+    check(tdag_tdag_serialize(t, &config, nullptr, 0, &size), "tdag_tdag_serialize");
+    std::vector<uint8_t> out(size);
+    check(tdag_tdag_serialize(t, &config, out.data(), out.size(), &size), "tdag_tdag_serialize");
+    return out;
+}
+
+} // namespace tdag_cpp
