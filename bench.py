@@ -107,6 +107,10 @@ def main():
     ap.add_argument("--max-level", type=int, default=5)
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU oracle time for cpu_baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--levels", default="", help="comma-separated subset of levels to time (default: all)")
+    ap.add_argument("--no-autotune", action="store_true", help="use the library's default search variants instead of tuning them first")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="only run a device-to-device copy of this many MiB three times (reference kernel for HBM byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -119,6 +123,15 @@ def main():
     import tdag_amd
 
     torch.cuda.init()
+    if args.calibrate_copy_mib:
+        n = args.calibrate_copy_mib * (1 << 20) // 8
+        src = torch.arange(n, dtype=torch.int64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": n * 8, "read_bytes": n * 8, "write_bytes": n * 8}))
+        return
     ctx = tdag_amd.Context(0)
     cfg, levels, info = build_workload(ctx, args.resolution, args.seed, args.max_level)
     print(f"[bench] workload ready: {info} items={ {l: v['items'] for l, v in levels.items()} }", file=sys.stderr, flush=True)
@@ -134,6 +147,9 @@ def main():
             "n": n,
         }
     order = sorted(levels, reverse=True)
+    if args.levels:
+        keep = {int(v) for v in args.levels.split(",")}
+        order = [l for l in order if l in keep]
 
     def step(record):
         total = 0
@@ -146,6 +162,13 @@ def main():
                 record[level]["total_ms"].append(st.ms_total)
         return total
 
+    tuning = {}
+    if not args.no_autotune:
+        for level in order:
+            if level >= 3 and dev[level]["n"] > 0:
+                best, ms = ctx.autotune(level, dev[level]["masks"], dev[level]["n"], cfg, repeats=1)
+                tuning[level] = {"variant": best, "search_ms_by_variant": ms}
+        print(f"[bench] autotuned search variants: {tuning}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         step(None)
     rec = {level: {"search_ms": [], "total_ms": []} for level in order}
@@ -158,7 +181,7 @@ def main():
     elapsed = time.perf_counter() - t0
 
     per_step_matches = matches // max(1, args.steps)
-    expected = sum(v["matched"] for v in levels.values())
+    expected = sum(levels[l]["matched"] for l in order)
     if per_step_matches != expected:
         print(f"[bench] WARNING: matches per step {per_step_matches} != driver's {expected}", file=sys.stderr)
 
@@ -204,11 +227,13 @@ def main():
             "tdag_bytes": info["tdag_bytes"],
             "matches_per_step": per_step_matches,
             "levels": per_level,
+            "search_variants": {l: ctx.get_variant(l) for l in order if l >= 3},
+            "autotune": tuning,
             "parallelism": "single GPU",
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": f"kSearch{'2' if dom == 2 else 'Nested<%d>' % dom} (level {dom})",
+            "kernel": ("kSearch2" if dom == 2 else f"kSearch<{dom}> variant {ctx.get_variant(dom)}") + f" (level {dom})",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -20,6 +20,7 @@ struct tdag_context {
     uint64_t* pinned = nullptr; // 4 words of pinned host memory for small readbacks
     hipEvent_t ev[4] = { nullptr, nullptr, nullptr, nullptr };
     tdag_match_stats last {};
+    int32_t variant[6] = { -1, -1, -1, -1, -1, -1 }; // per level; -1 = library default
 };
 
 namespace {
@@ -158,7 +159,8 @@ tdag_status tdag_match_level_device(tdag_context* ctx, uint32_t level, const uin
         return fail(TDAG_EINVAL, "d_masks is null");
     CAPI_HIP(hipSetDevice(ctx->device), "hipSetDevice");
     hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
-    const tdag::MatchPlan plan = tdag::planMatch(level, n);
+    const int32_t v = ctx->variant[level];
+    const tdag::MatchPlan plan = tdag::planMatch(level, n, v >= 0 ? (uint32_t)v : tdag::defaultVariant(level));
     if (tdag_status s = ensureBuffer(ctx->scratch, ctx->scratchBytes, plan.scratchBytes, "matcher scratch allocation"); s != TDAG_OK)
         return s;
     CAPI_HIP(hipEventRecord(ctx->ev[0], st), "hipEventRecord");
@@ -195,7 +197,7 @@ tdag_status tdag_match_level(tdag_context* ctx, uint32_t level, const uint64_t* 
     if (n > 0 && (!masks || !out_mapping || !out_unique))
         return fail(TDAG_EINVAL, "masks, out_mapping and out_unique are required");
     CAPI_HIP(hipSetDevice(ctx->device), "hipSetDevice");
-    const tdag::MatchPlan plan = tdag::planMatch(level, n);
+    const tdag::MatchPlan plan = tdag::planMatch(level, n, tdag::defaultVariant(level));
     const size_t maskBytes = (size_t)n * plan.W * 8;
     const size_t a = (maskBytes + 255) / 256 * 256;
     const size_t b = ((size_t)n * 8 + 255) / 256 * 256;
@@ -223,6 +225,66 @@ tdag_status tdag_match_level(tdag_context* ctx, uint32_t level, const uint64_t* 
         *out_num_unique = numUnique;
     if (stats)
         *stats = st;
+    return TDAG_OK;
+}
+
+tdag_status tdag_context_set_variant(tdag_context* ctx, uint32_t level, int32_t variant)
+{
+    if (!ctx || level < 2 || level > 5)
+        return fail(TDAG_EINVAL, "bad context or level");
+    if (variant >= 0 && !tdag::variantValid(level, (uint32_t)variant))
+        return fail(TDAG_EINVAL, "variant " + std::to_string(variant) + " is not valid at level " + std::to_string(level));
+    ctx->variant[level] = variant < 0 ? -1 : variant;
+    return TDAG_OK;
+}
+
+tdag_status tdag_context_get_variant(const tdag_context* ctx, uint32_t level, int32_t* variant)
+{
+    if (!ctx || !variant || level < 2 || level > 5)
+        return fail(TDAG_EINVAL, "bad argument");
+    *variant = ctx->variant[level] >= 0 ? ctx->variant[level] : (int32_t)tdag::defaultVariant(level);
+    return TDAG_OK;
+}
+
+tdag_status tdag_autotune_level(tdag_context* ctx, uint32_t level, const uint64_t* d_masks, uint64_t n, const tdag_config* cfg, uint32_t repeats,
+    float* out_ms, int32_t* out_best)
+{
+    if (tdag_status s = checkArgs(ctx, level, n, cfg); s != TDAG_OK)
+        return s;
+    if (n > 0 && !d_masks)
+        return fail(TDAG_EINVAL, "d_masks is null");
+    CAPI_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    repeats = std::max<uint32_t>(1, repeats);
+    int32_t best = -1;
+    float bestMs = 0.f;
+    for (uint32_t v = 0; v < tdag::kNumVariants; ++v) {
+        if (out_ms)
+            out_ms[v] = -1.f;
+        if (!tdag::variantValid(level, v))
+            continue;
+        const tdag::MatchPlan plan = tdag::planMatch(level, n, v);
+        if (tdag_status s = ensureBuffer(ctx->scratch, ctx->scratchBytes, plan.scratchBytes, "matcher scratch allocation"); s != TDAG_OK)
+            return s;
+        float total = 0.f;
+        for (uint32_t rep = 0; rep <= repeats; ++rep) { // the first run warms caches and is not counted
+            CAPI_HIP(tdag::enqueueMatch(plan, d_masks, cfgBitsOf(cfg), ctx->scratch, ctx->stream, nullptr, nullptr, nullptr, ctx->ev[1], ctx->ev[2]), "matcher launch");
+            CAPI_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+            float ms = 0.f;
+            CAPI_HIP(hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]), "hipEventElapsedTime");
+            if (rep > 0)
+                total += ms;
+        }
+        const float avg = total / repeats;
+        if (out_ms)
+            out_ms[v] = avg;
+        if (best < 0 || avg < bestMs) {
+            best = (int32_t)v;
+            bestMs = avg;
+        }
+    }
+    ctx->variant[level] = best;
+    if (out_best)
+        *out_best = best;
     return TDAG_OK;
 }
 

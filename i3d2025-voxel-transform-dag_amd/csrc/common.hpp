@@ -146,18 +146,30 @@ TDAG_HD uint64_t translate4(uint64_t m, int tx, int ty, int tz)
 // 32-bit nested sub-box hash for levels >= 3 (this library's own function). Same "started"
 // rule as the reference's passes (find_translation.cpp:213-217): the state stays 0 while only
 // zeros were seen and is never 0 afterwards, so a hash is 0 exactly for an empty sub-box.
+// Per step: rotate, xor, odd multiply (a bijection of the rotated state for a fixed input);
+// the final value of the last pass goes through an avalanche (finalHash) before it is used as
+// a table or filter key.
 // ---------------------------------------------------------------------------------------
+TDAG_HD uint32_t rotl32(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
+
 TDAG_HD uint32_t hashStep(uint32_t s, uint32_t v)
 {
-    if ((s | v) == 0u)
-        return 0u;
-    uint32_t x = s ^ (v * 0x9E3779B1u + 0x7F4A7C15u);
-    x ^= x >> 16;
-    x *= 0x85EBCA6Bu;
-    x ^= x >> 13;
-    x *= 0xC2B2AE35u;
-    x ^= x >> 16;
-    return x ? x : 1u;
+    // Branch-free: x is forced non-zero, then masked to 0 while nothing has been seen yet.
+    uint32_t x = (rotl32(s, 7) ^ v) * 0x9E3779B1u;
+    x += (x == 0u) ? 1u : 0u;
+    const uint32_t started = 0u - (uint32_t)((s | v) != 0u);
+    return x & started;
+}
+
+// murmur3 fmix32: a bijection with fmix(0) == 0, so empty stays 0 and non-empty stays non-zero.
+TDAG_HD uint32_t finalHash(uint32_t h)
+{
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    return h;
 }
 
 TDAG_HD uint32_t homeSlot64(uint64_t key, uint32_t logCap)
@@ -168,5 +180,12 @@ TDAG_HD uint32_t homeSlot32(uint32_t h, uint32_t logCap)
 {
     return (uint32_t)(((uint64_t)h * 0x9E3779B97F4A7C15ull) >> (64 - logCap));
 }
+
+// Blocked Bloom filter over the child table keys: one 32-bit word per key, two bits set in it.
+// `f` is a well-mixed 32-bit value of the key; the word index uses its top bits and the two bit
+// positions its low ten bits (disjoint for filters up to 2^22 words).
+TDAG_HD uint32_t bloomWord(uint32_t f, uint32_t logWords) { return f >> (32 - logWords); }
+TDAG_HD uint32_t bloomBits(uint32_t f) { return (1u << (f & 31u)) | (1u << ((f >> 5) & 31u)); }
+TDAG_HD uint32_t bloomMix64(uint64_t key) { return (uint32_t)((key * 0xD6E8FEB86659FD93ull) >> 32); }
 
 } // namespace tdag

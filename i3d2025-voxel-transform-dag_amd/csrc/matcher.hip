@@ -89,7 +89,8 @@ __global__ __launch_bounds__(256) void kGatherInit(const uint64_t* __restrict__ 
 // ---------------------------------------------------------------------------------------
 
 // Corner k (bit a set = Positive along axis a) sees the grid mirrored on its Negative axes.
-__global__ __launch_bounds__(256) void kBuildTable2(const uint64_t* __restrict__ sortedMasks, uint32_t n, uint64_t* __restrict__ tabKeys, uint32_t* __restrict__ tabVals, uint32_t logCap)
+__global__ __launch_bounds__(256) void kBuildTable2(const uint64_t* __restrict__ sortedMasks, uint32_t n, uint64_t* __restrict__ tabKeys, uint32_t* __restrict__ tabVals, uint32_t logCap,
+    uint32_t* __restrict__ bloom, uint32_t logBloom)
 {
     const uint32_t gid = blockIdx.x * 256u + threadIdx.x;
     if (gid >= n * 8u)
@@ -100,6 +101,8 @@ __global__ __launch_bounds__(256) void kBuildTable2(const uint64_t* __restrict__
     if (m == 0)
         return;
     const uint64_t key = mirror4(m, (~corner) & 7u);
+    const uint32_t f = bloomMix64(key);
+    atomicOr(&bloom[bloomWord(f, logBloom)], bloomBits(f));
     const uint32_t capMask = (1u << logCap) - 1u;
     uint32_t s = homeSlot64(key, logCap);
     for (;;) {
@@ -115,7 +118,7 @@ __global__ __launch_bounds__(256) void kBuildTable2(const uint64_t* __restrict__
 // One wave per parent rank; lane = anchored voxel v' = (x, y, z) of the sub-box. For each
 // permutation the 8 direction keys are formed first and their table loads issued together.
 __global__ __launch_bounds__(256) void kSearch2(const uint64_t* __restrict__ sortedMasks, uint32_t n, const uint64_t* __restrict__ tabKeys, const uint32_t* __restrict__ tabVals,
-    uint32_t logCap, uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t rankBase)
+    uint32_t logCap, const uint32_t* __restrict__ bloom, uint32_t logBloom, uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t rankBase)
 {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t r = rankBase + blockIdx.x * 4u + (threadIdx.x >> 6);
@@ -137,20 +140,20 @@ __global__ __launch_bounds__(256) void kSearch2(const uint64_t* __restrict__ sor
         const uint32_t src = (uint32_t)(v[0] + 4 * v[1] + 16 * v[2]);
         const uint64_t Q = (uint64_t)__ballot((P >> src) & 1ull);
         uint64_t key[8];
-        uint32_t slot[8];
-        uint64_t first[8];
-#pragma unroll
-        for (uint32_t d = 0; d < 8; ++d) {
-            key[d] = (mirror4(Q, (~d) & 7u) >> lane) & clip;
-            slot[d] = homeSlot64(key[d], logCap);
-        }
+        uint32_t bw[8];
 #pragma unroll
         for (uint32_t d = 0; d < 8; ++d)
-            first[d] = key[d] ? tabKeys[slot[d]] : 0ull;
+            key[d] = (mirror4(Q, (~d) & 7u) >> lane) & clip;
+#pragma unroll
+        for (uint32_t d = 0; d < 8; ++d)
+            bw[d] = key[d] ? bloom[bloomWord(bloomMix64(key[d]), logBloom)] : 0u;
 #pragma unroll
         for (uint32_t d = 0; d < 8; ++d) {
-            uint64_t k = first[d];
-            uint32_t s = slot[d];
+            const uint32_t need = bloomBits(bloomMix64(key[d]));
+            if (key[d] == 0 || (bw[d] & need) != need)
+                continue;
+            uint32_t s = homeSlot64(key[d], logCap);
+            uint64_t k = tabKeys[s];
             while (k != 0) {
                 if (k == key[d]) {
                     const uint32_t e = tabVals[s];
@@ -189,13 +192,18 @@ struct Geo {
     static constexpr int R3 = R2 * R;
     static constexpr int W = R3 / 64;
     static constexpr int Threads = R2;
+    static constexpr int Slab = R2 + R; // padded z-slab pitch of a hash buffer (words)
+    static constexpr int Buf = R * Slab; // words per hash buffer
 };
 
+// LDS index of voxel (x, y, z) in a hash buffer: x is XOR-swizzled by y and every z-slab is
+// padded by R words, which keeps the 64 lanes of a wave on 64 distinct banks in all three pass
+// orientations (lanes own lines indexed (t mod R, t div R)).
 template <int L>
 __device__ __forceinline__ uint32_t sidx(int x, int y, int z)
 {
     constexpr int R = Geo<L>::R;
-    return (uint32_t)((x ^ (y & (R - 1))) + R * y + R * R * z);
+    return (uint32_t)((x ^ (y & (R - 1))) + R * y + Geo<L>::Slab * z);
 }
 
 template <int L>
@@ -206,17 +214,9 @@ __device__ __forceinline__ uint32_t lbit(const uint64_t* s_bits, int x, int y, i
     return (uint32_t)((s_bits[i >> 6] >> (i & 63u)) & 1ull);
 }
 
-// Line coordinates of thread t for a pass along axis A: the other two axes (ascending) get
-// (t mod R, t div R).
-__device__ __forceinline__ void lineAxes(int A, int& B, int& C)
-{
-    B = (A == 0) ? 1 : 0;
-    C = (A == 2) ? 1 : 2;
-}
-
-// Three bit layouts of one item in LDS: layout a has axis a fastest, so a line along axis a is
-// R consecutive bits starting at bit R * (u + R * w), where (u, w) are the line's coordinates
-// along the other two axes in ascending axis order (lineAxes).
+// Three bit layouts of one item: layout a has axis a fastest, so a line along axis a is R
+// consecutive bits starting at bit R * (u + R * w), where (u, w) are the line's coordinates
+// along the other two axes in ascending axis order.
 //   layout 0: x + R*y + R*R*z (the LargeSubGrid layout)
 //   layout 1: y + R*x + R*R*z
 //   layout 2: z + R*x + R*R*y
@@ -233,51 +233,57 @@ __device__ __forceinline__ uint32_t lineBits(const uint64_t* s_layouts, int A, i
         return (uint32_t)(word >> (start & 63u)) & ((1u << R) - 1u);
 }
 
-// Nested hash of the sub-box anchored at every voxel for (permCode, direction d), stored in s_h.
-// Pass k runs along axis permAxis(pc, k) in direction bit k of d ("Positive" scans from R-1
-// down to 0, so the value at v covers [v, R-1] on that axis).
-template <int L>
-__device__ __forceinline__ void hashGrid(const uint64_t* s_layouts, uint32_t* s_h, uint32_t pc, uint32_t d)
+// One pass of the nested hash along axis A in direction Pos (both compile time; "Positive"
+// scans from R-1 down to 0, so the value at v covers [v, R-1] on that axis). Every thread owns
+// one line. Rows: the input is the parent's bits (first pass), otherwise the previous pass's
+// hashes. Final: the last pass stores finalHash(state) instead of the state.
+template <int L, int A, bool Pos, bool Rows, bool Final>
+__device__ __forceinline__ void passAlong(const uint64_t* s_layouts, const uint32_t* in, uint32_t* out)
 {
     constexpr int R = Geo<L>::R;
+    constexpr int B = (A == 0) ? 1 : 0;
+    constexpr int C = (A == 2) ? 1 : 2;
     const int t = (int)threadIdx.x;
     const int u = t & (R - 1), w = t >> L;
-    {
-        const int A = (int)permAxis(pc, 0);
-        int B, C;
-        lineAxes(A, B, C);
-        const bool pos = d & 1u;
-        const uint32_t row = lineBits<L>(s_layouts, A, u, w);
-        int c[3];
-        c[B] = u;
-        c[C] = w;
-        uint32_t s = 0;
-#pragma unroll 8
-        for (int k = 0; k < R; ++k) {
-            c[A] = pos ? (R - 1 - k) : k;
-            s = hashStep(s, (row >> c[A]) & 1u);
-            s_h[sidx<L>(c[0], c[1], c[2])] = s;
-        }
-    }
-    __syncthreads();
+    uint32_t row = 0;
+    if constexpr (Rows)
+        row = lineBits<L>(s_layouts, A, u, w);
+    uint32_t s = 0;
 #pragma unroll
-    for (int pass = 1; pass < 3; ++pass) {
-        const int A = (int)permAxis(pc, pass);
-        int B, C;
-        lineAxes(A, B, C);
-        const bool pos = (d >> pass) & 1u;
+    for (int k = 0; k < R; ++k) {
         int c[3];
+        c[A] = Pos ? (R - 1 - k) : k;
         c[B] = u;
         c[C] = w;
-        uint32_t s = 0;
-#pragma unroll 8
-        for (int k = 0; k < R; ++k) {
-            c[A] = pos ? (R - 1 - k) : k;
-            const uint32_t i = sidx<L>(c[0], c[1], c[2]);
-            s = hashStep(s, s_h[i]);
-            s_h[i] = s;
-        }
-        __syncthreads();
+        const uint32_t i = sidx<L>(c[0], c[1], c[2]);
+        const uint32_t v = Rows ? ((row >> c[A]) & 1u) : in[i];
+        s = hashStep(s, v);
+        out[i] = Final ? finalHash(s) : s;
+    }
+}
+
+template <int L, bool Rows, bool Final>
+__device__ __forceinline__ void pass(int A, uint32_t pos, const uint64_t* s_layouts, const uint32_t* in, uint32_t* out)
+{
+    switch (A * 2 + (int)(pos & 1u)) {
+    case 0:
+        passAlong<L, 0, false, Rows, Final>(s_layouts, in, out);
+        break;
+    case 1:
+        passAlong<L, 0, true, Rows, Final>(s_layouts, in, out);
+        break;
+    case 2:
+        passAlong<L, 1, false, Rows, Final>(s_layouts, in, out);
+        break;
+    case 3:
+        passAlong<L, 1, true, Rows, Final>(s_layouts, in, out);
+        break;
+    case 4:
+        passAlong<L, 2, false, Rows, Final>(s_layouts, in, out);
+        break;
+    default:
+        passAlong<L, 2, true, Rows, Final>(s_layouts, in, out);
+        break;
     }
 }
 
@@ -361,29 +367,36 @@ __global__ __launch_bounds__(256) void kLayouts(uint64_t* __restrict__ layouts, 
 }
 
 // One block per item; the hash of the whole grid seen from each of its 8 corners goes into the
-// table (identity permutation, direction = corner; the anchor is the corner itself).
+// table and the Bloom filter (identity permutation, direction = corner, value at the corner).
 template <int L>
-__global__ __launch_bounds__(Geo<L>::Threads) void kCornerTable(const uint64_t* __restrict__ layouts, uint32_t n, uint64_t* __restrict__ table, uint32_t logCap)
+__global__ __launch_bounds__(Geo<L>::Threads) void kCornerTable(const uint64_t* __restrict__ layouts, uint32_t n, uint64_t* __restrict__ table, uint32_t logCap,
+    uint32_t* __restrict__ bloom, uint32_t logBloom)
 {
     constexpr int R = Geo<L>::R;
     constexpr int W = Geo<L>::W;
     __shared__ uint64_t s_layouts[3 * W];
-    __shared__ uint32_t s_h[Geo<L>::R3];
+    __shared__ uint32_t s_h[Geo<L>::Buf];
     __shared__ uint32_t s_any;
     const uint32_t r = blockIdx.x;
     if (!loadLayouts<L>(layouts, n, r, s_layouts, &s_any))
         return;
     const uint32_t capMask = (1u << logCap) - 1u;
     for (uint32_t corner = 0; corner < 8; ++corner) {
-        hashGrid<L>(s_layouts, s_h, 0, corner);
+        pass<L, true, false>(0, corner & 1u, s_layouts, nullptr, s_h);
+        __syncthreads();
+        pass<L, false, false>(1, (corner >> 1) & 1u, s_layouts, s_h, s_h);
+        __syncthreads();
+        pass<L, false, true>(2, (corner >> 2) & 1u, s_layouts, s_h, s_h);
+        __syncthreads();
         if (threadIdx.x == 0) {
             const int ax = (corner & 1u) ? 0 : R - 1;
             const int ay = (corner & 2u) ? 0 : R - 1;
             const int az = (corner & 4u) ? 0 : R - 1;
-            const uint32_t s = s_h[sidx<L>(ax, ay, az)];
-            if (s != 0) {
-                const uint64_t packed = ((uint64_t)s << 32) | (uint64_t)((r << 3) | corner);
-                uint32_t slot = homeSlot32(s, logCap);
+            const uint32_t h = s_h[sidx<L>(ax, ay, az)];
+            if (h != 0) {
+                atomicOr(&bloom[bloomWord(h, logBloom)], bloomBits(h));
+                const uint64_t packed = ((uint64_t)h << 32) | (uint64_t)((r << 3) | corner);
+                uint32_t slot = homeSlot32(h, logCap);
                 for (;;) {
                     const unsigned long long prev = atomicCAS((unsigned long long*)&table[slot], 0ull, (unsigned long long)packed);
                     if (prev == 0ull)
@@ -396,115 +409,178 @@ __global__ __launch_bounds__(Geo<L>::Threads) void kCornerTable(const uint64_t* 
     }
 }
 
-// One block per (parent rank, group of combos). For each combo the block builds the hash grid
-// in LDS, then every thread probes R voxels, kProbeBatch table loads in flight at a time. Hits
-// that pass the key checks are verified cooperatively by the wave that found them.
+// Probes every voxel's final hash in s_h for combo (pc, d) and merges verified hits into best.
+// Up to 8 Bloom-filter words per thread are in flight at a time; only hashes that pass the filter
+// touch the table. Hits that beat best[c] are verified cooperatively by the wave that found them.
 template <int L>
-__global__ __launch_bounds__(Geo<L>::Threads) void kSearchNested(const uint64_t* __restrict__ layouts, uint32_t n, const uint64_t* __restrict__ table, uint32_t logCap,
-    uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t combosPerBlock)
+__device__ __forceinline__ void probeGrid(const uint32_t* s_h, const uint64_t* s_layouts, const uint64_t* __restrict__ sortedMasks,
+    const uint64_t* __restrict__ table, uint32_t logCap, const uint32_t* __restrict__ bloom, uint32_t logBloom,
+    uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t r, uint32_t pc, uint32_t d)
 {
     constexpr int R = Geo<L>::R;
     constexpr int W = Geo<L>::W;
     constexpr int T = Geo<L>::Threads;
     constexpr int PerThread = Geo<L>::R3 / T; // == R
     constexpr int Batch = PerThread < 8 ? PerThread : 8;
-    __shared__ uint64_t s_layouts[3 * W];
-    __shared__ uint32_t s_h[Geo<L>::R3];
-    __shared__ uint32_t s_any;
-    const uint32_t r = blockIdx.x;
-    if (!loadLayouts<L>(layouts, n, r, s_layouts, &s_any))
-        return;
-    const uint64_t* sortedMasks = layouts; // layout 0
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t capMask = (1u << logCap) - 1u;
-    const uint32_t comboBegin = blockIdx.y * combosPerBlock;
-    const uint32_t comboEnd = min(comboBegin + combosPerBlock, 48u);
-    for (uint32_t combo = comboBegin; combo < comboEnd; ++combo) {
-        const uint32_t pc = combo >> 3;
-        const uint32_t d = combo & 7u;
-        if (!(cfgBits & kCfgAxis) && pc != 0)
-            continue; // uniform across the block
-        hashGrid<L>(s_layouts, s_h, pc, d);
-        const int a0 = (int)permAxis(pc, 0), a1 = (int)permAxis(pc, 1), a2 = (int)permAxis(pc, 2);
-        for (int j0 = 0; j0 < PerThread; j0 += Batch) {
-            uint32_t hv[Batch];
-            uint32_t sl[Batch];
-            uint64_t ent[Batch];
+    const int a0 = (int)permAxis(pc, 0), a1 = (int)permAxis(pc, 1), a2 = (int)permAxis(pc, 2);
+    for (int j0 = 0; j0 < PerThread; j0 += Batch) {
+        uint32_t hv[Batch];
+        uint32_t bw[Batch];
 #pragma unroll
-            for (int b = 0; b < Batch; ++b) {
-                hv[b] = s_h[(j0 + b) * T + threadIdx.x];
-                sl[b] = homeSlot32(hv[b], logCap);
-            }
+        for (int b = 0; b < Batch; ++b)
+            hv[b] = s_h[(j0 + b) * Geo<L>::Slab + threadIdx.x]; // slab z = j0 + b, in-slab index = thread
 #pragma unroll
-            for (int b = 0; b < Batch; ++b)
-                ent[b] = hv[b] ? table[sl[b]] : 0ull;
+        for (int b = 0; b < Batch; ++b)
+            bw[b] = hv[b] ? bloom[bloomWord(hv[b], logBloom)] : 0u;
 #pragma unroll
-            for (int b = 0; b < Batch; ++b) {
-                const uint32_t h = hv[b];
-                bool active = h != 0;
-                uint32_t slot = sl[b];
-                uint64_t e = ent[b];
-                while (__ballot(active)) {
-                    bool found = false;
-                    uint32_t hitC = 0;
-                    uint64_t hitKey = 0;
-                    while (active) {
-                        if (e == 0) {
-                            active = false;
-                            break;
-                        }
-                        const uint64_t cur = e;
-                        slot = (slot + 1) & capMask;
-                        const bool same = (uint32_t)(cur >> 32) == h;
-                        if (same) {
-                            const uint32_t cc = (uint32_t)cur >> 3;
-                            const uint32_t kc = (uint32_t)cur & 7u;
-                            if (cc > r) {
-                                // Voxel of this hash (undo the LDS swizzle), in the permuted frame.
-                                const uint32_t idx = (uint32_t)((j0 + b) * T) + threadIdx.x;
-                                const int z = (int)(idx >> (2 * L));
-                                const int y = (int)((idx >> L) & (R - 1));
-                                const int x = (int)(idx & (R - 1)) ^ (y & (R - 1));
-                                const int c3[3] = { x, y, z };
-                                const int voxel[3] = { c3[a0], c3[a1], c3[a2] };
-                                const uint32_t sym = d ^ kc;
-                                int tt[3];
-#pragma unroll
-                                for (int a = 0; a < 3; ++a) {
-                                    const int anchor = ((d >> a) & 1u) ? 0 : R - 1;
-                                    tt[a] = ((sym >> a) & 1u) ? voxel[a] - anchor : anchor - voxel[a];
-                                }
-                                if (configAllows(cfgBits, pc, sym, tt[0], tt[1], tt[2])) {
-                                    const uint64_t k64 = encodeKey(r, pc, sym, tt[0], tt[1], tt[2], L);
-                                    if (k64 < loadRelaxed(&best[cc])) {
-                                        found = true;
-                                        hitC = cc;
-                                        hitKey = k64;
-                                    }
-                                }
-                            }
-                        }
-                        e = table[slot];
-                        if (found)
-                            break;
+        for (int b = 0; b < Batch; ++b) {
+            const uint32_t h = hv[b];
+            const uint32_t need = bloomBits(h);
+            bool active = h != 0 && (bw[b] & need) == need;
+            uint32_t slot = homeSlot32(h, logCap);
+            while (__ballot(active)) {
+                bool found = false;
+                uint32_t hitC = 0;
+                uint64_t hitKey = 0;
+                while (active) {
+                    const uint64_t e = table[slot];
+                    if (e == 0) {
+                        active = false;
+                        break;
                     }
-                    uint64_t pending = __ballot(found);
-                    while (pending) {
-                        const int leader = __ffsll((unsigned long long)pending) - 1;
-                        const uint32_t bc = (uint32_t)__shfl((int)hitC, leader, 64);
-                        const uint64_t bk = (uint64_t)__shfl((long long)hitKey, leader, 64);
-                        bool ok = true;
-                        for (uint32_t wi = lane; wi < (uint32_t)W; wi += 64u)
-                            ok = ok && (transformedWord<L>(s_layouts, bk, wi) == sortedMasks[(size_t)bc * W + wi]);
-                        const bool allOk = (__ballot(!ok) == 0ull);
-                        if ((int)lane == leader && allOk)
-                            atomicMin((unsigned long long*)&best[bc], (unsigned long long)bk);
-                        pending &= pending - 1ull;
+                    slot = (slot + 1) & capMask;
+                    if ((uint32_t)(e >> 32) != h)
+                        continue;
+                    const uint32_t cc = (uint32_t)e >> 3;
+                    const uint32_t kc = (uint32_t)e & 7u;
+                    if (cc <= r)
+                        continue;
+                    // Voxel of this hash (undo the LDS swizzle), in the permuted frame.
+                    const int z = j0 + b;
+                    const int y = (int)(threadIdx.x >> L);
+                    const int x = (int)(threadIdx.x & (R - 1)) ^ y;
+                    const int c3[3] = { x, y, z };
+                    const int voxel[3] = { c3[a0], c3[a1], c3[a2] };
+                    const uint32_t sym = d ^ kc;
+                    int tt[3];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        const int anchor = ((d >> a) & 1u) ? 0 : R - 1;
+                        tt[a] = ((sym >> a) & 1u) ? voxel[a] - anchor : anchor - voxel[a];
                     }
+                    if (!configAllows(cfgBits, pc, sym, tt[0], tt[1], tt[2]))
+                        continue;
+                    const uint64_t k64 = encodeKey(r, pc, sym, tt[0], tt[1], tt[2], L);
+                    if (k64 >= loadRelaxed(&best[cc]))
+                        continue;
+                    found = true;
+                    hitC = cc;
+                    hitKey = k64;
+                    break;
+                }
+                uint64_t pending = __ballot(found);
+                while (pending) {
+                    const int leader = __ffsll((unsigned long long)pending) - 1;
+                    const uint32_t bc = (uint32_t)__shfl((int)hitC, leader, 64);
+                    const uint64_t bk = (uint64_t)__shfl((long long)hitKey, leader, 64);
+                    bool ok = true;
+                    for (uint32_t wi = lane; wi < (uint32_t)W; wi += 64u)
+                        ok = ok && (transformedWord<L>(s_layouts, bk, wi) == sortedMasks[(size_t)bc * W + wi]);
+                    const bool allOk = (__ballot(!ok) == 0ull);
+                    if ((int)lane == leader && allOk)
+                        atomicMin((unsigned long long*)&best[bc], (unsigned long long)bk);
+                    pending &= pending - 1ull;
                 }
             }
         }
+    }
+}
+
+// Search kernel. One block per (parent rank, unit); R^2 threads own one line each per pass.
+//   Units == 48: a unit is one combo (perm, direction): 3 passes, one hash buffer.
+//   Units == 6:  a unit is a family — first pass along axis a0 in direction dx — covering its
+//                8 combos (2 second axes x 2 directions x 2 directions of the third pass).
+//                Buffers = 1: 24 passes, one buffer; 2: 16 passes (the first pass is redone per
+//                second-pass variant, the third pass overwrites it); 3: 13 passes.
+// More buffers trade LDS (and occupancy) for fewer passes; the autotuner picks per level.
+template <int L, int Units, int Buffers>
+__global__ __launch_bounds__(Geo<L>::Threads) void kSearch(const uint64_t* __restrict__ layouts, uint32_t n, const uint64_t* __restrict__ table, uint32_t logCap,
+    const uint32_t* __restrict__ bloom, uint32_t logBloom, uint64_t* __restrict__ best, uint32_t cfgBits)
+{
+    constexpr int W = Geo<L>::W;
+    __shared__ uint64_t s_layouts[3 * W];
+    __shared__ uint32_t s_buf[Buffers][Geo<L>::Buf];
+    __shared__ uint32_t s_any;
+    const uint32_t r = blockIdx.x;
+    const bool axisOn = (cfgBits & kCfgAxis) != 0;
+    if constexpr (Units == 48) {
+        const uint32_t combo = blockIdx.y;
+        const uint32_t pc = combo >> 3;
+        const uint32_t d = combo & 7u;
+        if (!axisOn && pc != 0)
+            return;
+        if (!loadLayouts<L>(layouts, n, r, s_layouts, &s_any))
+            return;
+        uint32_t* h = s_buf[0];
+        pass<L, true, false>((int)permAxis(pc, 0), d, s_layouts, nullptr, h);
         __syncthreads();
+        pass<L, false, false>((int)permAxis(pc, 1), d >> 1, s_layouts, h, h);
+        __syncthreads();
+        pass<L, false, true>((int)permAxis(pc, 2), d >> 2, s_layouts, h, h);
+        __syncthreads();
+        probeGrid<L>(h, s_layouts, layouts, table, logCap, bloom, logBloom, best, cfgBits, r, pc, d);
+    } else {
+        const uint32_t family = blockIdx.y;
+        const int a0 = (int)(family >> 1);
+        const uint32_t dx = family & 1u;
+        if (!axisOn && a0 != 0)
+            return; // permutation code 0 is the only one left and has p0 == 0
+        if (!loadLayouts<L>(layouts, n, r, s_layouts, &s_any))
+            return;
+        if constexpr (Buffers == 3) {
+            pass<L, true, false>(a0, dx, s_layouts, nullptr, s_buf[0]);
+            __syncthreads();
+        }
+        for (uint32_t sel = 0; sel < 2; ++sel) {
+            const uint32_t pc = 2u * (uint32_t)a0 + sel; // permutation codes with p0 == a0
+            if (!axisOn && pc != 0)
+                continue;
+            const int a1 = (int)permAxis(pc, 1), a2 = (int)permAxis(pc, 2);
+            for (uint32_t dy = 0; dy < 2; ++dy) {
+                if constexpr (Buffers == 2) {
+                    pass<L, true, false>(a0, dx, s_layouts, nullptr, s_buf[0]);
+                    __syncthreads();
+                    pass<L, false, false>(a1, dy, s_layouts, s_buf[0], s_buf[1]);
+                    __syncthreads();
+                } else if constexpr (Buffers == 3) {
+                    pass<L, false, false>(a1, dy, s_layouts, s_buf[0], s_buf[1]);
+                    __syncthreads();
+                }
+                for (uint32_t dz = 0; dz < 2; ++dz) {
+                    const uint32_t d = dx | (dy << 1) | (dz << 2);
+                    uint32_t* h;
+                    if constexpr (Buffers == 1) {
+                        h = s_buf[0];
+                        pass<L, true, false>(a0, dx, s_layouts, nullptr, h);
+                        __syncthreads();
+                        pass<L, false, false>(a1, dy, s_layouts, h, h);
+                        __syncthreads();
+                        pass<L, false, true>(a2, dz, s_layouts, h, h);
+                    } else if constexpr (Buffers == 2) {
+                        h = s_buf[0];
+                        pass<L, false, true>(a2, dz, s_layouts, s_buf[1], h);
+                    } else {
+                        h = s_buf[2];
+                        pass<L, false, true>(a2, dz, s_layouts, s_buf[1], h);
+                    }
+                    __syncthreads();
+                    probeGrid<L>(h, s_layouts, layouts, table, logCap, bloom, logBloom, best, cfgBits, r, pc, d);
+                    __syncthreads();
+                }
+            }
+        }
     }
 }
 
@@ -559,6 +635,36 @@ __global__ __launch_bounds__(256) void kFinalMapping(const uint64_t* __restrict_
             return e_;                   \
     } while (0)
 
+template <int L>
+hipError_t launchSearch(uint32_t variant, uint32_t n, const uint64_t* layouts, const uint64_t* table, uint32_t logCap, const uint32_t* bloom, uint32_t logBloom,
+    uint64_t* best, uint32_t cfgBits, hipStream_t stream)
+{
+    const dim3 block(Geo<L>::Threads);
+    switch (variant) {
+    case kVariantCombo:
+        hipLaunchKernelGGL((kSearch<L, 48, 1>), dim3(n, 48), block, 0, stream, layouts, n, table, logCap, bloom, logBloom, best, cfgBits);
+        break;
+    case kVariantFamily1:
+        hipLaunchKernelGGL((kSearch<L, 6, 1>), dim3(n, 6), block, 0, stream, layouts, n, table, logCap, bloom, logBloom, best, cfgBits);
+        break;
+    case kVariantFamily2:
+        if constexpr (L < 5)
+            hipLaunchKernelGGL((kSearch<L, 6, 2>), dim3(n, 6), block, 0, stream, layouts, n, table, logCap, bloom, logBloom, best, cfgBits);
+        else
+            return hipErrorInvalidValue;
+        break;
+    case kVariantFamily3:
+        if constexpr (L < 5)
+            hipLaunchKernelGGL((kSearch<L, 6, 3>), dim3(n, 6), block, 0, stream, layouts, n, table, logCap, bloom, logBloom, best, cfgBits);
+        else
+            return hipErrorInvalidValue;
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 uint32_t log2Ceil(uint64_t v)
 {
     uint32_t l = 0;
@@ -569,10 +675,32 @@ uint32_t log2Ceil(uint64_t v)
 
 } // namespace
 
-MatchPlan planMatch(uint32_t level, uint64_t n)
+bool variantValid(uint32_t level, uint32_t variant)
+{
+    if (level == 2)
+        return variant == kVariantCombo;
+    if (level < 2 || level > 5 || variant >= kNumVariants)
+        return false;
+    return level < 5 || variant <= kVariantFamily1;
+}
+
+uint32_t defaultVariant(uint32_t level)
+{
+    switch (level) {
+    case 3:
+        return kVariantFamily3;
+    case 4:
+        return kVariantFamily2;
+    default:
+        return kVariantCombo;
+    }
+}
+
+MatchPlan planMatch(uint32_t level, uint64_t n, uint32_t variant)
 {
     MatchPlan p {};
     p.level = level;
+    p.variant = variantValid(level, variant) ? variant : defaultVariant(level);
     p.n = n;
     p.R = 1u << level;
     p.R3 = p.R * p.R * p.R;
@@ -593,6 +721,8 @@ MatchPlan planMatch(uint32_t level, uint64_t n)
     p.offBest = take(n * 8);
     p.offTabKeys = take(cap * 8);
     p.offTabVals = take(level == 2 ? cap * 4 : 0);
+    p.logBloom = std::min<uint32_t>(22, std::max<uint32_t>(6, log2Ceil(entries / 2)));
+    p.offBloom = take((1ull << p.logBloom) * 4);
     p.offParentTp = take(n * 8);
     p.offIsUnique = take(n * 4);
     p.offNewIdx = take(n * 4);
@@ -611,6 +741,7 @@ hipError_t enqueueMatch(const MatchPlan& p, const uint64_t* d_masks, uint32_t cf
     uint64_t* sortedMasks = (uint64_t*)(base + p.offSortedMasks);
     uint64_t* best = (uint64_t*)(base + p.offBest);
     uint64_t* tabKeys = (uint64_t*)(base + p.offTabKeys);
+    uint32_t* bloom = (uint32_t*)(base + p.offBloom);
     uint32_t* tabVals = (uint32_t*)(base + p.offTabVals);
     uint64_t* parentTp = (uint64_t*)(base + p.offParentTp);
     uint32_t* isUnique = (uint32_t*)(base + p.offIsUnique);
@@ -649,23 +780,24 @@ hipError_t enqueueMatch(const MatchPlan& p, const uint64_t* d_masks, uint32_t cf
         hipLaunchKernelGGL(kGatherInit, dim3(blocks), dim3(256), 0, stream, d_masks, order, keys, n, W, p.R3, p.level, sortedMasks, best);
         TDAG_TRY(hipGetLastError());
     }
-    // 3. Child table (levels >= 3 first build the transposed bit layouts).
+    // 3. Child table and its Bloom filter (levels >= 3 first build the transposed bit layouts).
     TDAG_TRY(fillU64(tabKeys, cap, 0ull, stream));
+    TDAG_TRY(fillU32(bloom, 1ull << p.logBloom, 0u, stream));
     switch (p.level) {
     case 2:
-        hipLaunchKernelGGL(kBuildTable2, dim3((unsigned)ceilDiv((size_t)n * 8, 256)), dim3(256), 0, stream, sortedMasks, n, tabKeys, tabVals, p.logCap);
+        hipLaunchKernelGGL(kBuildTable2, dim3((unsigned)ceilDiv((size_t)n * 8, 256)), dim3(256), 0, stream, sortedMasks, n, tabKeys, tabVals, p.logCap, bloom, p.logBloom);
         break;
     case 3:
         hipLaunchKernelGGL(kLayouts<3>, dim3(n), dim3(256), 0, stream, sortedMasks, n);
-        hipLaunchKernelGGL(kCornerTable<3>, dim3(n), dim3(Geo<3>::Threads), 0, stream, sortedMasks, n, tabKeys, p.logCap);
+        hipLaunchKernelGGL(kCornerTable<3>, dim3(n), dim3(Geo<3>::Threads), 0, stream, sortedMasks, n, tabKeys, p.logCap, bloom, p.logBloom);
         break;
     case 4:
         hipLaunchKernelGGL(kLayouts<4>, dim3(n), dim3(256), 0, stream, sortedMasks, n);
-        hipLaunchKernelGGL(kCornerTable<4>, dim3(n), dim3(Geo<4>::Threads), 0, stream, sortedMasks, n, tabKeys, p.logCap);
+        hipLaunchKernelGGL(kCornerTable<4>, dim3(n), dim3(Geo<4>::Threads), 0, stream, sortedMasks, n, tabKeys, p.logCap, bloom, p.logBloom);
         break;
     case 5:
         hipLaunchKernelGGL(kLayouts<5>, dim3(n), dim3(256), 0, stream, sortedMasks, n);
-        hipLaunchKernelGGL(kCornerTable<5>, dim3(n), dim3(Geo<5>::Threads), 0, stream, sortedMasks, n, tabKeys, p.logCap);
+        hipLaunchKernelGGL(kCornerTable<5>, dim3(n), dim3(Geo<5>::Threads), 0, stream, sortedMasks, n, tabKeys, p.logCap, bloom, p.logBloom);
         break;
     default:
         return hipErrorInvalidValue;
@@ -676,16 +808,16 @@ hipError_t enqueueMatch(const MatchPlan& p, const uint64_t* d_masks, uint32_t cf
         TDAG_TRY(hipEventRecord(evSearch0, stream));
     switch (p.level) {
     case 2:
-        hipLaunchKernelGGL(kSearch2, dim3((unsigned)ceilDiv(n, 4)), dim3(256), 0, stream, sortedMasks, n, tabKeys, tabVals, p.logCap, best, cfgBits, 0u);
+        hipLaunchKernelGGL(kSearch2, dim3((unsigned)ceilDiv(n, 4)), dim3(256), 0, stream, sortedMasks, n, tabKeys, tabVals, p.logCap, bloom, p.logBloom, best, cfgBits, 0u);
         break;
     case 3:
-        hipLaunchKernelGGL(kSearchNested<3>, dim3(n, 1), dim3(Geo<3>::Threads), 0, stream, sortedMasks, n, tabKeys, p.logCap, best, cfgBits, 48u);
+        TDAG_TRY(launchSearch<3>(p.variant, n, sortedMasks, tabKeys, p.logCap, bloom, p.logBloom, best, cfgBits, stream));
         break;
     case 4:
-        hipLaunchKernelGGL(kSearchNested<4>, dim3(n, 6), dim3(Geo<4>::Threads), 0, stream, sortedMasks, n, tabKeys, p.logCap, best, cfgBits, 8u);
+        TDAG_TRY(launchSearch<4>(p.variant, n, sortedMasks, tabKeys, p.logCap, bloom, p.logBloom, best, cfgBits, stream));
         break;
     case 5:
-        hipLaunchKernelGGL(kSearchNested<5>, dim3(n, 48), dim3(Geo<5>::Threads), 0, stream, sortedMasks, n, tabKeys, p.logCap, best, cfgBits, 1u);
+        TDAG_TRY(launchSearch<5>(p.variant, n, sortedMasks, tabKeys, p.logCap, bloom, p.logBloom, best, cfgBits, stream));
         break;
     }
     TDAG_TRY(hipGetLastError());

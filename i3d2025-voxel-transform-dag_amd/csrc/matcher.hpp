@@ -13,16 +13,27 @@ constexpr uint32_t kCfgSymmetry = 1u;
 constexpr uint32_t kCfgAxis = 2u;
 constexpr uint32_t kCfgTranslation = 4u;
 
+// Search-kernel variants for levels 3..5 (the autotuner's tile shapes): one combo per block,
+// or one family of 8 combos per block with 1, 2 or 3 LDS hash buffers.
+constexpr uint32_t kVariantCombo = 0;
+constexpr uint32_t kVariantFamily1 = 1;
+constexpr uint32_t kVariantFamily2 = 2;
+constexpr uint32_t kVariantFamily3 = 3;
+constexpr uint32_t kNumVariants = 4;
+
+bool variantValid(uint32_t level, uint32_t variant);
+uint32_t defaultVariant(uint32_t level);
+
 // Device scratch layout of one matcher call.
 struct MatchPlan {
-    uint32_t level, R, R3, W, logCap;
+    uint32_t level, R, R3, W, logCap, logBloom, variant;
     uint64_t n;
     size_t offKeys, offOrder, offSortScratch, offSortedMasks, offBest;
-    size_t offTabKeys, offTabVals, offParentTp, offIsUnique, offNewIdx, offScanScratch, offCounters;
+    size_t offTabKeys, offTabVals, offBloom, offParentTp, offIsUnique, offNewIdx, offScanScratch, offCounters;
     size_t scratchBytes;
 };
 
-MatchPlan planMatch(uint32_t level, uint64_t n);
+MatchPlan planMatch(uint32_t level, uint64_t n, uint32_t variant);
 
 // Enqueues the whole matcher on `stream`. Outputs may be null. The device counters at
 // scratch + offCounters hold {num_matched (u64), num_unique (u32)} when the stream completes.

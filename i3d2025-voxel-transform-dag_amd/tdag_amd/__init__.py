@@ -94,7 +94,12 @@ def lib() -> ctypes.CDLL:
         _u64p, _u64p, _u32p, _u64p, ctypes.POINTER(TdagMatchStats),
     ]
     L.tdag_last_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(TdagMatchStats)]
-    for name in ("tdag_context_create", "tdag_context_destroy", "tdag_match_level_device", "tdag_match_level", "tdag_last_stats"):
+    L.tdag_context_set_variant.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int32]
+    L.tdag_context_get_variant.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int32)]
+    L.tdag_autotune_level.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(TdagConfig), ctypes.c_uint32,
+                                      ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int32)]
+    for name in ("tdag_context_create", "tdag_context_destroy", "tdag_match_level_device", "tdag_match_level", "tdag_last_stats",
+                 "tdag_context_set_variant", "tdag_context_get_variant", "tdag_autotune_level"):
         getattr(L, name).restype = ctypes.c_int
     _extra_bindings(L)
     _lib = L
@@ -103,7 +108,7 @@ def lib() -> ctypes.CDLL:
 
 def _extra_bindings(L):
     """Bindings of the pipeline entry points (SVDAG build, driver, .tdag writer)."""
-    if hasattr(L, "tdag_scene_generate"):
+    if hasattr(L, "tdag_scene_dense"):
         from . import pipeline  # noqa: F401  (binds its own symbols)
 
         pipeline.bind(L)
@@ -179,6 +184,29 @@ class Context:
         if want_stats:
             return int(nu.value), st
         return None, None
+
+    # -- tuning ---------------------------------------------------------------------------
+    VARIANTS = {0: "one combo per block", 1: "family, 1 LDS buffer", 2: "family, 2 LDS buffers", 3: "family, 3 LDS buffers"}
+
+    def set_variant(self, level: int, variant: int) -> None:
+        """Search-kernel variant for `level` (-1 = library default)."""
+        check(lib().tdag_context_set_variant(self._ptr, int(level), int(variant)), "tdag_context_set_variant")
+
+    def get_variant(self, level: int) -> int:
+        v = ctypes.c_int32()
+        check(lib().tdag_context_get_variant(self._ptr, int(level), ctypes.byref(v)), "tdag_context_get_variant")
+        return int(v.value)
+
+    def autotune(self, level: int, d_masks, n: int, config: TransformDAGConfig | None = None, repeats: int = 2):
+        """Times every valid search variant on device masks, keeps the fastest; returns
+        (best_variant, {variant: ms})."""
+        cfg = (config or TransformDAGConfig()).to_c()
+        ms = (ctypes.c_float * 4)()
+        best = ctypes.c_int32()
+        ptr = ctypes.c_void_p(d_masks if isinstance(d_masks, int) else d_masks.data_ptr())
+        check(lib().tdag_autotune_level(self._ptr, int(level), ptr, int(n), ctypes.byref(cfg), int(repeats), ms, ctypes.byref(best)),
+              "tdag_autotune_level")
+        return int(best.value), {v: round(float(ms[v]), 4) for v in range(4) if ms[v] >= 0}
 
     # -- host-buffer drop-in ------------------------------------------------------------
     def match_level(self, level: int, masks: np.ndarray, config: TransformDAGConfig | None = None, want_encoded=True):

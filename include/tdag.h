@@ -100,6 +100,20 @@ tdag_status tdag_match_level(tdag_context* ctx, uint32_t level, const uint64_t* 
 /* Statistics of the last matcher call on ctx. */
 tdag_status tdag_last_stats(const tdag_context* ctx, tdag_match_stats* out);
 
+/* Search-kernel variant used at `level` (3..5) by this context: 0 = one (permutation, direction)
+ * combo per block; 1, 2, 3 = one family of 8 combos per block with 1, 2 or 3 LDS hash buffers
+ * (more buffers share more hash passes, at the cost of LDS). -1 restores the library default.
+ * Level 2 has a single kernel (variant 0). No reference counterpart: a tuning knob. */
+tdag_status tdag_context_set_variant(tdag_context* ctx, uint32_t level, int32_t variant);
+tdag_status tdag_context_get_variant(const tdag_context* ctx, uint32_t level, int32_t* variant);
+
+/* Autotuner: runs the level-`level` matcher once per valid variant on the device masks (plus
+ * one untimed warm-up run each), keeps the fastest search kernel for this context, and writes
+ * the average search time per variant (out_ms[4], -1 for invalid) and the winner. No
+ * reference counterpart. */
+tdag_status tdag_autotune_level(tdag_context* ctx, uint32_t level, const uint64_t* d_masks, uint64_t n, const tdag_config* cfg, uint32_t repeats,
+    float* out_ms, int32_t* out_best);
+
 /* ------------------------------------------------------------------------------------------
  * Pipeline around the matcher: scene synthesis, SVDAG producer, flattening, the
  * constructTransformDAG driver and the .tdag writer.

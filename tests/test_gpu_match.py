@@ -66,3 +66,17 @@ def test_larger_sets(oracle, gpu_ctx, level, n):
     masks = sets.planted_set(rng, level, n // 2, n // 2, oracle.transform, edge_cases=False)
     print(f"level={level} n={len(masks)}", flush=True)
     _check(oracle, gpu_ctx, level, masks)
+
+
+@pytest.mark.parametrize("level,n_base,variant", [(3, 60, 0), (3, 60, 1), (3, 60, 2), (3, 60, 3), (4, 20, 0), (4, 20, 1), (4, 20, 2), (4, 20, 3),
+                                                   (5, 6, 0), (5, 6, 1)])
+def test_search_variants(oracle, gpu_ctx, level, n_base, variant):
+    """Every search-kernel variant (tile shape) the autotuner may pick is bit-exact."""
+    rng = np.random.default_rng(500 + 10 * level + variant)
+    masks = sets.planted_set(rng, level, n_base, n_base, oracle.transform)
+    print(f"level={level} variant={variant} n={len(masks)}", flush=True)
+    gpu_ctx.set_variant(level, variant)
+    try:
+        _check(oracle, gpu_ctx, level, masks)
+    finally:
+        gpu_ctx.set_variant(level, -1)
