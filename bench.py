@@ -109,6 +109,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--levels", default="", help="comma-separated subset of levels to time (default: all)")
     ap.add_argument("--no-autotune", action="store_true", help="use the library's default search variants instead of tuning them first")
+    ap.add_argument("--counters", action="store_true", help="also run each level once with per-stage probe counters (diagnostics, untimed)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB three times (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -165,10 +166,20 @@ def main():
     tuning = {}
     if not args.no_autotune:
         for level in order:
-            if level >= 3 and dev[level]["n"] > 0:
-                best, ms = ctx.autotune(level, dev[level]["masks"], dev[level]["n"], cfg, repeats=1)
-                tuning[level] = {"variant": best, "search_ms_by_variant": ms}
+            if dev[level]["n"] > 0:
+                best, first, ms, tuned = ctx.autotune(level, dev[level]["masks"], dev[level]["n"], cfg, repeats=1)
+                tuning[level] = {"variant": best, "first_batch": first, "filter_bits": ctx.get_filter_bits(level), "search_ms_by_variant": ms,
+                                 "tuned_search_ms": tuned}
         print(f"[bench] autotuned search variants: {tuning}", file=sys.stderr, flush=True)
+    counters = {}
+    if args.counters:
+        ctx.set_counters(True)
+        for level in order:
+            d = dev[level]
+            _, st = ctx.match_level_device(level, d["masks"], d["n"], cfg, d_mapping=d["mapping"], d_unique=d["unique"])
+            counters[level] = st.counters()
+        ctx.set_counters(False)
+        print(f"[bench] probe counters: {counters}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         step(None)
     rec = {level: {"search_ms": [], "total_ms": []} for level in order}
@@ -229,6 +240,7 @@ def main():
             "levels": per_level,
             "search_variants": {l: ctx.get_variant(l) for l in order if l >= 3},
             "autotune": tuning,
+            **({"probe_counters": counters} if counters else {}),
             "parallelism": "single GPU",
         },
         "roofline": {

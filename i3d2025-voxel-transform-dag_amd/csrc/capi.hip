@@ -21,6 +21,10 @@ struct tdag_context {
     hipEvent_t ev[4] = { nullptr, nullptr, nullptr, nullptr };
     tdag_match_stats last {};
     int32_t variant[6] = { -1, -1, -1, -1, -1, -1 }; // per level; -1 = library default
+    bool counters = false; // collect per-stage probe counters
+    uint32_t firstBatch[6] = { 0, 0, 0, 0, 0, 0 }; // rank-batched search schedule per level (0 = default)
+    uint32_t batchGrowth[6] = { 0, 0, 0, 0, 0, 0 };
+    uint32_t bloomBits[6] = { 0, 0, 0, 0, 0, 0 }; // Bloom filter bits per table entry (0 = default)
 };
 
 namespace {
@@ -160,21 +164,28 @@ tdag_status tdag_match_level_device(tdag_context* ctx, uint32_t level, const uin
     CAPI_HIP(hipSetDevice(ctx->device), "hipSetDevice");
     hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
     const int32_t v = ctx->variant[level];
-    const tdag::MatchPlan plan = tdag::planMatch(level, n, v >= 0 ? (uint32_t)v : tdag::defaultVariant(level));
+    const tdag::MatchPlan plan = tdag::planMatch(level, n, v >= 0 ? (uint32_t)v : tdag::defaultVariant(level), ctx->firstBatch[level], ctx->batchGrowth[level], ctx->bloomBits[level]);
     if (tdag_status s = ensureBuffer(ctx->scratch, ctx->scratchBytes, plan.scratchBytes, "matcher scratch allocation"); s != TDAG_OK)
         return s;
     CAPI_HIP(hipEventRecord(ctx->ev[0], st), "hipEventRecord");
-    CAPI_HIP(tdag::enqueueMatch(plan, d_masks, cfgBitsOf(cfg), ctx->scratch, st, d_encoded, d_mapping, d_unique, ctx->ev[1], ctx->ev[2]), "matcher launch");
+    const uint32_t cfgBits = cfgBitsOf(cfg) | (ctx->counters ? tdag::kCfgCount : 0u);
+    CAPI_HIP(tdag::enqueueMatch(plan, d_masks, cfgBits, ctx->scratch, st, d_encoded, d_mapping, d_unique, ctx->ev[1], ctx->ev[2]), "matcher launch");
     CAPI_HIP(hipEventRecord(ctx->ev[3], st), "hipEventRecord");
     const bool needSync = out_num_unique || stats;
     if (needSync) {
-        CAPI_HIP(hipMemcpyAsync(ctx->pinned, (char*)ctx->scratch + plan.offCounters, 16, hipMemcpyDeviceToHost, st), "counter readback");
+        CAPI_HIP(hipMemcpyAsync(ctx->pinned, (char*)ctx->scratch + plan.offCounters, 8 * tdag::kNumCounters, hipMemcpyDeviceToHost, st), "counter readback");
         CAPI_HIP(hipStreamSynchronize(st), "hipStreamSynchronize");
         tdag_match_stats s {};
         s.num_items = n;
         s.num_matched = ctx->pinned[0];
         s.num_unique = n > 0 ? (uint32_t)(ctx->pinned[1] & 0xFFFFFFFFull) : 0;
         s.table_entries = 0;
+        s.probes = ctx->pinned[2];
+        s.filter_pass = ctx->pinned[3];
+        s.key_hits = ctx->pinned[4];
+        s.candidates = ctx->pinned[5];
+        s.verifications = ctx->pinned[6];
+        s.verified = ctx->pinned[7];
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[3]) == hipSuccess)
             s.ms_total = ms;
@@ -228,6 +239,14 @@ tdag_status tdag_match_level(tdag_context* ctx, uint32_t level, const uint64_t* 
     return TDAG_OK;
 }
 
+tdag_status tdag_context_set_counters(tdag_context* ctx, int enable)
+{
+    if (!ctx)
+        return fail(TDAG_EINVAL, "context is null");
+    ctx->counters = enable != 0;
+    return TDAG_OK;
+}
+
 tdag_status tdag_context_set_variant(tdag_context* ctx, uint32_t level, int32_t variant)
 {
     if (!ctx || level < 2 || level > 5)
@@ -247,7 +266,7 @@ tdag_status tdag_context_get_variant(const tdag_context* ctx, uint32_t level, in
 }
 
 tdag_status tdag_autotune_level(tdag_context* ctx, uint32_t level, const uint64_t* d_masks, uint64_t n, const tdag_config* cfg, uint32_t repeats,
-    float* out_ms, int32_t* out_best)
+    float* out_ms, int32_t* out_best, uint32_t* out_first_batch)
 {
     if (tdag_status s = checkArgs(ctx, level, n, cfg); s != TDAG_OK)
         return s;
@@ -257,6 +276,8 @@ tdag_status tdag_autotune_level(tdag_context* ctx, uint32_t level, const uint64_
     repeats = std::max<uint32_t>(1, repeats);
     int32_t best = -1;
     float bestMs = 0.f;
+    if (out_ms)
+        out_ms[tdag::kNumVariants] = -1.f;
     for (uint32_t v = 0; v < tdag::kNumVariants; ++v) {
         if (out_ms)
             out_ms[v] = -1.f;
@@ -285,6 +306,98 @@ tdag_status tdag_autotune_level(tdag_context* ctx, uint32_t level, const uint64_
     ctx->variant[level] = best;
     if (out_best)
         *out_best = best;
+    // Then the first search batch (parents searched before the first table pruning) for the winner.
+    if (n > 0) {
+        const uint32_t cands[4] = { (uint32_t)std::max<uint64_t>(64, n / 256), (uint32_t)std::max<uint64_t>(64, n / 64),
+            (uint32_t)std::max<uint64_t>(64, n / 16), (uint32_t)n };
+        float bestBatchMs = 0.f;
+        uint32_t bestFirst = 0;
+        for (uint32_t ci = 0; ci < 4; ++ci) {
+            const tdag::MatchPlan plan = tdag::planMatch(level, n, (uint32_t)best, cands[ci], 2);
+            float total = 0.f;
+            for (uint32_t rep = 0; rep < repeats; ++rep) {
+                CAPI_HIP(tdag::enqueueMatch(plan, d_masks, cfgBitsOf(cfg), ctx->scratch, ctx->stream, nullptr, nullptr, nullptr, ctx->ev[1], ctx->ev[2]), "matcher launch");
+                CAPI_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+                float ms = 0.f;
+                CAPI_HIP(hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]), "hipEventElapsedTime");
+                total += ms;
+            }
+            const float avg = total / repeats;
+            if (ci == 0 || avg < bestBatchMs) {
+                bestBatchMs = avg;
+                bestFirst = cands[ci];
+            }
+        }
+        ctx->firstBatch[level] = bestFirst;
+        ctx->batchGrowth[level] = 2;
+        // Finally the Bloom filter size (bits per table entry): small filters stay in L1.
+        const uint32_t bitsCands[4] = { 4, 8, 16, 32 };
+        uint32_t bestBits = 16;
+        float bestBitsMs = bestBatchMs;
+        for (uint32_t bi = 0; bi < 4; ++bi) {
+            if (bitsCands[bi] == 16)
+                continue;
+            const tdag::MatchPlan plan = tdag::planMatch(level, n, (uint32_t)best, bestFirst, 2, bitsCands[bi]);
+            if (tdag_status s = ensureBuffer(ctx->scratch, ctx->scratchBytes, plan.scratchBytes, "matcher scratch allocation"); s != TDAG_OK)
+                return s;
+            float total = 0.f;
+            for (uint32_t rep = 0; rep < repeats; ++rep) {
+                CAPI_HIP(tdag::enqueueMatch(plan, d_masks, cfgBitsOf(cfg), ctx->scratch, ctx->stream, nullptr, nullptr, nullptr, ctx->ev[1], ctx->ev[2]), "matcher launch");
+                CAPI_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+                float ms = 0.f;
+                CAPI_HIP(hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]), "hipEventElapsedTime");
+                total += ms;
+            }
+            const float avg = total / repeats;
+            if (avg < bestBitsMs) {
+                bestBitsMs = avg;
+                bestBits = bitsCands[bi];
+            }
+        }
+        ctx->bloomBits[level] = bestBits;
+        if (out_ms)
+            out_ms[tdag::kNumVariants] = bestBitsMs;
+        if (out_first_batch)
+            *out_first_batch = bestFirst;
+    }
+    return TDAG_OK;
+}
+
+tdag_status tdag_context_set_filter_bits(tdag_context* ctx, uint32_t level, uint32_t bits_per_entry)
+{
+    if (!ctx || level < 2 || level > 5 || bits_per_entry > 64)
+        return fail(TDAG_EINVAL, "bad context, level or bits per entry");
+    ctx->bloomBits[level] = bits_per_entry;
+    return TDAG_OK;
+}
+
+tdag_status tdag_context_get_filter_bits(const tdag_context* ctx, uint32_t level, uint32_t* bits_per_entry)
+{
+    if (!ctx || !bits_per_entry || level < 2 || level > 5)
+        return fail(TDAG_EINVAL, "bad argument");
+    *bits_per_entry = ctx->bloomBits[level] ? ctx->bloomBits[level] : 16;
+    return TDAG_OK;
+}
+
+tdag_status tdag_context_set_batching(tdag_context* ctx, uint32_t level, uint32_t first_batch, uint32_t growth)
+{
+    if (!ctx || level < 2 || level > 5)
+        return fail(TDAG_EINVAL, "bad context or level");
+    if (growth == 1)
+        return fail(TDAG_EINVAL, "batch growth must be 0 (default) or >= 2");
+    ctx->firstBatch[level] = first_batch;
+    ctx->batchGrowth[level] = growth;
+    return TDAG_OK;
+}
+
+tdag_status tdag_context_get_batching(const tdag_context* ctx, uint32_t level, uint32_t* first_batch, uint32_t* growth)
+{
+    if (!ctx || level < 2 || level > 5)
+        return fail(TDAG_EINVAL, "bad context or level");
+    if (first_batch)
+        *first_batch = ctx->firstBatch[level];
+    if (growth)
+        *growth = ctx->batchGrowth[level];
     return TDAG_OK;
 }
 

@@ -44,6 +44,34 @@ __device__ __forceinline__ bool configAllows(uint32_t cfgBits, uint32_t permCode
     return true;
 }
 
+// Child rank stored in a table entry whose child can no longer improve (tombstone; the key stays
+// so linear-probing clusters remain intact). Ranks are < 2^28, so this never names an item.
+constexpr uint32_t kDeadRank = 0x1FFFFFFFu;
+
+// Per-thread probe statistics, summed per wave into the call's counters when kCfgCount is set.
+struct ProbeCounts {
+    uint32_t probes = 0, filterPass = 0, keyHits = 0, candidates = 0, verifications = 0, verified = 0;
+};
+
+__device__ __forceinline__ uint32_t waveSum(uint32_t v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ void flushCounts(const ProbeCounts& c, unsigned long long* counters, uint32_t cfgBits)
+{
+    if (!(cfgBits & kCfgCount))
+        return;
+    const uint32_t v[6] = { waveSum(c.probes), waveSum(c.filterPass), waveSum(c.keyHits), waveSum(c.candidates), waveSum(c.verifications), waveSum(c.verified) };
+    if ((threadIdx.x & 63u) == 0)
+        for (int i = 0; i < 6; ++i)
+            if (v[i])
+                atomicAdd(&counters[2 + i], (unsigned long long)v[i]);
+}
+
 // ---------------------------------------------------------------------------------------
 // Rank order and initialisation
 // ---------------------------------------------------------------------------------------
@@ -118,7 +146,8 @@ __global__ __launch_bounds__(256) void kBuildTable2(const uint64_t* __restrict__
 // One wave per parent rank; lane = anchored voxel v' = (x, y, z) of the sub-box. For each
 // permutation the 8 direction keys are formed first and their table loads issued together.
 __global__ __launch_bounds__(256) void kSearch2(const uint64_t* __restrict__ sortedMasks, uint32_t n, const uint64_t* __restrict__ tabKeys, const uint32_t* __restrict__ tabVals,
-    uint32_t logCap, const uint32_t* __restrict__ bloom, uint32_t logBloom, uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t rankBase)
+    uint32_t logCap, const uint32_t* __restrict__ bloom, uint32_t logBloom, uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t rankBase,
+    unsigned long long* __restrict__ counters)
 {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t r = rankBase + blockIdx.x * 4u + (threadIdx.x >> 6);
@@ -127,6 +156,7 @@ __global__ __launch_bounds__(256) void kSearch2(const uint64_t* __restrict__ sor
     const uint64_t P = sortedMasks[r];
     if (P == 0)
         return;
+    ProbeCounts cnt;
     const int vx = (int)(lane & 3u), vy = (int)((lane >> 2) & 3u), vz = (int)(lane >> 4);
     const uint64_t clip = rangeMaskX4(0, 4 - vx) & rangeMaskY4(0, 4 - vy) & rangeMaskZ4(0, 4 - vz);
     const uint32_t capMask = (1u << logCap) - 1u;
@@ -150,27 +180,34 @@ __global__ __launch_bounds__(256) void kSearch2(const uint64_t* __restrict__ sor
 #pragma unroll
         for (uint32_t d = 0; d < 8; ++d) {
             const uint32_t need = bloomBits(bloomMix64(key[d]));
+            cnt.probes += key[d] != 0;
             if (key[d] == 0 || (bw[d] & need) != need)
                 continue;
+            ++cnt.filterPass;
             uint32_t s = homeSlot64(key[d], logCap);
             uint64_t k = tabKeys[s];
             while (k != 0) {
                 if (k == key[d]) {
+                    ++cnt.keyHits;
                     const uint32_t e = tabVals[s];
                     const uint32_t c = e >> 3;
                     const uint32_t kc = e & 7u;
-                    if (c > r) {
+                    if (c > r && c != kDeadRank) {
                         const uint32_t sym = d ^ kc;
                         const int tx = (kc & 1u) ? -vx : vx;
                         const int ty = (kc & 2u) ? -vy : vy;
                         const int tz = (kc & 4u) ? -vz : vz;
                         if (configAllows(cfgBits, pc, sym, tx, ty, tz)) {
+                            ++cnt.candidates;
                             const uint64_t k64 = encodeKey(r, pc, sym, tx, ty, tz, 2);
                             if (k64 < loadRelaxed(&best[c])) {
+                                ++cnt.verifications;
                                 // Bitwise verification of the derived transform.
                                 const uint64_t T = translate4(mirror4(Q, sym), tx, ty, tz);
-                                if (T == sortedMasks[c])
+                                if (T == sortedMasks[c]) {
+                                    ++cnt.verified;
                                     atomicMin((unsigned long long*)&best[c], (unsigned long long)k64);
+                                }
                             }
                         }
                     }
@@ -180,6 +217,7 @@ __global__ __launch_bounds__(256) void kSearch2(const uint64_t* __restrict__ sor
             }
         }
     }
+    flushCounts(cnt, counters, cfgBits);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -415,7 +453,7 @@ __global__ __launch_bounds__(Geo<L>::Threads) void kCornerTable(const uint64_t* 
 template <int L>
 __device__ __forceinline__ void probeGrid(const uint32_t* s_h, const uint64_t* s_layouts, const uint64_t* __restrict__ sortedMasks,
     const uint64_t* __restrict__ table, uint32_t logCap, const uint32_t* __restrict__ bloom, uint32_t logBloom,
-    uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t r, uint32_t pc, uint32_t d)
+    uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t r, uint32_t pc, uint32_t d, ProbeCounts& cnt)
 {
     constexpr int R = Geo<L>::R;
     constexpr int W = Geo<L>::W;
@@ -439,6 +477,8 @@ __device__ __forceinline__ void probeGrid(const uint32_t* s_h, const uint64_t* s
             const uint32_t h = hv[b];
             const uint32_t need = bloomBits(h);
             bool active = h != 0 && (bw[b] & need) == need;
+            cnt.probes += h != 0;
+            cnt.filterPass += active;
             uint32_t slot = homeSlot32(h, logCap);
             while (__ballot(active)) {
                 bool found = false;
@@ -453,9 +493,10 @@ __device__ __forceinline__ void probeGrid(const uint32_t* s_h, const uint64_t* s
                     slot = (slot + 1) & capMask;
                     if ((uint32_t)(e >> 32) != h)
                         continue;
+                    ++cnt.keyHits;
                     const uint32_t cc = (uint32_t)e >> 3;
                     const uint32_t kc = (uint32_t)e & 7u;
-                    if (cc <= r)
+                    if (cc <= r || cc == kDeadRank)
                         continue;
                     // Voxel of this hash (undo the LDS swizzle), in the permuted frame.
                     const int z = j0 + b;
@@ -472,9 +513,11 @@ __device__ __forceinline__ void probeGrid(const uint32_t* s_h, const uint64_t* s
                     }
                     if (!configAllows(cfgBits, pc, sym, tt[0], tt[1], tt[2]))
                         continue;
+                    ++cnt.candidates;
                     const uint64_t k64 = encodeKey(r, pc, sym, tt[0], tt[1], tt[2], L);
                     if (k64 >= loadRelaxed(&best[cc]))
                         continue;
+                    ++cnt.verifications;
                     found = true;
                     hitC = cc;
                     hitKey = k64;
@@ -489,8 +532,10 @@ __device__ __forceinline__ void probeGrid(const uint32_t* s_h, const uint64_t* s
                     for (uint32_t wi = lane; wi < (uint32_t)W; wi += 64u)
                         ok = ok && (transformedWord<L>(s_layouts, bk, wi) == sortedMasks[(size_t)bc * W + wi]);
                     const bool allOk = (__ballot(!ok) == 0ull);
-                    if ((int)lane == leader && allOk)
+                    if ((int)lane == leader && allOk) {
+                        ++cnt.verified;
                         atomicMin((unsigned long long*)&best[bc], (unsigned long long)bk);
+                    }
                     pending &= pending - 1ull;
                 }
             }
@@ -507,13 +552,15 @@ __device__ __forceinline__ void probeGrid(const uint32_t* s_h, const uint64_t* s
 // More buffers trade LDS (and occupancy) for fewer passes; the autotuner picks per level.
 template <int L, int Units, int Buffers>
 __global__ __launch_bounds__(Geo<L>::Threads) void kSearch(const uint64_t* __restrict__ layouts, uint32_t n, const uint64_t* __restrict__ table, uint32_t logCap,
-    const uint32_t* __restrict__ bloom, uint32_t logBloom, uint64_t* __restrict__ best, uint32_t cfgBits)
+    const uint32_t* __restrict__ bloom, uint32_t logBloom, uint64_t* __restrict__ best, uint32_t cfgBits, unsigned long long* __restrict__ counters,
+    uint32_t rankBase)
 {
+    ProbeCounts cnt;
     constexpr int W = Geo<L>::W;
     __shared__ uint64_t s_layouts[3 * W];
     __shared__ uint32_t s_buf[Buffers][Geo<L>::Buf];
     __shared__ uint32_t s_any;
-    const uint32_t r = blockIdx.x;
+    const uint32_t r = rankBase + blockIdx.x;
     const bool axisOn = (cfgBits & kCfgAxis) != 0;
     if constexpr (Units == 48) {
         const uint32_t combo = blockIdx.y;
@@ -530,7 +577,8 @@ __global__ __launch_bounds__(Geo<L>::Threads) void kSearch(const uint64_t* __res
         __syncthreads();
         pass<L, false, true>((int)permAxis(pc, 2), d >> 2, s_layouts, h, h);
         __syncthreads();
-        probeGrid<L>(h, s_layouts, layouts, table, logCap, bloom, logBloom, best, cfgBits, r, pc, d);
+        probeGrid<L>(h, s_layouts, layouts, table, logCap, bloom, logBloom, best, cfgBits, r, pc, d, cnt);
+        flushCounts(cnt, counters, cfgBits);
     } else {
         const uint32_t family = blockIdx.y;
         const int a0 = (int)(family >> 1);
@@ -576,11 +624,58 @@ __global__ __launch_bounds__(Geo<L>::Threads) void kSearch(const uint64_t* __res
                         pass<L, false, true>(a2, dz, s_layouts, s_buf[1], h);
                     }
                     __syncthreads();
-                    probeGrid<L>(h, s_layouts, layouts, table, logCap, bloom, logBloom, best, cfgBits, r, pc, d);
+                    probeGrid<L>(h, s_layouts, layouts, table, logCap, bloom, logBloom, best, cfgBits, r, pc, d, cnt);
                     __syncthreads();
                 }
             }
         }
+    }
+    flushCounts(cnt, counters, cfgBits);
+}
+
+// ---------------------------------------------------------------------------------------
+// Rank-batched search: after parents [0, r1) have been searched, every child whose best key is
+// below the smallest key a parent >= r1 can produce is final. Its table entries become
+// tombstones and the Bloom filter is rebuilt from the live entries, so later batches neither
+// probe nor load best[] for it. (The reference prunes its candidate ranges between search
+// batches for the same reason, find_translation.cpp:887-915.)
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void kPrunePacked(uint64_t* __restrict__ table, uint64_t cap, const uint64_t* __restrict__ best, uint64_t minKey,
+    uint32_t* __restrict__ bloom, uint32_t logBloom)
+{
+    for (uint64_t s = (uint64_t)blockIdx.x * 256 + threadIdx.x; s < cap; s += (uint64_t)gridDim.x * 256) {
+        const uint64_t e = table[s];
+        if (e == 0)
+            continue;
+        const uint32_t c = (uint32_t)e >> 3;
+        if (c == kDeadRank)
+            continue;
+        if (best[c] < minKey) {
+            table[s] = (e & 0xFFFFFFFF00000007ull) | ((uint64_t)kDeadRank << 3);
+            continue;
+        }
+        const uint32_t h = (uint32_t)(e >> 32);
+        atomicOr(&bloom[bloomWord(h, logBloom)], bloomBits(h));
+    }
+}
+
+__global__ __launch_bounds__(256) void kPruneSplit(const uint64_t* __restrict__ tabKeys, uint32_t* __restrict__ tabVals, uint64_t cap, const uint64_t* __restrict__ best,
+    uint64_t minKey, uint32_t* __restrict__ bloom, uint32_t logBloom)
+{
+    for (uint64_t s = (uint64_t)blockIdx.x * 256 + threadIdx.x; s < cap; s += (uint64_t)gridDim.x * 256) {
+        const uint64_t k = tabKeys[s];
+        if (k == 0)
+            continue;
+        const uint32_t e = tabVals[s];
+        const uint32_t c = e >> 3;
+        if (c == kDeadRank)
+            continue;
+        if (best[c] < minKey) {
+            tabVals[s] = (kDeadRank << 3) | (e & 7u);
+            continue;
+        }
+        const uint32_t f = bloomMix64(k);
+        atomicOr(&bloom[bloomWord(f, logBloom)], bloomBits(f));
     }
 }
 
@@ -636,26 +731,27 @@ __global__ __launch_bounds__(256) void kFinalMapping(const uint64_t* __restrict_
     } while (0)
 
 template <int L>
-hipError_t launchSearch(uint32_t variant, uint32_t n, const uint64_t* layouts, const uint64_t* table, uint32_t logCap, const uint32_t* bloom, uint32_t logBloom,
-    uint64_t* best, uint32_t cfgBits, hipStream_t stream)
+hipError_t launchSearch(uint32_t variant, uint32_t n, uint32_t r0, uint32_t r1, const uint64_t* layouts, const uint64_t* table, uint32_t logCap, const uint32_t* bloom,
+    uint32_t logBloom, uint64_t* best, uint32_t cfgBits, unsigned long long* counters, hipStream_t stream)
 {
     const dim3 block(Geo<L>::Threads);
+    const uint32_t parents = r1 - r0;
     switch (variant) {
     case kVariantCombo:
-        hipLaunchKernelGGL((kSearch<L, 48, 1>), dim3(n, 48), block, 0, stream, layouts, n, table, logCap, bloom, logBloom, best, cfgBits);
+        hipLaunchKernelGGL((kSearch<L, 48, 1>), dim3(parents, 48), block, 0, stream, layouts, n, table, logCap, bloom, logBloom, best, cfgBits, counters, r0);
         break;
     case kVariantFamily1:
-        hipLaunchKernelGGL((kSearch<L, 6, 1>), dim3(n, 6), block, 0, stream, layouts, n, table, logCap, bloom, logBloom, best, cfgBits);
+        hipLaunchKernelGGL((kSearch<L, 6, 1>), dim3(parents, 6), block, 0, stream, layouts, n, table, logCap, bloom, logBloom, best, cfgBits, counters, r0);
         break;
     case kVariantFamily2:
         if constexpr (L < 5)
-            hipLaunchKernelGGL((kSearch<L, 6, 2>), dim3(n, 6), block, 0, stream, layouts, n, table, logCap, bloom, logBloom, best, cfgBits);
+            hipLaunchKernelGGL((kSearch<L, 6, 2>), dim3(parents, 6), block, 0, stream, layouts, n, table, logCap, bloom, logBloom, best, cfgBits, counters, r0);
         else
             return hipErrorInvalidValue;
         break;
     case kVariantFamily3:
         if constexpr (L < 5)
-            hipLaunchKernelGGL((kSearch<L, 6, 3>), dim3(n, 6), block, 0, stream, layouts, n, table, logCap, bloom, logBloom, best, cfgBits);
+            hipLaunchKernelGGL((kSearch<L, 6, 3>), dim3(parents, 6), block, 0, stream, layouts, n, table, logCap, bloom, logBloom, best, cfgBits, counters, r0);
         else
             return hipErrorInvalidValue;
         break;
@@ -696,10 +792,12 @@ uint32_t defaultVariant(uint32_t level)
     }
 }
 
-MatchPlan planMatch(uint32_t level, uint64_t n, uint32_t variant)
+MatchPlan planMatch(uint32_t level, uint64_t n, uint32_t variant, uint32_t firstBatch, uint32_t batchGrowth, uint32_t bloomBitsPerEntry)
 {
     MatchPlan p {};
     p.level = level;
+    p.firstBatch = firstBatch ? firstBatch : (uint32_t)std::max<uint64_t>(256, n / 64);
+    p.batchGrowth = std::max<uint32_t>(2, batchGrowth ? batchGrowth : 2);
     p.variant = variantValid(level, variant) ? variant : defaultVariant(level);
     p.n = n;
     p.R = 1u << level;
@@ -721,7 +819,9 @@ MatchPlan planMatch(uint32_t level, uint64_t n, uint32_t variant)
     p.offBest = take(n * 8);
     p.offTabKeys = take(cap * 8);
     p.offTabVals = take(level == 2 ? cap * 4 : 0);
-    p.logBloom = std::min<uint32_t>(22, std::max<uint32_t>(6, log2Ceil(entries / 2)));
+    // Filter words = entries * bitsPerEntry / 32 (rounded up to a power of two).
+    p.bloomBitsPerEntry = bloomBitsPerEntry ? bloomBitsPerEntry : 16;
+    p.logBloom = std::min<uint32_t>(22, std::max<uint32_t>(6, log2Ceil((entries * p.bloomBitsPerEntry + 31) / 32)));
     p.offBloom = take((1ull << p.logBloom) * 4);
     p.offParentTp = take(n * 8);
     p.offIsUnique = take(n * 4);
@@ -752,7 +852,7 @@ hipError_t enqueueMatch(const MatchPlan& p, const uint64_t* d_masks, uint32_t cf
     const uint32_t W = p.W;
     const uint64_t cap = 1ull << p.logCap;
 
-    TDAG_TRY(fillU64((uint64_t*)counters, 2, 0ull, stream));
+    TDAG_TRY(fillU64((uint64_t*)counters, kNumCounters, 0ull, stream));
     if (n == 0) {
         if (evSearch0)
             TDAG_TRY(hipEventRecord(evSearch0, stream));
@@ -803,24 +903,44 @@ hipError_t enqueueMatch(const MatchPlan& p, const uint64_t* d_masks, uint32_t cf
         return hipErrorInvalidValue;
     }
     TDAG_TRY(hipGetLastError());
-    // 4. Search.
+    // 4. Search, in rank-ordered batches of growing size with table pruning in between.
     if (evSearch0)
         TDAG_TRY(hipEventRecord(evSearch0, stream));
-    switch (p.level) {
-    case 2:
-        hipLaunchKernelGGL(kSearch2, dim3((unsigned)ceilDiv(n, 4)), dim3(256), 0, stream, sortedMasks, n, tabKeys, tabVals, p.logCap, bloom, p.logBloom, best, cfgBits, 0u);
-        break;
-    case 3:
-        TDAG_TRY(launchSearch<3>(p.variant, n, sortedMasks, tabKeys, p.logCap, bloom, p.logBloom, best, cfgBits, stream));
-        break;
-    case 4:
-        TDAG_TRY(launchSearch<4>(p.variant, n, sortedMasks, tabKeys, p.logCap, bloom, p.logBloom, best, cfgBits, stream));
-        break;
-    case 5:
-        TDAG_TRY(launchSearch<5>(p.variant, n, sortedMasks, tabKeys, p.logCap, bloom, p.logBloom, best, cfgBits, stream));
-        break;
+    {
+        uint32_t r0 = 0;
+        uint32_t batch = std::max<uint32_t>(1u, std::min<uint32_t>(n, p.firstBatch));
+        while (r0 < n) {
+            const uint32_t r1 = (uint32_t)std::min<uint64_t>(n, (uint64_t)r0 + batch);
+            switch (p.level) {
+            case 2:
+                hipLaunchKernelGGL(kSearch2, dim3((unsigned)ceilDiv(r1 - r0, 4)), dim3(256), 0, stream, sortedMasks, r1, tabKeys, tabVals, p.logCap, bloom, p.logBloom, best,
+                    cfgBits, r0, counters);
+                TDAG_TRY(hipGetLastError());
+                break;
+            case 3:
+                TDAG_TRY(launchSearch<3>(p.variant, n, r0, r1, sortedMasks, tabKeys, p.logCap, bloom, p.logBloom, best, cfgBits, counters, stream));
+                break;
+            case 4:
+                TDAG_TRY(launchSearch<4>(p.variant, n, r0, r1, sortedMasks, tabKeys, p.logCap, bloom, p.logBloom, best, cfgBits, counters, stream));
+                break;
+            case 5:
+                TDAG_TRY(launchSearch<5>(p.variant, n, r0, r1, sortedMasks, tabKeys, p.logCap, bloom, p.logBloom, best, cfgBits, counters, stream));
+                break;
+            }
+            r0 = r1;
+            if (r0 < n) {
+                const uint64_t minKey = encodeKey(r0, 0, 0, 1 - (int)p.R, 1 - (int)p.R, 1 - (int)p.R, p.level);
+                const unsigned pruneBlocks = (unsigned)std::min<uint64_t>(ceilDiv(cap, 256), 8192);
+                TDAG_TRY(fillU32(bloom, 1ull << p.logBloom, 0u, stream));
+                if (p.level == 2)
+                    hipLaunchKernelGGL(kPruneSplit, dim3(pruneBlocks), dim3(256), 0, stream, tabKeys, tabVals, cap, best, minKey, bloom, p.logBloom);
+                else
+                    hipLaunchKernelGGL(kPrunePacked, dim3(pruneBlocks), dim3(256), 0, stream, tabKeys, cap, best, minKey, bloom, p.logBloom);
+                TDAG_TRY(hipGetLastError());
+                batch = (uint32_t)std::min<uint64_t>((uint64_t)batch * p.batchGrowth, n);
+            }
+        }
     }
-    TDAG_TRY(hipGetLastError());
     if (evSearch1)
         TDAG_TRY(hipEventRecord(evSearch1, stream));
     // 5. Post-processing.

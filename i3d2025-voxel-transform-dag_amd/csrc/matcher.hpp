@@ -12,6 +12,11 @@ namespace tdag {
 constexpr uint32_t kCfgSymmetry = 1u;
 constexpr uint32_t kCfgAxis = 2u;
 constexpr uint32_t kCfgTranslation = 4u;
+// Not a TransformDAGConfig flag: collect per-stage probe counters (diagnostics).
+constexpr uint32_t kCfgCount = 8u;
+// Device counters of one call: matched, unique (u32), probes, filter passes, key hits,
+// candidates, verifications, verified.
+constexpr int kNumCounters = 8;
 
 // Search-kernel variants for levels 3..5 (the autotuner's tile shapes): one combo per block,
 // or one family of 8 combos per block with 1, 2 or 3 LDS hash buffers.
@@ -27,13 +32,17 @@ uint32_t defaultVariant(uint32_t level);
 // Device scratch layout of one matcher call.
 struct MatchPlan {
     uint32_t level, R, R3, W, logCap, logBloom, variant;
+    uint32_t firstBatch, batchGrowth; // rank-batched search schedule
+    uint32_t bloomBitsPerEntry; // Bloom filter size per table entry (bits)
     uint64_t n;
     size_t offKeys, offOrder, offSortScratch, offSortedMasks, offBest;
     size_t offTabKeys, offTabVals, offBloom, offParentTp, offIsUnique, offNewIdx, offScanScratch, offCounters;
     size_t scratchBytes;
 };
 
-MatchPlan planMatch(uint32_t level, uint64_t n, uint32_t variant);
+// firstBatch = 0 / batchGrowth = 0 select the defaults (max(256, n/64) parents, doubling).
+// bloomBitsPerEntry = 0 selects 16.
+MatchPlan planMatch(uint32_t level, uint64_t n, uint32_t variant, uint32_t firstBatch = 0, uint32_t batchGrowth = 0, uint32_t bloomBitsPerEntry = 0);
 
 // Enqueues the whole matcher on `stream`. Outputs may be null. The device counters at
 // scratch + offCounters hold {num_matched (u64), num_unique (u32)} when the stream completes.

@@ -49,7 +49,16 @@ class TdagMatchStats(ctypes.Structure):
         ("table_entries", ctypes.c_uint64),
         ("ms_total", ctypes.c_float),
         ("ms_search", ctypes.c_float),
+        ("probes", ctypes.c_uint64),
+        ("filter_pass", ctypes.c_uint64),
+        ("key_hits", ctypes.c_uint64),
+        ("candidates", ctypes.c_uint64),
+        ("verifications", ctypes.c_uint64),
+        ("verified", ctypes.c_uint64),
     ]
+
+    def counters(self) -> dict:
+        return {k: int(getattr(self, k)) for k in ("probes", "filter_pass", "key_hits", "candidates", "verifications", "verified")}
 
 
 @dataclass
@@ -95,11 +104,17 @@ def lib() -> ctypes.CDLL:
     ]
     L.tdag_last_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(TdagMatchStats)]
     L.tdag_context_set_variant.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int32]
+    L.tdag_context_set_counters.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.tdag_context_get_variant.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int32)]
     L.tdag_autotune_level.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(TdagConfig), ctypes.c_uint32,
-                                      ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int32)]
+                                      ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_uint32)]
+    L.tdag_context_set_batching.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
+    L.tdag_context_set_filter_bits.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]
+    L.tdag_context_get_filter_bits.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
+    L.tdag_context_get_batching.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]
     for name in ("tdag_context_create", "tdag_context_destroy", "tdag_match_level_device", "tdag_match_level", "tdag_last_stats",
-                 "tdag_context_set_variant", "tdag_context_get_variant", "tdag_autotune_level"):
+                 "tdag_context_set_variant", "tdag_context_get_variant", "tdag_autotune_level", "tdag_context_set_counters",
+                 "tdag_context_set_batching", "tdag_context_get_batching", "tdag_context_set_filter_bits", "tdag_context_get_filter_bits"):
         getattr(L, name).restype = ctypes.c_int
     _extra_bindings(L)
     _lib = L
@@ -188,6 +203,10 @@ class Context:
     # -- tuning ---------------------------------------------------------------------------
     VARIANTS = {0: "one combo per block", 1: "family, 1 LDS buffer", 2: "family, 2 LDS buffers", 3: "family, 3 LDS buffers"}
 
+    def set_counters(self, enable: bool) -> None:
+        """Per-stage probe counters in the returned stats (diagnostics)."""
+        check(lib().tdag_context_set_counters(self._ptr, int(bool(enable))), "tdag_context_set_counters")
+
     def set_variant(self, level: int, variant: int) -> None:
         """Search-kernel variant for `level` (-1 = library default)."""
         check(lib().tdag_context_set_variant(self._ptr, int(level), int(variant)), "tdag_context_set_variant")
@@ -198,15 +217,34 @@ class Context:
         return int(v.value)
 
     def autotune(self, level: int, d_masks, n: int, config: TransformDAGConfig | None = None, repeats: int = 2):
-        """Times every valid search variant on device masks, keeps the fastest; returns
-        (best_variant, {variant: ms})."""
+        """Times every valid search variant, then four first-batch sizes, on device masks and
+        keeps the fastest; returns (best_variant, first_batch, {variant: ms}, tuned_ms)."""
         cfg = (config or TransformDAGConfig()).to_c()
-        ms = (ctypes.c_float * 4)()
+        ms = (ctypes.c_float * 5)()
         best = ctypes.c_int32()
+        first = ctypes.c_uint32()
         ptr = ctypes.c_void_p(d_masks if isinstance(d_masks, int) else d_masks.data_ptr())
-        check(lib().tdag_autotune_level(self._ptr, int(level), ptr, int(n), ctypes.byref(cfg), int(repeats), ms, ctypes.byref(best)),
-              "tdag_autotune_level")
-        return int(best.value), {v: round(float(ms[v]), 4) for v in range(4) if ms[v] >= 0}
+        check(lib().tdag_autotune_level(self._ptr, int(level), ptr, int(n), ctypes.byref(cfg), int(repeats), ms, ctypes.byref(best),
+                                        ctypes.byref(first)), "tdag_autotune_level")
+        return int(best.value), int(first.value), {v: round(float(ms[v]), 4) for v in range(4) if ms[v] >= 0}, round(float(ms[4]), 4)
+
+    def set_filter_bits(self, level: int, bits_per_entry: int) -> None:
+        """Bloom filter bits per table entry (0 = default 16)."""
+        check(lib().tdag_context_set_filter_bits(self._ptr, int(level), int(bits_per_entry)), "tdag_context_set_filter_bits")
+
+    def get_filter_bits(self, level: int) -> int:
+        v = ctypes.c_uint32()
+        check(lib().tdag_context_get_filter_bits(self._ptr, int(level), ctypes.byref(v)), "tdag_context_get_filter_bits")
+        return int(v.value)
+
+    def get_batching(self, level: int):
+        f, g = ctypes.c_uint32(), ctypes.c_uint32()
+        check(lib().tdag_context_get_batching(self._ptr, int(level), ctypes.byref(f), ctypes.byref(g)), "tdag_context_get_batching")
+        return int(f.value), int(g.value)
+
+    def set_batching(self, level: int, first_batch: int, growth: int = 2) -> None:
+        """Rank-batched search schedule (0, 0 = defaults)."""
+        check(lib().tdag_context_set_batching(self._ptr, int(level), int(first_batch), int(growth)), "tdag_context_set_batching")
 
     # -- host-buffer drop-in ------------------------------------------------------------
     def match_level(self, level: int, masks: np.ndarray, config: TransformDAGConfig | None = None, want_encoded=True):

@@ -52,6 +52,11 @@ typedef struct tdag_match_stats {
     uint64_t table_entries;
     float ms_total; /* device time of the whole call (hipEvents on the call's stream) */
     float ms_search; /* device time of the search kernel(s) alone */
+    /* Per-stage probe counters, filled only when tdag_context_set_counters(ctx, 1) is on:
+     * non-empty sub-boxes probed, Bloom-filter passes, table entries with an equal key,
+     * candidates (child ranked after the parent, transform allowed by the config),
+     * candidates that beat the child's current best and were verified, and verified matches. */
+    uint64_t probes, filter_pass, key_hits, candidates, verifications, verified;
 } tdag_match_stats;
 
 /* Copies the calling thread's last error message into buf (NUL-terminated, truncated to
@@ -100,6 +105,10 @@ tdag_status tdag_match_level(tdag_context* ctx, uint32_t level, const uint64_t* 
 /* Statistics of the last matcher call on ctx. */
 tdag_status tdag_last_stats(const tdag_context* ctx, tdag_match_stats* out);
 
+/* Enables (1) or disables (0) the per-stage probe counters of tdag_match_stats for this context
+ * (diagnostics; costs a few percent). No reference counterpart. */
+tdag_status tdag_context_set_counters(tdag_context* ctx, int enable);
+
 /* Search-kernel variant used at `level` (3..5) by this context: 0 = one (permutation, direction)
  * combo per block; 1, 2, 3 = one family of 8 combos per block with 1, 2 or 3 LDS hash buffers
  * (more buffers share more hash passes, at the cost of LDS). -1 restores the library default.
@@ -107,12 +116,28 @@ tdag_status tdag_last_stats(const tdag_context* ctx, tdag_match_stats* out);
 tdag_status tdag_context_set_variant(tdag_context* ctx, uint32_t level, int32_t variant);
 tdag_status tdag_context_get_variant(const tdag_context* ctx, uint32_t level, int32_t* variant);
 
-/* Autotuner: runs the level-`level` matcher once per valid variant on the device masks (plus
- * one untimed warm-up run each), keeps the fastest search kernel for this context, and writes
- * the average search time per variant (out_ms[4], -1 for invalid) and the winner. No
+/* Rank-batched search schedule at `level`: parents are searched in rank order, first
+ * `first_batch` of them, then batches growing by `growth`; between batches the child table drops
+ * every child that can no longer improve. 0 selects the defaults (max(256, n/64), 2). A
+ * tuning knob with no reference counterpart (the reference prunes between fixed-size batches,
+ * find_translation.cpp:771, 887-915). */
+tdag_status tdag_context_set_batching(tdag_context* ctx, uint32_t level, uint32_t first_batch, uint32_t growth);
+tdag_status tdag_context_get_batching(const tdag_context* ctx, uint32_t level, uint32_t* first_batch, uint32_t* growth);
+
+/* Size of the Bloom filter in front of the child table, in bits per table entry (0 = default 16).
+ * Small filters stay in the L1 cache at the cost of more false positives. A tuning knob with no
  * reference counterpart. */
+tdag_status tdag_context_set_filter_bits(tdag_context* ctx, uint32_t level, uint32_t bits_per_entry);
+tdag_status tdag_context_get_filter_bits(const tdag_context* ctx, uint32_t level, uint32_t* bits_per_entry);
+
+/* Autotuner: runs the level-`level` matcher once per valid variant on the device masks (plus
+ * one untimed warm-up run each) and keeps the fastest search kernel for this context, then tries
+ * four first-batch sizes (n/256, n/64, n/16, n) with it and keeps the fastest, then filter sizes
+ * of 4, 8, 16 and 32 bits per table entry. Writes the
+ * average search time per variant (out_ms[0..3], -1 for invalid; out_ms[4] = the tuned
+ * schedule's time), the winning variant and first batch. No reference counterpart. */
 tdag_status tdag_autotune_level(tdag_context* ctx, uint32_t level, const uint64_t* d_masks, uint64_t n, const tdag_config* cfg, uint32_t repeats,
-    float* out_ms, int32_t* out_best);
+    float* out_ms, int32_t* out_best, uint32_t* out_first_batch);
 
 /* ------------------------------------------------------------------------------------------
  * Pipeline around the matcher: scene synthesis, SVDAG producer, flattening, the

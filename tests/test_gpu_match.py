@@ -80,3 +80,49 @@ def test_search_variants(oracle, gpu_ctx, level, n_base, variant):
         _check(oracle, gpu_ctx, level, masks)
     finally:
         gpu_ctx.set_variant(level, -1)
+
+
+@pytest.mark.parametrize("level", [2, 3, 5])
+def test_probe_counters_consistent(oracle, gpu_ctx, level):
+    """Per-stage counters are ordered (probes >= filter passes, candidates >= verifications >=
+    verified >= foreign matches) and turning them on does not change the result."""
+    rng = np.random.default_rng(900 + level)
+    masks = sets.planted_set(rng, level, 40 if level < 5 else 6, 40 if level < 5 else 6, oracle.transform)
+    gpu_ctx.set_counters(True)
+    try:
+        st = _check(oracle, gpu_ctx, level, masks)
+    finally:
+        gpu_ctx.set_counters(False)
+    c = st.counters()
+    print(level, c, st.num_matched, flush=True)
+    assert c["probes"] >= c["filter_pass"] >= 0
+    assert c["key_hits"] >= c["candidates"] >= c["verifications"] >= c["verified"] >= st.num_matched
+    assert c["probes"] > 0
+
+
+@pytest.mark.parametrize("level,first,growth", [(2, 1, 2), (2, 7, 3), (3, 1, 2), (3, 5, 4), (4, 2, 2), (5, 1, 2)])
+def test_rank_batched_search(oracle, gpu_ctx, level, first, growth):
+    """Searching parents in rank batches with table pruning in between gives the same result
+    as one batch (pruning only drops children that can no longer improve)."""
+    rng = np.random.default_rng(1200 + level * 10 + first)
+    nb = {2: 150, 3: 50, 4: 16, 5: 5}[level]
+    masks = sets.planted_set(rng, level, nb, nb, oracle.transform)
+    print(f"level={level} first={first} growth={growth} n={len(masks)}", flush=True)
+    gpu_ctx.set_batching(level, first, growth)
+    try:
+        _check(oracle, gpu_ctx, level, masks)
+    finally:
+        gpu_ctx.set_batching(level, 0, 0)
+
+
+@pytest.mark.parametrize("level,bits", [(2, 1), (3, 4), (5, 2)])
+def test_small_filters(oracle, gpu_ctx, level, bits):
+    """A tiny (mostly saturated) Bloom filter only costs time, never changes the result."""
+    rng = np.random.default_rng(1300 + level)
+    nb = {2: 150, 3: 50, 5: 5}[level]
+    masks = sets.planted_set(rng, level, nb, nb, oracle.transform)
+    gpu_ctx.set_filter_bits(level, bits)
+    try:
+        _check(oracle, gpu_ctx, level, masks)
+    finally:
+        gpu_ctx.set_filter_bits(level, 0)
