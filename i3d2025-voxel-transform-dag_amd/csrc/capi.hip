@@ -276,11 +276,13 @@ tdag_status tdag_autotune_level(tdag_context* ctx, uint32_t level, const uint64_
     repeats = std::max<uint32_t>(1, repeats);
     int32_t best = -1;
     float bestMs = 0.f;
+    // out_ms holds kVariantSlots per-variant times followed by the tuned time.
+    constexpr uint32_t kVariantSlots = 4;
+    static_assert(tdag::kNumVariants <= kVariantSlots, "out_ms has four variant slots");
     if (out_ms)
-        out_ms[tdag::kNumVariants] = -1.f;
-    for (uint32_t v = 0; v < tdag::kNumVariants; ++v) {
-        if (out_ms)
+        for (uint32_t v = 0; v <= kVariantSlots; ++v)
             out_ms[v] = -1.f;
+    for (uint32_t v = 0; v < tdag::kNumVariants; ++v) {
         if (!tdag::variantValid(level, v))
             continue;
         const tdag::MatchPlan plan = tdag::planMatch(level, n, v);
@@ -356,7 +358,7 @@ tdag_status tdag_autotune_level(tdag_context* ctx, uint32_t level, const uint64_
         }
         ctx->bloomBits[level] = bestBits;
         if (out_ms)
-            out_ms[tdag::kNumVariants] = bestBitsMs;
+            out_ms[kVariantSlots] = bestBitsMs;
         if (out_first_batch)
             *out_first_batch = bestFirst;
     }

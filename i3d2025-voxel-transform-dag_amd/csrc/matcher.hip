@@ -271,21 +271,56 @@ __device__ __forceinline__ uint32_t lineBits(const uint64_t* s_layouts, int A, i
         return (uint32_t)(word >> (start & 63u)) & ((1u << R) - 1u);
 }
 
-// One pass of the nested hash along axis A in direction Pos (both compile time; "Positive"
-// scans from R-1 down to 0, so the value at v covers [v, R-1] on that axis). Every thread owns
-// one line. Rows: the input is the parent's bits (first pass), otherwise the previous pass's
-// hashes. Final: the last pass stores finalHash(state) instead of the state.
-template <int L, int A, bool Pos, bool Rows, bool Final>
-__device__ __forceinline__ void passAlong(const uint64_t* s_layouts, const uint32_t* in, uint32_t* out)
+// Exact value of the first pass: the trimmed bit sequence of a row segment, read from the scan
+// start toward v, is the binary number formed by those bits, so it needs no hashing. Positive
+// (scan from R-1 down to v): bits [v, R-1] with bit R-1 most significant, i.e. row >> v.
+// Negative (scan from 0 up to v): bits [0, v] with bit 0 most significant. 0 iff empty.
+template <int L, bool Pos>
+__device__ __forceinline__ uint32_t rowPrefix(uint32_t row, int v)
+{
+    constexpr int R = Geo<L>::R;
+    if constexpr (Pos)
+        return row >> v;
+    else
+        return (__brev(row) >> (32 - R)) >> (R - 1 - v);
+}
+
+// Passes 1 and 2 of the nested hash fused into one sweep: every thread owns a line along A1 and
+// reads, at each step, the bit row along A0 through that voxel from the transposed layouts.
+// Axes and directions are compile time; the result (pass-2 state) goes to `out`.
+template <int L, int A0, int A1, bool Pos0, bool Pos1>
+__device__ __forceinline__ void pass12(const uint64_t* s_layouts, uint32_t* out)
+{
+    constexpr int R = Geo<L>::R;
+    constexpr int A2 = 3 - A0 - A1;
+    constexpr int B = A0 < A2 ? A0 : A2; // the thread's line coordinates, ascending axis order
+    constexpr int C = A0 < A2 ? A2 : A0;
+    constexpr int RB = A1 < A2 ? A1 : A2; // coordinates of a row along A0, ascending axis order
+    constexpr int RC = A1 < A2 ? A2 : A1;
+    const int t = (int)threadIdx.x;
+    const int u = t & (R - 1), w = t >> L;
+    int c[3];
+    c[B] = u;
+    c[C] = w;
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        c[A1] = Pos1 ? (R - 1 - k) : k;
+        const uint32_t row = lineBits<L>(s_layouts, A0, c[RB], c[RC]);
+        s = hashStep(s, rowPrefix<L, Pos0>(row, c[A0]));
+        out[sidx<L>(c[0], c[1], c[2])] = s;
+    }
+}
+
+// Pass 3 along A2 (compile time) over the pass-2 states; stores finalHash(state) per voxel.
+template <int L, int A, bool Pos>
+__device__ __forceinline__ void pass3(const uint32_t* in, uint32_t* out)
 {
     constexpr int R = Geo<L>::R;
     constexpr int B = (A == 0) ? 1 : 0;
     constexpr int C = (A == 2) ? 1 : 2;
     const int t = (int)threadIdx.x;
     const int u = t & (R - 1), w = t >> L;
-    uint32_t row = 0;
-    if constexpr (Rows)
-        row = lineBits<L>(s_layouts, A, u, w);
     uint32_t s = 0;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
@@ -294,35 +329,90 @@ __device__ __forceinline__ void passAlong(const uint64_t* s_layouts, const uint3
         c[B] = u;
         c[C] = w;
         const uint32_t i = sidx<L>(c[0], c[1], c[2]);
-        const uint32_t v = Rows ? ((row >> c[A]) & 1u) : in[i];
-        s = hashStep(s, v);
-        out[i] = Final ? finalHash(s) : s;
+        s = hashStep(s, in[i]);
+        out[i] = finalHash(s);
     }
 }
 
-template <int L, bool Rows, bool Final>
-__device__ __forceinline__ void pass(int A, uint32_t pos, const uint64_t* s_layouts, const uint32_t* in, uint32_t* out)
+template <int L, int A0, int A1>
+__device__ __forceinline__ void pass12Dirs(uint32_t d, const uint64_t* s_layouts, uint32_t* out)
+{
+    switch (d & 3u) {
+    case 0:
+        pass12<L, A0, A1, false, false>(s_layouts, out);
+        break;
+    case 1:
+        pass12<L, A0, A1, true, false>(s_layouts, out);
+        break;
+    case 2:
+        pass12<L, A0, A1, false, true>(s_layouts, out);
+        break;
+    default:
+        pass12<L, A0, A1, true, true>(s_layouts, out);
+        break;
+    }
+}
+
+// First two passes for permutation code pc (axes p0, p1) and direction bits d (bit 0: pass 1,
+// bit 1: pass 2).
+template <int L>
+__device__ __forceinline__ void passes12(uint32_t pc, uint32_t d, const uint64_t* s_layouts, uint32_t* out)
+{
+    switch (pc) { // transform_dag.h:138-145
+    case 0:
+        pass12Dirs<L, 0, 1>(d, s_layouts, out);
+        break;
+    case 1:
+        pass12Dirs<L, 0, 2>(d, s_layouts, out);
+        break;
+    case 2:
+        pass12Dirs<L, 1, 0>(d, s_layouts, out);
+        break;
+    case 3:
+        pass12Dirs<L, 1, 2>(d, s_layouts, out);
+        break;
+    case 4:
+        pass12Dirs<L, 2, 0>(d, s_layouts, out);
+        break;
+    default:
+        pass12Dirs<L, 2, 1>(d, s_layouts, out);
+        break;
+    }
+}
+
+template <int L>
+__device__ __forceinline__ void passLast(int A, uint32_t pos, const uint32_t* in, uint32_t* out)
 {
     switch (A * 2 + (int)(pos & 1u)) {
     case 0:
-        passAlong<L, 0, false, Rows, Final>(s_layouts, in, out);
+        pass3<L, 0, false>(in, out);
         break;
     case 1:
-        passAlong<L, 0, true, Rows, Final>(s_layouts, in, out);
+        pass3<L, 0, true>(in, out);
         break;
     case 2:
-        passAlong<L, 1, false, Rows, Final>(s_layouts, in, out);
+        pass3<L, 1, false>(in, out);
         break;
     case 3:
-        passAlong<L, 1, true, Rows, Final>(s_layouts, in, out);
+        pass3<L, 1, true>(in, out);
         break;
     case 4:
-        passAlong<L, 2, false, Rows, Final>(s_layouts, in, out);
+        pass3<L, 2, false>(in, out);
         break;
     default:
-        passAlong<L, 2, true, Rows, Final>(s_layouts, in, out);
+        pass3<L, 2, true>(in, out);
         break;
     }
+}
+
+// Full nested hash of every voxel for (pc, d) into h (final values).
+template <int L>
+__device__ __forceinline__ void hashGrid(uint32_t pc, uint32_t d, const uint64_t* s_layouts, uint32_t* h)
+{
+    passes12<L>(pc, d, s_layouts, h);
+    __syncthreads();
+    passLast<L>((int)permAxis(pc, 2), d >> 2, h, h);
+    __syncthreads();
 }
 
 // Expected child word `wi` for the transform encoded in `key` applied to the parent bits.
@@ -420,12 +510,7 @@ __global__ __launch_bounds__(Geo<L>::Threads) void kCornerTable(const uint64_t* 
         return;
     const uint32_t capMask = (1u << logCap) - 1u;
     for (uint32_t corner = 0; corner < 8; ++corner) {
-        pass<L, true, false>(0, corner & 1u, s_layouts, nullptr, s_h);
-        __syncthreads();
-        pass<L, false, false>(1, (corner >> 1) & 1u, s_layouts, s_h, s_h);
-        __syncthreads();
-        pass<L, false, true>(2, (corner >> 2) & 1u, s_layouts, s_h, s_h);
-        __syncthreads();
+        hashGrid<L>(0, corner, s_layouts, s_h);
         if (threadIdx.x == 0) {
             const int ax = (corner & 1u) ? 0 : R - 1;
             const int ay = (corner & 2u) ? 0 : R - 1;
@@ -463,6 +548,9 @@ __device__ __forceinline__ void probeGrid(const uint32_t* s_h, const uint64_t* s
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t capMask = (1u << logCap) - 1u;
     const int a0 = (int)permAxis(pc, 0), a1 = (int)permAxis(pc, 1), a2 = (int)permAxis(pc, 2);
+    // Filter phase: bit j of `pending` = the thread's voxel in slab j is non-empty and passes
+    // the Bloom filter. Loads are issued in batches for memory-level parallelism.
+    uint32_t pendingZ = 0;
     for (int j0 = 0; j0 < PerThread; j0 += Batch) {
         uint32_t hv[Batch];
         uint32_t bw[Batch];
@@ -474,82 +562,96 @@ __device__ __forceinline__ void probeGrid(const uint32_t* s_h, const uint64_t* s
             bw[b] = hv[b] ? bloom[bloomWord(hv[b], logBloom)] : 0u;
 #pragma unroll
         for (int b = 0; b < Batch; ++b) {
-            const uint32_t h = hv[b];
-            const uint32_t need = bloomBits(h);
-            bool active = h != 0 && (bw[b] & need) == need;
-            cnt.probes += h != 0;
-            cnt.filterPass += active;
-            uint32_t slot = homeSlot32(h, logCap);
-            while (__ballot(active)) {
-                bool found = false;
-                uint32_t hitC = 0;
-                uint64_t hitKey = 0;
-                while (active) {
-                    const uint64_t e = table[slot];
-                    if (e == 0) {
-                        active = false;
-                        break;
-                    }
-                    slot = (slot + 1) & capMask;
-                    if ((uint32_t)(e >> 32) != h)
-                        continue;
-                    ++cnt.keyHits;
-                    const uint32_t cc = (uint32_t)e >> 3;
-                    const uint32_t kc = (uint32_t)e & 7u;
-                    if (cc <= r || cc == kDeadRank)
-                        continue;
-                    // Voxel of this hash (undo the LDS swizzle), in the permuted frame.
-                    const int z = j0 + b;
-                    const int y = (int)(threadIdx.x >> L);
-                    const int x = (int)(threadIdx.x & (R - 1)) ^ y;
-                    const int c3[3] = { x, y, z };
-                    const int voxel[3] = { c3[a0], c3[a1], c3[a2] };
-                    const uint32_t sym = d ^ kc;
-                    int tt[3];
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) {
-                        const int anchor = ((d >> a) & 1u) ? 0 : R - 1;
-                        tt[a] = ((sym >> a) & 1u) ? voxel[a] - anchor : anchor - voxel[a];
-                    }
-                    if (!configAllows(cfgBits, pc, sym, tt[0], tt[1], tt[2]))
-                        continue;
-                    ++cnt.candidates;
-                    const uint64_t k64 = encodeKey(r, pc, sym, tt[0], tt[1], tt[2], L);
-                    if (k64 >= loadRelaxed(&best[cc]))
-                        continue;
-                    ++cnt.verifications;
-                    found = true;
-                    hitC = cc;
-                    hitKey = k64;
+            const uint32_t need = bloomBits(hv[b]);
+            const bool pass = hv[b] != 0 && (bw[b] & need) == need;
+            cnt.probes += hv[b] != 0;
+            cnt.filterPass += pass;
+            pendingZ |= (uint32_t)pass << (j0 + b);
+        }
+    }
+    // Walk phase: every lane walks the table for its pending voxels one after another and
+    // stops at each candidate that beats best[]; the wave then verifies the stopped lanes'
+    // candidates together and resumes.
+    const int y = (int)(threadIdx.x >> L);
+    const int x = (int)(threadIdx.x & (R - 1)) ^ y; // undo the LDS swizzle
+    bool active = false;
+    uint32_t h = 0, slot = 0;
+    int z = 0;
+    while (true) {
+        bool found = false;
+        uint32_t hitC = 0;
+        uint64_t hitKey = 0;
+        while (true) {
+            if (!active) {
+                if (pendingZ == 0)
                     break;
-                }
-                uint64_t pending = __ballot(found);
-                while (pending) {
-                    const int leader = __ffsll((unsigned long long)pending) - 1;
-                    const uint32_t bc = (uint32_t)__shfl((int)hitC, leader, 64);
-                    const uint64_t bk = (uint64_t)__shfl((long long)hitKey, leader, 64);
-                    bool ok = true;
-                    for (uint32_t wi = lane; wi < (uint32_t)W; wi += 64u)
-                        ok = ok && (transformedWord<L>(s_layouts, bk, wi) == sortedMasks[(size_t)bc * W + wi]);
-                    const bool allOk = (__ballot(!ok) == 0ull);
-                    if ((int)lane == leader && allOk) {
-                        ++cnt.verified;
-                        atomicMin((unsigned long long*)&best[bc], (unsigned long long)bk);
-                    }
-                    pending &= pending - 1ull;
-                }
+                z = __ffs(pendingZ) - 1;
+                pendingZ &= pendingZ - 1u;
+                h = s_h[z * Geo<L>::Slab + threadIdx.x];
+                slot = homeSlot32(h, logCap);
+                active = true;
             }
+            const uint64_t e = table[slot];
+            if (e == 0) {
+                active = false;
+                continue;
+            }
+            slot = (slot + 1) & capMask;
+            if ((uint32_t)(e >> 32) != h)
+                continue;
+            ++cnt.keyHits;
+            const uint32_t cc = (uint32_t)e >> 3;
+            const uint32_t kc = (uint32_t)e & 7u;
+            if (cc <= r || cc == kDeadRank)
+                continue;
+            // Voxel of this hash in the permuted frame.
+            const int c3[3] = { x, y, z };
+            const int voxel[3] = { c3[a0], c3[a1], c3[a2] };
+            const uint32_t sym = d ^ kc;
+            int tt[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const int anchor = ((d >> a) & 1u) ? 0 : R - 1;
+                tt[a] = ((sym >> a) & 1u) ? voxel[a] - anchor : anchor - voxel[a];
+            }
+            if (!configAllows(cfgBits, pc, sym, tt[0], tt[1], tt[2]))
+                continue;
+            ++cnt.candidates;
+            const uint64_t k64 = encodeKey(r, pc, sym, tt[0], tt[1], tt[2], L);
+            if (k64 >= loadRelaxed(&best[cc]))
+                continue;
+            ++cnt.verifications;
+            found = true;
+            hitC = cc;
+            hitKey = k64;
+            break;
+        }
+        uint64_t stopped = __ballot(found);
+        if (stopped == 0)
+            break; // no lane found a candidate, so every lane ran out of work
+        while (stopped) {
+            const int leader = __ffsll((unsigned long long)stopped) - 1;
+            const uint32_t bc = (uint32_t)__shfl((int)hitC, leader, 64);
+            const uint64_t bk = (uint64_t)__shfl((long long)hitKey, leader, 64);
+            bool ok = true;
+            for (uint32_t wi = lane; wi < (uint32_t)W; wi += 64u)
+                ok = ok && (transformedWord<L>(s_layouts, bk, wi) == sortedMasks[(size_t)bc * W + wi]);
+            const bool allOk = (__ballot(!ok) == 0ull);
+            if ((int)lane == leader && allOk) {
+                ++cnt.verified;
+                atomicMin((unsigned long long*)&best[bc], (unsigned long long)bk);
+            }
+            stopped &= stopped - 1ull;
         }
     }
 }
 
-// Search kernel. One block per (parent rank, unit); R^2 threads own one line each per pass.
-//   Units == 48: a unit is one combo (perm, direction): 3 passes, one hash buffer.
-//   Units == 6:  a unit is a family — first pass along axis a0 in direction dx — covering its
-//                8 combos (2 second axes x 2 directions x 2 directions of the third pass).
-//                Buffers = 1: 24 passes, one buffer; 2: 16 passes (the first pass is redone per
-//                second-pass variant, the third pass overwrites it); 3: 13 passes.
-// More buffers trade LDS (and occupancy) for fewer passes; the autotuner picks per level.
+// Search kernel. One block per (parent rank, unit); R^2 threads own one line each per sweep.
+//   Units == 48: a unit is one combo (perm, direction): two sweeps, one hash buffer.
+//   Units == 6:  a unit is a family (first-pass axis a0 and direction dx) covering 8 combos.
+//                Buffers = 1: the fused first/second sweep is redone per combo;
+//                Buffers = 2: it is shared by the two directions of the third pass.
+// The autotuner picks per level.
 template <int L, int Units, int Buffers>
 __global__ __launch_bounds__(Geo<L>::Threads) void kSearch(const uint64_t* __restrict__ layouts, uint32_t n, const uint64_t* __restrict__ table, uint32_t logCap,
     const uint32_t* __restrict__ bloom, uint32_t logBloom, uint64_t* __restrict__ best, uint32_t cfgBits, unsigned long long* __restrict__ counters,
@@ -570,15 +672,8 @@ __global__ __launch_bounds__(Geo<L>::Threads) void kSearch(const uint64_t* __res
             return;
         if (!loadLayouts<L>(layouts, n, r, s_layouts, &s_any))
             return;
-        uint32_t* h = s_buf[0];
-        pass<L, true, false>((int)permAxis(pc, 0), d, s_layouts, nullptr, h);
-        __syncthreads();
-        pass<L, false, false>((int)permAxis(pc, 1), d >> 1, s_layouts, h, h);
-        __syncthreads();
-        pass<L, false, true>((int)permAxis(pc, 2), d >> 2, s_layouts, h, h);
-        __syncthreads();
-        probeGrid<L>(h, s_layouts, layouts, table, logCap, bloom, logBloom, best, cfgBits, r, pc, d, cnt);
-        flushCounts(cnt, counters, cfgBits);
+        hashGrid<L>(pc, d, s_layouts, s_buf[0]);
+        probeGrid<L>(s_buf[0], s_layouts, layouts, table, logCap, bloom, logBloom, best, cfgBits, r, pc, d, cnt);
     } else {
         const uint32_t family = blockIdx.y;
         const int a0 = (int)(family >> 1);
@@ -587,44 +682,25 @@ __global__ __launch_bounds__(Geo<L>::Threads) void kSearch(const uint64_t* __res
             return; // permutation code 0 is the only one left and has p0 == 0
         if (!loadLayouts<L>(layouts, n, r, s_layouts, &s_any))
             return;
-        if constexpr (Buffers == 3) {
-            pass<L, true, false>(a0, dx, s_layouts, nullptr, s_buf[0]);
-            __syncthreads();
-        }
         for (uint32_t sel = 0; sel < 2; ++sel) {
             const uint32_t pc = 2u * (uint32_t)a0 + sel; // permutation codes with p0 == a0
             if (!axisOn && pc != 0)
                 continue;
-            const int a1 = (int)permAxis(pc, 1), a2 = (int)permAxis(pc, 2);
+            const int a2 = (int)permAxis(pc, 2);
             for (uint32_t dy = 0; dy < 2; ++dy) {
                 if constexpr (Buffers == 2) {
-                    pass<L, true, false>(a0, dx, s_layouts, nullptr, s_buf[0]);
-                    __syncthreads();
-                    pass<L, false, false>(a1, dy, s_layouts, s_buf[0], s_buf[1]);
-                    __syncthreads();
-                } else if constexpr (Buffers == 3) {
-                    pass<L, false, false>(a1, dy, s_layouts, s_buf[0], s_buf[1]);
+                    passes12<L>(pc, dx | (dy << 1), s_layouts, s_buf[1]);
                     __syncthreads();
                 }
                 for (uint32_t dz = 0; dz < 2; ++dz) {
                     const uint32_t d = dx | (dy << 1) | (dz << 2);
-                    uint32_t* h;
                     if constexpr (Buffers == 1) {
-                        h = s_buf[0];
-                        pass<L, true, false>(a0, dx, s_layouts, nullptr, h);
-                        __syncthreads();
-                        pass<L, false, false>(a1, dy, s_layouts, h, h);
-                        __syncthreads();
-                        pass<L, false, true>(a2, dz, s_layouts, h, h);
-                    } else if constexpr (Buffers == 2) {
-                        h = s_buf[0];
-                        pass<L, false, true>(a2, dz, s_layouts, s_buf[1], h);
+                        hashGrid<L>(pc, d, s_layouts, s_buf[0]);
                     } else {
-                        h = s_buf[2];
-                        pass<L, false, true>(a2, dz, s_layouts, s_buf[1], h);
+                        passLast<L>(a2, dz, s_buf[1], s_buf[0]);
+                        __syncthreads();
                     }
-                    __syncthreads();
-                    probeGrid<L>(h, s_layouts, layouts, table, logCap, bloom, logBloom, best, cfgBits, r, pc, d, cnt);
+                    probeGrid<L>(s_buf[0], s_layouts, layouts, table, logCap, bloom, logBloom, best, cfgBits, r, pc, d, cnt);
                     __syncthreads();
                 }
             }
@@ -749,12 +825,6 @@ hipError_t launchSearch(uint32_t variant, uint32_t n, uint32_t r0, uint32_t r1, 
         else
             return hipErrorInvalidValue;
         break;
-    case kVariantFamily3:
-        if constexpr (L < 5)
-            hipLaunchKernelGGL((kSearch<L, 6, 3>), dim3(parents, 6), block, 0, stream, layouts, n, table, logCap, bloom, logBloom, best, cfgBits, counters, r0);
-        else
-            return hipErrorInvalidValue;
-        break;
     default:
         return hipErrorInvalidValue;
     }
@@ -783,10 +853,8 @@ bool variantValid(uint32_t level, uint32_t variant)
 uint32_t defaultVariant(uint32_t level)
 {
     switch (level) {
-    case 3:
-        return kVariantFamily3;
-    case 4:
-        return kVariantFamily2;
+    case 5:
+        return kVariantFamily1;
     default:
         return kVariantCombo;
     }

@@ -19,12 +19,11 @@ constexpr uint32_t kCfgCount = 8u;
 constexpr int kNumCounters = 8;
 
 // Search-kernel variants for levels 3..5 (the autotuner's tile shapes): one combo per block,
-// or one family of 8 combos per block with 1, 2 or 3 LDS hash buffers.
+// or one family of 8 combos per block with 1 or 2 LDS hash buffers.
 constexpr uint32_t kVariantCombo = 0;
 constexpr uint32_t kVariantFamily1 = 1;
 constexpr uint32_t kVariantFamily2 = 2;
-constexpr uint32_t kVariantFamily3 = 3;
-constexpr uint32_t kNumVariants = 4;
+constexpr uint32_t kNumVariants = 3;
 
 bool variantValid(uint32_t level, uint32_t variant);
 uint32_t defaultVariant(uint32_t level);

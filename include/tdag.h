@@ -110,8 +110,9 @@ tdag_status tdag_last_stats(const tdag_context* ctx, tdag_match_stats* out);
 tdag_status tdag_context_set_counters(tdag_context* ctx, int enable);
 
 /* Search-kernel variant used at `level` (3..5) by this context: 0 = one (permutation, direction)
- * combo per block; 1, 2, 3 = one family of 8 combos per block with 1, 2 or 3 LDS hash buffers
- * (more buffers share more hash passes, at the cost of LDS). -1 restores the library default.
+ * combo per block; 1, 2 = one family of 8 combos per block with 1 or 2 LDS hash buffers (the
+ * second buffer shares the fused first/second hash sweep between the two directions of the
+ * third, at the cost of LDS; levels 3 and 4 only). -1 restores the library default.
  * Level 2 has a single kernel (variant 0). No reference counterpart: a tuning knob. */
 tdag_status tdag_context_set_variant(tdag_context* ctx, uint32_t level, int32_t variant);
 tdag_status tdag_context_get_variant(const tdag_context* ctx, uint32_t level, int32_t* variant);
@@ -134,7 +135,7 @@ tdag_status tdag_context_get_filter_bits(const tdag_context* ctx, uint32_t level
  * one untimed warm-up run each) and keeps the fastest search kernel for this context, then tries
  * four first-batch sizes (n/256, n/64, n/16, n) with it and keeps the fastest, then filter sizes
  * of 4, 8, 16 and 32 bits per table entry. Writes the
- * average search time per variant (out_ms[0..3], -1 for invalid; out_ms[4] = the tuned
+ * average search time per variant (out_ms[0..3], -1 for invalid or unused; out_ms[4] = the tuned
  * schedule's time), the winning variant and first batch. No reference counterpart. */
 tdag_status tdag_autotune_level(tdag_context* ctx, uint32_t level, const uint64_t* d_masks, uint64_t n, const tdag_config* cfg, uint32_t repeats,
     float* out_ms, int32_t* out_best, uint32_t* out_first_batch);
