@@ -276,9 +276,9 @@ tdag_status tdag_autotune_level(tdag_context* ctx, uint32_t level, const uint64_
     repeats = std::max<uint32_t>(1, repeats);
     int32_t best = -1;
     float bestMs = 0.f;
-    // out_ms holds kVariantSlots per-variant times followed by the tuned time.
-    constexpr uint32_t kVariantSlots = 4;
-    static_assert(tdag::kNumVariants <= kVariantSlots, "out_ms has four variant slots");
+    // out_ms holds TDAG_VARIANT_SLOTS per-variant times followed by the tuned time.
+    constexpr uint32_t kVariantSlots = TDAG_VARIANT_SLOTS;
+    static_assert(tdag::kNumVariants <= kVariantSlots, "out_ms has TDAG_VARIANT_SLOTS variant slots");
     if (out_ms)
         for (uint32_t v = 0; v <= kVariantSlots; ++v)
             out_ms[v] = -1.f;

@@ -146,31 +146,24 @@ TDAG_HD uint64_t translate4(uint64_t m, int tx, int ty, int tz)
 // 32-bit nested sub-box hash for levels >= 3 (this library's own function). Same "started"
 // rule as the reference's passes (find_translation.cpp:213-217): the state stays 0 while only
 // zeros were seen and is never 0 afterwards, so a hash is 0 exactly for an empty sub-box.
-// Per step: rotate, xor, odd multiply (a bijection of the rotated state for a fixed input);
-// the final value of the last pass goes through an avalanche (finalHash) before it is used as
-// a table or filter key.
+// Per step: rotate, xor, odd multiply (a bijection of the rotated state for a fixed input),
+// then max(x, min(s | v, 1)): 0 before anything was seen, >= 1 afterwards (only the product 0
+// is merged into 1). The final value of the last pass goes through finalHash before it is used
+// as a table or filter key.
 // ---------------------------------------------------------------------------------------
 TDAG_HD uint32_t rotl32(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
 
 TDAG_HD uint32_t hashStep(uint32_t s, uint32_t v)
 {
-    // Branch-free: x is forced non-zero, then masked to 0 while nothing has been seen yet.
-    uint32_t x = (rotl32(s, 7) ^ v) * 0x9E3779B1u;
-    x += (x == 0u) ? 1u : 0u;
-    const uint32_t started = 0u - (uint32_t)((s | v) != 0u);
-    return x & started;
+    const uint32_t x = (rotl32(s, 7) ^ v) * 0x9E3779B1u;
+    const uint32_t t = s | v;
+    const uint32_t seen = t < 1u ? t : 1u;
+    return x > seen ? x : seen;
 }
 
-// murmur3 fmix32: a bijection with fmix(0) == 0, so empty stays 0 and non-empty stays non-zero.
-TDAG_HD uint32_t finalHash(uint32_t h)
-{
-    h ^= h >> 16;
-    h *= 0x85EBCA6Bu;
-    h ^= h >> 13;
-    h *= 0xC2B2AE35u;
-    h ^= h >> 16;
-    return h;
-}
+// xorshift: a bijection with finalHash(0) == 0 that folds the well-mixed high half of the last
+// product into the low bits (the Bloom filter's bit positions use the low ten bits).
+TDAG_HD uint32_t finalHash(uint32_t h) { return h ^ (h >> 16); }
 
 TDAG_HD uint32_t homeSlot64(uint64_t key, uint32_t logCap)
 {

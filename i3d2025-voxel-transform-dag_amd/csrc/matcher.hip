@@ -224,26 +224,44 @@ __global__ __launch_bounds__(256) void kSearch2(const uint64_t* __restrict__ sor
 // Levels 3..5: 32-bit nested sub-box hashes computed in LDS, one line per thread per pass.
 // ---------------------------------------------------------------------------------------
 template <int L>
+struct RowType;
+template <>
+struct RowType<3> {
+    using T = uint8_t;
+};
+template <>
+struct RowType<4> {
+    using T = uint16_t;
+};
+template <>
+struct RowType<5> {
+    using T = uint32_t;
+};
+
+// Geometry of the level-L (R = 2^L) search. A "row" is the R bits of one line of the grid along
+// one axis, stored as an R-bit unsigned integer (bit i = coordinate i).
+template <int L>
 struct Geo {
+    using Row = typename RowType<L>::T;
     static constexpr int R = 1 << L;
     static constexpr int R2 = R * R;
     static constexpr int R3 = R2 * R;
-    static constexpr int W = R3 / 64;
-    static constexpr int Threads = R2;
-    static constexpr int Slab = R2 + R; // padded z-slab pitch of a hash buffer (words)
-    static constexpr int Buf = R * Slab; // words per hash buffer
+    static constexpr int W = R3 / 64; // u64 words per bit layout
+    static constexpr int Pitch = R + 1; // padded pitch of rows in LDS and of lines in a plane
+    static constexpr int Rows = R * Pitch; // padded rows per layout in LDS
+    static constexpr int P = 64 / R; // planes per wave
+    static constexpr int Slab = R * Pitch; // words per plane in LDS
+    static constexpr int MaxWaves = R / P; // waves that cover all R planes
 };
 
-// LDS index of voxel (x, y, z) in a hash buffer: x is XOR-swizzled by y and every z-slab is
-// padded by R words, which keeps the 64 lanes of a wave on 64 distinct banks in all three pass
-// orientations (lanes own lines indexed (t mod R, t div R)).
-template <int L>
-__device__ __forceinline__ uint32_t sidx(int x, int y, int z)
-{
-    constexpr int R = Geo<L>::R;
-    return (uint32_t)((x ^ (y & (R - 1))) + R * y + Geo<L>::Slab * z);
-}
-
+// Three bit layouts of one item in global memory: layout a has axis a fastest, so the line along
+// axis a at (u, w) — its coordinates along the other two axes in ascending axis order — is row
+// j = u + R*w, bits [R*j, R*j + R).
+//   layout 0: x + R*y + R*R*z (the LargeSubGrid layout)
+//   layout 1: y + R*x + R*R*z
+//   layout 2: z + R*x + R*R*y
+// In LDS row (u, w) of layout a sits at a*Rows + u + Pitch*w; the odd pitch keeps lanes that
+// read rows at stride Pitch on distinct banks.
 template <int L>
 __device__ __forceinline__ uint32_t lbit(const uint64_t* s_bits, int x, int y, int z)
 {
@@ -252,219 +270,285 @@ __device__ __forceinline__ uint32_t lbit(const uint64_t* s_bits, int x, int y, i
     return (uint32_t)((s_bits[i >> 6] >> (i & 63u)) & 1ull);
 }
 
-// Three bit layouts of one item: layout a has axis a fastest, so a line along axis a is R
-// consecutive bits starting at bit R * (u + R * w), where (u, w) are the line's coordinates
-// along the other two axes in ascending axis order.
-//   layout 0: x + R*y + R*R*z (the LargeSubGrid layout)
-//   layout 1: y + R*x + R*R*z
-//   layout 2: z + R*x + R*R*y
-template <int L>
-__device__ __forceinline__ uint32_t lineBits(const uint64_t* s_layouts, int A, int u, int w)
+// Loads the three layouts of item r into LDS rows. Returns whether the item is non-empty (the
+// result is uniform over the block; contains a block barrier).
+template <int L, int Threads>
+__device__ __forceinline__ bool loadRows(const uint64_t* __restrict__ layouts, uint32_t n, uint32_t r, typename Geo<L>::Row* s_rows, int numLayouts)
 {
-    constexpr int R = Geo<L>::R;
-    constexpr int W = Geo<L>::W;
-    const uint32_t start = (uint32_t)(R * (u + R * w));
-    const uint64_t word = s_layouts[A * W + (start >> 6)];
-    if constexpr (R == 32)
-        return (uint32_t)(word >> (start & 63u));
-    else
-        return (uint32_t)(word >> (start & 63u)) & ((1u << R) - 1u);
-}
-
-// Exact value of the first pass: the trimmed bit sequence of a row segment, read from the scan
-// start toward v, is the binary number formed by those bits, so it needs no hashing. Positive
-// (scan from R-1 down to v): bits [v, R-1] with bit R-1 most significant, i.e. row >> v.
-// Negative (scan from 0 up to v): bits [0, v] with bit 0 most significant. 0 iff empty.
-template <int L, bool Pos>
-__device__ __forceinline__ uint32_t rowPrefix(uint32_t row, int v)
-{
-    constexpr int R = Geo<L>::R;
-    if constexpr (Pos)
-        return row >> v;
-    else
-        return (__brev(row) >> (32 - R)) >> (R - 1 - v);
-}
-
-// Passes 1 and 2 of the nested hash fused into one sweep: every thread owns a line along A1 and
-// reads, at each step, the bit row along A0 through that voxel from the transposed layouts.
-// Axes and directions are compile time; the result (pass-2 state) goes to `out`.
-template <int L, int A0, int A1, bool Pos0, bool Pos1>
-__device__ __forceinline__ void pass12(const uint64_t* s_layouts, uint32_t* out)
-{
-    constexpr int R = Geo<L>::R;
-    constexpr int A2 = 3 - A0 - A1;
-    constexpr int B = A0 < A2 ? A0 : A2; // the thread's line coordinates, ascending axis order
-    constexpr int C = A0 < A2 ? A2 : A0;
-    constexpr int RB = A1 < A2 ? A1 : A2; // coordinates of a row along A0, ascending axis order
-    constexpr int RC = A1 < A2 ? A2 : A1;
-    const int t = (int)threadIdx.x;
-    const int u = t & (R - 1), w = t >> L;
-    int c[3];
-    c[B] = u;
-    c[C] = w;
-    uint32_t s = 0;
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-        c[A1] = Pos1 ? (R - 1 - k) : k;
-        const uint32_t row = lineBits<L>(s_layouts, A0, c[RB], c[RC]);
-        s = hashStep(s, rowPrefix<L, Pos0>(row, c[A0]));
-        out[sidx<L>(c[0], c[1], c[2])] = s;
+    using G = Geo<L>;
+    using Row = typename G::Row;
+    const Row* g = reinterpret_cast<const Row*>(layouts);
+    int any = 0;
+    for (int i = (int)threadIdx.x; i < numLayouts * G::R2; i += Threads) {
+        const int lay = i >> (2 * L), j = i & (G::R2 - 1);
+        const Row v = g[((size_t)lay * n + r) * G::R2 + j];
+        s_rows[lay * G::Rows + j + (j >> L)] = v;
+        any |= (v != 0);
     }
+    return __syncthreads_or(any) != 0;
 }
 
-// Pass 3 along A2 (compile time) over the pass-2 states; stores finalHash(state) per voxel.
-template <int L, int A, bool Pos>
-__device__ __forceinline__ void pass3(const uint32_t* in, uint32_t* out)
+// Orders LDS accesses of the lanes of one wave (LDS operations of a wave execute in order; this
+// keeps the compiler from moving them across).
+__device__ __forceinline__ void waveSync()
 {
-    constexpr int R = Geo<L>::R;
-    constexpr int B = (A == 0) ? 1 : 0;
-    constexpr int C = (A == 2) ? 1 : 2;
-    const int t = (int)threadIdx.x;
-    const int u = t & (R - 1), w = t >> L;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Value of the first pass (along A0) at coordinate c0 of a row: the trimmed bit sequence of the
+// row segment read from the scan start toward c0 is the binary number formed by those bits, so
+// it needs no hashing. Positive (scan from R-1 down to c0): bits [c0, R-1] with bit R-1 most
+// significant, i.e. row >> c0. Negative (scan from 0 up to c0): bits [0, c0] with bit 0 most
+// significant, i.e. bitreverse32(row) >> (31 - c0). 0 iff the segment is empty. `sh` is c0 or
+// 31 - c0 respectively.
+template <bool Pos>
+__device__ __forceinline__ uint32_t rowPrefix(uint32_t row, int sh)
+{
+    return Pos ? (row >> sh) : (__brev(row) >> sh);
+}
+
+// Passes 1 and 2 of the nested hash on plane c0 (the coordinate along the first pass axis A0):
+// the lane owns the line along A1 at c[A2] = u and reads, at each step, the row along A0
+// through that voxel. The pass-2 state of voxel (c1, c2) goes to plane[c1 + Pitch*c2].
+template <int L, int A0, int A1, bool Pos0, bool Pos1>
+__device__ __forceinline__ void pass12(const typename Geo<L>::Row* s_rows, int c0, int u, uint32_t* plane)
+{
+    using G = Geo<L>;
+    constexpr int R = G::R;
+    constexpr int A2 = 3 - A0 - A1;
+    constexpr int Step = (A1 < A2) ? 1 : G::Pitch; // row (lo, hi) = (c[A1], c[A2]) or (c[A2], c[A1])
+    const typename G::Row* rows = s_rows + A0 * G::Rows + ((A1 < A2) ? G::Pitch * u : u);
+    uint32_t* line = plane + G::Pitch * u;
+    const int sh = Pos0 ? c0 : 31 - c0;
     uint32_t s = 0;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-        int c[3];
-        c[A] = Pos ? (R - 1 - k) : k;
-        c[B] = u;
-        c[C] = w;
-        const uint32_t i = sidx<L>(c[0], c[1], c[2]);
-        s = hashStep(s, in[i]);
-        out[i] = finalHash(s);
+        const int c1 = Pos1 ? (R - 1 - k) : k;
+        s = hashStep(s, rowPrefix<Pos0>((uint32_t)rows[Step * c1], sh));
+        line[c1] = s;
     }
 }
 
 template <int L, int A0, int A1>
-__device__ __forceinline__ void pass12Dirs(uint32_t d, const uint64_t* s_layouts, uint32_t* out)
+__device__ __forceinline__ void pass12Dirs(uint32_t d, const typename Geo<L>::Row* s_rows, int c0, int u, uint32_t* plane)
 {
     switch (d & 3u) {
     case 0:
-        pass12<L, A0, A1, false, false>(s_layouts, out);
+        pass12<L, A0, A1, false, false>(s_rows, c0, u, plane);
         break;
     case 1:
-        pass12<L, A0, A1, true, false>(s_layouts, out);
+        pass12<L, A0, A1, true, false>(s_rows, c0, u, plane);
         break;
     case 2:
-        pass12<L, A0, A1, false, true>(s_layouts, out);
+        pass12<L, A0, A1, false, true>(s_rows, c0, u, plane);
         break;
     default:
-        pass12<L, A0, A1, true, true>(s_layouts, out);
+        pass12<L, A0, A1, true, true>(s_rows, c0, u, plane);
         break;
     }
 }
 
-// First two passes for permutation code pc (axes p0, p1) and direction bits d (bit 0: pass 1,
+// Passes 1 and 2 for permutation code pc (axes p0, p1) and direction bits d (bit 0: pass 1,
 // bit 1: pass 2).
 template <int L>
-__device__ __forceinline__ void passes12(uint32_t pc, uint32_t d, const uint64_t* s_layouts, uint32_t* out)
+__device__ __forceinline__ void passes12(uint32_t pc, uint32_t d, const typename Geo<L>::Row* s_rows, int c0, int u, uint32_t* plane)
 {
     switch (pc) { // transform_dag.h:138-145
     case 0:
-        pass12Dirs<L, 0, 1>(d, s_layouts, out);
+        pass12Dirs<L, 0, 1>(d, s_rows, c0, u, plane);
         break;
     case 1:
-        pass12Dirs<L, 0, 2>(d, s_layouts, out);
+        pass12Dirs<L, 0, 2>(d, s_rows, c0, u, plane);
         break;
     case 2:
-        pass12Dirs<L, 1, 0>(d, s_layouts, out);
+        pass12Dirs<L, 1, 0>(d, s_rows, c0, u, plane);
         break;
     case 3:
-        pass12Dirs<L, 1, 2>(d, s_layouts, out);
+        pass12Dirs<L, 1, 2>(d, s_rows, c0, u, plane);
         break;
     case 4:
-        pass12Dirs<L, 2, 0>(d, s_layouts, out);
+        pass12Dirs<L, 2, 0>(d, s_rows, c0, u, plane);
         break;
     default:
-        pass12Dirs<L, 2, 1>(d, s_layouts, out);
+        pass12Dirs<L, 2, 1>(d, s_rows, c0, u, plane);
         break;
     }
 }
 
+// Out-of-line copy of passes12 for the family kernels: inlined into the combo loop, its 24
+// variants are duplicated and hoisted by the compiler until the kernel spills.
 template <int L>
-__device__ __forceinline__ void passLast(int A, uint32_t pos, const uint32_t* in, uint32_t* out)
+__device__ __noinline__ void passes12Call(uint32_t pc, uint32_t d, const typename Geo<L>::Row* s_rows, int c0, int u, uint32_t* plane)
 {
-    switch (A * 2 + (int)(pos & 1u)) {
-    case 0:
-        pass3<L, 0, false>(in, out);
-        break;
-    case 1:
-        pass3<L, 0, true>(in, out);
-        break;
-    case 2:
-        pass3<L, 1, false>(in, out);
-        break;
-    case 3:
-        pass3<L, 1, true>(in, out);
-        break;
-    case 4:
-        pass3<L, 2, false>(in, out);
-        break;
-    default:
-        pass3<L, 2, true>(in, out);
-        break;
-    }
+    passes12<L>(pc, d, s_rows, c0, u, plane);
 }
 
-// Full nested hash of every voxel for (pc, d) into h (final values).
-template <int L>
-__device__ __forceinline__ void hashGrid(uint32_t pc, uint32_t d, const uint64_t* s_layouts, uint32_t* h)
-{
-    passes12<L>(pc, d, s_layouts, h);
-    __syncthreads();
-    passLast<L>((int)permAxis(pc, 2), d >> 2, h, h);
-    __syncthreads();
-}
-
-// Expected child word `wi` for the transform encoded in `key` applied to the parent bits.
-template <int L>
-__device__ __forceinline__ uint64_t transformedWord(const uint64_t* s_bits, uint64_t key, uint32_t wi)
+// Pass 3 along the lane's line (c1 = u) of its plane over the pass-2 states st[c2], fused with
+// the Bloom filter: stores the final hash of every voxel in the plane (in place of the pass-2
+// states) and returns the bit set of voxels c2 that are non-empty and pass the filter. The
+// filter word is loaded unconditionally (every hash indexes a valid word) so the sweep has no
+// branches and its loads overlap the hashing.
+template <int L, bool Pos2>
+__device__ __forceinline__ uint32_t pass3Filter(const uint32_t (&st)[Geo<L>::R], uint32_t* line, const uint32_t* __restrict__ bloom, uint32_t logBloom, ProbeCounts& cnt)
 {
     constexpr int R = Geo<L>::R;
-    constexpr uint32_t b = L + 1;
-    constexpr uint64_t m = (1ull << b) - 1ull;
-    const int tx = (int)(key & m) - R;
-    const int ty = (int)((key >> b) & m) - R;
-    const int tz = (int)((key >> (2 * b)) & m) - R;
-    const uint32_t sym = (uint32_t)((key >> (3 * b)) & 7u);
-    const uint32_t pc = (uint32_t)((key >> (3 * b + 3)) & 7u);
-    const int a0 = (int)permAxis(pc, 0), a1 = (int)permAxis(pc, 1), a2 = (int)permAxis(pc, 2);
-    uint64_t out = 0;
-    for (int j = 0; j < 64; ++j) {
-        const uint32_t i = wi * 64u + (uint32_t)j;
-        int q[3] = { (int)(i & (R - 1)) - tx, (int)((i >> L) & (R - 1)) - ty, (int)(i >> (2 * L)) - tz }; // out[o] = M[o - t]
-        if (q[0] < 0 || q[1] < 0 || q[2] < 0 || q[0] >= R || q[1] >= R || q[2] >= R)
-            continue;
-        if (sym & 1u)
-            q[0] = R - 1 - q[0];
-        if (sym & 2u)
-            q[1] = R - 1 - q[1];
-        if (sym & 4u)
-            q[2] = R - 1 - q[2];
-        int v[3];
-        v[a0] = q[0];
-        v[a1] = q[1];
-        v[a2] = q[2];
-        out |= (uint64_t)lbit<L>(s_bits, v[0], v[1], v[2]) << j;
+    const uint32_t shift = 32u - logBloom;
+    uint32_t s = 0, pending = 0, nonEmpty = 0;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int c2 = Pos2 ? (R - 1 - k) : k;
+        s = hashStep(s, st[c2]);
+        const uint32_t f = finalHash(s);
+        line[Geo<L>::Pitch * c2] = f;
+        const uint32_t bw = bloom[f >> shift];
+        const uint32_t need = bloomBits(f);
+        const uint32_t miss = need & ~bw;
+        const uint32_t nz = f < 1u ? f : 1u;
+        const uint32_t ok = nz & ((miss < 1u ? miss : 1u) ^ 1u);
+        pending |= ok << c2;
+        nonEmpty += nz;
     }
-    return out;
+    cnt.probes += nonEmpty;
+    cnt.filterPass += (uint32_t)__builtin_popcount(pending);
+    return pending;
 }
 
+// Expected child row (yo, zo) (bits along x) for the transform encoded in `key` applied to the
+// parent rows: out[o] = M[o - t] with M = mirror(permute(P)) (find_translation.cpp:66-84 for the
+// translation). The source of an output row is one line of the parent along axis p0, read from
+// layout p0.
 template <int L>
-__device__ __forceinline__ bool loadLayouts(const uint64_t* __restrict__ layouts, uint32_t n, uint32_t r, uint64_t* s_layouts, uint32_t* s_any)
+__device__ __forceinline__ uint32_t transformedRow(const typename Geo<L>::Row* s_rows, int tx, int ty, int tz, uint32_t sym, uint32_t pc, int yo, int zo)
 {
-    constexpr int W = Geo<L>::W;
-    if (threadIdx.x == 0)
-        *s_any = 0;
-    __syncthreads();
-    for (int i = (int)threadIdx.x; i < 3 * W; i += Geo<L>::Threads) {
-        const int lay = i / W, wi = i % W;
-        const uint64_t v = layouts[((size_t)lay * n + r) * W + wi];
-        s_layouts[i] = v;
-        if (v)
-            atomicOr(s_any, 1u);
+    using G = Geo<L>;
+    constexpr int R = G::R;
+    const int q1 = yo - ty, q2 = zo - tz;
+    if (q1 < 0 || q1 >= R || q2 < 0 || q2 >= R)
+        return 0u;
+    const int m1 = (sym & 2u) ? R - 1 - q1 : q1;
+    const int m2 = (sym & 4u) ? R - 1 - q2 : q2;
+    const int a1 = (int)permAxis(pc, 1), a2 = (int)permAxis(pc, 2);
+    // Parent coordinates along a1 and a2; the row along a0 is indexed by them in ascending axis order.
+    const int lo = a1 < a2 ? m1 : m2, hi = a1 < a2 ? m2 : m1;
+    uint32_t line = (uint32_t)s_rows[(int)permAxis(pc, 0) * G::Rows + lo + G::Pitch * hi];
+    if (sym & 1u)
+        line = __brev(line) >> (32 - R);
+    const uint32_t rowMask = R == 32 ? 0xFFFFFFFFu : ((1u << R) - 1u);
+    return (tx >= 0 ? (line << tx) : (line >> -tx)) & rowMask;
+}
+
+// Wave-cooperative bitwise check child bc == T(parent) for the transform in key bk.
+template <int L>
+__device__ __forceinline__ bool verifyWave(const typename Geo<L>::Row* s_rows, const uint64_t* __restrict__ sortedMasks, uint32_t bc, uint64_t bk, uint32_t lane)
+{
+    using G = Geo<L>;
+    constexpr int R = G::R;
+    constexpr uint32_t b = L + 1;
+    constexpr uint64_t m = (1ull << b) - 1ull;
+    const int tx = (int)(bk & m) - R;
+    const int ty = (int)((bk >> b) & m) - R;
+    const int tz = (int)((bk >> (2 * b)) & m) - R;
+    const uint32_t sym = (uint32_t)((bk >> (3 * b)) & 7u);
+    const uint32_t pc = (uint32_t)((bk >> (3 * b + 3)) & 7u);
+    const typename G::Row* child = reinterpret_cast<const typename G::Row*>(sortedMasks) + (size_t)bc * G::R2;
+    bool ok = true;
+    for (int j = (int)lane; j < G::R2; j += 64)
+        ok = ok && (transformedRow<L>(s_rows, tx, ty, tz, sym, pc, j & (R - 1), j >> L) == (uint32_t)child[j]);
+    return __ballot(!ok) == 0ull;
+}
+
+// Walks the child table for the voxels in `pending` (bit c2 = voxel (c0, u, c2) of the lane's
+// line passed the filter; its hash is in the plane) and merges verified hits into best. Lanes stop
+// at each candidate that beats best[c]; the wave then verifies the stopped lanes' candidates
+// together and resumes.
+template <int L>
+__device__ __forceinline__ void probeLine(uint32_t pending, const uint32_t* line, const typename Geo<L>::Row* s_rows,
+    const uint64_t* __restrict__ sortedMasks, const uint64_t* __restrict__ table, uint32_t logCap,
+    uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t r, uint32_t pc, uint32_t d, int c0, int u, ProbeCounts& cnt)
+{
+    using G = Geo<L>;
+    constexpr int R = G::R;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t capMask = (1u << logCap) - 1u;
+    bool active = false;
+    uint32_t h = 0, slot = 0;
+    int c2 = 0;
+    while (true) {
+        bool found = false;
+        uint32_t hitC = 0;
+        uint64_t hitKey = 0;
+        while (true) {
+            if (!active) {
+                if (pending == 0)
+                    break;
+                c2 = __ffs(pending) - 1;
+                pending &= pending - 1u;
+                h = line[G::Pitch * c2];
+                slot = homeSlot32(h, logCap);
+                active = true;
+            }
+            const uint64_t e = table[slot];
+            if (e == 0) {
+                active = false;
+                continue;
+            }
+            slot = (slot + 1) & capMask;
+            if ((uint32_t)(e >> 32) != h)
+                continue;
+            ++cnt.keyHits;
+            const uint32_t cc = (uint32_t)e >> 3;
+            const uint32_t kc = (uint32_t)e & 7u;
+            if (cc <= r || cc == kDeadRank)
+                continue;
+            // Translation of the anchored match (find_translation.cpp:835-837); the voxel's
+            // coordinates along the three pass axes are (c0, u, c2).
+            const int voxel[3] = { c0, u, c2 };
+            const uint32_t sym = d ^ kc;
+            int tt[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const int anchor = ((d >> a) & 1u) ? 0 : R - 1;
+                tt[a] = ((sym >> a) & 1u) ? voxel[a] - anchor : anchor - voxel[a];
+            }
+            if (!configAllows(cfgBits, pc, sym, tt[0], tt[1], tt[2]))
+                continue;
+            ++cnt.candidates;
+            const uint64_t k64 = encodeKey(r, pc, sym, tt[0], tt[1], tt[2], L);
+            if (k64 >= loadRelaxed(&best[cc]))
+                continue;
+            ++cnt.verifications;
+            found = true;
+            hitC = cc;
+            hitKey = k64;
+            break;
+        }
+        uint64_t stopped = __ballot(found);
+        if (stopped == 0)
+            break; // no lane found a candidate, so every lane ran out of work
+        while (stopped) {
+            const int leader = __ffsll((unsigned long long)stopped) - 1;
+            const uint32_t bc = (uint32_t)__shfl((int)hitC, leader, 64);
+            const uint64_t bk = (uint64_t)__shfl((long long)hitKey, leader, 64);
+            const bool allOk = verifyWave<L>(s_rows, sortedMasks, bc, bk, lane);
+            if ((int)lane == leader && allOk) {
+                ++cnt.verified;
+                atomicMin((unsigned long long*)&best[bc], (unsigned long long)bk);
+            }
+            stopped &= stopped - 1ull;
+        }
     }
-    __syncthreads();
-    return *s_any != 0;
+}
+
+// Pass 3 in direction dz with the filter, then the table walk of the lane's line.
+template <int L>
+__device__ __forceinline__ void lastPassProbe(uint32_t dz, const uint32_t (&st)[Geo<L>::R], uint32_t* line, const typename Geo<L>::Row* s_rows,
+    const uint64_t* __restrict__ sortedMasks, const uint64_t* __restrict__ table, uint32_t logCap, const uint32_t* __restrict__ bloom, uint32_t logBloom,
+    uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t r, uint32_t pc, uint32_t d, int c0, int u, ProbeCounts& cnt)
+{
+    const uint32_t pending = dz ? pass3Filter<L, true>(st, line, bloom, logBloom, cnt) : pass3Filter<L, false>(st, line, bloom, logBloom, cnt);
+    probeLine<L>(pending, line, s_rows, sortedMasks, table, logCap, best, cfgBits, r, pc, d, c0, u, cnt);
 }
 
 // Transposed bit layouts 1 and 2 of every item (layout 0 is the sorted masks themselves).
@@ -494,215 +578,112 @@ __global__ __launch_bounds__(256) void kLayouts(uint64_t* __restrict__ layouts, 
     }
 }
 
-// One block per item; the hash of the whole grid seen from each of its 8 corners goes into the
-// table and the Bloom filter (identity permutation, direction = corner, value at the corner).
+// One wave per item: the hash of the whole grid seen from each of its 8 corners (identity
+// permutation, direction = corner) goes into the table and the Bloom filter. It is the value the
+// search computes at the voxel where every scan ends, computed with the same functions: R lanes
+// per corner run passes 1 and 2 along y for one z each; the corner's first lane folds pass 3.
 template <int L>
-__global__ __launch_bounds__(Geo<L>::Threads) void kCornerTable(const uint64_t* __restrict__ layouts, uint32_t n, uint64_t* __restrict__ table, uint32_t logCap,
+__global__ __launch_bounds__(64) void kCornerTable(const uint64_t* __restrict__ layouts, uint32_t n, uint64_t* __restrict__ table, uint32_t logCap,
     uint32_t* __restrict__ bloom, uint32_t logBloom)
 {
-    constexpr int R = Geo<L>::R;
-    constexpr int W = Geo<L>::W;
-    __shared__ uint64_t s_layouts[3 * W];
-    __shared__ uint32_t s_h[Geo<L>::Buf];
-    __shared__ uint32_t s_any;
+    using G = Geo<L>;
+    constexpr int R = G::R;
+    __shared__ typename G::Row s_rows[G::Rows];
     const uint32_t r = blockIdx.x;
-    if (!loadLayouts<L>(layouts, n, r, s_layouts, &s_any))
+    if (!loadRows<L, 64>(layouts, n, r, s_rows, 1))
         return;
     const uint32_t capMask = (1u << logCap) - 1u;
-    for (uint32_t corner = 0; corner < 8; ++corner) {
-        hashGrid<L>(0, corner, s_layouts, s_h);
-        if (threadIdx.x == 0) {
-            const int ax = (corner & 1u) ? 0 : R - 1;
-            const int ay = (corner & 2u) ? 0 : R - 1;
-            const int az = (corner & 4u) ? 0 : R - 1;
-            const uint32_t h = s_h[sidx<L>(ax, ay, az)];
-            if (h != 0) {
-                atomicOr(&bloom[bloomWord(h, logBloom)], bloomBits(h));
-                const uint64_t packed = ((uint64_t)h << 32) | (uint64_t)((r << 3) | corner);
-                uint32_t slot = homeSlot32(h, logCap);
-                for (;;) {
-                    const unsigned long long prev = atomicCAS((unsigned long long*)&table[slot], 0ull, (unsigned long long)packed);
-                    if (prev == 0ull)
-                        break;
-                    slot = (slot + 1) & capMask;
-                }
-            }
+    const int lane = (int)threadIdx.x;
+    const int c2 = lane & (R - 1), sub = lane >> L;
+    for (int cb = 0; cb < 8; cb += G::P) {
+        const uint32_t corner = (uint32_t)(cb + sub);
+        const bool pos0 = corner & 1u, pos1 = corner & 2u, pos2 = corner & 4u;
+        const int c0 = pos0 ? 0 : R - 1;
+        const int sh = pos0 ? c0 : 31 - c0;
+        uint32_t s = 0;
+        for (int k = 0; k < R; ++k) {
+            const int c1 = pos1 ? (R - 1 - k) : k;
+            const uint32_t row = (uint32_t)s_rows[c1 + G::Pitch * c2]; // layout 0: row along x at (y, z) = (c1, c2)
+            s = hashStep(s, pos0 ? rowPrefix<true>(row, sh) : rowPrefix<false>(row, sh));
         }
-        __syncthreads();
-    }
-}
-
-// Probes every voxel's final hash in s_h for combo (pc, d) and merges verified hits into best.
-// Up to 8 Bloom-filter words per thread are in flight at a time; only hashes that pass the filter
-// touch the table. Hits that beat best[c] are verified cooperatively by the wave that found them.
-template <int L>
-__device__ __forceinline__ void probeGrid(const uint32_t* s_h, const uint64_t* s_layouts, const uint64_t* __restrict__ sortedMasks,
-    const uint64_t* __restrict__ table, uint32_t logCap, const uint32_t* __restrict__ bloom, uint32_t logBloom,
-    uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t r, uint32_t pc, uint32_t d, ProbeCounts& cnt)
-{
-    constexpr int R = Geo<L>::R;
-    constexpr int W = Geo<L>::W;
-    constexpr int T = Geo<L>::Threads;
-    constexpr int PerThread = Geo<L>::R3 / T; // == R
-    constexpr int Batch = PerThread < 8 ? PerThread : 8;
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t capMask = (1u << logCap) - 1u;
-    const int a0 = (int)permAxis(pc, 0), a1 = (int)permAxis(pc, 1), a2 = (int)permAxis(pc, 2);
-    // Filter phase: bit j of `pending` = the thread's voxel in slab j is non-empty and passes
-    // the Bloom filter. Loads are issued in batches for memory-level parallelism.
-    uint32_t pendingZ = 0;
-    for (int j0 = 0; j0 < PerThread; j0 += Batch) {
-        uint32_t hv[Batch];
-        uint32_t bw[Batch];
-#pragma unroll
-        for (int b = 0; b < Batch; ++b)
-            hv[b] = s_h[(j0 + b) * Geo<L>::Slab + threadIdx.x]; // slab z = j0 + b, in-slab index = thread
-#pragma unroll
-        for (int b = 0; b < Batch; ++b)
-            bw[b] = hv[b] ? bloom[bloomWord(hv[b], logBloom)] : 0u;
-#pragma unroll
-        for (int b = 0; b < Batch; ++b) {
-            const uint32_t need = bloomBits(hv[b]);
-            const bool pass = hv[b] != 0 && (bw[b] & need) == need;
-            cnt.probes += hv[b] != 0;
-            cnt.filterPass += pass;
-            pendingZ |= (uint32_t)pass << (j0 + b);
+        uint32_t s3 = 0;
+        for (int k = 0; k < R; ++k) {
+            const int z = pos2 ? (R - 1 - k) : k;
+            s3 = hashStep(s3, (uint32_t)__shfl((int)s, sub * R + z, 64));
         }
-    }
-    // Walk phase: every lane walks the table for its pending voxels one after another and
-    // stops at each candidate that beats best[]; the wave then verifies the stopped lanes'
-    // candidates together and resumes.
-    const int y = (int)(threadIdx.x >> L);
-    const int x = (int)(threadIdx.x & (R - 1)) ^ y; // undo the LDS swizzle
-    bool active = false;
-    uint32_t h = 0, slot = 0;
-    int z = 0;
-    while (true) {
-        bool found = false;
-        uint32_t hitC = 0;
-        uint64_t hitKey = 0;
-        while (true) {
-            if (!active) {
-                if (pendingZ == 0)
+        const uint32_t h = finalHash(s3);
+        if (c2 == 0 && h != 0) {
+            atomicOr(&bloom[bloomWord(h, logBloom)], bloomBits(h));
+            const uint64_t packed = ((uint64_t)h << 32) | (uint64_t)((r << 3) | corner);
+            uint32_t slot = homeSlot32(h, logCap);
+            for (;;) {
+                const unsigned long long prev = atomicCAS((unsigned long long*)&table[slot], 0ull, (unsigned long long)packed);
+                if (prev == 0ull)
                     break;
-                z = __ffs(pendingZ) - 1;
-                pendingZ &= pendingZ - 1u;
-                h = s_h[z * Geo<L>::Slab + threadIdx.x];
-                slot = homeSlot32(h, logCap);
-                active = true;
+                slot = (slot + 1) & capMask;
             }
-            const uint64_t e = table[slot];
-            if (e == 0) {
-                active = false;
-                continue;
-            }
-            slot = (slot + 1) & capMask;
-            if ((uint32_t)(e >> 32) != h)
-                continue;
-            ++cnt.keyHits;
-            const uint32_t cc = (uint32_t)e >> 3;
-            const uint32_t kc = (uint32_t)e & 7u;
-            if (cc <= r || cc == kDeadRank)
-                continue;
-            // Voxel of this hash in the permuted frame.
-            const int c3[3] = { x, y, z };
-            const int voxel[3] = { c3[a0], c3[a1], c3[a2] };
-            const uint32_t sym = d ^ kc;
-            int tt[3];
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                const int anchor = ((d >> a) & 1u) ? 0 : R - 1;
-                tt[a] = ((sym >> a) & 1u) ? voxel[a] - anchor : anchor - voxel[a];
-            }
-            if (!configAllows(cfgBits, pc, sym, tt[0], tt[1], tt[2]))
-                continue;
-            ++cnt.candidates;
-            const uint64_t k64 = encodeKey(r, pc, sym, tt[0], tt[1], tt[2], L);
-            if (k64 >= loadRelaxed(&best[cc]))
-                continue;
-            ++cnt.verifications;
-            found = true;
-            hitC = cc;
-            hitKey = k64;
-            break;
-        }
-        uint64_t stopped = __ballot(found);
-        if (stopped == 0)
-            break; // no lane found a candidate, so every lane ran out of work
-        while (stopped) {
-            const int leader = __ffsll((unsigned long long)stopped) - 1;
-            const uint32_t bc = (uint32_t)__shfl((int)hitC, leader, 64);
-            const uint64_t bk = (uint64_t)__shfl((long long)hitKey, leader, 64);
-            bool ok = true;
-            for (uint32_t wi = lane; wi < (uint32_t)W; wi += 64u)
-                ok = ok && (transformedWord<L>(s_layouts, bk, wi) == sortedMasks[(size_t)bc * W + wi]);
-            const bool allOk = (__ballot(!ok) == 0ull);
-            if ((int)lane == leader && allOk) {
-                ++cnt.verified;
-                atomicMin((unsigned long long*)&best[bc], (unsigned long long)bk);
-            }
-            stopped &= stopped - 1ull;
         }
     }
 }
 
-// Search kernel. One block per (parent rank, unit); R^2 threads own one line each per sweep.
-//   Units == 48: a unit is one combo (perm, direction): two sweeps, one hash buffer.
-//   Units == 6:  a unit is a family (first-pass axis a0 and direction dx) covering 8 combos.
-//                Buffers = 1: the fused first/second sweep is redone per combo;
-//                Buffers = 2: it is shared by the two directions of the third pass.
-// The autotuner picks per level.
-template <int L, int Units, int Buffers>
-__global__ __launch_bounds__(Geo<L>::Threads) void kSearch(const uint64_t* __restrict__ layouts, uint32_t n, const uint64_t* __restrict__ table, uint32_t logCap,
+// Search kernel for levels 3..5. One block per (parent rank, unit, plane chunk) with `Waves`
+// waves; every wave owns P = 64/R planes (fixed coordinate along the first pass axis) and runs
+// all passes, the filter and the table walk for them on its own: planes are independent, so
+// after the rows are loaded there is no block barrier.
+//   Family == false: a unit is one (permutation, direction) combo — grid.y = 48.
+//   Family == true:  a unit is a family (first-pass axis a0, direction dx) of 8 combos —
+//                    grid.y = 6; passes 1 and 2 are shared by the two directions of pass 3.
+// grid.z = MaxWaves / Waves plane chunks. The autotuner picks the variant per level.
+template <int L, int Waves, bool Family>
+__global__ __launch_bounds__(64 * Waves) void kSearch(const uint64_t* __restrict__ layouts, uint32_t n, const uint64_t* __restrict__ table, uint32_t logCap,
     const uint32_t* __restrict__ bloom, uint32_t logBloom, uint64_t* __restrict__ best, uint32_t cfgBits, unsigned long long* __restrict__ counters,
     uint32_t rankBase)
 {
+    using G = Geo<L>;
+    constexpr int R = G::R;
+    __shared__ typename G::Row s_rows[3 * G::Rows];
+    __shared__ uint32_t s_planes[Waves * G::P * G::Slab];
     ProbeCounts cnt;
-    constexpr int W = Geo<L>::W;
-    __shared__ uint64_t s_layouts[3 * W];
-    __shared__ uint32_t s_buf[Buffers][Geo<L>::Buf];
-    __shared__ uint32_t s_any;
     const uint32_t r = rankBase + blockIdx.x;
     const bool axisOn = (cfgBits & kCfgAxis) != 0;
-    if constexpr (Units == 48) {
-        const uint32_t combo = blockIdx.y;
-        const uint32_t pc = combo >> 3;
-        const uint32_t d = combo & 7u;
-        if (!axisOn && pc != 0)
-            return;
-        if (!loadLayouts<L>(layouts, n, r, s_layouts, &s_any))
-            return;
-        hashGrid<L>(pc, d, s_layouts, s_buf[0]);
-        probeGrid<L>(s_buf[0], s_layouts, layouts, table, logCap, bloom, logBloom, best, cfgBits, r, pc, d, cnt);
+    const uint32_t unit = blockIdx.y;
+    if (!axisOn && (Family ? (unit >> 1) != 0 : (unit >> 3) != 0))
+        return; // only permutation code 0 (p0 == 0) is allowed
+    if (!loadRows<L, 64 * Waves>(layouts, n, r, s_rows, 3))
+        return;
+    const int lane = (int)(threadIdx.x & 63u);
+    const int wave = (int)(threadIdx.x >> 6);
+    const int u = lane & (R - 1), w = lane >> L;
+    const int c0 = ((int)blockIdx.z * Waves + wave) * G::P + w;
+    uint32_t* plane = s_planes + (wave * G::P + w) * G::Slab;
+    uint32_t* line = plane + u; // the lane's pass-3 line: c1 = u, stride Pitch along c2
+    uint32_t st[R];
+    if constexpr (!Family) {
+        const uint32_t pc = unit >> 3, d = unit & 7u;
+        passes12<L>(pc, d, s_rows, c0, u, plane);
+        waveSync();
+#pragma unroll
+        for (int c2 = 0; c2 < R; ++c2)
+            st[c2] = line[G::Pitch * c2];
+        lastPassProbe<L>((d >> 2) & 1u, st, line, s_rows, layouts, table, logCap, bloom, logBloom, best, cfgBits, r, pc, d, c0, u, cnt);
     } else {
-        const uint32_t family = blockIdx.y;
-        const int a0 = (int)(family >> 1);
-        const uint32_t dx = family & 1u;
-        if (!axisOn && a0 != 0)
-            return; // permutation code 0 is the only one left and has p0 == 0
-        if (!loadLayouts<L>(layouts, n, r, s_layouts, &s_any))
-            return;
+        const uint32_t a0 = unit >> 1, dx = unit & 1u;
+#pragma unroll 1
         for (uint32_t sel = 0; sel < 2; ++sel) {
-            const uint32_t pc = 2u * (uint32_t)a0 + sel; // permutation codes with p0 == a0
+            const uint32_t pc = 2u * a0 + sel; // the permutation codes with p0 == a0
             if (!axisOn && pc != 0)
                 continue;
-            const int a2 = (int)permAxis(pc, 2);
+#pragma unroll 1
             for (uint32_t dy = 0; dy < 2; ++dy) {
-                if constexpr (Buffers == 2) {
-                    passes12<L>(pc, dx | (dy << 1), s_layouts, s_buf[1]);
-                    __syncthreads();
-                }
-                for (uint32_t dz = 0; dz < 2; ++dz) {
-                    const uint32_t d = dx | (dy << 1) | (dz << 2);
-                    if constexpr (Buffers == 1) {
-                        hashGrid<L>(pc, d, s_layouts, s_buf[0]);
-                    } else {
-                        passLast<L>(a2, dz, s_buf[1], s_buf[0]);
-                        __syncthreads();
-                    }
-                    probeGrid<L>(s_buf[0], s_layouts, layouts, table, logCap, bloom, logBloom, best, cfgBits, r, pc, d, cnt);
-                    __syncthreads();
-                }
+                waveSync();
+                passes12Call<L>(pc, dx | (dy << 1), s_rows, c0, u, plane);
+                waveSync();
+#pragma unroll
+                for (int c2 = 0; c2 < R; ++c2)
+                    st[c2] = line[G::Pitch * c2];
+#pragma unroll 1
+                for (uint32_t dz = 0; dz < 2; ++dz)
+                    lastPassProbe<L>(dz, st, line, s_rows, layouts, table, logCap, bloom, logBloom, best, cfgBits, r, pc, dx | (dy << 1) | (dz << 2), c0, u, cnt);
             }
         }
     }
@@ -806,29 +787,43 @@ __global__ __launch_bounds__(256) void kFinalMapping(const uint64_t* __restrict_
             return e_;                   \
     } while (0)
 
+template <int L, int Waves, bool Family>
+hipError_t launchSearchShape(uint32_t parents, uint32_t r0, uint32_t n, const uint64_t* layouts, const uint64_t* table, uint32_t logCap, const uint32_t* bloom,
+    uint32_t logBloom, uint64_t* best, uint32_t cfgBits, unsigned long long* counters, hipStream_t stream)
+{
+    if constexpr (Waves >= 1 && Waves <= Geo<L>::MaxWaves) {
+        const dim3 grid(parents, Family ? 6 : 48, Geo<L>::MaxWaves / Waves);
+        hipLaunchKernelGGL((kSearch<L, Waves, Family>), grid, dim3(64 * Waves), 0, stream, layouts, n, table, logCap, bloom, logBloom, best, cfgBits, counters, r0);
+        return hipGetLastError();
+    } else {
+        return hipErrorInvalidValue;
+    }
+}
+
+// Variant v: bit 0 = family units, v >> 1 = log2 of the plane-chunk split (a block covers
+// MaxWaves >> (v >> 1) waves' worth of planes).
 template <int L>
 hipError_t launchSearch(uint32_t variant, uint32_t n, uint32_t r0, uint32_t r1, const uint64_t* layouts, const uint64_t* table, uint32_t logCap, const uint32_t* bloom,
     uint32_t logBloom, uint64_t* best, uint32_t cfgBits, unsigned long long* counters, hipStream_t stream)
 {
-    const dim3 block(Geo<L>::Threads);
+    constexpr int M = Geo<L>::MaxWaves;
     const uint32_t parents = r1 - r0;
     switch (variant) {
-    case kVariantCombo:
-        hipLaunchKernelGGL((kSearch<L, 48, 1>), dim3(parents, 48), block, 0, stream, layouts, n, table, logCap, bloom, logBloom, best, cfgBits, counters, r0);
-        break;
-    case kVariantFamily1:
-        hipLaunchKernelGGL((kSearch<L, 6, 1>), dim3(parents, 6), block, 0, stream, layouts, n, table, logCap, bloom, logBloom, best, cfgBits, counters, r0);
-        break;
-    case kVariantFamily2:
-        if constexpr (L < 5)
-            hipLaunchKernelGGL((kSearch<L, 6, 2>), dim3(parents, 6), block, 0, stream, layouts, n, table, logCap, bloom, logBloom, best, cfgBits, counters, r0);
-        else
-            return hipErrorInvalidValue;
-        break;
+    case 0:
+        return launchSearchShape<L, M, false>(parents, r0, n, layouts, table, logCap, bloom, logBloom, best, cfgBits, counters, stream);
+    case 1:
+        return launchSearchShape<L, M, true>(parents, r0, n, layouts, table, logCap, bloom, logBloom, best, cfgBits, counters, stream);
+    case 2:
+        return launchSearchShape<L, M / 2, false>(parents, r0, n, layouts, table, logCap, bloom, logBloom, best, cfgBits, counters, stream);
+    case 3:
+        return launchSearchShape<L, M / 2, true>(parents, r0, n, layouts, table, logCap, bloom, logBloom, best, cfgBits, counters, stream);
+    case 4:
+        return launchSearchShape<L, M / 4, false>(parents, r0, n, layouts, table, logCap, bloom, logBloom, best, cfgBits, counters, stream);
+    case 5:
+        return launchSearchShape<L, M / 4, true>(parents, r0, n, layouts, table, logCap, bloom, logBloom, best, cfgBits, counters, stream);
     default:
         return hipErrorInvalidValue;
     }
-    return hipGetLastError();
 }
 
 uint32_t log2Ceil(uint64_t v)
@@ -844,19 +839,22 @@ uint32_t log2Ceil(uint64_t v)
 bool variantValid(uint32_t level, uint32_t variant)
 {
     if (level == 2)
-        return variant == kVariantCombo;
+        return variant == 0;
     if (level < 2 || level > 5 || variant >= kNumVariants)
         return false;
-    return level < 5 || variant <= kVariantFamily1;
+    const uint32_t maxWaves = (1u << (2 * level)) / 64u;
+    return (maxWaves >> (variant >> 1)) >= 1u;
 }
 
 uint32_t defaultVariant(uint32_t level)
 {
     switch (level) {
     case 5:
-        return kVariantFamily1;
+        return 3; // family units, half the planes per block (two blocks per CU)
+    case 4:
+        return 1;
     default:
-        return kVariantCombo;
+        return 0;
     }
 }
 
@@ -957,15 +955,15 @@ hipError_t enqueueMatch(const MatchPlan& p, const uint64_t* d_masks, uint32_t cf
         break;
     case 3:
         hipLaunchKernelGGL(kLayouts<3>, dim3(n), dim3(256), 0, stream, sortedMasks, n);
-        hipLaunchKernelGGL(kCornerTable<3>, dim3(n), dim3(Geo<3>::Threads), 0, stream, sortedMasks, n, tabKeys, p.logCap, bloom, p.logBloom);
+        hipLaunchKernelGGL(kCornerTable<3>, dim3(n), dim3(64), 0, stream, sortedMasks, n, tabKeys, p.logCap, bloom, p.logBloom);
         break;
     case 4:
         hipLaunchKernelGGL(kLayouts<4>, dim3(n), dim3(256), 0, stream, sortedMasks, n);
-        hipLaunchKernelGGL(kCornerTable<4>, dim3(n), dim3(Geo<4>::Threads), 0, stream, sortedMasks, n, tabKeys, p.logCap, bloom, p.logBloom);
+        hipLaunchKernelGGL(kCornerTable<4>, dim3(n), dim3(64), 0, stream, sortedMasks, n, tabKeys, p.logCap, bloom, p.logBloom);
         break;
     case 5:
         hipLaunchKernelGGL(kLayouts<5>, dim3(n), dim3(256), 0, stream, sortedMasks, n);
-        hipLaunchKernelGGL(kCornerTable<5>, dim3(n), dim3(Geo<5>::Threads), 0, stream, sortedMasks, n, tabKeys, p.logCap, bloom, p.logBloom);
+        hipLaunchKernelGGL(kCornerTable<5>, dim3(n), dim3(64), 0, stream, sortedMasks, n, tabKeys, p.logCap, bloom, p.logBloom);
         break;
     default:
         return hipErrorInvalidValue;

@@ -18,12 +18,10 @@ constexpr uint32_t kCfgCount = 8u;
 // candidates, verifications, verified.
 constexpr int kNumCounters = 8;
 
-// Search-kernel variants for levels 3..5 (the autotuner's tile shapes): one combo per block,
-// or one family of 8 combos per block with 1 or 2 LDS hash buffers.
-constexpr uint32_t kVariantCombo = 0;
-constexpr uint32_t kVariantFamily1 = 1;
-constexpr uint32_t kVariantFamily2 = 2;
-constexpr uint32_t kNumVariants = 3;
+// Search-kernel variants for levels 3..5 (the autotuner's tile shapes). Bit 0: a block runs one
+// (permutation, direction) combo (0) or a family of 8 combos sharing passes 1 and 2 (1).
+// variant >> 1: log2 of the split of the R planes of a unit over blocks (0, 1, 2).
+constexpr uint32_t kNumVariants = 6;
 
 bool variantValid(uint32_t level, uint32_t variant);
 uint32_t defaultVariant(uint32_t level);

@@ -26,6 +26,7 @@ _u64p = ctypes.POINTER(ctypes.c_uint64)
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 
 TDAG_OK = 0
+VARIANT_SLOTS = 8  # TDAG_VARIANT_SLOTS in include/tdag.h
 _STATUS = {0: "OK", 1: "EINVAL", 2: "ENOMEM", 3: "EDEVICE", 4: "EINTERNAL"}
 
 
@@ -220,13 +221,13 @@ class Context:
         """Times every valid search variant, then four first-batch sizes, on device masks and
         keeps the fastest; returns (best_variant, first_batch, {variant: ms}, tuned_ms)."""
         cfg = (config or TransformDAGConfig()).to_c()
-        ms = (ctypes.c_float * 5)()
+        ms = (ctypes.c_float * (VARIANT_SLOTS + 1))()
         best = ctypes.c_int32()
         first = ctypes.c_uint32()
         ptr = ctypes.c_void_p(d_masks if isinstance(d_masks, int) else d_masks.data_ptr())
         check(lib().tdag_autotune_level(self._ptr, int(level), ptr, int(n), ctypes.byref(cfg), int(repeats), ms, ctypes.byref(best),
                                         ctypes.byref(first)), "tdag_autotune_level")
-        return int(best.value), int(first.value), {v: round(float(ms[v]), 4) for v in range(4) if ms[v] >= 0}, round(float(ms[4]), 4)
+        return int(best.value), int(first.value), {v: round(float(ms[v]), 4) for v in range(VARIANT_SLOTS) if ms[v] >= 0}, round(float(ms[VARIANT_SLOTS]), 4)
 
     def set_filter_bits(self, level: int, bits_per_entry: int) -> None:
         """Bloom filter bits per table entry (0 = default 16)."""

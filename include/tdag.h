@@ -109,10 +109,11 @@ tdag_status tdag_last_stats(const tdag_context* ctx, tdag_match_stats* out);
  * (diagnostics; costs a few percent). No reference counterpart. */
 tdag_status tdag_context_set_counters(tdag_context* ctx, int enable);
 
-/* Search-kernel variant used at `level` (3..5) by this context: 0 = one (permutation, direction)
- * combo per block; 1, 2 = one family of 8 combos per block with 1 or 2 LDS hash buffers (the
- * second buffer shares the fused first/second hash sweep between the two directions of the
- * third, at the cost of LDS; levels 3 and 4 only). -1 restores the library default.
+/* Search-kernel variant used at `level` (3..5) by this context. Bit 0: a block runs one
+ * (permutation, direction) combo (0) or a family of 8 combos that share the first two hash
+ * passes (1). variant >> 1 (0, 1, 2): log2 of how many blocks share the R planes of one unit
+ * (more blocks = less LDS per block = more blocks per CU). Level 3 has variants 0 and 1, levels
+ * 4 and 5 have 0..5. -1 restores the library default.
  * Level 2 has a single kernel (variant 0). No reference counterpart: a tuning knob. */
 tdag_status tdag_context_set_variant(tdag_context* ctx, uint32_t level, int32_t variant);
 tdag_status tdag_context_get_variant(const tdag_context* ctx, uint32_t level, int32_t* variant);
@@ -134,9 +135,11 @@ tdag_status tdag_context_get_filter_bits(const tdag_context* ctx, uint32_t level
 /* Autotuner: runs the level-`level` matcher once per valid variant on the device masks (plus
  * one untimed warm-up run each) and keeps the fastest search kernel for this context, then tries
  * four first-batch sizes (n/256, n/64, n/16, n) with it and keeps the fastest, then filter sizes
- * of 4, 8, 16 and 32 bits per table entry. Writes the
- * average search time per variant (out_ms[0..3], -1 for invalid or unused; out_ms[4] = the tuned
- * schedule's time), the winning variant and first batch. No reference counterpart. */
+ * of 4, 8, 16 and 32 bits per table entry. out_ms (TDAG_VARIANT_SLOTS + 1 floats, may be NULL)
+ * receives the average search time per variant (-1 for invalid or unused slots) followed by the
+ * tuned schedule's time; also writes the winning variant and first batch. No reference
+ * counterpart. */
+#define TDAG_VARIANT_SLOTS 8
 tdag_status tdag_autotune_level(tdag_context* ctx, uint32_t level, const uint64_t* d_masks, uint64_t n, const tdag_config* cfg, uint32_t repeats,
     float* out_ms, int32_t* out_best, uint32_t* out_first_batch);
 

@@ -68,8 +68,7 @@ def test_larger_sets(oracle, gpu_ctx, level, n):
     _check(oracle, gpu_ctx, level, masks)
 
 
-@pytest.mark.parametrize("level,n_base,variant", [(3, 60, 0), (3, 60, 1), (3, 60, 2), (4, 20, 0), (4, 20, 1), (4, 20, 2),
-                                                   (5, 6, 0), (5, 6, 1)])
+@pytest.mark.parametrize("level,n_base,variant", [(3, 60, 0), (3, 60, 1)] + [(4, 20, v) for v in range(6)] + [(5, 6, v) for v in range(6)])
 def test_search_variants(oracle, gpu_ctx, level, n_base, variant):
     """Every search-kernel variant (tile shape) the autotuner may pick is bit-exact."""
     rng = np.random.default_rng(500 + 10 * level + variant)
