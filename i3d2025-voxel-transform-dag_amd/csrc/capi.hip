@@ -334,10 +334,10 @@ tdag_status tdag_autotune_level(tdag_context* ctx, uint32_t level, const uint64_
         ctx->batchGrowth[level] = 2;
         // Finally the Bloom filter size (bits per table entry): small filters stay in L1.
         const uint32_t bitsCands[4] = { 4, 8, 16, 32 };
-        uint32_t bestBits = 16;
+        uint32_t bestBits = tdag::defaultFilterBits(level);
         float bestBitsMs = bestBatchMs;
         for (uint32_t bi = 0; bi < 4; ++bi) {
-            if (bitsCands[bi] == 16)
+            if (bitsCands[bi] == bestBits)
                 continue;
             const tdag::MatchPlan plan = tdag::planMatch(level, n, (uint32_t)best, bestFirst, 2, bitsCands[bi]);
             if (tdag_status s = ensureBuffer(ctx->scratch, ctx->scratchBytes, plan.scratchBytes, "matcher scratch allocation"); s != TDAG_OK)
@@ -377,7 +377,7 @@ tdag_status tdag_context_get_filter_bits(const tdag_context* ctx, uint32_t level
 {
     if (!ctx || !bits_per_entry || level < 2 || level > 5)
         return fail(TDAG_EINVAL, "bad argument");
-    *bits_per_entry = ctx->bloomBits[level] ? ctx->bloomBits[level] : 16;
+    *bits_per_entry = ctx->bloomBits[level] ? ctx->bloomBits[level] : tdag::defaultFilterBits(level);
     return TDAG_OK;
 }
 

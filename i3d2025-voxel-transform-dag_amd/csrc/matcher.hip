@@ -44,6 +44,9 @@ __device__ __forceinline__ bool configAllows(uint32_t cfgBits, uint32_t permCode
     return true;
 }
 
+// LDS bytes per workgroup on gfx950.
+constexpr size_t kLdsBytes = 160 * 1024;
+
 // Child rank stored in a table entry whose child can no longer improve (tombstone; the key stays
 // so linear-probing clusters remain intact). Ranks are < 2^28, so this never names an item.
 constexpr uint32_t kDeadRank = 0x1FFFFFFFu;
@@ -377,31 +380,27 @@ __device__ __forceinline__ void passes12(uint32_t pc, uint32_t d, const typename
     }
 }
 
-// Out-of-line copy of passes12 for the family kernels: inlined into the combo loop, its 24
-// variants are duplicated and hoisted by the compiler until the kernel spills.
-template <int L>
-__device__ __noinline__ void passes12Call(uint32_t pc, uint32_t d, const typename Geo<L>::Row* s_rows, int c0, int u, uint32_t* plane)
-{
-    passes12<L>(pc, d, s_rows, c0, u, plane);
-}
-
-// Pass 3 along the lane's line (c1 = u) of its plane over the pass-2 states st[c2], fused with
-// the Bloom filter: stores the final hash of every voxel in the plane (in place of the pass-2
-// states) and returns the bit set of voxels c2 that are non-empty and pass the filter. The
-// filter word is loaded unconditionally (every hash indexes a valid word) so the sweep has no
-// branches and its loads overlap the hashing.
+// Pass 3 along the lane's line (c1 = u) of its plane over the pass-2 states in[Pitch*c2], fused
+// with the Bloom filter: stores the final hash of every voxel at out[Pitch*c2] (out may equal in)
+// and returns the bit set of voxels c2 that are non-empty and pass the filter. The filter word is
+// loaded unconditionally (every hash indexes a valid word) so the sweep has no branches and its
+// loads overlap the hashing.
 template <int L, bool Pos2>
-__device__ __forceinline__ uint32_t pass3Filter(const uint32_t (&st)[Geo<L>::R], uint32_t* line, const uint32_t* __restrict__ bloom, uint32_t logBloom, ProbeCounts& cnt)
+__device__ __forceinline__ uint32_t pass3Filter(const uint32_t* in, uint32_t* out, const uint32_t* __restrict__ bloom, uint32_t logBloom, ProbeCounts& cnt)
 {
     constexpr int R = Geo<L>::R;
     const uint32_t shift = 32u - logBloom;
+    uint32_t st[R];
+#pragma unroll
+    for (int c2 = 0; c2 < R; ++c2)
+        st[c2] = in[Geo<L>::Pitch * c2];
     uint32_t s = 0, pending = 0, nonEmpty = 0;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const int c2 = Pos2 ? (R - 1 - k) : k;
         s = hashStep(s, st[c2]);
         const uint32_t f = finalHash(s);
-        line[Geo<L>::Pitch * c2] = f;
+        out[Geo<L>::Pitch * c2] = f;
         const uint32_t bw = bloom[f >> shift];
         const uint32_t need = bloomBits(f);
         const uint32_t miss = need & ~bw;
@@ -541,14 +540,15 @@ __device__ __forceinline__ void probeLine(uint32_t pending, const uint32_t* line
     }
 }
 
-// Pass 3 in direction dz with the filter, then the table walk of the lane's line.
+// Pass 3 in direction dz with the filter (pass-2 states read from `in`, final hashes written to
+// `out`), then the table walk of the lane's line.
 template <int L>
-__device__ __forceinline__ void lastPassProbe(uint32_t dz, const uint32_t (&st)[Geo<L>::R], uint32_t* line, const typename Geo<L>::Row* s_rows,
+__device__ __forceinline__ void lastPassProbe(uint32_t dz, const uint32_t* in, uint32_t* out, const typename Geo<L>::Row* s_rows,
     const uint64_t* __restrict__ sortedMasks, const uint64_t* __restrict__ table, uint32_t logCap, const uint32_t* __restrict__ bloom, uint32_t logBloom,
     uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t r, uint32_t pc, uint32_t d, int c0, int u, ProbeCounts& cnt)
 {
-    const uint32_t pending = dz ? pass3Filter<L, true>(st, line, bloom, logBloom, cnt) : pass3Filter<L, false>(st, line, bloom, logBloom, cnt);
-    probeLine<L>(pending, line, s_rows, sortedMasks, table, logCap, best, cfgBits, r, pc, d, c0, u, cnt);
+    const uint32_t pending = dz ? pass3Filter<L, true>(in, out, bloom, logBloom, cnt) : pass3Filter<L, false>(in, out, bloom, logBloom, cnt);
+    probeLine<L>(pending, out, s_rows, sortedMasks, table, logCap, best, cfgBits, r, pc, d, c0, u, cnt);
 }
 
 // Transposed bit layouts 1 and 2 of every item (layout 0 is the sorted masks themselves).
@@ -626,28 +626,43 @@ __global__ __launch_bounds__(64) void kCornerTable(const uint64_t* __restrict__ 
     }
 }
 
+// Search-unit kinds of the level-3..5 kernel.
+constexpr int kUnitCombo = 0; // one (permutation, direction) combo per block
+constexpr int kUnitFamily = 1; // a family of 8 combos per block, each run from scratch
+constexpr int kUnitShared = 2; // a family of 8 combos; passes 1 and 2 shared by both pass-3 directions
+
+// LDS bytes of one kSearch block.
+template <int L, int Waves, int Kind>
+constexpr size_t searchLds()
+{
+    return 3 * Geo<L>::Rows * sizeof(typename Geo<L>::Row) + (Kind == kUnitShared ? 2 : 1) * (size_t)Waves * Geo<L>::P * Geo<L>::Slab * 4;
+}
+
 // Search kernel for levels 3..5. One block per (parent rank, unit, plane chunk) with `Waves`
 // waves; every wave owns P = 64/R planes (fixed coordinate along the first pass axis) and runs
 // all passes, the filter and the table walk for them on its own: planes are independent, so
 // after the rows are loaded there is no block barrier.
-//   Family == false: a unit is one (permutation, direction) combo — grid.y = 48.
-//   Family == true:  a unit is a family (first-pass axis a0, direction dx) of 8 combos —
-//                    grid.y = 6; passes 1 and 2 are shared by the two directions of pass 3.
+//   kUnitCombo:  grid.y = 48 combos.
+//   kUnitFamily: grid.y = 6 families (first-pass axis a0, direction dx) of 8 combos each; the
+//                rows are loaded once per family.
+//   kUnitShared: as kUnitFamily, and passes 1 and 2 run once for both directions of pass 3
+//                (their states stay in a second plane buffer).
 // grid.z = MaxWaves / Waves plane chunks. The autotuner picks the variant per level.
-template <int L, int Waves, bool Family>
+template <int L, int Waves, int Kind>
 __global__ __launch_bounds__(64 * Waves) void kSearch(const uint64_t* __restrict__ layouts, uint32_t n, const uint64_t* __restrict__ table, uint32_t logCap,
     const uint32_t* __restrict__ bloom, uint32_t logBloom, uint64_t* __restrict__ best, uint32_t cfgBits, unsigned long long* __restrict__ counters,
     uint32_t rankBase)
 {
     using G = Geo<L>;
     constexpr int R = G::R;
+    constexpr int Buffers = Kind == kUnitShared ? 2 : 1;
     __shared__ typename G::Row s_rows[3 * G::Rows];
-    __shared__ uint32_t s_planes[Waves * G::P * G::Slab];
+    __shared__ uint32_t s_planes[Buffers][Waves * G::P * G::Slab];
     ProbeCounts cnt;
     const uint32_t r = rankBase + blockIdx.x;
     const bool axisOn = (cfgBits & kCfgAxis) != 0;
     const uint32_t unit = blockIdx.y;
-    if (!axisOn && (Family ? (unit >> 1) != 0 : (unit >> 3) != 0))
+    if (!axisOn && (Kind == kUnitCombo ? (unit >> 3) != 0 : (unit >> 1) != 0))
         return; // only permutation code 0 (p0 == 0) is allowed
     if (!loadRows<L, 64 * Waves>(layouts, n, r, s_rows, 3))
         return;
@@ -655,17 +670,15 @@ __global__ __launch_bounds__(64 * Waves) void kSearch(const uint64_t* __restrict
     const int wave = (int)(threadIdx.x >> 6);
     const int u = lane & (R - 1), w = lane >> L;
     const int c0 = ((int)blockIdx.z * Waves + wave) * G::P + w;
-    uint32_t* plane = s_planes + (wave * G::P + w) * G::Slab;
+    const int planeOff = (wave * G::P + w) * G::Slab;
+    uint32_t* plane = s_planes[0] + planeOff;
     uint32_t* line = plane + u; // the lane's pass-3 line: c1 = u, stride Pitch along c2
-    uint32_t st[R];
-    if constexpr (!Family) {
+    uint32_t* outLine = s_planes[Buffers - 1] + planeOff + u;
+    if constexpr (Kind == kUnitCombo) {
         const uint32_t pc = unit >> 3, d = unit & 7u;
         passes12<L>(pc, d, s_rows, c0, u, plane);
         waveSync();
-#pragma unroll
-        for (int c2 = 0; c2 < R; ++c2)
-            st[c2] = line[G::Pitch * c2];
-        lastPassProbe<L>((d >> 2) & 1u, st, line, s_rows, layouts, table, logCap, bloom, logBloom, best, cfgBits, r, pc, d, c0, u, cnt);
+        lastPassProbe<L>((d >> 2) & 1u, line, line, s_rows, layouts, table, logCap, bloom, logBloom, best, cfgBits, r, pc, d, c0, u, cnt);
     } else {
         const uint32_t a0 = unit >> 1, dx = unit & 1u;
 #pragma unroll 1
@@ -675,15 +688,21 @@ __global__ __launch_bounds__(64 * Waves) void kSearch(const uint64_t* __restrict
                 continue;
 #pragma unroll 1
             for (uint32_t dy = 0; dy < 2; ++dy) {
-                waveSync();
-                passes12Call<L>(pc, dx | (dy << 1), s_rows, c0, u, plane);
-                waveSync();
-#pragma unroll
-                for (int c2 = 0; c2 < R; ++c2)
-                    st[c2] = line[G::Pitch * c2];
+                if constexpr (Kind == kUnitShared) {
+                    waveSync();
+                    passes12<L>(pc, dx | (dy << 1), s_rows, c0, u, plane);
+                    waveSync();
+                }
 #pragma unroll 1
-                for (uint32_t dz = 0; dz < 2; ++dz)
-                    lastPassProbe<L>(dz, st, line, s_rows, layouts, table, logCap, bloom, logBloom, best, cfgBits, r, pc, dx | (dy << 1) | (dz << 2), c0, u, cnt);
+                for (uint32_t dz = 0; dz < 2; ++dz) {
+                    const uint32_t d = dx | (dy << 1) | (dz << 2);
+                    if constexpr (Kind == kUnitFamily) {
+                        waveSync();
+                        passes12<L>(pc, d, s_rows, c0, u, plane);
+                        waveSync();
+                    }
+                    lastPassProbe<L>(dz, line, outLine, s_rows, layouts, table, logCap, bloom, logBloom, best, cfgBits, r, pc, d, c0, u, cnt);
+                }
             }
         }
     }
@@ -787,43 +806,51 @@ __global__ __launch_bounds__(256) void kFinalMapping(const uint64_t* __restrict_
             return e_;                   \
     } while (0)
 
-template <int L, int Waves, bool Family>
+template <int L, int Waves, int Kind>
 hipError_t launchSearchShape(uint32_t parents, uint32_t r0, uint32_t n, const uint64_t* layouts, const uint64_t* table, uint32_t logCap, const uint32_t* bloom,
     uint32_t logBloom, uint64_t* best, uint32_t cfgBits, unsigned long long* counters, hipStream_t stream)
 {
-    if constexpr (Waves >= 1 && Waves <= Geo<L>::MaxWaves) {
-        const dim3 grid(parents, Family ? 6 : 48, Geo<L>::MaxWaves / Waves);
-        hipLaunchKernelGGL((kSearch<L, Waves, Family>), grid, dim3(64 * Waves), 0, stream, layouts, n, table, logCap, bloom, logBloom, best, cfgBits, counters, r0);
+    if constexpr (Waves >= 1 && Waves <= Geo<L>::MaxWaves && searchLds<L, Waves, Kind>() <= kLdsBytes) {
+        const dim3 grid(parents, Kind == kUnitCombo ? 48 : 6, Geo<L>::MaxWaves / Waves);
+        hipLaunchKernelGGL((kSearch<L, Waves, Kind>), grid, dim3(64 * Waves), 0, stream, layouts, n, table, logCap, bloom, logBloom, best, cfgBits, counters, r0);
         return hipGetLastError();
     } else {
         return hipErrorInvalidValue;
     }
 }
 
-// Variant v: bit 0 = family units, v >> 1 = log2 of the plane-chunk split (a block covers
-// MaxWaves >> (v >> 1) waves' worth of planes).
+// Variant v: unit kind v % 3, plane-chunk split 2^(v / 3) (a block covers MaxWaves >> (v / 3)
+// waves' worth of planes).
 template <int L>
 hipError_t launchSearch(uint32_t variant, uint32_t n, uint32_t r0, uint32_t r1, const uint64_t* layouts, const uint64_t* table, uint32_t logCap, const uint32_t* bloom,
     uint32_t logBloom, uint64_t* best, uint32_t cfgBits, unsigned long long* counters, hipStream_t stream)
 {
     constexpr int M = Geo<L>::MaxWaves;
-    const uint32_t parents = r1 - r0;
+    const uint32_t p = r1 - r0;
+#define TDAG_SHAPE(W, K) launchSearchShape<L, W, K>(p, r0, n, layouts, table, logCap, bloom, logBloom, best, cfgBits, counters, stream)
     switch (variant) {
     case 0:
-        return launchSearchShape<L, M, false>(parents, r0, n, layouts, table, logCap, bloom, logBloom, best, cfgBits, counters, stream);
+        return TDAG_SHAPE(M, kUnitCombo);
     case 1:
-        return launchSearchShape<L, M, true>(parents, r0, n, layouts, table, logCap, bloom, logBloom, best, cfgBits, counters, stream);
+        return TDAG_SHAPE(M, kUnitFamily);
     case 2:
-        return launchSearchShape<L, M / 2, false>(parents, r0, n, layouts, table, logCap, bloom, logBloom, best, cfgBits, counters, stream);
+        return TDAG_SHAPE(M, kUnitShared);
     case 3:
-        return launchSearchShape<L, M / 2, true>(parents, r0, n, layouts, table, logCap, bloom, logBloom, best, cfgBits, counters, stream);
+        return TDAG_SHAPE(M / 2, kUnitCombo);
     case 4:
-        return launchSearchShape<L, M / 4, false>(parents, r0, n, layouts, table, logCap, bloom, logBloom, best, cfgBits, counters, stream);
+        return TDAG_SHAPE(M / 2, kUnitFamily);
     case 5:
-        return launchSearchShape<L, M / 4, true>(parents, r0, n, layouts, table, logCap, bloom, logBloom, best, cfgBits, counters, stream);
+        return TDAG_SHAPE(M / 2, kUnitShared);
+    case 6:
+        return TDAG_SHAPE(M / 4, kUnitCombo);
+    case 7:
+        return TDAG_SHAPE(M / 4, kUnitFamily);
+    case 8:
+        return TDAG_SHAPE(M / 4, kUnitShared);
     default:
         return hipErrorInvalidValue;
     }
+#undef TDAG_SHAPE
 }
 
 uint32_t log2Ceil(uint64_t v)
@@ -842,20 +869,32 @@ bool variantValid(uint32_t level, uint32_t variant)
         return variant == 0;
     if (level < 2 || level > 5 || variant >= kNumVariants)
         return false;
-    const uint32_t maxWaves = (1u << (2 * level)) / 64u;
-    return (maxWaves >> (variant >> 1)) >= 1u;
+    const uint32_t R = 1u << level;
+    const uint32_t maxWaves = R * R / 64u;
+    const uint32_t waves = maxWaves >> (variant / 3);
+    if (waves < 1u)
+        return false;
+    const uint32_t rowBytes = R / 8u;
+    const size_t lds = 3ull * R * (R + 1) * rowBytes + (variant % 3 == 2 ? 2ull : 1ull) * waves * (64u / R) * R * (R + 1) * 4ull;
+    return lds <= kLdsBytes;
 }
 
 uint32_t defaultVariant(uint32_t level)
 {
     switch (level) {
     case 5:
-        return 3; // family units, half the planes per block (two blocks per CU)
+        return 4; // family units, half the planes per block (two blocks per CU)
     case 4:
-        return 1;
+    case 3:
+        return 2; // family units sharing passes 1 and 2, all planes in one block
     default:
         return 0;
     }
+}
+
+uint32_t defaultFilterBits(uint32_t level)
+{
+    return level == 3 ? 16u : 4u;
 }
 
 MatchPlan planMatch(uint32_t level, uint64_t n, uint32_t variant, uint32_t firstBatch, uint32_t batchGrowth, uint32_t bloomBitsPerEntry)
@@ -886,7 +925,7 @@ MatchPlan planMatch(uint32_t level, uint64_t n, uint32_t variant, uint32_t first
     p.offTabKeys = take(cap * 8);
     p.offTabVals = take(level == 2 ? cap * 4 : 0);
     // Filter words = entries * bitsPerEntry / 32 (rounded up to a power of two).
-    p.bloomBitsPerEntry = bloomBitsPerEntry ? bloomBitsPerEntry : 16;
+    p.bloomBitsPerEntry = bloomBitsPerEntry ? bloomBitsPerEntry : defaultFilterBits(level);
     p.logBloom = std::min<uint32_t>(22, std::max<uint32_t>(6, log2Ceil((entries * p.bloomBitsPerEntry + 31) / 32)));
     p.offBloom = take((1ull << p.logBloom) * 4);
     p.offParentTp = take(n * 8);

@@ -18,13 +18,17 @@ constexpr uint32_t kCfgCount = 8u;
 // candidates, verifications, verified.
 constexpr int kNumCounters = 8;
 
-// Search-kernel variants for levels 3..5 (the autotuner's tile shapes). Bit 0: a block runs one
-// (permutation, direction) combo (0) or a family of 8 combos sharing passes 1 and 2 (1).
-// variant >> 1: log2 of the split of the R planes of a unit over blocks (0, 1, 2).
-constexpr uint32_t kNumVariants = 6;
+// Search-kernel variants for levels 3..5 (the autotuner's tile shapes). variant % 3: a block
+// runs one (permutation, direction) combo (0), a family of 8 combos (1), or a family of 8 combos
+// that shares passes 1 and 2 between the two pass-3 directions (2). variant / 3: log2 of the
+// split of the R planes of a unit over blocks (0, 1, 2). Variants whose blocks exceed the LDS
+// are invalid.
+constexpr uint32_t kNumVariants = 9;
 
 bool variantValid(uint32_t level, uint32_t variant);
 uint32_t defaultVariant(uint32_t level);
+// Bloom filter bits per table entry used when a plan asks for 0.
+uint32_t defaultFilterBits(uint32_t level);
 
 // Device scratch layout of one matcher call.
 struct MatchPlan {
@@ -38,7 +42,7 @@ struct MatchPlan {
 };
 
 // firstBatch = 0 / batchGrowth = 0 select the defaults (max(256, n/64) parents, doubling).
-// bloomBitsPerEntry = 0 selects 16.
+// bloomBitsPerEntry = 0 selects defaultFilterBits(level).
 MatchPlan planMatch(uint32_t level, uint64_t n, uint32_t variant, uint32_t firstBatch = 0, uint32_t batchGrowth = 0, uint32_t bloomBitsPerEntry = 0);
 
 // Enqueues the whole matcher on `stream`. Outputs may be null. The device counters at

@@ -26,7 +26,7 @@ _u64p = ctypes.POINTER(ctypes.c_uint64)
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 
 TDAG_OK = 0
-VARIANT_SLOTS = 8  # TDAG_VARIANT_SLOTS in include/tdag.h
+VARIANT_SLOTS = 16  # TDAG_VARIANT_SLOTS in include/tdag.h
 _STATUS = {0: "OK", 1: "EINVAL", 2: "ENOMEM", 3: "EDEVICE", 4: "EINTERNAL"}
 
 

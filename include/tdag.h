@@ -126,7 +126,8 @@ tdag_status tdag_context_get_variant(const tdag_context* ctx, uint32_t level, in
 tdag_status tdag_context_set_batching(tdag_context* ctx, uint32_t level, uint32_t first_batch, uint32_t growth);
 tdag_status tdag_context_get_batching(const tdag_context* ctx, uint32_t level, uint32_t* first_batch, uint32_t* growth);
 
-/* Size of the Bloom filter in front of the child table, in bits per table entry (0 = default 16).
+/* Size of the Bloom filter in front of the child table, in bits per table entry (0 = the library
+ * default: 16 at level 3, 4 elsewhere).
  * Small filters stay in the L1 cache at the cost of more false positives. A tuning knob with no
  * reference counterpart. */
 tdag_status tdag_context_set_filter_bits(tdag_context* ctx, uint32_t level, uint32_t bits_per_entry);
@@ -139,7 +140,7 @@ tdag_status tdag_context_get_filter_bits(const tdag_context* ctx, uint32_t level
  * receives the average search time per variant (-1 for invalid or unused slots) followed by the
  * tuned schedule's time; also writes the winning variant and first batch. No reference
  * counterpart. */
-#define TDAG_VARIANT_SLOTS 8
+#define TDAG_VARIANT_SLOTS 16
 tdag_status tdag_autotune_level(tdag_context* ctx, uint32_t level, const uint64_t* d_masks, uint64_t n, const tdag_config* cfg, uint32_t repeats,
     float* out_ms, int32_t* out_best, uint32_t* out_first_batch);
 
