@@ -2,8 +2,11 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "errors.hpp"
 #include "matcher.hpp"
@@ -265,6 +268,28 @@ tdag_status tdag_context_get_variant(const tdag_context* ctx, uint32_t level, in
     return TDAG_OK;
 }
 
+namespace {
+
+// Average search time (ms) of `plan` over `repeats` runs, after `warm` untimed runs.
+tdag_status timePlan(tdag_context* ctx, const tdag::MatchPlan& plan, const uint64_t* d_masks, uint32_t cfgBits, uint32_t warm, uint32_t repeats, float* avg)
+{
+    if (tdag_status s = ensureBuffer(ctx->scratch, ctx->scratchBytes, plan.scratchBytes, "matcher scratch allocation"); s != TDAG_OK)
+        return s;
+    float total = 0.f;
+    for (uint32_t rep = 0; rep < warm + repeats; ++rep) {
+        CAPI_HIP(tdag::enqueueMatch(plan, d_masks, cfgBits, ctx->scratch, ctx->stream, nullptr, nullptr, nullptr, ctx->ev[1], ctx->ev[2]), "matcher launch");
+        CAPI_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        float ms = 0.f;
+        CAPI_HIP(hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]), "hipEventElapsedTime");
+        if (rep >= warm)
+            total += ms;
+    }
+    *avg = total / (float)repeats;
+    return TDAG_OK;
+}
+
+} // namespace
+
 tdag_status tdag_autotune_level(tdag_context* ctx, uint32_t level, const uint64_t* d_masks, uint64_t n, const tdag_config* cfg, uint32_t repeats,
     float* out_ms, int32_t* out_best, uint32_t* out_first_batch)
 {
@@ -274,94 +299,74 @@ tdag_status tdag_autotune_level(tdag_context* ctx, uint32_t level, const uint64_
         return fail(TDAG_EINVAL, "d_masks is null");
     CAPI_HIP(hipSetDevice(ctx->device), "hipSetDevice");
     repeats = std::max<uint32_t>(1, repeats);
-    int32_t best = -1;
-    float bestMs = 0.f;
+    const uint32_t cfgBits = cfgBitsOf(cfg);
     // out_ms holds TDAG_VARIANT_SLOTS per-variant times followed by the tuned time.
     constexpr uint32_t kVariantSlots = TDAG_VARIANT_SLOTS;
     static_assert(tdag::kNumVariants <= kVariantSlots, "out_ms has TDAG_VARIANT_SLOTS variant slots");
     if (out_ms)
         for (uint32_t v = 0; v <= kVariantSlots; ++v)
             out_ms[v] = -1.f;
+    // 1. Every valid variant at the default schedule.
+    std::vector<std::pair<float, uint32_t>> byTime;
     for (uint32_t v = 0; v < tdag::kNumVariants; ++v) {
         if (!tdag::variantValid(level, v))
             continue;
-        const tdag::MatchPlan plan = tdag::planMatch(level, n, v);
-        if (tdag_status s = ensureBuffer(ctx->scratch, ctx->scratchBytes, plan.scratchBytes, "matcher scratch allocation"); s != TDAG_OK)
+        float avg = 0.f;
+        if (tdag_status s = timePlan(ctx, tdag::planMatch(level, n, v), d_masks, cfgBits, 1, repeats, &avg); s != TDAG_OK)
             return s;
-        float total = 0.f;
-        for (uint32_t rep = 0; rep <= repeats; ++rep) { // the first run warms caches and is not counted
-            CAPI_HIP(tdag::enqueueMatch(plan, d_masks, cfgBitsOf(cfg), ctx->scratch, ctx->stream, nullptr, nullptr, nullptr, ctx->ev[1], ctx->ev[2]), "matcher launch");
-            CAPI_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-            float ms = 0.f;
-            CAPI_HIP(hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]), "hipEventElapsedTime");
-            if (rep > 0)
-                total += ms;
-        }
-        const float avg = total / repeats;
         if (out_ms)
             out_ms[v] = avg;
-        if (best < 0 || avg < bestMs) {
-            best = (int32_t)v;
-            bestMs = avg;
-        }
+        byTime.emplace_back(avg, v);
     }
-    ctx->variant[level] = best;
-    if (out_best)
-        *out_best = best;
-    // Then the first search batch (parents searched before the first table pruning) for the winner.
-    if (n > 0) {
+    std::sort(byTime.begin(), byTime.end());
+    int32_t best = byTime.empty() ? 0 : (int32_t)byTime[0].second;
+    uint32_t bestFirst = 0;
+    float bestMs = byTime.empty() ? 0.f : byTime[0].first;
+    if (n > 0 && !byTime.empty()) {
+        // 2. The first search batch (parents searched before the first table pruning), jointly
+        //    with the three fastest variants: the best schedule differs between variants.
         const uint32_t cands[4] = { (uint32_t)std::max<uint64_t>(64, n / 256), (uint32_t)std::max<uint64_t>(64, n / 64),
             (uint32_t)std::max<uint64_t>(64, n / 16), (uint32_t)n };
-        float bestBatchMs = 0.f;
-        uint32_t bestFirst = 0;
-        for (uint32_t ci = 0; ci < 4; ++ci) {
-            const tdag::MatchPlan plan = tdag::planMatch(level, n, (uint32_t)best, cands[ci], 2);
-            float total = 0.f;
-            for (uint32_t rep = 0; rep < repeats; ++rep) {
-                CAPI_HIP(tdag::enqueueMatch(plan, d_masks, cfgBitsOf(cfg), ctx->scratch, ctx->stream, nullptr, nullptr, nullptr, ctx->ev[1], ctx->ev[2]), "matcher launch");
-                CAPI_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-                float ms = 0.f;
-                CAPI_HIP(hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]), "hipEventElapsedTime");
-                total += ms;
+        bool first = true;
+        for (size_t vi = 0; vi < std::min<size_t>(3, byTime.size()); ++vi) {
+            for (uint32_t ci = 0; ci < 4; ++ci) {
+                float avg = 0.f;
+                if (tdag_status s = timePlan(ctx, tdag::planMatch(level, n, byTime[vi].second, cands[ci], 2), d_masks, cfgBits, 0, repeats, &avg); s != TDAG_OK)
+                    return s;
+                if (first || avg < bestMs) {
+                    first = false;
+                    bestMs = avg;
+                    best = (int32_t)byTime[vi].second;
+                    bestFirst = cands[ci];
+                }
             }
-            const float avg = total / repeats;
-            if (ci == 0 || avg < bestBatchMs) {
-                bestBatchMs = avg;
-                bestFirst = cands[ci];
+        }
+        // 3. The Bloom filter size (bits per table entry): small filters stay in L1.
+        const uint32_t bitsCands[4] = { 4, 8, 16, 32 };
+        const uint32_t defaultBits = tdag::defaultFilterBits(level);
+        uint32_t bestBits = defaultBits;
+        for (uint32_t bi = 0; bi < 4; ++bi) {
+            if (bitsCands[bi] == defaultBits)
+                continue;
+            float avg = 0.f;
+            if (tdag_status s = timePlan(ctx, tdag::planMatch(level, n, (uint32_t)best, bestFirst, 2, bitsCands[bi]), d_masks, cfgBits, 0, repeats, &avg); s != TDAG_OK)
+                return s;
+            if (avg < bestMs) {
+                bestMs = avg;
+                bestBits = bitsCands[bi];
             }
         }
         ctx->firstBatch[level] = bestFirst;
         ctx->batchGrowth[level] = 2;
-        // Finally the Bloom filter size (bits per table entry): small filters stay in L1.
-        const uint32_t bitsCands[4] = { 4, 8, 16, 32 };
-        uint32_t bestBits = tdag::defaultFilterBits(level);
-        float bestBitsMs = bestBatchMs;
-        for (uint32_t bi = 0; bi < 4; ++bi) {
-            if (bitsCands[bi] == bestBits)
-                continue;
-            const tdag::MatchPlan plan = tdag::planMatch(level, n, (uint32_t)best, bestFirst, 2, bitsCands[bi]);
-            if (tdag_status s = ensureBuffer(ctx->scratch, ctx->scratchBytes, plan.scratchBytes, "matcher scratch allocation"); s != TDAG_OK)
-                return s;
-            float total = 0.f;
-            for (uint32_t rep = 0; rep < repeats; ++rep) {
-                CAPI_HIP(tdag::enqueueMatch(plan, d_masks, cfgBitsOf(cfg), ctx->scratch, ctx->stream, nullptr, nullptr, nullptr, ctx->ev[1], ctx->ev[2]), "matcher launch");
-                CAPI_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-                float ms = 0.f;
-                CAPI_HIP(hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]), "hipEventElapsedTime");
-                total += ms;
-            }
-            const float avg = total / repeats;
-            if (avg < bestBitsMs) {
-                bestBitsMs = avg;
-                bestBits = bitsCands[bi];
-            }
-        }
         ctx->bloomBits[level] = bestBits;
         if (out_ms)
-            out_ms[kVariantSlots] = bestBitsMs;
-        if (out_first_batch)
-            *out_first_batch = bestFirst;
+            out_ms[kVariantSlots] = bestMs;
     }
+    ctx->variant[level] = best;
+    if (out_best)
+        *out_best = best;
+    if (out_first_batch)
+        *out_first_batch = bestFirst;
     return TDAG_OK;
 }
 

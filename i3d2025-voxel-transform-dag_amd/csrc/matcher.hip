@@ -47,6 +47,28 @@ __device__ __forceinline__ bool configAllows(uint32_t cfgBits, uint32_t permCode
 // LDS bytes per workgroup on gfx950.
 constexpr size_t kLdsBytes = 160 * 1024;
 
+// Workgroups of a launch are dealt to the 8 XCDs round-robin (block b runs on XCD b mod 8), and
+// each XCD has its own L2. A launch of `groups` groups of `perGroup` blocks that share data uses
+// a 1-D grid of xcdGridBlocks(groups, perGroup) blocks, decoded so that the k-th block of XCD x
+// is member k mod perGroup of group (k div perGroup) * 8 + x: a group's blocks run back to back
+// on one XCD. Blocks whose group is >= groups (padding to a multiple of 8) exit at once.
+constexpr uint32_t kXcds = 8;
+
+struct XcdBlock {
+    uint32_t group, member;
+};
+
+__device__ __forceinline__ XcdBlock xcdBlock(uint32_t b, uint32_t perGroup)
+{
+    const uint32_t k = b / kXcds;
+    return { (k / perGroup) * kXcds + b % kXcds, k % perGroup };
+}
+
+inline uint32_t xcdGridBlocks(uint32_t groups, uint32_t perGroup)
+{
+    return (groups + kXcds - 1) / kXcds * kXcds * perGroup;
+}
+
 // Child rank stored in a table entry whose child can no longer improve (tombstone; the key stays
 // so linear-probing clusters remain intact). Ranks are < 2^28, so this never names an item.
 constexpr uint32_t kDeadRank = 0x1FFFFFFFu;
@@ -647,11 +669,14 @@ constexpr size_t searchLds()
 //                rows are loaded once per family.
 //   kUnitShared: as kUnitFamily, and passes 1 and 2 run once for both directions of pass 3
 //                (their states stay in a second plane buffer).
-// grid.z = MaxWaves / Waves plane chunks. The autotuner picks the variant per level.
-template <int L, int Waves, int Kind>
+// The grid is one-dimensional. XcdGrouped: decoded by xcdBlock, so all (unit, chunk) blocks of
+// one parent run back to back on one XCD and share its L2 copy of the parent's rows; otherwise
+// the parent varies fastest. There are MaxWaves / Waves plane chunks per unit. The autotuner
+// picks the variant per level.
+template <int L, int Waves, int Kind, bool XcdGrouped>
 __global__ __launch_bounds__(64 * Waves) void kSearch(const uint64_t* __restrict__ layouts, uint32_t n, const uint64_t* __restrict__ table, uint32_t logCap,
     const uint32_t* __restrict__ bloom, uint32_t logBloom, uint64_t* __restrict__ best, uint32_t cfgBits, unsigned long long* __restrict__ counters,
-    uint32_t rankBase)
+    uint32_t rankBase, uint32_t numParents)
 {
     using G = Geo<L>;
     constexpr int R = G::R;
@@ -659,9 +684,23 @@ __global__ __launch_bounds__(64 * Waves) void kSearch(const uint64_t* __restrict
     __shared__ typename G::Row s_rows[3 * G::Rows];
     __shared__ uint32_t s_planes[Buffers][Waves * G::P * G::Slab];
     ProbeCounts cnt;
-    const uint32_t r = rankBase + blockIdx.x;
+    constexpr uint32_t Units = Kind == kUnitCombo ? 48u : 6u;
+    constexpr uint32_t Chunks = (uint32_t)(G::MaxWaves / Waves);
+    uint32_t parent, member;
+    if constexpr (XcdGrouped) {
+        const XcdBlock xb = xcdBlock(blockIdx.x, Units * Chunks);
+        parent = xb.group;
+        member = xb.member;
+    } else { // parent fastest: consecutive blocks run the same (unit, chunk) for consecutive parents
+        parent = blockIdx.x % numParents;
+        member = blockIdx.x / numParents;
+    }
+    if (parent >= numParents)
+        return;
+    const uint32_t r = rankBase + parent;
+    const uint32_t unit = member / Chunks;
+    const uint32_t chunk = member % Chunks;
     const bool axisOn = (cfgBits & kCfgAxis) != 0;
-    const uint32_t unit = blockIdx.y;
     if (!axisOn && (Kind == kUnitCombo ? (unit >> 3) != 0 : (unit >> 1) != 0))
         return; // only permutation code 0 (p0 == 0) is allowed
     if (!loadRows<L, 64 * Waves>(layouts, n, r, s_rows, 3))
@@ -669,7 +708,7 @@ __global__ __launch_bounds__(64 * Waves) void kSearch(const uint64_t* __restrict
     const int lane = (int)(threadIdx.x & 63u);
     const int wave = (int)(threadIdx.x >> 6);
     const int u = lane & (R - 1), w = lane >> L;
-    const int c0 = ((int)blockIdx.z * Waves + wave) * G::P + w;
+    const int c0 = ((int)chunk * Waves + wave) * G::P + w;
     const int planeOff = (wave * G::P + w) * G::Slab;
     uint32_t* plane = s_planes[0] + planeOff;
     uint32_t* line = plane + u; // the lane's pass-3 line: c1 = u, stride Pitch along c2
@@ -806,47 +845,68 @@ __global__ __launch_bounds__(256) void kFinalMapping(const uint64_t* __restrict_
             return e_;                   \
     } while (0)
 
-template <int L, int Waves, int Kind>
+template <int L, int Waves, int Kind, bool XcdGrouped>
 hipError_t launchSearchShape(uint32_t parents, uint32_t r0, uint32_t n, const uint64_t* layouts, const uint64_t* table, uint32_t logCap, const uint32_t* bloom,
     uint32_t logBloom, uint64_t* best, uint32_t cfgBits, unsigned long long* counters, hipStream_t stream)
 {
     if constexpr (Waves >= 1 && Waves <= Geo<L>::MaxWaves && searchLds<L, Waves, Kind>() <= kLdsBytes) {
-        const dim3 grid(parents, Kind == kUnitCombo ? 48 : 6, Geo<L>::MaxWaves / Waves);
-        hipLaunchKernelGGL((kSearch<L, Waves, Kind>), grid, dim3(64 * Waves), 0, stream, layouts, n, table, logCap, bloom, logBloom, best, cfgBits, counters, r0);
+        const uint32_t perParent = (Kind == kUnitCombo ? 48u : 6u) * (uint32_t)(Geo<L>::MaxWaves / Waves);
+        const dim3 grid(XcdGrouped ? xcdGridBlocks(parents, perParent) : parents * perParent);
+        hipLaunchKernelGGL((kSearch<L, Waves, Kind, XcdGrouped>), grid, dim3(64 * Waves), 0, stream, layouts, n, table, logCap, bloom, logBloom, best, cfgBits,
+            counters, r0, parents);
         return hipGetLastError();
     } else {
         return hipErrorInvalidValue;
     }
 }
 
-// Variant v: unit kind v % 3, plane-chunk split 2^(v / 3) (a block covers MaxWaves >> (v / 3)
-// waves' worth of planes).
+// Variant v: unit kind v % 3, plane-chunk split 2^((v / 3) % 3) (a block covers
+// MaxWaves >> ((v / 3) % 3) waves' worth of planes), block order v / 9 (0: parent fastest,
+// 1: XCD-grouped).
 template <int L>
 hipError_t launchSearch(uint32_t variant, uint32_t n, uint32_t r0, uint32_t r1, const uint64_t* layouts, const uint64_t* table, uint32_t logCap, const uint32_t* bloom,
     uint32_t logBloom, uint64_t* best, uint32_t cfgBits, unsigned long long* counters, hipStream_t stream)
 {
     constexpr int M = Geo<L>::MaxWaves;
     const uint32_t p = r1 - r0;
-#define TDAG_SHAPE(W, K) launchSearchShape<L, W, K>(p, r0, n, layouts, table, logCap, bloom, logBloom, best, cfgBits, counters, stream)
+#define TDAG_SHAPE(W, K, X) launchSearchShape<L, W, K, X>(p, r0, n, layouts, table, logCap, bloom, logBloom, best, cfgBits, counters, stream)
     switch (variant) {
     case 0:
-        return TDAG_SHAPE(M, kUnitCombo);
+        return TDAG_SHAPE(M, kUnitCombo, false);
     case 1:
-        return TDAG_SHAPE(M, kUnitFamily);
+        return TDAG_SHAPE(M, kUnitFamily, false);
     case 2:
-        return TDAG_SHAPE(M, kUnitShared);
+        return TDAG_SHAPE(M, kUnitShared, false);
     case 3:
-        return TDAG_SHAPE(M / 2, kUnitCombo);
+        return TDAG_SHAPE(M / 2, kUnitCombo, false);
     case 4:
-        return TDAG_SHAPE(M / 2, kUnitFamily);
+        return TDAG_SHAPE(M / 2, kUnitFamily, false);
     case 5:
-        return TDAG_SHAPE(M / 2, kUnitShared);
+        return TDAG_SHAPE(M / 2, kUnitShared, false);
     case 6:
-        return TDAG_SHAPE(M / 4, kUnitCombo);
+        return TDAG_SHAPE(M / 4, kUnitCombo, false);
     case 7:
-        return TDAG_SHAPE(M / 4, kUnitFamily);
+        return TDAG_SHAPE(M / 4, kUnitFamily, false);
     case 8:
-        return TDAG_SHAPE(M / 4, kUnitShared);
+        return TDAG_SHAPE(M / 4, kUnitShared, false);
+    case 9:
+        return TDAG_SHAPE(M, kUnitCombo, true);
+    case 10:
+        return TDAG_SHAPE(M, kUnitFamily, true);
+    case 11:
+        return TDAG_SHAPE(M, kUnitShared, true);
+    case 12:
+        return TDAG_SHAPE(M / 2, kUnitCombo, true);
+    case 13:
+        return TDAG_SHAPE(M / 2, kUnitFamily, true);
+    case 14:
+        return TDAG_SHAPE(M / 2, kUnitShared, true);
+    case 15:
+        return TDAG_SHAPE(M / 4, kUnitCombo, true);
+    case 16:
+        return TDAG_SHAPE(M / 4, kUnitFamily, true);
+    case 17:
+        return TDAG_SHAPE(M / 4, kUnitShared, true);
     default:
         return hipErrorInvalidValue;
     }
@@ -871,7 +931,7 @@ bool variantValid(uint32_t level, uint32_t variant)
         return false;
     const uint32_t R = 1u << level;
     const uint32_t maxWaves = R * R / 64u;
-    const uint32_t waves = maxWaves >> (variant / 3);
+    const uint32_t waves = maxWaves >> ((variant / 3) % 3);
     if (waves < 1u)
         return false;
     const uint32_t rowBytes = R / 8u;
@@ -883,10 +943,11 @@ uint32_t defaultVariant(uint32_t level)
 {
     switch (level) {
     case 5:
-        return 4; // family units, half the planes per block (two blocks per CU)
+        return 16; // family units, a quarter of the planes per block, XCD-grouped
     case 4:
+        return 9; // combo units, XCD-grouped
     case 3:
-        return 2; // family units sharing passes 1 and 2, all planes in one block
+        return 2; // family units sharing passes 1 and 2, parent fastest
     default:
         return 0;
     }

@@ -20,10 +20,11 @@ constexpr int kNumCounters = 8;
 
 // Search-kernel variants for levels 3..5 (the autotuner's tile shapes). variant % 3: a block
 // runs one (permutation, direction) combo (0), a family of 8 combos (1), or a family of 8 combos
-// that shares passes 1 and 2 between the two pass-3 directions (2). variant / 3: log2 of the
-// split of the R planes of a unit over blocks (0, 1, 2). Variants whose blocks exceed the LDS
+// that shares passes 1 and 2 between the two pass-3 directions (2). (variant / 3) % 3: log2 of
+// the split of the R planes of a unit over blocks (0, 1, 2). variant / 9: block order (0: parent
+// fastest, 1: all blocks of a parent together on one XCD). Variants whose blocks exceed the LDS
 // are invalid.
-constexpr uint32_t kNumVariants = 9;
+constexpr uint32_t kNumVariants = 18;
 
 bool variantValid(uint32_t level, uint32_t variant);
 uint32_t defaultVariant(uint32_t level);

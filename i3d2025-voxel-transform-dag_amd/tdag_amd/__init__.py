@@ -26,7 +26,7 @@ _u64p = ctypes.POINTER(ctypes.c_uint64)
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 
 TDAG_OK = 0
-VARIANT_SLOTS = 16  # TDAG_VARIANT_SLOTS in include/tdag.h
+VARIANT_SLOTS = 32  # TDAG_VARIANT_SLOTS in include/tdag.h
 _STATUS = {0: "OK", 1: "EINVAL", 2: "ENOMEM", 3: "EDEVICE", 4: "EINTERNAL"}
 
 
@@ -218,8 +218,9 @@ class Context:
         return int(v.value)
 
     def autotune(self, level: int, d_masks, n: int, config: TransformDAGConfig | None = None, repeats: int = 2):
-        """Times every valid search variant, then four first-batch sizes, on device masks and
-        keeps the fastest; returns (best_variant, first_batch, {variant: ms}, tuned_ms)."""
+        """Times every valid search variant, then four first-batch sizes with each of the three
+        fastest variants, then four filter sizes, on device masks, and keeps the fastest for this
+        context; returns (best_variant, first_batch, {variant: ms}, tuned_ms)."""
         cfg = (config or TransformDAGConfig()).to_c()
         ms = (ctypes.c_float * (VARIANT_SLOTS + 1))()
         best = ctypes.c_int32()

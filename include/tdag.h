@@ -109,11 +109,13 @@ tdag_status tdag_last_stats(const tdag_context* ctx, tdag_match_stats* out);
  * (diagnostics; costs a few percent). No reference counterpart. */
 tdag_status tdag_context_set_counters(tdag_context* ctx, int enable);
 
-/* Search-kernel variant used at `level` (3..5) by this context. Bit 0: a block runs one
- * (permutation, direction) combo (0) or a family of 8 combos that share the first two hash
- * passes (1). variant >> 1 (0, 1, 2): log2 of how many blocks share the R planes of one unit
- * (more blocks = less LDS per block = more blocks per CU). Level 3 has variants 0 and 1, levels
- * 4 and 5 have 0..5. -1 restores the library default.
+/* Search-kernel variant used at `level` (3..5) by this context. variant % 3: a block runs one
+ * (permutation, direction) combo (0), a family of 8 combos (1), or a family of 8 combos sharing
+ * the first two hash passes between both directions of the third (2). (variant / 3) % 3: log2
+ * of how many blocks share the R planes of one unit (more blocks = less LDS per block = more
+ * blocks per CU). variant / 9: block order (0: parent fastest; 1: all blocks of a parent together
+ * on one XCD, sharing its L2). Variants whose blocks exceed the LDS are rejected with EINVAL.
+ * -1 restores the library default.
  * Level 2 has a single kernel (variant 0). No reference counterpart: a tuning knob. */
 tdag_status tdag_context_set_variant(tdag_context* ctx, uint32_t level, int32_t variant);
 tdag_status tdag_context_get_variant(const tdag_context* ctx, uint32_t level, int32_t* variant);
@@ -133,14 +135,16 @@ tdag_status tdag_context_get_batching(const tdag_context* ctx, uint32_t level, u
 tdag_status tdag_context_set_filter_bits(tdag_context* ctx, uint32_t level, uint32_t bits_per_entry);
 tdag_status tdag_context_get_filter_bits(const tdag_context* ctx, uint32_t level, uint32_t* bits_per_entry);
 
-/* Autotuner: runs the level-`level` matcher once per valid variant on the device masks (plus
- * one untimed warm-up run each) and keeps the fastest search kernel for this context, then tries
- * four first-batch sizes (n/256, n/64, n/16, n) with it and keeps the fastest, then filter sizes
- * of 4, 8, 16 and 32 bits per table entry. out_ms (TDAG_VARIANT_SLOTS + 1 floats, may be NULL)
- * receives the average search time per variant (-1 for invalid or unused slots) followed by the
- * tuned schedule's time; also writes the winning variant and first batch. No reference
- * counterpart. */
-#define TDAG_VARIANT_SLOTS 16
+/* Number of per-variant slots in the autotuner's timing output. */
+#define TDAG_VARIANT_SLOTS 32
+
+/* Autotuner: runs the level-`level` matcher on the device masks once per valid variant at the
+ * default schedule (after one untimed warm-up run each), then tries four first-batch sizes
+ * (n/256, n/64, n/16, n) with each of the three fastest variants and keeps the fastest pair,
+ * then filter sizes of 4, 8, 16 and 32 bits per table entry; the winners become this context's
+ * settings for `level`. out_ms (TDAG_VARIANT_SLOTS + 1 floats, may be NULL) receives the average
+ * search time per variant at the default schedule (-1 for invalid or unused slots) followed by
+ * the tuned time; also writes the winning variant and first batch. No reference counterpart. */
 tdag_status tdag_autotune_level(tdag_context* ctx, uint32_t level, const uint64_t* d_masks, uint64_t n, const tdag_config* cfg, uint32_t repeats,
     float* out_ms, int32_t* out_best, uint32_t* out_first_batch);
 

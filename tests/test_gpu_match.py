@@ -68,9 +68,9 @@ def test_larger_sets(oracle, gpu_ctx, level, n):
     _check(oracle, gpu_ctx, level, masks)
 
 
-# Search-kernel variants per level: unit kind (combo / family / shared passes) x plane split;
-# variants whose blocks do not fit the LDS are rejected by the library.
-VALID_VARIANTS = {3: [0, 1, 2], 4: list(range(9)), 5: [0, 1, 3, 4, 5, 6, 7, 8]}
+# Search-kernel variants per level: unit kind (combo / family / shared passes) x plane split x
+# block order; variants whose blocks do not fit the LDS are rejected by the library.
+VALID_VARIANTS = {3: [0, 1, 2, 9, 10, 11], 4: list(range(18)), 5: [0, 1, 3, 4, 5, 6, 7, 8, 9, 10, 12, 13, 14, 15, 16, 17]}
 
 
 @pytest.mark.parametrize("level,n_base,variant", [(3, 60, v) for v in VALID_VARIANTS[3]] + [(4, 20, v) for v in VALID_VARIANTS[4]]
