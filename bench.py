@@ -97,6 +97,53 @@ def cpu_baseline(levels, cfg, budget_s: float, threads: int):
     }
 
 
+def bench_batched(ctx, cfg, dev, order, args):
+    """The batched form on the same level inputs: every level's parent search in one persistent
+    kernel, the whole multi-level call captured once as a hipGraph and replayed per step. Its
+    outputs are compared with the per-level launches' on the device (bit-exact)."""
+    import torch
+
+    jobs, outs = [], {}
+    for level in order:
+        d = dev[level]
+        n = d["n"]
+        outs[level] = {"enc": torch.empty(max(1, n), dtype=torch.int64, device="cuda"),
+                       "map": torch.empty(max(1, n), dtype=torch.int64, device="cuda")}
+        jobs.append({"level": level, "n": n, "d_masks": d["masks"], "d_encoded": outs[level]["enc"], "d_mapping": outs[level]["map"],
+                     "d_unique": torch.empty(max(1, n), dtype=torch.int32, device="cuda")})
+    torch.cuda.synchronize()
+    b = ctx.batch(jobs, cfg, use_graph=True)
+    try:
+        grab = b.autotune(1)
+        for _ in range(max(1, args.warmup)):
+            b.run()
+        t0 = time.perf_counter()
+        matches = 0
+        for _ in range(args.steps):
+            stats, _ = b.run()
+            matches += sum(int(s.num_matched) for s in stats)
+        elapsed = time.perf_counter() - t0
+    finally:
+        b.close()
+    # Bit-exactness against the per-level launches on the full-size inputs.
+    exact = True
+    for level in order:
+        d = dev[level]
+        n = d["n"]
+        ref = torch.empty(max(1, n), dtype=torch.int64, device="cuda")
+        ref_map = torch.empty(max(1, n), dtype=torch.int64, device="cuda")
+        ctx.match_level_device(level, d["masks"], n, cfg, d_encoded=ref, d_mapping=ref_map, d_unique=d["unique"])
+        torch.cuda.synchronize()
+        exact = exact and bool(torch.equal(ref[:n], outs[level]["enc"][:n])) and bool(torch.equal(ref_map[:n], outs[level]["map"][:n]))
+    return {
+        "form": "one persistent kernel for all levels (device work queue, phases = popcount buckets of parents), hipGraph replay",
+        "ms_per_step": round(elapsed * 1e3 / max(1, args.steps), 4),
+        "matches_per_s": matches / elapsed if elapsed > 0 else None,
+        "tiles_per_pop": grab,
+        "bit_exact_vs_per_level": exact,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +157,7 @@ def main():
     ap.add_argument("--levels", default="", help="comma-separated subset of levels to time (default: all)")
     ap.add_argument("--no-autotune", action="store_true", help="use the library's default search variants instead of tuning them first")
     ap.add_argument("--counters", action="store_true", help="also run each level once with per-stage probe counters (diagnostics, untimed)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched form (one persistent kernel for all levels, replayed hipGraph)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB three times (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -191,6 +239,8 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
 
+    batched = None if args.no_batched else bench_batched(ctx, cfg, dev, order, args)
+
     per_step_matches = matches // max(1, args.steps)
     expected = sum(levels[l]["matched"] for l in order)
     if per_step_matches != expected:
@@ -241,6 +291,7 @@ def main():
             "search_variants": {l: ctx.get_variant(l) for l in order if l >= 3},
             "autotune": tuning,
             **({"probe_counters": counters} if counters else {}),
+            **({"batched": batched} if batched else {}),
             "parallelism": "single GPU",
         },
         "roofline": {

@@ -1,0 +1,341 @@
+// Batched form of the exhaustive matcher on gfx950: the parent searches of several levels run in
+// one persistent kernel fed by a device work queue.
+//
+// The grid is sized so that every block is resident (blocks per CU from the occupancy API times
+// the CU count) and loops: it pops `grab` consecutive tile indices from the current phase's queue
+// (one atomic per block), runs the tiles, and pops again until the phase is drained. Phase p holds batch p of every
+// level's rank-batched search (ranks are popcount-descending, so a batch is a popcount bucket of
+// parents; bucket widths come from the per-level schedule the autotuner picks). Between phases a
+// grid barrier separates the search from the table pruning and filter rebuild of every level,
+// done by all blocks together. Tiles are ordered heaviest level first inside a phase.
+//
+// Tile shapes (four waves per block):
+//   level 5: one family of 8 combos on 8 of the 32 planes            (24 tiles per parent)
+//   level 4: one (permutation, direction) combo on all 16 planes      (48 tiles per parent)
+//   level 3: four wave items, each a family sharing passes 1 and 2 on all 8 planes (6 per parent)
+//   level 2: four parents, one per wave
+// Results are bit-identical to the per-level launches: the search computes the same exhaustive
+// minimum, whatever the schedule.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "batched.hpp"
+#include "common.hpp"
+#include "primitives.hpp"
+#include "search_device.hpp"
+
+namespace tdag {
+
+namespace {
+
+using namespace dev;
+
+constexpr int kBatchThreads = 256;
+constexpr int kBatchWaves = kBatchThreads / 64;
+
+template <int L>
+constexpr size_t rowsBytes()
+{
+    return 3 * Geo<L>::Rows * sizeof(typename Geo<L>::Row);
+}
+
+constexpr size_t alignUp16(size_t v) { return (v + 15) / 16 * 16; }
+
+// Per-wave LDS of a level-3 wave item: rows, then two plane buffers.
+constexpr size_t kL3RowsBytes = alignUp16(rowsBytes<3>());
+constexpr size_t kL3WaveBytes = kL3RowsBytes + 2 * (size_t)Geo<3>::P * Geo<3>::Slab * 4;
+
+constexpr size_t kL5Bytes = alignUp16(rowsBytes<5>()) + (size_t)kBatchWaves * Geo<5>::P * Geo<5>::Slab * 4;
+constexpr size_t kL4Bytes = alignUp16(rowsBytes<4>()) + (size_t)kBatchWaves * Geo<4>::P * Geo<4>::Slab * 4;
+constexpr size_t kL3Bytes = kBatchWaves * kL3WaveBytes;
+constexpr size_t kBatchLds = std::max(kL5Bytes, std::max(kL4Bytes, kL3Bytes));
+static_assert(kBatchLds <= kLdsBytes, "batched tile LDS");
+
+constexpr uint32_t kL5Chunks = Geo<5>::MaxWaves / kBatchWaves; // plane chunks of a level-5 tile
+static_assert(Geo<4>::MaxWaves == kBatchWaves, "a level-4 tile covers all planes");
+static_assert(Geo<3>::MaxWaves == 1, "a level-3 wave item covers all planes");
+
+uint32_t tilesFor(uint32_t level, uint32_t parents)
+{
+    switch (level) {
+    case 5:
+        return parents * 6u * kL5Chunks;
+    case 4:
+        return parents * 48u;
+    case 3:
+        return (parents * 6u + kBatchWaves - 1) / kBatchWaves;
+    default:
+        return (parents + kBatchWaves - 1) / kBatchWaves;
+    }
+}
+
+// Block-level tile of levels 4 and 5: all four waves search one unit of parent r.
+template <int L, int Kind>
+__device__ __forceinline__ void blockTile(const LevelView& v, uint32_t r, uint32_t unit, uint32_t chunk, unsigned char* s_mem, uint32_t cfgBits)
+{
+    using G = Geo<L>;
+    if (!(cfgBits & kCfgAxis) && (Kind == kUnitCombo ? (unit >> 3) : (unit >> 1)) != 0)
+        return; // only permutation code 0 (p0 == 0) is allowed
+    auto* s_rows = reinterpret_cast<typename G::Row*>(s_mem);
+    uint32_t* planes = reinterpret_cast<uint32_t*>(s_mem + alignUp16(rowsBytes<L>()));
+    if (!loadRows<L, kBatchThreads>(v.sortedMasks, v.n, r, s_rows, 3))
+        return;
+    ProbeCounts cnt;
+    searchUnit<L, kBatchWaves, Kind>(r, unit, chunk, (int)(threadIdx.x >> 6), s_rows, planes, planes, v.sortedMasks, v.tabKeys, v.logCap, v.bloom, v.logBloom, v.best,
+        cfgBits, cnt);
+    flushCounts(cnt, v.counters, cfgBits);
+}
+
+// Wave item of level 3: the calling wave alone searches family `unit` of parent r.
+__device__ __forceinline__ void waveItem3(const LevelView& v, uint32_t r, uint32_t unit, unsigned char* s_mem, uint32_t cfgBits)
+{
+    using G = Geo<3>;
+    if (!(cfgBits & kCfgAxis) && (unit >> 1) != 0)
+        return;
+    unsigned char* base = s_mem + (threadIdx.x >> 6) * kL3WaveBytes;
+    auto* s_rows = reinterpret_cast<G::Row*>(base);
+    uint32_t* planes0 = reinterpret_cast<uint32_t*>(base + kL3RowsBytes);
+    uint32_t* planes1 = planes0 + G::P * G::Slab;
+    if (!loadRowsWave<3>(v.sortedMasks, v.n, r, s_rows, 3))
+        return;
+    ProbeCounts cnt;
+    searchUnit<3, 1, kUnitShared>(r, unit, 0, 0, s_rows, planes0, planes1, v.sortedMasks, v.tabKeys, v.logCap, v.bloom, v.logBloom, v.best, cfgBits, cnt);
+    flushCounts(cnt, v.counters, cfgBits);
+}
+
+// Grid-wide barrier of the persistent kernel (all blocks are resident). Thread 0 of each block
+// arrives with an agent-scope release and waits for the generation to move with an acquire, so
+// table, filter and best[] writes made before the barrier are visible after it on every XCD.
+// A wait longer than `spinTicks` wall-clock ticks sets sync->error and gives up (the results
+// are then flagged as invalid by the host) instead of hanging; once the flag is set no block
+// waits again.
+__device__ __forceinline__ void gridBarrier(BatchSync* sync, uint64_t spinTicks)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t gen = __hip_atomic_load(&sync->generation, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        const uint32_t arrived = __hip_atomic_fetch_add(&sync->arrive, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+        if (arrived == gridDim.x) {
+            __hip_atomic_store(&sync->arrive, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&sync->generation, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            const uint64_t t0 = (uint64_t)wall_clock64();
+            while (__hip_atomic_load(&sync->generation, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+                __builtin_amdgcn_s_sleep(4);
+                if (__hip_atomic_load(&sync->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
+                    break; // another block already gave up: do not wait again
+                if ((uint64_t)wall_clock64() - t0 > spinTicks) {
+                    __hip_atomic_store(&sync->error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+            }
+        }
+        __threadfence();
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kBatchThreads, 3) void kBatched(const BatchDesc* __restrict__ desc, BatchSync* __restrict__ sync, uint64_t spinTicks)
+{
+    __shared__ __align__(16) unsigned char s_mem[kBatchLds];
+    __shared__ uint32_t s_tile;
+    const uint32_t numLevels = desc->numLevels;
+    const uint32_t numPhases = desc->numPhases;
+    const uint32_t cfgBits = desc->cfgBits;
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint64_t gtid = (uint64_t)blockIdx.x * kBatchThreads + threadIdx.x;
+    const uint64_t gstride = (uint64_t)gridDim.x * kBatchThreads;
+    for (uint32_t p = 0; p < numPhases; ++p) {
+        const uint32_t total = desc->totalTiles[p];
+        const uint32_t grab = desc->grab;
+        for (;;) {
+            if (threadIdx.x == 0)
+                s_tile = atomicAdd(&sync->queue[p], grab);
+            __syncthreads();
+            const uint32_t t0 = s_tile;
+            __syncthreads();
+            if (t0 >= total)
+                break;
+            const uint32_t tEnd = min(total, t0 + grab);
+            for (uint32_t t = t0; t < tEnd; ++t) {
+                if (t != t0)
+                    __syncthreads(); // the previous tile's waves are done with the LDS
+                uint32_t li = 0;
+                while (li + 1 < numLevels && t >= desc->ph[p][li + 1].tileBase)
+                    ++li;
+                const BatchPhaseLevel& pl = desc->ph[p][li];
+                const LevelView& v = desc->lv[li];
+                const uint32_t k = t - pl.tileBase;
+                switch (v.level) {
+                case 5: {
+                    const uint32_t m = k % (6u * kL5Chunks);
+                    blockTile<5, kUnitFamily>(v, pl.r0 + k / (6u * kL5Chunks), m / kL5Chunks, m % kL5Chunks, s_mem, cfgBits);
+                    break;
+                }
+                case 4:
+                    blockTile<4, kUnitCombo>(v, pl.r0 + k / 48u, k % 48u, 0, s_mem, cfgBits);
+                    break;
+                case 3: {
+                    const uint32_t item = k * kBatchWaves + wave;
+                    if (item < (pl.r1 - pl.r0) * 6u)
+                        waveItem3(v, pl.r0 + item / 6u, item % 6u, s_mem, cfgBits);
+                    break;
+                }
+                default: {
+                    const uint32_t r = pl.r0 + k * kBatchWaves + wave;
+                    if (r < pl.r1) {
+                        ProbeCounts cnt;
+                        search2Parent(r, v.sortedMasks, v.tabKeys, v.tabVals, v.logCap, v.bloom, v.logBloom, v.best, cfgBits, cnt);
+                        flushCounts(cnt, v.counters, cfgBits);
+                    }
+                    break;
+                }
+                }
+            }
+        }
+        if (p + 1 == numPhases)
+            break;
+        // Between phases: prune every level that has further batches (rank-batched search).
+        gridBarrier(sync, spinTicks);
+        for (uint32_t li = 0; li < numLevels; ++li)
+            if (desc->ph[p][li].prune) {
+                const LevelView& v = desc->lv[li];
+                const uint64_t words = 1ull << v.logBloom;
+                for (uint64_t i = gtid; i < words; i += gstride)
+                    v.bloom[i] = 0u;
+            }
+        gridBarrier(sync, spinTicks);
+        for (uint32_t li = 0; li < numLevels; ++li)
+            if (desc->ph[p][li].prune) {
+                const LevelView& v = desc->lv[li];
+                const uint64_t cap = 1ull << v.logCap;
+                if (v.level == 2)
+                    pruneSplit(v.tabKeys, v.tabVals, cap, v.best, desc->ph[p][li].pruneKey, v.bloom, v.logBloom, gtid, gstride);
+                else
+                    prunePacked(v.tabKeys, cap, v.best, desc->ph[p][li].pruneKey, v.bloom, v.logBloom, gtid, gstride);
+            }
+        gridBarrier(sync, spinTicks);
+    }
+}
+
+inline size_t alignUp(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+} // namespace
+
+hipError_t planBatch(uint32_t count, const uint32_t* levels, const uint64_t* ns, const uint32_t* firstBatch, const uint32_t* bloomBits, int device, BatchPlan* out)
+{
+    if (count == 0 || count > kBatchMaxLevels)
+        return hipErrorInvalidValue;
+    BatchPlan b;
+    b.count = count;
+    // Heaviest level first: its tiles lead every phase's queue.
+    std::vector<uint32_t> idx(count);
+    for (uint32_t i = 0; i < count; ++i)
+        idx[i] = i;
+    std::sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t c) { return levels[a] > levels[c]; });
+    for (uint32_t i = 0; i + 1 < count; ++i)
+        if (levels[idx[i]] == levels[idx[i + 1]])
+            return hipErrorInvalidValue; // one job per level
+    size_t off = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        const uint32_t j = idx[i];
+        if (levels[j] < 2 || levels[j] > 5 || ns[j] >= (1ull << 28))
+            return hipErrorInvalidValue;
+        b.job[i] = j;
+        b.plans[i] = planMatch(levels[j], ns[j], 0, firstBatch ? firstBatch[j] : 0, 2, bloomBits ? bloomBits[j] : 0);
+        b.scratchOff[i] = off;
+        off += alignUp(b.plans[i].scratchBytes, 256);
+    }
+    b.descOff = off;
+    off += alignUp(sizeof(BatchDesc), 256);
+    b.syncOff = off;
+    off += alignUp(sizeof(BatchSync), 256);
+    b.scratchBytes = off;
+    uint32_t phases = 0;
+    for (uint32_t i = 0; i < count; ++i)
+        phases = std::max<uint32_t>(phases, (uint32_t)batchEnds(b.plans[i]).size());
+    if (phases > kBatchMaxPhases)
+        return hipErrorInvalidValue;
+    b.numPhases = std::max<uint32_t>(phases, 1u);
+    int blocksPerCu = 0, cus = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocksPerCu, kBatched, kBatchThreads, 0);
+    if (e != hipSuccess)
+        return e;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    if (e != hipSuccess)
+        return e;
+    if (blocksPerCu < 1 || cus < 1)
+        return hipErrorInvalidConfiguration;
+    b.gridBlocks = (uint32_t)(blocksPerCu * cus);
+    *out = b;
+    return hipSuccess;
+}
+
+void fillBatchDesc(const BatchPlan& b, void* d_scratch, uint32_t cfgBits, uint32_t grab, BatchDesc* desc)
+{
+    *desc = BatchDesc {};
+    desc->grab = std::max<uint32_t>(1u, grab);
+    desc->numLevels = b.count;
+    desc->numPhases = b.numPhases;
+    desc->cfgBits = cfgBits;
+    std::vector<std::vector<uint32_t>> ends(b.count);
+    for (uint32_t i = 0; i < b.count; ++i) {
+        desc->lv[i] = levelView(b.plans[i], (char*)d_scratch + b.scratchOff[i]);
+        ends[i] = batchEnds(b.plans[i]);
+    }
+    for (uint32_t p = 0; p < b.numPhases; ++p) {
+        uint32_t base = 0;
+        for (uint32_t i = 0; i < b.count; ++i) {
+            BatchPhaseLevel& pl = desc->ph[p][i];
+            pl.tileBase = base;
+            if (p < ends[i].size()) {
+                pl.r0 = p == 0 ? 0u : ends[i][p - 1];
+                pl.r1 = ends[i][p];
+                pl.tiles = tilesFor(b.plans[i].level, pl.r1 - pl.r0);
+                pl.prune = pl.r1 < (uint32_t)b.plans[i].n ? 1u : 0u;
+                pl.pruneKey = pruneKey(b.plans[i].level, pl.r1);
+            }
+            base += pl.tiles;
+        }
+        desc->totalTiles[p] = base;
+    }
+}
+
+hipError_t enqueueBatch(const BatchPlan& b, const uint64_t* const* d_masks, void* d_scratch, hipStream_t stream, uint64_t* const* d_encoded,
+    uint64_t* const* d_mapping, uint32_t* const* d_unique, hipEvent_t evSearch0, hipEvent_t evSearch1, uint64_t spinTicks)
+{
+    char* base = (char*)d_scratch;
+    for (uint32_t i = 0; i < b.count; ++i) {
+        const hipError_t e = enqueuePrepare(b.plans[i], d_masks[b.job[i]], base + b.scratchOff[i], stream);
+        if (e != hipSuccess)
+            return e;
+    }
+    hipError_t e = fillU32((uint32_t*)(base + b.syncOff), sizeof(BatchSync) / 4, 0u, stream);
+    if (e != hipSuccess)
+        return e;
+    if (evSearch0 && (e = hipEventRecord(evSearch0, stream)) != hipSuccess)
+        return e;
+    bool any = false;
+    for (uint32_t i = 0; i < b.count; ++i)
+        any = any || b.plans[i].n > 0;
+    if (any) {
+        hipLaunchKernelGGL(kBatched, dim3(b.gridBlocks), dim3(kBatchThreads), 0, stream, (const BatchDesc*)(base + b.descOff), (BatchSync*)(base + b.syncOff), spinTicks);
+        if ((e = hipGetLastError()) != hipSuccess)
+            return e;
+    }
+    if (evSearch1 && (e = hipEventRecord(evSearch1, stream)) != hipSuccess)
+        return e;
+    for (uint32_t i = 0; i < b.count; ++i) {
+        const uint32_t j = b.job[i];
+        e = enqueueFinish(b.plans[i], base + b.scratchOff[i], stream, d_encoded ? d_encoded[j] : nullptr, d_mapping ? d_mapping[j] : nullptr,
+            d_unique ? d_unique[j] : nullptr);
+        if (e != hipSuccess)
+            return e;
+    }
+    return hipSuccess;
+}
+
+} // namespace tdag

@@ -3,12 +3,14 @@
 
 #include <cstdio>
 #include <algorithm>
+#include <cstddef>
 #include <cstring>
 #include <string>
 #include <utility>
 #include <vector>
 
 #include "errors.hpp"
+#include "batched.hpp"
 #include "matcher.hpp"
 #include "tdag.h"
 
@@ -87,6 +89,21 @@ tdag_status checkArgs(tdag_context* ctx, uint32_t level, uint64_t n, const tdag_
 }
 
 } // namespace
+
+// A batched matcher call: per-level plans, its own device scratch, the uploaded BatchDesc and,
+// when requested, the captured hipGraph of the whole call.
+struct tdag_batch {
+    tdag_context* ctx = nullptr;
+    tdag::BatchPlan plan {};
+    std::vector<tdag_level_job> jobs;
+    void* scratch = nullptr;
+    uint64_t* pinned = nullptr; // counters of every level, then the barrier error word
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t spinTicks = 0;
+    uint32_t cfgBits = 0;
+    uint32_t grab = 1;
+};
 
 namespace tdag {
 tdag_status setError(tdag_status status, const std::string& message)
@@ -414,6 +431,209 @@ tdag_status tdag_last_stats(const tdag_context* ctx, tdag_match_stats* out)
         return fail(TDAG_EINVAL, "null argument");
     *out = ctx->last;
     return TDAG_OK;
+}
+
+
+} // extern "C"
+
+namespace {
+
+tdag_status uploadBatchDesc(tdag_batch* b)
+{
+    tdag::BatchDesc desc;
+    tdag::fillBatchDesc(b->plan, b->scratch, b->cfgBits, b->grab, &desc);
+    CAPI_HIP(hipMemcpy((char*)b->scratch + b->plan.descOff, &desc, sizeof(desc), hipMemcpyHostToDevice), "batch descriptor upload");
+    return TDAG_OK;
+}
+
+tdag_status enqueueBatchCall(tdag_batch* b)
+{
+    const uint32_t k = b->plan.count;
+    std::vector<const uint64_t*> masks(k);
+    std::vector<uint64_t*> enc(k), map(k);
+    std::vector<uint32_t*> uniq(k);
+    for (uint32_t i = 0; i < k; ++i) {
+        masks[i] = b->jobs[i].d_masks;
+        enc[i] = b->jobs[i].d_encoded;
+        map[i] = b->jobs[i].d_mapping;
+        uniq[i] = b->jobs[i].d_unique;
+    }
+    CAPI_HIP(tdag::enqueueBatch(b->plan, masks.data(), b->scratch, b->ctx->stream, enc.data(), map.data(), uniq.data(), nullptr, nullptr, b->spinTicks),
+        "batched matcher launch");
+    return TDAG_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+tdag_status tdag_batch_create(tdag_context* ctx, const tdag_level_job* jobs, uint32_t count, const tdag_config* cfg, int use_graph, tdag_batch** out)
+{
+    if (!ctx || !jobs || !cfg || !out)
+        return fail(TDAG_EINVAL, "null argument");
+    if (count == 0 || count > tdag::kBatchMaxLevels)
+        return fail(TDAG_EINVAL, "a batch holds 1 to 4 levels");
+    uint32_t levels[tdag::kBatchMaxLevels], first[tdag::kBatchMaxLevels], bits[tdag::kBatchMaxLevels];
+    uint64_t ns[tdag::kBatchMaxLevels];
+    for (uint32_t i = 0; i < count; ++i) {
+        if (tdag_status s = checkArgs(ctx, jobs[i].level, jobs[i].n, cfg); s != TDAG_OK)
+            return s;
+        if (jobs[i].n > 0 && !jobs[i].d_masks)
+            return fail(TDAG_EINVAL, "d_masks is null");
+        for (uint32_t j = 0; j < i; ++j)
+            if (jobs[j].level == jobs[i].level)
+                return fail(TDAG_EINVAL, "a batch holds at most one job per level");
+        levels[i] = jobs[i].level;
+        ns[i] = jobs[i].n;
+        first[i] = ctx->firstBatch[jobs[i].level];
+        bits[i] = ctx->bloomBits[jobs[i].level];
+    }
+    CAPI_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    auto* b = new tdag_batch();
+    b->ctx = ctx;
+    b->jobs.assign(jobs, jobs + count);
+    auto cleanup = [&](tdag_status s) {
+        tdag_batch_destroy(b);
+        return s;
+    };
+    if (hipError_t e = tdag::planBatch(count, levels, ns, first, bits, ctx->device, &b->plan); e != hipSuccess)
+        return cleanup(failHip(e, "batched matcher plan"));
+    if (hipError_t e = hipMalloc(&b->scratch, b->plan.scratchBytes); e != hipSuccess)
+        return cleanup(failHip(e, "batched matcher scratch allocation"));
+    if (hipError_t e = hipHostMalloc((void**)&b->pinned, 8 * (tdag::kNumCounters * count + 1)); e != hipSuccess)
+        return cleanup(failHip(e, "pinned readback allocation"));
+    int rateKHz = 0;
+    if (hipError_t e = hipDeviceGetAttribute(&rateKHz, hipDeviceAttributeWallClockRate, ctx->device); e != hipSuccess)
+        return cleanup(failHip(e, "wall clock rate"));
+    b->spinTicks = (uint64_t)std::max(rateKHz, 1) * 1000ull * 10ull; // 10 s
+    b->cfgBits = cfgBitsOf(cfg) | (ctx->counters ? tdag::kCfgCount : 0u);
+    b->grab = 4;
+    if (tdag_status s = uploadBatchDesc(b); s != TDAG_OK)
+        return cleanup(s);
+    if (use_graph) {
+        if (hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal); e != hipSuccess)
+            return cleanup(failHip(e, "hipStreamBeginCapture"));
+        const tdag_status s = enqueueBatchCall(b);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        b->graph = g;
+        if (s != TDAG_OK)
+            return cleanup(s);
+        if (e != hipSuccess)
+            return cleanup(failHip(e, "hipStreamEndCapture"));
+        if (hipError_t e2 = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0); e2 != hipSuccess)
+            return cleanup(failHip(e2, "hipGraphInstantiate"));
+    }
+    *out = b;
+    return TDAG_OK;
+}
+
+tdag_status tdag_batch_run(tdag_batch* b, tdag_match_stats* out_stats, float* out_ms)
+{
+    if (!b)
+        return fail(TDAG_EINVAL, "batch is null");
+    tdag_context* ctx = b->ctx;
+    CAPI_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    CAPI_HIP(hipEventRecord(ctx->ev[0], ctx->stream), "hipEventRecord");
+    if (b->exec) {
+        CAPI_HIP(hipGraphLaunch(b->exec, ctx->stream), "hipGraphLaunch");
+    } else if (tdag_status s = enqueueBatchCall(b); s != TDAG_OK) {
+        return s;
+    }
+    CAPI_HIP(hipEventRecord(ctx->ev[3], ctx->stream), "hipEventRecord");
+    const uint32_t k = b->plan.count;
+    for (uint32_t i = 0; i < k; ++i) {
+        const tdag::LevelView v = tdag::levelView(b->plan.plans[i], (char*)b->scratch + b->plan.scratchOff[i]);
+        CAPI_HIP(hipMemcpyAsync(b->pinned + tdag::kNumCounters * b->plan.job[i], v.counters, 8 * tdag::kNumCounters, hipMemcpyDeviceToHost, ctx->stream),
+            "counter readback");
+    }
+    CAPI_HIP(hipMemcpyAsync(b->pinned + tdag::kNumCounters * k, (char*)b->scratch + b->plan.syncOff + offsetof(tdag::BatchSync, error), 4, hipMemcpyDeviceToHost,
+                 ctx->stream),
+        "barrier status readback");
+    CAPI_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    if ((uint32_t)b->pinned[tdag::kNumCounters * k] != 0)
+        return fail(TDAG_EINTERNAL, "batched matcher: a grid barrier timed out (not all blocks were resident); results are invalid");
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[3]);
+    if (out_ms)
+        *out_ms = ms;
+    if (out_stats)
+        for (uint32_t j = 0; j < k; ++j) {
+            const uint64_t* c = b->pinned + tdag::kNumCounters * j;
+            tdag_match_stats s {};
+            s.num_items = b->jobs[j].n;
+            s.num_matched = c[0];
+            s.num_unique = b->jobs[j].n > 0 ? (uint32_t)(c[1] & 0xFFFFFFFFull) : 0;
+            s.probes = c[2];
+            s.filter_pass = c[3];
+            s.key_hits = c[4];
+            s.candidates = c[5];
+            s.verifications = c[6];
+            s.verified = c[7];
+            s.ms_total = ms;
+            s.ms_search = ms;
+            out_stats[j] = s;
+        }
+    return TDAG_OK;
+}
+
+tdag_status tdag_batch_autotune(tdag_batch* b, uint32_t repeats, uint32_t* out_grab)
+{
+    if (!b)
+        return fail(TDAG_EINVAL, "batch is null");
+    repeats = std::max<uint32_t>(1, repeats);
+    const uint32_t cands[5] = { 1, 2, 4, 8, 16 };
+    uint32_t best = b->grab;
+    float bestMs = -1.f;
+    for (uint32_t c : cands) {
+        b->grab = c;
+        if (tdag_status s = uploadBatchDesc(b); s != TDAG_OK)
+            return s;
+        float total = 0.f;
+        for (uint32_t rep = 0; rep <= repeats; ++rep) { // the first run is a warm-up
+            float ms = 0.f;
+            if (tdag_status s = tdag_batch_run(b, nullptr, &ms); s != TDAG_OK)
+                return s;
+            if (rep > 0)
+                total += ms;
+        }
+        if (bestMs < 0.f || total / repeats < bestMs) {
+            bestMs = total / repeats;
+            best = c;
+        }
+    }
+    b->grab = best;
+    if (tdag_status s = uploadBatchDesc(b); s != TDAG_OK)
+        return s;
+    if (out_grab)
+        *out_grab = best;
+    return TDAG_OK;
+}
+
+tdag_status tdag_batch_destroy(tdag_batch* b)
+{
+    if (!b)
+        return TDAG_OK;
+    if (b->exec)
+        (void)hipGraphExecDestroy(b->exec);
+    if (b->graph)
+        (void)hipGraphDestroy(b->graph);
+    if (b->scratch)
+        (void)hipFree(b->scratch);
+    if (b->pinned)
+        (void)hipHostFree(b->pinned);
+    delete b;
+    return TDAG_OK;
+}
+
+tdag_status tdag_match_levels_batched_device(tdag_context* ctx, const tdag_level_job* jobs, uint32_t count, const tdag_config* cfg, tdag_match_stats* out_stats)
+{
+    tdag_batch* b = nullptr;
+    if (tdag_status s = tdag_batch_create(ctx, jobs, count, cfg, 0, &b); s != TDAG_OK)
+        return s;
+    const tdag_status s = tdag_batch_run(b, out_stats, nullptr);
+    tdag_batch_destroy(b);
+    return s;
 }
 
 } // extern "C"

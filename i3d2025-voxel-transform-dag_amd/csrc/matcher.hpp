@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <vector>
 
 namespace tdag {
 
@@ -48,7 +49,35 @@ MatchPlan planMatch(uint32_t level, uint64_t n, uint32_t variant, uint32_t first
 
 // Enqueues the whole matcher on `stream`. Outputs may be null. The device counters at
 // scratch + offCounters hold {num_matched (u64), num_unique (u32)} when the stream completes.
+// Equivalent to enqueuePrepare, enqueueSearch and enqueueFinish in order (with the events
+// around the search).
 hipError_t enqueueMatch(const MatchPlan& plan, const uint64_t* d_masks, uint32_t cfgBits, void* d_scratch, hipStream_t stream,
     uint64_t* d_encoded, uint64_t* d_mapping, uint32_t* d_unique, hipEvent_t evSearch0, hipEvent_t evSearch1);
+
+// Stages of one matcher call. Prepare: rank order, sorted masks, best[] seeds, child table and
+// filter. Search: the rank-batched parent search with table pruning between batches. Finish:
+// parents, unique flags and the mapping.
+hipError_t enqueuePrepare(const MatchPlan& plan, const uint64_t* d_masks, void* d_scratch, hipStream_t stream);
+hipError_t enqueueSearch(const MatchPlan& plan, uint32_t cfgBits, void* d_scratch, hipStream_t stream);
+hipError_t enqueueFinish(const MatchPlan& plan, void* d_scratch, hipStream_t stream, uint64_t* d_encoded, uint64_t* d_mapping, uint32_t* d_unique);
+
+// Device pointers of one matcher call's search state inside its scratch.
+struct LevelView {
+    uint32_t level, n;
+    uint64_t* sortedMasks; // rank-ordered masks; levels >= 3 followed by the two transposed layouts
+    uint64_t* best;
+    uint64_t* tabKeys; // level 2: keys; levels >= 3: packed (hash << 32 | rank << 3 | corner)
+    uint32_t* tabVals; // level 2 only
+    uint32_t* bloom;
+    unsigned long long* counters;
+    uint32_t logCap, logBloom;
+};
+LevelView levelView(const MatchPlan& plan, void* d_scratch);
+
+// End ranks (exclusive) of the rank-batched search's batches.
+std::vector<uint32_t> batchEnds(const MatchPlan& plan);
+// Smallest key a parent of rank >= r1 can produce at `level`: children whose best key is below
+// it are final once parents [0, r1) have been searched.
+uint64_t pruneKey(uint32_t level, uint32_t r1);
 
 } // namespace tdag

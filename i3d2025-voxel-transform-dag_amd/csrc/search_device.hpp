@@ -1,0 +1,610 @@
+// Device-side building blocks of the exhaustive matcher's search, shared by the per-level
+// kernels and the batched persistent kernel: filter/table probing, the level-2 wave search, the
+// level-3..5 plane-per-wave nested-hash search, bitwise verification and table pruning.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "common.hpp"
+#include "matcher.hpp"
+
+namespace tdag {
+namespace dev {
+
+__device__ __forceinline__ uint64_t loadRelaxed(const uint64_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ bool configAllows(uint32_t cfgBits, uint32_t permCode, uint32_t sym, int tx, int ty, int tz)
+{
+    if (!(cfgBits & kCfgAxis) && permCode != 0)
+        return false;
+    if (!(cfgBits & kCfgSymmetry) && sym != 0)
+        return false;
+    if (!(cfgBits & kCfgTranslation) && (tx | ty | tz) != 0)
+        return false;
+    return true;
+}
+
+// LDS bytes per workgroup on gfx950.
+constexpr size_t kLdsBytes = 160 * 1024;
+
+// Workgroups of a launch are dealt to the 8 XCDs round-robin (block b runs on XCD b mod 8), and
+// each XCD has its own L2. A launch of `groups` groups of `perGroup` blocks that share data uses
+// a 1-D grid of xcdGridBlocks(groups, perGroup) blocks, decoded so that the k-th block of XCD x
+// is member k mod perGroup of group (k div perGroup) * 8 + x: a group's blocks run back to back
+// on one XCD. Blocks whose group is >= groups (padding to a multiple of 8) exit at once.
+constexpr uint32_t kXcds = 8;
+
+struct XcdBlock {
+    uint32_t group, member;
+};
+
+__device__ __forceinline__ XcdBlock xcdBlock(uint32_t b, uint32_t perGroup)
+{
+    const uint32_t k = b / kXcds;
+    return { (k / perGroup) * kXcds + b % kXcds, k % perGroup };
+}
+
+
+// Child rank stored in a table entry whose child can no longer improve (tombstone; the key stays
+// so linear-probing clusters remain intact). Ranks are < 2^28, so this never names an item.
+constexpr uint32_t kDeadRank = 0x1FFFFFFFu;
+
+// Per-thread probe statistics, summed per wave into the call's counters when kCfgCount is set.
+struct ProbeCounts {
+    uint32_t probes = 0, filterPass = 0, keyHits = 0, candidates = 0, verifications = 0, verified = 0;
+};
+
+__device__ __forceinline__ uint32_t waveSum(uint32_t v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ void flushCounts(const ProbeCounts& c, unsigned long long* counters, uint32_t cfgBits)
+{
+    if (!(cfgBits & kCfgCount))
+        return;
+    const uint32_t v[6] = { waveSum(c.probes), waveSum(c.filterPass), waveSum(c.keyHits), waveSum(c.candidates), waveSum(c.verifications), waveSum(c.verified) };
+    if ((threadIdx.x & 63u) == 0)
+        for (int i = 0; i < 6; ++i)
+            if (v[i])
+                atomicAdd(&counters[2 + i], (unsigned long long)v[i]);
+}
+
+// Level-2 search of parent rank r by one wave; lane = anchored voxel v' = (x, y, z) of the
+// sub-box. For each permutation the 8 direction keys are formed first and their filter loads
+// issued together.
+__device__ __forceinline__ void search2Parent(uint32_t r, const uint64_t* __restrict__ sortedMasks, const uint64_t* __restrict__ tabKeys,
+    const uint32_t* __restrict__ tabVals, uint32_t logCap, const uint32_t* __restrict__ bloom, uint32_t logBloom, uint64_t* __restrict__ best, uint32_t cfgBits,
+    ProbeCounts& cnt)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t P = sortedMasks[r];
+    if (P == 0)
+        return;
+    const int vx = (int)(lane & 3u), vy = (int)((lane >> 2) & 3u), vz = (int)(lane >> 4);
+    const uint64_t clip = rangeMaskX4(0, 4 - vx) & rangeMaskY4(0, 4 - vy) & rangeMaskZ4(0, 4 - vz);
+    const uint32_t capMask = (1u << logCap) - 1u;
+    const uint32_t numPerms = (cfgBits & kCfgAxis) ? 6u : 1u;
+    for (uint32_t pc = 0; pc < numPerms; ++pc) {
+        // permute (find_translation.cpp:109-125): out(o) = P(v) with v[p[a]] = o[a].
+        int v[3];
+        v[permAxis(pc, 0)] = vx;
+        v[permAxis(pc, 1)] = vy;
+        v[permAxis(pc, 2)] = vz;
+        const uint32_t src = (uint32_t)(v[0] + 4 * v[1] + 16 * v[2]);
+        const uint64_t Q = (uint64_t)__ballot((P >> src) & 1ull);
+        uint64_t key[8];
+        uint32_t bw[8];
+#pragma unroll
+        for (uint32_t d = 0; d < 8; ++d)
+            key[d] = (mirror4(Q, (~d) & 7u) >> lane) & clip;
+#pragma unroll
+        for (uint32_t d = 0; d < 8; ++d)
+            bw[d] = key[d] ? bloom[bloomWord(bloomMix64(key[d]), logBloom)] : 0u;
+#pragma unroll
+        for (uint32_t d = 0; d < 8; ++d) {
+            const uint32_t need = bloomBits(bloomMix64(key[d]));
+            cnt.probes += key[d] != 0;
+            if (key[d] == 0 || (bw[d] & need) != need)
+                continue;
+            ++cnt.filterPass;
+            uint32_t s = homeSlot64(key[d], logCap);
+            uint64_t k = tabKeys[s];
+            while (k != 0) {
+                if (k == key[d]) {
+                    ++cnt.keyHits;
+                    const uint32_t e = tabVals[s];
+                    const uint32_t c = e >> 3;
+                    const uint32_t kc = e & 7u;
+                    if (c > r && c != kDeadRank) {
+                        const uint32_t sym = d ^ kc;
+                        const int tx = (kc & 1u) ? -vx : vx;
+                        const int ty = (kc & 2u) ? -vy : vy;
+                        const int tz = (kc & 4u) ? -vz : vz;
+                        if (configAllows(cfgBits, pc, sym, tx, ty, tz)) {
+                            ++cnt.candidates;
+                            const uint64_t k64 = encodeKey(r, pc, sym, tx, ty, tz, 2);
+                            if (k64 < loadRelaxed(&best[c])) {
+                                ++cnt.verifications;
+                                // Bitwise verification of the derived transform.
+                                const uint64_t T = translate4(mirror4(Q, sym), tx, ty, tz);
+                                if (T == sortedMasks[c]) {
+                                    ++cnt.verified;
+                                    atomicMin((unsigned long long*)&best[c], (unsigned long long)k64);
+                                }
+                            }
+                        }
+                    }
+                }
+                s = (s + 1) & capMask;
+                k = tabKeys[s];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// ---------------------------------------------------------------------------------------
+// Levels 3..5: 32-bit nested sub-box hashes computed in LDS, one line per thread per pass.
+// ---------------------------------------------------------------------------------------
+template <int L>
+struct RowType;
+template <>
+struct RowType<3> {
+    using T = uint8_t;
+};
+template <>
+struct RowType<4> {
+    using T = uint16_t;
+};
+template <>
+struct RowType<5> {
+    using T = uint32_t;
+};
+
+// Geometry of the level-L (R = 2^L) search. A "row" is the R bits of one line of the grid along
+// one axis, stored as an R-bit unsigned integer (bit i = coordinate i).
+template <int L>
+struct Geo {
+    using Row = typename RowType<L>::T;
+    static constexpr int R = 1 << L;
+    static constexpr int R2 = R * R;
+    static constexpr int R3 = R2 * R;
+    static constexpr int W = R3 / 64; // u64 words per bit layout
+    static constexpr int Pitch = R + 1; // padded pitch of rows in LDS and of lines in a plane
+    static constexpr int Rows = R * Pitch; // padded rows per layout in LDS
+    static constexpr int P = 64 / R; // planes per wave
+    static constexpr int Slab = R * Pitch; // words per plane in LDS
+    static constexpr int MaxWaves = R / P; // waves that cover all R planes
+};
+
+// Three bit layouts of one item in global memory: layout a has axis a fastest, so the line along
+// axis a at (u, w) — its coordinates along the other two axes in ascending axis order — is row
+// j = u + R*w, bits [R*j, R*j + R).
+//   layout 0: x + R*y + R*R*z (the LargeSubGrid layout)
+//   layout 1: y + R*x + R*R*z
+//   layout 2: z + R*x + R*R*y
+// In LDS row (u, w) of layout a sits at a*Rows + u + Pitch*w; the odd pitch keeps lanes that
+// read rows at stride Pitch on distinct banks.
+template <int L>
+__device__ __forceinline__ uint32_t lbit(const uint64_t* s_bits, int x, int y, int z)
+{
+    constexpr int R = Geo<L>::R;
+    const uint32_t i = (uint32_t)(x + R * y + R * R * z);
+    return (uint32_t)((s_bits[i >> 6] >> (i & 63u)) & 1ull);
+}
+
+// Loads the three layouts of item r into LDS rows. Returns whether the item is non-empty (the
+// result is uniform over the block; contains a block barrier).
+template <int L, int Threads>
+__device__ __forceinline__ bool loadRows(const uint64_t* __restrict__ layouts, uint32_t n, uint32_t r, typename Geo<L>::Row* s_rows, int numLayouts)
+{
+    using G = Geo<L>;
+    using Row = typename G::Row;
+    const Row* g = reinterpret_cast<const Row*>(layouts);
+    int any = 0;
+    for (int i = (int)threadIdx.x; i < numLayouts * G::R2; i += Threads) {
+        const int lay = i >> (2 * L), j = i & (G::R2 - 1);
+        const Row v = g[((size_t)lay * n + r) * G::R2 + j];
+        s_rows[lay * G::Rows + j + (j >> L)] = v;
+        any |= (v != 0);
+    }
+    return __syncthreads_or(any) != 0;
+}
+
+// Orders LDS accesses of the lanes of one wave (LDS operations of a wave execute in order; this
+// keeps the compiler from moving them across).
+__device__ __forceinline__ void waveSync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Wave-level loadRows: the calling wave alone loads item r's layouts into its own s_rows.
+// Returns whether the item is non-empty (uniform over the wave).
+template <int L>
+__device__ __forceinline__ bool loadRowsWave(const uint64_t* __restrict__ layouts, uint32_t n, uint32_t r, typename Geo<L>::Row* s_rows, int numLayouts)
+{
+    using G = Geo<L>;
+    using Row = typename G::Row;
+    const Row* g = reinterpret_cast<const Row*>(layouts);
+    int any = 0;
+    for (int i = (int)(threadIdx.x & 63u); i < numLayouts * G::R2; i += 64) {
+        const int lay = i >> (2 * L), j = i & (G::R2 - 1);
+        const Row v = g[((size_t)lay * n + r) * G::R2 + j];
+        s_rows[lay * G::Rows + j + (j >> L)] = v;
+        any |= (v != 0);
+    }
+    waveSync();
+    return __ballot(any) != 0ull;
+}
+
+// Value of the first pass (along A0) at coordinate c0 of a row: the trimmed bit sequence of the
+// row segment read from the scan start toward c0 is the binary number formed by those bits, so
+// it needs no hashing. Positive (scan from R-1 down to c0): bits [c0, R-1] with bit R-1 most
+// significant, i.e. row >> c0. Negative (scan from 0 up to c0): bits [0, c0] with bit 0 most
+// significant, i.e. bitreverse32(row) >> (31 - c0). 0 iff the segment is empty. `sh` is c0 or
+// 31 - c0 respectively.
+template <bool Pos>
+__device__ __forceinline__ uint32_t rowPrefix(uint32_t row, int sh)
+{
+    return Pos ? (row >> sh) : (__brev(row) >> sh);
+}
+
+// Passes 1 and 2 of the nested hash on plane c0 (the coordinate along the first pass axis A0):
+// the lane owns the line along A1 at c[A2] = u and reads, at each step, the row along A0
+// through that voxel. The pass-2 state of voxel (c1, c2) goes to plane[c1 + Pitch*c2].
+template <int L, int A0, int A1, bool Pos0, bool Pos1>
+__device__ __forceinline__ void pass12(const typename Geo<L>::Row* s_rows, int c0, int u, uint32_t* plane)
+{
+    using G = Geo<L>;
+    constexpr int R = G::R;
+    constexpr int A2 = 3 - A0 - A1;
+    constexpr int Step = (A1 < A2) ? 1 : G::Pitch; // row (lo, hi) = (c[A1], c[A2]) or (c[A2], c[A1])
+    const typename G::Row* rows = s_rows + A0 * G::Rows + ((A1 < A2) ? G::Pitch * u : u);
+    uint32_t* line = plane + G::Pitch * u;
+    const int sh = Pos0 ? c0 : 31 - c0;
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int c1 = Pos1 ? (R - 1 - k) : k;
+        s = hashStep(s, rowPrefix<Pos0>((uint32_t)rows[Step * c1], sh));
+        line[c1] = s;
+    }
+}
+
+template <int L, int A0, int A1>
+__device__ __forceinline__ void pass12Dirs(uint32_t d, const typename Geo<L>::Row* s_rows, int c0, int u, uint32_t* plane)
+{
+    switch (d & 3u) {
+    case 0:
+        pass12<L, A0, A1, false, false>(s_rows, c0, u, plane);
+        break;
+    case 1:
+        pass12<L, A0, A1, true, false>(s_rows, c0, u, plane);
+        break;
+    case 2:
+        pass12<L, A0, A1, false, true>(s_rows, c0, u, plane);
+        break;
+    default:
+        pass12<L, A0, A1, true, true>(s_rows, c0, u, plane);
+        break;
+    }
+}
+
+// Passes 1 and 2 for permutation code pc (axes p0, p1) and direction bits d (bit 0: pass 1,
+// bit 1: pass 2).
+template <int L>
+__device__ __forceinline__ void passes12(uint32_t pc, uint32_t d, const typename Geo<L>::Row* s_rows, int c0, int u, uint32_t* plane)
+{
+    switch (pc) { // transform_dag.h:138-145
+    case 0:
+        pass12Dirs<L, 0, 1>(d, s_rows, c0, u, plane);
+        break;
+    case 1:
+        pass12Dirs<L, 0, 2>(d, s_rows, c0, u, plane);
+        break;
+    case 2:
+        pass12Dirs<L, 1, 0>(d, s_rows, c0, u, plane);
+        break;
+    case 3:
+        pass12Dirs<L, 1, 2>(d, s_rows, c0, u, plane);
+        break;
+    case 4:
+        pass12Dirs<L, 2, 0>(d, s_rows, c0, u, plane);
+        break;
+    default:
+        pass12Dirs<L, 2, 1>(d, s_rows, c0, u, plane);
+        break;
+    }
+}
+
+// Pass 3 along the lane's line (c1 = u) of its plane over the pass-2 states in[Pitch*c2], fused
+// with the Bloom filter: stores the final hash of every voxel at out[Pitch*c2] (out may equal in)
+// and returns the bit set of voxels c2 that are non-empty and pass the filter. The filter word is
+// loaded unconditionally (every hash indexes a valid word) so the sweep has no branches and its
+// loads overlap the hashing.
+template <int L, bool Pos2>
+__device__ __forceinline__ uint32_t pass3Filter(const uint32_t* in, uint32_t* out, const uint32_t* __restrict__ bloom, uint32_t logBloom, ProbeCounts& cnt)
+{
+    constexpr int R = Geo<L>::R;
+    const uint32_t shift = 32u - logBloom;
+    uint32_t st[R];
+#pragma unroll
+    for (int c2 = 0; c2 < R; ++c2)
+        st[c2] = in[Geo<L>::Pitch * c2];
+    uint32_t s = 0, pending = 0, nonEmpty = 0;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int c2 = Pos2 ? (R - 1 - k) : k;
+        s = hashStep(s, st[c2]);
+        const uint32_t f = finalHash(s);
+        out[Geo<L>::Pitch * c2] = f;
+        const uint32_t bw = bloom[f >> shift];
+        const uint32_t need = bloomBits(f);
+        const uint32_t miss = need & ~bw;
+        const uint32_t nz = f < 1u ? f : 1u;
+        const uint32_t ok = nz & ((miss < 1u ? miss : 1u) ^ 1u);
+        pending |= ok << c2;
+        nonEmpty += nz;
+    }
+    cnt.probes += nonEmpty;
+    cnt.filterPass += (uint32_t)__builtin_popcount(pending);
+    return pending;
+}
+
+// Expected child row (yo, zo) (bits along x) for the transform encoded in `key` applied to the
+// parent rows: out[o] = M[o - t] with M = mirror(permute(P)) (find_translation.cpp:66-84 for the
+// translation). The source of an output row is one line of the parent along axis p0, read from
+// layout p0.
+template <int L>
+__device__ __forceinline__ uint32_t transformedRow(const typename Geo<L>::Row* s_rows, int tx, int ty, int tz, uint32_t sym, uint32_t pc, int yo, int zo)
+{
+    using G = Geo<L>;
+    constexpr int R = G::R;
+    const int q1 = yo - ty, q2 = zo - tz;
+    if (q1 < 0 || q1 >= R || q2 < 0 || q2 >= R)
+        return 0u;
+    const int m1 = (sym & 2u) ? R - 1 - q1 : q1;
+    const int m2 = (sym & 4u) ? R - 1 - q2 : q2;
+    const int a1 = (int)permAxis(pc, 1), a2 = (int)permAxis(pc, 2);
+    // Parent coordinates along a1 and a2; the row along a0 is indexed by them in ascending axis order.
+    const int lo = a1 < a2 ? m1 : m2, hi = a1 < a2 ? m2 : m1;
+    uint32_t line = (uint32_t)s_rows[(int)permAxis(pc, 0) * G::Rows + lo + G::Pitch * hi];
+    if (sym & 1u)
+        line = __brev(line) >> (32 - R);
+    const uint32_t rowMask = R == 32 ? 0xFFFFFFFFu : ((1u << R) - 1u);
+    return (tx >= 0 ? (line << tx) : (line >> -tx)) & rowMask;
+}
+
+// Wave-cooperative bitwise check child bc == T(parent) for the transform in key bk.
+template <int L>
+__device__ __forceinline__ bool verifyWave(const typename Geo<L>::Row* s_rows, const uint64_t* __restrict__ sortedMasks, uint32_t bc, uint64_t bk, uint32_t lane)
+{
+    using G = Geo<L>;
+    constexpr int R = G::R;
+    constexpr uint32_t b = L + 1;
+    constexpr uint64_t m = (1ull << b) - 1ull;
+    const int tx = (int)(bk & m) - R;
+    const int ty = (int)((bk >> b) & m) - R;
+    const int tz = (int)((bk >> (2 * b)) & m) - R;
+    const uint32_t sym = (uint32_t)((bk >> (3 * b)) & 7u);
+    const uint32_t pc = (uint32_t)((bk >> (3 * b + 3)) & 7u);
+    const typename G::Row* child = reinterpret_cast<const typename G::Row*>(sortedMasks) + (size_t)bc * G::R2;
+    bool ok = true;
+    for (int j = (int)lane; j < G::R2; j += 64)
+        ok = ok && (transformedRow<L>(s_rows, tx, ty, tz, sym, pc, j & (R - 1), j >> L) == (uint32_t)child[j]);
+    return __ballot(!ok) == 0ull;
+}
+
+// Walks the child table for the voxels in `pending` (bit c2 = voxel (c0, u, c2) of the lane's
+// line passed the filter; its hash is in the plane) and merges verified hits into best. Lanes stop
+// at each candidate that beats best[c]; the wave then verifies the stopped lanes' candidates
+// together and resumes.
+template <int L>
+__device__ __forceinline__ void probeLine(uint32_t pending, const uint32_t* line, const typename Geo<L>::Row* s_rows,
+    const uint64_t* __restrict__ sortedMasks, const uint64_t* __restrict__ table, uint32_t logCap,
+    uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t r, uint32_t pc, uint32_t d, int c0, int u, ProbeCounts& cnt)
+{
+    using G = Geo<L>;
+    constexpr int R = G::R;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t capMask = (1u << logCap) - 1u;
+    bool active = false;
+    uint32_t h = 0, slot = 0;
+    int c2 = 0;
+    while (true) {
+        bool found = false;
+        uint32_t hitC = 0;
+        uint64_t hitKey = 0;
+        while (true) {
+            if (!active) {
+                if (pending == 0)
+                    break;
+                c2 = __ffs(pending) - 1;
+                pending &= pending - 1u;
+                h = line[G::Pitch * c2];
+                slot = homeSlot32(h, logCap);
+                active = true;
+            }
+            const uint64_t e = table[slot];
+            if (e == 0) {
+                active = false;
+                continue;
+            }
+            slot = (slot + 1) & capMask;
+            if ((uint32_t)(e >> 32) != h)
+                continue;
+            ++cnt.keyHits;
+            const uint32_t cc = (uint32_t)e >> 3;
+            const uint32_t kc = (uint32_t)e & 7u;
+            if (cc <= r || cc == kDeadRank)
+                continue;
+            // Translation of the anchored match (find_translation.cpp:835-837); the voxel's
+            // coordinates along the three pass axes are (c0, u, c2).
+            const int voxel[3] = { c0, u, c2 };
+            const uint32_t sym = d ^ kc;
+            int tt[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const int anchor = ((d >> a) & 1u) ? 0 : R - 1;
+                tt[a] = ((sym >> a) & 1u) ? voxel[a] - anchor : anchor - voxel[a];
+            }
+            if (!configAllows(cfgBits, pc, sym, tt[0], tt[1], tt[2]))
+                continue;
+            ++cnt.candidates;
+            const uint64_t k64 = encodeKey(r, pc, sym, tt[0], tt[1], tt[2], L);
+            if (k64 >= loadRelaxed(&best[cc]))
+                continue;
+            ++cnt.verifications;
+            found = true;
+            hitC = cc;
+            hitKey = k64;
+            break;
+        }
+        uint64_t stopped = __ballot(found);
+        if (stopped == 0)
+            break; // no lane found a candidate, so every lane ran out of work
+        while (stopped) {
+            const int leader = __ffsll((unsigned long long)stopped) - 1;
+            const uint32_t bc = (uint32_t)__shfl((int)hitC, leader, 64);
+            const uint64_t bk = (uint64_t)__shfl((long long)hitKey, leader, 64);
+            const bool allOk = verifyWave<L>(s_rows, sortedMasks, bc, bk, lane);
+            if ((int)lane == leader && allOk) {
+                ++cnt.verified;
+                atomicMin((unsigned long long*)&best[bc], (unsigned long long)bk);
+            }
+            stopped &= stopped - 1ull;
+        }
+    }
+}
+
+// Pass 3 in direction dz with the filter (pass-2 states read from `in`, final hashes written to
+// `out`), then the table walk of the lane's line.
+template <int L>
+__device__ __forceinline__ void lastPassProbe(uint32_t dz, const uint32_t* in, uint32_t* out, const typename Geo<L>::Row* s_rows,
+    const uint64_t* __restrict__ sortedMasks, const uint64_t* __restrict__ table, uint32_t logCap, const uint32_t* __restrict__ bloom, uint32_t logBloom,
+    uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t r, uint32_t pc, uint32_t d, int c0, int u, ProbeCounts& cnt)
+{
+    const uint32_t pending = dz ? pass3Filter<L, true>(in, out, bloom, logBloom, cnt) : pass3Filter<L, false>(in, out, bloom, logBloom, cnt);
+    probeLine<L>(pending, out, s_rows, sortedMasks, table, logCap, best, cfgBits, r, pc, d, c0, u, cnt);
+}
+
+// Search-unit kinds of the level-3..5 kernel.
+constexpr int kUnitCombo = 0; // one (permutation, direction) combo per block
+constexpr int kUnitFamily = 1; // a family of 8 combos per block, each run from scratch
+constexpr int kUnitShared = 2; // a family of 8 combos; passes 1 and 2 shared by both pass-3 directions
+
+// LDS bytes of one kSearch block.
+template <int L, int Waves, int Kind>
+constexpr size_t searchLds()
+{
+    return 3 * Geo<L>::Rows * sizeof(typename Geo<L>::Row) + (Kind == kUnitShared ? 2 : 1) * (size_t)Waves * Geo<L>::P * Geo<L>::Slab * 4;
+}
+
+// Search of one unit of parent rank r (rows already in s_rows) by `Waves` waves: `wave` is the
+// caller's wave index within them and covers planes ((chunk * Waves + wave) * P + w) for
+// w < P. planes0 (and planes1 for kUnitShared) hold Waves * P * Slab words each.
+template <int L, int Waves, int Kind>
+__device__ __forceinline__ void searchUnit(uint32_t r, uint32_t unit, uint32_t chunk, int wave, const typename Geo<L>::Row* s_rows, uint32_t* planes0,
+    uint32_t* planes1, const uint64_t* __restrict__ layouts, const uint64_t* __restrict__ table, uint32_t logCap, const uint32_t* __restrict__ bloom,
+    uint32_t logBloom, uint64_t* __restrict__ best, uint32_t cfgBits, ProbeCounts& cnt)
+{
+    using G = Geo<L>;
+    constexpr int R = G::R;
+    const bool axisOn = (cfgBits & kCfgAxis) != 0;
+    const int lane = (int)(threadIdx.x & 63u);
+    const int u = lane & (R - 1), w = lane >> L;
+    const int c0 = ((int)chunk * Waves + wave) * G::P + w;
+    const int planeOff = (wave * G::P + w) * G::Slab;
+    uint32_t* plane = planes0 + planeOff;
+    uint32_t* line = plane + u; // the lane's pass-3 line: c1 = u, stride Pitch along c2
+    uint32_t* outLine = planes1 + planeOff + u;
+    if constexpr (Kind == kUnitCombo) {
+        const uint32_t pc = unit >> 3, d = unit & 7u;
+        passes12<L>(pc, d, s_rows, c0, u, plane);
+        waveSync();
+        lastPassProbe<L>((d >> 2) & 1u, line, line, s_rows, layouts, table, logCap, bloom, logBloom, best, cfgBits, r, pc, d, c0, u, cnt);
+    } else {
+        const uint32_t a0 = unit >> 1, dx = unit & 1u;
+#pragma unroll 1
+        for (uint32_t sel = 0; sel < 2; ++sel) {
+            const uint32_t pc = 2u * a0 + sel; // the permutation codes with p0 == a0
+            if (!axisOn && pc != 0)
+                continue;
+#pragma unroll 1
+            for (uint32_t dy = 0; dy < 2; ++dy) {
+                if constexpr (Kind == kUnitShared) {
+                    waveSync();
+                    passes12<L>(pc, dx | (dy << 1), s_rows, c0, u, plane);
+                    waveSync();
+                }
+#pragma unroll 1
+                for (uint32_t dz = 0; dz < 2; ++dz) {
+                    const uint32_t d = dx | (dy << 1) | (dz << 2);
+                    if constexpr (Kind == kUnitFamily) {
+                        waveSync();
+                        passes12<L>(pc, d, s_rows, c0, u, plane);
+                        waveSync();
+                    }
+                    lastPassProbe<L>(dz, line, outLine, s_rows, layouts, table, logCap, bloom, logBloom, best, cfgBits, r, pc, d, c0, u, cnt);
+                }
+            }
+        }
+    }
+}
+
+// Rank-batched search: after parents [0, r1) have been searched, every child whose best key is
+// below the smallest key a parent >= r1 can produce is final. Its table entries become
+// tombstones and the Bloom filter is rebuilt from the live entries, so later batches neither
+// probe nor load best[] for it. (The reference prunes its candidate ranges between search
+// batches for the same reason, find_translation.cpp:887-915.) Slots [start, cap) step `stride`.
+__device__ __forceinline__ void prunePacked(uint64_t* __restrict__ table, uint64_t cap, const uint64_t* __restrict__ best, uint64_t minKey, uint32_t* __restrict__ bloom,
+    uint32_t logBloom, uint64_t start, uint64_t stride)
+{
+    for (uint64_t s = start; s < cap; s += stride) {
+        const uint64_t e = table[s];
+        if (e == 0)
+            continue;
+        const uint32_t c = (uint32_t)e >> 3;
+        if (c == kDeadRank)
+            continue;
+        if (best[c] < minKey) {
+            table[s] = (e & 0xFFFFFFFF00000007ull) | ((uint64_t)kDeadRank << 3);
+            continue;
+        }
+        const uint32_t h = (uint32_t)(e >> 32);
+        atomicOr(&bloom[bloomWord(h, logBloom)], bloomBits(h));
+    }
+}
+
+__device__ __forceinline__ void pruneSplit(const uint64_t* __restrict__ tabKeys, uint32_t* __restrict__ tabVals, uint64_t cap, const uint64_t* __restrict__ best,
+    uint64_t minKey, uint32_t* __restrict__ bloom, uint32_t logBloom, uint64_t start, uint64_t stride)
+{
+    for (uint64_t s = start; s < cap; s += stride) {
+        const uint64_t k = tabKeys[s];
+        if (k == 0)
+            continue;
+        const uint32_t e = tabVals[s];
+        const uint32_t c = e >> 3;
+        if (c == kDeadRank)
+            continue;
+        if (best[c] < minKey) {
+            tabVals[s] = (kDeadRank << 3) | (e & 7u);
+            continue;
+        }
+        const uint32_t f = bloomMix64(k);
+        atomicOr(&bloom[bloomWord(f, logBloom)], bloomBits(f));
+    }
+}
+
+} // namespace dev
+} // namespace tdag

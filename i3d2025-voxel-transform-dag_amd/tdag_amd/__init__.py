@@ -62,6 +62,20 @@ class TdagMatchStats(ctypes.Structure):
         return {k: int(getattr(self, k)) for k in ("probes", "filter_pass", "key_hits", "candidates", "verifications", "verified")}
 
 
+class TdagLevelJob(ctypes.Structure):
+    """One level of a batched matcher call (tdag_level_job in include/tdag.h)."""
+
+    _fields_ = [
+        ("level", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("n", ctypes.c_uint64),
+        ("d_masks", ctypes.c_void_p),
+        ("d_encoded", ctypes.c_void_p),
+        ("d_mapping", ctypes.c_void_p),
+        ("d_unique", ctypes.c_void_p),
+    ]
+
+
 @dataclass
 class TransformDAGConfig:
     """Mirror of voxcom::TransformDAGConfig with the reference defaults."""
@@ -113,6 +127,15 @@ def lib() -> ctypes.CDLL:
     L.tdag_context_set_filter_bits.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]
     L.tdag_context_get_filter_bits.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
     L.tdag_context_get_batching.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]
+    L.tdag_batch_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(TdagLevelJob), ctypes.c_uint32, ctypes.POINTER(TdagConfig), ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_void_p)]
+    L.tdag_batch_run.argtypes = [ctypes.c_void_p, ctypes.POINTER(TdagMatchStats), ctypes.POINTER(ctypes.c_float)]
+    L.tdag_batch_destroy.argtypes = [ctypes.c_void_p]
+    L.tdag_batch_autotune.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
+    L.tdag_match_levels_batched_device.argtypes = [ctypes.c_void_p, ctypes.POINTER(TdagLevelJob), ctypes.c_uint32, ctypes.POINTER(TdagConfig),
+                                                   ctypes.POINTER(TdagMatchStats)]
+    for name in ("tdag_batch_create", "tdag_batch_run", "tdag_batch_destroy", "tdag_batch_autotune", "tdag_match_levels_batched_device"):
+        getattr(L, name).restype = ctypes.c_int
     for name in ("tdag_context_create", "tdag_context_destroy", "tdag_match_level_device", "tdag_match_level", "tdag_last_stats",
                  "tdag_context_set_variant", "tdag_context_get_variant", "tdag_autotune_level", "tdag_context_set_counters",
                  "tdag_context_set_batching", "tdag_context_get_batching", "tdag_context_set_filter_bits", "tdag_context_get_filter_bits"):
@@ -139,6 +162,63 @@ def last_error() -> str:
 def check(status: int, what: str) -> None:
     if status != TDAG_OK:
         raise TdagError(f"{what} failed with {_STATUS.get(status, status)}: {last_error()}")
+
+
+def _dev_ptr(x):
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    return x.data_ptr()
+
+
+def _jobs_array(jobs):
+    arr = (TdagLevelJob * len(jobs))()
+    keep = []
+    for i, j in enumerate(jobs):
+        arr[i].level = int(j["level"])
+        arr[i].n = int(j["n"])
+        for k in ("d_masks", "d_encoded", "d_mapping", "d_unique"):
+            setattr(arr[i], k, _dev_ptr(j.get(k)))
+            keep.append(j.get(k))
+    return arr, keep
+
+
+class Batch:
+    """A planned (and, with use_graph, captured) batched matcher call; see Context.batch."""
+
+    def __init__(self, ctx: "Context", jobs, config, use_graph: bool):
+        self._ptr = ctypes.c_void_p()
+        self._ctx = ctx
+        self._jobs, self._keep = _jobs_array(jobs)
+        self.count = len(jobs)
+        cfg = (config or TransformDAGConfig()).to_c()
+        check(lib().tdag_batch_create(ctx._ptr, self._jobs, self.count, ctypes.byref(cfg), int(bool(use_graph)), ctypes.byref(self._ptr)),
+              "tdag_batch_create")
+
+    def run(self):
+        """Runs the call and waits; returns (per-job stats, milliseconds of the whole call)."""
+        stats = (TdagMatchStats * self.count)()
+        ms = ctypes.c_float()
+        check(lib().tdag_batch_run(self._ptr, stats, ctypes.byref(ms)), "tdag_batch_run")
+        return list(stats), float(ms.value)
+
+    def autotune(self, repeats: int = 1) -> int:
+        """Picks the queue granularity (tiles per pop); returns it."""
+        g = ctypes.c_uint32()
+        check(lib().tdag_batch_autotune(self._ptr, int(repeats), ctypes.byref(g)), "tdag_batch_autotune")
+        return int(g.value)
+
+    def close(self):
+        if self._ptr:
+            lib().tdag_batch_destroy(self._ptr)
+            self._ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def version() -> str:
@@ -201,8 +281,23 @@ class Context:
             return int(nu.value), st
         return None, None
 
+    # -- batched form ------------------------------------------------------------------------
+    def batch(self, jobs, config: TransformDAGConfig | None = None, use_graph: bool = True) -> "Batch":
+        """Batched matcher over several levels (one job per level): every level's parent search
+        runs in one persistent kernel; with use_graph the whole call is captured once as a
+        hipGraph. `jobs` holds dicts with keys level, n, d_masks and optional d_encoded,
+        d_mapping, d_unique (torch tensors or raw device pointers)."""
+        return Batch(self, jobs, config, use_graph)
+
+    def match_levels_batched_device(self, jobs, config: TransformDAGConfig | None = None):
+        """One-shot batched call (no graph); returns per-job stats."""
+        arr, _ = _jobs_array(jobs)
+        cfg = (config or TransformDAGConfig()).to_c()
+        stats = (TdagMatchStats * len(jobs))()
+        check(lib().tdag_match_levels_batched_device(self._ptr, arr, len(jobs), ctypes.byref(cfg), stats), "tdag_match_levels_batched_device")
+        return list(stats)
+
     # -- tuning ---------------------------------------------------------------------------
-    VARIANTS = {0: "one combo per block", 1: "family, 1 LDS buffer", 2: "family, 2 LDS buffers", 3: "family, 3 LDS buffers"}
 
     def set_counters(self, enable: bool) -> None:
         """Per-stage probe counters in the returned stats (diagnostics)."""

@@ -149,6 +149,44 @@ tdag_status tdag_autotune_level(tdag_context* ctx, uint32_t level, const uint64_
     float* out_ms, int32_t* out_best, uint32_t* out_first_batch);
 
 /* ------------------------------------------------------------------------------------------
+ * Batched form: several levels whose inputs are known (one job per level) are matched in one
+ * call, with every level's parent search in ONE persistent kernel fed by a device work queue,
+ * optionally captured once as a hipGraph and replayed. Each level's rank-batched schedule (the
+ * popcount buckets of parents) comes from the context's per-level settings (the autotuner's
+ * picks). Results are bit-identical to tdag_match_level_device on each job. No reference
+ * counterpart: the reference launches its GPU matcher per level and per batch
+ * (find_translation.cu:1159-1563).
+ * ---------------------------------------------------------------------------------------- */
+typedef struct tdag_level_job {
+    uint32_t level; /* 2..5; at most one job per level */
+    uint32_t reserved;
+    uint64_t n;
+    const uint64_t* d_masks; /* device, n x R^3/64 words, linear layout */
+    uint64_t* d_encoded; /* optional device outputs, as in tdag_match_level_device */
+    uint64_t* d_mapping;
+    uint32_t* d_unique;
+} tdag_level_job;
+
+typedef struct tdag_batch tdag_batch;
+
+/* Plans the batch (device scratch owned by the batch), uploads its descriptor and, when
+ * use_graph != 0, captures the whole call (all levels' preparation, the persistent search, all
+ * levels' post-processing) into a hipGraph on the context's stream. The jobs' device buffers
+ * must stay valid until tdag_batch_destroy. */
+tdag_status tdag_batch_create(tdag_context* ctx, const tdag_level_job* jobs, uint32_t count, const tdag_config* cfg, int use_graph, tdag_batch** out);
+/* Runs the batch on the context's stream (graph replay or eager launches) and waits. out_stats
+ * (count entries, job order, may be NULL) receives per-level results; ms_total and ms_search
+ * are the whole call's time. Fails with TDAG_EINTERNAL if a grid barrier timed out. */
+tdag_status tdag_batch_run(tdag_batch* b, tdag_match_stats* out_stats, float* out_ms);
+/* Tunes the batch's queue granularity (consecutive tiles per pop: 1, 2, 4, 8, 16) by running it
+ * (1 warm-up + `repeats` timed runs each) and keeps the fastest; the captured graph stays valid
+ * (it reads the descriptor at run time). */
+tdag_status tdag_batch_autotune(tdag_batch* b, uint32_t repeats, uint32_t* out_grab);
+tdag_status tdag_batch_destroy(tdag_batch* b);
+/* One-shot batched call without a graph: create, run, destroy. */
+tdag_status tdag_match_levels_batched_device(tdag_context* ctx, const tdag_level_job* jobs, uint32_t count, const tdag_config* cfg, tdag_match_stats* out_stats);
+
+/* ------------------------------------------------------------------------------------------
  * Pipeline around the matcher: scene synthesis, SVDAG producer, flattening, the
  * constructTransformDAG driver and the .tdag writer.
  * ---------------------------------------------------------------------------------------- */

@@ -26,8 +26,13 @@ def oracle():
 
 @pytest.fixture(scope="session")
 def gpu_ctx():
+    import torch
+
     import tdag_amd
 
+    # torch's bundled HIP runtime must initialise the device before libtdag's ROCm runtime does;
+    # tests hand torch device buffers to the library.
+    torch.cuda.init()
     ctx = tdag_amd.Context(0)
     yield ctx
     ctx.close()
