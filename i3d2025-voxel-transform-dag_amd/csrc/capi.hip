@@ -9,32 +9,19 @@
 #include <utility>
 #include <vector>
 
-#include "errors.hpp"
 #include "batched.hpp"
+#include "context.hpp"
+#include "errors.hpp"
 #include "matcher.hpp"
 #include "tdag.h"
 
-struct tdag_context {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    void* scratch = nullptr;
-    size_t scratchBytes = 0;
-    // Host-buffer entry points stage their inputs/outputs here.
-    void* io = nullptr;
-    size_t ioBytes = 0;
-    uint64_t* pinned = nullptr; // 4 words of pinned host memory for small readbacks
-    hipEvent_t ev[4] = { nullptr, nullptr, nullptr, nullptr };
-    tdag_match_stats last {};
-    int32_t variant[6] = { -1, -1, -1, -1, -1, -1 }; // per level; -1 = library default
-    bool counters = false; // collect per-stage probe counters
-    uint32_t firstBatch[6] = { 0, 0, 0, 0, 0, 0 }; // rank-batched search schedule per level (0 = default)
-    uint32_t batchGrowth[6] = { 0, 0, 0, 0, 0, 0 };
-    uint32_t bloomBits[6] = { 0, 0, 0, 0, 0, 0 }; // Bloom filter bits per table entry (0 = default)
-};
 
 namespace {
-
 thread_local std::string g_lastError;
+} // namespace
+
+namespace tdag {
+namespace capi {
 
 tdag_status fail(tdag_status s, const std::string& msg)
 {
@@ -47,13 +34,6 @@ tdag_status failHip(hipError_t e, const char* what)
     g_lastError = std::string(what) + ": " + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ")";
     return e == hipErrorOutOfMemory ? TDAG_ENOMEM : TDAG_EDEVICE;
 }
-
-#define CAPI_HIP(expr, what)                 \
-    do {                                     \
-        const hipError_t e_ = (expr);        \
-        if (e_ != hipSuccess)                \
-            return failHip(e_, what);        \
-    } while (0)
 
 tdag_status ensureBuffer(void*& buf, size_t& have, size_t need, const char* what)
 {
@@ -88,7 +68,10 @@ tdag_status checkArgs(tdag_context* ctx, uint32_t level, uint64_t n, const tdag_
     return TDAG_OK;
 }
 
-} // namespace
+} // namespace capi
+} // namespace tdag
+
+using namespace tdag::capi;
 
 // A batched matcher call: per-level plans, its own device scratch, the uploaded BatchDesc and,
 // when requested, the captured hipGraph of the whole call.

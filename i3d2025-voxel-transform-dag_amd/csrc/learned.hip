@@ -1,0 +1,959 @@
+// Learned proposal for the level-2 matcher on gfx950 (no reference counterpart).
+//
+//  * Features: 64 mask bits and 12 transform-invariant descriptor bytes (per-axis occupancy
+//    histograms, each trimmed of zero ends and left-aligned, replaced by the smaller of itself
+//    and its reverse, the three sorted: invariant under the 48 mirror/permutation transforms and
+//    unclipped translations), as bf16 rows of 80 (zero-padded).
+//  * MLP 80 -> 256 -> 256 -> 256 -> 128 (ReLU), L2-normalised. Every dense layer, forward and
+//    backward, is one bf16 MFMA GEMM (v_mfma_f32_16x16x16_bf16, fp32 accumulate, 64x64 tiles).
+//  * Training: InfoNCE over (query, matched parent) pairs emitted by the exact matcher, Adam on
+//    fp32 master weights; the similarity/softmax part runs in fp32 on the vector units.
+//  * Matching: proposals = the k most similar earlier-ranked leaves (bf16 MFMA scores, per-query
+//    top-k heaps); every proposal is verified bitwise against all 48 transforms and the possible
+//    translations; queries with no verified proposal go to the exhaustive search with only
+//    them in the child table. Every accepted match is therefore exact.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <random>
+#include <vector>
+
+#include "common.hpp"
+#include "learned.hpp"
+#include "primitives.hpp"
+#include "search_device.hpp"
+
+namespace tdag {
+namespace learn {
+
+namespace {
+
+using namespace dev;
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+#define LEARN_TRY(expr)                  \
+    do {                                 \
+        const hipError_t e_ = (expr);    \
+        if (e_ != hipSuccess)            \
+            return e_;                   \
+    } while (0)
+
+inline size_t cdiv(size_t a, size_t b) { return (a + b - 1) / b; }
+inline size_t align256(size_t v) { return (v + 255) / 256 * 256; }
+
+// float -> bf16 bits, round to nearest, ties to even (finite inputs).
+__host__ __device__ __forceinline__ uint16_t f2bf(float f)
+{
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+// ---------------------------------------------------------------------------------------
+// Features
+// ---------------------------------------------------------------------------------------
+
+// Per-axis occupancy histogram of a 4^3 mask along axis a (0: x, 1: y, 2: z).
+__device__ __forceinline__ void axisHist(uint64_t m, int a, uint32_t h[4])
+{
+    for (int v = 0; v < 4; ++v) {
+        uint64_t plane;
+        if (a == 0)
+            plane = rangeMaskX4(v, v + 1);
+        else if (a == 1)
+            plane = rangeMaskY4(v, v + 1);
+        else
+            plane = rangeMaskZ4(v, v + 1);
+        h[v] = (uint32_t)__popcll(m & plane);
+    }
+}
+
+// Lexicographic comparison of two 4-vectors.
+__device__ __forceinline__ bool lexLess(const uint32_t* a, const uint32_t* b)
+{
+    for (int i = 0; i < 4; ++i)
+        if (a[i] != b[i])
+            return a[i] < b[i];
+    return false;
+}
+
+__global__ __launch_bounds__(256) void kFeatures(const uint64_t* __restrict__ masks, uint32_t n, uint16_t* __restrict__ X)
+{
+    const uint32_t r = blockIdx.x * 256u + threadIdx.x;
+    if (r >= n)
+        return;
+    const uint64_t m = masks[r];
+    uint16_t* row = X + (size_t)r * kIn;
+    for (int i = 0; i < 64; ++i)
+        row[i] = ((m >> i) & 1ull) ? 0x3F80u : 0u; // 1.0 or 0.0
+    uint32_t h[3][4];
+    for (int a = 0; a < 3; ++a) {
+        uint32_t raw[4];
+        axisHist(m, a, raw);
+        int lo = 4, hi = -1;
+        for (int v = 0; v < 4; ++v)
+            if (raw[v]) {
+                lo = lo < v ? lo : v;
+                hi = v;
+            }
+        // span trimmed of zero ends, left-aligned, forwards and reversed; keep the smaller
+        uint32_t fwd[4] = { 0, 0, 0, 0 }, rev[4] = { 0, 0, 0, 0 };
+        for (int v = lo; v <= hi; ++v) {
+            fwd[v - lo] = raw[v];
+            rev[hi - v] = raw[v];
+        }
+        const bool useRev = lexLess(rev, fwd);
+        for (int i = 0; i < 4; ++i)
+            h[a][i] = useRev ? rev[i] : fwd[i];
+    }
+    // sort the three canonical histograms (3-element sorting network)
+    auto swapIf = [&](int i, int j) {
+        if (lexLess(h[j], h[i]))
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t t = h[i][k];
+                h[i][k] = h[j][k];
+                h[j][k] = t;
+            }
+    };
+    swapIf(0, 1);
+    swapIf(1, 2);
+    swapIf(0, 1);
+    for (int a = 0; a < 3; ++a)
+        for (int i = 0; i < 4; ++i)
+            row[64 + 4 * a + i] = f2bf((float)h[a][i] * (1.0f / 16.0f));
+    for (int i = 76; i < kIn; ++i)
+        row[i] = 0u;
+}
+
+__global__ __launch_bounds__(256) void kGatherPairs(const uint64_t* __restrict__ q, const uint64_t* __restrict__ p, const uint32_t* __restrict__ idx, uint32_t B,
+    uint64_t* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < B) {
+        out[i] = q[idx[i]];
+        out[B + i] = p[idx[i]];
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// bf16 MFMA GEMM: C[M x N] (fp32) = opA[M x K] * opB[K x N].
+//   opA[m][k] = TA ? A[k * lda + m] : A[m * lda + k];  opB[k][n] = TB ? B[n * ldb + k] : B[k * ldb + n].
+// 256 threads per 64x64 tile of C; waves in 2x2, each 32x32 = 2x2 v_mfma_f32_16x16x16_bf16 tiles.
+// Each 16-deep K slice of both operands is staged in LDS (B transposed so a lane's four k values
+// are contiguous); out-of-range elements read as 0 and out-of-range C is not written.
+// MFMA 16x16x16 operand layout: lane l holds A[l % 16][4 (l / 16) + i] and B[4 (l / 16) + i][l % 16]
+// for i < 4, and receives D[4 (l / 16) + i][l % 16].
+// ---------------------------------------------------------------------------------------
+constexpr int kTile = 64;
+constexpr int kKs = 16;
+constexpr int kLdsPitch = kKs + 4; // bf16 elements per staged row (padding against bank conflicts)
+
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void kGemm(const uint16_t* __restrict__ A, int lda, const uint16_t* __restrict__ B, int ldb, float* __restrict__ C, int ldc, int M, int N,
+    int K)
+{
+    __shared__ __align__(16) uint16_t sA[kTile][kLdsPitch];
+    __shared__ __align__(16) uint16_t sB[kTile][kLdsPitch]; // sB[n][k]
+    const int t = (int)threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
+    const int m0 = (int)blockIdx.y * kTile, n0 = (int)blockIdx.x * kTile;
+    const int wr = wave >> 1, wc = wave & 1;
+    f32x4 acc[2][2];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            acc[i][j] = f32x4 { 0.f, 0.f, 0.f, 0.f };
+    const int lr = t >> 2, lk = (t & 3) * 4; // staging: row lr, k lk..lk+3
+    for (int k0 = 0; k0 < K; k0 += kKs) {
+        for (int i = 0; i < 4; ++i) {
+            const int m = m0 + lr, k = k0 + lk + i;
+            uint16_t va = 0;
+            if (m < M && k < K)
+                va = TA ? A[(size_t)k * lda + m] : A[(size_t)m * lda + k];
+            sA[lr][lk + i] = va;
+            const int nn = n0 + lr;
+            uint16_t vb = 0;
+            if (nn < N && k < K)
+                vb = TB ? B[(size_t)nn * ldb + k] : B[(size_t)k * ldb + nn];
+            sB[lr][lk + i] = vb;
+        }
+        __syncthreads();
+        const int kq = 4 * (lane >> 4);
+        bf16x4 a[2], b[2];
+        for (int i = 0; i < 2; ++i) {
+            const uint16_t* pa = &sA[wr * 32 + i * 16 + (lane & 15)][kq];
+            const uint16_t* pb = &sB[wc * 32 + i * 16 + (lane & 15)][kq];
+            a[i] = bf16x4 { (short)pa[0], (short)pa[1], (short)pa[2], (short)pa[3] };
+            b[i] = bf16x4 { (short)pb[0], (short)pb[1], (short)pb[2], (short)pb[3] };
+        }
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+        __syncthreads();
+    }
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            for (int r = 0; r < 4; ++r) {
+                const int m = m0 + wr * 32 + i * 16 + 4 * (lane >> 4) + r;
+                const int nn = n0 + wc * 32 + j * 16 + (lane & 15);
+                if (m < M && nn < N)
+                    C[(size_t)m * ldc + nn] = acc[i][j][r];
+            }
+}
+
+hipError_t gemm(bool ta, bool tb, const uint16_t* A, int lda, const uint16_t* B, int ldb, float* C, int ldc, int M, int N, int K, hipStream_t s)
+{
+    if (M <= 0 || N <= 0)
+        return hipSuccess;
+    const dim3 grid((unsigned)cdiv(N, kTile), (unsigned)cdiv(M, kTile));
+    if (!ta && !tb)
+        hipLaunchKernelGGL((kGemm<false, false>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, M, N, K);
+    else if (ta && !tb)
+        hipLaunchKernelGGL((kGemm<true, false>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, M, N, K);
+    else if (!ta && tb)
+        hipLaunchKernelGGL((kGemm<false, true>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, M, N, K);
+    else
+        hipLaunchKernelGGL((kGemm<true, true>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, M, N, K);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------
+// Elementwise parts of the forward and backward passes
+// ---------------------------------------------------------------------------------------
+
+// out = C + bias, optionally ReLU; written as bf16 (hidden layers) and/or fp32 (last layer).
+__global__ __launch_bounds__(256) void kBiasAct(const float* __restrict__ C, const float* __restrict__ bias, uint32_t M, uint32_t N, int relu, uint16_t* __restrict__ outBf,
+    float* __restrict__ outF)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (size_t)M * N)
+        return;
+    float v = C[i] + bias[i % N];
+    if (relu)
+        v = v > 0.f ? v : 0.f;
+    if (outBf)
+        outBf[i] = f2bf(v);
+    if (outF)
+        outF[i] = v;
+}
+
+// e = z / |z| per row of 128; invNorm[r] = 1 / |z|. One wave per row.
+__global__ __launch_bounds__(256) void kNormalize(const float* __restrict__ Z, uint32_t rows, float* __restrict__ E, float* __restrict__ invNorm)
+{
+    const uint32_t r = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63u;
+    if (r >= rows)
+        return;
+    const float a = Z[(size_t)r * kEmb + lane], b = Z[(size_t)r * kEmb + 64 + lane];
+    float ss = a * a + b * b;
+    for (int off = 32; off > 0; off >>= 1)
+        ss += __shfl_xor(ss, off, 64);
+    const float inv = 1.0f / fmaxf(sqrtf(ss), 1e-20f);
+    E[(size_t)r * kEmb + lane] = a * inv;
+    E[(size_t)r * kEmb + 64 + lane] = b * inv;
+    if (lane == 0)
+        invNorm[r] = inv;
+}
+
+// InfoNCE, rows: one block per query i. S_ij = <e_q_i, e_p_j> / tau; loss_i = logsumexp_j S_ij -
+// S_ii; G_ij = (softmax_j S_ij - [i == j]) / (B tau) is stored, and dE_q[i] = sum_j G_ij e_p_j.
+__global__ __launch_bounds__(256) void kInfoNceRows(const float* __restrict__ E, uint32_t B, float invTau, float* __restrict__ G, float* __restrict__ dE, float* __restrict__ loss)
+{
+    extern __shared__ float s_dyn[];
+    float* S = s_dyn; // B scores
+    __shared__ float s_red[256];
+    const uint32_t i = blockIdx.x;
+    const float* q = E + (size_t)i * kEmb;
+    for (uint32_t j = threadIdx.x; j < B; j += 256) {
+        const float* p = E + (size_t)(B + j) * kEmb;
+        float d = 0.f;
+        for (int k = 0; k < kEmb; ++k)
+            d += q[k] * p[k];
+        S[j] = d * invTau;
+    }
+    __syncthreads();
+    float mx = -INFINITY;
+    for (uint32_t j = threadIdx.x; j < B; j += 256)
+        mx = fmaxf(mx, S[j]);
+    s_red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            s_red[threadIdx.x] = fmaxf(s_red[threadIdx.x], s_red[threadIdx.x + off]);
+        __syncthreads();
+    }
+    mx = s_red[0];
+    __syncthreads();
+    float sum = 0.f;
+    for (uint32_t j = threadIdx.x; j < B; j += 256)
+        sum += expf(S[j] - mx);
+    s_red[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            s_red[threadIdx.x] += s_red[threadIdx.x + off];
+        __syncthreads();
+    }
+    sum = s_red[0];
+    const float scale = invTau / (float)B;
+    for (uint32_t j = threadIdx.x; j < B; j += 256) {
+        const float pj = expf(S[j] - mx) / sum;
+        G[(size_t)i * B + j] = (pj - (j == i ? 1.f : 0.f)) * scale;
+    }
+    if (threadIdx.x == 0)
+        loss[i] = logf(sum) + mx - S[i];
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < kEmb; k += 256) {
+        float d = 0.f;
+        for (uint32_t j = 0; j < B; ++j)
+            d += G[(size_t)i * B + j] * E[(size_t)(B + j) * kEmb + k];
+        dE[(size_t)i * kEmb + k] = d;
+    }
+}
+
+// InfoNCE, columns: dE_p[j] = sum_i G_ij e_q_i. One block per positive j.
+__global__ __launch_bounds__(128) void kInfoNceCols(const float* __restrict__ E, uint32_t B, const float* __restrict__ G, float* __restrict__ dE)
+{
+    const uint32_t j = blockIdx.x;
+    const uint32_t k = threadIdx.x;
+    float d = 0.f;
+    for (uint32_t i = 0; i < B; ++i)
+        d += G[(size_t)i * B + j] * E[(size_t)i * kEmb + k];
+    dE[(size_t)(B + j) * kEmb + k] = d;
+}
+
+// Through the normalisation: dz = (de - e (e . de)) / |z|; fp32 and bf16 copies. One wave per row.
+__global__ __launch_bounds__(256) void kNormBackward(const float* __restrict__ dE, const float* __restrict__ E, const float* __restrict__ invNorm, uint32_t rows,
+    float* __restrict__ dZ, uint16_t* __restrict__ dZbf)
+{
+    const uint32_t r = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63u;
+    if (r >= rows)
+        return;
+    const size_t o = (size_t)r * kEmb;
+    const float e0 = E[o + lane], e1 = E[o + 64 + lane], d0 = dE[o + lane], d1 = dE[o + 64 + lane];
+    float dot = e0 * d0 + e1 * d1;
+    for (int off = 32; off > 0; off >>= 1)
+        dot += __shfl_xor(dot, off, 64);
+    const float inv = invNorm[r];
+    const float z0 = (d0 - e0 * dot) * inv, z1 = (d1 - e1 * dot) * inv;
+    dZ[o + lane] = z0;
+    dZ[o + 64 + lane] = z1;
+    dZbf[o + lane] = f2bf(z0);
+    dZbf[o + 64 + lane] = f2bf(z1);
+}
+
+// ReLU backward: g = dH * (H > 0) (H = the layer's bf16 ReLU output); fp32 in place and bf16.
+__global__ __launch_bounds__(256) void kReluBack(float* __restrict__ dH, const uint16_t* __restrict__ H, size_t count, uint16_t* __restrict__ outBf)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= count)
+        return;
+    const float v = (H[i] != 0u && (H[i] & 0x8000u) == 0u) ? dH[i] : 0.f;
+    dH[i] = v;
+    outBf[i] = f2bf(v);
+}
+
+// out[c] = sum over rows of X[r][c] (bias gradient). One thread per column, 256 per block.
+__global__ __launch_bounds__(256) void kColSum(const float* __restrict__ X, uint32_t rows, uint32_t cols, float* __restrict__ out)
+{
+    const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+    if (c >= cols)
+        return;
+    float s = 0.f;
+    for (uint32_t r = 0; r < rows; ++r)
+        s += X[(size_t)r * cols + c];
+    out[c] = s;
+}
+
+__global__ __launch_bounds__(256) void kCastBf(const float* __restrict__ x, size_t n, uint16_t* __restrict__ out)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n)
+        out[i] = f2bf(x[i]);
+}
+
+__global__ __launch_bounds__(256) void kAdam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v, uint16_t* __restrict__ wbf, size_t n,
+    float lr, float b1, float b2, float eps, float c1, float c2)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n)
+        return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    w[i] = wi;
+    wbf[i] = f2bf(wi);
+}
+
+// ---------------------------------------------------------------------------------------
+// Workspace of one forward/backward pass over `rows` feature rows.
+// ---------------------------------------------------------------------------------------
+struct Work {
+    uint64_t* masks;
+    uint16_t* act[kLayers]; // layer inputs (bf16): features, h1, h2, h3
+    float* Z;
+    float* E;
+    float* invNorm;
+    float* C; // GEMM output scratch (rows x 256)
+    float* dE;
+    float* G; // B x B
+    float* loss;
+    uint16_t* gbf; // back-propagated signal (bf16)
+    uint32_t* idx;
+};
+
+size_t workBytes(size_t rows, size_t B)
+{
+    size_t s = 0;
+    s += align256(rows * 8);
+    s += align256(rows * kIn * 2) + 3 * align256(rows * 256 * 2);
+    s += 3 * align256(rows * kEmb * 4) + align256(rows * 4);
+    s += align256(rows * 256 * 4);
+    s += align256(B * B * 4) + align256(B * 4);
+    s += align256(rows * 256 * 2);
+    s += align256(B * 4);
+    return s;
+}
+
+Work carve(void* base, size_t rows, size_t B)
+{
+    char* p = (char*)base;
+    auto take = [&](size_t bytes) {
+        char* r = p;
+        p += align256(bytes);
+        return (void*)r;
+    };
+    Work w {};
+    w.masks = (uint64_t*)take(rows * 8);
+    w.act[0] = (uint16_t*)take(rows * kIn * 2);
+    for (int l = 1; l < kLayers; ++l)
+        w.act[l] = (uint16_t*)take(rows * 256 * 2);
+    w.Z = (float*)take(rows * kEmb * 4);
+    w.E = (float*)take(rows * kEmb * 4);
+    w.dE = (float*)take(rows * kEmb * 4);
+    w.invNorm = (float*)take(rows * 4);
+    w.C = (float*)take(rows * 256 * 4);
+    w.G = (float*)take(B * B * 4);
+    w.loss = (float*)take(B * 4);
+    w.gbf = (uint16_t*)take(rows * 256 * 2);
+    w.idx = (uint32_t*)take(B * 4);
+    return w;
+}
+
+hipError_t ensureWork(Mlp& mlp, size_t rows, size_t B)
+{
+    const size_t need = workBytes(rows, B);
+    if (mlp.wsBytes >= need)
+        return hipSuccess;
+    if (mlp.ws)
+        (void)hipFree(mlp.ws);
+    mlp.ws = nullptr;
+    mlp.wsBytes = 0;
+    LEARN_TRY(hipMalloc(&mlp.ws, need));
+    mlp.wsBytes = need;
+    return hipSuccess;
+}
+
+// Forward pass of `rows` rows whose features are in w.act[0]; leaves Z, E, invNorm.
+hipError_t forward(const Mlp& mlp, Work& w, uint32_t rows, hipStream_t s)
+{
+    for (int l = 0; l < kLayers; ++l) {
+        const int in = kLayerIn[l], out = kLayerOut[l];
+        LEARN_TRY(gemm(false, false, w.act[l], in, mlp.wbf + paramOffsetW(l), out, w.C, out, (int)rows, out, in, s));
+        const size_t cnt = (size_t)rows * out;
+        if (l + 1 < kLayers)
+            hipLaunchKernelGGL(kBiasAct, dim3((unsigned)cdiv(cnt, 256)), dim3(256), 0, s, w.C, mlp.w + paramOffsetB(l), rows, (uint32_t)out, 1, w.act[l + 1], nullptr);
+        else
+            hipLaunchKernelGGL(kBiasAct, dim3((unsigned)cdiv(cnt, 256)), dim3(256), 0, s, w.C, mlp.w + paramOffsetB(l), rows, (uint32_t)out, 0, nullptr, w.Z);
+        LEARN_TRY(hipGetLastError());
+    }
+    hipLaunchKernelGGL(kNormalize, dim3((unsigned)cdiv(rows, 4)), dim3(256), 0, s, w.Z, rows, w.E, w.invNorm);
+    return hipGetLastError();
+}
+
+// Loss and gradients (into mlp.g) for 2B rows of features in w.act[0].
+hipError_t lossGrad(Mlp& mlp, Work& w, uint32_t B, float tau, hipStream_t s)
+{
+    const uint32_t rows = 2 * B;
+    LEARN_TRY(forward(mlp, w, rows, s));
+    const size_t shm = (size_t)B * 4;
+    hipLaunchKernelGGL(kInfoNceRows, dim3(B), dim3(256), shm, s, w.E, B, 1.f / tau, w.G, w.dE, w.loss);
+    LEARN_TRY(hipGetLastError());
+    hipLaunchKernelGGL(kInfoNceCols, dim3(B), dim3(128), 0, s, w.E, B, w.G, w.dE);
+    LEARN_TRY(hipGetLastError());
+    // g (fp32) lives in w.C, its bf16 copy in w.gbf
+    hipLaunchKernelGGL(kNormBackward, dim3((unsigned)cdiv(rows, 4)), dim3(256), 0, s, w.dE, w.E, w.invNorm, rows, w.C, w.gbf);
+    LEARN_TRY(hipGetLastError());
+    for (int l = kLayers - 1; l >= 0; --l) {
+        const int in = kLayerIn[l], out = kLayerOut[l];
+        // dW_l = act_l^T g ; db_l = column sums of g
+        LEARN_TRY(gemm(true, false, w.act[l], in, w.gbf, out, mlp.g + paramOffsetW(l), out, in, out, (int)rows, s));
+        hipLaunchKernelGGL(kColSum, dim3((unsigned)cdiv(out, 256)), dim3(256), 0, s, w.C, rows, (uint32_t)out, mlp.g + paramOffsetB(l));
+        LEARN_TRY(hipGetLastError());
+        if (l > 0) {
+            // dH = g W_l^T, masked by the ReLU of act_l; becomes the next g
+            LEARN_TRY(gemm(false, true, w.gbf, out, mlp.wbf + paramOffsetW(l), out, w.C, in, (int)rows, in, out, s));
+            const size_t cnt = (size_t)rows * in;
+            hipLaunchKernelGGL(kReluBack, dim3((unsigned)cdiv(cnt, 256)), dim3(256), 0, s, w.C, w.act[l], cnt, w.gbf);
+            LEARN_TRY(hipGetLastError());
+        }
+    }
+    return hipSuccess;
+}
+
+// ---------------------------------------------------------------------------------------
+// Proposals and verification (level 2, ranks in popcount-descending order)
+// ---------------------------------------------------------------------------------------
+
+// Per block: 64 queries [q0, q0 + 64). For every 64-candidate tile of earlier ranks, the 64 x 64
+// scores come from MFMA (bf16 embeddings, K = 128) into LDS; then thread q keeps the k best
+// candidates of rank < q in a min-heap (LDS, candidate-major so the 64 threads hit 64 banks).
+constexpr int kQ = 64;
+
+__global__ __launch_bounds__(256) void kTopK(const uint16_t* __restrict__ Ebf, uint32_t n, uint32_t k, uint32_t* __restrict__ cand)
+{
+    __shared__ float s_score[kQ][kQ + 1]; // [candidate][query]
+    __shared__ float s_heapS[kMaxProposals][kQ];
+    __shared__ uint32_t s_heapI[kMaxProposals][kQ];
+    const int t = (int)threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
+    const uint32_t q0 = blockIdx.x * kQ;
+    const int wr = wave >> 1, wc = wave & 1;
+    uint32_t size = 0; // heap size of query t (t < 64)
+    for (uint32_t c0 = 0; c0 < q0 + kQ && c0 < n; c0 += kQ) {
+        f32x4 acc[2][2];
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j)
+                acc[i][j] = f32x4 { 0.f, 0.f, 0.f, 0.f };
+        for (int kk = 0; kk < kEmb; kk += 16) {
+            const int kq = kk + 4 * (lane >> 4);
+            bf16x4 a[2], b[2];
+            for (int i = 0; i < 2; ++i) {
+                const uint32_t qr = q0 + wr * 32 + i * 16 + (lane & 15);
+                const uint32_t cr = c0 + wc * 32 + i * 16 + (lane & 15);
+                const uint16_t* pa = Ebf + (size_t)min(qr, n - 1) * kEmb + kq;
+                const uint16_t* pb = Ebf + (size_t)min(cr, n - 1) * kEmb + kq;
+                a[i] = bf16x4 { (short)pa[0], (short)pa[1], (short)pa[2], (short)pa[3] };
+                b[i] = bf16x4 { (short)pb[0], (short)pb[1], (short)pb[2], (short)pb[3] };
+            }
+            for (int i = 0; i < 2; ++i)
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j)
+                for (int r = 0; r < 4; ++r)
+                    s_score[wc * 32 + j * 16 + (lane & 15)][wr * 32 + i * 16 + 4 * (lane >> 4) + r] = acc[i][j][r];
+        __syncthreads();
+        if (t < kQ) {
+            const uint32_t q = q0 + (uint32_t)t;
+            for (int j = 0; j < kQ; ++j) {
+                const uint32_t c = c0 + (uint32_t)j;
+                if (c >= q || q >= n)
+                    break; // candidates must rank before the query
+                const float sc = s_score[j][t];
+                if (size < k) { // heap not full: sift up
+                    uint32_t pos = size++;
+                    while (pos > 0) {
+                        const uint32_t par = (pos - 1) >> 1;
+                        if (s_heapS[par][t] <= sc)
+                            break;
+                        s_heapS[pos][t] = s_heapS[par][t];
+                        s_heapI[pos][t] = s_heapI[par][t];
+                        pos = par;
+                    }
+                    s_heapS[pos][t] = sc;
+                    s_heapI[pos][t] = c;
+                } else if (sc > s_heapS[0][t]) { // replace the smallest: sift down
+                    uint32_t pos = 0;
+                    for (;;) {
+                        const uint32_t l = 2 * pos + 1;
+                        if (l >= k)
+                            break;
+                        uint32_t ch = l;
+                        if (l + 1 < k && s_heapS[l + 1][t] < s_heapS[l][t])
+                            ch = l + 1;
+                        if (s_heapS[ch][t] >= sc)
+                            break;
+                        s_heapS[pos][t] = s_heapS[ch][t];
+                        s_heapI[pos][t] = s_heapI[ch][t];
+                        pos = ch;
+                    }
+                    s_heapS[pos][t] = sc;
+                    s_heapI[pos][t] = c;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (t < kQ) {
+        const uint32_t q = q0 + (uint32_t)t;
+        if (q < n)
+            for (uint32_t s = 0; s < k; ++s)
+                cand[(size_t)q * k + s] = s < size ? s_heapI[s][t] : 0xFFFFFFFFu;
+    }
+}
+
+// Baseline proposals without a network: the k leaves ranked just before the query (the nearest
+// popcounts at or above the query's).
+__global__ __launch_bounds__(256) void kNeighbourProposals(uint32_t n, uint32_t k, uint32_t* __restrict__ cand)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (size_t)n * k)
+        return;
+    const uint32_t q = (uint32_t)(i / k), s = (uint32_t)(i % k);
+    cand[i] = s < q ? q - 1 - s : 0xFFFFFFFFu;
+}
+
+// Lowest / highest occupied coordinate of a non-empty 4^3 mask along each axis.
+__device__ __forceinline__ void bbox4(uint64_t m, int lo[3], int hi[3])
+{
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = 4;
+        hi[a] = -1;
+        for (int v = 0; v < 4; ++v) {
+            const uint64_t plane = a == 0 ? rangeMaskX4(v, v + 1) : (a == 1 ? rangeMaskY4(v, v + 1) : rangeMaskZ4(v, v + 1));
+            if (m & plane) {
+                lo[a] = min(lo[a], v);
+                hi[a] = v;
+            }
+        }
+    }
+}
+
+// permute (find_translation.cpp:109-125): out(o) = P(v) with v[p[a]] = o[a].
+__device__ __forceinline__ uint64_t permute4(uint64_t P, uint32_t pc)
+{
+    uint64_t out = 0;
+    const uint32_t a0 = permAxis(pc, 0), a1 = permAxis(pc, 1), a2 = permAxis(pc, 2);
+    for (uint32_t o = 0; o < 64; ++o) {
+        int v[3];
+        v[a0] = (int)(o & 3u);
+        v[a1] = (int)((o >> 2) & 3u);
+        v[a2] = (int)(o >> 4);
+        out |= ((P >> (v[0] + 4 * v[1] + 16 * v[2])) & 1ull) << o;
+    }
+    return out;
+}
+
+// One thread per (query, proposal): the smallest key of a transform T with C == T(P_r) (perm,
+// then mirror, then the clipping translation), merged into best[q]. For each axis one side of
+// the translated grid is not clipped, so t_a is C.lo_a - M.lo_a or C.hi_a - M.hi_a.
+__global__ __launch_bounds__(256) void kVerifyProposals(const uint64_t* __restrict__ sortedMasks, uint32_t n, uint32_t k, const uint32_t* __restrict__ cand, uint32_t cfgBits,
+    uint64_t* __restrict__ best, unsigned long long* __restrict__ hits)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (size_t)n * k)
+        return;
+    const uint32_t q = (uint32_t)(i / k);
+    const uint32_t r = cand[i];
+    if (r == 0xFFFFFFFFu || r >= q)
+        return;
+    const uint64_t C = sortedMasks[q], P = sortedMasks[r];
+    if (C == 0 || P == 0 || __popcll(P) < __popcll(C))
+        return;
+    int clo[3], chi[3];
+    bbox4(C, clo, chi);
+    uint64_t bestKey = ~0ull;
+    const uint32_t numPerms = (cfgBits & kCfgAxis) ? 6u : 1u;
+    const uint32_t numSyms = (cfgBits & kCfgSymmetry) ? 8u : 1u;
+    for (uint32_t pc = 0; pc < numPerms; ++pc) {
+        const uint64_t Q = permute4(P, pc);
+        for (uint32_t sym = 0; sym < numSyms; ++sym) {
+            const uint64_t M = mirror4(Q, sym);
+            int mlo[3], mhi[3];
+            bbox4(M, mlo, mhi);
+            int cands[3][2];
+            int nc[3];
+            for (int a = 0; a < 3; ++a) {
+                cands[a][0] = clo[a] - mlo[a];
+                cands[a][1] = chi[a] - mhi[a];
+                nc[a] = cands[a][1] != cands[a][0] ? 2 : 1;
+            }
+            for (int ix = 0; ix < nc[0]; ++ix)
+                for (int iy = 0; iy < nc[1]; ++iy)
+                    for (int iz = 0; iz < nc[2]; ++iz) {
+                        const int tx = cands[0][ix], ty = cands[1][iy], tz = cands[2][iz];
+                        if (!configAllows(cfgBits, pc, sym, tx, ty, tz))
+                            continue;
+                        if (translate4(M, tx, ty, tz) != C)
+                            continue;
+                        const uint64_t key = encodeKey(r, pc, sym, tx, ty, tz, 2);
+                        bestKey = key < bestKey ? key : bestKey;
+                    }
+        }
+    }
+    if (bestKey != ~0ull) {
+        const unsigned long long prev = atomicMin((unsigned long long*)&best[q], (unsigned long long)bestKey);
+        if (prev == (unsigned long long)selfKey(q, 2))
+            atomicAdd(hits, 1ull); // first verified proposal of this query
+    }
+}
+
+// Fallback flags: non-empty queries whose best key is still the self-match.
+__global__ __launch_bounds__(256) void kFallbackFlags(const uint64_t* __restrict__ sortedMasks, const uint64_t* __restrict__ best, uint32_t n, uint32_t* __restrict__ flags,
+    unsigned long long* __restrict__ count)
+{
+    const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+    if (q >= n)
+        return;
+    const bool f = sortedMasks[q] != 0 && best[q] == selfKey(q, 2);
+    flags[q] = f ? 1u : 0u;
+    if (f)
+        atomicAdd(count, 1ull);
+}
+
+} // namespace
+
+size_t learnedWorkBytes(uint64_t n)
+{
+    // proposals + flags + Ebf + counters, then the embedding workspace is in the Mlp
+    return align256(n * kMaxProposals * 4) + align256(n * 4) + align256((n + kQ) * kEmb * 2) + 256;
+}
+
+hipError_t gemmBf16(bool ta, bool tb, const uint16_t* A, int lda, const uint16_t* B, int ldb, float* C, int ldc, int M, int N, int K, hipStream_t stream)
+{
+    return gemm(ta, tb, A, lda, B, ldb, C, ldc, M, N, K, stream);
+}
+
+hipError_t mlpAlloc(Mlp& mlp)
+{
+    LEARN_TRY(hipMalloc(&mlp.w, kParams * 4));
+    LEARN_TRY(hipMalloc(&mlp.wbf, kParams * 2));
+    LEARN_TRY(hipMalloc(&mlp.g, kParams * 4));
+    LEARN_TRY(hipMalloc(&mlp.m, kParams * 4));
+    LEARN_TRY(hipMalloc(&mlp.v, kParams * 4));
+    return hipSuccess;
+}
+
+void mlpFree(Mlp& mlp)
+{
+    for (void* p : { (void*)mlp.w, (void*)mlp.wbf, (void*)mlp.g, (void*)mlp.m, (void*)mlp.v, mlp.ws })
+        if (p)
+            (void)hipFree(p);
+    mlp = Mlp {};
+}
+
+hipError_t mlpSetParams(Mlp& mlp, const float* host, hipStream_t s)
+{
+    LEARN_TRY(hipMemcpyAsync(mlp.w, host, kParams * 4, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(kCastBf, dim3((unsigned)cdiv(kParams, 256)), dim3(256), 0, s, mlp.w, kParams, mlp.wbf);
+    LEARN_TRY(hipGetLastError());
+    LEARN_TRY(fillU32((uint32_t*)mlp.m, kParams, 0u, s));
+    LEARN_TRY(fillU32((uint32_t*)mlp.v, kParams, 0u, s));
+    mlp.step = 0;
+    return hipStreamSynchronize(s);
+}
+
+hipError_t mlpGetParams(const Mlp& mlp, float* host, hipStream_t s)
+{
+    LEARN_TRY(hipMemcpyAsync(host, mlp.w, kParams * 4, hipMemcpyDeviceToHost, s));
+    return hipStreamSynchronize(s);
+}
+
+hipError_t mlpInit(Mlp& mlp, uint64_t seed, hipStream_t s)
+{
+    std::vector<float> h(kParams, 0.f);
+    std::mt19937_64 rng(seed);
+    for (int l = 0; l < kLayers; ++l) {
+        const float a = std::sqrt(6.0f / (float)kLayerIn[l]);
+        std::uniform_real_distribution<float> U(-a, a);
+        float* W = h.data() + paramOffsetW(l);
+        for (size_t i = 0; i < (size_t)kLayerIn[l] * kLayerOut[l]; ++i)
+            W[i] = U(rng);
+    }
+    return mlpSetParams(mlp, h.data(), s);
+}
+
+hipError_t mlpLossGrad(Mlp& mlp, const uint64_t* d_q, const uint64_t* d_p, uint32_t B, float tau, float* out_loss, hipStream_t s)
+{
+    if (B == 0)
+        return hipErrorInvalidValue;
+    LEARN_TRY(ensureWork(mlp, 2 * (size_t)B, B));
+    Work w = carve(mlp.ws, 2 * (size_t)B, B);
+    LEARN_TRY(hipMemcpyAsync(w.masks, d_q, (size_t)B * 8, hipMemcpyDeviceToDevice, s));
+    LEARN_TRY(hipMemcpyAsync(w.masks + B, d_p, (size_t)B * 8, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(kFeatures, dim3((unsigned)cdiv(2 * B, 256)), dim3(256), 0, s, w.masks, 2 * B, w.act[0]);
+    LEARN_TRY(hipGetLastError());
+    LEARN_TRY(lossGrad(mlp, w, B, tau, s));
+    if (out_loss) {
+        std::vector<float> l(B);
+        LEARN_TRY(hipMemcpyAsync(l.data(), w.loss, (size_t)B * 4, hipMemcpyDeviceToHost, s));
+        LEARN_TRY(hipStreamSynchronize(s));
+        double sum = 0;
+        for (float v : l)
+            sum += v;
+        *out_loss = (float)(sum / B);
+    }
+    return hipSuccess;
+}
+
+hipError_t mlpTrain(Mlp& mlp, const uint64_t* d_q, const uint64_t* d_p, uint64_t n, uint32_t steps, uint32_t batch, float lr, float tau, uint64_t seed, float* out_losses,
+    hipStream_t s)
+{
+    if (n < 2 || batch < 2)
+        return hipErrorInvalidValue;
+    const uint32_t B = (uint32_t)std::min<uint64_t>(batch, n);
+    LEARN_TRY(ensureWork(mlp, 2 * (size_t)B, B));
+    Work w = carve(mlp.ws, 2 * (size_t)B, B);
+    std::vector<uint32_t> perm(n);
+    std::iota(perm.begin(), perm.end(), 0u);
+    std::mt19937_64 rng(seed);
+    uint64_t pos = n;
+    std::vector<uint32_t> idx(B);
+    std::vector<float> lossHost(B);
+    uint32_t* pinnedIdx = nullptr;
+    LEARN_TRY(hipHostMalloc((void**)&pinnedIdx, (size_t)B * 4 * 2));
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    hipError_t err = hipSuccess;
+    for (uint32_t st = 0; st < steps && err == hipSuccess; ++st) {
+        if (pos + B > n) { // new epoch
+            std::shuffle(perm.begin(), perm.end(), rng);
+            pos = 0;
+        }
+        uint32_t* slot = pinnedIdx + (st & 1u) * B; // double-buffered: the previous copy may be in flight
+        if (st >= 2)
+            err = hipStreamSynchronize(s);
+        std::copy(perm.begin() + pos, perm.begin() + pos + B, slot);
+        pos += B;
+        if (err == hipSuccess)
+            err = hipMemcpyAsync(w.idx, slot, (size_t)B * 4, hipMemcpyHostToDevice, s);
+        if (err != hipSuccess)
+            break;
+        hipLaunchKernelGGL(kGatherPairs, dim3((unsigned)cdiv(B, 256)), dim3(256), 0, s, d_q, d_p, w.idx, B, w.masks);
+        hipLaunchKernelGGL(kFeatures, dim3((unsigned)cdiv(2 * B, 256)), dim3(256), 0, s, w.masks, 2 * B, w.act[0]);
+        if ((err = hipGetLastError()) != hipSuccess)
+            break;
+        if ((err = lossGrad(mlp, w, B, tau, s)) != hipSuccess)
+            break;
+        ++mlp.step;
+        const float c1 = 1.f - std::pow(b1, (float)mlp.step), c2 = 1.f - std::pow(b2, (float)mlp.step);
+        hipLaunchKernelGGL(kAdam, dim3((unsigned)cdiv(kParams, 256)), dim3(256), 0, s, mlp.w, mlp.g, mlp.m, mlp.v, mlp.wbf, kParams, lr, b1, b2, eps, c1, c2);
+        if ((err = hipGetLastError()) != hipSuccess)
+            break;
+        if (out_losses) {
+            if ((err = hipMemcpyAsync(lossHost.data(), w.loss, (size_t)B * 4, hipMemcpyDeviceToHost, s)) != hipSuccess)
+                break;
+            if ((err = hipStreamSynchronize(s)) != hipSuccess)
+                break;
+            double sum = 0;
+            for (float v : lossHost)
+                sum += v;
+            out_losses[st] = (float)(sum / B);
+        }
+    }
+    const hipError_t e2 = hipStreamSynchronize(s);
+    (void)hipHostFree(pinnedIdx);
+    return err != hipSuccess ? err : e2;
+}
+
+hipError_t mlpEmbed(Mlp& mlp, const uint64_t* d_masks, uint64_t n, float* d_out, hipStream_t s)
+{
+    if (n == 0)
+        return hipSuccess;
+    // Chunks of at most 65536 rows keep the workspace bounded.
+    const uint64_t chunk = std::min<uint64_t>(n, 65536);
+    LEARN_TRY(ensureWork(mlp, chunk, 2));
+    Work w = carve(mlp.ws, chunk, 2);
+    for (uint64_t r0 = 0; r0 < n; r0 += chunk) {
+        const uint32_t rows = (uint32_t)std::min<uint64_t>(chunk, n - r0);
+        hipLaunchKernelGGL(kFeatures, dim3((unsigned)cdiv(rows, 256)), dim3(256), 0, s, d_masks + r0, rows, w.act[0]);
+        LEARN_TRY(hipGetLastError());
+        LEARN_TRY(forward(mlp, w, rows, s));
+        LEARN_TRY(hipMemcpyAsync(d_out + r0 * kEmb, w.E, (size_t)rows * kEmb * 4, hipMemcpyDeviceToDevice, s));
+    }
+    return hipSuccess;
+}
+
+hipError_t matchLearned2(const MatchPlan& p, Mlp* mlp, uint32_t k, const uint64_t* d_masks, uint32_t cfgBits, void* d_scratch, void* d_work, hipStream_t s,
+    uint64_t* d_encoded, uint64_t* d_mapping, uint32_t* d_unique, LearnedStats* stats)
+{
+    if (p.level != 2 || k == 0 || k > kMaxProposals)
+        return hipErrorInvalidValue;
+    const uint32_t n = (uint32_t)p.n;
+    const LevelView v = levelView(p, d_scratch);
+    char* wb = (char*)d_work;
+    uint32_t* cand = (uint32_t*)wb;
+    uint32_t* flags = (uint32_t*)(wb + align256((size_t)n * kMaxProposals * 4));
+    uint16_t* Ebf = (uint16_t*)((char*)flags + align256((size_t)n * 4));
+    unsigned long long* cnt = (unsigned long long*)((char*)Ebf + align256(((size_t)n + kQ) * kEmb * 2));
+    hipEvent_t ev[5];
+    for (auto& e : ev)
+        LEARN_TRY(hipEventCreate(&e));
+    auto cleanup = [&](hipError_t e) {
+        for (auto& x : ev)
+            (void)hipEventDestroy(x);
+        return e;
+    };
+    hipError_t e;
+    if ((e = fillU64((uint64_t*)cnt, 2, 0ull, s)) != hipSuccess)
+        return cleanup(e);
+    if ((e = enqueuePrepareOrder(p, d_masks, d_scratch, s)) != hipSuccess)
+        return cleanup(e);
+    (void)hipEventRecord(ev[0], s);
+    if (n > 0) {
+        // 1. Proposals.
+        if (mlp) {
+            float* E = nullptr;
+            if ((e = hipMallocAsync((void**)&E, (size_t)n * kEmb * 4, s)) != hipSuccess)
+                return cleanup(e);
+            e = mlpEmbed(*mlp, v.sortedMasks, n, E, s);
+            if (e == hipSuccess) {
+                hipLaunchKernelGGL(kCastBf, dim3((unsigned)cdiv((size_t)n * kEmb, 256)), dim3(256), 0, s, E, (size_t)n * kEmb, Ebf);
+                e = hipGetLastError();
+            }
+            (void)hipFreeAsync(E, s);
+            if (e != hipSuccess)
+                return cleanup(e);
+            (void)hipEventRecord(ev[1], s);
+            hipLaunchKernelGGL(kTopK, dim3((unsigned)cdiv(n, kQ)), dim3(256), 0, s, Ebf, n, k, cand);
+        } else {
+            (void)hipEventRecord(ev[1], s);
+            hipLaunchKernelGGL(kNeighbourProposals, dim3((unsigned)cdiv((size_t)n * k, 256)), dim3(256), 0, s, n, k, cand);
+        }
+        if ((e = hipGetLastError()) != hipSuccess)
+            return cleanup(e);
+        (void)hipEventRecord(ev[2], s);
+        // 2. Bitwise verification of every proposal.
+        hipLaunchKernelGGL(kVerifyProposals, dim3((unsigned)cdiv((size_t)n * k, 256)), dim3(256), 0, s, v.sortedMasks, n, k, cand, cfgBits, v.best, cnt);
+        hipLaunchKernelGGL(kFallbackFlags, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, s, v.sortedMasks, v.best, n, flags, cnt + 1);
+        if ((e = hipGetLastError()) != hipSuccess)
+            return cleanup(e);
+        (void)hipEventRecord(ev[3], s);
+        // 3. Exhaustive search for the queries without a verified proposal.
+        if ((e = enqueueBuildTable(p, d_scratch, s, flags)) != hipSuccess)
+            return cleanup(e);
+        if ((e = enqueueSearch(p, cfgBits, d_scratch, s)) != hipSuccess)
+            return cleanup(e);
+        (void)hipEventRecord(ev[4], s);
+    }
+    if ((e = enqueueFinish(p, d_scratch, s, d_encoded, d_mapping, d_unique)) != hipSuccess)
+        return cleanup(e);
+    if ((e = hipStreamSynchronize(s)) != hipSuccess)
+        return cleanup(e);
+    if (stats) {
+        unsigned long long c[2] = { 0, 0 };
+        if ((e = hipMemcpy(c, cnt, 16, hipMemcpyDeviceToHost)) != hipSuccess)
+            return cleanup(e);
+        stats->proposalHits = c[0];
+        stats->fallback = c[1];
+        if (n > 0) {
+            (void)hipEventElapsedTime(&stats->msEmbed, ev[0], ev[1]);
+            (void)hipEventElapsedTime(&stats->msPropose, ev[1], ev[2]);
+            (void)hipEventElapsedTime(&stats->msVerify, ev[2], ev[3]);
+            (void)hipEventElapsedTime(&stats->msFallback, ev[3], ev[4]);
+        }
+    }
+    return cleanup(hipSuccess);
+}
+
+} // namespace learn
+} // namespace tdag

@@ -85,8 +85,9 @@ __global__ __launch_bounds__(256) void kGatherInit(const uint64_t* __restrict__ 
 // ---------------------------------------------------------------------------------------
 
 // Corner k (bit a set = Positive along axis a) sees the grid mirrored on its Negative axes.
+// childFlags (optional): only ranks with a non-zero flag enter the table as children.
 __global__ __launch_bounds__(256) void kBuildTable2(const uint64_t* __restrict__ sortedMasks, uint32_t n, uint64_t* __restrict__ tabKeys, uint32_t* __restrict__ tabVals, uint32_t logCap,
-    uint32_t* __restrict__ bloom, uint32_t logBloom)
+    uint32_t* __restrict__ bloom, uint32_t logBloom, const uint32_t* __restrict__ childFlags)
 {
     const uint32_t gid = blockIdx.x * 256u + threadIdx.x;
     if (gid >= n * 8u)
@@ -94,7 +95,7 @@ __global__ __launch_bounds__(256) void kBuildTable2(const uint64_t* __restrict__
     const uint32_t r = gid >> 3;
     const uint32_t corner = gid & 7u;
     const uint64_t m = sortedMasks[r];
-    if (m == 0)
+    if (m == 0 || (childFlags && childFlags[r] == 0u))
         return;
     const uint64_t key = mirror4(m, (~corner) & 7u);
     const uint32_t f = bloomMix64(key);
@@ -504,7 +505,7 @@ uint64_t pruneKey(uint32_t level, uint32_t r1)
     return encodeKey(r1, 0, 0, 1 - R, 1 - R, 1 - R, level);
 }
 
-hipError_t enqueuePrepare(const MatchPlan& p, const uint64_t* d_masks, void* d_scratch, hipStream_t stream)
+hipError_t enqueuePrepareOrder(const MatchPlan& p, const uint64_t* d_masks, void* d_scratch, hipStream_t stream)
 {
     char* base = (char*)d_scratch;
     uint32_t* keys = (uint32_t*)(base + p.offKeys);
@@ -512,7 +513,6 @@ hipError_t enqueuePrepare(const MatchPlan& p, const uint64_t* d_masks, void* d_s
     const LevelView v = levelView(p, d_scratch);
     const uint32_t n = v.n;
     const uint32_t W = p.W;
-    const uint64_t cap = 1ull << p.logCap;
 
     TDAG_TRY(fillU64((uint64_t*)v.counters, kNumCounters, 0ull, stream));
     if (n == 0)
@@ -536,13 +536,25 @@ hipError_t enqueuePrepare(const MatchPlan& p, const uint64_t* d_masks, void* d_s
         hipLaunchKernelGGL(kGatherInit, dim3(blocks), dim3(256), 0, stream, d_masks, order, keys, n, W, p.R3, p.level, v.sortedMasks, v.best);
         TDAG_TRY(hipGetLastError());
     }
+    return hipSuccess;
+}
+
+hipError_t enqueueBuildTable(const MatchPlan& p, void* d_scratch, hipStream_t stream, const uint32_t* childFlags)
+{
+    const LevelView v = levelView(p, d_scratch);
+    const uint32_t n = v.n;
+    const uint64_t cap = 1ull << p.logCap;
+    if (n == 0)
+        return hipSuccess;
+    if (childFlags && p.level != 2)
+        return hipErrorInvalidValue;
     // 3. Child table and its Bloom filter (levels >= 3 first build the transposed bit layouts).
     TDAG_TRY(fillU64(v.tabKeys, cap, 0ull, stream));
     TDAG_TRY(fillU32(v.bloom, 1ull << p.logBloom, 0u, stream));
     switch (p.level) {
     case 2:
         hipLaunchKernelGGL(kBuildTable2, dim3((unsigned)ceilDiv((size_t)n * 8, 256)), dim3(256), 0, stream, v.sortedMasks, n, v.tabKeys, v.tabVals, p.logCap, v.bloom,
-            p.logBloom);
+            p.logBloom, childFlags);
         break;
     case 3:
         hipLaunchKernelGGL(kLayouts<3>, dim3(n), dim3(256), 0, stream, v.sortedMasks, n);
@@ -560,6 +572,12 @@ hipError_t enqueuePrepare(const MatchPlan& p, const uint64_t* d_masks, void* d_s
         return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t enqueuePrepare(const MatchPlan& p, const uint64_t* d_masks, void* d_scratch, hipStream_t stream)
+{
+    TDAG_TRY(enqueuePrepareOrder(p, d_masks, d_scratch, stream));
+    return enqueueBuildTable(p, d_scratch, stream, nullptr);
 }
 
 hipError_t enqueueSearch(const MatchPlan& p, uint32_t cfgBits, void* d_scratch, hipStream_t stream)

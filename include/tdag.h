@@ -187,6 +187,55 @@ tdag_status tdag_batch_destroy(tdag_batch* b);
 tdag_status tdag_match_levels_batched_device(tdag_context* ctx, const tdag_level_job* jobs, uint32_t count, const tdag_config* cfg, tdag_match_stats* out_stats);
 
 /* ------------------------------------------------------------------------------------------
+ * Learned proposal (level 2, 4^3 leaves). An MLP embeds each leaf (64 mask bits plus 12
+ * transform-invariant descriptor bytes: per-axis occupancy histograms, each trimmed of zero ends
+ * and left-aligned, replaced by the smaller of itself and its reverse, the three sorted) into
+ * 128 dimensions; 80 -> 256 -> 256 ->
+ * 256 -> 128, ReLU, L2-normalised, bf16 MFMA with fp32 accumulate in training and inference.
+ * It is trained with InfoNCE on (query, matched parent) pairs from the exact matcher. Matching:
+ * the k (<= 64) most similar earlier-ranked leaves are proposed, each proposal is verified
+ * bitwise under all 48 transforms and the possible translations, and queries without a
+ * verified proposal fall back to the exhaustive search. Every accepted match is exact; the
+ * chosen parent can differ from the exhaustive minimum when the minimum is not proposed. No
+ * reference counterpart.
+ * ---------------------------------------------------------------------------------------- */
+typedef struct tdag_mlp tdag_mlp;
+
+typedef struct tdag_learned_stats {
+    uint64_t num_items;
+    uint64_t num_unique;
+    uint64_t num_matched;
+    uint64_t proposal_hits; /* queries resolved by a verified proposal */
+    uint64_t fallback_queries; /* non-empty queries sent to the exhaustive search */
+    float ms_embed, ms_propose, ms_verify, ms_fallback, ms_total;
+} tdag_learned_stats;
+
+/* Number of fp32 parameters (W_l row-major in x out, then b_l, for l = 0..3). */
+uint64_t tdag_mlp_param_count(void);
+/* A network on ctx's device with He-uniform weights from `seed` and zero biases. */
+tdag_status tdag_mlp_create(tdag_context* ctx, uint64_t seed, tdag_mlp** out);
+tdag_status tdag_mlp_destroy(tdag_mlp* m);
+tdag_status tdag_mlp_get_params(tdag_mlp* m, float* out /* host, param_count floats */);
+/* Replaces the parameters (host, param_count floats) and resets the optimiser state. */
+tdag_status tdag_mlp_set_params(tdag_mlp* m, const float* in);
+/* InfoNCE loss over `batch` pairs (d_q[i], d_p[i]) (device 4^3 masks) at temperature tau, and its
+ * gradient (host, param_count floats, may be NULL). Parameters are not changed. */
+tdag_status tdag_mlp_loss_grad(tdag_mlp* m, const uint64_t* d_q, const uint64_t* d_p, uint32_t batch, float tau, float* out_loss, float* out_grad);
+/* `steps` Adam steps (learning rate lr) on batches of `batch` pairs drawn from the n_pairs pairs
+ * (a seeded shuffle per epoch); out_losses (steps floats, may be NULL) receives each step's loss. */
+tdag_status tdag_mlp_train(tdag_mlp* m, const uint64_t* d_q, const uint64_t* d_p, uint64_t n_pairs, uint32_t steps, uint32_t batch, float lr, float tau, uint64_t seed,
+    float* out_losses);
+/* L2-normalised embeddings (device, n x 128 fp32) of n device 4^3 masks. */
+tdag_status tdag_mlp_embed(tdag_mlp* m, const uint64_t* d_masks, uint64_t n, float* d_out);
+/* The bf16 MFMA GEMM of the network, exposed for tests: C (m x n, fp32, row-major, ldc) =
+ * op(A) op(B), op(X) = X or X^T of row-major bf16 (raw uint16) device matrices. */
+tdag_status tdag_gemm_bf16(tdag_context* ctx, int trans_a, int trans_b, const uint16_t* d_a, int lda, const uint16_t* d_b, int ldb, float* d_c, int ldc, int m, int n, int k);
+/* Level-2 matcher with proposals from `mlp` (NULL: the k leaves ranked just before each query,
+ * i.e. popcount-neighbour proposals without a network). Outputs as tdag_match_level_device. */
+tdag_status tdag_match_level_learned_device(tdag_context* ctx, tdag_mlp* mlp, uint32_t level, const uint64_t* d_masks, uint64_t n, const tdag_config* cfg, uint32_t k,
+    uint64_t* d_encoded, uint64_t* d_mapping, uint32_t* d_unique, tdag_learned_stats* stats);
+
+/* ------------------------------------------------------------------------------------------
  * Pipeline around the matcher: scene synthesis, SVDAG producer, flattening, the
  * constructTransformDAG driver and the .tdag writer.
  * ---------------------------------------------------------------------------------------- */
