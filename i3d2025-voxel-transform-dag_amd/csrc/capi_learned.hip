@@ -158,4 +158,41 @@ tdag_status tdag_match_level_learned_device(tdag_context* ctx, tdag_mlp* mlp, ui
     return TDAG_OK;
 }
 
+tdag_status tdag_match_level_learned(tdag_context* ctx, tdag_mlp* mlp, uint32_t level, const uint64_t* masks, uint64_t n, const tdag_config* cfg, uint32_t k,
+    uint64_t* out_encoded, uint64_t* out_mapping, uint32_t* out_unique, uint64_t* out_num_unique, tdag_learned_stats* stats)
+{
+    if (tdag_status s = checkArgs(ctx, level, n, cfg); s != TDAG_OK)
+        return s;
+    if (n > 0 && (!masks || !out_mapping || !out_unique))
+        return fail(TDAG_EINVAL, "masks, out_mapping and out_unique are required");
+    CAPI_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    const size_t W = level >= 2 ? std::max<size_t>(1, ((size_t)1 << (3 * level)) / 64) : 1;
+    const size_t a = ((size_t)n * W * 8 + 255) / 256 * 256;
+    const size_t b = ((size_t)n * 8 + 255) / 256 * 256;
+    const size_t c = ((size_t)n * 4 + 255) / 256 * 256;
+    if (tdag_status s = ensureBuffer(ctx->io, ctx->ioBytes, a + 2 * b + c, "io buffer allocation"); s != TDAG_OK)
+        return s;
+    uint64_t* dMasks = (uint64_t*)ctx->io;
+    uint64_t* dEncoded = (uint64_t*)((char*)ctx->io + a);
+    uint64_t* dMapping = (uint64_t*)((char*)ctx->io + a + b);
+    uint32_t* dUnique = (uint32_t*)((char*)ctx->io + a + 2 * b);
+    if (n > 0)
+        CAPI_HIP(hipMemcpyAsync(dMasks, masks, (size_t)n * W * 8, hipMemcpyHostToDevice, ctx->stream), "mask upload");
+    tdag_learned_stats st {};
+    if (tdag_status s = tdag_match_level_learned_device(ctx, mlp, level, dMasks, n, cfg, k, out_encoded ? dEncoded : nullptr, dMapping, dUnique, &st); s != TDAG_OK)
+        return s;
+    if (n > 0) {
+        if (out_encoded)
+            CAPI_HIP(hipMemcpyAsync(out_encoded, dEncoded, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream), "encoded readback");
+        CAPI_HIP(hipMemcpyAsync(out_mapping, dMapping, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream), "mapping readback");
+        CAPI_HIP(hipMemcpyAsync(out_unique, dUnique, (size_t)st.num_unique * 4, hipMemcpyDeviceToHost, ctx->stream), "unique readback");
+        CAPI_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    }
+    if (out_num_unique)
+        *out_num_unique = st.num_unique;
+    if (stats)
+        *stats = st;
+    return TDAG_OK;
+}
+
 } // extern "C"

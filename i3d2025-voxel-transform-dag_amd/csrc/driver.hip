@@ -120,7 +120,8 @@ tdag_status tdag_svdag_free(tdag_svdag* h)
     return TDAG_OK;
 }
 
-tdag_status tdag_construct_transform_dag(tdag_context* ctx, const tdag_svdag* in, const tdag_config* cfg, uint32_t flags, tdag_tdag** out)
+static tdag_status constructTransformDag(tdag_context* ctx, const tdag_svdag* in, const tdag_config* cfg, uint32_t flags, bool learned, tdag_mlp* mlp, uint32_t k,
+    tdag_tdag** out)
 {
     if (!in || !cfg || !out)
         return setErr(TDAG_EINVAL, "null argument");
@@ -172,7 +173,18 @@ tdag_status tdag_construct_transform_dag(tdag_context* ctx, const tdag_svdag* in
             std::vector<uint32_t> uniq(indices.size());
             uint64_t nu = 0;
             tdag_match_stats st {};
-            const tdag_status s = tdag_match_level(ctx, level, rec.masks.data(), indices.size(), cfg, nullptr, mapping.data(), uniq.data(), &nu, &st);
+            tdag_status s;
+            if (learned && level == 2) {
+                tdag_learned_stats ls {};
+                s = tdag_match_level_learned(ctx, mlp, level, rec.masks.data(), indices.size(), cfg, k, nullptr, mapping.data(), uniq.data(), &nu, &ls);
+                st.num_matched = ls.num_matched;
+                st.ms_total = ls.ms_total;
+                rec.learned = true;
+                rec.proposalHits = ls.proposal_hits;
+                rec.fallback = ls.fallback_queries;
+            } else {
+                s = tdag_match_level(ctx, level, rec.masks.data(), indices.size(), cfg, nullptr, mapping.data(), uniq.data(), &nu, &st);
+            }
             if (s != TDAG_OK) {
                 delete res;
                 char msg[512];
@@ -210,6 +222,21 @@ tdag_status tdag_construct_transform_dag(tdag_context* ctx, const tdag_svdag* in
     return TDAG_OK;
 }
 
+tdag_status tdag_construct_transform_dag(tdag_context* ctx, const tdag_svdag* in, const tdag_config* cfg, uint32_t flags, tdag_tdag** out)
+{
+    return constructTransformDag(ctx, in, cfg, flags, false, nullptr, 0, out);
+}
+
+tdag_status tdag_construct_transform_dag_learned(tdag_context* ctx, const tdag_svdag* in, const tdag_config* cfg, uint32_t flags, tdag_mlp* mlp, uint32_t k,
+    tdag_tdag** out)
+{
+    if (!ctx || !mlp)
+        return setErr(TDAG_EINVAL, "the learned driver needs a context and a network");
+    if (k == 0 || k > 64)
+        return setErr(TDAG_EINVAL, "k must be in [1, 64]");
+    return constructTransformDag(ctx, in, cfg, flags, true, mlp, k, out);
+}
+
 tdag_status tdag_tdag_serialize(const tdag_tdag* t, const tdag_config* cfg, uint8_t* buf, uint64_t cap, uint64_t* out_size)
 {
     if (!t || !cfg || !out_size)
@@ -236,6 +263,9 @@ tdag_status tdag_tdag_level_record(const tdag_tdag* t, uint32_t level, tdag_leve
             out->matched = r.matched;
             out->ms_match = r.msMatch;
             out->has_masks = r.masks.empty() ? 0u : 1u;
+            out->learned = r.learned ? 1u : 0u;
+            out->proposal_hits = r.proposalHits;
+            out->fallback_queries = r.fallback;
             return TDAG_OK;
         }
     return setErr(TDAG_EINVAL, "no matcher record for level " + std::to_string(level));

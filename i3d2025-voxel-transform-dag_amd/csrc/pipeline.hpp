@@ -39,6 +39,8 @@ struct LevelRecord {
     uint64_t unique = 0;
     uint64_t matched = 0; // items whose parent is another item
     float msMatch = 0.f; // device time of the matcher call
+    bool learned = false; // matched with the learned proposal
+    uint64_t proposalHits = 0, fallback = 0;
     std::vector<uint64_t> masks; // the matcher's input (kept when requested)
 };
 

@@ -29,6 +29,9 @@ class LevelRecord(ctypes.Structure):
         ("unique", ctypes.c_uint64),
         ("matched", ctypes.c_uint64),
         ("ms_match", ctypes.c_float),
+        ("learned", ctypes.c_uint32),
+        ("proposal_hits", ctypes.c_uint64),
+        ("fallback_queries", ctypes.c_uint64),
     ]
 
 
@@ -41,6 +44,8 @@ def bind(L):
     L.tdag_svdag_flatten.argtypes = [vp, ctypes.c_uint32, _u32p, ctypes.c_uint64, _u64p]
     L.tdag_svdag_free.argtypes = [vp]
     L.tdag_construct_transform_dag.argtypes = [vp, vp, ctypes.POINTER(TdagConfig), ctypes.c_uint32, ctypes.POINTER(vp)]
+    L.tdag_construct_transform_dag_learned.argtypes = [vp, vp, ctypes.POINTER(TdagConfig), ctypes.c_uint32, vp, ctypes.c_uint32, ctypes.POINTER(vp)]
+    L.tdag_construct_transform_dag_learned.restype = ctypes.c_int
     L.tdag_tdag_serialize.argtypes = [vp, ctypes.POINTER(TdagConfig), _u8p, ctypes.c_uint64, _u64p]
     L.tdag_tdag_info.argtypes = [vp, _u32p, _u64p, ctypes.c_uint32]
     L.tdag_tdag_level_record.argtypes = [vp, ctypes.c_uint32, ctypes.POINTER(LevelRecord)]
@@ -152,6 +157,15 @@ class TDag:
             self.close()
         except Exception:
             pass
+
+
+def construct_transform_dag_learned(ctx, svdag: Svdag, config: TransformDAGConfig, mlp, k: int = 64, keep_level_inputs: bool = False) -> TDag:
+    """constructTransformDAG with the learned proposal (a tdag_amd.learned.MLP) at level 2."""
+    cfg = config.to_c()
+    h = ctypes.c_void_p()
+    check(lib().tdag_construct_transform_dag_learned(ctx._ptr, svdag.handle, ctypes.byref(cfg), KEEP_LEVEL_INPUTS if keep_level_inputs else 0, mlp._ptr,
+                                                     int(k), ctypes.byref(h)), "tdag_construct_transform_dag_learned")
+    return TDag(h, config)
 
 
 def construct_transform_dag(ctx, svdag: Svdag, config: TransformDAGConfig, keep_level_inputs: bool = False) -> TDag:

@@ -234,6 +234,9 @@ tdag_status tdag_gemm_bf16(tdag_context* ctx, int trans_a, int trans_b, const ui
  * i.e. popcount-neighbour proposals without a network). Outputs as tdag_match_level_device. */
 tdag_status tdag_match_level_learned_device(tdag_context* ctx, tdag_mlp* mlp, uint32_t level, const uint64_t* d_masks, uint64_t n, const tdag_config* cfg, uint32_t k,
     uint64_t* d_encoded, uint64_t* d_mapping, uint32_t* d_unique, tdag_learned_stats* stats);
+/* Host-buffer form of tdag_match_level_learned_device (as tdag_match_level). */
+tdag_status tdag_match_level_learned(tdag_context* ctx, tdag_mlp* mlp, uint32_t level, const uint64_t* masks, uint64_t n, const tdag_config* cfg, uint32_t k,
+    uint64_t* out_encoded, uint64_t* out_mapping, uint32_t* out_unique, uint64_t* out_num_unique, tdag_learned_stats* stats);
 
 /* ------------------------------------------------------------------------------------------
  * Pipeline around the matcher: scene synthesis, SVDAG producer, flattening, the
@@ -254,6 +257,9 @@ typedef struct tdag_level_record {
     uint64_t unique;
     uint64_t matched;
     float ms_match;
+    uint32_t learned; /* level 2 of tdag_construct_transform_dag_learned: 1 */
+    uint64_t proposal_hits; /* learned: queries resolved by a verified proposal */
+    uint64_t fallback_queries; /* learned: queries sent to the exhaustive search */
 } tdag_level_record;
 
 /* Flag of tdag_construct_transform_dag: keep each level's matcher input masks. */
@@ -282,6 +288,11 @@ tdag_status tdag_svdag_free(tdag_svdag* svdag);
  * when cfg->translation is 0 (no matcher call is made). */
 tdag_status tdag_construct_transform_dag(tdag_context* ctx, const tdag_svdag* in, const tdag_config* cfg, uint32_t flags, tdag_tdag** out);
 /* The .tdag byte stream (voxelizer/src/main.cpp:144-151): pass buf = NULL to query the size. */
+/* constructTransformDAG with the learned proposal at level 2 (mlp; k proposals per query) and the
+ * exhaustive matcher at levels 3..maxT. Every match is exact; the output can differ from
+ * tdag_construct_transform_dag where a query's minimum-key parent was not proposed. */
+tdag_status tdag_construct_transform_dag_learned(tdag_context* ctx, const tdag_svdag* in, const tdag_config* cfg, uint32_t flags, tdag_mlp* mlp, uint32_t k,
+    tdag_tdag** out);
 tdag_status tdag_tdag_serialize(const tdag_tdag* tdag, const tdag_config* cfg, uint8_t* buf, uint64_t cap, uint64_t* out_size);
 tdag_status tdag_tdag_info(const tdag_tdag* tdag, uint32_t* root_level, uint64_t* level_sizes, uint32_t cap);
 tdag_status tdag_tdag_level_record(const tdag_tdag* tdag, uint32_t level, tdag_level_record* out);

@@ -255,3 +255,40 @@ def test_trained_mlp_beats_popcount_neighbours(oracle, gpu_ctx):
     assert st_mlp.num_matched == st_base.num_matched  # the fallback completes both
     assert st_mlp.proposal_hits > 2 * st_base.proposal_hits
     mlp.close()
+
+
+def test_learned_driver_decodes_exactly(oracle, gpu_ctx):
+    """constructTransformDAG with the learned proposal at level 2: the .tdag expands back to the
+    scene's voxels exactly (every match exact) and stays close in size to the exhaustive one."""
+    import torch
+
+    import tdag_amd
+    import tdag_decode
+    from tdag_amd import learned
+    from tdag_amd import pipeline as pl
+
+    R, maxT = 128, 3
+    cfg = tdag_amd.TransformDAGConfig(True, True, True, maxT)
+    # training pairs: the exact matcher on another seeded scene's level-2 inputs
+    tr = pl.construct_transform_dag(gpu_ctx, pl.Svdag(pl.scene_dense(1, R, 21), R), cfg, keep_level_inputs=True)
+    m2 = tr.level_masks(2).reshape(-1)
+    enc = oracle.match_encoded(2, m2, oracle.config(True, True, True, maxT))
+    q, p = learned.pairs_from_encoded(m2, enc)
+    d_q = torch.from_numpy(q.view(np.int64).copy()).cuda()
+    d_p = torch.from_numpy(p.view(np.int64).copy()).cuda()
+    torch.cuda.synchronize()
+    mlp = learned.MLP(gpu_ctx, seed=11)
+    mlp.train(d_q, d_p, len(q), steps=300, batch=256, lr=1e-3, tau=0.1, seed=12)
+    bits = pl.scene_dense(1, R, 22)
+    svdag = pl.Svdag(bits, R)
+    exact = pl.construct_transform_dag(gpu_ctx, svdag, cfg).tdag_bytes()
+    t = pl.construct_transform_dag_learned(gpu_ctx, svdag, cfg, mlp, 64)
+    mine = t.tdag_bytes()
+    rec = t.record(2)
+    print(f"learned tdag {len(mine)} B vs exhaustive {len(exact)} B; level 2: items={rec.items} matched={rec.matched} "
+          f"proposal_hits={rec.proposal_hits} fallback={rec.fallback_queries}", flush=True)
+    assert rec.learned == 1
+    d = tdag_decode.parse(mine)
+    assert np.array_equal(tdag_decode.expand(d), tdag_decode.dense_to_cube(bits, R))
+    assert len(mine) <= 1.05 * len(exact)
+    mlp.close()
