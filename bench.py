@@ -49,6 +49,7 @@ def build_workload(ctx, resolution: int, seed: int, max_level: int):
         rec = tdag.record(level)
         levels[level] = {"masks": masks, "items": int(rec.items), "unique": int(rec.unique), "matched": int(rec.matched)}
     info = {
+        "svdag": svdag,
         "tdag_bytes": len(tdag.tdag_bytes()),
         "svdag_level_sizes": svdag.level_sizes,
         "tdag_level_sizes": tdag.level_sizes,
@@ -144,6 +145,84 @@ def bench_batched(ctx, cfg, dev, order, args):
     }
 
 
+def bench_learned(ctx, cfg, dev, levels, info, args):
+    """Learned proposal at level 2: trained on the exact matcher's (leaf, parent) pairs from three
+    other seeded scenes of the same size, then run on this workload's level-2 input. Reports
+    recall of the verified proposals, the matcher time against the exhaustive call and the .tdag
+    size against the exhaustive build (every accepted match is exact either way)."""
+    import numpy as np
+    import torch
+
+    from tdag_amd import learned
+    from tdag_amd import pipeline as pl
+
+    t0 = time.time()
+    qs, ps = [], []
+    for seed in (args.seed + 101, args.seed + 102, args.seed + 103):
+        bits = pl.scene_dense(1, args.resolution, seed)
+        t = pl.construct_transform_dag(ctx, pl.Svdag(bits, args.resolution), cfg, keep_level_inputs=True)
+        m2 = np.ascontiguousarray(t.level_masks(2).reshape(-1))
+        d_m = torch.from_numpy(m2.view(np.int64)).cuda()
+        d_enc = torch.empty(max(1, len(m2)), dtype=torch.int64, device="cuda")
+        torch.cuda.synchronize()
+        ctx.match_level_device(2, d_m, len(m2), cfg, d_encoded=d_enc)
+        enc = d_enc.cpu().numpy().view(np.uint64)[: len(m2)]
+        q, p = learned.pairs_from_encoded(m2, enc)
+        qs.append(q)
+        ps.append(p)
+        t.close()
+    q, p = np.concatenate(qs), np.concatenate(ps)
+    d_q = torch.from_numpy(q.view(np.int64).copy()).cuda()
+    d_p = torch.from_numpy(p.view(np.int64).copy()).cuda()
+    torch.cuda.synchronize()
+    mlp = learned.MLP(ctx, seed=args.seed)
+    t1 = time.time()
+    losses = mlp.train(d_q, d_p, len(q), steps=args.learned_steps, batch=512, lr=1e-3, tau=0.1, seed=args.seed + 7)
+    train_s = time.time() - t1
+    d = dev[2]
+    n = d["n"]
+    d_enc = torch.empty(max(1, n), dtype=torch.int64, device="cuda")
+    ref_enc = torch.empty(max(1, n), dtype=torch.int64, device="cuda")
+    reps = max(1, args.steps)
+    ms_ex = 0.0
+    for _ in range(reps):
+        _, stp = ctx.match_level_device(2, d["masks"], n, cfg, d_encoded=ref_enc)
+        ms_ex += stp.ms_total
+    learned.match_level_learned_device(ctx, mlp, d["masks"], n, cfg, 64, d_encoded=d_enc)  # warm-up
+    ms, st = 0.0, None
+    for _ in range(reps):
+        st = learned.match_level_learned_device(ctx, mlp, d["masks"], n, cfg, 64, d_encoded=d_enc)
+        ms += st.ms_total
+    base = learned.match_level_learned_device(ctx, None, d["masks"], n, cfg, 64)
+    torch.cuda.synchronize()
+    same_key = int((d_enc[:n] == ref_enc[:n]).sum().item())
+    tdag_l = pl.construct_transform_dag_learned(ctx, info["svdag"], cfg, mlp, 64)
+    learned_bytes = len(tdag_l.tdag_bytes())
+    tdag_l.close()
+    mlp.close()
+    matched = int(st.num_matched)
+    return {
+        "level": 2,
+        "train_pairs": int(len(q)),
+        "train_scenes": [args.seed + 101, args.seed + 102, args.seed + 103],
+        "train_steps": args.learned_steps,
+        "train_s": round(train_s, 2),
+        "loss_first_last": [round(float(losses[:20].mean()), 4), round(float(losses[-20:].mean()), 4)],
+        "matcher_ms": round(ms / reps, 4),
+        "exhaustive_matcher_ms": round(ms_ex / reps, 4),
+        "ms_split": {"embed": round(st.ms_embed, 4), "propose": round(st.ms_propose, 4), "verify": round(st.ms_verify, 4), "fallback": round(st.ms_fallback, 4)},
+        "matched": matched,
+        "recall_at_64": round(st.proposal_hits / matched, 4) if matched else None,
+        "recall_at_64_popcount_neighbours": round(base.proposal_hits / matched, 4) if matched else None,
+        "fallback_queries": int(st.fallback_queries),
+        "same_parent_as_exhaustive": round(same_key / n, 4) if n else None,
+        "tdag_bytes": learned_bytes,
+        "tdag_bytes_exhaustive": info["tdag_bytes"],
+        "tdag_size_delta": round(learned_bytes / info["tdag_bytes"] - 1.0, 5),
+        "setup_s": round(time.time() - t0, 2),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,6 +237,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="use the library's default search variants instead of tuning them first")
     ap.add_argument("--counters", action="store_true", help="also run each level once with per-stage probe counters (diagnostics, untimed)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched form (one persistent kernel for all levels, replayed hipGraph)")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal measurement at level 2")
+    ap.add_argument("--learned-steps", type=int, default=1500, help="training steps of the learned proposal")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB three times (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -183,7 +264,8 @@ def main():
         return
     ctx = tdag_amd.Context(0)
     cfg, levels, info = build_workload(ctx, args.resolution, args.seed, args.max_level)
-    print(f"[bench] workload ready: {info} items={ {l: v['items'] for l, v in levels.items()} }", file=sys.stderr, flush=True)
+    print(f"[bench] workload ready: { {k: v for k, v in info.items() if k != 'svdag'} } items={ {l: v['items'] for l, v in levels.items()} }", file=sys.stderr,
+          flush=True)
 
     dev = {}
     for level, lv in levels.items():
@@ -240,6 +322,7 @@ def main():
     elapsed = time.perf_counter() - t0
 
     batched = None if args.no_batched else bench_batched(ctx, cfg, dev, order, args)
+    learned_res = None if (args.no_learned or 2 not in dev) else bench_learned(ctx, cfg, dev, levels, info, args)
 
     per_step_matches = matches // max(1, args.steps)
     expected = sum(levels[l]["matched"] for l in order)
@@ -292,6 +375,7 @@ def main():
             "autotune": tuning,
             **({"probe_counters": counters} if counters else {}),
             **({"batched": batched} if batched else {}),
+            **({"learned": learned_res} if learned_res else {}),
             "parallelism": "single GPU",
         },
         "roofline": {

@@ -514,20 +514,62 @@ hipError_t lossGrad(Mlp& mlp, Work& w, uint32_t B, float tau, hipStream_t s)
 // ---------------------------------------------------------------------------------------
 
 // Per block: 64 queries [q0, q0 + 64). For every 64-candidate tile of earlier ranks, the 64 x 64
-// scores come from MFMA (bf16 embeddings, K = 128) into LDS; then thread q keeps the k best
-// candidates of rank < q in a min-heap (LDS, candidate-major so the 64 threads hit 64 banks).
+// scores come from MFMA (bf16 embeddings, K = 128) into LDS. All 256 threads then filter the tile
+// against each query's current k-th best score (4 threads per query, 16 candidates each, kept in
+// candidate order), and thread q merges its query's survivors into a min-heap of the k best
+// (LDS, candidate-major so the 64 threads hit 64 banks). The survivors are merged in a fixed order,
+// so the result does not depend on thread timing.
 constexpr int kQ = 64;
+
+__device__ __forceinline__ void heapInsert(float (*hs)[kQ], uint32_t (*hi)[kQ], uint32_t& size, uint32_t k, int t, float sc, uint32_t c)
+{
+    if (size < k) { // not full: sift up
+        uint32_t pos = size++;
+        while (pos > 0) {
+            const uint32_t par = (pos - 1) >> 1;
+            if (hs[par][t] <= sc)
+                break;
+            hs[pos][t] = hs[par][t];
+            hi[pos][t] = hi[par][t];
+            pos = par;
+        }
+        hs[pos][t] = sc;
+        hi[pos][t] = c;
+    } else if (sc > hs[0][t]) { // replace the smallest: sift down
+        uint32_t pos = 0;
+        for (;;) {
+            const uint32_t l = 2 * pos + 1;
+            if (l >= k)
+                break;
+            uint32_t ch = l;
+            if (l + 1 < k && hs[l + 1][t] < hs[l][t])
+                ch = l + 1;
+            if (hs[ch][t] >= sc)
+                break;
+            hs[pos][t] = hs[ch][t];
+            hi[pos][t] = hi[ch][t];
+            pos = ch;
+        }
+        hs[pos][t] = sc;
+        hi[pos][t] = c;
+    }
+}
 
 __global__ __launch_bounds__(256) void kTopK(const uint16_t* __restrict__ Ebf, uint32_t n, uint32_t k, uint32_t* __restrict__ cand)
 {
     __shared__ float s_score[kQ][kQ + 1]; // [candidate][query]
     __shared__ float s_heapS[kMaxProposals][kQ];
     __shared__ uint32_t s_heapI[kMaxProposals][kQ];
+    __shared__ uint8_t s_list[4][16][kQ]; // survivors per (filter thread, query), candidate-major
+    __shared__ uint8_t s_cnt[4][kQ];
+    __shared__ float s_thr[kQ];
     const int t = (int)threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
     const uint32_t q0 = blockIdx.x * kQ;
     const int wr = wave >> 1, wc = wave & 1;
     uint32_t size = 0; // heap size of query t (t < 64)
+    if (t < kQ)
+        s_thr[t] = -INFINITY;
     for (uint32_t c0 = 0; c0 < q0 + kQ && c0 < n; c0 += kQ) {
         f32x4 acc[2][2];
         for (int i = 0; i < 2; ++i)
@@ -553,44 +595,28 @@ __global__ __launch_bounds__(256) void kTopK(const uint16_t* __restrict__ Ebf, u
                 for (int r = 0; r < 4; ++r)
                     s_score[wc * 32 + j * 16 + (lane & 15)][wr * 32 + i * 16 + 4 * (lane >> 4) + r] = acc[i][j][r];
         __syncthreads();
-        if (t < kQ) {
-            const uint32_t q = q0 + (uint32_t)t;
-            for (int j = 0; j < kQ; ++j) {
+        { // filter: thread (sub, qi) scans candidates sub * 16 .. sub * 16 + 15 of query qi
+            const int qi = t & (kQ - 1), sub = t >> 6;
+            const uint32_t q = q0 + (uint32_t)qi;
+            const float thr = s_thr[qi];
+            uint32_t cnt = 0;
+            for (int jj = 0; jj < 16; ++jj) {
+                const int j = sub * 16 + jj;
                 const uint32_t c = c0 + (uint32_t)j;
-                if (c >= q || q >= n)
-                    break; // candidates must rank before the query
-                const float sc = s_score[j][t];
-                if (size < k) { // heap not full: sift up
-                    uint32_t pos = size++;
-                    while (pos > 0) {
-                        const uint32_t par = (pos - 1) >> 1;
-                        if (s_heapS[par][t] <= sc)
-                            break;
-                        s_heapS[pos][t] = s_heapS[par][t];
-                        s_heapI[pos][t] = s_heapI[par][t];
-                        pos = par;
-                    }
-                    s_heapS[pos][t] = sc;
-                    s_heapI[pos][t] = c;
-                } else if (sc > s_heapS[0][t]) { // replace the smallest: sift down
-                    uint32_t pos = 0;
-                    for (;;) {
-                        const uint32_t l = 2 * pos + 1;
-                        if (l >= k)
-                            break;
-                        uint32_t ch = l;
-                        if (l + 1 < k && s_heapS[l + 1][t] < s_heapS[l][t])
-                            ch = l + 1;
-                        if (s_heapS[ch][t] >= sc)
-                            break;
-                        s_heapS[pos][t] = s_heapS[ch][t];
-                        s_heapI[pos][t] = s_heapI[ch][t];
-                        pos = ch;
-                    }
-                    s_heapS[pos][t] = sc;
-                    s_heapI[pos][t] = c;
-                }
+                if (c < q && q < n && s_score[j][qi] > thr)
+                    s_list[sub][cnt++][qi] = (uint8_t)j;
             }
+            s_cnt[sub][qi] = (uint8_t)cnt;
+        }
+        __syncthreads();
+        if (t < kQ) {
+            for (int sub = 0; sub < 4; ++sub)
+                for (uint32_t e = 0; e < s_cnt[sub][t]; ++e) {
+                    const int j = s_list[sub][e][t];
+                    heapInsert(s_heapS, s_heapI, size, k, t, s_score[j][t], c0 + (uint32_t)j);
+                }
+            if (size == k)
+                s_thr[t] = s_heapS[0][t];
         }
         __syncthreads();
     }
@@ -644,11 +670,32 @@ __device__ __forceinline__ uint64_t permute4(uint64_t P, uint32_t pc)
     return out;
 }
 
+// Packs the lowest / highest occupied coordinate per axis (2 bits each) of a non-empty mask.
+__device__ __forceinline__ uint32_t packBox(uint64_t m)
+{
+    int lo[3], hi[3];
+    bbox4(m, lo, hi);
+    return (uint32_t)(lo[0] | lo[1] << 2 | lo[2] << 4 | hi[0] << 6 | hi[1] << 8 | hi[2] << 10);
+}
+
+// The 48 images M = mirror(permute(P_r)) of every leaf (index pc * 8 + sym) and their boxes.
+__global__ __launch_bounds__(256) void kTransforms48(const uint64_t* __restrict__ sortedMasks, uint32_t n, uint64_t* __restrict__ images, uint16_t* __restrict__ boxes)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (size_t)n * 48)
+        return;
+    const uint32_t r = (uint32_t)(i / 48), tf = (uint32_t)(i % 48);
+    const uint64_t P = sortedMasks[r];
+    const uint64_t M = P ? mirror4(permute4(P, tf >> 3), tf & 7u) : 0ull;
+    images[i] = M;
+    boxes[i] = (uint16_t)(M ? packBox(M) : 0u);
+}
+
 // One thread per (query, proposal): the smallest key of a transform T with C == T(P_r) (perm,
 // then mirror, then the clipping translation), merged into best[q]. For each axis one side of
 // the translated grid is not clipped, so t_a is C.lo_a - M.lo_a or C.hi_a - M.hi_a.
-__global__ __launch_bounds__(256) void kVerifyProposals(const uint64_t* __restrict__ sortedMasks, uint32_t n, uint32_t k, const uint32_t* __restrict__ cand, uint32_t cfgBits,
-    uint64_t* __restrict__ best, unsigned long long* __restrict__ hits)
+__global__ __launch_bounds__(256) void kVerifyProposals(const uint64_t* __restrict__ sortedMasks, const uint64_t* __restrict__ images, const uint16_t* __restrict__ boxes,
+    uint32_t n, uint32_t k, const uint32_t* __restrict__ cand, uint32_t cfgBits, uint64_t* __restrict__ best, unsigned long long* __restrict__ hits)
 {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= (size_t)n * k)
@@ -660,22 +707,23 @@ __global__ __launch_bounds__(256) void kVerifyProposals(const uint64_t* __restri
     const uint64_t C = sortedMasks[q], P = sortedMasks[r];
     if (C == 0 || P == 0 || __popcll(P) < __popcll(C))
         return;
-    int clo[3], chi[3];
-    bbox4(C, clo, chi);
+    const uint32_t cb = boxes[(size_t)q * 48]; // identity image = C itself
+    const int clo[3] = { (int)(cb & 3u), (int)((cb >> 2) & 3u), (int)((cb >> 4) & 3u) };
+    const int chi[3] = { (int)((cb >> 6) & 3u), (int)((cb >> 8) & 3u), (int)((cb >> 10) & 3u) };
     uint64_t bestKey = ~0ull;
     const uint32_t numPerms = (cfgBits & kCfgAxis) ? 6u : 1u;
     const uint32_t numSyms = (cfgBits & kCfgSymmetry) ? 8u : 1u;
+    const uint64_t* img = images + (size_t)r * 48;
+    const uint16_t* box = boxes + (size_t)r * 48;
     for (uint32_t pc = 0; pc < numPerms; ++pc) {
-        const uint64_t Q = permute4(P, pc);
         for (uint32_t sym = 0; sym < numSyms; ++sym) {
-            const uint64_t M = mirror4(Q, sym);
-            int mlo[3], mhi[3];
-            bbox4(M, mlo, mhi);
+            const uint64_t M = img[pc * 8 + sym];
+            const uint32_t mb = box[pc * 8 + sym];
             int cands[3][2];
             int nc[3];
             for (int a = 0; a < 3; ++a) {
-                cands[a][0] = clo[a] - mlo[a];
-                cands[a][1] = chi[a] - mhi[a];
+                cands[a][0] = clo[a] - (int)((mb >> (2 * a)) & 3u);
+                cands[a][1] = chi[a] - (int)((mb >> (6 + 2 * a)) & 3u);
                 nc[a] = cands[a][1] != cands[a][0] ? 2 : 1;
             }
             for (int ix = 0; ix < nc[0]; ++ix)
@@ -715,8 +763,9 @@ __global__ __launch_bounds__(256) void kFallbackFlags(const uint64_t* __restrict
 
 size_t learnedWorkBytes(uint64_t n)
 {
-    // proposals + flags + Ebf + counters, then the embedding workspace is in the Mlp
-    return align256(n * kMaxProposals * 4) + align256(n * 4) + align256((n + kQ) * kEmb * 2) + 256;
+    // proposals, flags, bf16 embeddings, counters, 48 images and boxes per leaf (the embedding
+    // workspace itself lives in the Mlp)
+    return align256(n * kMaxProposals * 4) + align256(n * 4) + align256((n + kQ) * kEmb * 2) + 256 + align256(n * 48 * 8) + align256(n * 48 * 2);
 }
 
 hipError_t gemmBf16(bool ta, bool tb, const uint16_t* A, int lda, const uint16_t* B, int ldb, float* C, int ldc, int M, int N, int K, hipStream_t stream)
@@ -885,6 +934,8 @@ hipError_t matchLearned2(const MatchPlan& p, Mlp* mlp, uint32_t k, const uint64_
     uint32_t* flags = (uint32_t*)(wb + align256((size_t)n * kMaxProposals * 4));
     uint16_t* Ebf = (uint16_t*)((char*)flags + align256((size_t)n * 4));
     unsigned long long* cnt = (unsigned long long*)((char*)Ebf + align256(((size_t)n + kQ) * kEmb * 2));
+    uint64_t* images = (uint64_t*)((char*)cnt + 256);
+    uint16_t* boxes = (uint16_t*)((char*)images + align256((size_t)n * 48 * 8));
     hipEvent_t ev[5];
     for (auto& e : ev)
         LEARN_TRY(hipEventCreate(&e));
@@ -922,8 +973,9 @@ hipError_t matchLearned2(const MatchPlan& p, Mlp* mlp, uint32_t k, const uint64_
         if ((e = hipGetLastError()) != hipSuccess)
             return cleanup(e);
         (void)hipEventRecord(ev[2], s);
-        // 2. Bitwise verification of every proposal.
-        hipLaunchKernelGGL(kVerifyProposals, dim3((unsigned)cdiv((size_t)n * k, 256)), dim3(256), 0, s, v.sortedMasks, n, k, cand, cfgBits, v.best, cnt);
+        // 2. Bitwise verification of every proposal against the 48 precomputed images.
+        hipLaunchKernelGGL(kTransforms48, dim3((unsigned)cdiv((size_t)n * 48, 256)), dim3(256), 0, s, v.sortedMasks, n, images, boxes);
+        hipLaunchKernelGGL(kVerifyProposals, dim3((unsigned)cdiv((size_t)n * k, 256)), dim3(256), 0, s, v.sortedMasks, images, boxes, n, k, cand, cfgBits, v.best, cnt);
         hipLaunchKernelGGL(kFallbackFlags, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, s, v.sortedMasks, v.best, n, flags, cnt + 1);
         if ((e = hipGetLastError()) != hipSuccess)
             return cleanup(e);

@@ -431,7 +431,7 @@ MatchPlan planMatch(uint32_t level, uint64_t n, uint32_t variant, uint32_t first
 {
     MatchPlan p {};
     p.level = level;
-    p.firstBatch = firstBatch ? firstBatch : (uint32_t)std::max<uint64_t>(256, n / 64);
+    p.firstBatch = firstBatch ? firstBatch : (uint32_t)std::max<uint64_t>(64, n / 64);
     p.batchGrowth = std::max<uint32_t>(2, batchGrowth ? batchGrowth : 2);
     p.variant = variantValid(level, variant) ? variant : defaultVariant(level);
     p.n = n;

@@ -43,7 +43,7 @@ struct MatchPlan {
     size_t scratchBytes;
 };
 
-// firstBatch = 0 / batchGrowth = 0 select the defaults (max(256, n/64) parents, doubling).
+// firstBatch = 0 / batchGrowth = 0 select the defaults (max(64, n/64) parents, doubling).
 // bloomBitsPerEntry = 0 selects defaultFilterBits(level).
 MatchPlan planMatch(uint32_t level, uint64_t n, uint32_t variant, uint32_t firstBatch = 0, uint32_t batchGrowth = 0, uint32_t bloomBitsPerEntry = 0);
 
