@@ -122,7 +122,7 @@ tdag_status tdag_context_get_variant(const tdag_context* ctx, uint32_t level, in
 
 /* Rank-batched search schedule at `level`: parents are searched in rank order, first
  * `first_batch` of them, then batches growing by `growth`; between batches the child table drops
- * every child that can no longer improve. 0 selects the defaults (max(256, n/64), 2). A
+ * every child that can no longer improve. 0 selects the defaults (max(64, n/64), 2). A
  * tuning knob with no reference counterpart (the reference prunes between fixed-size batches,
  * find_translation.cpp:771, 887-915). */
 tdag_status tdag_context_set_batching(tdag_context* ctx, uint32_t level, uint32_t first_batch, uint32_t growth);
