@@ -34,12 +34,15 @@ def bytes_per_query(level: int) -> int:
     return 384 * (1 << (3 * level))
 
 
-def build_workload(ctx, resolution: int, seed: int, max_level: int):
+SCENE_KINDS = {"scene": 1, "city": 2}
+
+
+def build_workload(ctx, resolution: int, seed: int, max_level: int, kind: int = 1):
     import tdag_amd
     from tdag_amd import pipeline as pl
 
     t0 = time.time()
-    bits = pl.scene_dense(1, resolution, seed)
+    bits = pl.scene_dense(kind, resolution, seed)
     svdag = pl.Svdag(bits, resolution)
     cfg = tdag_amd.TransformDAGConfig(True, True, True, max_level)
     tdag = pl.construct_transform_dag(ctx, svdag, cfg, keep_level_inputs=True)
@@ -158,9 +161,10 @@ def bench_learned(ctx, cfg, dev, levels, info, args):
 
     t0 = time.time()
     qs, ps = [], []
+    train_res = min(args.resolution, 1024)
     for seed in (args.seed + 101, args.seed + 102, args.seed + 103):
-        bits = pl.scene_dense(1, args.resolution, seed)
-        t = pl.construct_transform_dag(ctx, pl.Svdag(bits, args.resolution), cfg, keep_level_inputs=True)
+        bits = pl.scene_dense(SCENE_KINDS[args.workload], train_res, seed)
+        t = pl.construct_transform_dag(ctx, pl.Svdag(bits, train_res), cfg, keep_level_inputs=True)
         m2 = np.ascontiguousarray(t.level_masks(2).reshape(-1))
         d_m = torch.from_numpy(m2.view(np.int64)).cuda()
         d_enc = torch.empty(max(1, len(m2)), dtype=torch.int64, device="cuda")
@@ -204,7 +208,7 @@ def bench_learned(ctx, cfg, dev, levels, info, args):
     return {
         "level": 2,
         "train_pairs": int(len(q)),
-        "train_scenes": [args.seed + 101, args.seed + 102, args.seed + 103],
+        "train_scenes": {"workload": args.workload, "resolution": train_res, "seeds": [args.seed + 101, args.seed + 102, args.seed + 103]},
         "train_steps": args.learned_steps,
         "train_s": round(train_s, 2),
         "loss_first_last": [round(float(losses[:20].mean()), 4), round(float(losses[-20:].mean()), 4)],
@@ -229,6 +233,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--resolution", type=int, default=1024)
+    ap.add_argument("--workload", choices=sorted(SCENE_KINDS), default="scene",
+                    help="scene: spheres + boxes + heightfield (the default workload); city: the seeded city of the largest configuration")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--max-level", type=int, default=5)
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU oracle time for cpu_baseline")
@@ -263,7 +269,7 @@ def main():
         print(json.dumps({"copy_bytes_per_launch": n * 8, "read_bytes": n * 8, "write_bytes": n * 8}))
         return
     ctx = tdag_amd.Context(0)
-    cfg, levels, info = build_workload(ctx, args.resolution, args.seed, args.max_level)
+    cfg, levels, info = build_workload(ctx, args.resolution, args.seed, args.max_level, SCENE_KINDS[args.workload])
     print(f"[bench] workload ready: { {k: v for k, v in info.items() if k != 'svdag'} } items={ {l: v['items'] for l, v in levels.items()} }", file=sys.stderr,
           flush=True)
 
@@ -348,7 +354,10 @@ def main():
     tpath = os.path.join(REPO, "profiles", f"traffic_search_L{dom}.json")
     if os.path.exists(tpath):
         try:
-            traffic = json.load(open(tpath)).get("bytes_per_launch")
+            prof = json.load(open(tpath))
+            want = {"workload": args.workload, "resolution": args.resolution, "seed": args.seed, "max_level": args.max_level}
+            if prof.get("match") == want:  # the counters were taken on this exact workload
+                traffic = prof.get("bytes_per_launch")
         except Exception:
             traffic = None
 
@@ -366,7 +375,8 @@ def main():
         "dtype": "u64",
         "data": "synthetic (seeded scene generator, no external data)",
         "config": {
-            "workload": f"{args.resolution}^3 sphere+box+heightfield shells seed {args.seed}, constructTransformDAG maxTranslationLevel={args.max_level}, "
+            "workload": (f"{args.resolution}^3 sphere+box+heightfield shells" if args.workload == "scene" else f"{args.resolution}^3 city")
+                        + f" seed {args.seed}, constructTransformDAG maxTranslationLevel={args.max_level}, "
                         "exhaustive GPU matcher one launch per level (32^3,16^3,8^3,4^3)",
             "tdag_bytes": info["tdag_bytes"],
             "matches_per_step": per_step_matches,

@@ -267,7 +267,9 @@ typedef struct tdag_level_record {
 
 /* Seeded synthetic scene as a dense bit grid (bit i = x + R*y + R*R*z, the VoxelGrid layout of
  * voxel_grid.h:70-73). kind 0: 200 sphere shells + 100 box shells; kind 1: kind 0 plus a
- * fractal value-noise heightfield shell. out_bytes >= resolution^3 / 8. */
+ * fractal value-noise heightfield shell; kind 2 (R >= 256): a city of box-shell buildings with
+ * window voids on 64-voxel lots, half of its 256-voxel blocks mirrored / axis-swapped copies of
+ * others, over a noise ground (the configs directory describes it). out_bytes >= resolution^3 / 8. */
 tdag_status tdag_scene_dense(uint32_t kind, uint32_t resolution, uint64_t seed, uint8_t* out_bits, uint64_t out_bytes);
 
 /* EditStructure<void, uint32_t>(VoxelGrid) followed by toDAG() (structure.h:423-429, 658-763). */

@@ -28,7 +28,7 @@ def _oracle_bytes(oracle, bits, R, cfg):
     return data
 
 
-@pytest.mark.parametrize("kind,R,seed,maxT", [(0, 64, 1, 2), (0, 64, 2, 5), (1, 128, 3, 3), (1, 128, 4, 4), (0, 256, 0, 2)])
+@pytest.mark.parametrize("kind,R,seed,maxT", [(0, 64, 1, 2), (0, 64, 2, 5), (1, 128, 3, 3), (1, 128, 4, 4), (0, 256, 0, 2), (2, 256, 5, 3)])
 def test_tdag_bytes_match_oracle(oracle, gpu_ctx, kind, R, seed, maxT):
     import tdag_amd
     from tdag_amd import pipeline as pl

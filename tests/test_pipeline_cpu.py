@@ -16,7 +16,7 @@ import pytest
 
 import tdag_decode
 
-SCENES = [(0, 64, 1), (0, 128, 2), (1, 128, 3), (0, 256, 0)]
+SCENES = [(0, 64, 1), (0, 128, 2), (1, 128, 3), (0, 256, 0), (2, 256, 4)]
 
 
 def _oracle_svdag(oracle, bits, R):
@@ -128,3 +128,25 @@ def test_empty_scene(oracle, pl):
     cfg = tdag_amd.TransformDAGConfig(True, True, False, 3)
     mine = pl.construct_transform_dag(None, s, cfg).tdag_bytes()
     assert mine == _oracle_tdag_bytes(oracle, bits, 64, oracle.config(True, True, False, 3))
+
+
+def test_city_scene_deterministic_with_transformed_copies(oracle, pl):
+    """The city (scene kind 2) is reproducible from its seed, and its mirrored / axis-swapped
+    block copies produce matches that need a mirror or a permutation (oracle matcher, level 4)."""
+    a = pl.scene_dense(2, 512, 9)
+    assert np.array_equal(a, pl.scene_dense(2, 512, 9))
+    assert not np.array_equal(a, pl.scene_dense(2, 512, 10))
+    s = pl.Svdag(a, 512)
+    nodes = np.arange(s.level_sizes[4], dtype=np.uint32)
+    masks = s.flatten(4, nodes)
+    enc = oracle.match_encoded(4, masks, oracle.config(True, True, True))
+    b = 4 + 1
+    low = 6 + 3 * b
+    order = np.argsort(-np.array([sum(bin(int(w)).count("1") for w in m) for m in masks]), kind="stable")
+    rank_of = np.empty_like(order)
+    rank_of[order] = np.arange(len(order))
+    matched = (enc >> np.uint64(low)).astype(np.int64) != rank_of
+    sym = (enc >> np.uint64(3 * b)) & np.uint64(7)
+    perm = (enc >> np.uint64(3 * b + 3)) & np.uint64(7)
+    assert matched.sum() > 0
+    assert np.sum(matched & ((sym != 0) | (perm != 0))) > 0
