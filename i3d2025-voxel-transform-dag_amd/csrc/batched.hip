@@ -73,17 +73,18 @@ uint32_t tilesFor(uint32_t level, uint32_t parents)
 
 // Block-level tile of levels 4 and 5: all four waves search one unit of parent r.
 template <int L, int Kind>
-__device__ __forceinline__ void blockTile(const LevelView& v, uint32_t r, uint32_t unit, uint32_t chunk, unsigned char* s_mem, uint32_t cfgBits)
+__device__ __forceinline__ void blockTile(const LevelView& v, uint32_t r, uint32_t unit, uint32_t chunk, unsigned char* s_mem, uint32_t* s_occ, uint32_t cfgBits)
 {
     using G = Geo<L>;
     if (!(cfgBits & kCfgAxis) && (Kind == kUnitCombo ? (unit >> 3) : (unit >> 1)) != 0)
         return; // only permutation code 0 (p0 == 0) is allowed
     auto* s_rows = reinterpret_cast<typename G::Row*>(s_mem);
     uint32_t* planes = reinterpret_cast<uint32_t*>(s_mem + alignUp16(rowsBytes<L>()));
-    if (!loadRows<L, kBatchThreads>(v.sortedMasks, v.n, r, s_rows, 3))
+    Occupancy occ;
+    if (!loadRowsOcc<L, kBatchThreads>(v.sortedMasks, v.n, r, s_rows, s_occ, occ))
         return;
     ProbeCounts cnt;
-    searchUnit<L, kBatchWaves, Kind>(r, unit, chunk, (int)(threadIdx.x >> 6), s_rows, planes, planes, v.sortedMasks, v.tabKeys, v.logCap, v.bloom, v.logBloom, v.best,
+    searchUnit<L, kBatchWaves, Kind>(r, unit, chunk, (int)(threadIdx.x >> 6), s_rows, occ, planes, planes, v.sortedMasks, v.tabKeys, v.logCap, v.bloom, v.logBloom, v.best,
         cfgBits, cnt);
     flushCounts(cnt, v.counters, cfgBits);
 }
@@ -98,10 +99,11 @@ __device__ __forceinline__ void waveItem3(const LevelView& v, uint32_t r, uint32
     auto* s_rows = reinterpret_cast<G::Row*>(base);
     uint32_t* planes0 = reinterpret_cast<uint32_t*>(base + kL3RowsBytes);
     uint32_t* planes1 = planes0 + G::P * G::Slab;
-    if (!loadRowsWave<3>(v.sortedMasks, v.n, r, s_rows, 3))
+    Occupancy occ;
+    if (!loadRowsWave<3>(v.sortedMasks, v.n, r, s_rows, occ))
         return;
     ProbeCounts cnt;
-    searchUnit<3, 1, kUnitShared>(r, unit, 0, 0, s_rows, planes0, planes1, v.sortedMasks, v.tabKeys, v.logCap, v.bloom, v.logBloom, v.best, cfgBits, cnt);
+    searchUnit<3, 1, kUnitShared>(r, unit, 0, 0, s_rows, occ, planes0, planes1, v.sortedMasks, v.tabKeys, v.logCap, v.bloom, v.logBloom, v.best, cfgBits, cnt);
     flushCounts(cnt, v.counters, cfgBits);
 }
 
@@ -142,6 +144,7 @@ __global__ __launch_bounds__(kBatchThreads, 3) void kBatched(const BatchDesc* __
 {
     __shared__ __align__(16) unsigned char s_mem[kBatchLds];
     __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_occ[3 * kBatchWaves];
     const uint32_t numLevels = desc->numLevels;
     const uint32_t numPhases = desc->numPhases;
     const uint32_t cfgBits = desc->cfgBits;
@@ -172,11 +175,11 @@ __global__ __launch_bounds__(kBatchThreads, 3) void kBatched(const BatchDesc* __
                 switch (v.level) {
                 case 5: {
                     const uint32_t m = k % (6u * kL5Chunks);
-                    blockTile<5, kUnitFamily>(v, pl.r0 + k / (6u * kL5Chunks), m / kL5Chunks, m % kL5Chunks, s_mem, cfgBits);
+                    blockTile<5, kUnitFamily>(v, pl.r0 + k / (6u * kL5Chunks), m / kL5Chunks, m % kL5Chunks, s_mem, s_occ, cfgBits);
                     break;
                 }
                 case 4:
-                    blockTile<4, kUnitCombo>(v, pl.r0 + k / 48u, k % 48u, 0, s_mem, cfgBits);
+                    blockTile<4, kUnitCombo>(v, pl.r0 + k / 48u, k % 48u, 0, s_mem, s_occ, cfgBits);
                     break;
                 case 3: {
                     const uint32_t item = k * kBatchWaves + wave;

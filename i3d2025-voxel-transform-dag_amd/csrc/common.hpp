@@ -157,7 +157,14 @@ TDAG_HD uint32_t hashStep(uint32_t s, uint32_t v)
 {
     const uint32_t x = (rotl32(s, 7) ^ v) * 0x9E3779B1u;
     const uint32_t t = s | v;
+#if defined(__HIP_DEVICE_COMPILE__)
+    // min(t, 1) as one v_min_u32: written as a compare and a select, it costs two instructions
+    // and a VCC hazard in the unrolled sweeps.
+    uint32_t seen;
+    asm("v_min_u32 %0, 1, %1" : "=v"(seen) : "v"(t));
+#else
     const uint32_t seen = t < 1u ? t : 1u;
+#endif
     return x > seen ? x : seen;
 }
 
@@ -179,6 +186,15 @@ TDAG_HD uint32_t homeSlot32(uint32_t h, uint32_t logCap)
 // positions its low ten bits (disjoint for filters up to 2^22 words).
 TDAG_HD uint32_t bloomWord(uint32_t f, uint32_t logWords) { return f >> (32 - logWords); }
 TDAG_HD uint32_t bloomBits(uint32_t f) { return (1u << (f & 31u)) | (1u << ((f >> 5) & 31u)); }
+// 1 when both bits of f are set in the filter word bw.
+TDAG_HD uint32_t bloomHit(uint32_t bw, uint32_t f) { return (bw >> (f & 31u)) & (bw >> ((f >> 5) & 31u)) & 1u; }
+// Word of a level-3..5 nested hash. Word 0 is never set (its keys share word 1), so the hash 0 of
+// an empty region always misses the filter and needs no test of its own.
+TDAG_HD uint32_t bloomWordNested(uint32_t f, uint32_t logWords)
+{
+    const uint32_t w = f >> (32 - logWords);
+    return w > 1u ? w : 1u;
+}
 TDAG_HD uint32_t bloomMix64(uint64_t key) { return (uint32_t)((key * 0xD6E8FEB86659FD93ull) >> 32); }
 
 } // namespace tdag

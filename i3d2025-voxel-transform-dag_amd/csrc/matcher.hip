@@ -190,7 +190,7 @@ __global__ __launch_bounds__(64) void kCornerTable(const uint64_t* __restrict__ 
         }
         const uint32_t h = finalHash(s3);
         if (c2 == 0 && h != 0) {
-            atomicOr(&bloom[bloomWord(h, logBloom)], bloomBits(h));
+            atomicOr(&bloom[bloomWordNested(h, logBloom)], bloomBits(h));
             const uint64_t packed = ((uint64_t)h << 32) | (uint64_t)((r << 3) | corner);
             uint32_t slot = homeSlot32(h, logCap);
             for (;;) {
@@ -225,6 +225,7 @@ __global__ __launch_bounds__(64 * Waves) void kSearch(const uint64_t* __restrict
     constexpr int Buffers = Kind == kUnitShared ? 2 : 1;
     __shared__ typename G::Row s_rows[3 * G::Rows];
     __shared__ uint32_t s_planes[Buffers][Waves * G::P * G::Slab];
+    __shared__ uint32_t s_occ[3 * Waves];
     ProbeCounts cnt;
     constexpr uint32_t Units = Kind == kUnitCombo ? 48u : 6u;
     constexpr uint32_t Chunks = (uint32_t)(G::MaxWaves / Waves);
@@ -244,10 +245,11 @@ __global__ __launch_bounds__(64 * Waves) void kSearch(const uint64_t* __restrict
     const uint32_t chunk = member % Chunks;
     if (!(cfgBits & kCfgAxis) && (Kind == kUnitCombo ? (unit >> 3) != 0 : (unit >> 1) != 0))
         return; // only permutation code 0 (p0 == 0) is allowed
-    if (!loadRows<L, 64 * Waves>(layouts, n, r, s_rows, 3))
+    Occupancy occ;
+    if (!loadRowsOcc<L, 64 * Waves>(layouts, n, r, s_rows, s_occ, occ))
         return;
-    searchUnit<L, Waves, Kind>(r, unit, chunk, (int)(threadIdx.x >> 6), s_rows, s_planes[0], s_planes[Buffers - 1], layouts, table, logCap, bloom, logBloom, best,
-        cfgBits, cnt);
+    searchUnit<L, Waves, Kind>(r, unit, chunk, (int)(threadIdx.x >> 6), s_rows, occ, s_planes[0], s_planes[Buffers - 1], layouts, table, logCap, bloom,
+        logBloom, best, cfgBits, cnt);
     flushCounts(cnt, counters, cfgBits);
 }
 

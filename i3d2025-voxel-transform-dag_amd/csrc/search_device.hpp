@@ -66,6 +66,14 @@ __device__ __forceinline__ uint32_t waveSum(uint32_t v)
     return v;
 }
 
+__device__ __forceinline__ uint32_t waveOr(uint32_t v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v |= (uint32_t)__shfl_xor((int)v, off, 64);
+    return v;
+}
+
 __device__ __forceinline__ void flushCounts(const ProbeCounts& c, unsigned long long* counters, uint32_t cfgBits)
 {
     if (!(cfgBits & kCfgCount))
@@ -219,6 +227,55 @@ __device__ __forceinline__ bool loadRows(const uint64_t* __restrict__ layouts, u
     return __syncthreads_or(any) != 0;
 }
 
+// Occupancy of an item along the three axes: bit i of a[k] is set iff some voxel has coordinate
+// i along axis k, i.e. the OR of the rows of layout k. Values are uniform (scalar registers).
+struct Occupancy {
+    uint32_t a[3];
+    // Selects without indexing, so the values stay in registers.
+    __device__ __forceinline__ uint32_t axis(uint32_t k) const { return k == 0u ? a[0] : (k == 1u ? a[1] : a[2]); }
+};
+
+// loadRows of all three layouts that also returns the item's occupancy. s_occ holds 3 words per
+// wave of the block. Contains one block barrier.
+template <int L, int Threads>
+__device__ __forceinline__ bool loadRowsOcc(const uint64_t* __restrict__ layouts, uint32_t n, uint32_t r, typename Geo<L>::Row* s_rows, uint32_t* s_occ,
+    Occupancy& occ)
+{
+    using G = Geo<L>;
+    using Row = typename G::Row;
+    constexpr int Waves = Threads / 64;
+    const Row* g = reinterpret_cast<const Row*>(layouts);
+    uint32_t acc0 = 0, acc1 = 0, acc2 = 0;
+    for (int i = (int)threadIdx.x; i < 3 * G::R2; i += Threads) {
+        const int lay = i >> (2 * L), j = i & (G::R2 - 1);
+        const Row v = g[((size_t)lay * n + r) * G::R2 + j];
+        s_rows[lay * G::Rows + j + (j >> L)] = v;
+        acc0 |= lay == 0 ? (uint32_t)v : 0u;
+        acc1 |= lay == 1 ? (uint32_t)v : 0u;
+        acc2 |= lay == 2 ? (uint32_t)v : 0u;
+    }
+    acc0 = waveOr(acc0);
+    acc1 = waveOr(acc1);
+    acc2 = waveOr(acc2);
+    const int wave = (int)(threadIdx.x >> 6);
+    if ((threadIdx.x & 63u) == 0) {
+        s_occ[3 * wave + 0] = acc0;
+        s_occ[3 * wave + 1] = acc1;
+        s_occ[3 * wave + 2] = acc2;
+    }
+    __syncthreads();
+    uint32_t o[3] = { 0, 0, 0 };
+#pragma unroll
+    for (int w = 0; w < Waves; ++w)
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            o[a] |= s_occ[3 * w + a];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        occ.a[a] = __builtin_amdgcn_readfirstlane(o[a]);
+    return (occ.a[0] | occ.a[1] | occ.a[2]) != 0u;
+}
+
 // Orders LDS accesses of the lanes of one wave (LDS operations of a wave execute in order; this
 // keeps the compiler from moving them across).
 __device__ __forceinline__ void waveSync()
@@ -228,23 +285,38 @@ __device__ __forceinline__ void waveSync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Wave-level loadRows: the calling wave alone loads item r's layouts into its own s_rows.
-// Returns whether the item is non-empty (uniform over the wave).
+// Wave-level loadRowsOcc: the calling wave alone loads item r's three layouts into its own
+// s_rows and returns its occupancy. Returns whether the item is non-empty (uniform over the wave).
 template <int L>
-__device__ __forceinline__ bool loadRowsWave(const uint64_t* __restrict__ layouts, uint32_t n, uint32_t r, typename Geo<L>::Row* s_rows, int numLayouts)
+__device__ __forceinline__ bool loadRowsWave(const uint64_t* __restrict__ layouts, uint32_t n, uint32_t r, typename Geo<L>::Row* s_rows, Occupancy& occ)
 {
     using G = Geo<L>;
     using Row = typename G::Row;
     const Row* g = reinterpret_cast<const Row*>(layouts);
-    int any = 0;
-    for (int i = (int)(threadIdx.x & 63u); i < numLayouts * G::R2; i += 64) {
+    uint32_t acc0 = 0, acc1 = 0, acc2 = 0;
+    for (int i = (int)(threadIdx.x & 63u); i < 3 * G::R2; i += 64) {
         const int lay = i >> (2 * L), j = i & (G::R2 - 1);
         const Row v = g[((size_t)lay * n + r) * G::R2 + j];
         s_rows[lay * G::Rows + j + (j >> L)] = v;
-        any |= (v != 0);
+        acc0 |= lay == 0 ? (uint32_t)v : 0u;
+        acc1 |= lay == 1 ? (uint32_t)v : 0u;
+        acc2 |= lay == 2 ? (uint32_t)v : 0u;
     }
+    occ.a[0] = __builtin_amdgcn_readfirstlane(waveOr(acc0));
+    occ.a[1] = __builtin_amdgcn_readfirstlane(waveOr(acc1));
+    occ.a[2] = __builtin_amdgcn_readfirstlane(waveOr(acc2));
     waveSync();
-    return __ballot(any) != 0ull;
+    return (occ.a[0] | occ.a[1] | occ.a[2]) != 0u;
+}
+
+// Steps at the start of a scan along an axis with occupancy `occ` that read only empty rows: a
+// positive scan runs from coordinate R-1 down, a negative one from 0 up. Their states stay 0.
+template <int L>
+__device__ __forceinline__ int emptyLead(uint32_t occ, uint32_t pos)
+{
+    if (occ == 0u)
+        return Geo<L>::R;
+    return pos ? __builtin_clz(occ) - (32 - Geo<L>::R) : __builtin_ctz(occ);
 }
 
 // Value of the first pass (along A0) at coordinate c0 of a row: the trimmed bit sequence of the
@@ -262,102 +334,147 @@ __device__ __forceinline__ uint32_t rowPrefix(uint32_t row, int sh)
 // Passes 1 and 2 of the nested hash on plane c0 (the coordinate along the first pass axis A0):
 // the lane owns the line along A1 at c[A2] = u and reads, at each step, the row along A0
 // through that voxel. The pass-2 state of voxel (c1, c2) goes to plane[c1 + Pitch*c2].
+// The first `lead` steps (a wave-uniform count, see emptyLead) read only empty rows: chunks of
+// R/4 steps inside them store the state 0 without hashing. All rows are read up front so the
+// chunk branches do not wait on LDS.
 template <int L, int A0, int A1, bool Pos0, bool Pos1>
-__device__ __forceinline__ void pass12(const typename Geo<L>::Row* s_rows, int c0, int u, uint32_t* plane)
+__device__ __forceinline__ void pass12(const typename Geo<L>::Row* s_rows, int c0, int u, uint32_t* plane, int lead)
 {
     using G = Geo<L>;
     constexpr int R = G::R;
+    constexpr int C = R / 4;
     constexpr int A2 = 3 - A0 - A1;
     constexpr int Step = (A1 < A2) ? 1 : G::Pitch; // row (lo, hi) = (c[A1], c[A2]) or (c[A2], c[A1])
     const typename G::Row* rows = s_rows + A0 * G::Rows + ((A1 < A2) ? G::Pitch * u : u);
     uint32_t* line = plane + G::Pitch * u;
     const int sh = Pos0 ? c0 : 31 - c0;
+    uint32_t v[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k)
+        v[k] = (uint32_t)rows[Step * (Pos1 ? (R - 1 - k) : k)];
     uint32_t s = 0;
 #pragma unroll
-    for (int k = 0; k < R; ++k) {
-        const int c1 = Pos1 ? (R - 1 - k) : k;
-        s = hashStep(s, rowPrefix<Pos0>((uint32_t)rows[Step * c1], sh));
-        line[c1] = s;
+    for (int k0 = 0; k0 < R; k0 += C) {
+        if (k0 + C <= lead) {
+#pragma unroll
+            for (int k = k0; k < k0 + C; ++k)
+                line[Pos1 ? (R - 1 - k) : k] = 0u;
+            continue;
+        }
+#pragma unroll
+        for (int k = k0; k < k0 + C; ++k) {
+            s = hashStep(s, rowPrefix<Pos0>(v[k], sh));
+            line[Pos1 ? (R - 1 - k) : k] = s;
+        }
     }
 }
 
 template <int L, int A0, int A1>
-__device__ __forceinline__ void pass12Dirs(uint32_t d, const typename Geo<L>::Row* s_rows, int c0, int u, uint32_t* plane)
+__device__ __forceinline__ void pass12Dirs(uint32_t d, const typename Geo<L>::Row* s_rows, int c0, int u, uint32_t* plane, int lead)
 {
     switch (d & 3u) {
     case 0:
-        pass12<L, A0, A1, false, false>(s_rows, c0, u, plane);
+        pass12<L, A0, A1, false, false>(s_rows, c0, u, plane, lead);
         break;
     case 1:
-        pass12<L, A0, A1, true, false>(s_rows, c0, u, plane);
+        pass12<L, A0, A1, true, false>(s_rows, c0, u, plane, lead);
         break;
     case 2:
-        pass12<L, A0, A1, false, true>(s_rows, c0, u, plane);
+        pass12<L, A0, A1, false, true>(s_rows, c0, u, plane, lead);
         break;
     default:
-        pass12<L, A0, A1, true, true>(s_rows, c0, u, plane);
+        pass12<L, A0, A1, true, true>(s_rows, c0, u, plane, lead);
         break;
     }
 }
 
 // Passes 1 and 2 for permutation code pc (axes p0, p1) and direction bits d (bit 0: pass 1,
-// bit 1: pass 2).
+// bit 1: pass 2); `lead` as in pass12.
 template <int L>
-__device__ __forceinline__ void passes12(uint32_t pc, uint32_t d, const typename Geo<L>::Row* s_rows, int c0, int u, uint32_t* plane)
+__device__ __forceinline__ void passes12(uint32_t pc, uint32_t d, const typename Geo<L>::Row* s_rows, int c0, int u, uint32_t* plane, int lead)
 {
     switch (pc) { // transform_dag.h:138-145
     case 0:
-        pass12Dirs<L, 0, 1>(d, s_rows, c0, u, plane);
+        pass12Dirs<L, 0, 1>(d, s_rows, c0, u, plane, lead);
         break;
     case 1:
-        pass12Dirs<L, 0, 2>(d, s_rows, c0, u, plane);
+        pass12Dirs<L, 0, 2>(d, s_rows, c0, u, plane, lead);
         break;
     case 2:
-        pass12Dirs<L, 1, 0>(d, s_rows, c0, u, plane);
+        pass12Dirs<L, 1, 0>(d, s_rows, c0, u, plane, lead);
         break;
     case 3:
-        pass12Dirs<L, 1, 2>(d, s_rows, c0, u, plane);
+        pass12Dirs<L, 1, 2>(d, s_rows, c0, u, plane, lead);
         break;
     case 4:
-        pass12Dirs<L, 2, 0>(d, s_rows, c0, u, plane);
+        pass12Dirs<L, 2, 0>(d, s_rows, c0, u, plane, lead);
         break;
     default:
-        pass12Dirs<L, 2, 1>(d, s_rows, c0, u, plane);
+        pass12Dirs<L, 2, 1>(d, s_rows, c0, u, plane, lead);
         break;
     }
 }
 
 // Pass 3 along the lane's line (c1 = u) of its plane over the pass-2 states in[Pitch*c2], fused
 // with the Bloom filter: stores the final hash of every voxel at out[Pitch*c2] (out may equal in)
-// and returns the bit set of voxels c2 that are non-empty and pass the filter. The filter word is
-// loaded unconditionally (every hash indexes a valid word) so the sweep has no branches and its
-// loads overlap the hashing.
+// and returns the bit set of voxels c2 that are non-empty and pass the filter. Filter words are
+// loaded unconditionally (every hash indexes a valid word), one chunk of R/4 steps at a time, and
+// tested while the next chunk hashes. Chunks inside the first `lead` steps (wave-uniform; their
+// states are 0) are skipped: their voxels are empty, never pending, and out keeps its old words.
 template <int L, bool Pos2>
-__device__ __forceinline__ uint32_t pass3Filter(const uint32_t* in, uint32_t* out, const uint32_t* __restrict__ bloom, uint32_t logBloom, ProbeCounts& cnt)
+__device__ __forceinline__ uint32_t pass3Filter(const uint32_t* in, uint32_t* out, const uint32_t* __restrict__ bloom, uint32_t logBloom, uint32_t cfgBits,
+    int lead, ProbeCounts& cnt)
 {
     constexpr int R = Geo<L>::R;
+    constexpr int C = R / 4;
     const uint32_t shift = 32u - logBloom;
     uint32_t st[R];
 #pragma unroll
-    for (int c2 = 0; c2 < R; ++c2)
-        st[c2] = in[Geo<L>::Pitch * c2];
-    uint32_t s = 0, pending = 0, nonEmpty = 0;
+    for (int k = 0; k < R; ++k)
+        st[k] = in[Geo<L>::Pitch * (Pos2 ? (R - 1 - k) : k)];
+    uint32_t s = 0, pending = 0;
+    uint32_t pf[C], pbw[C]; // the previous chunk's hashes and filter words (0: nothing passes)
 #pragma unroll
-    for (int k = 0; k < R; ++k) {
-        const int c2 = Pos2 ? (R - 1 - k) : k;
-        s = hashStep(s, st[c2]);
-        const uint32_t f = finalHash(s);
-        out[Geo<L>::Pitch * c2] = f;
-        const uint32_t bw = bloom[f >> shift];
-        const uint32_t need = bloomBits(f);
-        const uint32_t miss = need & ~bw;
-        const uint32_t nz = f < 1u ? f : 1u;
-        const uint32_t ok = nz & ((miss < 1u ? miss : 1u) ^ 1u);
-        pending |= ok << c2;
-        nonEmpty += nz;
+    for (int k = 0; k < C; ++k)
+        pf[k] = pbw[k] = 0u;
+#pragma unroll
+    for (int k0 = 0; k0 < R; k0 += C) {
+        if (k0 + C <= lead)
+            continue;
+        uint32_t f[C], bw[C];
+#pragma unroll
+        for (int k = 0; k < C; ++k) {
+            s = hashStep(s, st[k0 + k]);
+            f[k] = finalHash(s);
+            out[Geo<L>::Pitch * (Pos2 ? (R - 1 - k0 - k) : k0 + k)] = f[k];
+            // An empty region has f == 0, whose word (0) is never set: no emptiness test needed.
+            const uint32_t w = f[k] >> shift;
+            bw[k] = bloom[w > 1u ? w : 1u];
+        }
+        if (k0 > 0) {
+#pragma unroll
+            for (int k = 0; k < C; ++k)
+                pending |= bloomHit(pbw[k], pf[k]) << (Pos2 ? (R - 1 - (k0 - C) - k) : (k0 - C) + k);
+        }
+#pragma unroll
+        for (int k = 0; k < C; ++k) {
+            pf[k] = f[k];
+            pbw[k] = bw[k];
+        }
     }
-    cnt.probes += nonEmpty;
-    cnt.filterPass += (uint32_t)__builtin_popcount(pending);
+#pragma unroll
+    for (int k = 0; k < C; ++k)
+        pending |= bloomHit(pbw[k], pf[k]) << (Pos2 ? (C - 1 - k) : (R - C) + k);
+    if (cfgBits & kCfgCount) {
+        uint32_t acc = 0, nonEmpty = 0;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            acc |= st[k];
+            nonEmpty += acc != 0u;
+        }
+        cnt.probes += nonEmpty;
+        cnt.filterPass += (uint32_t)__builtin_popcount(pending);
+    }
     return pending;
 }
 
@@ -490,11 +607,12 @@ __device__ __forceinline__ void probeLine(uint32_t pending, const uint32_t* line
 // Pass 3 in direction dz with the filter (pass-2 states read from `in`, final hashes written to
 // `out`), then the table walk of the lane's line.
 template <int L>
-__device__ __forceinline__ void lastPassProbe(uint32_t dz, const uint32_t* in, uint32_t* out, const typename Geo<L>::Row* s_rows,
+__device__ __forceinline__ void lastPassProbe(uint32_t dz, int lead, const uint32_t* in, uint32_t* out, const typename Geo<L>::Row* s_rows,
     const uint64_t* __restrict__ sortedMasks, const uint64_t* __restrict__ table, uint32_t logCap, const uint32_t* __restrict__ bloom, uint32_t logBloom,
     uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t r, uint32_t pc, uint32_t d, int c0, int u, ProbeCounts& cnt)
 {
-    const uint32_t pending = dz ? pass3Filter<L, true>(in, out, bloom, logBloom, cnt) : pass3Filter<L, false>(in, out, bloom, logBloom, cnt);
+    const uint32_t pending = dz ? pass3Filter<L, true>(in, out, bloom, logBloom, cfgBits, lead, cnt)
+                                : pass3Filter<L, false>(in, out, bloom, logBloom, cfgBits, lead, cnt);
     probeLine<L>(pending, out, s_rows, sortedMasks, table, logCap, best, cfgBits, r, pc, d, c0, u, cnt);
 }
 
@@ -507,16 +625,18 @@ constexpr int kUnitShared = 2; // a family of 8 combos; passes 1 and 2 shared by
 template <int L, int Waves, int Kind>
 constexpr size_t searchLds()
 {
-    return 3 * Geo<L>::Rows * sizeof(typename Geo<L>::Row) + (Kind == kUnitShared ? 2 : 1) * (size_t)Waves * Geo<L>::P * Geo<L>::Slab * 4;
+    return 3 * Geo<L>::Rows * sizeof(typename Geo<L>::Row) + (Kind == kUnitShared ? 2 : 1) * (size_t)Waves * Geo<L>::P * Geo<L>::Slab * 4 + 3 * Waves * 4;
 }
 
-// Search of one unit of parent rank r (rows already in s_rows) by `Waves` waves: `wave` is the
-// caller's wave index within them and covers planes ((chunk * Waves + wave) * P + w) for
-// w < P. planes0 (and planes1 for kUnitShared) hold Waves * P * Slab words each.
+// Search of one unit of parent rank r (rows already in s_rows, occupancy occ) by `Waves` waves:
+// `wave` is the caller's wave index within them and covers planes ((chunk * Waves + wave) * P + w)
+// for w < P. planes0 (and planes1 for kUnitShared) hold Waves * P * Slab words each. Scan steps
+// over empty rows are skipped (emptyLead), and so is the whole unit for a wave whose planes hold
+// no voxel.
 template <int L, int Waves, int Kind>
-__device__ __forceinline__ void searchUnit(uint32_t r, uint32_t unit, uint32_t chunk, int wave, const typename Geo<L>::Row* s_rows, uint32_t* planes0,
-    uint32_t* planes1, const uint64_t* __restrict__ layouts, const uint64_t* __restrict__ table, uint32_t logCap, const uint32_t* __restrict__ bloom,
-    uint32_t logBloom, uint64_t* __restrict__ best, uint32_t cfgBits, ProbeCounts& cnt)
+__device__ __forceinline__ void searchUnit(uint32_t r, uint32_t unit, uint32_t chunk, int wave, const typename Geo<L>::Row* s_rows, const Occupancy& occ,
+    uint32_t* planes0, uint32_t* planes1, const uint64_t* __restrict__ layouts, const uint64_t* __restrict__ table, uint32_t logCap,
+    const uint32_t* __restrict__ bloom, uint32_t logBloom, uint64_t* __restrict__ best, uint32_t cfgBits, ProbeCounts& cnt)
 {
     using G = Geo<L>;
     constexpr int R = G::R;
@@ -528,23 +648,32 @@ __device__ __forceinline__ void searchUnit(uint32_t r, uint32_t unit, uint32_t c
     uint32_t* plane = planes0 + planeOff;
     uint32_t* line = plane + u; // the lane's pass-3 line: c1 = u, stride Pitch along c2
     uint32_t* outLine = planes1 + planeOff + u;
+    // Pass 1 along p0 in direction bit 0 of d reads the rows' segments toward c0; plane c0 is
+    // empty iff that segment of the occupancy along p0 is. p0 and d's bit 0 are fixed per unit.
+    const uint32_t p0 = Kind == kUnitCombo ? (unit >> 3) >> 1 : unit >> 1;
+    const uint32_t dx = unit & 1u;
+    const uint32_t segment = dx ? rowPrefix<true>(occ.axis(p0), c0) : rowPrefix<false>(occ.axis(p0), 31 - c0);
+    if (__ballot(segment != 0u) == 0ull)
+        return;
     if constexpr (Kind == kUnitCombo) {
         const uint32_t pc = unit >> 3, d = unit & 7u;
-        passes12<L>(pc, d, s_rows, c0, u, plane);
+        passes12<L>(pc, d, s_rows, c0, u, plane, emptyLead<L>(occ.axis(permAxis(pc, 1)), (d >> 1) & 1u));
         waveSync();
-        lastPassProbe<L>((d >> 2) & 1u, line, line, s_rows, layouts, table, logCap, bloom, logBloom, best, cfgBits, r, pc, d, c0, u, cnt);
+        lastPassProbe<L>((d >> 2) & 1u, emptyLead<L>(occ.axis(permAxis(pc, 2)), (d >> 2) & 1u), line, line, s_rows, layouts, table, logCap, bloom, logBloom,
+            best, cfgBits, r, pc, d, c0, u, cnt);
     } else {
-        const uint32_t a0 = unit >> 1, dx = unit & 1u;
 #pragma unroll 1
         for (uint32_t sel = 0; sel < 2; ++sel) {
-            const uint32_t pc = 2u * a0 + sel; // the permutation codes with p0 == a0
+            const uint32_t pc = 2u * p0 + sel; // the permutation codes with p0
             if (!axisOn && pc != 0)
                 continue;
+            const uint32_t occ1 = occ.axis(permAxis(pc, 1)), occ2 = occ.axis(permAxis(pc, 2));
 #pragma unroll 1
             for (uint32_t dy = 0; dy < 2; ++dy) {
+                const int lead2 = emptyLead<L>(occ1, dy);
                 if constexpr (Kind == kUnitShared) {
                     waveSync();
-                    passes12<L>(pc, dx | (dy << 1), s_rows, c0, u, plane);
+                    passes12<L>(pc, dx | (dy << 1), s_rows, c0, u, plane, lead2);
                     waveSync();
                 }
 #pragma unroll 1
@@ -552,10 +681,11 @@ __device__ __forceinline__ void searchUnit(uint32_t r, uint32_t unit, uint32_t c
                     const uint32_t d = dx | (dy << 1) | (dz << 2);
                     if constexpr (Kind == kUnitFamily) {
                         waveSync();
-                        passes12<L>(pc, d, s_rows, c0, u, plane);
+                        passes12<L>(pc, d, s_rows, c0, u, plane, lead2);
                         waveSync();
                     }
-                    lastPassProbe<L>(dz, line, outLine, s_rows, layouts, table, logCap, bloom, logBloom, best, cfgBits, r, pc, d, c0, u, cnt);
+                    lastPassProbe<L>(dz, emptyLead<L>(occ2, dz), line, outLine, s_rows, layouts, table, logCap, bloom, logBloom, best, cfgBits, r, pc, d,
+                        c0, u, cnt);
                 }
             }
         }
@@ -582,7 +712,7 @@ __device__ __forceinline__ void prunePacked(uint64_t* __restrict__ table, uint64
             continue;
         }
         const uint32_t h = (uint32_t)(e >> 32);
-        atomicOr(&bloom[bloomWord(h, logBloom)], bloomBits(h));
+        atomicOr(&bloom[bloomWordNested(h, logBloom)], bloomBits(h));
     }
 }
 
