@@ -38,7 +38,7 @@ constexpr int kBatchWaves = kBatchThreads / 64;
 template <int L>
 constexpr size_t rowsBytes()
 {
-    return 3 * Geo<L>::Rows * sizeof(typename Geo<L>::Row);
+    return Geo<L>::Rows * sizeof(typename Geo<L>::Row); // the layout of the unit's first-pass axis
 }
 
 constexpr size_t alignUp16(size_t v) { return (v + 15) / 16 * 16; }
@@ -73,7 +73,8 @@ uint32_t tilesFor(uint32_t level, uint32_t parents)
 
 // Block-level tile of levels 4 and 5: all four waves search one unit of parent r.
 template <int L, int Kind>
-__device__ __forceinline__ void blockTile(const LevelView& v, uint32_t r, uint32_t unit, uint32_t chunk, unsigned char* s_mem, uint32_t* s_occ, uint32_t cfgBits)
+__device__ __forceinline__ void blockTile(const LevelView& v, uint32_t r, uint32_t unit, uint32_t chunk, unsigned char* s_mem, uint32_t* s_occ, uint16_t* queue,
+    uint32_t cfgBits)
 {
     using G = Geo<L>;
     if (!(cfgBits & kCfgAxis) && (Kind == kUnitCombo ? (unit >> 3) : (unit >> 1)) != 0)
@@ -81,16 +82,16 @@ __device__ __forceinline__ void blockTile(const LevelView& v, uint32_t r, uint32
     auto* s_rows = reinterpret_cast<typename G::Row*>(s_mem);
     uint32_t* planes = reinterpret_cast<uint32_t*>(s_mem + alignUp16(rowsBytes<L>()));
     Occupancy occ;
-    if (!loadRowsOcc<L, kBatchThreads>(v.sortedMasks, v.n, r, s_rows, s_occ, occ))
+    if (!loadRowsOcc<L, kBatchThreads>(v.sortedMasks, v.n, r, Kind == kUnitCombo ? (unit >> 3) >> 1 : unit >> 1, s_rows, s_occ, occ))
         return;
     ProbeCounts cnt;
-    searchUnit<L, kBatchWaves, Kind>(r, unit, chunk, (int)(threadIdx.x >> 6), s_rows, occ, planes, planes, v.sortedMasks, v.tabKeys, v.logCap, v.bloom, v.logBloom, v.best,
+    searchUnit<L, kBatchWaves, Kind>(r, unit, chunk, (int)(threadIdx.x >> 6), s_rows, occ, planes, planes, queue, v.sortedMasks, v.tabKeys, v.logCap, v.bloom, v.logBloom, v.best,
         cfgBits, cnt);
     flushCounts(cnt, v.counters, cfgBits);
 }
 
 // Wave item of level 3: the calling wave alone searches family `unit` of parent r.
-__device__ __forceinline__ void waveItem3(const LevelView& v, uint32_t r, uint32_t unit, unsigned char* s_mem, uint32_t cfgBits)
+__device__ __forceinline__ void waveItem3(const LevelView& v, uint32_t r, uint32_t unit, unsigned char* s_mem, uint16_t* queue, uint32_t cfgBits)
 {
     using G = Geo<3>;
     if (!(cfgBits & kCfgAxis) && (unit >> 1) != 0)
@@ -100,10 +101,10 @@ __device__ __forceinline__ void waveItem3(const LevelView& v, uint32_t r, uint32
     uint32_t* planes0 = reinterpret_cast<uint32_t*>(base + kL3RowsBytes);
     uint32_t* planes1 = planes0 + G::P * G::Slab;
     Occupancy occ;
-    if (!loadRowsWave<3>(v.sortedMasks, v.n, r, s_rows, occ))
+    if (!loadRowsWave<3>(v.sortedMasks, v.n, r, unit >> 1, s_rows, occ))
         return;
     ProbeCounts cnt;
-    searchUnit<3, 1, kUnitShared>(r, unit, 0, 0, s_rows, occ, planes0, planes1, v.sortedMasks, v.tabKeys, v.logCap, v.bloom, v.logBloom, v.best, cfgBits, cnt);
+    searchUnit<3, 1, kUnitShared>(r, unit, 0, 0, s_rows, occ, planes0, planes1, queue, v.sortedMasks, v.tabKeys, v.logCap, v.bloom, v.logBloom, v.best, cfgBits, cnt);
     flushCounts(cnt, v.counters, cfgBits);
 }
 
@@ -145,6 +146,7 @@ __global__ __launch_bounds__(kBatchThreads, 3) void kBatched(const BatchDesc* __
     __shared__ __align__(16) unsigned char s_mem[kBatchLds];
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_occ[3 * kBatchWaves];
+    __shared__ uint16_t s_queue[kBatchWaves][kProbeQueue];
     const uint32_t numLevels = desc->numLevels;
     const uint32_t numPhases = desc->numPhases;
     const uint32_t cfgBits = desc->cfgBits;
@@ -175,16 +177,16 @@ __global__ __launch_bounds__(kBatchThreads, 3) void kBatched(const BatchDesc* __
                 switch (v.level) {
                 case 5: {
                     const uint32_t m = k % (6u * kL5Chunks);
-                    blockTile<5, kUnitFamily>(v, pl.r0 + k / (6u * kL5Chunks), m / kL5Chunks, m % kL5Chunks, s_mem, s_occ, cfgBits);
+                    blockTile<5, kUnitFamily>(v, pl.r0 + k / (6u * kL5Chunks), m / kL5Chunks, m % kL5Chunks, s_mem, s_occ, s_queue[wave], cfgBits);
                     break;
                 }
                 case 4:
-                    blockTile<4, kUnitCombo>(v, pl.r0 + k / 48u, k % 48u, 0, s_mem, s_occ, cfgBits);
+                    blockTile<4, kUnitCombo>(v, pl.r0 + k / 48u, k % 48u, 0, s_mem, s_occ, s_queue[wave], cfgBits);
                     break;
                 case 3: {
                     const uint32_t item = k * kBatchWaves + wave;
                     if (item < (pl.r1 - pl.r0) * 6u)
-                        waveItem3(v, pl.r0 + item / 6u, item % 6u, s_mem, cfgBits);
+                        waveItem3(v, pl.r0 + item / 6u, item % 6u, s_mem, s_queue[wave], cfgBits);
                     break;
                 }
                 default: {

@@ -223,9 +223,10 @@ __global__ __launch_bounds__(64 * Waves) void kSearch(const uint64_t* __restrict
 {
     using G = Geo<L>;
     constexpr int Buffers = Kind == kUnitShared ? 2 : 1;
-    __shared__ typename G::Row s_rows[3 * G::Rows];
+    __shared__ typename G::Row s_rows[G::Rows];
     __shared__ uint32_t s_planes[Buffers][Waves * G::P * G::Slab];
     __shared__ uint32_t s_occ[3 * Waves];
+    __shared__ uint16_t s_queue[Waves][kProbeQueue];
     ProbeCounts cnt;
     constexpr uint32_t Units = Kind == kUnitCombo ? 48u : 6u;
     constexpr uint32_t Chunks = (uint32_t)(G::MaxWaves / Waves);
@@ -245,11 +246,13 @@ __global__ __launch_bounds__(64 * Waves) void kSearch(const uint64_t* __restrict
     const uint32_t chunk = member % Chunks;
     if (!(cfgBits & kCfgAxis) && (Kind == kUnitCombo ? (unit >> 3) != 0 : (unit >> 1) != 0))
         return; // only permutation code 0 (p0 == 0) is allowed
+    // p0 of the unit's permutation codes: the axis of the first pass and the one layout it reads.
+    const uint32_t a0 = Kind == kUnitCombo ? (unit >> 3) >> 1 : unit >> 1;
     Occupancy occ;
-    if (!loadRowsOcc<L, 64 * Waves>(layouts, n, r, s_rows, s_occ, occ))
+    if (!loadRowsOcc<L, 64 * Waves>(layouts, n, r, a0, s_rows, s_occ, occ))
         return;
-    searchUnit<L, Waves, Kind>(r, unit, chunk, (int)(threadIdx.x >> 6), s_rows, occ, s_planes[0], s_planes[Buffers - 1], layouts, table, logCap, bloom,
-        logBloom, best, cfgBits, cnt);
+    searchUnit<L, Waves, Kind>(r, unit, chunk, (int)(threadIdx.x >> 6), s_rows, occ, s_planes[0], s_planes[Buffers - 1], s_queue[threadIdx.x >> 6], layouts,
+        table, logCap, bloom, logBloom, best, cfgBits, cnt);
     flushCounts(cnt, counters, cfgBits);
 }
 

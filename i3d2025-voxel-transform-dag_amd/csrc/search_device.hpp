@@ -235,33 +235,51 @@ struct Occupancy {
     __device__ __forceinline__ uint32_t axis(uint32_t k) const { return k == 0u ? a[0] : (k == 1u ? a[1] : a[2]); }
 };
 
-// loadRows of all three layouts that also returns the item's occupancy. s_occ holds 3 words per
-// wave of the block. Contains one block barrier.
+// Occupancy contribution of row (u, w) of layout a0 (rows along a0; u and w index the lower and
+// the higher of the other two axes).
+__device__ __forceinline__ void occupancyAdd(uint32_t row, int j, int L, uint32_t& accA0, uint32_t& accLo, uint32_t& accHi)
+{
+    const uint32_t any = row != 0u ? 1u : 0u;
+    accA0 |= row;
+    accLo |= any << (j & ((1 << L) - 1));
+    accHi |= any << (j >> L);
+}
+
+__device__ __forceinline__ Occupancy occupancyOf(uint32_t a0, uint32_t occA0, uint32_t occLo, uint32_t occHi)
+{
+    // The other two axes in ascending order: a0 = 0 -> (1, 2), 1 -> (0, 2), 2 -> (0, 1).
+    Occupancy o;
+    o.a[0] = a0 == 0u ? occA0 : occLo;
+    o.a[1] = a0 == 1u ? occA0 : (a0 == 0u ? occLo : occHi);
+    o.a[2] = a0 == 2u ? occA0 : occHi;
+    return o;
+}
+
+// Loads the rows of item r's layout a0 (rows along axis a0, the only layout a search unit whose
+// first pass runs along a0 reads) into s_rows and returns the item's occupancy along all three
+// axes, derived from them. s_occ holds 3 words per wave of the block. Contains one block barrier.
 template <int L, int Threads>
-__device__ __forceinline__ bool loadRowsOcc(const uint64_t* __restrict__ layouts, uint32_t n, uint32_t r, typename Geo<L>::Row* s_rows, uint32_t* s_occ,
-    Occupancy& occ)
+__device__ __forceinline__ bool loadRowsOcc(const uint64_t* __restrict__ layouts, uint32_t n, uint32_t r, uint32_t a0, typename Geo<L>::Row* s_rows,
+    uint32_t* s_occ, Occupancy& occ)
 {
     using G = Geo<L>;
     using Row = typename G::Row;
     constexpr int Waves = Threads / 64;
-    const Row* g = reinterpret_cast<const Row*>(layouts);
-    uint32_t acc0 = 0, acc1 = 0, acc2 = 0;
-    for (int i = (int)threadIdx.x; i < 3 * G::R2; i += Threads) {
-        const int lay = i >> (2 * L), j = i & (G::R2 - 1);
-        const Row v = g[((size_t)lay * n + r) * G::R2 + j];
-        s_rows[lay * G::Rows + j + (j >> L)] = v;
-        acc0 |= lay == 0 ? (uint32_t)v : 0u;
-        acc1 |= lay == 1 ? (uint32_t)v : 0u;
-        acc2 |= lay == 2 ? (uint32_t)v : 0u;
+    const Row* g = reinterpret_cast<const Row*>(layouts) + ((size_t)a0 * n + r) * G::R2;
+    uint32_t acc0 = 0, accLo = 0, accHi = 0;
+    for (int j = (int)threadIdx.x; j < G::R2; j += Threads) {
+        const Row v = g[j];
+        s_rows[j + (j >> L)] = v;
+        occupancyAdd((uint32_t)v, j, L, acc0, accLo, accHi);
     }
     acc0 = waveOr(acc0);
-    acc1 = waveOr(acc1);
-    acc2 = waveOr(acc2);
+    accLo = waveOr(accLo);
+    accHi = waveOr(accHi);
     const int wave = (int)(threadIdx.x >> 6);
     if ((threadIdx.x & 63u) == 0) {
         s_occ[3 * wave + 0] = acc0;
-        s_occ[3 * wave + 1] = acc1;
-        s_occ[3 * wave + 2] = acc2;
+        s_occ[3 * wave + 1] = accLo;
+        s_occ[3 * wave + 2] = accHi;
     }
     __syncthreads();
     uint32_t o[3] = { 0, 0, 0 };
@@ -270,10 +288,8 @@ __device__ __forceinline__ bool loadRowsOcc(const uint64_t* __restrict__ layouts
 #pragma unroll
         for (int a = 0; a < 3; ++a)
             o[a] |= s_occ[3 * w + a];
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-        occ.a[a] = __builtin_amdgcn_readfirstlane(o[a]);
-    return (occ.a[0] | occ.a[1] | occ.a[2]) != 0u;
+    occ = occupancyOf(a0, __builtin_amdgcn_readfirstlane(o[0]), __builtin_amdgcn_readfirstlane(o[1]), __builtin_amdgcn_readfirstlane(o[2]));
+    return o[0] != 0u;
 }
 
 // Orders LDS accesses of the lanes of one wave (LDS operations of a wave execute in order; this
@@ -285,28 +301,26 @@ __device__ __forceinline__ void waveSync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Wave-level loadRowsOcc: the calling wave alone loads item r's three layouts into its own
-// s_rows and returns its occupancy. Returns whether the item is non-empty (uniform over the wave).
+// Wave-level loadRowsOcc: the calling wave alone loads the rows of item r's layout a0 into its
+// own s_rows and returns its occupancy. Returns whether the item is non-empty (uniform over the
+// wave).
 template <int L>
-__device__ __forceinline__ bool loadRowsWave(const uint64_t* __restrict__ layouts, uint32_t n, uint32_t r, typename Geo<L>::Row* s_rows, Occupancy& occ)
+__device__ __forceinline__ bool loadRowsWave(const uint64_t* __restrict__ layouts, uint32_t n, uint32_t r, uint32_t a0, typename Geo<L>::Row* s_rows,
+    Occupancy& occ)
 {
     using G = Geo<L>;
     using Row = typename G::Row;
-    const Row* g = reinterpret_cast<const Row*>(layouts);
-    uint32_t acc0 = 0, acc1 = 0, acc2 = 0;
-    for (int i = (int)(threadIdx.x & 63u); i < 3 * G::R2; i += 64) {
-        const int lay = i >> (2 * L), j = i & (G::R2 - 1);
-        const Row v = g[((size_t)lay * n + r) * G::R2 + j];
-        s_rows[lay * G::Rows + j + (j >> L)] = v;
-        acc0 |= lay == 0 ? (uint32_t)v : 0u;
-        acc1 |= lay == 1 ? (uint32_t)v : 0u;
-        acc2 |= lay == 2 ? (uint32_t)v : 0u;
+    const Row* g = reinterpret_cast<const Row*>(layouts) + ((size_t)a0 * n + r) * G::R2;
+    uint32_t acc0 = 0, accLo = 0, accHi = 0;
+    for (int j = (int)(threadIdx.x & 63u); j < G::R2; j += 64) {
+        const Row v = g[j];
+        s_rows[j + (j >> L)] = v;
+        occupancyAdd((uint32_t)v, j, L, acc0, accLo, accHi);
     }
-    occ.a[0] = __builtin_amdgcn_readfirstlane(waveOr(acc0));
-    occ.a[1] = __builtin_amdgcn_readfirstlane(waveOr(acc1));
-    occ.a[2] = __builtin_amdgcn_readfirstlane(waveOr(acc2));
+    const uint32_t o0 = __builtin_amdgcn_readfirstlane(waveOr(acc0));
+    occ = occupancyOf(a0, o0, __builtin_amdgcn_readfirstlane(waveOr(accLo)), __builtin_amdgcn_readfirstlane(waveOr(accHi)));
     waveSync();
-    return (occ.a[0] | occ.a[1] | occ.a[2]) != 0u;
+    return o0 != 0u;
 }
 
 // Steps at the start of a scan along an axis with occupancy `occ` that read only empty rows: a
@@ -345,7 +359,7 @@ __device__ __forceinline__ void pass12(const typename Geo<L>::Row* s_rows, int c
     constexpr int C = R / 4;
     constexpr int A2 = 3 - A0 - A1;
     constexpr int Step = (A1 < A2) ? 1 : G::Pitch; // row (lo, hi) = (c[A1], c[A2]) or (c[A2], c[A1])
-    const typename G::Row* rows = s_rows + A0 * G::Rows + ((A1 < A2) ? G::Pitch * u : u);
+    const typename G::Row* rows = s_rows + ((A1 < A2) ? G::Pitch * u : u); // s_rows holds layout A0
     uint32_t* line = plane + G::Pitch * u;
     const int sh = Pos0 ? c0 : 31 - c0;
     uint32_t v[R];
@@ -495,7 +509,7 @@ __device__ __forceinline__ uint32_t transformedRow(const typename Geo<L>::Row* s
     const int a1 = (int)permAxis(pc, 1), a2 = (int)permAxis(pc, 2);
     // Parent coordinates along a1 and a2; the row along a0 is indexed by them in ascending axis order.
     const int lo = a1 < a2 ? m1 : m2, hi = a1 < a2 ? m2 : m1;
-    uint32_t line = (uint32_t)s_rows[(int)permAxis(pc, 0) * G::Rows + lo + G::Pitch * hi];
+    uint32_t line = (uint32_t)s_rows[lo + G::Pitch * hi]; // s_rows holds layout p0
     if (sym & 1u)
         line = __brev(line) >> (32 - R);
     const uint32_t rowMask = R == 32 ? 0xFFFFFFFFu : ((1u << R) - 1u);
@@ -522,98 +536,140 @@ __device__ __forceinline__ bool verifyWave(const typename Geo<L>::Row* s_rows, c
     return __ballot(!ok) == 0ull;
 }
 
-// Walks the child table for the voxels in `pending` (bit c2 = voxel (c0, u, c2) of the lane's
-// line passed the filter; its hash is in the plane) and merges verified hits into best. Lanes stop
-// at each candidate that beats best[c]; the wave then verifies the stopped lanes' candidates
-// together and resumes.
+// Entries of a wave's probe queue in LDS: voxel (w, u, c2) of the wave's planes as
+// (lane << 5) | c2 with lane = u + R*w.
+constexpr uint32_t kProbeQueue = 128;
+
+// Walks the child table for `count` queued voxels (their final hashes are in the wave's planes
+// fWave, plane w at fWave + w*Slab, voxel (u, c2) at u + Pitch*c2) and merges verified hits into
+// best. Lane j takes entry base + j, so the walks of a round run side by side whatever line the
+// voxels came from. Lanes stop at each candidate that beats best[c]; the wave then verifies the
+// stopped lanes' candidates together and resumes.
 template <int L>
-__device__ __forceinline__ void probeLine(uint32_t pending, const uint32_t* line, const typename Geo<L>::Row* s_rows,
-    const uint64_t* __restrict__ sortedMasks, const uint64_t* __restrict__ table, uint32_t logCap,
-    uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t r, uint32_t pc, uint32_t d, int c0, int u, ProbeCounts& cnt)
+__device__ __forceinline__ void probeQueue(const uint16_t* queue, uint32_t count, const uint32_t* fWave, int c0Base, const typename Geo<L>::Row* s_rows,
+    const uint64_t* __restrict__ sortedMasks, const uint64_t* __restrict__ table, uint32_t logCap, uint64_t* __restrict__ best, uint32_t cfgBits,
+    uint32_t r, uint32_t pc, uint32_t d, ProbeCounts& cnt)
 {
     using G = Geo<L>;
     constexpr int R = G::R;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t capMask = (1u << logCap) - 1u;
-    bool active = false;
-    uint32_t h = 0, slot = 0;
-    int c2 = 0;
-    while (true) {
-        bool found = false;
-        uint32_t hitC = 0;
-        uint64_t hitKey = 0;
-        while (true) {
-            if (!active) {
-                if (pending == 0)
-                    break;
-                c2 = __ffs(pending) - 1;
-                pending &= pending - 1u;
-                h = line[G::Pitch * c2];
-                slot = homeSlot32(h, logCap);
-                active = true;
-            }
-            const uint64_t e = table[slot];
-            if (e == 0) {
-                active = false;
-                continue;
-            }
-            slot = (slot + 1) & capMask;
-            if ((uint32_t)(e >> 32) != h)
-                continue;
-            ++cnt.keyHits;
-            const uint32_t cc = (uint32_t)e >> 3;
-            const uint32_t kc = (uint32_t)e & 7u;
-            if (cc <= r || cc == kDeadRank)
-                continue;
-            // Translation of the anchored match (find_translation.cpp:835-837); the voxel's
-            // coordinates along the three pass axes are (c0, u, c2).
-            const int voxel[3] = { c0, u, c2 };
-            const uint32_t sym = d ^ kc;
-            int tt[3];
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                const int anchor = ((d >> a) & 1u) ? 0 : R - 1;
-                tt[a] = ((sym >> a) & 1u) ? voxel[a] - anchor : anchor - voxel[a];
-            }
-            if (!configAllows(cfgBits, pc, sym, tt[0], tt[1], tt[2]))
-                continue;
-            ++cnt.candidates;
-            const uint64_t k64 = encodeKey(r, pc, sym, tt[0], tt[1], tt[2], L);
-            if (k64 >= loadRelaxed(&best[cc]))
-                continue;
-            ++cnt.verifications;
-            found = true;
-            hitC = cc;
-            hitKey = k64;
-            break;
+    for (uint32_t base = 0; base < count; base += 64) {
+        bool active = base + lane < count;
+        uint32_t h = 0, slot = 0;
+        int c0 = 0, u = 0, c2 = 0;
+        if (active) {
+            const uint32_t e = queue[base + lane];
+            const int src = (int)(e >> 5), w = src >> L;
+            c2 = (int)(e & 31u);
+            u = src & (R - 1);
+            c0 = c0Base + w;
+            h = fWave[w * G::Slab + u + G::Pitch * c2];
+            slot = homeSlot32(h, logCap);
         }
-        uint64_t stopped = __ballot(found);
-        if (stopped == 0)
-            break; // no lane found a candidate, so every lane ran out of work
-        while (stopped) {
-            const int leader = __ffsll((unsigned long long)stopped) - 1;
-            const uint32_t bc = (uint32_t)__shfl((int)hitC, leader, 64);
-            const uint64_t bk = (uint64_t)__shfl((long long)hitKey, leader, 64);
-            const bool allOk = verifyWave<L>(s_rows, sortedMasks, bc, bk, lane);
-            if ((int)lane == leader && allOk) {
-                ++cnt.verified;
-                atomicMin((unsigned long long*)&best[bc], (unsigned long long)bk);
+        while (true) {
+            bool found = false;
+            uint32_t hitC = 0;
+            uint64_t hitKey = 0;
+            while (active) {
+                const uint64_t e = table[slot];
+                if (e == 0) {
+                    active = false;
+                    break;
+                }
+                slot = (slot + 1) & capMask;
+                if ((uint32_t)(e >> 32) != h)
+                    continue;
+                ++cnt.keyHits;
+                const uint32_t cc = (uint32_t)e >> 3;
+                const uint32_t kc = (uint32_t)e & 7u;
+                if (cc <= r || cc == kDeadRank)
+                    continue;
+                // Translation of the anchored match (find_translation.cpp:835-837); the voxel's
+                // coordinates along the three pass axes are (c0, u, c2).
+                const int voxel[3] = { c0, u, c2 };
+                const uint32_t sym = d ^ kc;
+                int tt[3];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const int anchor = ((d >> a) & 1u) ? 0 : R - 1;
+                    tt[a] = ((sym >> a) & 1u) ? voxel[a] - anchor : anchor - voxel[a];
+                }
+                if (!configAllows(cfgBits, pc, sym, tt[0], tt[1], tt[2]))
+                    continue;
+                ++cnt.candidates;
+                const uint64_t k64 = encodeKey(r, pc, sym, tt[0], tt[1], tt[2], L);
+                if (k64 >= loadRelaxed(&best[cc]))
+                    continue;
+                ++cnt.verifications;
+                found = true;
+                hitC = cc;
+                hitKey = k64;
+                break;
             }
-            stopped &= stopped - 1ull;
+            uint64_t stopped = __ballot(found);
+            if (stopped == 0)
+                break; // no lane found a candidate, so every walk has ended
+            while (stopped) {
+                const int leader = __ffsll((unsigned long long)stopped) - 1;
+                const uint32_t bc = (uint32_t)__shfl((int)hitC, leader, 64);
+                const uint64_t bk = (uint64_t)__shfl((long long)hitKey, leader, 64);
+                const bool allOk = verifyWave<L>(s_rows, sortedMasks, bc, bk, lane);
+                if ((int)lane == leader && allOk) {
+                    ++cnt.verified;
+                    atomicMin((unsigned long long*)&best[bc], (unsigned long long)bk);
+                }
+                stopped &= stopped - 1ull;
+            }
         }
     }
 }
 
-// Pass 3 in direction dz with the filter (pass-2 states read from `in`, final hashes written to
-// `out`), then the table walk of the lane's line.
+// Moves the lanes' filter passes (bit c2 of `pending` = voxel (c0, u, c2)) into the wave's
+// queue, one voxel per lane and step, and hands the queue to probeQueue whenever the next step
+// might not fit and at the end.
 template <int L>
-__device__ __forceinline__ void lastPassProbe(uint32_t dz, int lead, const uint32_t* in, uint32_t* out, const typename Geo<L>::Row* s_rows,
-    const uint64_t* __restrict__ sortedMasks, const uint64_t* __restrict__ table, uint32_t logCap, const uint32_t* __restrict__ bloom, uint32_t logBloom,
-    uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t r, uint32_t pc, uint32_t d, int c0, int u, ProbeCounts& cnt)
+__device__ __forceinline__ void probePending(uint32_t pending, uint16_t* queue, const uint32_t* fWave, int c0Base, const typename Geo<L>::Row* s_rows,
+    const uint64_t* __restrict__ sortedMasks, const uint64_t* __restrict__ table, uint32_t logCap, uint64_t* __restrict__ best, uint32_t cfgBits,
+    uint32_t r, uint32_t pc, uint32_t d, ProbeCounts& cnt)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t count = 0;
+    for (;;) {
+        const uint64_t has = __ballot(pending != 0u);
+        const uint32_t n = (uint32_t)__popcll(has);
+        if (n != 0 && count + n <= kProbeQueue) {
+            if (pending != 0u) {
+                const uint32_t pos = count + __builtin_amdgcn_mbcnt_hi((uint32_t)(has >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)has, 0u));
+                const uint32_t c2 = (uint32_t)__ffs(pending) - 1u;
+                pending &= pending - 1u;
+                queue[pos] = (uint16_t)((lane << 5) | c2);
+            }
+            count += n;
+            continue;
+        }
+        if (count == 0)
+            break;
+        waveSync();
+        probeQueue<L>(queue, count, fWave, c0Base, s_rows, sortedMasks, table, logCap, best, cfgBits, r, pc, d, cnt);
+        waveSync();
+        count = 0;
+    }
+}
+
+// Pass 3 in direction dz with the filter (pass-2 states read from `in`, final hashes written to
+// `out`, which is the lane's line of the wave's planes outWave), then the table walks of the
+// wave's filter passes.
+template <int L>
+__device__ __forceinline__ void lastPassProbe(uint32_t dz, int lead, const uint32_t* in, uint32_t* out, const uint32_t* outWave, uint16_t* queue, int c0Base,
+    const typename Geo<L>::Row* s_rows, const uint64_t* __restrict__ sortedMasks, const uint64_t* __restrict__ table, uint32_t logCap,
+    const uint32_t* __restrict__ bloom, uint32_t logBloom, uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t r, uint32_t pc, uint32_t d,
+    ProbeCounts& cnt)
 {
     const uint32_t pending = dz ? pass3Filter<L, true>(in, out, bloom, logBloom, cfgBits, lead, cnt)
                                 : pass3Filter<L, false>(in, out, bloom, logBloom, cfgBits, lead, cnt);
-    probeLine<L>(pending, out, s_rows, sortedMasks, table, logCap, best, cfgBits, r, pc, d, c0, u, cnt);
+    waveSync();
+    probePending<L>(pending, queue, outWave, c0Base, s_rows, sortedMasks, table, logCap, best, cfgBits, r, pc, d, cnt);
 }
 
 // Search-unit kinds of the level-3..5 kernel.
@@ -625,7 +681,8 @@ constexpr int kUnitShared = 2; // a family of 8 combos; passes 1 and 2 shared by
 template <int L, int Waves, int Kind>
 constexpr size_t searchLds()
 {
-    return 3 * Geo<L>::Rows * sizeof(typename Geo<L>::Row) + (Kind == kUnitShared ? 2 : 1) * (size_t)Waves * Geo<L>::P * Geo<L>::Slab * 4 + 3 * Waves * 4;
+    return Geo<L>::Rows * sizeof(typename Geo<L>::Row) + (Kind == kUnitShared ? 2 : 1) * (size_t)Waves * Geo<L>::P * Geo<L>::Slab * 4 + 3 * Waves * 4 +
+        (size_t)Waves * kProbeQueue * 2;
 }
 
 // Search of one unit of parent rank r (rows already in s_rows, occupancy occ) by `Waves` waves:
@@ -635,7 +692,7 @@ constexpr size_t searchLds()
 // no voxel.
 template <int L, int Waves, int Kind>
 __device__ __forceinline__ void searchUnit(uint32_t r, uint32_t unit, uint32_t chunk, int wave, const typename Geo<L>::Row* s_rows, const Occupancy& occ,
-    uint32_t* planes0, uint32_t* planes1, const uint64_t* __restrict__ layouts, const uint64_t* __restrict__ table, uint32_t logCap,
+    uint32_t* planes0, uint32_t* planes1, uint16_t* queue, const uint64_t* __restrict__ layouts, const uint64_t* __restrict__ table, uint32_t logCap,
     const uint32_t* __restrict__ bloom, uint32_t logBloom, uint64_t* __restrict__ best, uint32_t cfgBits, ProbeCounts& cnt)
 {
     using G = Geo<L>;
@@ -643,11 +700,13 @@ __device__ __forceinline__ void searchUnit(uint32_t r, uint32_t unit, uint32_t c
     const bool axisOn = (cfgBits & kCfgAxis) != 0;
     const int lane = (int)(threadIdx.x & 63u);
     const int u = lane & (R - 1), w = lane >> L;
-    const int c0 = ((int)chunk * Waves + wave) * G::P + w;
+    const int c0Base = ((int)chunk * Waves + wave) * G::P;
+    const int c0 = c0Base + w;
     const int planeOff = (wave * G::P + w) * G::Slab;
     uint32_t* plane = planes0 + planeOff;
     uint32_t* line = plane + u; // the lane's pass-3 line: c1 = u, stride Pitch along c2
     uint32_t* outLine = planes1 + planeOff + u;
+    const uint32_t* outWave = planes1 + wave * G::P * G::Slab;
     // Pass 1 along p0 in direction bit 0 of d reads the rows' segments toward c0; plane c0 is
     // empty iff that segment of the occupancy along p0 is. p0 and d's bit 0 are fixed per unit.
     const uint32_t p0 = Kind == kUnitCombo ? (unit >> 3) >> 1 : unit >> 1;
@@ -659,8 +718,8 @@ __device__ __forceinline__ void searchUnit(uint32_t r, uint32_t unit, uint32_t c
         const uint32_t pc = unit >> 3, d = unit & 7u;
         passes12<L>(pc, d, s_rows, c0, u, plane, emptyLead<L>(occ.axis(permAxis(pc, 1)), (d >> 1) & 1u));
         waveSync();
-        lastPassProbe<L>((d >> 2) & 1u, emptyLead<L>(occ.axis(permAxis(pc, 2)), (d >> 2) & 1u), line, line, s_rows, layouts, table, logCap, bloom, logBloom,
-            best, cfgBits, r, pc, d, c0, u, cnt);
+        lastPassProbe<L>((d >> 2) & 1u, emptyLead<L>(occ.axis(permAxis(pc, 2)), (d >> 2) & 1u), line, line, planes0 + wave * G::P * G::Slab, queue, c0Base,
+            s_rows, layouts, table, logCap, bloom, logBloom, best, cfgBits, r, pc, d, cnt);
     } else {
 #pragma unroll 1
         for (uint32_t sel = 0; sel < 2; ++sel) {
@@ -684,8 +743,8 @@ __device__ __forceinline__ void searchUnit(uint32_t r, uint32_t unit, uint32_t c
                         passes12<L>(pc, d, s_rows, c0, u, plane, lead2);
                         waveSync();
                     }
-                    lastPassProbe<L>(dz, emptyLead<L>(occ2, dz), line, outLine, s_rows, layouts, table, logCap, bloom, logBloom, best, cfgBits, r, pc, d,
-                        c0, u, cnt);
+                    lastPassProbe<L>(dz, emptyLead<L>(occ2, dz), line, outLine, outWave, queue, c0Base, s_rows, layouts, table, logCap, bloom, logBloom, best,
+                        cfgBits, r, pc, d, cnt);
                 }
             }
         }
