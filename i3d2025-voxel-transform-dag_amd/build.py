@@ -30,6 +30,7 @@ COMMON = [
     "-I", os.path.join(REPO, "include"),
     "-I", CSRC,
     "-Wall",
+    "-Wshadow",
     "-Wno-unused-result",
 ]
 

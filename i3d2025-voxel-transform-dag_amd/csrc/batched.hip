@@ -57,15 +57,22 @@ constexpr uint32_t kL5Chunks = Geo<5>::MaxWaves / kBatchWaves; // plane chunks o
 static_assert(Geo<4>::MaxWaves == kBatchWaves, "a level-4 tile covers all planes");
 static_assert(Geo<3>::MaxWaves == 1, "a level-3 wave item covers all planes");
 
-uint32_t tilesFor(uint32_t level, uint32_t parents)
+// Search configuration bits of level view v (adds kCfgChildFrames where its table uses them).
+__device__ __forceinline__ uint32_t levelCfgBits(const LevelView& v, uint32_t cfgBits)
 {
-    switch (level) {
+    return v.childFrames == 6 ? (cfgBits | kCfgChildFrames) : (cfgBits & ~kCfgChildFrames);
+}
+
+uint32_t tilesFor(const MatchPlan& p, uint32_t parents)
+{
+    const uint32_t cfg = p.childFrames == 6 ? kCfgChildFrames : 0u;
+    switch (p.level) {
     case 5:
-        return parents * 6u * kL5Chunks;
+        return parents * searchUnits(kUnitFamily, cfg) * kL5Chunks;
     case 4:
-        return parents * 48u;
+        return parents * searchUnits(kUnitCombo, cfg);
     case 3:
-        return (parents * 6u + kBatchWaves - 1) / kBatchWaves;
+        return (parents * searchUnits(kUnitShared, cfg) + kBatchWaves - 1) / kBatchWaves;
     default:
         return (parents + kBatchWaves - 1) / kBatchWaves;
     }
@@ -85,7 +92,7 @@ __device__ __forceinline__ void blockTile(const LevelView& v, uint32_t r, uint32
     if (!loadRowsOcc<L, kBatchThreads>(v.sortedMasks, v.n, r, Kind == kUnitCombo ? (unit >> 3) >> 1 : unit >> 1, s_rows, s_occ, occ))
         return;
     ProbeCounts cnt;
-    searchUnit<L, kBatchWaves, Kind>(r, unit, chunk, (int)(threadIdx.x >> 6), s_rows, occ, planes, planes, queue, v.sortedMasks, v.tabKeys, v.logCap, v.bloom, v.logBloom, v.best,
+    searchUnit<L, kBatchWaves, Kind>(r, unit, chunk, (int)(threadIdx.x >> 6), s_rows, occ, planes, planes, queue, v.sortedMasks, v.n, v.tabKeys, v.logCap, v.bloom, v.logBloom, v.best,
         cfgBits, cnt);
     flushCounts(cnt, v.counters, cfgBits);
 }
@@ -104,7 +111,7 @@ __device__ __forceinline__ void waveItem3(const LevelView& v, uint32_t r, uint32
     if (!loadRowsWave<3>(v.sortedMasks, v.n, r, unit >> 1, s_rows, occ))
         return;
     ProbeCounts cnt;
-    searchUnit<3, 1, kUnitShared>(r, unit, 0, 0, s_rows, occ, planes0, planes1, queue, v.sortedMasks, v.tabKeys, v.logCap, v.bloom, v.logBloom, v.best, cfgBits, cnt);
+    searchUnit<3, 1, kUnitShared>(r, unit, 0, 0, s_rows, occ, planes0, planes1, queue, v.sortedMasks, v.n, v.tabKeys, v.logCap, v.bloom, v.logBloom, v.best, cfgBits, cnt);
     flushCounts(cnt, v.counters, cfgBits);
 }
 
@@ -174,19 +181,24 @@ __global__ __launch_bounds__(kBatchThreads, 3) void kBatched(const BatchDesc* __
                 const BatchPhaseLevel& pl = desc->ph[p][li];
                 const LevelView& v = desc->lv[li];
                 const uint32_t k = t - pl.tileBase;
+                const uint32_t lvCfg = levelCfgBits(v, cfgBits);
                 switch (v.level) {
                 case 5: {
-                    const uint32_t m = k % (6u * kL5Chunks);
-                    blockTile<5, kUnitFamily>(v, pl.r0 + k / (6u * kL5Chunks), m / kL5Chunks, m % kL5Chunks, s_mem, s_occ, s_queue[wave], cfgBits);
+                    const uint32_t per = searchUnits(kUnitFamily, lvCfg) * kL5Chunks;
+                    const uint32_t m = k % per;
+                    blockTile<5, kUnitFamily>(v, pl.r0 + k / per, m / kL5Chunks, m % kL5Chunks, s_mem, s_occ, s_queue[wave], lvCfg);
                     break;
                 }
-                case 4:
-                    blockTile<4, kUnitCombo>(v, pl.r0 + k / 48u, k % 48u, 0, s_mem, s_occ, s_queue[wave], cfgBits);
+                case 4: {
+                    const uint32_t per = searchUnits(kUnitCombo, lvCfg);
+                    blockTile<4, kUnitCombo>(v, pl.r0 + k / per, k % per, 0, s_mem, s_occ, s_queue[wave], lvCfg);
                     break;
+                }
                 case 3: {
+                    const uint32_t per = searchUnits(kUnitShared, lvCfg);
                     const uint32_t item = k * kBatchWaves + wave;
-                    if (item < (pl.r1 - pl.r0) * 6u)
-                        waveItem3(v, pl.r0 + item / 6u, item % 6u, s_mem, s_queue[wave], cfgBits);
+                    if (item < (pl.r1 - pl.r0) * per)
+                        waveItem3(v, pl.r0 + item / per, item % per, s_mem, s_queue[wave], lvCfg);
                     break;
                 }
                 default: {
@@ -230,7 +242,8 @@ inline size_t alignUp(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 } // namespace
 
-hipError_t planBatch(uint32_t count, const uint32_t* levels, const uint64_t* ns, const uint32_t* firstBatch, const uint32_t* bloomBits, int device, BatchPlan* out)
+hipError_t planBatch(uint32_t count, const uint32_t* levels, const uint64_t* ns, const uint32_t* firstBatch, const uint32_t* bloomBits, const uint32_t* childFrames,
+    int device, BatchPlan* out)
 {
     if (count == 0 || count > kBatchMaxLevels)
         return hipErrorInvalidValue;
@@ -247,10 +260,10 @@ hipError_t planBatch(uint32_t count, const uint32_t* levels, const uint64_t* ns,
     size_t off = 0;
     for (uint32_t i = 0; i < count; ++i) {
         const uint32_t j = idx[i];
-        if (levels[j] < 2 || levels[j] > 5 || ns[j] >= (1ull << 28))
+        if (levels[j] < 2 || levels[j] > 5 || ns[j] >= (levels[j] == 2 ? (1ull << 28) : (1ull << 26)))
             return hipErrorInvalidValue;
         b.job[i] = j;
-        b.plans[i] = planMatch(levels[j], ns[j], 0, firstBatch ? firstBatch[j] : 0, 2, bloomBits ? bloomBits[j] : 0);
+        b.plans[i] = planMatch(levels[j], ns[j], 0, firstBatch ? firstBatch[j] : 0, 2, bloomBits ? bloomBits[j] : 0, childFrames ? childFrames[j] : 0);
         b.scratchOff[i] = off;
         off += alignUp(b.plans[i].scratchBytes, 256);
     }
@@ -299,7 +312,7 @@ void fillBatchDesc(const BatchPlan& b, void* d_scratch, uint32_t cfgBits, uint32
             if (p < ends[i].size()) {
                 pl.r0 = p == 0 ? 0u : ends[i][p - 1];
                 pl.r1 = ends[i][p];
-                pl.tiles = tilesFor(b.plans[i].level, pl.r1 - pl.r0);
+                pl.tiles = tilesFor(b.plans[i], pl.r1 - pl.r0);
                 pl.prune = pl.r1 < (uint32_t)b.plans[i].n ? 1u : 0u;
                 pl.pruneKey = pruneKey(b.plans[i].level, pl.r1);
             }

@@ -49,10 +49,11 @@ struct BatchPlan {
 };
 
 // Plans a batched call over `count` levels (distinct, 2..5) with per-level schedules taken from
-// firstBatch[i] / bloomBits[i] (0 = defaults). The phase p of the persistent kernel runs batch p
-// of every level: ranks are popcount-descending, so each batch is a popcount bucket of parents.
-hipError_t planBatch(uint32_t count, const uint32_t* levels, const uint64_t* ns, const uint32_t* firstBatch, const uint32_t* bloomBits, int device,
-    BatchPlan* out);
+// firstBatch[i] / bloomBits[i] / childFrames[i] (0 = defaults; each array may be null). The phase
+// p of the persistent kernel runs batch p of every level: ranks are popcount-descending, so each
+// batch is a popcount bucket of parents.
+hipError_t planBatch(uint32_t count, const uint32_t* levels, const uint64_t* ns, const uint32_t* firstBatch, const uint32_t* bloomBits, const uint32_t* childFrames,
+    int device, BatchPlan* out);
 
 // Writes the BatchDesc for scratch base `d_scratch` into host memory `desc`.
 void fillBatchDesc(const BatchPlan& plan, void* d_scratch, uint32_t cfgBits, uint32_t grab, BatchDesc* desc);

@@ -63,8 +63,11 @@ tdag_status checkArgs(tdag_context* ctx, uint32_t level, uint64_t n, const tdag_
         return fail(TDAG_EINVAL, "config is null");
     if (level < 2 || level > 5)
         return fail(TDAG_EINVAL, "level " + std::to_string(level) + " is outside the supported range [2, 5]");
-    if (n >= (1ull << 28))
-        return fail(TDAG_EINVAL, "item count " + std::to_string(n) + " exceeds the supported maximum 2^28 - 1");
+    // Level 2 stores ranks in 29 bits of a table value, levels 3..5 in 26 bits of a table entry.
+    const uint32_t rankBits = level == 2 ? 28 : 26;
+    if (n >= (1ull << rankBits))
+        return fail(TDAG_EINVAL, "item count " + std::to_string(n) + " exceeds the supported maximum 2^" + std::to_string(rankBits) + " - 1 at level " +
+            std::to_string(level));
     return TDAG_OK;
 }
 
@@ -167,7 +170,8 @@ tdag_status tdag_match_level_device(tdag_context* ctx, uint32_t level, const uin
     CAPI_HIP(hipSetDevice(ctx->device), "hipSetDevice");
     hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
     const int32_t v = ctx->variant[level];
-    const tdag::MatchPlan plan = tdag::planMatch(level, n, v >= 0 ? (uint32_t)v : tdag::defaultVariant(level), ctx->firstBatch[level], ctx->batchGrowth[level], ctx->bloomBits[level]);
+    const tdag::MatchPlan plan = tdag::planMatch(level, n, v >= 0 ? (uint32_t)v : tdag::defaultVariant(level), ctx->firstBatch[level], ctx->batchGrowth[level],
+        ctx->bloomBits[level], ctx->childFrames[level]);
     if (tdag_status s = ensureBuffer(ctx->scratch, ctx->scratchBytes, plan.scratchBytes, "matcher scratch allocation"); s != TDAG_OK)
         return s;
     CAPI_HIP(hipEventRecord(ctx->ev[0], st), "hipEventRecord");
@@ -306,19 +310,35 @@ tdag_status tdag_autotune_level(tdag_context* ctx, uint32_t level, const uint64_
     if (out_ms)
         for (uint32_t v = 0; v <= kVariantSlots; ++v)
             out_ms[v] = -1.f;
-    // 1. Every valid variant at the default schedule.
+    // 1. Every valid variant at the default schedule, for both child-table frame counts
+    //    (levels 3..5: the parent runs 48 combos against 8 entries per child, or 8 combos against
+    //    48 entries per child).
     std::vector<std::pair<float, uint32_t>> byTime;
-    for (uint32_t v = 0; v < tdag::kNumVariants; ++v) {
-        if (!tdag::variantValid(level, v))
-            continue;
-        float avg = 0.f;
-        if (tdag_status s = timePlan(ctx, tdag::planMatch(level, n, v), d_masks, cfgBits, 1, repeats, &avg); s != TDAG_OK)
-            return s;
+    uint32_t bestFrames = level == 2 ? 1u : tdag::defaultChildFrames(level);
+    {
+        const uint32_t frameCands[2] = { 1, 6 };
+        float bestOfFrames = -1.f;
+        for (uint32_t fi = 0; fi < (level == 2 ? 1u : 2u); ++fi) {
+            std::vector<std::pair<float, uint32_t>> times;
+            for (uint32_t v = 0; v < tdag::kNumVariants; ++v) {
+                if (!tdag::variantValid(level, v))
+                    continue;
+                float avg = 0.f;
+                if (tdag_status s = timePlan(ctx, tdag::planMatch(level, n, v, 0, 0, 0, frameCands[fi]), d_masks, cfgBits, 1, repeats, &avg); s != TDAG_OK)
+                    return s;
+                times.emplace_back(avg, v);
+            }
+            std::sort(times.begin(), times.end());
+            if (!times.empty() && (bestOfFrames < 0.f || times[0].first < bestOfFrames)) {
+                bestOfFrames = times[0].first;
+                bestFrames = level == 2 ? 1u : frameCands[fi];
+                byTime = times;
+            }
+        }
         if (out_ms)
-            out_ms[v] = avg;
-        byTime.emplace_back(avg, v);
+            for (const auto& t : byTime)
+                out_ms[t.second] = t.first;
     }
-    std::sort(byTime.begin(), byTime.end());
     int32_t best = byTime.empty() ? 0 : (int32_t)byTime[0].second;
     uint32_t bestFirst = 0;
     float bestMs = byTime.empty() ? 0.f : byTime[0].first;
@@ -331,7 +351,8 @@ tdag_status tdag_autotune_level(tdag_context* ctx, uint32_t level, const uint64_
         for (size_t vi = 0; vi < std::min<size_t>(3, byTime.size()); ++vi) {
             for (uint32_t ci = 0; ci < 4; ++ci) {
                 float avg = 0.f;
-                if (tdag_status s = timePlan(ctx, tdag::planMatch(level, n, byTime[vi].second, cands[ci], 2), d_masks, cfgBits, 0, repeats, &avg); s != TDAG_OK)
+                if (tdag_status s = timePlan(ctx, tdag::planMatch(level, n, byTime[vi].second, cands[ci], 2, 0, bestFrames), d_masks, cfgBits, 0, repeats, &avg);
+                    s != TDAG_OK)
                     return s;
                 if (first || avg < bestMs) {
                     first = false;
@@ -349,7 +370,8 @@ tdag_status tdag_autotune_level(tdag_context* ctx, uint32_t level, const uint64_
             if (bitsCands[bi] == defaultBits)
                 continue;
             float avg = 0.f;
-            if (tdag_status s = timePlan(ctx, tdag::planMatch(level, n, (uint32_t)best, bestFirst, 2, bitsCands[bi]), d_masks, cfgBits, 0, repeats, &avg); s != TDAG_OK)
+            if (tdag_status s = timePlan(ctx, tdag::planMatch(level, n, (uint32_t)best, bestFirst, 2, bitsCands[bi], bestFrames), d_masks, cfgBits, 0, repeats, &avg);
+                s != TDAG_OK)
                 return s;
             if (avg < bestMs) {
                 bestMs = avg;
@@ -362,11 +384,28 @@ tdag_status tdag_autotune_level(tdag_context* ctx, uint32_t level, const uint64_
         if (out_ms)
             out_ms[kVariantSlots] = bestMs;
     }
+    ctx->childFrames[level] = bestFrames;
     ctx->variant[level] = best;
     if (out_best)
         *out_best = best;
     if (out_first_batch)
         *out_first_batch = bestFirst;
+    return TDAG_OK;
+}
+
+tdag_status tdag_context_set_child_frames(tdag_context* ctx, uint32_t level, uint32_t frames)
+{
+    if (!ctx || level < 3 || level > 5 || (frames != 0 && frames != 1 && frames != 6))
+        return fail(TDAG_EINVAL, "bad context, level (3..5) or frame count (0, 1 or 6)");
+    ctx->childFrames[level] = frames;
+    return TDAG_OK;
+}
+
+tdag_status tdag_context_get_child_frames(const tdag_context* ctx, uint32_t level, uint32_t* frames)
+{
+    if (!ctx || !frames || level < 3 || level > 5)
+        return fail(TDAG_EINVAL, "bad argument");
+    *frames = ctx->childFrames[level] ? ctx->childFrames[level] : tdag::defaultChildFrames(level);
     return TDAG_OK;
 }
 
@@ -456,7 +495,7 @@ tdag_status tdag_batch_create(tdag_context* ctx, const tdag_level_job* jobs, uin
         return fail(TDAG_EINVAL, "null argument");
     if (count == 0 || count > tdag::kBatchMaxLevels)
         return fail(TDAG_EINVAL, "a batch holds 1 to 4 levels");
-    uint32_t levels[tdag::kBatchMaxLevels], first[tdag::kBatchMaxLevels], bits[tdag::kBatchMaxLevels];
+    uint32_t levels[tdag::kBatchMaxLevels], first[tdag::kBatchMaxLevels], bits[tdag::kBatchMaxLevels], frames[tdag::kBatchMaxLevels];
     uint64_t ns[tdag::kBatchMaxLevels];
     for (uint32_t i = 0; i < count; ++i) {
         if (tdag_status s = checkArgs(ctx, jobs[i].level, jobs[i].n, cfg); s != TDAG_OK)
@@ -470,6 +509,7 @@ tdag_status tdag_batch_create(tdag_context* ctx, const tdag_level_job* jobs, uin
         ns[i] = jobs[i].n;
         first[i] = ctx->firstBatch[jobs[i].level];
         bits[i] = ctx->bloomBits[jobs[i].level];
+        frames[i] = ctx->childFrames[jobs[i].level];
     }
     CAPI_HIP(hipSetDevice(ctx->device), "hipSetDevice");
     auto* b = new tdag_batch();
@@ -479,7 +519,7 @@ tdag_status tdag_batch_create(tdag_context* ctx, const tdag_level_job* jobs, uin
         tdag_batch_destroy(b);
         return s;
     };
-    if (hipError_t e = tdag::planBatch(count, levels, ns, first, bits, ctx->device, &b->plan); e != hipSuccess)
+    if (hipError_t e = tdag::planBatch(count, levels, ns, first, bits, frames, ctx->device, &b->plan); e != hipSuccess)
         return cleanup(failHip(e, "batched matcher plan"));
     if (hipError_t e = hipMalloc(&b->scratch, b->plan.scratchBytes); e != hipSuccess)
         return cleanup(failHip(e, "batched matcher scratch allocation"));

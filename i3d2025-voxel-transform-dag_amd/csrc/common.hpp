@@ -21,6 +21,9 @@ TDAG_HD uint32_t permAxis(uint32_t code, uint32_t a)
     return (packed[code] >> (2 * a)) & 3u;
 }
 
+// Code of the permutation (p0, p1, p2), the inverse of permAxis (transform_dag.h:326-331).
+TDAG_HD uint32_t permCode(uint32_t p0, uint32_t p1, uint32_t p2) { return 2u * p0 + (p2 < p1 ? 1u : 0u); }
+
 // Key field widths: translation fields are level+1 bits each, biased by 2^level.
 TDAG_HD uint32_t keyTBits(uint32_t level) { return level + 1; }
 TDAG_HD uint32_t keyLowBits(uint32_t level) { return 6 + 3 * (level + 1); }

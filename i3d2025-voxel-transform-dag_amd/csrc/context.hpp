@@ -25,6 +25,7 @@ struct tdag_context {
     uint32_t firstBatch[6] = { 0, 0, 0, 0, 0, 0 }; // rank-batched search schedule per level (0 = default)
     uint32_t batchGrowth[6] = { 0, 0, 0, 0, 0, 0 };
     uint32_t bloomBits[6] = { 0, 0, 0, 0, 0, 0 }; // Bloom filter bits per table entry (0 = default)
+    uint32_t childFrames[6] = { 0, 0, 0, 0, 0, 0 }; // levels 3..5: child-table permutation frames, 1 or 6 (0 = default)
 };
 
 namespace tdag {

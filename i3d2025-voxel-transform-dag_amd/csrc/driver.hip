@@ -198,12 +198,12 @@ static tdag_status constructTransformDag(tdag_context* ctx, const tdag_svdag* in
                 std::vector<uint64_t>().swap(rec.masks);
             if (level == 2) {
                 t.leaves.resize(nu);
-                for (uint64_t k = 0; k < nu; ++k)
-                    t.leaves[k] = dag.leaves[indices[uniq[k]]];
+                for (uint64_t ui = 0; ui < nu; ++ui)
+                    t.leaves[ui] = dag.leaves[indices[uniq[ui]]];
             } else {
                 t.nodes[level].resize(nu);
-                for (uint64_t k = 0; k < nu; ++k)
-                    t.nodes[level][k] = toTransformNode(dag.nodes[level][indices[uniq[k]]]);
+                for (uint64_t ui = 0; ui < nu; ++ui)
+                    t.nodes[level][ui] = toTransformNode(dag.nodes[level][indices[uniq[ui]]]);
             }
             for (auto& node : parents)
                 for (uint64_t& c : node) {

@@ -155,23 +155,42 @@ __global__ __launch_bounds__(256) void kLayouts(uint64_t* __restrict__ layouts, 
     }
 }
 
-// One wave per item: the hash of the whole grid seen from each of its 8 corners (identity
-// permutation, direction = corner) goes into the table and the Bloom filter. It is the value the
-// search computes at the voxel where every scan ends, computed with the same functions: R lanes
-// per corner run passes 1 and 2 along y for one z each; the corner's first lane folds pass 3.
+// One wave per (item, frame): the hash of the whole grid in the pass frame of permutation code
+// q = blockIdx.y (1 frame: q = 0 only; 6 frames: every q) seen from each of its 8 corners goes
+// into the table and the Bloom filter. Pass a reads child axis sigma_a with sigma = q^-1, so the
+// value equals what the search computes, at the voxel where every scan ends, for a parent whose
+// content is the child read in that frame. R lanes per corner run passes 1 and 2 along sigma_1
+// for one coordinate along sigma_2 each; the corner's first lane folds pass 3.
 template <int L>
 __global__ __launch_bounds__(64) void kCornerTable(const uint64_t* __restrict__ layouts, uint32_t n, uint64_t* __restrict__ table, uint32_t logCap,
     uint32_t* __restrict__ bloom, uint32_t logBloom)
 {
     using G = Geo<L>;
+    using Row = typename G::Row;
     constexpr int R = G::R;
-    __shared__ typename G::Row s_rows[G::Rows];
+    __shared__ Row s_rows[G::Rows];
     const uint32_t r = blockIdx.x;
-    if (!loadRows<L, 64>(layouts, n, r, s_rows, 1))
+    const uint32_t q = blockIdx.y;
+    // sigma = q^-1: pass a reads the child axis b with permAxis(q, b) == a.
+    uint32_t sigma[3];
+#pragma unroll
+    for (uint32_t b = 0; b < 3; ++b)
+        sigma[permAxis(q, b)] = b;
+    const Row* g = reinterpret_cast<const Row*>(layouts) + ((size_t)sigma[0] * n + r) * G::R2;
+    int any = 0;
+    for (int j = (int)threadIdx.x; j < G::R2; j += 64) {
+        const Row v = g[j];
+        s_rows[j + (j >> L)] = v;
+        any |= (v != 0);
+    }
+    if (__syncthreads_or(any) == 0)
         return;
     const uint32_t capMask = (1u << logCap) - 1u;
     const int lane = (int)threadIdx.x;
     const int c2 = lane & (R - 1), sub = lane >> L;
+    // Rows of layout sigma_0 are indexed by the coordinates along the other two axes in ascending
+    // axis order.
+    const bool lowFirst = sigma[1] < sigma[2];
     for (int cb = 0; cb < 8; cb += G::P) {
         const uint32_t corner = (uint32_t)(cb + sub);
         const bool pos0 = corner & 1u, pos1 = corner & 2u, pos2 = corner & 4u;
@@ -180,7 +199,7 @@ __global__ __launch_bounds__(64) void kCornerTable(const uint64_t* __restrict__ 
         uint32_t s = 0;
         for (int k = 0; k < R; ++k) {
             const int c1 = pos1 ? (R - 1 - k) : k;
-            const uint32_t row = (uint32_t)s_rows[c1 + G::Pitch * c2]; // layout 0: row along x at (y, z) = (c1, c2)
+            const uint32_t row = (uint32_t)s_rows[lowFirst ? c1 + G::Pitch * c2 : c2 + G::Pitch * c1];
             s = hashStep(s, pos0 ? rowPrefix<true>(row, sh) : rowPrefix<false>(row, sh));
         }
         uint32_t s3 = 0;
@@ -191,7 +210,7 @@ __global__ __launch_bounds__(64) void kCornerTable(const uint64_t* __restrict__ 
         const uint32_t h = finalHash(s3);
         if (c2 == 0 && h != 0) {
             atomicOr(&bloom[bloomWordNested(h, logBloom)], bloomBits(h));
-            const uint64_t packed = ((uint64_t)h << 32) | (uint64_t)((r << 3) | corner);
+            const uint64_t packed = packEntry(h, r, q, corner);
             uint32_t slot = homeSlot32(h, logCap);
             for (;;) {
                 const unsigned long long prev = atomicCAS((unsigned long long*)&table[slot], 0ull, (unsigned long long)packed);
@@ -206,10 +225,11 @@ __global__ __launch_bounds__(64) void kCornerTable(const uint64_t* __restrict__ 
 // Search kernel for levels 3..5. One block per (parent rank, unit, plane chunk) with `Waves`
 // waves; every wave owns P = 64/R planes (fixed coordinate along the first pass axis) and runs
 // all passes, the filter and the table walk for them on its own: planes are independent, so
-// after the rows are loaded there is no block barrier.
-//   kUnitCombo:  grid.y = 48 combos.
-//   kUnitFamily: grid.y = 6 families (first-pass axis a0, direction dx) of 8 combos each; the
-//                rows are loaded once per family.
+// after the rows are loaded there is no block barrier. Units per parent (searchUnits):
+//   kUnitCombo:  48 (permutation, direction) combos, or the 8 directions of permutation 0 when
+//                the table holds the children in all six frames (kCfgChildFrames).
+//   kUnitFamily: 6 families (first-pass axis a0, direction dx) of 8 combos each, or 2 families
+//                of 4 directions; the rows are loaded once per family.
 //   kUnitShared: as kUnitFamily, and passes 1 and 2 run once for both directions of pass 3
 //                (their states stay in a second plane buffer).
 // The grid is one-dimensional. XcdGrouped: decoded by xcdBlock, so all (unit, chunk) blocks of
@@ -228,7 +248,7 @@ __global__ __launch_bounds__(64 * Waves) void kSearch(const uint64_t* __restrict
     __shared__ uint32_t s_occ[3 * Waves];
     __shared__ uint16_t s_queue[Waves][kProbeQueue];
     ProbeCounts cnt;
-    constexpr uint32_t Units = Kind == kUnitCombo ? 48u : 6u;
+    const uint32_t Units = searchUnits(Kind, cfgBits);
     constexpr uint32_t Chunks = (uint32_t)(G::MaxWaves / Waves);
     uint32_t parent, member;
     if constexpr (XcdGrouped) {
@@ -252,7 +272,7 @@ __global__ __launch_bounds__(64 * Waves) void kSearch(const uint64_t* __restrict
     if (!loadRowsOcc<L, 64 * Waves>(layouts, n, r, a0, s_rows, s_occ, occ))
         return;
     searchUnit<L, Waves, Kind>(r, unit, chunk, (int)(threadIdx.x >> 6), s_rows, occ, s_planes[0], s_planes[Buffers - 1], s_queue[threadIdx.x >> 6], layouts,
-        table, logCap, bloom, logBloom, best, cfgBits, cnt);
+        n, table, logCap, bloom, logBloom, best, cfgBits, cnt);
     flushCounts(cnt, counters, cfgBits);
 }
 
@@ -324,7 +344,7 @@ hipError_t launchSearchShape(uint32_t parents, uint32_t r0, uint32_t n, const ui
     uint32_t logBloom, uint64_t* best, uint32_t cfgBits, unsigned long long* counters, hipStream_t stream)
 {
     if constexpr (Waves >= 1 && Waves <= Geo<L>::MaxWaves && searchLds<L, Waves, Kind>() <= kLdsBytes) {
-        const uint32_t perParent = (Kind == kUnitCombo ? 48u : 6u) * (uint32_t)(Geo<L>::MaxWaves / Waves);
+        const uint32_t perParent = searchUnits(Kind, cfgBits) * (uint32_t)(Geo<L>::MaxWaves / Waves);
         const dim3 grid(XcdGrouped ? xcdGridBlocks(parents, perParent) : parents * perParent);
         hipLaunchKernelGGL((kSearch<L, Waves, Kind, XcdGrouped>), grid, dim3(64 * Waves), 0, stream, layouts, n, table, logCap, bloom, logBloom, best, cfgBits,
             counters, r0, parents);
@@ -432,7 +452,17 @@ uint32_t defaultFilterBits(uint32_t level)
     return level == 3 ? 16u : 4u;
 }
 
-MatchPlan planMatch(uint32_t level, uint64_t n, uint32_t variant, uint32_t firstBatch, uint32_t batchGrowth, uint32_t bloomBitsPerEntry)
+uint32_t defaultChildFrames(uint32_t level)
+{
+    return level >= 4 ? 6u : 1u;
+}
+
+uint32_t searchCfgBits(const MatchPlan& p, uint32_t cfgBits)
+{
+    return (p.level >= 3 && p.childFrames == 6) ? (cfgBits | kCfgChildFrames) : (cfgBits & ~kCfgChildFrames);
+}
+
+MatchPlan planMatch(uint32_t level, uint64_t n, uint32_t variant, uint32_t firstBatch, uint32_t batchGrowth, uint32_t bloomBitsPerEntry, uint32_t childFrames)
 {
     MatchPlan p {};
     p.level = level;
@@ -443,7 +473,8 @@ MatchPlan planMatch(uint32_t level, uint64_t n, uint32_t variant, uint32_t first
     p.R = 1u << level;
     p.R3 = p.R * p.R * p.R;
     p.W = p.R3 >= 64 ? p.R3 / 64 : 1;
-    const uint64_t entries = 8 * (n > 0 ? n : 1);
+    p.childFrames = level == 2 ? 1u : (childFrames == 1 || childFrames == 6 ? childFrames : defaultChildFrames(level));
+    const uint64_t entries = 8ull * p.childFrames * (n > 0 ? n : 1);
     p.logCap = std::max<uint32_t>(6, log2Ceil(4 * entries)); // load factor <= 1/4
     const uint64_t cap = 1ull << p.logCap;
     size_t off = 0;
@@ -486,6 +517,7 @@ LevelView levelView(const MatchPlan& p, void* d_scratch)
     v.counters = (unsigned long long*)(base + p.offCounters);
     v.logCap = p.logCap;
     v.logBloom = p.logBloom;
+    v.childFrames = p.childFrames;
     return v;
 }
 
@@ -563,15 +595,15 @@ hipError_t enqueueBuildTable(const MatchPlan& p, void* d_scratch, hipStream_t st
         break;
     case 3:
         hipLaunchKernelGGL(kLayouts<3>, dim3(n), dim3(256), 0, stream, v.sortedMasks, n);
-        hipLaunchKernelGGL(kCornerTable<3>, dim3(n), dim3(64), 0, stream, v.sortedMasks, n, v.tabKeys, p.logCap, v.bloom, p.logBloom);
+        hipLaunchKernelGGL(kCornerTable<3>, dim3(n, p.childFrames), dim3(64), 0, stream, v.sortedMasks, n, v.tabKeys, p.logCap, v.bloom, p.logBloom);
         break;
     case 4:
         hipLaunchKernelGGL(kLayouts<4>, dim3(n), dim3(256), 0, stream, v.sortedMasks, n);
-        hipLaunchKernelGGL(kCornerTable<4>, dim3(n), dim3(64), 0, stream, v.sortedMasks, n, v.tabKeys, p.logCap, v.bloom, p.logBloom);
+        hipLaunchKernelGGL(kCornerTable<4>, dim3(n, p.childFrames), dim3(64), 0, stream, v.sortedMasks, n, v.tabKeys, p.logCap, v.bloom, p.logBloom);
         break;
     case 5:
         hipLaunchKernelGGL(kLayouts<5>, dim3(n), dim3(256), 0, stream, v.sortedMasks, n);
-        hipLaunchKernelGGL(kCornerTable<5>, dim3(n), dim3(64), 0, stream, v.sortedMasks, n, v.tabKeys, p.logCap, v.bloom, p.logBloom);
+        hipLaunchKernelGGL(kCornerTable<5>, dim3(n, p.childFrames), dim3(64), 0, stream, v.sortedMasks, n, v.tabKeys, p.logCap, v.bloom, p.logBloom);
         break;
     default:
         return hipErrorInvalidValue;
@@ -585,8 +617,9 @@ hipError_t enqueuePrepare(const MatchPlan& p, const uint64_t* d_masks, void* d_s
     return enqueueBuildTable(p, d_scratch, stream, nullptr);
 }
 
-hipError_t enqueueSearch(const MatchPlan& p, uint32_t cfgBits, void* d_scratch, hipStream_t stream)
+hipError_t enqueueSearch(const MatchPlan& p, uint32_t cfgBitsIn, void* d_scratch, hipStream_t stream)
 {
+    const uint32_t cfgBits = searchCfgBits(p, cfgBitsIn);
     const LevelView v = levelView(p, d_scratch);
     const uint32_t n = v.n;
     const uint64_t cap = 1ull << p.logCap;

@@ -15,6 +15,10 @@ constexpr uint32_t kCfgAxis = 2u;
 constexpr uint32_t kCfgTranslation = 4u;
 // Not a TransformDAGConfig flag: collect per-stage probe counters (diagnostics).
 constexpr uint32_t kCfgCount = 8u;
+// Not a TransformDAGConfig flag: the level-3..5 child table holds every child under all six
+// axis permutations (48 entries per child), so the parent search runs the identity permutation
+// only (8 scan directions instead of 48 combos).
+constexpr uint32_t kCfgChildFrames = 16u;
 // Device counters of one call: matched, unique (u32), probes, filter passes, key hits,
 // candidates, verifications, verified.
 constexpr int kNumCounters = 8;
@@ -37,6 +41,7 @@ struct MatchPlan {
     uint32_t level, R, R3, W, logCap, logBloom, variant;
     uint32_t firstBatch, batchGrowth; // rank-batched search schedule
     uint32_t bloomBitsPerEntry; // Bloom filter size per table entry (bits)
+    uint32_t childFrames; // levels 3..5: child-table permutation frames, 1 or 6 (kCfgChildFrames)
     uint64_t n;
     size_t offKeys, offOrder, offSortScratch, offSortedMasks, offBest;
     size_t offTabKeys, offTabVals, offBloom, offParentTp, offIsUnique, offNewIdx, offScanScratch, offCounters;
@@ -44,8 +49,13 @@ struct MatchPlan {
 };
 
 // firstBatch = 0 / batchGrowth = 0 select the defaults (max(64, n/64) parents, doubling).
-// bloomBitsPerEntry = 0 selects defaultFilterBits(level).
-MatchPlan planMatch(uint32_t level, uint64_t n, uint32_t variant, uint32_t firstBatch = 0, uint32_t batchGrowth = 0, uint32_t bloomBitsPerEntry = 0);
+// bloomBitsPerEntry = 0 selects defaultFilterBits(level); childFrames = 0 selects
+// defaultChildFrames(level) (levels 3..5; 1 or 6).
+MatchPlan planMatch(uint32_t level, uint64_t n, uint32_t variant, uint32_t firstBatch = 0, uint32_t batchGrowth = 0, uint32_t bloomBitsPerEntry = 0,
+    uint32_t childFrames = 0);
+uint32_t defaultChildFrames(uint32_t level);
+// Configuration bits of the search for `plan` (adds kCfgChildFrames where the plan uses it).
+uint32_t searchCfgBits(const MatchPlan& plan, uint32_t cfgBits);
 
 // Enqueues the whole matcher on `stream`. Outputs may be null. The device counters at
 // scratch + offCounters hold {num_matched (u64), num_unique (u32)} when the stream completes.
@@ -75,6 +85,7 @@ struct LevelView {
     uint32_t* bloom;
     unsigned long long* counters;
     uint32_t logCap, logBloom;
+    uint32_t childFrames;
 };
 LevelView levelView(const MatchPlan& plan, void* d_scratch);
 

@@ -53,6 +53,16 @@ __device__ __forceinline__ XcdBlock xcdBlock(uint32_t b, uint32_t perGroup)
 // so linear-probing clusters remain intact). Ranks are < 2^28, so this never names an item.
 constexpr uint32_t kDeadRank = 0x1FFFFFFFu;
 
+// Level-3..5 table entries pack (hash << 32) | (rank << 6) | (perm << 3) | corner: the child's
+// grid hashed in the pass frame of axis-permutation code `perm` (pass a reads child axis
+// sigma_a with perm = sigma^-1, so child axis b is read by pass permAxis(perm, b)) from `corner`
+// (bit a: direction of pass a). Ranks are < 2^26; kPackedDead marks a tombstone.
+constexpr uint32_t kPackedDead = 0x3FFFFFFu;
+__device__ __forceinline__ uint64_t packEntry(uint32_t h, uint32_t rank, uint32_t perm, uint32_t corner)
+{
+    return ((uint64_t)h << 32) | (uint64_t)((rank << 6) | (perm << 3) | corner);
+}
+
 // Per-thread probe statistics, summed per wave into the call's counters when kCfgCount is set.
 struct ProbeCounts {
     uint32_t probes = 0, filterPass = 0, keyHits = 0, candidates = 0, verifications = 0, verified = 0;
@@ -495,9 +505,10 @@ __device__ __forceinline__ uint32_t pass3Filter(const uint32_t* in, uint32_t* ou
 // Expected child row (yo, zo) (bits along x) for the transform encoded in `key` applied to the
 // parent rows: out[o] = M[o - t] with M = mirror(permute(P)) (find_translation.cpp:66-84 for the
 // translation). The source of an output row is one line of the parent along axis p0, read from
-// layout p0.
+// the parent's layout p0 (`prow`, rows (lo, hi) at lo + R*hi).
 template <int L>
-__device__ __forceinline__ uint32_t transformedRow(const typename Geo<L>::Row* s_rows, int tx, int ty, int tz, uint32_t sym, uint32_t pc, int yo, int zo)
+__device__ __forceinline__ uint32_t transformedRow(const typename Geo<L>::Row* __restrict__ prow, int tx, int ty, int tz, uint32_t sym, uint32_t pc, int yo,
+    int zo)
 {
     using G = Geo<L>;
     constexpr int R = G::R;
@@ -509,18 +520,20 @@ __device__ __forceinline__ uint32_t transformedRow(const typename Geo<L>::Row* s
     const int a1 = (int)permAxis(pc, 1), a2 = (int)permAxis(pc, 2);
     // Parent coordinates along a1 and a2; the row along a0 is indexed by them in ascending axis order.
     const int lo = a1 < a2 ? m1 : m2, hi = a1 < a2 ? m2 : m1;
-    uint32_t line = (uint32_t)s_rows[lo + G::Pitch * hi]; // s_rows holds layout p0
+    uint32_t line = (uint32_t)prow[lo + R * hi];
     if (sym & 1u)
         line = __brev(line) >> (32 - R);
     const uint32_t rowMask = R == 32 ? 0xFFFFFFFFu : ((1u << R) - 1u);
     return (tx >= 0 ? (line << tx) : (line >> -tx)) & rowMask;
 }
 
-// Wave-cooperative bitwise check child bc == T(parent) for the transform in key bk.
+// Wave-cooperative bitwise check child bc == T(parent r) for the transform in key bk; the n
+// items' three layouts follow each other in `layouts` (layout 0 = the rank-ordered masks).
 template <int L>
-__device__ __forceinline__ bool verifyWave(const typename Geo<L>::Row* s_rows, const uint64_t* __restrict__ sortedMasks, uint32_t bc, uint64_t bk, uint32_t lane)
+__device__ __forceinline__ bool verifyWave(const uint64_t* __restrict__ layouts, uint32_t n, uint32_t r, uint32_t bc, uint64_t bk, uint32_t lane)
 {
     using G = Geo<L>;
+    using Row = typename G::Row;
     constexpr int R = G::R;
     constexpr uint32_t b = L + 1;
     constexpr uint64_t m = (1ull << b) - 1ull;
@@ -529,10 +542,11 @@ __device__ __forceinline__ bool verifyWave(const typename Geo<L>::Row* s_rows, c
     const int tz = (int)((bk >> (2 * b)) & m) - R;
     const uint32_t sym = (uint32_t)((bk >> (3 * b)) & 7u);
     const uint32_t pc = (uint32_t)((bk >> (3 * b + 3)) & 7u);
-    const typename G::Row* child = reinterpret_cast<const typename G::Row*>(sortedMasks) + (size_t)bc * G::R2;
+    const Row* child = reinterpret_cast<const Row*>(layouts) + (size_t)bc * G::R2;
+    const Row* prow = reinterpret_cast<const Row*>(layouts) + ((size_t)permAxis(pc, 0) * n + r) * G::R2;
     bool ok = true;
     for (int j = (int)lane; j < G::R2; j += 64)
-        ok = ok && (transformedRow<L>(s_rows, tx, ty, tz, sym, pc, j & (R - 1), j >> L) == (uint32_t)child[j]);
+        ok = ok && (transformedRow<L>(prow, tx, ty, tz, sym, pc, j & (R - 1), j >> L) == (uint32_t)child[j]);
     return __ballot(!ok) == 0ull;
 }
 
@@ -546,9 +560,9 @@ constexpr uint32_t kProbeQueue = 128;
 // voxels came from. Lanes stop at each candidate that beats best[c]; the wave then verifies the
 // stopped lanes' candidates together and resumes.
 template <int L>
-__device__ __forceinline__ void probeQueue(const uint16_t* queue, uint32_t count, const uint32_t* fWave, int c0Base, const typename Geo<L>::Row* s_rows,
-    const uint64_t* __restrict__ sortedMasks, const uint64_t* __restrict__ table, uint32_t logCap, uint64_t* __restrict__ best, uint32_t cfgBits,
-    uint32_t r, uint32_t pc, uint32_t d, ProbeCounts& cnt)
+__device__ __forceinline__ void probeQueue(const uint16_t* queue, uint32_t count, const uint32_t* fWave, int c0Base, const uint64_t* __restrict__ layouts,
+    uint32_t n, const uint64_t* __restrict__ table, uint32_t logCap, uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t r, uint32_t pc, uint32_t d,
+    ProbeCounts& cnt)
 {
     using G = Geo<L>;
     constexpr int R = G::R;
@@ -581,24 +595,38 @@ __device__ __forceinline__ void probeQueue(const uint16_t* queue, uint32_t count
                 if ((uint32_t)(e >> 32) != h)
                     continue;
                 ++cnt.keyHits;
-                const uint32_t cc = (uint32_t)e >> 3;
-                const uint32_t kc = (uint32_t)e & 7u;
-                if (cc <= r || cc == kDeadRank)
+                const uint32_t cc = (uint32_t)e >> 6;
+                const uint32_t q = ((uint32_t)e >> 3) & 7u, kc = (uint32_t)e & 7u;
+                if (cc <= r || cc == kPackedDead)
                     continue;
-                // Translation of the anchored match (find_translation.cpp:835-837); the voxel's
-                // coordinates along the three pass axes are (c0, u, c2).
+                // Per pass index a: mirror bit and translation of the anchored match
+                // (find_translation.cpp:835-837); the voxel's coordinates along the three pass
+                // axes are (c0, u, c2).
                 const int voxel[3] = { c0, u, c2 };
-                const uint32_t sym = d ^ kc;
-                int tt[3];
+                const uint32_t sa = d ^ kc;
+                int ta[3];
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
                     const int anchor = ((d >> a) & 1u) ? 0 : R - 1;
-                    tt[a] = ((sym >> a) & 1u) ? voxel[a] - anchor : anchor - voxel[a];
+                    ta[a] = ((sa >> a) & 1u) ? voxel[a] - anchor : anchor - voxel[a];
                 }
-                if (!configAllows(cfgBits, pc, sym, tt[0], tt[1], tt[2]))
+                // Pass a reads parent axis permAxis(pc, a) and child axis sigma_a; child axis b
+                // is read by pass a = permAxis(q, b). Output axis b of the transform thus takes
+                // parent axis permAxis(pc, a), mirror bit sa_a and translation ta_a.
+                uint32_t pb[3], sym = 0;
+                int tt[3];
+#pragma unroll
+                for (int bo = 0; bo < 3; ++bo) {
+                    const uint32_t a = permAxis(q, bo);
+                    pb[bo] = permAxis(pc, a);
+                    sym |= ((sa >> a) & 1u) << bo;
+                    tt[bo] = a == 0 ? ta[0] : (a == 1 ? ta[1] : ta[2]);
+                }
+                const uint32_t tc = permCode(pb[0], pb[1], pb[2]);
+                if (!configAllows(cfgBits, tc, sym, tt[0], tt[1], tt[2]))
                     continue;
                 ++cnt.candidates;
-                const uint64_t k64 = encodeKey(r, pc, sym, tt[0], tt[1], tt[2], L);
+                const uint64_t k64 = encodeKey(r, tc, sym, tt[0], tt[1], tt[2], L);
                 if (k64 >= loadRelaxed(&best[cc]))
                     continue;
                 ++cnt.verifications;
@@ -614,7 +642,7 @@ __device__ __forceinline__ void probeQueue(const uint16_t* queue, uint32_t count
                 const int leader = __ffsll((unsigned long long)stopped) - 1;
                 const uint32_t bc = (uint32_t)__shfl((int)hitC, leader, 64);
                 const uint64_t bk = (uint64_t)__shfl((long long)hitKey, leader, 64);
-                const bool allOk = verifyWave<L>(s_rows, sortedMasks, bc, bk, lane);
+                const bool allOk = verifyWave<L>(layouts, n, r, bc, bk, lane);
                 if ((int)lane == leader && allOk) {
                     ++cnt.verified;
                     atomicMin((unsigned long long*)&best[bc], (unsigned long long)bk);
@@ -629,29 +657,29 @@ __device__ __forceinline__ void probeQueue(const uint16_t* queue, uint32_t count
 // queue, one voxel per lane and step, and hands the queue to probeQueue whenever the next step
 // might not fit and at the end.
 template <int L>
-__device__ __forceinline__ void probePending(uint32_t pending, uint16_t* queue, const uint32_t* fWave, int c0Base, const typename Geo<L>::Row* s_rows,
-    const uint64_t* __restrict__ sortedMasks, const uint64_t* __restrict__ table, uint32_t logCap, uint64_t* __restrict__ best, uint32_t cfgBits,
-    uint32_t r, uint32_t pc, uint32_t d, ProbeCounts& cnt)
+__device__ __forceinline__ void probePending(uint32_t pending, uint16_t* queue, const uint32_t* fWave, int c0Base, const uint64_t* __restrict__ layouts,
+    uint32_t n, const uint64_t* __restrict__ table, uint32_t logCap, uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t r, uint32_t pc, uint32_t d,
+    ProbeCounts& cnt)
 {
     const uint32_t lane = threadIdx.x & 63u;
     uint32_t count = 0;
     for (;;) {
         const uint64_t has = __ballot(pending != 0u);
-        const uint32_t n = (uint32_t)__popcll(has);
-        if (n != 0 && count + n <= kProbeQueue) {
+        const uint32_t pushed = (uint32_t)__popcll(has);
+        if (pushed != 0 && count + pushed <= kProbeQueue) {
             if (pending != 0u) {
                 const uint32_t pos = count + __builtin_amdgcn_mbcnt_hi((uint32_t)(has >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)has, 0u));
                 const uint32_t c2 = (uint32_t)__ffs(pending) - 1u;
                 pending &= pending - 1u;
                 queue[pos] = (uint16_t)((lane << 5) | c2);
             }
-            count += n;
+            count += pushed;
             continue;
         }
         if (count == 0)
             break;
         waveSync();
-        probeQueue<L>(queue, count, fWave, c0Base, s_rows, sortedMasks, table, logCap, best, cfgBits, r, pc, d, cnt);
+        probeQueue<L>(queue, count, fWave, c0Base, layouts, n, table, logCap, best, cfgBits, r, pc, d, cnt);
         waveSync();
         count = 0;
     }
@@ -662,20 +690,27 @@ __device__ __forceinline__ void probePending(uint32_t pending, uint16_t* queue, 
 // wave's filter passes.
 template <int L>
 __device__ __forceinline__ void lastPassProbe(uint32_t dz, int lead, const uint32_t* in, uint32_t* out, const uint32_t* outWave, uint16_t* queue, int c0Base,
-    const typename Geo<L>::Row* s_rows, const uint64_t* __restrict__ sortedMasks, const uint64_t* __restrict__ table, uint32_t logCap,
-    const uint32_t* __restrict__ bloom, uint32_t logBloom, uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t r, uint32_t pc, uint32_t d,
-    ProbeCounts& cnt)
+    const uint64_t* __restrict__ layouts, uint32_t n, const uint64_t* __restrict__ table, uint32_t logCap, const uint32_t* __restrict__ bloom,
+    uint32_t logBloom, uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t r, uint32_t pc, uint32_t d, ProbeCounts& cnt)
 {
     const uint32_t pending = dz ? pass3Filter<L, true>(in, out, bloom, logBloom, cfgBits, lead, cnt)
                                 : pass3Filter<L, false>(in, out, bloom, logBloom, cfgBits, lead, cnt);
     waveSync();
-    probePending<L>(pending, queue, outWave, c0Base, s_rows, sortedMasks, table, logCap, best, cfgBits, r, pc, d, cnt);
+    probePending<L>(pending, queue, outWave, c0Base, layouts, n, table, logCap, best, cfgBits, r, pc, d, cnt);
 }
 
 // Search-unit kinds of the level-3..5 kernel.
 constexpr int kUnitCombo = 0; // one (permutation, direction) combo per block
 constexpr int kUnitFamily = 1; // a family of 8 combos per block, each run from scratch
 constexpr int kUnitShared = 2; // a family of 8 combos; passes 1 and 2 shared by both pass-3 directions
+
+// Search units per parent: combos (pc, d), or families (p0, dx) of 8 combos; with
+// kCfgChildFrames the parent runs permutation code 0 only (combos 0..7, families 0 and 1).
+__host__ __device__ __forceinline__ uint32_t searchUnits(int kind, uint32_t cfgBits)
+{
+    const bool childFrames = (cfgBits & kCfgChildFrames) != 0;
+    return kind == kUnitCombo ? (childFrames ? 8u : 48u) : (childFrames ? 2u : 6u);
+}
 
 // LDS bytes of one kSearch block.
 template <int L, int Waves, int Kind>
@@ -692,12 +727,14 @@ constexpr size_t searchLds()
 // no voxel.
 template <int L, int Waves, int Kind>
 __device__ __forceinline__ void searchUnit(uint32_t r, uint32_t unit, uint32_t chunk, int wave, const typename Geo<L>::Row* s_rows, const Occupancy& occ,
-    uint32_t* planes0, uint32_t* planes1, uint16_t* queue, const uint64_t* __restrict__ layouts, const uint64_t* __restrict__ table, uint32_t logCap,
-    const uint32_t* __restrict__ bloom, uint32_t logBloom, uint64_t* __restrict__ best, uint32_t cfgBits, ProbeCounts& cnt)
+    uint32_t* planes0, uint32_t* planes1, uint16_t* queue, const uint64_t* __restrict__ layouts, uint32_t n, const uint64_t* __restrict__ table,
+    uint32_t logCap, const uint32_t* __restrict__ bloom, uint32_t logBloom, uint64_t* __restrict__ best, uint32_t cfgBits, ProbeCounts& cnt)
 {
     using G = Geo<L>;
     constexpr int R = G::R;
-    const bool axisOn = (cfgBits & kCfgAxis) != 0;
+    // Parent-side permutations: all six, unless the table holds the children under all six
+    // (kCfgChildFrames) or permutations are off.
+    const bool parentPerms = (cfgBits & kCfgAxis) != 0 && (cfgBits & kCfgChildFrames) == 0;
     const int lane = (int)(threadIdx.x & 63u);
     const int u = lane & (R - 1), w = lane >> L;
     const int c0Base = ((int)chunk * Waves + wave) * G::P;
@@ -719,12 +756,12 @@ __device__ __forceinline__ void searchUnit(uint32_t r, uint32_t unit, uint32_t c
         passes12<L>(pc, d, s_rows, c0, u, plane, emptyLead<L>(occ.axis(permAxis(pc, 1)), (d >> 1) & 1u));
         waveSync();
         lastPassProbe<L>((d >> 2) & 1u, emptyLead<L>(occ.axis(permAxis(pc, 2)), (d >> 2) & 1u), line, line, planes0 + wave * G::P * G::Slab, queue, c0Base,
-            s_rows, layouts, table, logCap, bloom, logBloom, best, cfgBits, r, pc, d, cnt);
+            layouts, n, table, logCap, bloom, logBloom, best, cfgBits, r, pc, d, cnt);
     } else {
 #pragma unroll 1
         for (uint32_t sel = 0; sel < 2; ++sel) {
             const uint32_t pc = 2u * p0 + sel; // the permutation codes with p0
-            if (!axisOn && pc != 0)
+            if (!parentPerms && pc != 0)
                 continue;
             const uint32_t occ1 = occ.axis(permAxis(pc, 1)), occ2 = occ.axis(permAxis(pc, 2));
 #pragma unroll 1
@@ -743,7 +780,7 @@ __device__ __forceinline__ void searchUnit(uint32_t r, uint32_t unit, uint32_t c
                         passes12<L>(pc, d, s_rows, c0, u, plane, lead2);
                         waveSync();
                     }
-                    lastPassProbe<L>(dz, emptyLead<L>(occ2, dz), line, outLine, outWave, queue, c0Base, s_rows, layouts, table, logCap, bloom, logBloom, best,
+                    lastPassProbe<L>(dz, emptyLead<L>(occ2, dz), line, outLine, outWave, queue, c0Base, layouts, n, table, logCap, bloom, logBloom, best,
                         cfgBits, r, pc, d, cnt);
                 }
             }
@@ -763,11 +800,11 @@ __device__ __forceinline__ void prunePacked(uint64_t* __restrict__ table, uint64
         const uint64_t e = table[s];
         if (e == 0)
             continue;
-        const uint32_t c = (uint32_t)e >> 3;
-        if (c == kDeadRank)
+        const uint32_t c = (uint32_t)e >> 6;
+        if (c == kPackedDead)
             continue;
         if (best[c] < minKey) {
-            table[s] = (e & 0xFFFFFFFF00000007ull) | ((uint64_t)kDeadRank << 3);
+            table[s] = (e & 0xFFFFFFFF0000003Full) | ((uint64_t)kPackedDead << 6);
             continue;
         }
         const uint32_t h = (uint32_t)(e >> 32);

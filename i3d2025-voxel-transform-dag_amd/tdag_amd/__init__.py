@@ -126,6 +126,8 @@ def lib() -> ctypes.CDLL:
     L.tdag_context_set_batching.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
     L.tdag_context_set_filter_bits.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]
     L.tdag_context_get_filter_bits.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
+    L.tdag_context_set_child_frames.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]
+    L.tdag_context_get_child_frames.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
     L.tdag_context_get_batching.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]
     L.tdag_batch_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(TdagLevelJob), ctypes.c_uint32, ctypes.POINTER(TdagConfig), ctypes.c_int,
                                     ctypes.POINTER(ctypes.c_void_p)]
@@ -138,7 +140,8 @@ def lib() -> ctypes.CDLL:
         getattr(L, name).restype = ctypes.c_int
     for name in ("tdag_context_create", "tdag_context_destroy", "tdag_match_level_device", "tdag_match_level", "tdag_last_stats",
                  "tdag_context_set_variant", "tdag_context_get_variant", "tdag_autotune_level", "tdag_context_set_counters",
-                 "tdag_context_set_batching", "tdag_context_get_batching", "tdag_context_set_filter_bits", "tdag_context_get_filter_bits"):
+                 "tdag_context_set_batching", "tdag_context_get_batching", "tdag_context_set_filter_bits", "tdag_context_get_filter_bits",
+                 "tdag_context_set_child_frames", "tdag_context_get_child_frames"):
         getattr(L, name).restype = ctypes.c_int
     _extra_bindings(L)
     _lib = L
@@ -326,12 +329,23 @@ class Context:
         return int(best.value), int(first.value), {v: round(float(ms[v]), 4) for v in range(VARIANT_SLOTS) if ms[v] >= 0}, round(float(ms[VARIANT_SLOTS]), 4)
 
     def set_filter_bits(self, level: int, bits_per_entry: int) -> None:
-        """Bloom filter bits per table entry (0 = default 16)."""
+        """Bloom filter bits per table entry (0 = library default: 16 at level 3, 4 elsewhere)."""
         check(lib().tdag_context_set_filter_bits(self._ptr, int(level), int(bits_per_entry)), "tdag_context_set_filter_bits")
 
     def get_filter_bits(self, level: int) -> int:
         v = ctypes.c_uint32()
         check(lib().tdag_context_get_filter_bits(self._ptr, int(level), ctypes.byref(v)), "tdag_context_get_filter_bits")
+        return int(v.value)
+
+    def set_child_frames(self, level: int, frames: int) -> None:
+        """Levels 3..5: 1 = parents run all 48 (permutation, direction) combos against 8 table
+        entries per child; 6 = parents run 8 directions against 48 entries per child (the
+        children in all six permutation frames); 0 = library default."""
+        check(lib().tdag_context_set_child_frames(self._ptr, int(level), int(frames)), "tdag_context_set_child_frames")
+
+    def get_child_frames(self, level: int) -> int:
+        v = ctypes.c_uint32()
+        check(lib().tdag_context_get_child_frames(self._ptr, int(level), ctypes.byref(v)), "tdag_context_get_child_frames")
         return int(v.value)
 
     def get_batching(self, level: int):

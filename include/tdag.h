@@ -135,11 +135,21 @@ tdag_status tdag_context_get_batching(const tdag_context* ctx, uint32_t level, u
 tdag_status tdag_context_set_filter_bits(tdag_context* ctx, uint32_t level, uint32_t bits_per_entry);
 tdag_status tdag_context_get_filter_bits(const tdag_context* ctx, uint32_t level, uint32_t* bits_per_entry);
 
+/* Where the 6 axis permutations are enumerated at `level` (3..5): frames = 1 hashes every
+ * parent sub-box under all 48 (permutation, direction) combos against a table holding each
+ * child from its 8 corners; frames = 6 hashes the parent under its 8 directions only against a
+ * table holding each child from its 8 corners in all 6 permutation frames (48 entries per
+ * child). Both find the same exhaustive minimum. 0 selects the default (6 at levels 4 and 5,
+ * 1 at level 3). A tuning knob with no reference counterpart. */
+tdag_status tdag_context_set_child_frames(tdag_context* ctx, uint32_t level, uint32_t frames);
+tdag_status tdag_context_get_child_frames(const tdag_context* ctx, uint32_t level, uint32_t* frames);
+
 /* Number of per-variant slots in the autotuner's timing output. */
 #define TDAG_VARIANT_SLOTS 32
 
 /* Autotuner: runs the level-`level` matcher on the device masks once per valid variant at the
- * default schedule (after one untimed warm-up run each), then tries four first-batch sizes
+ * default schedule (after one untimed warm-up run each; at levels 3..5 for both child-frame
+ * counts, keeping the one with the fastest variant), then tries four first-batch sizes
  * (n/256, n/64, n/16, n) with each of the three fastest variants and keeps the fastest pair,
  * then filter sizes of 4, 8, 16 and 32 bits per table entry; the winners become this context's
  * settings for `level`. out_ms (TDAG_VARIANT_SLOTS + 1 floats, may be NULL) receives the average

@@ -87,6 +87,41 @@ def test_search_variants(oracle, gpu_ctx, level, n_base, variant):
         gpu_ctx.set_variant(level, -1)
 
 
+@pytest.mark.parametrize("level,n_base,frames,variant", [(lv, nb, f, v) for lv, nb in ((3, 60), (4, 20), (5, 6)) for f in (1, 6)
+                                                         for v in (0, 1, 2, 16) if v in VALID_VARIANTS[lv]])
+def test_child_frames(oracle, gpu_ctx, level, n_base, frames, variant):
+    """Both places of the axis permutations (48 parent combos against 8 entries per child, or 8
+    parent directions against 48 entries per child) give the oracle's result bit for bit."""
+    rng = np.random.default_rng(700 + 10 * level + variant + frames)
+    masks = sets.planted_set(rng, level, n_base, n_base, oracle.transform)
+    print(f"level={level} frames={frames} variant={variant} n={len(masks)}", flush=True)
+    gpu_ctx.set_variant(level, variant)
+    gpu_ctx.set_child_frames(level, frames)
+    try:
+        assert gpu_ctx.get_child_frames(level) == frames
+        _check(oracle, gpu_ctx, level, masks)
+    finally:
+        gpu_ctx.set_variant(level, -1)
+        gpu_ctx.set_child_frames(level, 0)
+
+
+@pytest.mark.parametrize("level", [3, 4, 5])
+@pytest.mark.parametrize("frames", [1, 6])
+def test_child_frames_config_switches(oracle, gpu_ctx, level, frames):
+    """With axis permutation or symmetry switched off, the permuted or mirrored table entries
+    must not produce matches the configuration forbids."""
+    rng = np.random.default_rng(800 + level + frames)
+    n = 30 if level < 5 else 5
+    masks = sets.planted_set(rng, level, n, n, oracle.transform)
+    gpu_ctx.set_child_frames(level, frames)
+    try:
+        for cfg in ((False, True, True), (True, False, True), (False, False, True), (True, True, False)):
+            print(f"level={level} frames={frames} cfg={cfg} n={len(masks)}", flush=True)
+            _check(oracle, gpu_ctx, level, masks, cfg)
+    finally:
+        gpu_ctx.set_child_frames(level, 0)
+
+
 @pytest.mark.parametrize("level", [2, 3, 5])
 def test_probe_counters_consistent(oracle, gpu_ctx, level):
     """Per-stage counters are ordered (probes >= filter passes, candidates >= verifications >=
