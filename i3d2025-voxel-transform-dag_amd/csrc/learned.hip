@@ -153,9 +153,30 @@ constexpr int kTile = 64;
 constexpr int kKs = 16;
 constexpr int kLdsPitch = kKs + 4; // bf16 elements per staged row (padding against bank conflicts)
 
+// Stages one operand's 64 x 16 slice (rows r0.., k0..) into s[row][k]. RowFast: the row index
+// is contiguous in memory (element (row, k) at p[k * ld + row]); thread t takes row t % 64 and
+// k = 4 (t / 64) + i, so a wave reads 64 consecutive elements. Otherwise k is contiguous
+// (p[row * ld + k]) and thread t takes row t / 4 and k = 4 (t % 4) + i.
+template <bool RowFast>
+__device__ __forceinline__ void stageSlice(const uint16_t* __restrict__ p, int ld, int r0, int rows, int k0, int kEnd, uint16_t (*sm)[kLdsPitch], int t)
+{
+    const int row = RowFast ? (t & 63) : (t >> 2);
+    const int kb = RowFast ? 4 * (t >> 6) : 4 * (t & 3);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int r = r0 + row, k = k0 + kb + i;
+        uint16_t v = 0;
+        if (r < rows && k < kEnd)
+            v = RowFast ? p[(size_t)k * ld + r] : p[(size_t)r * ld + k];
+        sm[row][kb + i] = v;
+    }
+}
+
+// C tile (or, with split K, the partial product of K range [kBegin, kEnd) of slice blockIdx.z,
+// written to C + blockIdx.z * sliceStride) of the 64 x 64 output tile (blockIdx.y, blockIdx.x).
 template <bool TA, bool TB>
 __global__ __launch_bounds__(256) void kGemm(const uint16_t* __restrict__ A, int lda, const uint16_t* __restrict__ B, int ldb, float* __restrict__ C, int ldc, int M, int N,
-    int K)
+    int K, int kChunk, size_t sliceStride)
 {
     __shared__ __align__(16) uint16_t sA[kTile][kLdsPitch];
     __shared__ __align__(16) uint16_t sB[kTile][kLdsPitch]; // sB[n][k]
@@ -163,24 +184,15 @@ __global__ __launch_bounds__(256) void kGemm(const uint16_t* __restrict__ A, int
     const int lane = t & 63, wave = t >> 6;
     const int m0 = (int)blockIdx.y * kTile, n0 = (int)blockIdx.x * kTile;
     const int wr = wave >> 1, wc = wave & 1;
+    const int kBegin = (int)blockIdx.z * kChunk, kEnd = min(K, kBegin + kChunk);
+    C += (size_t)blockIdx.z * sliceStride;
     f32x4 acc[2][2];
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j)
             acc[i][j] = f32x4 { 0.f, 0.f, 0.f, 0.f };
-    const int lr = t >> 2, lk = (t & 3) * 4; // staging: row lr, k lk..lk+3
-    for (int k0 = 0; k0 < K; k0 += kKs) {
-        for (int i = 0; i < 4; ++i) {
-            const int m = m0 + lr, k = k0 + lk + i;
-            uint16_t va = 0;
-            if (m < M && k < K)
-                va = TA ? A[(size_t)k * lda + m] : A[(size_t)m * lda + k];
-            sA[lr][lk + i] = va;
-            const int nn = n0 + lr;
-            uint16_t vb = 0;
-            if (nn < N && k < K)
-                vb = TB ? B[(size_t)nn * ldb + k] : B[(size_t)k * ldb + nn];
-            sB[lr][lk + i] = vb;
-        }
+    for (int k0 = kBegin; k0 < kEnd; k0 += kKs) {
+        stageSlice<TA>(A, lda, m0, M, k0, kEnd, sA, t);
+        stageSlice<!TB>(B, ldb, n0, N, k0, kEnd, sB, t);
         __syncthreads();
         const int kq = 4 * (lane >> 4);
         bf16x4 a[2], b[2];
@@ -205,20 +217,57 @@ __global__ __launch_bounds__(256) void kGemm(const uint16_t* __restrict__ A, int
             }
 }
 
-hipError_t gemm(bool ta, bool tb, const uint16_t* A, int lda, const uint16_t* B, int ldb, float* C, int ldc, int M, int N, int K, hipStream_t s)
+// C[m][n] = sum of the `slices` partial products in slice order (deterministic split K).
+__global__ __launch_bounds__(256) void kSumSlices(const float* __restrict__ P, int slices, int M, int N, float* __restrict__ C, int ldc)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t mn = (size_t)M * N;
+    if (i >= mn)
+        return;
+    float acc = 0.f;
+    for (int z = 0; z < slices; ++z)
+        acc += P[(size_t)z * mn + i];
+    C[(i / N) * (size_t)ldc + i % N] = acc;
+}
+
+template <bool TA, bool TB>
+void launchGemm(dim3 grid, const uint16_t* A, int lda, const uint16_t* B, int ldb, float* C, int ldc, int M, int N, int K, int kChunk, size_t sliceStride, hipStream_t s)
+{
+    hipLaunchKernelGGL((kGemm<TA, TB>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, M, N, K, kChunk, sliceStride);
+}
+
+// Split K applies when the output has few tiles and K is long (the weight gradients, K = batch
+// rows): kSplitSlices slices of K, partial products in `splitWork` (kSplitSlices x M x N floats).
+constexpr int kSplitSlices = 8;
+constexpr size_t kSplitWorkFloats = (size_t)kSplitSlices * 256 * 256;
+
+hipError_t gemm(bool ta, bool tb, const uint16_t* A, int lda, const uint16_t* B, int ldb, float* C, int ldc, int M, int N, int K, hipStream_t s,
+    float* splitWork = nullptr)
 {
     if (M <= 0 || N <= 0)
         return hipSuccess;
-    const dim3 grid((unsigned)cdiv(N, kTile), (unsigned)cdiv(M, kTile));
+    const int tiles = (int)(cdiv(N, kTile) * cdiv(M, kTile));
+    const bool split = splitWork && tiles < 64 && K >= 32 * kKs * kSplitSlices / 4 && (size_t)M * N * kSplitSlices <= kSplitWorkFloats;
+    const int slices = split ? kSplitSlices : 1;
+    const int kChunk = split ? (int)(cdiv(cdiv(K, kSplitSlices), kKs) * kKs) : std::max(K, 1);
+    float* out = split ? splitWork : C;
+    const int ldo = split ? N : ldc;
+    const size_t stride = split ? (size_t)M * N : 0;
+    const dim3 grid((unsigned)cdiv(N, kTile), (unsigned)cdiv(M, kTile), (unsigned)slices);
     if (!ta && !tb)
-        hipLaunchKernelGGL((kGemm<false, false>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, M, N, K);
+        launchGemm<false, false>(grid, A, lda, B, ldb, out, ldo, M, N, K, kChunk, stride, s);
     else if (ta && !tb)
-        hipLaunchKernelGGL((kGemm<true, false>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, M, N, K);
+        launchGemm<true, false>(grid, A, lda, B, ldb, out, ldo, M, N, K, kChunk, stride, s);
     else if (!ta && tb)
-        hipLaunchKernelGGL((kGemm<false, true>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, M, N, K);
+        launchGemm<false, true>(grid, A, lda, B, ldb, out, ldo, M, N, K, kChunk, stride, s);
     else
-        hipLaunchKernelGGL((kGemm<true, true>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, M, N, K);
-    return hipGetLastError();
+        launchGemm<true, true>(grid, A, lda, B, ldb, out, ldo, M, N, K, kChunk, stride, s);
+    LEARN_TRY(hipGetLastError());
+    if (split) {
+        hipLaunchKernelGGL(kSumSlices, dim3((unsigned)cdiv((size_t)M * N, 256)), dim3(256), 0, s, splitWork, slices, M, N, C, ldc);
+        LEARN_TRY(hipGetLastError());
+    }
+    return hipSuccess;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -358,16 +407,27 @@ __global__ __launch_bounds__(256) void kReluBack(float* __restrict__ dH, const u
     outBf[i] = f2bf(v);
 }
 
-// out[c] = sum over rows of X[r][c] (bias gradient). One thread per column, 256 per block.
-__global__ __launch_bounds__(256) void kColSum(const float* __restrict__ X, uint32_t rows, uint32_t cols, float* __restrict__ out)
+// out[c] = sum over rows of X[r][c] (bias gradient). One block of 16 x 64 threads per 64 columns:
+// thread (g, c) sums rows g, g + 16, ... (coalesced along c), then the 16 partial sums are added
+// in a fixed order, so the result does not depend on scheduling.
+__global__ __launch_bounds__(1024) void kColSum(const float* __restrict__ X, uint32_t rows, uint32_t cols, float* __restrict__ out)
 {
-    const uint32_t c = blockIdx.x * 256u + threadIdx.x;
-    if (c >= cols)
-        return;
-    float s = 0.f;
-    for (uint32_t r = 0; r < rows; ++r)
-        s += X[(size_t)r * cols + c];
-    out[c] = s;
+    __shared__ float s_part[16][64];
+    const uint32_t cl = threadIdx.x & 63u, g = threadIdx.x >> 6;
+    const uint32_t c = blockIdx.x * 64u + cl;
+    float acc = 0.f;
+    if (c < cols)
+        for (uint32_t r = g; r < rows; r += 16)
+            acc += X[(size_t)r * cols + c];
+    s_part[g][cl] = acc;
+    __syncthreads();
+    if (g == 0 && c < cols) {
+        float sum = 0.f;
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            sum += s_part[i][cl];
+        out[c] = sum;
+    }
 }
 
 __global__ __launch_bounds__(256) void kCastBf(const float* __restrict__ x, size_t n, uint16_t* __restrict__ out)
@@ -408,6 +468,7 @@ struct Work {
     float* loss;
     uint16_t* gbf; // back-propagated signal (bf16)
     uint32_t* idx;
+    float* split; // split-K partial products (kSplitWorkFloats)
 };
 
 size_t workBytes(size_t rows, size_t B)
@@ -420,6 +481,7 @@ size_t workBytes(size_t rows, size_t B)
     s += align256(B * B * 4) + align256(B * 4);
     s += align256(rows * 256 * 2);
     s += align256(B * 4);
+    s += align256(kSplitWorkFloats * 4);
     return s;
 }
 
@@ -445,6 +507,7 @@ Work carve(void* base, size_t rows, size_t B)
     w.loss = (float*)take(B * 4);
     w.gbf = (uint16_t*)take(rows * 256 * 2);
     w.idx = (uint32_t*)take(B * 4);
+    w.split = (float*)take(kSplitWorkFloats * 4);
     return w;
 }
 
@@ -495,8 +558,8 @@ hipError_t lossGrad(Mlp& mlp, Work& w, uint32_t B, float tau, hipStream_t s)
     for (int l = kLayers - 1; l >= 0; --l) {
         const int in = kLayerIn[l], out = kLayerOut[l];
         // dW_l = act_l^T g ; db_l = column sums of g
-        LEARN_TRY(gemm(true, false, w.act[l], in, w.gbf, out, mlp.g + paramOffsetW(l), out, in, out, (int)rows, s));
-        hipLaunchKernelGGL(kColSum, dim3((unsigned)cdiv(out, 256)), dim3(256), 0, s, w.C, rows, (uint32_t)out, mlp.g + paramOffsetB(l));
+        LEARN_TRY(gemm(true, false, w.act[l], in, w.gbf, out, mlp.g + paramOffsetW(l), out, in, out, (int)rows, s, w.split));
+        hipLaunchKernelGGL(kColSum, dim3((unsigned)cdiv(out, 64)), dim3(1024), 0, s, w.C, rows, (uint32_t)out, mlp.g + paramOffsetB(l));
         LEARN_TRY(hipGetLastError());
         if (l > 0) {
             // dH = g W_l^T, masked by the ReLU of act_l; becomes the next g
@@ -513,119 +576,301 @@ hipError_t lossGrad(Mlp& mlp, Work& w, uint32_t B, float tau, hipStream_t s)
 // Proposals and verification (level 2, ranks in popcount-descending order)
 // ---------------------------------------------------------------------------------------
 
-// Per block: 64 queries [q0, q0 + 64). For every 64-candidate tile of earlier ranks, the 64 x 64
-// scores come from MFMA (bf16 embeddings, K = 128) into LDS. All 256 threads then filter the tile
-// against each query's current k-th best score (4 threads per query, 16 candidates each, kept in
-// candidate order), and thread q merges its query's survivors into a min-heap of the k best
-// (LDS, candidate-major so the 64 threads hit 64 banks). The survivors are merged in a fixed order,
-// so the result does not depend on thread timing.
+// Retrieval: for query q (rank order) the k best earlier leaves c < q under the strict order
+// "higher score first, then lower rank", where score = <E_q, E_c> of the bf16 embeddings
+// (MFMA, fp32 accumulate). Query block qb holds queries [64 qb, 64 qb + 64) and scores candidate
+// tiles 0..qb (64 leaves each); those tiles are cut into chunks of `ct` tiles and every
+// (query block, chunk) pair is one block of kTopKPartial, so the triangular total is spread
+// evenly over the GPU. Each query's 64 best so far are a wave queue: one 64-bit key per lane,
+// sorted descending across the lanes; a tile's candidates that beat the queue's last key are
+// merged in with a bitonic sort and merge across the lanes. kTopKMerge merges a query's chunk
+// queues the same way. Keys order (score, rank) exactly, so the proposal set is the top k under
+// that order, independent of the chunking and of thread timing.
 constexpr int kQ = 64;
 
-__device__ __forceinline__ void heapInsert(float (*hs)[kQ], uint32_t (*hi)[kQ], uint32_t& size, uint32_t k, int t, float sc, uint32_t c)
+// Blocks of query blocks 0..qb-1: sum over m = 1..qb of ceil(m / ct).
+__host__ __device__ __forceinline__ uint64_t chunksBefore(uint32_t qb, uint32_t ct)
 {
-    if (size < k) { // not full: sift up
-        uint32_t pos = size++;
-        while (pos > 0) {
-            const uint32_t par = (pos - 1) >> 1;
-            if (hs[par][t] <= sc)
-                break;
-            hs[pos][t] = hs[par][t];
-            hi[pos][t] = hi[par][t];
-            pos = par;
+    const uint64_t g = qb / ct, rem = qb % ct;
+    return (uint64_t)ct * g * (g + 1) / 2 + rem * (g + 1);
+}
+
+// Chunk length: at least 32 tiles, and at most 16 chunks per query block (bounds the merge).
+constexpr uint32_t kMaxChunks = 16;
+__host__ __device__ __forceinline__ uint32_t chunkTiles(uint32_t qblocks) { return qblocks / kMaxChunks + 32u; }
+
+// Key of candidate c with score s: the score's bits mapped to an unsigned integer of the same
+// order, then the complemented rank, so a larger key is a better candidate. 0 (below every real
+// key) marks an empty entry and rank 0xFFFFFFFF; NaN scores get 0 and never enter a queue.
+__device__ __forceinline__ uint64_t scoreKey(float s, uint32_t c)
+{
+    if (s != s)
+        return 0ull;
+    const uint32_t b = __float_as_uint(s);
+    const uint32_t o = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+    return ((uint64_t)o << 32) | (uint64_t)(0xFFFFFFFFu - c);
+}
+
+// Score of a key (inverse of scoreKey's order map); -inf for the empty key 0.
+__device__ __forceinline__ float keyScore(uint64_t key)
+{
+    if (key == 0ull)
+        return -INFINITY;
+    const uint32_t o = (uint32_t)(key >> 32);
+    return __uint_as_float((o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o);
+}
+
+__device__ __forceinline__ uint64_t shflXor64(uint64_t v, int m)
+{
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, m, 64);
+    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), m, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Lane keys forming a bitonic sequence -> sorted descending across the 64 lanes.
+__device__ __forceinline__ uint64_t bitonicMergeDesc(uint64_t v, uint32_t lane)
+{
+#pragma unroll
+    for (int stride = 32; stride > 0; stride >>= 1) {
+        const uint64_t o = shflXor64(v, stride);
+        v = (lane & (uint32_t)stride) == 0u ? (v > o ? v : o) : (v < o ? v : o);
+    }
+    return v;
+}
+
+// Arbitrary lane keys -> sorted descending across the 64 lanes.
+__device__ __forceinline__ uint64_t bitonicSortDesc(uint64_t v, uint32_t lane)
+{
+#pragma unroll
+    for (int size = 2; size <= 64; size <<= 1)
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            const uint64_t o = shflXor64(v, stride);
+            const bool keepMax = ((lane & (uint32_t)stride) == 0u) == ((lane & (uint32_t)size) == 0u);
+            v = keepMax ? (v > o ? v : o) : (v < o ? v : o);
         }
-        hs[pos][t] = sc;
-        hi[pos][t] = c;
-    } else if (sc > hs[0][t]) { // replace the smallest: sift down
-        uint32_t pos = 0;
-        for (;;) {
-            const uint32_t l = 2 * pos + 1;
-            if (l >= k)
-                break;
-            uint32_t ch = l;
-            if (l + 1 < k && hs[l + 1][t] < hs[l][t])
-                ch = l + 1;
-            if (hs[ch][t] >= sc)
-                break;
-            hs[pos][t] = hs[ch][t];
-            hi[pos][t] = hi[ch][t];
-            pos = ch;
-        }
-        hs[pos][t] = sc;
-        hi[pos][t] = c;
+    return v;
+}
+
+// Queue q (sorted descending) and a second sorted-descending list l given reversed (lane i holds
+// l[63 - i]): the 64 best of both, sorted descending. The lane-wise maximum of a descending and
+// an ascending sequence is bitonic and holds the 64 largest of the union.
+__device__ __forceinline__ uint64_t mergeReversed(uint64_t q, uint64_t lRev, uint32_t lane)
+{
+    return bitonicMergeDesc(q > lRev ? q : lRev, lane);
+}
+
+// Loads a lane's 32 embedding values of one row. The 128 dimensions are split as
+// k = 32 g + 4 s + i for lane group g = lane / 16, MFMA step s < 8 and element i < 4: both
+// operands use the same split, so the 8 steps sum the full dot product, and each lane reads 64
+// contiguous bytes per row.
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ void loadFrag(const uint16_t* __restrict__ Ebf, uint32_t row, uint32_t g, bf16x4 (&f)[8])
+{
+    const bf16x8* p = reinterpret_cast<const bf16x8*>(Ebf + (size_t)row * kEmb + 32u * g);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const bf16x8 x = p[v];
+        f[2 * v] = x.lo;
+        f[2 * v + 1] = x.hi;
     }
 }
 
-__global__ __launch_bounds__(256) void kTopK(const uint16_t* __restrict__ Ebf, uint32_t n, uint32_t k, uint32_t* __restrict__ cand)
+// Buffered survivors (lane i < count holds one, others 0) merged into a queue; the rare path of
+// kTopKPartial, kept out of line so the unrolled per-query filter stays small.
+__device__ __noinline__ uint64_t mergeBuffer(uint64_t queue, uint64_t buffered, uint32_t lane)
 {
+    const uint64_t sorted = bitonicSortDesc(buffered, lane);
+    return mergeReversed(queue, shflXor64(sorted, 63), lane);
+}
+
+// Block barrier ordering LDS only: unlike __syncthreads it does not wait for outstanding global
+// loads, so a prefetch issued before it stays in flight across it.
+__device__ __forceinline__ void ldsBarrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Orders the calling wave's LDS accesses (as waveSync, without waiting for global loads).
+__device__ __forceinline__ void ldsWaveSync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
+__device__ __forceinline__ uint64_t readLane64(uint64_t v, int l)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// One block per (query block, chunk): 256 threads, waves in 2 x 2 over the 64 x 64 score tile,
+// each wave 32 x 32 = 2 x 2 MFMA 16x16x16 tiles. The wave's query fragments stay in registers;
+// the next candidate tile's fragments are loaded while the current one is filtered. Scores go to
+// LDS; wave w then owns queries 16 w + i, i < 16: their queues live in registers (lane = entry),
+// their thresholds (the queue's last key) and buffer fill counts are wave-uniform. Lane j takes
+// candidate j of the tile; keys above the threshold are appended to the query's 64-entry buffer
+// in LDS, and a full buffer is sorted and merged into the queue (then the threshold rises).
+// Writes each query's queue (64 keys, sorted descending) to part[(block * 64 + query) * 64 + entry].
+// The chunks of one query block share a threshold per query, shared[q]: after each merge a block
+// raises it to its queue's last key with a 64-bit atomicMax and filters against the larger of its
+// own and the shared one, refreshed every tile. Any chunk's 64th best key is a lower bound of the
+// query's final 64th best, so a key at or below it can never be among the 64 best: the result is
+// exact whatever the timing, and the shared bound keeps the later chunks' survivors few.
+__global__ __launch_bounds__(256) void kTopKPartial(const uint16_t* __restrict__ Ebf, uint32_t n, uint32_t qblocks, uint32_t ct, uint64_t* __restrict__ part,
+    uint64_t* __restrict__ shared)
+{
+    constexpr int QW = kQ / 4; // queries per wave
     __shared__ float s_score[kQ][kQ + 1]; // [candidate][query]
-    __shared__ float s_heapS[kMaxProposals][kQ];
-    __shared__ uint32_t s_heapI[kMaxProposals][kQ];
-    __shared__ uint8_t s_list[4][16][kQ]; // survivors per (filter thread, query), candidate-major
-    __shared__ uint8_t s_cnt[4][kQ];
-    __shared__ float s_thr[kQ];
+    __shared__ uint64_t s_buf[kQ][64]; // [query][entry]
     const int t = (int)threadIdx.x;
-    const int lane = t & 63, wave = t >> 6;
-    const uint32_t q0 = blockIdx.x * kQ;
+    const uint32_t lane = (uint32_t)t & 63u;
+    const int wave = t >> 6;
+    const uint32_t g = lane >> 4;
     const int wr = wave >> 1, wc = wave & 1;
-    uint32_t size = 0; // heap size of query t (t < 64)
-    if (t < kQ)
-        s_thr[t] = -INFINITY;
-    for (uint32_t c0 = 0; c0 < q0 + kQ && c0 < n; c0 += kQ) {
+    // (query block, chunk) of this block: the largest qb with chunksBefore(qb) <= blockIdx.x.
+    uint32_t lo = 0, hi = qblocks - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (chunksBefore(mid, ct) <= blockIdx.x)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const uint32_t qb = lo;
+    // Chunks and the tiles inside a chunk are visited from the query block's own tile backwards:
+    // matches have popcounts at or just above the query's, i.e. nearby ranks, so the best
+    // candidates come early and the thresholds rise fast. The result does not depend on order.
+    const uint32_t chunks = qb / ct + 1;
+    const uint32_t chunk = chunks - 1u - (blockIdx.x - (uint32_t)chunksBefore(qb, ct));
+    const uint32_t tile0 = chunk * ct, tile1 = min(tile0 + ct, qb + 1);
+    const uint32_t q0 = qb * kQ;
+    uint64_t queue[QW], thr[QW];
+    float thrScore[QW]; // keyScore(thr[i]): score >= thrScore is a superset test of key > thr
+    uint32_t cnt[QW];
+#pragma unroll
+    for (int i = 0; i < QW; ++i) {
+        queue[i] = 0ull;
+        thr[i] = 0ull;
+        thrScore[i] = -INFINITY;
+        cnt[i] = 0u;
+    }
+    bf16x4 a[2][8], b[2][8];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        loadFrag(Ebf, min(q0 + wr * 32 + i * 16 + (lane & 15), n - 1), g, a[i]);
+        loadFrag(Ebf, min((tile1 - 1u) * kQ + wc * 32 + i * 16 + (lane & 15), n - 1), g, b[i]);
+    }
+    for (uint32_t step = 0; step < tile1 - tile0; ++step) {
+        const uint32_t tile = tile1 - 1u - step;
+        const uint32_t c0 = tile * kQ;
         f32x4 acc[2][2];
+#pragma unroll
         for (int i = 0; i < 2; ++i)
+#pragma unroll
             for (int j = 0; j < 2; ++j)
                 acc[i][j] = f32x4 { 0.f, 0.f, 0.f, 0.f };
-        for (int kk = 0; kk < kEmb; kk += 16) {
-            const int kq = kk + 4 * (lane >> 4);
-            bf16x4 a[2], b[2];
-            for (int i = 0; i < 2; ++i) {
-                const uint32_t qr = q0 + wr * 32 + i * 16 + (lane & 15);
-                const uint32_t cr = c0 + wc * 32 + i * 16 + (lane & 15);
-                const uint16_t* pa = Ebf + (size_t)min(qr, n - 1) * kEmb + kq;
-                const uint16_t* pb = Ebf + (size_t)min(cr, n - 1) * kEmb + kq;
-                a[i] = bf16x4 { (short)pa[0], (short)pa[1], (short)pa[2], (short)pa[3] };
-                b[i] = bf16x4 { (short)pb[0], (short)pb[1], (short)pb[2], (short)pb[3] };
-            }
+#pragma unroll
+        for (int st = 0; st < 8; ++st)
+#pragma unroll
             for (int i = 0; i < 2; ++i)
+#pragma unroll
                 for (int j = 0; j < 2; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
-        }
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i][st], b[j][st], acc[i][j], 0, 0, 0);
+        // The shared thresholds (lane i < 16: query 16 w + i; the address is clamped so the load
+        // is unconditional) are loaded before the next tile's prefetch: loads complete in order,
+        // so waiting for them does not wait for the prefetch.
+        const uint64_t sharedThr = __hip_atomic_load(&shared[min(q0 + wave * QW + (lane & (QW - 1)), n - 1)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // Prefetch of the next tile's candidate fragments (unconditional, rows clamped: after the
+        // last tile it reads rows that are never used; a branch here would make the compiler wait
+        // for every outstanding load at the join).
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            loadFrag(Ebf, min((tile > 0u ? c0 - kQ : 0u) + wc * 32 + j * 16 + (lane & 15), n - 1), g, b[j]);
+        // D[row][col]: lane holds rows 4 (lane / 16) + r, column lane % 16 (row = query, col = candidate).
+#pragma unroll
         for (int i = 0; i < 2; ++i)
+#pragma unroll
             for (int j = 0; j < 2; ++j)
+#pragma unroll
                 for (int r = 0; r < 4; ++r)
                     s_score[wc * 32 + j * 16 + (lane & 15)][wr * 32 + i * 16 + 4 * (lane >> 4) + r] = acc[i][j][r];
-        __syncthreads();
-        { // filter: thread (sub, qi) scans candidates sub * 16 .. sub * 16 + 15 of query qi
-            const int qi = t & (kQ - 1), sub = t >> 6;
+        ldsBarrier();
+        const uint32_t c = c0 + lane;
+        const bool refresh = (step & 3u) == 0u; // shared bounds every 4 tiles
+#pragma unroll
+        for (int i = 0; i < QW; ++i) {
+            const int qi = wave * QW + i;
             const uint32_t q = q0 + (uint32_t)qi;
-            const float thr = s_thr[qi];
-            uint32_t cnt = 0;
-            for (int jj = 0; jj < 16; ++jj) {
-                const int j = sub * 16 + jj;
-                const uint32_t c = c0 + (uint32_t)j;
-                if (c < q && q < n && s_score[j][qi] > thr)
-                    s_list[sub][cnt++][qi] = (uint8_t)j;
-            }
-            s_cnt[sub][qi] = (uint8_t)cnt;
-        }
-        __syncthreads();
-        if (t < kQ) {
-            for (int sub = 0; sub < 4; ++sub)
-                for (uint32_t e = 0; e < s_cnt[sub][t]; ++e) {
-                    const int j = s_list[sub][e][t];
-                    heapInsert(s_heapS, s_heapI, size, k, t, s_score[j][t], c0 + (uint32_t)j);
+            if (refresh) {
+                // For q >= n the clamped load read query n - 1's bound: harmless, as no key passes.
+                const uint64_t sh = readLane64(sharedThr, i);
+                if (sh > thr[i]) {
+                    thr[i] = sh;
+                    thrScore[i] = keyScore(sh);
                 }
-            if (size == k)
-                s_thr[t] = s_heapS[0][t];
+            }
+            // One float compare per candidate; the exact key (score, then rank) only where a
+            // score reaches the threshold's. NaN fails both tests.
+            const float sc = s_score[lane][qi];
+            if (__ballot(sc >= thrScore[i]) == 0ull)
+                continue;
+            const uint64_t key = (c < q && q < n) ? scoreKey(sc, c) : 0ull;
+            uint64_t m = __ballot(key > thr[i]);
+            if (m == 0ull)
+                continue;
+            if (cnt[i] + (uint32_t)__popcll(m) > 64u) { // buffer full: merge it, raise the threshold
+                ldsWaveSync();
+                queue[i] = mergeBuffer(queue[i], lane < cnt[i] ? s_buf[qi][lane] : 0ull, lane);
+                const uint64_t last = readLane64(queue[i], 63);
+                if (last > thr[i]) {
+                    thr[i] = last;
+                    thrScore[i] = keyScore(last);
+                    if (lane == 0u && q < n)
+                        __hip_atomic_fetch_max(&shared[q], last, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                cnt[i] = 0u;
+                ldsWaveSync();
+                m = __ballot(key > thr[i]);
+            }
+            if (key > thr[i])
+                s_buf[qi][cnt[i] + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] = key;
+            cnt[i] += (uint32_t)__popcll(m);
         }
-        __syncthreads();
+        ldsBarrier();
     }
-    if (t < kQ) {
-        const uint32_t q = q0 + (uint32_t)t;
-        if (q < n)
-            for (uint32_t s = 0; s < k; ++s)
-                cand[(size_t)q * k + s] = s < size ? s_heapI[s][t] : 0xFFFFFFFFu;
+#pragma unroll
+    for (int i = 0; i < QW; ++i) {
+        const int qi = wave * QW + i;
+        if (cnt[i] > 0u) {
+            ldsWaveSync();
+            queue[i] = mergeBuffer(queue[i], lane < cnt[i] ? s_buf[qi][lane] : 0ull, lane);
+        }
+        part[((size_t)blockIdx.x * kQ + (uint32_t)qi) * 64 + lane] = queue[i];
     }
+}
+
+// One wave per query: the 64 best keys of its chunk queues (merged pairwise), then the first k
+// ranks, best first, into cand[q * k ..]; 0xFFFFFFFF where fewer than k earlier leaves exist.
+__global__ __launch_bounds__(256) void kTopKMerge(const uint64_t* __restrict__ part, uint32_t n, uint32_t k, uint32_t ct, uint32_t* __restrict__ cand)
+{
+    const uint32_t q = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (q >= n)
+        return;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t qb = q / kQ, t = q % kQ;
+    const uint32_t b0 = (uint32_t)chunksBefore(qb, ct);
+    const uint32_t chunks = qb / ct + 1;
+    const uint64_t* base = part + ((size_t)b0 * kQ + t) * 64;
+    uint64_t v = base[lane];
+    for (uint32_t ch = 1; ch < chunks; ++ch)
+        v = mergeReversed(v, base[(size_t)ch * kQ * 64 + (63u - lane)], lane);
+    if (lane < k)
+        cand[(size_t)q * k + lane] = 0xFFFFFFFFu - (uint32_t)v;
 }
 
 // Baseline proposals without a network: the k leaves ranked just before the query (the nearest
@@ -761,16 +1006,51 @@ __global__ __launch_bounds__(256) void kFallbackFlags(const uint64_t* __restrict
 
 } // namespace
 
+size_t topkWorkBytes(uint64_t n)
+{
+    if (n == 0)
+        return 256;
+    const uint32_t qblocks = (uint32_t)cdiv(n, kQ);
+    const uint64_t blocks = chunksBefore(qblocks, chunkTiles(qblocks));
+    return align256((size_t)blocks * kQ * 64 * 8) + align256((size_t)n * 8);
+}
+
+hipError_t topkPropose(const uint16_t* Ebf, uint32_t n, uint32_t k, uint32_t* cand, void* work, hipStream_t s)
+{
+    if (n == 0)
+        return hipSuccess;
+    if (k == 0 || k > kMaxProposals)
+        return hipErrorInvalidValue;
+    const uint32_t qblocks = (uint32_t)cdiv(n, kQ);
+    const uint32_t ct = chunkTiles(qblocks);
+    const uint64_t blocks = chunksBefore(qblocks, ct);
+    if (blocks > 0x7FFFFFFFull)
+        return hipErrorInvalidValue;
+    uint64_t* shared = (uint64_t*)((char*)work + align256((size_t)blocks * kQ * 64 * 8));
+    LEARN_TRY(fillU64(shared, n, 0ull, s));
+    hipLaunchKernelGGL(kTopKPartial, dim3((unsigned)blocks), dim3(256), 0, s, Ebf, n, qblocks, ct, (uint64_t*)work, shared);
+    LEARN_TRY(hipGetLastError());
+    hipLaunchKernelGGL(kTopKMerge, dim3((unsigned)cdiv(n, 4)), dim3(256), 0, s, (const uint64_t*)work, n, k, ct, cand);
+    return hipGetLastError();
+}
+
 size_t learnedWorkBytes(uint64_t n)
 {
-    // proposals, flags, bf16 embeddings, counters, 48 images and boxes per leaf (the embedding
-    // workspace itself lives in the Mlp)
-    return align256(n * kMaxProposals * 4) + align256(n * 4) + align256((n + kQ) * kEmb * 2) + 256 + align256(n * 48 * 8) + align256(n * 48 * 2);
+    // proposals, flags, bf16 embeddings, counters, 48 images and boxes per leaf, the retrieval's
+    // per-chunk lists (the embedding workspace itself lives in the Mlp)
+    return align256(n * kMaxProposals * 4) + align256(n * 4) + align256((n + kQ) * kEmb * 2) + 256 + align256(n * 48 * 8) + align256(n * 48 * 2) +
+        topkWorkBytes(n);
 }
 
 hipError_t gemmBf16(bool ta, bool tb, const uint16_t* A, int lda, const uint16_t* B, int ldb, float* C, int ldc, int M, int N, int K, hipStream_t stream)
 {
-    return gemm(ta, tb, A, lda, B, ldb, C, ldc, M, N, K, stream);
+    // With a split-K workspace, as the network's weight gradients use it (applies to few output
+    // tiles and long K).
+    float* split = nullptr;
+    LEARN_TRY(hipMallocAsync((void**)&split, kSplitWorkFloats * 4, stream));
+    const hipError_t e = gemm(ta, tb, A, lda, B, ldb, C, ldc, M, N, K, stream, split);
+    const hipError_t f = hipFreeAsync(split, stream);
+    return e != hipSuccess ? e : f;
 }
 
 hipError_t mlpAlloc(Mlp& mlp)
@@ -936,6 +1216,7 @@ hipError_t matchLearned2(const MatchPlan& p, Mlp* mlp, uint32_t k, const uint64_
     unsigned long long* cnt = (unsigned long long*)((char*)Ebf + align256(((size_t)n + kQ) * kEmb * 2));
     uint64_t* images = (uint64_t*)((char*)cnt + 256);
     uint16_t* boxes = (uint16_t*)((char*)images + align256((size_t)n * 48 * 8));
+    void* topkWork = (char*)boxes + align256((size_t)n * 48 * 2);
     hipEvent_t ev[5];
     for (auto& e : ev)
         LEARN_TRY(hipEventCreate(&e));
@@ -965,7 +1246,8 @@ hipError_t matchLearned2(const MatchPlan& p, Mlp* mlp, uint32_t k, const uint64_
             if (e != hipSuccess)
                 return cleanup(e);
             (void)hipEventRecord(ev[1], s);
-            hipLaunchKernelGGL(kTopK, dim3((unsigned)cdiv(n, kQ)), dim3(256), 0, s, Ebf, n, k, cand);
+            if ((e = topkPropose(Ebf, n, k, cand, topkWork, s)) != hipSuccess)
+                return cleanup(e);
         } else {
             (void)hipEventRecord(ev[1], s);
             hipLaunchKernelGGL(kNeighbourProposals, dim3((unsigned)cdiv((size_t)n * k, 256)), dim3(256), 0, s, n, k, cand);

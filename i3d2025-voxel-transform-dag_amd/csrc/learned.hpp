@@ -74,6 +74,12 @@ hipError_t matchLearned2(const MatchPlan& plan, Mlp* mlp, uint32_t k, const uint
 // Device workspace bytes matchLearned2 needs for n leaves (besides the plan's scratch).
 size_t learnedWorkBytes(uint64_t n);
 
+// Retrieval of the learned proposal: for each of the n leaves (rank order, bf16 embeddings n x 128
+// row-major) the k <= 64 best earlier leaves under "higher score first, then lower rank", best
+// first, into cand[q * k ..] (0xFFFFFFFF where fewer exist). `work` holds topkWorkBytes(n) bytes.
+size_t topkWorkBytes(uint64_t n);
+hipError_t topkPropose(const uint16_t* Ebf, uint32_t n, uint32_t k, uint32_t* cand, void* work, hipStream_t stream);
+
 // Test hook: C = opA x opB with the bf16 MFMA kernel (row-major storage; TA/TB transpose).
 hipError_t gemmBf16(bool ta, bool tb, const uint16_t* A, int lda, const uint16_t* B, int ldb, float* C, int ldc, int M, int N, int K, hipStream_t stream);
 

@@ -52,9 +52,10 @@ def _bind():
     L.tdag_mlp_train.argtypes = [vp, vp, vp, u64, u32, u32, f32, f32, u64, vp]
     L.tdag_mlp_embed.argtypes = [vp, vp, u64, vp]
     L.tdag_gemm_bf16.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.tdag_mlp_topk.argtypes = [vp, vp, u64, u32, vp]
     L.tdag_match_level_learned_device.argtypes = [vp, vp, u32, vp, u64, ctypes.POINTER(TdagConfig), u32, vp, vp, vp, ctypes.POINTER(TdagLearnedStats)]
     for name in ("tdag_mlp_create", "tdag_mlp_destroy", "tdag_mlp_get_params", "tdag_mlp_set_params", "tdag_mlp_loss_grad", "tdag_mlp_train", "tdag_mlp_embed",
-                 "tdag_gemm_bf16", "tdag_match_level_learned_device"):
+                 "tdag_gemm_bf16", "tdag_mlp_topk", "tdag_match_level_learned_device"):
         getattr(L, name).restype = ctypes.c_int
     PARAMS = int(L.tdag_mlp_param_count())
     _bound = True
@@ -126,6 +127,13 @@ class MLP:
 def gemm_bf16(ctx, ta: bool, tb: bool, d_a, lda: int, d_b, ldb: int, d_c, ldc: int, m: int, n: int, k: int) -> None:
     L = _bind()
     check(L.tdag_gemm_bf16(ctx._ptr, int(ta), int(tb), _ptr(d_a), int(lda), _ptr(d_b), int(ldb), _ptr(d_c), int(ldc), int(m), int(n), int(k)), "tdag_gemm_bf16")
+
+
+def topk(ctx, d_emb, n: int, k: int, d_cand) -> None:
+    """Retrieval step: d_cand (n x k uint32) = the k best earlier rows of d_emb (n x 128 bf16 bits)
+    by dot product, ties to the lower row, best first (0xFFFFFFFF where fewer exist)."""
+    L = _bind()
+    check(L.tdag_mlp_topk(ctx._ptr, _ptr(d_emb), int(n), int(k), _ptr(d_cand)), "tdag_mlp_topk")
 
 
 def match_level_learned_device(ctx, mlp: MLP | None, d_masks, n: int, config: TransformDAGConfig | None = None, k: int = 64, d_encoded=None,

@@ -240,6 +240,11 @@ tdag_status tdag_mlp_embed(tdag_mlp* m, const uint64_t* d_masks, uint64_t n, flo
 /* The bf16 MFMA GEMM of the network, exposed for tests: C (m x n, fp32, row-major, ldc) =
  * op(A) op(B), op(X) = X or X^T of row-major bf16 (raw uint16) device matrices. */
 tdag_status tdag_gemm_bf16(tdag_context* ctx, int trans_a, int trans_b, const uint16_t* d_a, int lda, const uint16_t* d_b, int ldb, float* d_c, int ldc, int m, int n, int k);
+/* The retrieval step of the learned proposal, exposed for tests: for each of the n rows of bf16
+ * embeddings (device, n x 128 raw uint16, row = rank) the k (<= 64) best earlier rows under
+ * "higher dot product first (bf16 MFMA, fp32 accumulate), then lower row", best first, into
+ * d_cand (device, n x k; 0xFFFFFFFF where fewer than k earlier rows exist). */
+tdag_status tdag_mlp_topk(tdag_context* ctx, const uint16_t* d_emb, uint64_t n, uint32_t k, uint32_t* d_cand);
 /* Level-2 matcher with proposals from `mlp` (NULL: the k leaves ranked just before each query,
  * i.e. popcount-neighbour proposals without a network). Outputs as tdag_match_level_device. */
 tdag_status tdag_match_level_learned_device(tdag_context* ctx, tdag_mlp* mlp, uint32_t level, const uint64_t* d_masks, uint64_t n, const tdag_config* cfg, uint32_t k,

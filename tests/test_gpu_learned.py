@@ -37,13 +37,16 @@ def _from_bf16_bits(b):
 
 
 @pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True), (True, True)])
-def test_gemm_bf16(gpu_ctx, ta, tb):
+@pytest.mark.parametrize("shape", [(100, 70, 37), (200, 130, 1500)])
+def test_gemm_bf16(gpu_ctx, ta, tb, shape):
+    """Ragged sizes; the second shape (few output tiles, long K) takes the split-K path of the
+    weight gradients, whose slices are summed in a fixed order."""
     import torch
 
     from tdag_amd import learned
 
     rng = np.random.default_rng(10 + 2 * ta + tb)
-    M, N, K = 100, 70, 37
+    M, N, K = shape
     A = rng.normal(size=(K, M) if ta else (M, K))
     B = rng.normal(size=(N, K) if tb else (K, N))
     Ab, Bb = _bf16_bits(A), _bf16_bits(B)
@@ -55,6 +58,51 @@ def test_gemm_bf16(gpu_ctx, ta, tb):
     learned.gemm_bf16(gpu_ctx, ta, tb, dA, A.shape[1], dB, B.shape[1], dC, N, M, N, K)
     C = dC.cpu().numpy().astype(np.float64)
     assert np.max(np.abs(C - ref)) < 1e-4 * (1 + np.max(np.abs(ref)))
+
+
+def _topk_reference(E, k):
+    """For each row q, the k rows c < q with the highest <E_q, E_c>, ties to the lower row, best
+    first; 0xFFFFFFFF pads rows with fewer than k earlier rows. E holds small integers, so float64
+    products are exact."""
+    n = E.shape[0]
+    out = np.full((n, k), 0xFFFFFFFF, dtype=np.uint32)
+    Ef = E.astype(np.float64)
+    for q0 in range(0, n, 512):
+        q1 = min(n, q0 + 512)
+        S = (Ef[q0:q1] @ Ef[:q1].T).astype(np.int64)
+        c = np.arange(q1, dtype=np.int64)
+        key = S * (1 << 22) + ((1 << 22) - 1 - c)[None, :]
+        for qi in range(q1 - q0):
+            q = q0 + qi
+            if q == 0:
+                continue
+            kk = key[qi, :q]
+            top = np.argsort(-kk, kind="stable")[:k]
+            out[q, : len(top)] = top
+    return out
+
+
+@pytest.mark.parametrize("n,k", [(1, 64), (5, 64), (64, 64), (65, 7), (1000, 64), (9000, 64), (9000, 13)])
+def test_topk_retrieval_exact(gpu_ctx, n, k):
+    """The retrieval returns exactly the k best earlier rows under (higher score, then lower row),
+    best first. Sparse {-1, 0, 1} embeddings make every fp32 score exact and ties frequent, so the
+    tie-break and the merge of chunk lists (9000 rows span several chunks per query block) are
+    checked bit for bit."""
+    import torch
+
+    from tdag_amd import learned
+
+    rng = np.random.default_rng(n * 131 + k)
+    E = (rng.random((n, 128)) < 0.08) * rng.choice([-1, 1], size=(n, 128))
+    Eb = _bf16_bits(E.astype(np.float32))
+    d_e = torch.from_numpy(Eb.view(np.int16).copy()).cuda()
+    d_c = torch.zeros(n * k, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    learned.topk(gpu_ctx, d_e, n, k, d_c)
+    got = d_c.cpu().numpy().view(np.uint32).reshape(n, k)
+    ref = _topk_reference(E, k)
+    bad = np.nonzero((got != ref).any(axis=1))[0]
+    assert bad.size == 0, f"{bad.size} queries differ, first {bad[:5]}: got {got[bad[0]][:8]} want {ref[bad[0]][:8]}"
 
 
 def _passthrough_params():
