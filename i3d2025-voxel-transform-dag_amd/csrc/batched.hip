@@ -10,7 +10,8 @@
 // done by all blocks together. Tiles are ordered heaviest level first inside a phase.
 //
 // Tile shapes (four waves per block):
-//   level 5: one family of 8 combos on 8 of the 32 planes            (24 tiles per parent)
+//   level 5: one (permutation, direction) combo on 8 of the 32 planes (192 tiles per parent, 32
+//            when the table holds the children in all six permutation frames)
 //   level 4: one (permutation, direction) combo on all 16 planes      (48 tiles per parent)
 //   level 3: four wave items, each a family sharing passes 1 and 2 on all 8 planes (6 per parent)
 //   level 2: four parents, one per wave
@@ -68,7 +69,7 @@ uint32_t tilesFor(const MatchPlan& p, uint32_t parents)
     const uint32_t cfg = p.childFrames == 6 ? kCfgChildFrames : 0u;
     switch (p.level) {
     case 5:
-        return parents * searchUnits(kUnitFamily, cfg) * kL5Chunks;
+        return parents * searchUnits(kUnitCombo, cfg) * kL5Chunks;
     case 4:
         return parents * searchUnits(kUnitCombo, cfg);
     case 3:
@@ -115,17 +116,27 @@ __device__ __forceinline__ void waveItem3(const LevelView& v, uint32_t r, uint32
     flushCounts(cnt, v.counters, cfgBits);
 }
 
+// Wave item of level 2: the calling wave searches parent r.
+__device__ __forceinline__ void waveItem2(const LevelView& v, uint32_t r, uint32_t cfgBits)
+{
+    ProbeCounts cnt;
+    search2Parent(r, v.sortedMasks, v.tabKeys, v.tabVals, v.logCap, v.bloom, v.logBloom, v.best, cfgBits, cnt);
+    flushCounts(cnt, v.counters, cfgBits);
+}
+
 // Grid-wide barrier of the persistent kernel (all blocks are resident). Thread 0 of each block
-// arrives with an agent-scope release and waits for the generation to move with an acquire, so
-// table, filter and best[] writes made before the barrier are visible after it on every XCD.
-// A wait longer than `spinTicks` wall-clock ticks sets sync->error and gives up (the results
-// are then flagged as invalid by the host) instead of hanging; once the flag is set no block
-// waits again.
+// arrives with an agent-scope release and, once the generation has moved, takes one agent-scope
+// acquire, so table, filter and best[] writes made before the barrier are visible after it on
+// every XCD. The wait itself polls with relaxed loads: an agent-scope acquire invalidates the
+// caches of the waiting CU's XCD, and blocks polling that way would keep evicting the tables
+// the blocks still searching on that XCD read. A wait longer than `spinTicks` wall-clock ticks
+// sets sync->error and gives up (the results are then flagged as invalid by the host) instead
+// of hanging; once the flag is set no block waits again.
 __device__ __forceinline__ void gridBarrier(BatchSync* sync, uint64_t spinTicks)
 {
     __syncthreads();
     if (threadIdx.x == 0) {
-        const uint32_t gen = __hip_atomic_load(&sync->generation, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t gen = __hip_atomic_load(&sync->generation, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __threadfence();
         const uint32_t arrived = __hip_atomic_fetch_add(&sync->arrive, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u;
         if (arrived == gridDim.x) {
@@ -133,7 +144,7 @@ __device__ __forceinline__ void gridBarrier(BatchSync* sync, uint64_t spinTicks)
             __hip_atomic_fetch_add(&sync->generation, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             const uint64_t t0 = (uint64_t)wall_clock64();
-            while (__hip_atomic_load(&sync->generation, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+            while (__hip_atomic_load(&sync->generation, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
                 __builtin_amdgcn_s_sleep(4);
                 if (__hip_atomic_load(&sync->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
                     break; // another block already gave up: do not wait again
@@ -148,6 +159,8 @@ __device__ __forceinline__ void gridBarrier(BatchSync* sync, uint64_t spinTicks)
     __syncthreads();
 }
 
+// Three blocks per CU: a budget of 168 VGPRs holds every tile path without spills (a fourth
+// block would fit the LDS, but the 128-VGPR budget it leaves spills and measured no faster).
 __global__ __launch_bounds__(kBatchThreads, 3) void kBatched(const BatchDesc* __restrict__ desc, BatchSync* __restrict__ sync, uint64_t spinTicks)
 {
     __shared__ __align__(16) unsigned char s_mem[kBatchLds];
@@ -184,9 +197,9 @@ __global__ __launch_bounds__(kBatchThreads, 3) void kBatched(const BatchDesc* __
                 const uint32_t lvCfg = levelCfgBits(v, cfgBits);
                 switch (v.level) {
                 case 5: {
-                    const uint32_t per = searchUnits(kUnitFamily, lvCfg) * kL5Chunks;
+                    const uint32_t per = searchUnits(kUnitCombo, lvCfg) * kL5Chunks;
                     const uint32_t m = k % per;
-                    blockTile<5, kUnitFamily>(v, pl.r0 + k / per, m / kL5Chunks, m % kL5Chunks, s_mem, s_occ, s_queue[wave], lvCfg);
+                    blockTile<5, kUnitCombo>(v, pl.r0 + k / per, m / kL5Chunks, m % kL5Chunks, s_mem, s_occ, s_queue[wave], lvCfg);
                     break;
                 }
                 case 4: {
@@ -203,11 +216,8 @@ __global__ __launch_bounds__(kBatchThreads, 3) void kBatched(const BatchDesc* __
                 }
                 default: {
                     const uint32_t r = pl.r0 + k * kBatchWaves + wave;
-                    if (r < pl.r1) {
-                        ProbeCounts cnt;
-                        search2Parent(r, v.sortedMasks, v.tabKeys, v.tabVals, v.logCap, v.bloom, v.logBloom, v.best, cfgBits, cnt);
-                        flushCounts(cnt, v.counters, cfgBits);
-                    }
+                    if (r < pl.r1)
+                        waveItem2(v, r, cfgBits);
                     break;
                 }
                 }
