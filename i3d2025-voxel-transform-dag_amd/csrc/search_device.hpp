@@ -12,9 +12,13 @@
 namespace tdag {
 namespace dev {
 
-__device__ __forceinline__ uint64_t loadRelaxed(const uint64_t* p)
+// best[c] as a filter before verification: a plain load, which may hit in the XCD's L2 and
+// may return an older (larger) value than another XCD's latest atomicMin. That only lets a
+// candidate through to verification; the atomicMin after it decides. (An agent-scope atomic
+// load would bypass the L2 on every candidate.)
+__device__ __forceinline__ uint64_t bestHint(const uint64_t* p)
 {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return *p;
 }
 
 __device__ __forceinline__ bool configAllows(uint32_t cfgBits, uint32_t permCode, uint32_t sym, int tx, int ty, int tz)
@@ -149,7 +153,7 @@ __device__ __forceinline__ void search2Parent(uint32_t r, const uint64_t* __rest
                         if (configAllows(cfgBits, pc, sym, tx, ty, tz)) {
                             ++cnt.candidates;
                             const uint64_t k64 = encodeKey(r, pc, sym, tx, ty, tz, 2);
-                            if (k64 < loadRelaxed(&best[c])) {
+                            if (k64 < bestHint(&best[c])) {
                                 ++cnt.verifications;
                                 // Bitwise verification of the derived transform.
                                 const uint64_t T = translate4(mirror4(Q, sym), tx, ty, tz);
@@ -627,7 +631,7 @@ __device__ __forceinline__ void probeQueue(const uint16_t* queue, uint32_t count
                     continue;
                 ++cnt.candidates;
                 const uint64_t k64 = encodeKey(r, tc, sym, tt[0], tt[1], tt[2], L);
-                if (k64 >= loadRelaxed(&best[cc]))
+                if (k64 >= bestHint(&best[cc]))
                     continue;
                 ++cnt.verifications;
                 found = true;
