@@ -587,6 +587,7 @@ hipError_t lossGrad(Mlp& mlp, Work& w, uint32_t B, float tau, hipStream_t s)
 // queues the same way. Keys order (score, rank) exactly, so the proposal set is the top k under
 // that order, independent of the chunking and of thread timing.
 constexpr int kQ = 64;
+constexpr int kChasePasses = 3; // parent-chase passes after the fallback search
 
 // Blocks of query blocks 0..qb-1: sum over m = 1..qb of ceil(m / ct).
 __host__ __device__ __forceinline__ uint64_t chunksBefore(uint32_t qb, uint32_t ct)
@@ -939,19 +940,15 @@ __global__ __launch_bounds__(256) void kTransforms48(const uint64_t* __restrict_
 // One thread per (query, proposal): the smallest key of a transform T with C == T(P_r) (perm,
 // then mirror, then the clipping translation), merged into best[q]. For each axis one side of
 // the translated grid is not clipped, so t_a is C.lo_a - M.lo_a or C.hi_a - M.hi_a.
-__global__ __launch_bounds__(256) void kVerifyProposals(const uint64_t* __restrict__ sortedMasks, const uint64_t* __restrict__ images, const uint16_t* __restrict__ boxes,
-    uint32_t n, uint32_t k, const uint32_t* __restrict__ cand, uint32_t cfgBits, uint64_t* __restrict__ best, unsigned long long* __restrict__ hits)
+// The smallest key of a transform T with C_q == T(P_r) (perm, then mirror, then the clipping
+// translation), or ~0 if there is none. For each axis one side of the translated grid is not
+// clipped, so t_a is C.lo_a - M.lo_a or C.hi_a - M.hi_a.
+__device__ uint64_t verifyPair(const uint64_t* __restrict__ sortedMasks, const uint64_t* __restrict__ images, const uint16_t* __restrict__ boxes, uint32_t q,
+    uint32_t r, uint32_t cfgBits)
 {
-    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= (size_t)n * k)
-        return;
-    const uint32_t q = (uint32_t)(i / k);
-    const uint32_t r = cand[i];
-    if (r == 0xFFFFFFFFu || r >= q)
-        return;
     const uint64_t C = sortedMasks[q], P = sortedMasks[r];
     if (C == 0 || P == 0 || __popcll(P) < __popcll(C))
-        return;
+        return ~0ull;
     const uint32_t cb = boxes[(size_t)q * 48]; // identity image = C itself
     const int clo[3] = { (int)(cb & 3u), (int)((cb >> 2) & 3u), (int)((cb >> 4) & 3u) };
     const int chi[3] = { (int)((cb >> 6) & 3u), (int)((cb >> 8) & 3u), (int)((cb >> 10) & 3u) };
@@ -984,11 +981,53 @@ __global__ __launch_bounds__(256) void kVerifyProposals(const uint64_t* __restri
                     }
         }
     }
+    return bestKey;
+}
+
+// One thread per (query, proposal): verifyPair merged into best[q].
+__global__ __launch_bounds__(256) void kVerifyProposals(const uint64_t* __restrict__ sortedMasks, const uint64_t* __restrict__ images, const uint16_t* __restrict__ boxes,
+    uint32_t n, uint32_t k, const uint32_t* __restrict__ cand, uint32_t cfgBits, uint64_t* __restrict__ best, unsigned long long* __restrict__ hits)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (size_t)n * k)
+        return;
+    const uint32_t q = (uint32_t)(i / k);
+    const uint32_t r = cand[i];
+    if (r == 0xFFFFFFFFu || r >= q)
+        return;
+    const uint64_t bestKey = verifyPair(sortedMasks, images, boxes, q, r, cfgBits);
     if (bestKey != ~0ull) {
         const unsigned long long prev = atomicMin((unsigned long long*)&best[q], (unsigned long long)bestKey);
         if (prev == (unsigned long long)selfKey(q, 2))
             atomicAdd(hits, 1ull); // first verified proposal of this query
     }
+}
+
+// Parent chase: a query q matched to p, where p is itself matched to a lower rank p', tries p'
+// (for unclipped transforms the 48-element group composes, so p' is often a parent of q too).
+// Accepting the smaller verified key moves q's parent down the chain as the exhaustive minimum
+// would, so fewer leaves stay parents. Keys are read from `prev`, a snapshot of best[] taken
+// before the pass, so the outcome does not depend on thread order; every accepted key is
+// verified bitwise.
+__global__ __launch_bounds__(256) void kChaseParents(const uint64_t* __restrict__ sortedMasks, const uint64_t* __restrict__ images, const uint16_t* __restrict__ boxes,
+    uint32_t n, uint32_t cfgBits, const uint64_t* __restrict__ prev, uint64_t* __restrict__ best)
+{
+    const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+    if (q >= n)
+        return;
+    const uint64_t kq = prev[q];
+    if (kq == sentinelKey(2) || kq == selfKey(q, 2))
+        return;
+    const uint32_t p = (uint32_t)keyPtr(kq, 2);
+    const uint64_t kp = prev[p];
+    if (kp == sentinelKey(2))
+        return;
+    const uint32_t pp = (uint32_t)keyPtr(kp, 2);
+    if (pp >= p)
+        return;
+    const uint64_t key = verifyPair(sortedMasks, images, boxes, q, pp, cfgBits);
+    if (key < kq)
+        best[q] = key; // only this thread writes best[q] during the pass
 }
 
 // Fallback flags: non-empty queries whose best key is still the self-match.
@@ -1039,7 +1078,7 @@ size_t learnedWorkBytes(uint64_t n)
     // proposals, flags, bf16 embeddings, counters, 48 images and boxes per leaf, the retrieval's
     // per-chunk lists (the embedding workspace itself lives in the Mlp)
     return align256(n * kMaxProposals * 4) + align256(n * 4) + align256((n + kQ) * kEmb * 2) + 256 + align256(n * 48 * 8) + align256(n * 48 * 2) +
-        topkWorkBytes(n);
+        align256(n * 8) + topkWorkBytes(n);
 }
 
 hipError_t gemmBf16(bool ta, bool tb, const uint16_t* A, int lda, const uint16_t* B, int ldb, float* C, int ldc, int M, int N, int K, hipStream_t stream)
@@ -1216,7 +1255,8 @@ hipError_t matchLearned2(const MatchPlan& p, Mlp* mlp, uint32_t k, const uint64_
     unsigned long long* cnt = (unsigned long long*)((char*)Ebf + align256(((size_t)n + kQ) * kEmb * 2));
     uint64_t* images = (uint64_t*)((char*)cnt + 256);
     uint16_t* boxes = (uint16_t*)((char*)images + align256((size_t)n * 48 * 8));
-    void* topkWork = (char*)boxes + align256((size_t)n * 48 * 2);
+    uint64_t* prevBest = (uint64_t*)((char*)boxes + align256((size_t)n * 48 * 2));
+    void* topkWork = (char*)prevBest + align256((size_t)n * 8);
     hipEvent_t ev[5];
     for (auto& e : ev)
         LEARN_TRY(hipEventCreate(&e));
@@ -1267,6 +1307,14 @@ hipError_t matchLearned2(const MatchPlan& p, Mlp* mlp, uint32_t k, const uint64_
             return cleanup(e);
         if ((e = enqueueSearch(p, cfgBits, d_scratch, s)) != hipSuccess)
             return cleanup(e);
+        // 4. Parent chase (pointer jumping along verified parent chains).
+        for (int pass = 0; pass < kChasePasses; ++pass) {
+            if ((e = hipMemcpyAsync(prevBest, v.best, (size_t)n * 8, hipMemcpyDeviceToDevice, s)) != hipSuccess)
+                return cleanup(e);
+            hipLaunchKernelGGL(kChaseParents, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, s, v.sortedMasks, images, boxes, n, cfgBits, prevBest, v.best);
+            if ((e = hipGetLastError()) != hipSuccess)
+                return cleanup(e);
+        }
         (void)hipEventRecord(ev[4], s);
     }
     if ((e = enqueueFinish(p, d_scratch, s, d_encoded, d_mapping, d_unique)) != hipSuccess)
