@@ -353,7 +353,7 @@ def main():
             if dev[level]["n"] > 0:
                 best, first, ms, tuned = ctx.autotune(level, dev[level]["masks"], dev[level]["n"], cfg, repeats=1)
                 tuning[level] = {"variant": best, "first_batch": first, "growth": ctx.get_batching(level)[1], "filter_bits": ctx.get_filter_bits(level),
-                                 "child_frames": ctx.get_child_frames(level) if level >= 3 else 1, "search_ms_by_variant": ms, "tuned_search_ms": tuned}
+                                 "child_frames": ctx.get_child_frames(level) if level >= 3 else 1, "call_ms_by_variant": ms, "tuned_call_ms": tuned}
         tuning_source = "autotuned in this run"
         print(f"[bench] autotuned search variants: {tuning}", file=sys.stderr, flush=True)
     if args.save_tuning and tuning:

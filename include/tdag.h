@@ -152,9 +152,11 @@ tdag_status tdag_context_get_child_frames(const tdag_context* ctx, uint32_t leve
  * counts, keeping the one with the fastest variant), then tries four first-batch sizes
  * (n/256, n/64, n/16, n) with each of the three fastest variants and keeps the fastest pair,
  * then filter sizes of 4, 8, 16 and 32 bits per table entry; the winners become this context's
- * settings for `level`. out_ms (TDAG_VARIANT_SLOTS + 1 floats, may be NULL) receives the average
- * search time per variant at the default schedule (-1 for invalid or unused slots) followed by
- * the tuned time; also writes the winning variant and first batch. No reference counterpart. */
+ * settings for `level`. Every trial times the whole matcher call (preparation, search,
+ * post-processing), since the frame count and the filter size also change the table build.
+ * out_ms (TDAG_VARIANT_SLOTS + 1 floats, may be NULL) receives the average call time per variant
+ * at the default schedule (-1 for invalid or unused slots) followed by the tuned time; also
+ * writes the winning variant and first batch. No reference counterpart. */
 tdag_status tdag_autotune_level(tdag_context* ctx, uint32_t level, const uint64_t* d_masks, uint64_t n, const tdag_config* cfg, uint32_t repeats,
     float* out_ms, int32_t* out_best, uint32_t* out_first_batch);
 
