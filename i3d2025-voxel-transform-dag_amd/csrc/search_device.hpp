@@ -443,11 +443,28 @@ __device__ __forceinline__ void passes12(uint32_t pc, uint32_t d, const typename
     }
 }
 
+// Buffer resource over the Bloom filter's 2^logBloom words: a raw buffer load at an offset past
+// the end returns 0 without touching memory.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t bloomRsrc(const uint32_t* bloom, uint32_t logBloom)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(bloom), (short)0, (int)(4u << logBloom), 0x00020000);
+}
+
+// Filter word of nested hash f (its top bits, word 0 folded into word 1), or 0 for f == 0: an
+// empty region's hash is never in the table, and its lane then issues no memory request (the
+// offset is out of range), which keeps the filter's cache-line traffic to the non-empty voxels.
+__device__ __forceinline__ uint32_t bloomWordOrZero(__amdgpu_buffer_rsrc_t rs, uint32_t f, uint32_t shift)
+{
+    const uint32_t w = f >> shift;
+    const uint32_t off = f != 0u ? (w > 1u ? w : 1u) * 4u : 0xFFFFFFF0u;
+    return __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0);
+}
+
 // Pass 3 along the lane's line (c1 = u) of its plane over the pass-2 states in[Pitch*c2], fused
 // with the Bloom filter: stores the final hash of every voxel at out[Pitch*c2] (out may equal in)
 // and returns the bit set of voxels c2 that are non-empty and pass the filter. Filter words are
-// loaded unconditionally (every hash indexes a valid word), one chunk of R/4 steps at a time, and
-// tested while the next chunk hashes. Chunks inside the first `lead` steps (wave-uniform; their
+// loaded without branches (empty voxels read 0 through an out-of-range buffer offset), one chunk
+// of R/4 steps at a time, and tested while the next chunk hashes. Chunks inside the first `lead` steps (wave-uniform; their
 // states are 0) are skipped: their voxels are empty, never pending, and out keeps its old words.
 template <int L, bool Pos2>
 __device__ __forceinline__ uint32_t pass3Filter(const uint32_t* in, uint32_t* out, const uint32_t* __restrict__ bloom, uint32_t logBloom, uint32_t cfgBits,
@@ -456,6 +473,7 @@ __device__ __forceinline__ uint32_t pass3Filter(const uint32_t* in, uint32_t* ou
     constexpr int R = Geo<L>::R;
     constexpr int C = R / 4;
     const uint32_t shift = 32u - logBloom;
+    const __amdgpu_buffer_rsrc_t rs = bloomRsrc(bloom, logBloom);
     uint32_t st[R];
 #pragma unroll
     for (int k = 0; k < R; ++k)
@@ -475,9 +493,7 @@ __device__ __forceinline__ uint32_t pass3Filter(const uint32_t* in, uint32_t* ou
             s = hashStep(s, st[k0 + k]);
             f[k] = finalHash(s);
             out[Geo<L>::Pitch * (Pos2 ? (R - 1 - k0 - k) : k0 + k)] = f[k];
-            // An empty region has f == 0, whose word (0) is never set: no emptiness test needed.
-            const uint32_t w = f[k] >> shift;
-            bw[k] = bloom[w > 1u ? w : 1u];
+            bw[k] = bloomWordOrZero(rs, f[k], shift);
         }
         if (k0 > 0) {
 #pragma unroll
