@@ -113,7 +113,7 @@ __global__ __launch_bounds__(256) void kBuildTable2(const uint64_t* __restrict__
 }
 
 // One wave per parent rank (four per block).
-__global__ __launch_bounds__(256) void kSearch2(const uint64_t* __restrict__ sortedMasks, uint32_t n, const uint64_t* __restrict__ tabKeys, const uint32_t* __restrict__ tabVals,
+__global__ __launch_bounds__(256, 8) void kSearch2(const uint64_t* __restrict__ sortedMasks, uint32_t n, const uint64_t* __restrict__ tabKeys, const uint32_t* __restrict__ tabVals,
     uint32_t logCap, const uint32_t* __restrict__ bloom, uint32_t logBloom, uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t rankBase,
     unsigned long long* __restrict__ counters)
 {
