@@ -112,16 +112,19 @@ __global__ __launch_bounds__(256) void kBuildTable2(const uint64_t* __restrict__
     }
 }
 
-// One wave per parent rank (four per block).
+// One wave per parent rank (four per block), each with its own work list in LDS.
 __global__ __launch_bounds__(256, 8) void kSearch2(const uint64_t* __restrict__ sortedMasks, uint32_t n, const uint64_t* __restrict__ tabKeys, const uint32_t* __restrict__ tabVals,
     uint32_t logCap, const uint32_t* __restrict__ bloom, uint32_t logBloom, uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t rankBase,
     unsigned long long* __restrict__ counters)
 {
-    const uint32_t r = rankBase + blockIdx.x * 4u + (threadIdx.x >> 6);
+    __shared__ uint64_t s_qKey[4][kWalkQueue2];
+    __shared__ uint16_t s_qMeta[4][kWalkQueue2];
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t r = rankBase + blockIdx.x * 4u + wave;
     if (r >= n)
         return;
     ProbeCounts cnt;
-    search2Parent(r, sortedMasks, tabKeys, tabVals, logCap, bloom, logBloom, best, cfgBits, cnt);
+    search2ParentQueued(r, sortedMasks, tabKeys, tabVals, logCap, bloom, logBloom, best, cfgBits, s_qKey[wave], s_qMeta[wave], cnt);
     flushCounts(cnt, counters, cfgBits);
 }
 

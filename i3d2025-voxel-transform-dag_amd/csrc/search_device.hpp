@@ -177,6 +177,135 @@ __device__ __forceinline__ void search2Parent(uint32_t r, const uint64_t* __rest
     }
 }
 
+// Orders the calling wave's LDS accesses without waiting for its global loads.
+__device__ __forceinline__ void waveSyncLds()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
+// Capacity of the level-2 work list of one wave (entries of 8 + 2 bytes in LDS).
+constexpr uint32_t kWalkQueue2 = 128;
+
+// Table walks of `count` queued level-2 keys of parent r under permutation pc (its permuted mask
+// Q): lane j walks entry base + j, so the walks of all directions run side by side. Entry meta:
+// source lane (anchored voxel) | direction << 6.
+__device__ __forceinline__ void walkQueue2(const uint64_t* qKey, const uint16_t* qMeta, uint32_t count, uint32_t r, uint32_t pc, uint64_t Q,
+    const uint64_t* __restrict__ sortedMasks, const uint64_t* __restrict__ tabKeys, const uint32_t* __restrict__ tabVals, uint32_t logCap,
+    uint64_t* __restrict__ best, uint32_t cfgBits, ProbeCounts& cnt)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t capMask = (1u << logCap) - 1u;
+    for (uint32_t base = 0; base < count; base += 64) {
+        if (base + lane >= count)
+            continue;
+        const uint64_t key = qKey[base + lane];
+        const uint32_t meta = qMeta[base + lane];
+        const uint32_t src = meta & 63u, d = meta >> 6;
+        const int vx = (int)(src & 3u), vy = (int)((src >> 2) & 3u), vz = (int)(src >> 4);
+        uint32_t s = homeSlot64(key, logCap);
+        uint64_t k = tabKeys[s];
+        while (k != 0) {
+            if (k == key) {
+                ++cnt.keyHits;
+                const uint32_t e = tabVals[s];
+                const uint32_t c = e >> 3;
+                const uint32_t kc = e & 7u;
+                if (c > r && c != kDeadRank) {
+                    const uint32_t sym = d ^ kc;
+                    const int tx = (kc & 1u) ? -vx : vx;
+                    const int ty = (kc & 2u) ? -vy : vy;
+                    const int tz = (kc & 4u) ? -vz : vz;
+                    if (configAllows(cfgBits, pc, sym, tx, ty, tz)) {
+                        ++cnt.candidates;
+                        const uint64_t k64 = encodeKey(r, pc, sym, tx, ty, tz, 2);
+                        if (k64 < bestHint(&best[c])) {
+                            ++cnt.verifications;
+                            // Bitwise verification of the derived transform.
+                            const uint64_t T = translate4(mirror4(Q, sym), tx, ty, tz);
+                            if (T == sortedMasks[c]) {
+                                ++cnt.verified;
+                                atomicMin((unsigned long long*)&best[c], (unsigned long long)k64);
+                            }
+                        }
+                    }
+                }
+            }
+            s = (s + 1) & capMask;
+            k = tabKeys[s];
+        }
+    }
+}
+
+// Level-2 search of parent r by one wave with a work list: for each permutation the 8 direction
+// keys of every lane (anchored voxel) are filtered, the passes are appended to the wave's LDS
+// queue (qKey/qMeta, kWalkQueue2 entries), and full queues, and the rest at the end of the
+// permutation, are walked by walkQueue2 with one entry per lane. Compared with walking each
+// direction's passes in place, the wave pays one chain of table round trips per 64 passes
+// instead of one per direction. Same candidates, same verification, same minimum.
+__device__ __forceinline__ void search2ParentQueued(uint32_t r, const uint64_t* __restrict__ sortedMasks, const uint64_t* __restrict__ tabKeys,
+    const uint32_t* __restrict__ tabVals, uint32_t logCap, const uint32_t* __restrict__ bloom, uint32_t logBloom, uint64_t* __restrict__ best, uint32_t cfgBits,
+    uint64_t* qKey, uint16_t* qMeta, ProbeCounts& cnt)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t P = sortedMasks[r];
+    if (P == 0)
+        return;
+    const int vx = (int)(lane & 3u), vy = (int)((lane >> 2) & 3u), vz = (int)(lane >> 4);
+    const uint64_t clip = rangeMaskX4(0, 4 - vx) & rangeMaskY4(0, 4 - vy) & rangeMaskZ4(0, 4 - vz);
+    const uint32_t numPerms = (cfgBits & kCfgAxis) ? 6u : 1u;
+    for (uint32_t pc = 0; pc < numPerms; ++pc) {
+        // permute (find_translation.cpp:109-125): out(o) = P(v) with v[p[a]] = o[a].
+        int v[3];
+        v[permAxis(pc, 0)] = vx;
+        v[permAxis(pc, 1)] = vy;
+        v[permAxis(pc, 2)] = vz;
+        const uint32_t src = (uint32_t)(v[0] + 4 * v[1] + 16 * v[2]);
+        const uint64_t Q = (uint64_t)__ballot((P >> src) & 1ull);
+        uint32_t count = 0;
+#pragma unroll 1
+        for (uint32_t d0 = 0; d0 < 8; d0 += kDirGroup) {
+            uint64_t key[kDirGroup];
+            uint32_t bw[kDirGroup];
+#pragma unroll
+            for (uint32_t j = 0; j < kDirGroup; ++j)
+                key[j] = (mirror4(Q, (~(d0 + j)) & 7u) >> lane) & clip;
+#pragma unroll
+            for (uint32_t j = 0; j < kDirGroup; ++j)
+                bw[j] = key[j] ? bloom[bloomWord(bloomMix64(key[j]), logBloom)] : 0u;
+#pragma unroll
+            for (uint32_t j = 0; j < kDirGroup; ++j) {
+                const uint32_t need = bloomBits(bloomMix64(key[j]));
+                cnt.probes += key[j] != 0;
+                const bool pass = key[j] != 0 && (bw[j] & need) == need;
+                const uint64_t m = __ballot(pass);
+                if (m == 0ull)
+                    continue;
+                const uint32_t np = (uint32_t)__popcll(m);
+                if (count + np > kWalkQueue2) {
+                    waveSyncLds();
+                    walkQueue2(qKey, qMeta, count, r, pc, Q, sortedMasks, tabKeys, tabVals, logCap, best, cfgBits, cnt);
+                    waveSyncLds();
+                    count = 0;
+                }
+                if (pass) {
+                    ++cnt.filterPass;
+                    const uint32_t pos = count + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                    qKey[pos] = key[j];
+                    qMeta[pos] = (uint16_t)(lane | ((d0 + j) << 6));
+                }
+                count += np;
+            }
+        }
+        if (count > 0) {
+            waveSyncLds();
+            walkQueue2(qKey, qMeta, count, r, pc, Q, sortedMasks, tabKeys, tabVals, logCap, best, cfgBits, cnt);
+            waveSyncLds();
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // ---------------------------------------------------------------------------------------
 // Levels 3..5: 32-bit nested sub-box hashes computed in LDS, one line per thread per pass.
