@@ -14,7 +14,7 @@
 //            when the table holds the children in all six permutation frames)
 //   level 4: one (permutation, direction) combo on all 16 planes      (48 tiles per parent)
 //   level 3: four wave items, each a family sharing passes 1 and 2 on all 8 planes (6 per parent)
-//   level 2: four parents, one per wave
+//   level 2: four parents, one per wave (each with its work list in its LDS slice)
 // Results are bit-identical to the per-level launches: the search computes the same exhaustive
 // minimum, whatever the schedule.
 #include <hip/hip_runtime.h>
@@ -116,11 +116,15 @@ __device__ __forceinline__ void waveItem3(const LevelView& v, uint32_t r, uint32
     flushCounts(cnt, v.counters, cfgBits);
 }
 
-// Wave item of level 2: the calling wave searches parent r.
-__device__ __forceinline__ void waveItem2(const LevelView& v, uint32_t r, uint32_t cfgBits)
+// Wave item of level 2: the calling wave searches parent r with its work list in its LDS slice.
+static_assert(kWalkQueue2 * (8 + 2) <= kL3WaveBytes, "level-2 work list fits a wave's LDS slice");
+__device__ __forceinline__ void waveItem2(const LevelView& v, uint32_t r, unsigned char* s_mem, uint32_t cfgBits)
 {
+    unsigned char* base = s_mem + (threadIdx.x >> 6) * kL3WaveBytes;
+    uint64_t* qKey = reinterpret_cast<uint64_t*>(base);
+    uint16_t* qMeta = reinterpret_cast<uint16_t*>(base + kWalkQueue2 * 8);
     ProbeCounts cnt;
-    search2Parent(r, v.sortedMasks, v.tabKeys, v.tabVals, v.logCap, v.bloom, v.logBloom, v.best, cfgBits, cnt);
+    search2ParentQueued(r, v.sortedMasks, v.tabKeys, v.tabVals, v.logCap, v.bloom, v.logBloom, v.best, cfgBits, qKey, qMeta, cnt);
     flushCounts(cnt, v.counters, cfgBits);
 }
 
@@ -217,7 +221,7 @@ __global__ __launch_bounds__(kBatchThreads, 3) void kBatched(const BatchDesc* __
                 default: {
                     const uint32_t r = pl.r0 + k * kBatchWaves + wave;
                     if (r < pl.r1)
-                        waveItem2(v, r, cfgBits);
+                        waveItem2(v, r, s_mem, cfgBits);
                     break;
                 }
                 }

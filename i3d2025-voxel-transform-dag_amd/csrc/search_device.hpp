@@ -99,83 +99,10 @@ __device__ __forceinline__ void flushCounts(const ProbeCounts& c, unsigned long 
                 atomicAdd(&counters[2 + i], (unsigned long long)v[i]);
 }
 
-// Level-2 search of parent rank r by one wave; lane = anchored voxel v' = (x, y, z) of the
-// sub-box. For each permutation the direction keys are formed kDirGroup at a time and their
-// filter loads issued together (a group of 2 keeps the kernel at 8 waves per SIMD without spills).
+// Level-2 search: lane = anchored voxel v' = (x, y, z) of the sub-box. For each permutation the
+// direction keys are formed kDirGroup at a time and their filter loads issued together (a group
+// of 2 keeps the kernel at 8 waves per SIMD).
 constexpr uint32_t kDirGroup = 2;
-__device__ __forceinline__ void search2Parent(uint32_t r, const uint64_t* __restrict__ sortedMasks, const uint64_t* __restrict__ tabKeys,
-    const uint32_t* __restrict__ tabVals, uint32_t logCap, const uint32_t* __restrict__ bloom, uint32_t logBloom, uint64_t* __restrict__ best, uint32_t cfgBits,
-    ProbeCounts& cnt)
-{
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint64_t P = sortedMasks[r];
-    if (P == 0)
-        return;
-    const int vx = (int)(lane & 3u), vy = (int)((lane >> 2) & 3u), vz = (int)(lane >> 4);
-    const uint64_t clip = rangeMaskX4(0, 4 - vx) & rangeMaskY4(0, 4 - vy) & rangeMaskZ4(0, 4 - vz);
-    const uint32_t capMask = (1u << logCap) - 1u;
-    const uint32_t numPerms = (cfgBits & kCfgAxis) ? 6u : 1u;
-    for (uint32_t pc = 0; pc < numPerms; ++pc) {
-        // permute (find_translation.cpp:109-125): out(o) = P(v) with v[p[a]] = o[a].
-        int v[3];
-        v[permAxis(pc, 0)] = vx;
-        v[permAxis(pc, 1)] = vy;
-        v[permAxis(pc, 2)] = vz;
-        const uint32_t src = (uint32_t)(v[0] + 4 * v[1] + 16 * v[2]);
-        const uint64_t Q = (uint64_t)__ballot((P >> src) & 1ull);
-#pragma unroll 1
-        for (uint32_t d0 = 0; d0 < 8; d0 += kDirGroup) {
-            uint64_t key[kDirGroup];
-            uint32_t bw[kDirGroup];
-#pragma unroll
-            for (uint32_t j = 0; j < kDirGroup; ++j)
-                key[j] = (mirror4(Q, (~(d0 + j)) & 7u) >> lane) & clip;
-#pragma unroll
-            for (uint32_t j = 0; j < kDirGroup; ++j)
-                bw[j] = key[j] ? bloom[bloomWord(bloomMix64(key[j]), logBloom)] : 0u;
-#pragma unroll
-            for (uint32_t j = 0; j < kDirGroup; ++j) {
-                const uint32_t d = d0 + j;
-                const uint32_t need = bloomBits(bloomMix64(key[j]));
-                cnt.probes += key[j] != 0;
-                if (key[j] == 0 || (bw[j] & need) != need)
-                    continue;
-                ++cnt.filterPass;
-                uint32_t s = homeSlot64(key[j], logCap);
-                uint64_t k = tabKeys[s];
-                while (k != 0) {
-                    if (k == key[j]) {
-                        ++cnt.keyHits;
-                        const uint32_t e = tabVals[s];
-                        const uint32_t c = e >> 3;
-                        const uint32_t kc = e & 7u;
-                        if (c > r && c != kDeadRank) {
-                            const uint32_t sym = d ^ kc;
-                            const int tx = (kc & 1u) ? -vx : vx;
-                            const int ty = (kc & 2u) ? -vy : vy;
-                            const int tz = (kc & 4u) ? -vz : vz;
-                            if (configAllows(cfgBits, pc, sym, tx, ty, tz)) {
-                                ++cnt.candidates;
-                                const uint64_t k64 = encodeKey(r, pc, sym, tx, ty, tz, 2);
-                                if (k64 < bestHint(&best[c])) {
-                                    ++cnt.verifications;
-                                    // Bitwise verification of the derived transform.
-                                    const uint64_t T = translate4(mirror4(Q, sym), tx, ty, tz);
-                                    if (T == sortedMasks[c]) {
-                                        ++cnt.verified;
-                                        atomicMin((unsigned long long*)&best[c], (unsigned long long)k64);
-                                    }
-                                }
-                            }
-                        }
-                    }
-                    s = (s + 1) & capMask;
-                    k = tabKeys[s];
-                }
-            }
-        }
-    }
-}
 
 // Orders the calling wave's LDS accesses without waiting for its global loads.
 __device__ __forceinline__ void waveSyncLds()
