@@ -81,7 +81,7 @@ uint32_t tilesFor(const MatchPlan& p, uint32_t parents)
 
 // Block-level tile of levels 4 and 5: all four waves search one unit of parent r.
 template <int L, int Kind>
-__device__ __forceinline__ void blockTile(const LevelView& v, uint32_t r, uint32_t unit, uint32_t chunk, unsigned char* s_mem, uint32_t* s_occ, uint16_t* queue,
+__device__ __forceinline__ void blockTile(const LevelView& v, uint32_t r, uint32_t unit, uint32_t chunk, unsigned char* s_mem, uint32_t* s_occ, uint32_t* qMeta, uint32_t* qHash,
     uint32_t cfgBits)
 {
     using G = Geo<L>;
@@ -93,13 +93,13 @@ __device__ __forceinline__ void blockTile(const LevelView& v, uint32_t r, uint32
     if (!loadRowsOcc<L, kBatchThreads>(v.sortedMasks, v.n, r, Kind == kUnitCombo ? (unit >> 3) >> 1 : unit >> 1, s_rows, s_occ, occ))
         return;
     ProbeCounts cnt;
-    searchUnit<L, kBatchWaves, Kind>(r, unit, chunk, (int)(threadIdx.x >> 6), s_rows, occ, planes, planes, queue, v.sortedMasks, v.n, v.tabKeys, v.logCap, v.bloom, v.logBloom, v.best,
+    searchUnit<L, kBatchWaves, Kind>(r, unit, chunk, (int)(threadIdx.x >> 6), s_rows, occ, planes, planes, qMeta, qHash, v.sortedMasks, v.n, v.tabKeys, v.logCap, v.bloom, v.logBloom, v.best,
         cfgBits, cnt);
     flushCounts(cnt, v.counters, cfgBits);
 }
 
 // Wave item of level 3: the calling wave alone searches family `unit` of parent r.
-__device__ __forceinline__ void waveItem3(const LevelView& v, uint32_t r, uint32_t unit, unsigned char* s_mem, uint16_t* queue, uint32_t cfgBits)
+__device__ __forceinline__ void waveItem3(const LevelView& v, uint32_t r, uint32_t unit, unsigned char* s_mem, uint32_t* qMeta, uint32_t* qHash, uint32_t cfgBits)
 {
     using G = Geo<3>;
     if (!(cfgBits & kCfgAxis) && (unit >> 1) != 0)
@@ -112,7 +112,7 @@ __device__ __forceinline__ void waveItem3(const LevelView& v, uint32_t r, uint32
     if (!loadRowsWave<3>(v.sortedMasks, v.n, r, unit >> 1, s_rows, occ))
         return;
     ProbeCounts cnt;
-    searchUnit<3, 1, kUnitShared>(r, unit, 0, 0, s_rows, occ, planes0, planes1, queue, v.sortedMasks, v.n, v.tabKeys, v.logCap, v.bloom, v.logBloom, v.best, cfgBits, cnt);
+    searchUnit<3, 1, kUnitShared>(r, unit, 0, 0, s_rows, occ, planes0, planes1, qMeta, qHash, v.sortedMasks, v.n, v.tabKeys, v.logCap, v.bloom, v.logBloom, v.best, cfgBits, cnt);
     flushCounts(cnt, v.counters, cfgBits);
 }
 
@@ -170,7 +170,8 @@ __global__ __launch_bounds__(kBatchThreads, 3) void kBatched(const BatchDesc* __
     __shared__ __align__(16) unsigned char s_mem[kBatchLds];
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_occ[3 * kBatchWaves];
-    __shared__ uint16_t s_queue[kBatchWaves][kProbeQueue];
+    __shared__ uint32_t s_qMeta[kBatchWaves][kProbeQueue];
+    __shared__ uint32_t s_qHash[kBatchWaves][kProbeQueue];
     const uint32_t numLevels = desc->numLevels;
     const uint32_t numPhases = desc->numPhases;
     const uint32_t cfgBits = desc->cfgBits;
@@ -203,19 +204,19 @@ __global__ __launch_bounds__(kBatchThreads, 3) void kBatched(const BatchDesc* __
                 case 5: {
                     const uint32_t per = searchUnits(kUnitCombo, lvCfg) * kL5Chunks;
                     const uint32_t m = k % per;
-                    blockTile<5, kUnitCombo>(v, pl.r0 + k / per, m / kL5Chunks, m % kL5Chunks, s_mem, s_occ, s_queue[wave], lvCfg);
+                    blockTile<5, kUnitCombo>(v, pl.r0 + k / per, m / kL5Chunks, m % kL5Chunks, s_mem, s_occ, s_qMeta[wave], s_qHash[wave], lvCfg);
                     break;
                 }
                 case 4: {
                     const uint32_t per = searchUnits(kUnitCombo, lvCfg);
-                    blockTile<4, kUnitCombo>(v, pl.r0 + k / per, k % per, 0, s_mem, s_occ, s_queue[wave], lvCfg);
+                    blockTile<4, kUnitCombo>(v, pl.r0 + k / per, k % per, 0, s_mem, s_occ, s_qMeta[wave], s_qHash[wave], lvCfg);
                     break;
                 }
                 case 3: {
                     const uint32_t per = searchUnits(kUnitShared, lvCfg);
                     const uint32_t item = k * kBatchWaves + wave;
                     if (item < (pl.r1 - pl.r0) * per)
-                        waveItem3(v, pl.r0 + item / per, item % per, s_mem, s_queue[wave], lvCfg);
+                        waveItem3(v, pl.r0 + item / per, item % per, s_mem, s_qMeta[wave], s_qHash[wave], lvCfg);
                     break;
                 }
                 default: {

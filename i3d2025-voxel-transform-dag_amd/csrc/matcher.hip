@@ -249,7 +249,8 @@ __global__ __launch_bounds__(64 * Waves) void kSearch(const uint64_t* __restrict
     __shared__ typename G::Row s_rows[G::Rows];
     __shared__ uint32_t s_planes[Buffers][Waves * G::P * G::Slab];
     __shared__ uint32_t s_occ[3 * Waves];
-    __shared__ uint16_t s_queue[Waves][kProbeQueue];
+    __shared__ uint32_t s_qMeta[Waves][kProbeQueue];
+    __shared__ uint32_t s_qHash[Waves][kProbeQueue];
     ProbeCounts cnt;
     const uint32_t Units = searchUnits(Kind, cfgBits);
     constexpr uint32_t Chunks = (uint32_t)(G::MaxWaves / Waves);
@@ -274,7 +275,7 @@ __global__ __launch_bounds__(64 * Waves) void kSearch(const uint64_t* __restrict
     Occupancy occ;
     if (!loadRowsOcc<L, 64 * Waves>(layouts, n, r, a0, s_rows, s_occ, occ))
         return;
-    searchUnit<L, Waves, Kind>(r, unit, chunk, (int)(threadIdx.x >> 6), s_rows, occ, s_planes[0], s_planes[Buffers - 1], s_queue[threadIdx.x >> 6], layouts,
+    searchUnit<L, Waves, Kind>(r, unit, chunk, (int)(threadIdx.x >> 6), s_rows, occ, s_planes[0], s_planes[Buffers - 1], s_qMeta[threadIdx.x >> 6], s_qHash[threadIdx.x >> 6], layouts,
         n, table, logCap, bloom, logBloom, best, cfgBits, cnt);
     flushCounts(cnt, counters, cfgBits);
 }

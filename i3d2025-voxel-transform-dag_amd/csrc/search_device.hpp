@@ -631,35 +631,44 @@ __device__ __forceinline__ bool verifyWave(const uint64_t* __restrict__ layouts,
     return __ballot(!ok) == 0ull;
 }
 
-// Entries of a wave's probe queue in LDS: voxel (w, u, c2) of the wave's planes as
-// (lane << 5) | c2 with lane = u + R*w.
-constexpr uint32_t kProbeQueue = 128;
+// A wave's list of filter passes in LDS, kept across the (permutation, direction) combos of a
+// search unit so the table is walked once per full list instead of once per combo. Entry i:
+// meta[i] = lane | c2 << 6 | d << 11 | pc << 14 for voxel (c0Base + lane / R, lane % R, c2) of
+// combo (pc, d), and hash[i] = its final nested hash (the planes are reused by the next combo).
+// `count` is uniform over the wave.
+constexpr uint32_t kProbeQueue = 64;
+struct ProbeList {
+    uint32_t* meta;
+    uint32_t* hash;
+    uint32_t count;
+};
 
-// Walks the child table for `count` queued voxels (their final hashes are in the wave's planes
-// fWave, plane w at fWave + w*Slab, voxel (u, c2) at u + Pitch*c2) and merges verified hits into
-// best. Lane j takes entry base + j, so the walks of a round run side by side whatever line the
-// voxels came from. Lanes stop at each candidate that beats best[c]; the wave then verifies the
-// stopped lanes' candidates together and resumes.
+// Walks the child table for the listed voxels and merges verified hits into best. Lane j takes
+// entry base + j, so the walks of a round run side by side whatever combo or line the voxels came
+// from. Lanes stop at each candidate that beats best[c]; the wave then verifies the stopped
+// lanes' candidates together and resumes.
 template <int L>
-__device__ __forceinline__ void probeQueue(const uint16_t* queue, uint32_t count, const uint32_t* fWave, int c0Base, const uint64_t* __restrict__ layouts,
-    uint32_t n, const uint64_t* __restrict__ table, uint32_t logCap, uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t r, uint32_t pc, uint32_t d,
-    ProbeCounts& cnt)
+__device__ __forceinline__ void probeQueue(const ProbeList& list, int c0Base, const uint64_t* __restrict__ layouts, uint32_t n,
+    const uint64_t* __restrict__ table, uint32_t logCap, uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t r, ProbeCounts& cnt)
 {
     using G = Geo<L>;
     constexpr int R = G::R;
     const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t count = list.count;
     const uint32_t capMask = (1u << logCap) - 1u;
     for (uint32_t base = 0; base < count; base += 64) {
         bool active = base + lane < count;
-        uint32_t h = 0, slot = 0;
+        uint32_t h = 0, slot = 0, pc = 0, d = 0;
         int c0 = 0, u = 0, c2 = 0;
         if (active) {
-            const uint32_t e = queue[base + lane];
-            const int src = (int)(e >> 5), w = src >> L;
-            c2 = (int)(e & 31u);
+            const uint32_t e = list.meta[base + lane];
+            const int src = (int)(e & 63u), w = src >> L;
+            c2 = (int)((e >> 6) & 31u);
+            d = (e >> 11) & 7u;
+            pc = (e >> 14) & 7u;
             u = src & (R - 1);
             c0 = c0Base + w;
-            h = fWave[w * G::Slab + u + G::Pitch * c2];
+            h = list.hash[base + lane];
             slot = homeSlot32(h, logCap);
         }
         while (true) {
@@ -734,50 +743,61 @@ __device__ __forceinline__ void probeQueue(const uint16_t* queue, uint32_t count
     }
 }
 
-// Moves the lanes' filter passes (bit c2 of `pending` = voxel (c0, u, c2)) into the wave's
-// queue, one voxel per lane and step, and hands the queue to probeQueue whenever the next step
-// might not fit and at the end.
+// Walks and empties the list (no-op when empty).
 template <int L>
-__device__ __forceinline__ void probePending(uint32_t pending, uint16_t* queue, const uint32_t* fWave, int c0Base, const uint64_t* __restrict__ layouts,
+__device__ __forceinline__ void flushProbes(ProbeList& list, int c0Base, const uint64_t* __restrict__ layouts, uint32_t n, const uint64_t* __restrict__ table,
+    uint32_t logCap, uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t r, ProbeCounts& cnt)
+{
+    if (list.count == 0)
+        return;
+    waveSync();
+    probeQueue<L>(list, c0Base, layouts, n, table, logCap, best, cfgBits, r, cnt);
+    waveSync();
+    list.count = 0;
+}
+
+// Appends the lanes' filter passes of combo (pc, d) (bit c2 of `pending` = voxel (c0, u, c2);
+// final hashes in the wave's planes fWave, plane w at fWave + w*Slab, voxel (u, c2) at
+// u + Pitch*c2) to the list, one voxel per lane and step, walking the list whenever the next step
+// might not fit.
+template <int L>
+__device__ __forceinline__ void probePending(uint32_t pending, ProbeList& list, const uint32_t* fWave, int c0Base, const uint64_t* __restrict__ layouts,
     uint32_t n, const uint64_t* __restrict__ table, uint32_t logCap, uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t r, uint32_t pc, uint32_t d,
     ProbeCounts& cnt)
 {
+    using G = Geo<L>;
     const uint32_t lane = threadIdx.x & 63u;
-    uint32_t count = 0;
+    const uint32_t u = lane & (G::R - 1), w = lane >> L;
     for (;;) {
         const uint64_t has = __ballot(pending != 0u);
         const uint32_t pushed = (uint32_t)__popcll(has);
-        if (pushed != 0 && count + pushed <= kProbeQueue) {
-            if (pending != 0u) {
-                const uint32_t pos = count + __builtin_amdgcn_mbcnt_hi((uint32_t)(has >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)has, 0u));
-                const uint32_t c2 = (uint32_t)__ffs(pending) - 1u;
-                pending &= pending - 1u;
-                queue[pos] = (uint16_t)((lane << 5) | c2);
-            }
-            count += pushed;
-            continue;
-        }
-        if (count == 0)
+        if (pushed == 0)
             break;
-        waveSync();
-        probeQueue<L>(queue, count, fWave, c0Base, layouts, n, table, logCap, best, cfgBits, r, pc, d, cnt);
-        waveSync();
-        count = 0;
+        if (list.count + pushed > kProbeQueue)
+            flushProbes<L>(list, c0Base, layouts, n, table, logCap, best, cfgBits, r, cnt);
+        if (pending != 0u) {
+            const uint32_t pos = list.count + __builtin_amdgcn_mbcnt_hi((uint32_t)(has >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)has, 0u));
+            const uint32_t c2 = (uint32_t)__ffs(pending) - 1u;
+            pending &= pending - 1u;
+            list.meta[pos] = lane | (c2 << 6) | (d << 11) | (pc << 14);
+            list.hash[pos] = fWave[w * G::Slab + u + G::Pitch * c2];
+        }
+        list.count += pushed;
     }
 }
 
 // Pass 3 in direction dz with the filter (pass-2 states read from `in`, final hashes written to
-// `out`, which is the lane's line of the wave's planes outWave), then the table walks of the
-// wave's filter passes.
+// `out`, which is the lane's line of the wave's planes outWave), then the wave's filter passes go
+// to its probe list.
 template <int L>
-__device__ __forceinline__ void lastPassProbe(uint32_t dz, int lead, const uint32_t* in, uint32_t* out, const uint32_t* outWave, uint16_t* queue, int c0Base,
+__device__ __forceinline__ void lastPassProbe(uint32_t dz, int lead, const uint32_t* in, uint32_t* out, const uint32_t* outWave, ProbeList& list, int c0Base,
     const uint64_t* __restrict__ layouts, uint32_t n, const uint64_t* __restrict__ table, uint32_t logCap, const uint32_t* __restrict__ bloom,
     uint32_t logBloom, uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t r, uint32_t pc, uint32_t d, ProbeCounts& cnt)
 {
     const uint32_t pending = dz ? pass3Filter<L, true>(in, out, bloom, logBloom, cfgBits, lead, cnt)
                                 : pass3Filter<L, false>(in, out, bloom, logBloom, cfgBits, lead, cnt);
     waveSync();
-    probePending<L>(pending, queue, outWave, c0Base, layouts, n, table, logCap, best, cfgBits, r, pc, d, cnt);
+    probePending<L>(pending, list, outWave, c0Base, layouts, n, table, logCap, best, cfgBits, r, pc, d, cnt);
 }
 
 // Search-unit kinds of the level-3..5 kernel.
@@ -798,17 +818,18 @@ template <int L, int Waves, int Kind>
 constexpr size_t searchLds()
 {
     return Geo<L>::Rows * sizeof(typename Geo<L>::Row) + (Kind == kUnitShared ? 2 : 1) * (size_t)Waves * Geo<L>::P * Geo<L>::Slab * 4 + 3 * Waves * 4 +
-        (size_t)Waves * kProbeQueue * 2;
+        (size_t)Waves * kProbeQueue * 8;
 }
 
 // Search of one unit of parent rank r (rows already in s_rows, occupancy occ) by `Waves` waves:
 // `wave` is the caller's wave index within them and covers planes ((chunk * Waves + wave) * P + w)
-// for w < P. planes0 (and planes1 for kUnitShared) hold Waves * P * Slab words each. Scan steps
-// over empty rows are skipped (emptyLead), and so is the whole unit for a wave whose planes hold
-// no voxel.
+// for w < P. planes0 (and planes1 for kUnitShared) hold Waves * P * Slab words each; qMeta and
+// qHash are the wave's probe list (kProbeQueue entries each), walked when full and once at the
+// end of the unit. Scan steps over empty rows are skipped (emptyLead), and so is the whole unit
+// for a wave whose planes hold no voxel.
 template <int L, int Waves, int Kind>
 __device__ __forceinline__ void searchUnit(uint32_t r, uint32_t unit, uint32_t chunk, int wave, const typename Geo<L>::Row* s_rows, const Occupancy& occ,
-    uint32_t* planes0, uint32_t* planes1, uint16_t* queue, const uint64_t* __restrict__ layouts, uint32_t n, const uint64_t* __restrict__ table,
+    uint32_t* planes0, uint32_t* planes1, uint32_t* qMeta, uint32_t* qHash, const uint64_t* __restrict__ layouts, uint32_t n, const uint64_t* __restrict__ table,
     uint32_t logCap, const uint32_t* __restrict__ bloom, uint32_t logBloom, uint64_t* __restrict__ best, uint32_t cfgBits, ProbeCounts& cnt)
 {
     using G = Geo<L>;
@@ -832,11 +853,12 @@ __device__ __forceinline__ void searchUnit(uint32_t r, uint32_t unit, uint32_t c
     const uint32_t segment = dx ? rowPrefix<true>(occ.axis(p0), c0) : rowPrefix<false>(occ.axis(p0), 31 - c0);
     if (__ballot(segment != 0u) == 0ull)
         return;
+    ProbeList list { qMeta, qHash, 0u };
     if constexpr (Kind == kUnitCombo) {
         const uint32_t pc = unit >> 3, d = unit & 7u;
         passes12<L>(pc, d, s_rows, c0, u, plane, emptyLead<L>(occ.axis(permAxis(pc, 1)), (d >> 1) & 1u));
         waveSync();
-        lastPassProbe<L>((d >> 2) & 1u, emptyLead<L>(occ.axis(permAxis(pc, 2)), (d >> 2) & 1u), line, line, planes0 + wave * G::P * G::Slab, queue, c0Base,
+        lastPassProbe<L>((d >> 2) & 1u, emptyLead<L>(occ.axis(permAxis(pc, 2)), (d >> 2) & 1u), line, line, planes0 + wave * G::P * G::Slab, list, c0Base,
             layouts, n, table, logCap, bloom, logBloom, best, cfgBits, r, pc, d, cnt);
     } else {
 #pragma unroll 1
@@ -861,12 +883,13 @@ __device__ __forceinline__ void searchUnit(uint32_t r, uint32_t unit, uint32_t c
                         passes12<L>(pc, d, s_rows, c0, u, plane, lead2);
                         waveSync();
                     }
-                    lastPassProbe<L>(dz, emptyLead<L>(occ2, dz), line, outLine, outWave, queue, c0Base, layouts, n, table, logCap, bloom, logBloom, best,
+                    lastPassProbe<L>(dz, emptyLead<L>(occ2, dz), line, outLine, outWave, list, c0Base, layouts, n, table, logCap, bloom, logBloom, best,
                         cfgBits, r, pc, d, cnt);
                 }
             }
         }
     }
+    flushProbes<L>(list, c0Base, layouts, n, table, logCap, best, cfgBits, r, cnt);
 }
 
 // Rank-batched search: after parents [0, r1) have been searched, every child whose best key is
