@@ -628,6 +628,13 @@ __device__ __forceinline__ uint64_t shflXor64(uint64_t v, int m)
     return ((uint64_t)hi << 32) | lo;
 }
 
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, uint32_t src)
+{
+    const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, (int)src, 64);
+    const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), (int)src, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+
 // Lane keys forming a bitonic sequence -> sorted descending across the 64 lanes.
 __device__ __forceinline__ uint64_t bitonicMergeDesc(uint64_t v, uint32_t lane)
 {
@@ -940,66 +947,128 @@ __global__ __launch_bounds__(256) void kTransforms48(const uint64_t* __restrict_
 // One thread per (query, proposal): the smallest key of a transform T with C == T(P_r) (perm,
 // then mirror, then the clipping translation), merged into best[q]. For each axis one side of
 // the translated grid is not clipped, so t_a is C.lo_a - M.lo_a or C.hi_a - M.hi_a.
+// The smallest key of the clipping translations t with C == translate(M, t), M = the image of
+// P_r under permutation pc then mirror sym (box mb of M, clo/chi of C), or ~0. For each axis one
+// side of the translated grid is not clipped, so t_a is C.lo_a - M.lo_a or C.hi_a - M.hi_a.
+__device__ __forceinline__ uint64_t keyForImage(uint64_t C, const int (&clo)[3], const int (&chi)[3], uint64_t M, uint32_t mb, uint32_t r, uint32_t pc,
+    uint32_t sym, uint32_t cfgBits)
+{
+    int cands[3][2];
+    int nc[3];
+    for (int a = 0; a < 3; ++a) {
+        cands[a][0] = clo[a] - (int)((mb >> (2 * a)) & 3u);
+        cands[a][1] = chi[a] - (int)((mb >> (6 + 2 * a)) & 3u);
+        nc[a] = cands[a][1] != cands[a][0] ? 2 : 1;
+    }
+    uint64_t bestKey = ~0ull;
+    for (int ix = 0; ix < nc[0]; ++ix)
+        for (int iy = 0; iy < nc[1]; ++iy)
+            for (int iz = 0; iz < nc[2]; ++iz) {
+                const int tx = cands[0][ix], ty = cands[1][iy], tz = cands[2][iz];
+                if (!configAllows(cfgBits, pc, sym, tx, ty, tz))
+                    continue;
+                if (translate4(M, tx, ty, tz) != C)
+                    continue;
+                const uint64_t key = encodeKey(r, pc, sym, tx, ty, tz, 2);
+                bestKey = key < bestKey ? key : bestKey;
+            }
+    return bestKey;
+}
+
+__device__ __forceinline__ void boxOf(uint32_t cb, int (&lo)[3], int (&hi)[3])
+{
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = (int)((cb >> (2 * a)) & 3u);
+        hi[a] = (int)((cb >> (6 + 2 * a)) & 3u);
+    }
+}
+
 // The smallest key of a transform T with C_q == T(P_r) (perm, then mirror, then the clipping
-// translation), or ~0 if there is none. For each axis one side of the translated grid is not
-// clipped, so t_a is C.lo_a - M.lo_a or C.hi_a - M.hi_a.
+// translation), or ~0 if there is none.
 __device__ uint64_t verifyPair(const uint64_t* __restrict__ sortedMasks, const uint64_t* __restrict__ images, const uint16_t* __restrict__ boxes, uint32_t q,
     uint32_t r, uint32_t cfgBits)
 {
     const uint64_t C = sortedMasks[q], P = sortedMasks[r];
     if (C == 0 || P == 0 || __popcll(P) < __popcll(C))
         return ~0ull;
-    const uint32_t cb = boxes[(size_t)q * 48]; // identity image = C itself
-    const int clo[3] = { (int)(cb & 3u), (int)((cb >> 2) & 3u), (int)((cb >> 4) & 3u) };
-    const int chi[3] = { (int)((cb >> 6) & 3u), (int)((cb >> 8) & 3u), (int)((cb >> 10) & 3u) };
+    int clo[3], chi[3];
+    boxOf(boxes[(size_t)q * 48], clo, chi); // identity image = C itself
     uint64_t bestKey = ~0ull;
     const uint32_t numPerms = (cfgBits & kCfgAxis) ? 6u : 1u;
     const uint32_t numSyms = (cfgBits & kCfgSymmetry) ? 8u : 1u;
     const uint64_t* img = images + (size_t)r * 48;
     const uint16_t* box = boxes + (size_t)r * 48;
-    for (uint32_t pc = 0; pc < numPerms; ++pc) {
+    for (uint32_t pc = 0; pc < numPerms; ++pc)
         for (uint32_t sym = 0; sym < numSyms; ++sym) {
-            const uint64_t M = img[pc * 8 + sym];
-            const uint32_t mb = box[pc * 8 + sym];
-            int cands[3][2];
-            int nc[3];
-            for (int a = 0; a < 3; ++a) {
-                cands[a][0] = clo[a] - (int)((mb >> (2 * a)) & 3u);
-                cands[a][1] = chi[a] - (int)((mb >> (6 + 2 * a)) & 3u);
-                nc[a] = cands[a][1] != cands[a][0] ? 2 : 1;
-            }
-            for (int ix = 0; ix < nc[0]; ++ix)
-                for (int iy = 0; iy < nc[1]; ++iy)
-                    for (int iz = 0; iz < nc[2]; ++iz) {
-                        const int tx = cands[0][ix], ty = cands[1][iy], tz = cands[2][iz];
-                        if (!configAllows(cfgBits, pc, sym, tx, ty, tz))
-                            continue;
-                        if (translate4(M, tx, ty, tz) != C)
-                            continue;
-                        const uint64_t key = encodeKey(r, pc, sym, tx, ty, tz, 2);
-                        bestKey = key < bestKey ? key : bestKey;
-                    }
+            const uint64_t key = keyForImage(C, clo, chi, img[pc * 8 + sym], box[pc * 8 + sym], r, pc, sym, cfgBits);
+            bestKey = key < bestKey ? key : bestKey;
         }
-    }
     return bestKey;
 }
 
-// One thread per (query, proposal): verifyPair merged into best[q].
+// One wave per query q. The usable proposals (earlier ranks with at least q's voxel count) are
+// sorted by rank, ascending, across the lanes; a key's leading field is the parent rank, so the
+// smallest verified key comes from the first proposal in that order that verifies. They are
+// checked four at a time: lane l handles proposal l / 16 of the group and transforms
+// 3 (l % 16) .. 3 (l % 16) + 2 (index pc * 8 + sym) of it, so a proposal's 48 images are one
+// coalesced read, and the loop stops after the first group with a verified key. That key is
+// merged into best[q]; `hits` counts the queries whose key moved off the self-match here.
 __global__ __launch_bounds__(256) void kVerifyProposals(const uint64_t* __restrict__ sortedMasks, const uint64_t* __restrict__ images, const uint16_t* __restrict__ boxes,
     uint32_t n, uint32_t k, const uint32_t* __restrict__ cand, uint32_t cfgBits, uint64_t* __restrict__ best, unsigned long long* __restrict__ hits)
 {
-    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= (size_t)n * k)
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t q = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (q >= n)
         return;
-    const uint32_t q = (uint32_t)(i / k);
-    const uint32_t r = cand[i];
-    if (r == 0xFFFFFFFFu || r >= q)
+    const uint64_t C = sortedMasks[q];
+    if (C == 0)
         return;
-    const uint64_t bestKey = verifyPair(sortedMasks, images, boxes, q, r, cfgBits);
-    if (bestKey != ~0ull) {
+    int clo[3], chi[3];
+    boxOf(boxes[(size_t)q * 48], clo, chi);
+    const int popC = __popcll(C);
+    // Lane s < k loads proposal s; usable ones enter the sort as 0xFFFFFFFF - rank (others as 0),
+    // so after a descending sort lane i holds the i-th smallest usable rank.
+    uint64_t sortKey = 0ull;
+    if (lane < k) {
+        const uint32_t r = cand[(size_t)q * k + lane];
+        if (r != 0xFFFFFFFFu && r < q) {
+            const uint64_t P = sortedMasks[r];
+            if (P != 0 && __popcll(P) >= popC)
+                sortKey = 0xFFFFFFFFull - r;
+        }
+    }
+    sortKey = bitonicSortDesc(sortKey, lane);
+    const uint32_t usable = (uint32_t)__popcll(__ballot(sortKey != 0ull));
+    const uint32_t numPerms = (cfgBits & kCfgAxis) ? 6u : 1u;
+    const uint32_t numSyms = (cfgBits & kCfgSymmetry) ? 8u : 1u;
+    const uint32_t slot = lane >> 4, sub = lane & 15u;
+    uint64_t bestKey = ~0ull;
+    for (uint32_t g = 0; 4 * g < usable; ++g) {
+        const uint32_t s = 4 * g + slot;
+        const uint64_t sk = shfl64(sortKey, s & 63u); // lane s's key
+        if (s < usable) {
+            const uint32_t r = 0xFFFFFFFFu - (uint32_t)sk;
+#pragma unroll
+            for (uint32_t i = 0; i < 3; ++i) {
+                const uint32_t t = sub * 3 + i, pc = t >> 3, sym = t & 7u;
+                if (pc >= numPerms || sym >= numSyms)
+                    continue;
+                const uint64_t key = keyForImage(C, clo, chi, images[(size_t)r * 48 + t], boxes[(size_t)r * 48 + t], r, pc, sym, cfgBits);
+                bestKey = key < bestKey ? key : bestKey;
+            }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const uint64_t o = shflXor64(bestKey, off);
+            bestKey = o < bestKey ? o : bestKey;
+        }
+        if (bestKey != ~0ull)
+            break; // later groups hold higher ranks only
+    }
+    if (lane == 0 && bestKey != ~0ull) {
         const unsigned long long prev = atomicMin((unsigned long long*)&best[q], (unsigned long long)bestKey);
         if (prev == (unsigned long long)selfKey(q, 2))
-            atomicAdd(hits, 1ull); // first verified proposal of this query
+            atomicAdd(hits, 1ull); // the query's first verified proposal
     }
 }
 
@@ -1297,7 +1366,7 @@ hipError_t matchLearned2(const MatchPlan& p, Mlp* mlp, uint32_t k, const uint64_
         (void)hipEventRecord(ev[2], s);
         // 2. Bitwise verification of every proposal against the 48 precomputed images.
         hipLaunchKernelGGL(kTransforms48, dim3((unsigned)cdiv((size_t)n * 48, 256)), dim3(256), 0, s, v.sortedMasks, n, images, boxes);
-        hipLaunchKernelGGL(kVerifyProposals, dim3((unsigned)cdiv((size_t)n * k, 256)), dim3(256), 0, s, v.sortedMasks, images, boxes, n, k, cand, cfgBits, v.best, cnt);
+        hipLaunchKernelGGL(kVerifyProposals, dim3((unsigned)cdiv(n, 4)), dim3(256), 0, s, v.sortedMasks, images, boxes, n, k, cand, cfgBits, v.best, cnt);
         hipLaunchKernelGGL(kFallbackFlags, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, s, v.sortedMasks, v.best, n, flags, cnt + 1);
         if ((e = hipGetLastError()) != hipSuccess)
             return cleanup(e);
