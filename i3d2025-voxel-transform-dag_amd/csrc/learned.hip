@@ -983,29 +983,6 @@ __device__ __forceinline__ void boxOf(uint32_t cb, int (&lo)[3], int (&hi)[3])
     }
 }
 
-// The smallest key of a transform T with C_q == T(P_r) (perm, then mirror, then the clipping
-// translation), or ~0 if there is none.
-__device__ uint64_t verifyPair(const uint64_t* __restrict__ sortedMasks, const uint64_t* __restrict__ images, const uint16_t* __restrict__ boxes, uint32_t q,
-    uint32_t r, uint32_t cfgBits)
-{
-    const uint64_t C = sortedMasks[q], P = sortedMasks[r];
-    if (C == 0 || P == 0 || __popcll(P) < __popcll(C))
-        return ~0ull;
-    int clo[3], chi[3];
-    boxOf(boxes[(size_t)q * 48], clo, chi); // identity image = C itself
-    uint64_t bestKey = ~0ull;
-    const uint32_t numPerms = (cfgBits & kCfgAxis) ? 6u : 1u;
-    const uint32_t numSyms = (cfgBits & kCfgSymmetry) ? 8u : 1u;
-    const uint64_t* img = images + (size_t)r * 48;
-    const uint16_t* box = boxes + (size_t)r * 48;
-    for (uint32_t pc = 0; pc < numPerms; ++pc)
-        for (uint32_t sym = 0; sym < numSyms; ++sym) {
-            const uint64_t key = keyForImage(C, clo, chi, img[pc * 8 + sym], box[pc * 8 + sym], r, pc, sym, cfgBits);
-            bestKey = key < bestKey ? key : bestKey;
-        }
-    return bestKey;
-}
-
 // One wave per query q. The usable proposals (earlier ranks with at least q's voxel count) are
 // sorted by rank, ascending, across the lanes; a key's leading field is the parent rank, so the
 // smallest verified key comes from the first proposal in that order that verifies. They are
@@ -1077,26 +1054,52 @@ __global__ __launch_bounds__(256) void kVerifyProposals(const uint64_t* __restri
 // Accepting the smaller verified key moves q's parent down the chain as the exhaustive minimum
 // would, so fewer leaves stay parents. Keys are read from `prev`, a snapshot of best[] taken
 // before the pass, so the outcome does not depend on thread order; every accepted key is
-// verified bitwise.
+// verified bitwise. Four queries per wave: lane l takes query l / 16 of the wave and transforms
+// 3 (l % 16) .. 3 (l % 16) + 2 of its candidate p'.
 __global__ __launch_bounds__(256) void kChaseParents(const uint64_t* __restrict__ sortedMasks, const uint64_t* __restrict__ images, const uint16_t* __restrict__ boxes,
     uint32_t n, uint32_t cfgBits, const uint64_t* __restrict__ prev, uint64_t* __restrict__ best)
 {
-    const uint32_t q = blockIdx.x * 256u + threadIdx.x;
-    if (q >= n)
-        return;
-    const uint64_t kq = prev[q];
-    if (kq == sentinelKey(2) || kq == selfKey(q, 2))
-        return;
-    const uint32_t p = (uint32_t)keyPtr(kq, 2);
-    const uint64_t kp = prev[p];
-    if (kp == sentinelKey(2))
-        return;
-    const uint32_t pp = (uint32_t)keyPtr(kp, 2);
-    if (pp >= p)
-        return;
-    const uint64_t key = verifyPair(sortedMasks, images, boxes, q, pp, cfgBits);
-    if (key < kq)
-        best[q] = key; // only this thread writes best[q] during the pass
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t q = (blockIdx.x * 4u + (threadIdx.x >> 6)) * 4u + (lane >> 4);
+    const uint32_t sub = lane & 15u;
+    uint64_t kq = ~0ull, key = ~0ull;
+    uint32_t pp = 0xFFFFFFFFu;
+    if (q < n) {
+        kq = prev[q];
+        if (kq != sentinelKey(2) && kq != selfKey(q, 2)) {
+            const uint32_t p = (uint32_t)keyPtr(kq, 2);
+            const uint64_t kp = prev[p];
+            if (kp != sentinelKey(2)) {
+                const uint32_t cand = (uint32_t)keyPtr(kp, 2);
+                if (cand < p)
+                    pp = cand;
+            }
+        }
+    }
+    if (pp != 0xFFFFFFFFu) {
+        const uint64_t C = sortedMasks[q], P = sortedMasks[pp];
+        if (C != 0 && P != 0 && __popcll(P) >= __popcll(C)) {
+            int clo[3], chi[3];
+            boxOf(boxes[(size_t)q * 48], clo, chi);
+            const uint32_t numPerms = (cfgBits & kCfgAxis) ? 6u : 1u;
+            const uint32_t numSyms = (cfgBits & kCfgSymmetry) ? 8u : 1u;
+#pragma unroll
+            for (uint32_t i = 0; i < 3; ++i) {
+                const uint32_t t = sub * 3 + i, pc = t >> 3, sym = t & 7u;
+                if (pc >= numPerms || sym >= numSyms)
+                    continue;
+                const uint64_t kk = keyForImage(C, clo, chi, images[(size_t)pp * 48 + t], boxes[(size_t)pp * 48 + t], pp, pc, sym, cfgBits);
+                key = kk < key ? kk : key;
+            }
+        }
+    }
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) { // minimum over the query's 16 lanes
+        const uint64_t o = shflXor64(key, off);
+        key = o < key ? o : key;
+    }
+    if (sub == 0 && q < n && key < kq)
+        best[q] = key; // only this lane writes best[q] during the pass
 }
 
 // Fallback flags: non-empty queries whose best key is still the self-match.
@@ -1380,7 +1383,7 @@ hipError_t matchLearned2(const MatchPlan& p, Mlp* mlp, uint32_t k, const uint64_
         for (int pass = 0; pass < kChasePasses; ++pass) {
             if ((e = hipMemcpyAsync(prevBest, v.best, (size_t)n * 8, hipMemcpyDeviceToDevice, s)) != hipSuccess)
                 return cleanup(e);
-            hipLaunchKernelGGL(kChaseParents, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, s, v.sortedMasks, images, boxes, n, cfgBits, prevBest, v.best);
+            hipLaunchKernelGGL(kChaseParents, dim3((unsigned)cdiv(n, 16)), dim3(256), 0, s, v.sortedMasks, images, boxes, n, cfgBits, prevBest, v.best);
             if ((e = hipGetLastError()) != hipSuccess)
                 return cleanup(e);
         }
