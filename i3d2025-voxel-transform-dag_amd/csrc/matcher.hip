@@ -131,31 +131,56 @@ __global__ __launch_bounds__(256, 8) void kSearch2(const uint64_t* __restrict__ 
 // ---------------------------------------------------------------------------------------
 // Levels 3..5: 32-bit nested sub-box hashes computed in LDS, one plane per wave.
 // ---------------------------------------------------------------------------------------
-// Transposed bit layouts 1 and 2 of every item (layout 0 is the sorted masks themselves).
+// In-register transpose of an R x R bit matrix (bit c of a[i] = M[i][c]; afterwards bit c of
+// a[i] = M[c][i]): log2(R) rounds, each exchanging the off-diagonal j x j blocks of every
+// 2j x 2j block with a shift, an XOR and a mask.
+template <int R>
+__device__ __forceinline__ void transposeBits(uint32_t (&a)[R])
+{
+    constexpr uint32_t kMasks[5] = { 0x55555555u, 0x33333333u, 0x0F0F0F0Fu, 0x00FF00FFu, 0x0000FFFFu };
+#pragma unroll
+    for (int lj = 4; lj >= 0; --lj) {
+        const int j = 1 << lj;
+        if (j >= R)
+            continue;
+        const uint32_t m = kMasks[lj];
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            if (i & j)
+                continue;
+            const uint32_t t = ((a[i] >> j) ^ a[i + j]) & m;
+            a[i] ^= t << j;
+            a[i + j] ^= t;
+        }
+    }
+}
+
+// Transposed bit layouts 1 and 2 of every item (layout 0 is the sorted masks themselves). One
+// thread per (item, layout, slab): layout 1's rows at z = slab are the transpose of layout 0's
+// rows (y, z = slab) for y < R; layout 2's rows at y = slab are the transpose of layout 0's rows
+// (y = slab, z) for z < R.
 template <int L>
 __global__ __launch_bounds__(256) void kLayouts(uint64_t* __restrict__ layouts, uint32_t n)
 {
-    constexpr int R = Geo<L>::R;
-    constexpr int W = Geo<L>::W;
-    __shared__ uint64_t s_bits[W];
-    const uint32_t r = blockIdx.x;
-    for (int i = (int)threadIdx.x; i < W; i += 256)
-        s_bits[i] = layouts[(size_t)r * W + i];
-    __syncthreads();
-    for (int i = (int)threadIdx.x; i < 2 * W; i += 256) {
-        const int lay = 1 + i / W, wi = i % W;
-        uint64_t out = 0;
-        for (int j = 0; j < 64; ++j) {
-            const uint32_t k = (uint32_t)(wi * 64 + j);
-            const int f = (int)(k & (R - 1)), a = (int)((k >> L) & (R - 1)), b = (int)(k >> (2 * L));
-            // layout 1: (y, x, z) = (f, a, b); layout 2: (z, x, y) = (f, a, b)
-            const int x = a;
-            const int y = lay == 1 ? f : b;
-            const int z = lay == 1 ? b : f;
-            out |= (uint64_t)lbit<L>(s_bits, x, y, z) << j;
-        }
-        layouts[((size_t)lay * n + r) * W + wi] = out;
-    }
+    using G = Geo<L>;
+    using Row = typename G::Row;
+    constexpr int R = G::R;
+    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (size_t)n * 2 * R)
+        return;
+    const uint32_t r = (uint32_t)(t / (2 * R));
+    const int lay = 1 + (int)((t / R) & 1u);
+    const int slab = (int)(t % R);
+    const Row* src = reinterpret_cast<const Row*>(layouts) + (size_t)r * G::R2;
+    uint32_t a[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+        a[i] = (uint32_t)src[lay == 1 ? i + R * slab : slab + R * i];
+    transposeBits<R>(a);
+    Row* dst = reinterpret_cast<Row*>(layouts) + ((size_t)lay * n + r) * G::R2 + (size_t)R * slab;
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+        dst[i] = (Row)a[i];
 }
 
 // One wave per (item, frame): the hash of the whole grid in the pass frame of permutation code
@@ -598,15 +623,15 @@ hipError_t enqueueBuildTable(const MatchPlan& p, void* d_scratch, hipStream_t st
             p.logBloom, childFlags);
         break;
     case 3:
-        hipLaunchKernelGGL(kLayouts<3>, dim3(n), dim3(256), 0, stream, v.sortedMasks, n);
+        hipLaunchKernelGGL(kLayouts<3>, dim3((unsigned)ceilDiv((size_t)n * 2 * 8, 256)), dim3(256), 0, stream, v.sortedMasks, n);
         hipLaunchKernelGGL(kCornerTable<3>, dim3(n, p.childFrames), dim3(64), 0, stream, v.sortedMasks, n, v.tabKeys, p.logCap, v.bloom, p.logBloom);
         break;
     case 4:
-        hipLaunchKernelGGL(kLayouts<4>, dim3(n), dim3(256), 0, stream, v.sortedMasks, n);
+        hipLaunchKernelGGL(kLayouts<4>, dim3((unsigned)ceilDiv((size_t)n * 2 * 16, 256)), dim3(256), 0, stream, v.sortedMasks, n);
         hipLaunchKernelGGL(kCornerTable<4>, dim3(n, p.childFrames), dim3(64), 0, stream, v.sortedMasks, n, v.tabKeys, p.logCap, v.bloom, p.logBloom);
         break;
     case 5:
-        hipLaunchKernelGGL(kLayouts<5>, dim3(n), dim3(256), 0, stream, v.sortedMasks, n);
+        hipLaunchKernelGGL(kLayouts<5>, dim3((unsigned)ceilDiv((size_t)n * 2 * 32, 256)), dim3(256), 0, stream, v.sortedMasks, n);
         hipLaunchKernelGGL(kCornerTable<5>, dim3(n, p.childFrames), dim3(64), 0, stream, v.sortedMasks, n, v.tabKeys, p.logCap, v.bloom, p.logBloom);
         break;
     default:
