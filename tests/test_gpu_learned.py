@@ -255,6 +255,23 @@ def test_learned_matcher_exact(oracle, gpu_ctx, use_mlp):
         mlp.close()
 
 
+def test_learned_matcher_deterministic(oracle, gpu_ctx):
+    """Repeated calls give bit-identical keys: retrieval ties break by rank, verification keeps the
+    smallest key of the lowest-ranked verified group, and the parent chase reads a snapshot of
+    the keys in each pass, so no result depends on thread timing. Large enough that the retrieval
+    splits query blocks into several chunks that share thresholds."""
+    from tdag_amd import learned
+
+    masks, enc, q, p = _planted_pairs(oracle, 8, 3000, 6000)
+    mlp = learned.MLP(gpu_ctx, seed=9)
+    try:
+        runs = [_run_learned(gpu_ctx, mlp, masks)[0] for _ in range(3)]
+    finally:
+        mlp.close()
+    assert np.array_equal(runs[0], runs[1]) and np.array_equal(runs[0], runs[2])
+    _check_learned_result(oracle, masks, runs[0], enc)
+
+
 def _fixed_popcount_set(oracle, seed, n_base, n_copies, pop=10):
     """Leaves that all have `pop` voxels: random bases plus copies under a random mirror/axis
     permutation and a translation that clips nothing. Popcount buckets are then useless for
