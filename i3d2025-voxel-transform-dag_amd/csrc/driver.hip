@@ -22,6 +22,7 @@ struct tdag_svdag {
 struct tdag_tdag {
     tdag::TDag t;
     std::vector<tdag::LevelRecord> records;
+    bool isStatic = false; // built by constructStaticTransformDAG: read back in the packed form
 };
 
 namespace {
@@ -120,8 +121,13 @@ tdag_status tdag_svdag_free(tdag_svdag* h)
     return TDAG_OK;
 }
 
+// Both drivers: constructTransformDAG (transform_dag.cpp:138-240) and, with staticForm,
+// constructStaticTransformDAG (transform_dag.cpp:242-390). The static driver makes the same
+// matcher calls on the same items: its hierarchical pass keeps the same nodes (only the order of
+// levels >= 4 differs) and it re-reads the same referenced nodes level by level in ascending
+// order; its packed output is produced from this node-index result on demand.
 static tdag_status constructTransformDag(tdag_context* ctx, const tdag_svdag* in, const tdag_config* cfg, uint32_t flags, bool learned, tdag_mlp* mlp, uint32_t k,
-    tdag_tdag** out)
+    bool staticForm, tdag_tdag** out)
 {
     if (!in || !cfg || !out)
         return setErr(TDAG_EINVAL, "null argument");
@@ -137,8 +143,9 @@ static tdag_status constructTransformDag(tdag_context* ctx, const tdag_svdag* in
         return setErr(TDAG_EINVAL, "max_translation_level + 1 exceeds the root level");
     const int threads = hostThreads();
     tdag_tdag* res = new tdag_tdag();
+    res->isStatic = staticForm;
     std::vector<uint32_t> translationParents;
-    res->t = tdag::hierarchicalDedup(dag, maxT + 1, translationParents, *cfg, threads);
+    res->t = tdag::hierarchicalDedup(dag, maxT + 1, translationParents, *cfg, threads, staticForm);
     if (cfg->translation) {
         tdag::TDag& t = res->t;
         t.leaves.clear();
@@ -224,7 +231,49 @@ static tdag_status constructTransformDag(tdag_context* ctx, const tdag_svdag* in
 
 tdag_status tdag_construct_transform_dag(tdag_context* ctx, const tdag_svdag* in, const tdag_config* cfg, uint32_t flags, tdag_tdag** out)
 {
-    return constructTransformDag(ctx, in, cfg, flags, false, nullptr, 0, out);
+    return constructTransformDag(ctx, in, cfg, flags, false, nullptr, 0, false, out);
+}
+
+tdag_status tdag_svdag_static_words(const tdag_svdag* h, uint32_t level, void* out, uint64_t cap, uint64_t* out_count)
+{
+    if (!h || !out_count)
+        return setErr(TDAG_EINVAL, "null argument");
+    if (level < 2 || level > std::max(2u, h->s.rootLevel))
+        return setErr(TDAG_EINVAL, "level " + std::to_string(level) + " outside [2, root level]");
+    if (level == 2) {
+        *out_count = h->s.leaves.size();
+        if (out) {
+            if (cap < h->s.leaves.size())
+                return setErr(TDAG_EINVAL, "buffer too small");
+            std::memcpy(out, h->s.leaves.data(), h->s.leaves.size() * 8);
+        }
+        return TDAG_OK;
+    }
+    const std::vector<uint32_t> w = tdag::staticSvdagWords(h->s, level);
+    *out_count = w.size();
+    if (out) {
+        if (cap < w.size())
+            return setErr(TDAG_EINVAL, "buffer too small");
+        std::memcpy(out, w.data(), w.size() * 4);
+    }
+    return TDAG_OK;
+}
+
+tdag_status tdag_construct_static_transform_dag(tdag_context* ctx, const tdag_static_svdag_view* in, const tdag_config* cfg, uint32_t flags, tdag_tdag** out)
+{
+    if (!in || !cfg || !out)
+        return setErr(TDAG_EINVAL, "null argument");
+    if (in->root_level < 2 || in->root_level > 12 || in->resolution != (1u << in->root_level))
+        return setErr(TDAG_EINVAL, "resolution must be 2^root_level with root_level in [2, 12]");
+    if (in->root_level >= 3 && (!in->level_words || !in->level_word_counts))
+        return setErr(TDAG_EINVAL, "level_words / level_word_counts are null");
+    if (in->num_leaves && !in->leaves)
+        return setErr(TDAG_EINVAL, "leaves is null");
+    tdag_svdag dag;
+    std::string err;
+    if (!tdag::svdagFromStatic(in->resolution, in->root_level, in->level_words, in->level_word_counts, in->leaves, in->num_leaves, dag.s, err))
+        return setErr(TDAG_EINVAL, "static SVDAG: " + err);
+    return constructTransformDag(ctx, &dag, cfg, flags, false, nullptr, 0, true, out);
 }
 
 tdag_status tdag_construct_transform_dag_learned(tdag_context* ctx, const tdag_svdag* in, const tdag_config* cfg, uint32_t flags, tdag_mlp* mlp, uint32_t k,
@@ -234,14 +283,50 @@ tdag_status tdag_construct_transform_dag_learned(tdag_context* ctx, const tdag_s
         return setErr(TDAG_EINVAL, "the learned driver needs a context and a network");
     if (k == 0 || k > 64)
         return setErr(TDAG_EINVAL, "k must be in [1, 64]");
-    return constructTransformDag(ctx, in, cfg, flags, true, mlp, k, out);
+    return constructTransformDag(ctx, in, cfg, flags, true, mlp, k, false, out);
 }
 
 tdag_status tdag_tdag_serialize(const tdag_tdag* t, const tdag_config* cfg, uint8_t* buf, uint64_t cap, uint64_t* out_size)
 {
     if (!t || !cfg || !out_size)
         return setErr(TDAG_EINVAL, "null argument");
+    if (t->isStatic)
+        return setErr(TDAG_EINVAL, "a static-structure result is written by tdag_tdag_static_serialize");
     const std::vector<uint8_t> bytes = tdag::serializeTdag(t->t, *cfg);
+    *out_size = bytes.size();
+    if (buf) {
+        if (cap < bytes.size())
+            return setErr(TDAG_EINVAL, "buffer too small");
+        std::memcpy(buf, bytes.data(), bytes.size());
+    }
+    return TDAG_OK;
+}
+
+tdag_status tdag_tdag_static_words(const tdag_tdag* t, uint32_t level, uint64_t* out, uint64_t cap, uint64_t* out_count)
+{
+    if (!t || !out_count)
+        return setErr(TDAG_EINVAL, "null argument");
+    if (!t->isStatic)
+        return setErr(TDAG_EINVAL, "not a static-structure result (tdag_construct_static_transform_dag)");
+    if (level < 2 || level > std::max(2u, t->t.rootLevel))
+        return setErr(TDAG_EINVAL, "level " + std::to_string(level) + " outside [2, root level]");
+    const std::vector<uint64_t> w = tdag::staticTdagWords(t->t, level);
+    *out_count = w.size();
+    if (out) {
+        if (cap < w.size())
+            return setErr(TDAG_EINVAL, "buffer too small");
+        std::memcpy(out, w.data(), w.size() * 8);
+    }
+    return TDAG_OK;
+}
+
+tdag_status tdag_tdag_static_serialize(const tdag_tdag* t, uint8_t* buf, uint64_t cap, uint64_t* out_size)
+{
+    if (!t || !out_size)
+        return setErr(TDAG_EINVAL, "null argument");
+    if (!t->isStatic)
+        return setErr(TDAG_EINVAL, "not a static-structure result (tdag_construct_static_transform_dag)");
+    const std::vector<uint8_t> bytes = tdag::serializeStaticTdag(t->t);
     *out_size = bytes.size();
     if (buf) {
         if (cap < bytes.size())

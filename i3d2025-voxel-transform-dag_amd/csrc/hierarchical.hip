@@ -157,17 +157,21 @@ struct Mapping {
 };
 
 // Reduces one level. `kept` receives the input indices of kept nodes in output order and
-// `mapping[i]` the pointer of input node i.
+// `mapping[i]` the pointer of input node i. Nodes are grouped by a hash of their sorted child
+// pointers; with `hashOf`, a child pointer p enters that hash as hashOf[p] (the empty child as
+// itself), which is how the StaticStructure pass keys its groups by child word offsets.
 void reduceLevel(const std::vector<Node>& nodes, uint32_t allowedIds, const std::vector<uint32_t>& idList, int threads,
-    std::vector<uint32_t>& kept, std::vector<Mapping>& mapping)
+    std::vector<uint32_t>& kept, std::vector<Mapping>& mapping, const std::vector<uint64_t>* hashOf = nullptr)
 {
     const Tables& T = tables();
     const size_t n = nodes.size();
     std::vector<std::pair<uint64_t, uint32_t>> keys(n);
     for (size_t i = 0; i < n; ++i) {
         uint64_t ptrs[8];
-        for (int c = 0; c < 8; ++c)
-            ptrs[c] = nodes[i].c[c].ptr;
+        for (int c = 0; c < 8; ++c) {
+            const uint64_t p = nodes[i].c[c].ptr;
+            ptrs[c] = (hashOf && p != kSentinelPtr) ? (*hashOf)[p] : p;
+        }
         std::sort(ptrs, ptrs + 8);
         uint64_t seed = 0;
         for (uint64_t p : ptrs)
@@ -287,7 +291,7 @@ void canonical2x2x2(const std::vector<uint32_t>& idList, Child out[256])
 
 } // namespace
 
-TDag hierarchicalDedup(const Svdag& in, uint32_t midLevel, std::vector<uint32_t>& outMidInv, const tdag_config& cfg, int threads)
+TDag hierarchicalDedup(const Svdag& in, uint32_t midLevel, std::vector<uint32_t>& outMidInv, const tdag_config& cfg, int threads, bool staticOffsets)
 {
     std::vector<uint32_t> idList;
     for (uint32_t id = 0; id < 48; ++id) {
@@ -300,7 +304,9 @@ TDag hierarchicalDedup(const Svdag& in, uint32_t midLevel, std::vector<uint32_t>
     TDag out;
     out.resolution = in.resolution;
     out.rootLevel = in.rootLevel;
-    out.structureType = in.structureType;
+    // The reference's output structure is default-constructed (transform_dag_hierarchical.cpp:307)
+    // and no later step sets its type, so a .tdag records StructureType::Tree (structure.h:171).
+    out.structureType = 0;
     out.nodes.resize(in.nodes.size());
     auto storeMid = [&](size_t numOut, const std::vector<Mapping>& m) {
         // transform_dag_hierarchical.cpp:311-318: canonical (untransformed) pointers only.
@@ -335,7 +341,12 @@ TDag hierarchicalDedup(const Svdag& in, uint32_t midLevel, std::vector<uint32_t>
             }
         std::vector<uint32_t> kept;
         std::vector<Mapping> mapping;
-        reduceLevel(nodes, 0, idList, threads, kept, mapping);
+        // constructStaticTransformDAGHierarchical (transform_dag_hierarchical.cpp:691-704) points
+        // at inner children by their word offset in the output level below (leaf indices stay).
+        std::vector<uint64_t> offsets;
+        if (staticOffsets && level >= 4)
+            offsets = packedNodeOffsets(out.nodes[level - 1]);
+        reduceLevel(nodes, 0, idList, threads, kept, mapping, (staticOffsets && level >= 4) ? &offsets : nullptr);
         auto& outNodes = out.nodes[level];
         outNodes.resize(kept.size());
         for (size_t k = 0; k < kept.size(); ++k)

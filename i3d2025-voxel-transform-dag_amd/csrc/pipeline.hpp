@@ -27,7 +27,7 @@ struct Svdag {
 struct TDag {
     uint32_t resolution = 0;
     uint32_t rootLevel = 0;
-    int32_t structureType = 1;
+    int32_t structureType = 0; // StructureType::Tree, as the reference's default-constructed output
     std::vector<std::vector<std::array<uint64_t, 8>>> nodes; // TransformPointer bit patterns
     std::vector<uint64_t> leaves;
 };
@@ -53,9 +53,32 @@ Svdag svdagFromDense(const uint8_t* bits, uint32_t R, int threads);
 void flattenNodes(const Svdag& s, uint32_t level, const uint32_t* nodes, uint64_t n, uint64_t* out, int threads);
 
 // constructTransformDAGHierarchical with midLevel; fills outMidInv (transform_dag_hierarchical.cpp:304-523).
-TDag hierarchicalDedup(const Svdag& in, uint32_t midLevel, std::vector<uint32_t>& outMidInv, const tdag_config& cfg, int threads);
+// With staticOffsets, groups are keyed as constructStaticTransformDAGHierarchical keys them
+// (transform_dag_hierarchical.cpp:531-743: children by output word offset), which changes the
+// output order of levels >= 4 but not the set of kept nodes; the result stays in node-index form.
+TDag hierarchicalDedup(const Svdag& in, uint32_t midLevel, std::vector<uint32_t>& outMidInv, const tdag_config& cfg, int threads,
+    bool staticOffsets = false);
 
 // .tdag byte stream (voxelizer/src/main.cpp:147-150).
 std::vector<uint8_t> serializeTdag(const TDag& t, const tdag_config& cfg);
+
+// StaticStructure packing (structure.h:299-320): a node is its child-mask word followed by one
+// word per present child in child-index order. Word offset of every node of an 8-slot level
+// (absent children: 0xFFFFFFFF in an SVDAG level, the sentinel TransformPointer in a TDag level).
+std::vector<uint64_t> packedNodeOffsets(const std::vector<std::array<uint32_t, 8>>& nodes);
+std::vector<uint64_t> packedNodeOffsets(const std::vector<std::array<uint64_t, 8>>& nodes);
+
+// StaticStructure<void, uint32_t>(EditStructure) words of one level (structure.h:766-798).
+std::vector<uint32_t> staticSvdagWords(const Svdag& s, uint32_t level);
+
+// Reads a StaticStructure<void, uint32_t> into the node-index form; false with a message when a
+// level does not parse or a child word is not a node offset (leaf index) of the level below.
+bool svdagFromStatic(uint32_t resolution, uint32_t rootLevel, const uint32_t* const* levelWords, const uint64_t* levelWordCounts, const uint64_t* leaves,
+    uint64_t numLeaves, Svdag& out, std::string& err);
+
+// StaticStructure<void, TransformPointer> words of one level of a node-index TDag (children by
+// word offset, leaf index at level 3), and its writeTo byte stream (structure.h:888-895).
+std::vector<uint64_t> staticTdagWords(const TDag& t, uint32_t level);
+std::vector<uint8_t> serializeStaticTdag(const TDag& t);
 
 } // namespace tdag

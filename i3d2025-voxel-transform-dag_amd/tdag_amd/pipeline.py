@@ -5,6 +5,8 @@ Mirrors the reference's construction API:
 * :class:`Svdag` ~ ``voxcom::EditStructure<void, uint32_t>`` after ``toDAG`` (structure.h:159-211)
 * :func:`construct_transform_dag` ~ ``constructTransformDAG`` (transform_dag.h:161-162)
 * :meth:`TDag.tdag_bytes` ~ the voxelizer's ``.tdag`` output (voxelizer/src/main.cpp:144-151)
+* :class:`StaticSvdag` ~ ``voxcom::StaticStructure<void, uint32_t>`` (structure.h:299-320)
+* :func:`construct_static_transform_dag` ~ ``constructStaticTransformDAG`` (transform_dag.h:163)
 """
 from __future__ import annotations
 
@@ -19,6 +21,17 @@ _u32p = ctypes.POINTER(ctypes.c_uint32)
 _u8p = ctypes.POINTER(ctypes.c_uint8)
 
 KEEP_LEVEL_INPUTS = 1
+
+
+class StaticSvdagView(ctypes.Structure):
+    _fields_ = [
+        ("resolution", ctypes.c_uint32),
+        ("root_level", ctypes.c_uint32),
+        ("level_words", ctypes.POINTER(_u32p)),
+        ("level_word_counts", _u64p),
+        ("leaves", _u64p),
+        ("num_leaves", ctypes.c_uint64),
+    ]
 
 
 class LevelRecord(ctypes.Structure):
@@ -51,9 +64,14 @@ def bind(L):
     L.tdag_tdag_level_record.argtypes = [vp, ctypes.c_uint32, ctypes.POINTER(LevelRecord)]
     L.tdag_tdag_level_masks.argtypes = [vp, ctypes.c_uint32, _u64p, ctypes.c_uint64]
     L.tdag_tdag_free.argtypes = [vp]
+    L.tdag_svdag_static_words.argtypes = [vp, ctypes.c_uint32, vp, ctypes.c_uint64, _u64p]
+    L.tdag_construct_static_transform_dag.argtypes = [vp, ctypes.POINTER(StaticSvdagView), ctypes.POINTER(TdagConfig), ctypes.c_uint32, ctypes.POINTER(vp)]
+    L.tdag_tdag_static_words.argtypes = [vp, ctypes.c_uint32, _u64p, ctypes.c_uint64, _u64p]
+    L.tdag_tdag_static_serialize.argtypes = [vp, _u8p, ctypes.c_uint64, _u64p]
     for name in ("tdag_scene_dense", "tdag_svdag_from_dense", "tdag_svdag_info", "tdag_svdag_copy_level", "tdag_svdag_flatten",
                  "tdag_svdag_free", "tdag_construct_transform_dag", "tdag_tdag_serialize", "tdag_tdag_info",
-                 "tdag_tdag_level_record", "tdag_tdag_level_masks", "tdag_tdag_free"):
+                 "tdag_tdag_level_record", "tdag_tdag_level_masks", "tdag_tdag_free", "tdag_svdag_static_words",
+                 "tdag_construct_static_transform_dag", "tdag_tdag_static_words", "tdag_tdag_static_serialize"):
         getattr(L, name).restype = ctypes.c_int
 
 
@@ -102,6 +120,14 @@ class Svdag:
         check(lib().tdag_svdag_flatten(self._h, level, idx.ctypes.data_as(_u32p), idx.size, out.ctypes.data_as(_u64p)), "tdag_svdag_flatten")
         return out[: idx.size]
 
+    def static_words(self, level: int) -> np.ndarray:
+        """StaticStructure(EditStructure) packed words of `level` (uint32; level 2: uint64 leaves)."""
+        n = ctypes.c_uint64()
+        check(lib().tdag_svdag_static_words(self._h, level, None, 0, ctypes.byref(n)), "tdag_svdag_static_words")
+        out = np.zeros(max(1, n.value), dtype=np.uint64 if level == 2 else np.uint32)
+        check(lib().tdag_svdag_static_words(self._h, level, out.ctypes.data_as(ctypes.c_void_p), out.size, ctypes.byref(n)), "tdag_svdag_static_words")
+        return out[: n.value]
+
     def close(self):
         if self._h:
             lib().tdag_svdag_free(self._h)
@@ -133,6 +159,22 @@ class TDag:
         buf = np.zeros(int(size.value), dtype=np.uint8)
         check(lib().tdag_tdag_serialize(self._h, ctypes.byref(cfg), buf.ctypes.data_as(_u8p), buf.size, ctypes.byref(size)), "tdag_tdag_serialize")
         return buf.tobytes()
+
+    def static_words(self, level: int) -> np.ndarray:
+        """Packed StaticStructure<void, TransformPointer> words of `level` (level 2: leaves)."""
+        n = ctypes.c_uint64()
+        check(lib().tdag_tdag_static_words(self._h, level, None, 0, ctypes.byref(n)), "tdag_tdag_static_words")
+        out = np.zeros(max(1, n.value), dtype=np.uint64)
+        check(lib().tdag_tdag_static_words(self._h, level, out.ctypes.data_as(_u64p), out.size, ctypes.byref(n)), "tdag_tdag_static_words")
+        return out[: n.value]
+
+    def static_bytes(self) -> bytes:
+        """StaticStructure<void, TransformPointer>::writeTo bytes (structure.h:888-895)."""
+        size = ctypes.c_uint64()
+        check(lib().tdag_tdag_static_serialize(self._h, None, 0, ctypes.byref(size)), "tdag_tdag_static_serialize")
+        buf = np.zeros(max(1, int(size.value)), dtype=np.uint8)
+        check(lib().tdag_tdag_static_serialize(self._h, buf.ctypes.data_as(_u8p), buf.size, ctypes.byref(size)), "tdag_tdag_static_serialize")
+        return buf[: int(size.value)].tobytes()
 
     def record(self, level: int) -> LevelRecord | None:
         rec = LevelRecord()
@@ -176,4 +218,43 @@ def construct_transform_dag(ctx, svdag: Svdag, config: TransformDAGConfig, keep_
     check(lib().tdag_construct_transform_dag(ctx._ptr if ctx is not None else None, svdag.handle, ctypes.byref(cfg),
                                              KEEP_LEVEL_INPUTS if keep_level_inputs else 0, ctypes.byref(h)),
           "tdag_construct_transform_dag")
+    return TDag(h, config)
+
+
+class StaticSvdag:
+    """StaticStructure<void, uint32_t>: packed words per level (3..root) and the leaf masks."""
+
+    def __init__(self, resolution: int, root_level: int, level_words: dict, leaves):
+        self.resolution = int(resolution)
+        self.root_level = int(root_level)
+        self.level_words = {l: np.ascontiguousarray(level_words.get(l, np.zeros(0, np.uint32)), dtype=np.uint32) for l in range(3, self.root_level + 1)}
+        self.leaves = np.ascontiguousarray(leaves, dtype=np.uint64)
+
+    @classmethod
+    def from_svdag(cls, svdag: Svdag) -> "StaticSvdag":
+        """StaticStructure(const EditStructure&) (structure.h:766-798), done by the library."""
+        return cls(svdag.resolution, svdag.root_level, {l: svdag.static_words(l) for l in range(3, svdag.root_level + 1)}, svdag.static_words(2))
+
+    def view(self):
+        n = self.root_level + 1
+        ptrs = (_u32p * n)()
+        counts = np.zeros(n, dtype=np.uint64)
+        for l, w in self.level_words.items():
+            ptrs[l] = w.ctypes.data_as(_u32p)
+            counts[l] = w.size
+        v = StaticSvdagView(self.resolution, self.root_level, ctypes.cast(ptrs, ctypes.POINTER(_u32p)), counts.ctypes.data_as(_u64p),
+                            self.leaves.ctypes.data_as(_u64p), self.leaves.size)
+        return v, (ptrs, counts)  # keep the arrays alive while the view is used
+
+
+def construct_static_transform_dag(ctx, static: StaticSvdag, config: TransformDAGConfig, keep_level_inputs: bool = False) -> TDag:
+    """constructStaticTransformDAG on the GPU matcher (``ctx`` may be None when translation is
+    off). Read the result with :meth:`TDag.static_words` / :meth:`TDag.static_bytes`."""
+    cfg = config.to_c()
+    v, keep = static.view()
+    h = ctypes.c_void_p()
+    check(lib().tdag_construct_static_transform_dag(ctx._ptr if ctx is not None else None, ctypes.byref(v), ctypes.byref(cfg),
+                                                    KEEP_LEVEL_INPUTS if keep_level_inputs else 0, ctypes.byref(h)),
+          "tdag_construct_static_transform_dag")
+    del keep
     return TDag(h, config)

@@ -306,18 +306,61 @@ tdag_status tdag_svdag_free(tdag_svdag* svdag);
  * max_translation_level down to 2. max_translation_level must lie in [2, 5]. ctx may be NULL
  * when cfg->translation is 0 (no matcher call is made). */
 tdag_status tdag_construct_transform_dag(tdag_context* ctx, const tdag_svdag* in, const tdag_config* cfg, uint32_t flags, tdag_tdag** out);
-/* The .tdag byte stream (voxelizer/src/main.cpp:144-151): pass buf = NULL to query the size. */
 /* constructTransformDAG with the learned proposal at level 2 (mlp; k proposals per query) and the
  * exhaustive matcher at levels 3..maxT. Every match is exact; the output can differ from
  * tdag_construct_transform_dag where a query's minimum-key parent was not proposed. */
 tdag_status tdag_construct_transform_dag_learned(tdag_context* ctx, const tdag_svdag* in, const tdag_config* cfg, uint32_t flags, tdag_mlp* mlp, uint32_t k,
     tdag_tdag** out);
+/* The .tdag byte stream (voxelizer/src/main.cpp:144-151): pass buf = NULL to query the size. The
+ * header's structureType is 0 (Tree), as in the reference, whose output structure is
+ * default-constructed (transform_dag_hierarchical.cpp:307, structure.h:171). */
 tdag_status tdag_tdag_serialize(const tdag_tdag* tdag, const tdag_config* cfg, uint8_t* buf, uint64_t cap, uint64_t* out_size);
 tdag_status tdag_tdag_info(const tdag_tdag* tdag, uint32_t* root_level, uint64_t* level_sizes, uint32_t cap);
 tdag_status tdag_tdag_level_record(const tdag_tdag* tdag, uint32_t level, tdag_level_record* out);
 /* Copies the kept matcher input masks of `level` (items * W words). */
 tdag_status tdag_tdag_level_masks(const tdag_tdag* tdag, uint32_t level, uint64_t* out, uint64_t cap_words);
 tdag_status tdag_tdag_free(tdag_tdag* tdag);
+
+/* ------------------------------------------------------------------------------------------
+ * StaticStructure form: constructStaticTransformDAG (transform_dag.h:163).
+ * ---------------------------------------------------------------------------------------- */
+
+/* voxcom::StaticStructure<void, uint32_t> (structure.h:299-320) as borrowed arrays. For level l in
+ * [3, root_level], level_words[l] holds level_word_counts[l] words: per node its child mask
+ * (bits 0..7), then one word per present child in child-index order, the child's word offset in
+ * level l - 1 (a leaf index at level 3). Entries 0..2 of both arrays are not read. Leaves are
+ * 4x4x4 masks in Morton bit order. resolution = 2^root_level. */
+typedef struct tdag_static_svdag_view {
+    uint32_t resolution;
+    uint32_t root_level;
+    const uint32_t* const* level_words;
+    const uint64_t* level_word_counts;
+    const uint64_t* leaves;
+    uint64_t num_leaves;
+} tdag_static_svdag_view;
+
+/* StaticStructure(const EditStructure&) (structure.h:766-798): the packed words of `level` of an
+ * SVDAG as uint32 (level 2: its leaf masks as uint64). *out_count receives the count; out may be
+ * NULL to query it. */
+tdag_status tdag_svdag_static_words(const tdag_svdag* svdag, uint32_t level, void* out, uint64_t cap, uint64_t* out_count);
+
+/* constructStaticTransformDAG (transform_dag.h:163, transform_dag.cpp:242-390;
+ * constructStaticTransformDAGHierarchical at transform_dag_hierarchical.cpp:531-743). Same
+ * matcher calls as tdag_construct_transform_dag on the same SVDAG; the hierarchical levels above
+ * max_translation_level + 1 come out in the static pass's own order. The input is validated
+ * (masks within 8 bits, nodes inside their level, every child word a node offset or leaf index of
+ * the level below: TDAG_EINVAL otherwise). The result is read with tdag_tdag_info,
+ * tdag_tdag_level_record, tdag_tdag_static_words and tdag_tdag_static_serialize. */
+tdag_status tdag_construct_static_transform_dag(tdag_context* ctx, const tdag_static_svdag_view* in, const tdag_config* cfg, uint32_t flags, tdag_tdag** out);
+/* StaticStructure<void, TransformPointer> words of `level` (level 2: the leaf masks); children are
+ * TransformPointer bit patterns whose ptr is a word offset in the level below (a leaf index at
+ * level 3). out may be NULL to query the count. */
+tdag_status tdag_tdag_static_words(const tdag_tdag* tdag, uint32_t level, uint64_t* out, uint64_t cap, uint64_t* out_count);
+/* StaticStructure<void, TransformPointer>::writeTo bytes (structure.h:888-895, binary_writer.h:48-78):
+ * per level u64 word count + words, u64 leaf count + leaves, u32 resolution, u32 rootLevel,
+ * i32 structureType (0, Tree: the reference's output is default-constructed,
+ * transform_dag_hierarchical.cpp:533). buf = NULL queries the size. */
+tdag_status tdag_tdag_static_serialize(const tdag_tdag* tdag, uint8_t* buf, uint64_t cap, uint64_t* out_size);
 
 #ifdef __cplusplus
 }

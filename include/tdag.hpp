@@ -2,6 +2,7 @@
 //
 //   findDuplicatesAmongFlattenedItems_gpu<Level, Item>   transform_dag.h:310-311
 //   constructTransformDAG(structure, config)              transform_dag.h:161-162
+//   constructStaticTransformDAG(structure, config)        transform_dag.h:163
 //
 // Header-only; link against libtdag.so. Errors are rethrown as std::runtime_error, which is
 // what the reference's assert_always does (error_handling.h:17-45).
@@ -118,6 +119,64 @@ inline std::vector<uint8_t> tdagBytes(const tdag_tdag* t, const tdag_config& con
     check(tdag_tdag_serialize(t, &config, nullptr, 0, &size), "tdag_tdag_serialize");
     std::vector<uint8_t> out(size);
     check(tdag_tdag_serialize(t, &config, out.data(), out.size(), &size), "tdag_tdag_serialize");
+    return out;
+}
+
+// The fields of voxcom::StaticStructure (structure.h:299-320) that the static driver reads and
+// writes: per level the packed words (child mask, then one word per present child), the 4x4x4
+// leaf masks (EditSubGrid<void> is one uint64), resolution and root level.
+template <typename Word>
+struct StaticStructure {
+    std::vector<std::vector<Word>> nodesPerLevel;
+    std::vector<uint64_t> subGrids;
+    uint32_t resolution = 0;
+    uint32_t rootLevel = 0;
+    int32_t structureType = 0;
+};
+
+// StaticStructure(const EditStructure&) (structure.h:766-798) of an SVDAG built by the library.
+inline StaticStructure<uint32_t> toStaticStructure(const tdag_svdag* svdag)
+{
+    StaticStructure<uint32_t> out;
+    check(tdag_svdag_info(svdag, &out.resolution, &out.rootLevel, nullptr, 0), "tdag_svdag_info");
+    out.structureType = 1;
+    out.nodesPerLevel.resize(out.rootLevel + 1);
+    uint64_t n = 0;
+    check(tdag_svdag_static_words(svdag, 2, nullptr, 0, &n), "tdag_svdag_static_words");
+    out.subGrids.resize(n);
+    check(tdag_svdag_static_words(svdag, 2, out.subGrids.data(), n, &n), "tdag_svdag_static_words");
+    for (uint32_t level = 3; level <= out.rootLevel; ++level) {
+        check(tdag_svdag_static_words(svdag, level, nullptr, 0, &n), "tdag_svdag_static_words");
+        out.nodesPerLevel[level].resize(n);
+        check(tdag_svdag_static_words(svdag, level, out.nodesPerLevel[level].data(), n, &n), "tdag_svdag_static_words");
+    }
+    return out;
+}
+
+// constructStaticTransformDAG (transform_dag.h:163) on the GPU matcher.
+inline StaticStructure<uint64_t> constructStaticTransformDAG(const StaticStructure<uint32_t>& inDag, const tdag_config& config, Context& ctx = Context::instance())
+{
+    std::vector<const uint32_t*> words(inDag.rootLevel + 1, nullptr);
+    std::vector<uint64_t> counts(inDag.rootLevel + 1, 0);
+    for (uint32_t level = 3; level <= inDag.rootLevel && level < inDag.nodesPerLevel.size(); ++level) {
+        words[level] = inDag.nodesPerLevel[level].data();
+        counts[level] = inDag.nodesPerLevel[level].size();
+    }
+    const tdag_static_svdag_view view { inDag.resolution, inDag.rootLevel, words.data(), counts.data(), inDag.subGrids.data(), inDag.subGrids.size() };
+    tdag_tdag* raw = nullptr;
+    check(tdag_construct_static_transform_dag(ctx.get(), &view, &config, 0, &raw), "tdag_construct_static_transform_dag");
+    TDag t(raw);
+    StaticStructure<uint64_t> out;
+    out.resolution = inDag.resolution;
+    out.rootLevel = inDag.rootLevel;
+    out.nodesPerLevel.resize(inDag.rootLevel + 1);
+    for (uint32_t level = 2; level <= inDag.rootLevel; ++level) {
+        uint64_t n = 0;
+        check(tdag_tdag_static_words(t.get(), level, nullptr, 0, &n), "tdag_tdag_static_words");
+        std::vector<uint64_t>& dst = level == 2 ? out.subGrids : out.nodesPerLevel[level];
+        dst.resize(n);
+        check(tdag_tdag_static_words(t.get(), level, dst.data(), n, &n), "tdag_tdag_static_words");
+    }
     return out;
 }
 

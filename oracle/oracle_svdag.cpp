@@ -5,6 +5,9 @@
 //   structure.get / createLargeSubGrid structure.h:334-355, large_sub_grid.h:127-141
 //   hierarchical sym+perm dedup       transform_dag_hierarchical.cpp:23-523
 //   constructTransformDAG driver      transform_dag.cpp:118-240
+//   StaticStructure conversion        structure.h:766-798, 859-874, 888-895
+//   static hierarchical dedup         transform_dag_hierarchical.cpp:531-743
+//   constructStaticTransformDAG       transform_dag.cpp:242-390
 //   .tdag serialisation               voxelizer/src/main.cpp:144-151, structure.h:493-501,
 //                                     binary_writer.h:48-78
 // The matcher it calls is the oracle restatement in oracle_match.cpp.
@@ -468,7 +471,7 @@ struct TDag {
     std::vector<std::vector<std::array<uint64_t, 8>>> nodesPerLevel;
     std::vector<uint64_t> subGrids;
     uint32_t resolution = 0, rootLevel = 0;
-    int32_t structureType = 1;
+    int32_t structureType = 0;
 };
 
 // constructTransformDAGHierarchical (transform_dag_hierarchical.cpp:304-523)
@@ -477,7 +480,7 @@ static TDag hierarchical(const Svdag& in, uint32_t midLevel, std::vector<uint32_
     TDag out;
     out.resolution = in.resolution;
     out.rootLevel = in.rootLevel;
-    out.structureType = in.structureType;
+    out.structureType = 0; // default-constructed output (:307): StructureType::Tree (structure.h:171)
     auto storeMid = [&](size_t numOut, const std::vector<Tagged>& mapping) {
         midInv.assign(numOut, 0);
         for (uint32_t old = 0; old < mapping.size(); ++old)
@@ -627,6 +630,295 @@ static std::vector<uint8_t> serialize(const TDag& t, const orc_config& cfg)
     return b;
 }
 
+// ---------------------------------------------------------------------------------------
+// StaticStructure path: constructStaticTransformDAG (transform_dag.cpp:242-390).
+// ---------------------------------------------------------------------------------------
+
+// StaticStructure<void, Child> (structure.h:299-320): per level one packed array; a node is its
+// child-mask word followed by one word per present child, in child-index order. Pointers are word
+// offsets of the child in the level below (leaf indices at level 3).
+template <typename Word>
+struct Static {
+    std::vector<std::vector<Word>> nodesPerLevel;
+    std::vector<uint64_t> subGrids;
+    uint32_t resolution = 0, rootLevel = 0;
+    int32_t structureType = 0; // StructureType::Tree unless set (structure.h:311)
+};
+using SSvdag = Static<uint32_t>; // StaticStructure<void, uint32_t>
+using STDag = Static<uint64_t>; // StaticStructure<void, TransformPointer>
+
+static inline uint32_t popc(uint32_t v) { return (uint32_t)__builtin_popcount(v); }
+static const uint64_t kPtrMask = (1ull << 37) - 1;
+
+// StaticStructure(const EditStructure&) (structure.h:766-798).
+static SSvdag toStatic(const Svdag& in)
+{
+    SSvdag out;
+    out.resolution = in.resolution;
+    out.rootLevel = in.rootLevel;
+    out.structureType = in.structureType;
+    out.subGrids = in.subGrids;
+    std::vector<uint32_t> prev(in.subGrids.size());
+    std::iota(prev.begin(), prev.end(), 0u);
+    out.nodesPerLevel.resize(in.nodesPerLevel.size());
+    for (uint32_t level = 3; level <= in.rootLevel; ++level) {
+        const auto& nodes = in.nodesPerLevel[level];
+        auto& words = out.nodesPerLevel[level];
+        std::vector<uint32_t> cur(nodes.size());
+        for (size_t i = 0; i < nodes.size(); ++i) {
+            const uint32_t at = (uint32_t)words.size();
+            cur[i] = at;
+            words.push_back(0);
+            for (uint32_t c = 0; c < 8; ++c)
+                if (nodes[i][c] != kEmpty) {
+                    words.push_back(prev[nodes[i][c]]);
+                    words[at] |= 1u << c;
+                }
+        }
+        prev = std::move(cur);
+    }
+    return out;
+}
+
+// getLevelNodeStarts (structure.h:859-874).
+template <typename Word>
+static std::vector<uint32_t> levelStarts(const Static<Word>& s, uint32_t level)
+{
+    std::vector<uint32_t> out;
+    if (level == 2) {
+        out.resize(s.subGrids.size());
+        std::iota(out.begin(), out.end(), 0u);
+        return out;
+    }
+    const auto& w = s.nodesPerLevel[level];
+    for (size_t cur = 0; cur < w.size(); cur += 1 + popc((uint32_t)w[cur]))
+        out.push_back((uint32_t)cur);
+    return out;
+}
+
+// StaticNode::getChildPointerAtIndex (structure.h:236-247); a TransformPointer child converts to
+// its ptr field (transform_dag.h:339-342).
+template <typename Word>
+static uint32_t childAt(const std::vector<Word>& w, uint32_t start, uint32_t childIdx)
+{
+    const uint32_t mask = (uint32_t)w[start];
+    const uint32_t off = popc(mask & ~(0xFFFFFFFFu << childIdx));
+    return (uint32_t)((uint64_t)w[start + 1 + off] & kPtrMask);
+}
+
+// StaticStructure::get(voxel, level, root) (structure.h:836-855).
+template <typename Word>
+static bool staticGet(const Static<Word>& s, uint32_t x, uint32_t y, uint32_t z, uint32_t level, uint32_t root)
+{
+    uint32_t at = root;
+    for (uint32_t l = level; l > 2; --l) {
+        const auto& w = s.nodesPerLevel[l];
+        const uint32_t cl = l - 1;
+        const uint32_t childIdx = morton3((x >> cl) & 1u, (y >> cl) & 1u, (z >> cl) & 1u);
+        if (!((w[at] >> childIdx) & 1u))
+            return false;
+        at = childAt(w, at, childIdx);
+    }
+    return (s.subGrids[at] >> morton3(x & 3u, y & 3u, z & 3u)) & 1ull;
+}
+
+// createLargeSubGrid over a StaticStructure (transform_dag.cpp:399-403).
+template <typename Word>
+static void staticFlatten(const Static<Word>& s, uint32_t level, uint32_t start, uint64_t* out)
+{
+    const uint32_t R = 1u << level;
+    const uint32_t W = std::max<uint32_t>(1, R * R * R / 64);
+    std::memset(out, 0, W * 8);
+    for (uint32_t z = 0; z < R; ++z)
+        for (uint32_t y = 0; y < R; ++y)
+            for (uint32_t x = 0; x < R; ++x)
+                if (staticGet(s, x, y, z, level, start)) {
+                    const uint32_t i = x + R * y + R * R * z;
+                    out[i >> 6] |= 1ull << (i & 63);
+                }
+}
+
+// constructStaticTransformDAGHierarchical (transform_dag_hierarchical.cpp:531-743). The groups
+// are keyed by the children's output word offsets, so the output order differs from the
+// EditStructure pass while the set of kept nodes is the same.
+static STDag staticHierarchical(const SSvdag& in, uint32_t midLevel, std::vector<uint32_t>& midStarts, const orc_config& cfg)
+{
+    STDag out; // default-constructed: StructureType::Tree (:533)
+    out.resolution = in.resolution;
+    out.rootLevel = in.rootLevel;
+    using Map = std::unordered_map<uint32_t, Tagged>;
+    auto storeMid = [&](const Map& m) { // :538-548
+        std::vector<std::pair<uint32_t, Tagged>> canon;
+        for (const auto& kv : m)
+            if (!hasTransform(kv.second.tp))
+                canon.push_back(kv);
+        std::sort(canon.begin(), canon.end(), [](const auto& a, const auto& b) { return a.second.tp.ptr < b.second.tp.ptr; });
+        midStarts.clear();
+        for (const auto& kv : canon)
+            midStarts.push_back(kv.first);
+    };
+    Map prev;
+    {
+        const auto canon = canonicalLeaves(cfg);
+        std::vector<TNode> tnodes(in.subGrids.size());
+        for (size_t i = 0; i < in.subGrids.size(); ++i)
+            for (uint32_t k = 0; k < 8; ++k)
+                tnodes[i].c[k] = canon[(in.subGrids[i] >> (8 * k)) & 0xFF];
+        std::vector<uint32_t> kept;
+        std::vector<Tagged> mapping;
+        processLevel(tnodes, cfg, kept, mapping);
+        for (uint32_t k : kept)
+            out.subGrids.push_back(in.subGrids[k]);
+        for (uint32_t i = 0; i < in.subGrids.size(); ++i)
+            prev[i] = mapping[i];
+        if (midLevel == 2)
+            storeMid(prev);
+    }
+    out.nodesPerLevel.resize(in.nodesPerLevel.size());
+    for (uint32_t level = 3; level <= in.rootLevel; ++level) {
+        const auto& w = in.nodesPerLevel[level];
+        const std::vector<uint32_t> starts = levelStarts(in, level);
+        std::vector<TNode> tnodes(starts.size());
+        for (size_t j = 0; j < starts.size(); ++j)
+            for (uint32_t c = 0; c < 8; ++c)
+                tnodes[j].c[c] = ((w[starts[j]] >> c) & 1u) ? prev.at(childAt(w, starts[j], c)) : taggedSentinel();
+        std::vector<uint32_t> kept;
+        std::vector<Tagged> mapping;
+        processLevel(tnodes, cfg, kept, mapping);
+        auto& ow = out.nodesPerLevel[level];
+        std::vector<uint32_t> outStart(kept.size());
+        for (size_t h = 0; h < kept.size(); ++h) { // :717-728
+            outStart[h] = (uint32_t)ow.size();
+            ow.push_back(w[starts[kept[h]]] & 0xFFFFFFFFu);
+            for (uint32_t c = 0; c < 8; ++c)
+                if (tnodes[kept[h]].c[c].tp.ptr != kSentinelTP.ptr)
+                    ow.push_back(tpBits(tnodes[kept[h]].c[c].tp));
+        }
+        Map next; // :731-737
+        for (size_t j = 0; j < starts.size(); ++j) {
+            Tagged t = mapping[j];
+            t.tp.ptr = outStart[t.tp.ptr];
+            next[starts[j]] = t;
+        }
+        if (level == midLevel)
+            storeMid(next);
+        prev = std::move(next);
+    }
+    return out;
+}
+
+static uint64_t withPtr(uint64_t tp, uint64_t ptr) { return (tp & ~kPtrMask) | (ptr & kPtrMask); }
+
+// constructStaticTransformDAG (transform_dag.cpp:242-390), with the oracle matcher.
+static STDag constructStatic(const SSvdag& in, const orc_config& cfg, int nthreads)
+{
+    const uint32_t maxT = cfg.maxTranslationLevel;
+    std::vector<uint32_t> translationParents;
+    STDag out = staticHierarchical(in, maxT + 1, translationParents, cfg);
+    for (auto& v : g_levelInputs)
+        v.clear();
+    if (!cfg.translation)
+        return out;
+    out.subGrids = in.subGrids; // :261-264
+    for (uint32_t level = 3; level <= maxT + 1; ++level) { // :268-293
+        const std::vector<uint32_t> starts = level == maxT + 1 ? translationParents : levelStarts(in, level);
+        const auto& w = in.nodesPerLevel[level];
+        auto& ow = out.nodesPerLevel[level];
+        const size_t before = ow.size();
+        ow.clear();
+        for (uint32_t st : starts) {
+            ow.push_back(w[st]);
+            for (uint32_t c = 0; c < 8; ++c)
+                if ((w[st] >> c) & 1u)
+                    ow.push_back(tpBits(tpCreate(childAt(w, st, c))));
+        }
+        if (level == maxT + 1 && ow.size() != before)
+            throw std::runtime_error("mid level size mismatch");
+    }
+    for (uint32_t level = maxT; level >= 2; --level) { // :298-387
+        std::vector<uint32_t> itemStarts = levelStarts(out, level);
+        const auto& pw = out.nodesPerLevel[level + 1];
+        std::vector<bool> referenced(level == 2 ? out.subGrids.size() : out.nodesPerLevel[level].size(), false);
+        for (size_t cur = 0; cur < pw.size(); cur += 1 + popc((uint32_t)pw[cur]))
+            for (uint32_t c = 0; c < 8; ++c)
+                if ((pw[cur] >> c) & 1u)
+                    referenced[childAt(pw, (uint32_t)cur, c)] = true;
+        size_t kept = 0;
+        for (uint32_t st : itemStarts)
+            if (referenced[st])
+                itemStarts[kept++] = st;
+        itemStarts.resize(kept);
+        const uint32_t R = 1u << level;
+        const uint32_t W = std::max<uint32_t>(1, R * R * R / 64);
+        std::vector<uint64_t> masks((size_t)itemStarts.size() * W);
+        for (size_t i = 0; i < itemStarts.size(); ++i)
+            staticFlatten(out, level, itemStarts[i], masks.data() + i * W);
+        g_levelInputs[level] = masks;
+        std::vector<uint64_t> enc(itemStarts.size()), mapping(itemStarts.size());
+        std::vector<uint32_t> uniq(itemStarts.size());
+        uint64_t nu = 0;
+        orc_match_encoded(level, masks.data(), itemStarts.size(), &cfg, enc.data(), nthreads, 0);
+        orc_postprocess(level, masks.data(), itemStarts.size(), enc.data(), mapping.data(), uniq.data(), &nu);
+        if (level == 2) {
+            std::vector<uint64_t> sub(nu);
+            for (uint64_t k = 0; k < nu; ++k)
+                sub[k] = out.subGrids[itemStarts[uniq[k]]];
+            out.subGrids = std::move(sub);
+        } else {
+            auto& ow = out.nodesPerLevel[level];
+            std::vector<uint64_t> nw;
+            std::vector<uint32_t> pointerMapping(nu);
+            for (uint64_t k = 0; k < nu; ++k) {
+                const uint32_t st = itemStarts[uniq[k]];
+                pointerMapping[k] = (uint32_t)nw.size();
+                for (uint32_t i = 0; i < 1 + popc((uint32_t)ow[st]); ++i)
+                    nw.push_back(ow[st + i]);
+            }
+            ow = std::move(nw);
+            for (auto& m : mapping)
+                m = withPtr(m, pointerMapping[m & kPtrMask]);
+        }
+        auto& parents = out.nodesPerLevel[level + 1]; // :373-386
+        for (size_t cur = 0; cur < parents.size(); cur += 1 + popc((uint32_t)parents[cur])) {
+            const uint32_t nc = popc((uint32_t)parents[cur]);
+            for (uint32_t o = 0; o < nc; ++o) {
+                const uint32_t old = (uint32_t)(parents[cur + 1 + o] & kPtrMask);
+                const auto it = std::lower_bound(itemStarts.begin(), itemStarts.end(), old);
+                if (it == itemStarts.end() || *it != old)
+                    throw std::runtime_error("parent pointer not among the level's items");
+                parents[cur + 1 + o] = mapping[it - itemStarts.begin()];
+            }
+        }
+        if (level == 2)
+            break;
+    }
+    return out;
+}
+
+// StaticStructure::writeTo (structure.h:888-895) through BinaryWriter (binary_writer.h:48-78).
+static std::vector<uint8_t> serializeStatic(const STDag& t)
+{
+    std::vector<uint8_t> b;
+    auto put = [&](const void* p, size_t n) {
+        const uint8_t* c = (const uint8_t*)p;
+        b.insert(b.end(), c, c + n);
+    };
+    const uint64_t nl = t.nodesPerLevel.size();
+    put(&nl, 8);
+    for (const auto& lvl : t.nodesPerLevel) {
+        const uint64_t n = lvl.size();
+        put(&n, 8);
+        put(lvl.data(), n * 8);
+    }
+    const uint64_t ns = t.subGrids.size();
+    put(&ns, 8);
+    put(t.subGrids.data(), ns * 8);
+    put(&t.resolution, 4);
+    put(&t.rootLevel, 4);
+    put(&t.structureType, 4);
+    return b;
+}
+
 } // namespace orc2
 
 using namespace orc2;
@@ -708,6 +1000,47 @@ int64_t orc_construct_tdag(void* h, const orc_config* cfg, int nthreads, const c
         return (int64_t)bytes.size();
     } catch (const std::exception& e) {
         std::fprintf(stderr, "orc_construct_tdag: %s\n", e.what());
+        return -1;
+    }
+}
+
+// StaticStructure<void, uint32_t>(EditStructure) words of `level` (level 2: the leaf masks as
+// uint64). Returns the word count; copies when `out` holds at least that many.
+int64_t orc_svdag_static_level(void* h, uint32_t level, void* out, uint64_t cap)
+{
+    const SSvdag st = toStatic(*(const Svdag*)h);
+    if (level == 2) {
+        if (out && cap >= st.subGrids.size())
+            std::memcpy(out, st.subGrids.data(), st.subGrids.size() * 8);
+        return (int64_t)st.subGrids.size();
+    }
+    if (level >= st.nodesPerLevel.size())
+        return -1;
+    const auto& w = st.nodesPerLevel[level];
+    if (out && cap >= w.size())
+        std::memcpy(out, w.data(), w.size() * 4);
+    return (int64_t)w.size();
+}
+
+// Runs constructStaticTransformDAG on the static form of the SVDAG and writes the
+// StaticStructure<void, TransformPointer>::writeTo bytes to `path`. Returns the byte count, or a
+// negative value on error.
+int64_t orc_construct_static_tdag(void* h, const orc_config* cfg, int nthreads, const char* path)
+{
+    try {
+        const Svdag& in = *(const Svdag*)h;
+        if (cfg->maxTranslationLevel + 1 > in.rootLevel)
+            return -3;
+        const STDag t = constructStatic(toStatic(in), *cfg, nthreads);
+        const std::vector<uint8_t> bytes = serializeStatic(t);
+        FILE* f = std::fopen(path, "wb");
+        if (!f)
+            return -2;
+        std::fwrite(bytes.data(), 1, bytes.size(), f);
+        std::fclose(f);
+        return (int64_t)bytes.size();
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "orc_construct_static_tdag: %s\n", e.what());
         return -1;
     }
 }

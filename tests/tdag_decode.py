@@ -40,6 +40,69 @@ def parse(b: bytes):
                 config=(sym, axis, trans, maxT))
 
 
+def parse_static(b: bytes):
+    """StaticStructure<void, TransformPointer>::writeTo bytes (structure.h:888-895 through
+    binary_writer.h:48-78): per level u64 word count + u64 words, u64 leaf count + leaf masks,
+    then u32 resolution, u32 rootLevel, i32 structureType. No config follows."""
+    off = 0
+
+    def u64():
+        nonlocal off
+        v = struct.unpack_from("<Q", b, off)[0]
+        off += 8
+        return v
+
+    nlevels = u64()
+    words = []
+    for _ in range(nlevels):
+        n = u64()
+        words.append(np.frombuffer(b, dtype="<u8", count=n, offset=off).copy() if n else np.zeros(0, dtype=np.uint64))
+        off += n * 8
+    ns = u64()
+    leaves = np.frombuffer(b, dtype="<u8", count=ns, offset=off) if ns else np.zeros(0, dtype=np.uint64)
+    off += ns * 8
+    resolution, root_level, structure_type = struct.unpack_from("<IIi", b, off)
+    off += 12
+    assert off == len(b)
+    return dict(words=words, leaves=leaves, resolution=resolution, root_level=root_level, structure_type=structure_type)
+
+
+def node_starts(words) -> list:
+    """getLevelNodeStarts (structure.h:859-874): a node is its child-mask word plus one word per child."""
+    out, cur = [], 0
+    while cur < len(words):
+        out.append(cur)
+        cur += 1 + bin(int(words[cur]) & 0xFFFFFFFF).count("1")
+    return out
+
+
+def static_to_edit(d):
+    """Packed levels -> 8 slots per node (SENTINEL for an absent child), pointers as node indices:
+    the EditStructure view of the same structure, which `expand` walks."""
+    levels = [np.zeros((0, 8), dtype=np.uint64) for _ in d["words"]]
+    index_of = None  # start -> index of the level below (None at level 3: leaf indices)
+    for level in range(3, len(d["words"])):
+        w = d["words"][level]
+        starts = node_starts(w)
+        arr = np.full((len(starts), 8), SENTINEL, dtype=np.uint64)
+        for i, st in enumerate(starts):
+            mask = int(w[st]) & 0xFFFFFFFF
+            assert mask < 256
+            o = 1
+            for c in range(8):
+                if (mask >> c) & 1:
+                    tp = int(w[st + o])
+                    o += 1
+                    ptr = tp & ((1 << 37) - 1)
+                    if index_of is not None:
+                        ptr = index_of[ptr]
+                    arr[i, c] = (tp & ~((1 << 37) - 1)) | ptr
+        levels[level] = arr
+        index_of = {st: i for i, st in enumerate(starts)}
+    return dict(levels=levels, leaves=d["leaves"], resolution=d["resolution"], root_level=d["root_level"],
+                structure_type=d["structure_type"])
+
+
 def _leaf_cube(mask):
     cube = np.zeros((4, 4, 4), dtype=bool)  # [x, y, z]
     m = int(mask)

@@ -72,3 +72,51 @@ def test_level_inputs_match_oracle_matcher(oracle, gpu_ctx):
         assert np.array_equal(enc, ref)
         rec = t.record(level)
         assert rec.items == len(masks) and rec.matched == st.num_matched
+
+
+def _oracle_static_bytes(oracle, bits, R, cfg):
+    L = oracle.lib()
+    h = ctypes.c_void_p()
+    assert L.orc_svdag_build(bits.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), R, ctypes.byref(h)) == 0
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "o.static")
+        assert L.orc_construct_static_tdag(h, ctypes.byref(cfg), os.cpu_count() or 1, path.encode()) > 0
+        data = open(path, "rb").read()
+    L.orc_svdag_free(h)
+    return data
+
+
+@pytest.mark.parametrize("kind,R,seed,maxT", [(0, 64, 1, 2), (0, 64, 2, 5), (1, 128, 3, 3), (1, 128, 4, 4), (2, 256, 5, 3)])
+def test_static_transform_dag_matches_oracle(oracle, gpu_ctx, kind, R, seed, maxT):
+    """constructStaticTransformDAG through the GPU matcher: writeTo bytes equal the oracle's
+    restated static driver, the result expands back to the voxels, and every level's matcher call
+    equals the EditStructure driver's (same items, uniques and matches)."""
+    import tdag_amd
+    from tdag_amd import pipeline as pl
+
+    print(f"static scene kind={kind} R={R} seed={seed} maxT={maxT}", flush=True)
+    bits = pl.scene_dense(kind, R, seed)
+    svdag = pl.Svdag(bits, R)
+    cfg = tdag_amd.TransformDAGConfig(True, True, True, maxT)
+    t = pl.construct_static_transform_dag(gpu_ctx, pl.StaticSvdag.from_svdag(svdag), cfg)
+    mine = t.static_bytes()
+    assert mine == _oracle_static_bytes(oracle, bits, R, oracle.config(True, True, True, maxT))
+    d = tdag_decode.parse_static(mine)
+    assert np.array_equal(tdag_decode.expand(tdag_decode.static_to_edit(d)), tdag_decode.dense_to_cube(bits, R))
+    e = pl.construct_transform_dag(gpu_ctx, svdag, cfg)
+    assert t.level_sizes == e.level_sizes
+    for level in range(2, maxT + 1):
+        a, b = t.record(level), e.record(level)
+        assert (a.items, a.unique, a.matched) == (b.items, b.unique, b.matched), level
+
+
+@pytest.mark.parametrize("sym,axis", [(False, True), (True, False)])
+def test_static_config_switches(oracle, gpu_ctx, sym, axis):
+    import tdag_amd
+    from tdag_amd import pipeline as pl
+
+    print(f"static cfg sym={sym} axis={axis}", flush=True)
+    bits = pl.scene_dense(1, 128, 9)
+    cfg = tdag_amd.TransformDAGConfig(sym, axis, True, 3)
+    t = pl.construct_static_transform_dag(gpu_ctx, pl.StaticSvdag.from_svdag(pl.Svdag(bits, 128)), cfg)
+    assert t.static_bytes() == _oracle_static_bytes(oracle, bits, 128, oracle.config(sym, axis, True, 3))

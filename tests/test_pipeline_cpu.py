@@ -96,6 +96,9 @@ def test_hierarchical_tdag_bytes(oracle, pl, kind, R, seed, sym, axis):
     ref = _oracle_tdag_bytes(oracle, bits, R, oracle.config(sym, axis, False, 3))
     assert mine == ref
     d = tdag_decode.parse(mine)
+    # The reference's output EditStructure is default-constructed (transform_dag_hierarchical.cpp:307)
+    # and never marked a DAG, so the header holds StructureType::Tree = 0 (structure.h:127-131, 171).
+    assert d["structure_type"] == 0
     assert np.array_equal(tdag_decode.expand(d), tdag_decode.dense_to_cube(bits, R))
 
 
