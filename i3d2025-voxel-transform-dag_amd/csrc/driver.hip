@@ -7,10 +7,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <string>
 #include <thread>
 
+#include "context.hpp"
 #include "errors.hpp"
 #include "pipeline.hpp"
 #include "tdag.h"
@@ -68,6 +70,26 @@ tdag_status tdag_svdag_from_dense(const uint8_t* bits, uint32_t resolution, tdag
         return setErr(TDAG_EINVAL, "resolution must be a power of two in [4, 4096]");
     tdag_svdag* h = new tdag_svdag();
     h->s = tdag::svdagFromDense(bits, resolution, hostThreads());
+    *out = h;
+    return TDAG_OK;
+}
+
+tdag_status tdag_svdag_from_dense_device(tdag_context* ctx, const uint8_t* d_bits, uint32_t resolution, tdag_svdag** out, float* out_ms)
+{
+    if (!ctx || !d_bits || !out)
+        return setErr(TDAG_EINVAL, "null argument");
+    if (resolution < 4 || (resolution & (resolution - 1)) || resolution > 4096)
+        return setErr(TDAG_EINVAL, "resolution must be a power of two in [4, 4096]");
+    if (hipError_t e = hipSetDevice(ctx->device); e != hipSuccess)
+        return setErr(TDAG_EDEVICE, std::string("hipSetDevice: ") + hipGetErrorName(e));
+    const auto t0 = std::chrono::steady_clock::now();
+    tdag_svdag* h = new tdag_svdag();
+    if (hipError_t e = tdag::svdagFromDenseDevice(d_bits, resolution, ctx->stream, h->s); e != hipSuccess) {
+        delete h;
+        return setErr(e == hipErrorOutOfMemory ? TDAG_ENOMEM : TDAG_EDEVICE, std::string("device SVDAG build: ") + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ")");
+    }
+    if (out_ms)
+        *out_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
     *out = h;
     return TDAG_OK;
 }

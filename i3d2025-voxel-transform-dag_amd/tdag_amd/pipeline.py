@@ -53,6 +53,8 @@ def bind(L):
     L.tdag_scene_dense.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, _u8p, ctypes.c_uint64]
     L.tdag_svdag_from_dense.argtypes = [_u8p, ctypes.c_uint32, ctypes.POINTER(vp)]
     L.tdag_svdag_info.argtypes = [vp, _u32p, _u32p, _u64p, ctypes.c_uint32]
+    L.tdag_svdag_from_dense_device.argtypes = [vp, vp, ctypes.c_uint32, ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_float)]
+    L.tdag_svdag_from_dense_device.restype = ctypes.c_int
     L.tdag_svdag_copy_level.argtypes = [vp, ctypes.c_uint32, vp]
     L.tdag_svdag_flatten.argtypes = [vp, ctypes.c_uint32, _u32p, ctypes.c_uint64, _u64p]
     L.tdag_svdag_free.argtypes = [vp]
@@ -90,6 +92,27 @@ class Svdag:
         b = np.ascontiguousarray(bits, dtype=np.uint8)
         self._h = ctypes.c_void_p()
         check(lib().tdag_svdag_from_dense(b.ctypes.data_as(_u8p), resolution, ctypes.byref(self._h)), "tdag_svdag_from_dense")
+        self._load_info()
+
+    @classmethod
+    def from_device(cls, ctx, d_bits, resolution: int):
+        """Built on the GPU from a dense grid in device memory (a torch uint8 tensor of R^3/8
+        bytes, bit i = x + R*y + R*R*z). Sets ``build_ms`` to the wall time of the call."""
+        if not d_bits.is_cuda or not d_bits.is_contiguous() or d_bits.numel() * d_bits.element_size() < resolution ** 3 // 8:
+            raise ValueError("d_bits must be a contiguous device tensor of at least resolution^3 / 8 bytes")
+        import torch
+
+        torch.cuda.synchronize()  # the library works on its own stream
+        self = cls.__new__(cls)
+        self._h = ctypes.c_void_p()
+        ms = ctypes.c_float()
+        check(lib().tdag_svdag_from_dense_device(ctx._ptr, ctypes.c_void_p(d_bits.data_ptr()), resolution, ctypes.byref(self._h), ctypes.byref(ms)),
+              "tdag_svdag_from_dense_device")
+        self.build_ms = float(ms.value)
+        self._load_info()
+        return self
+
+    def _load_info(self):
         r = ctypes.c_uint32()
         root = ctypes.c_uint32()
         sizes = np.zeros(16, dtype=np.uint64)

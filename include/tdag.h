@@ -291,6 +291,12 @@ tdag_status tdag_scene_dense(uint32_t kind, uint32_t resolution, uint64_t seed, 
 
 /* EditStructure<void, uint32_t>(VoxelGrid) followed by toDAG() (structure.h:423-429, 658-763). */
 tdag_status tdag_svdag_from_dense(const uint8_t* bits, uint32_t resolution, tdag_svdag** out);
+/* The same SVDAG, node for node, built on the device from a dense grid already in device memory
+ * (d_bits: resolution^3 / 8 bytes, bit i = x + R*y + R*R*z). Leaves are the non-empty 4^3 blocks in
+ * Morton order and every level keeps the first occurrence of each node, which is the index order
+ * of the reference's DFS construction and toDAG. Work runs on ctx's stream; out_ms (optional)
+ * receives the wall time of the call, including the copy of the result to host memory. */
+tdag_status tdag_svdag_from_dense_device(tdag_context* ctx, const uint8_t* d_bits, uint32_t resolution, tdag_svdag** out, float* out_ms);
 /* resolution, root level and the node count of levels [0, cap) (level 2 = leaves). */
 tdag_status tdag_svdag_info(const tdag_svdag* svdag, uint32_t* resolution, uint32_t* root_level, uint64_t* level_sizes, uint32_t cap);
 /* Copies level `level`: 8 uint32 children per node (0xFFFFFFFF = empty), or for level 2 the

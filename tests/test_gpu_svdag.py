@@ -1,0 +1,63 @@
+"""The device SVDAG producer (tdag_svdag_from_dense_device) against the host producer, which the
+CPU tests pin node for node to the oracle's restatement of the reference's recursive
+EditStructure(VoxelGrid) constructor + toDAG (structure.h:658-763): same leaves, same nodes, same
+indices, at every level; edge cases are the empty grid, the full grid, the smallest resolution and
+a noise grid whose leaves are nearly all distinct."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _same(host, dev):
+    assert dev.resolution == host.resolution and dev.root_level == host.root_level
+    assert dev.level_sizes == host.level_sizes
+    assert np.array_equal(dev.leaves(), host.leaves())
+    for level in range(3, host.root_level + 1):
+        assert np.array_equal(dev.nodes(level), host.nodes(level)), level
+
+
+def _device_bits(bits):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(bits)).cuda()
+
+
+@pytest.mark.parametrize("kind,R,seed", [(0, 64, 1), (1, 128, 3), (2, 256, 4), (1, 512, 7)])
+def test_device_svdag_equals_host(gpu_ctx, kind, R, seed):
+    from tdag_amd import pipeline as pl
+
+    print(f"device svdag kind={kind} R={R} seed={seed}", flush=True)
+    bits = pl.scene_dense(kind, R, seed)
+    _same(pl.Svdag(bits, R), pl.Svdag.from_device(gpu_ctx, _device_bits(bits), R))
+
+
+@pytest.mark.parametrize("case", ["empty", "full", "R4", "noise"])
+def test_device_svdag_edge_cases(gpu_ctx, case):
+    from tdag_amd import pipeline as pl
+
+    print(f"device svdag edge case {case}", flush=True)
+    R = 4 if case == "R4" else 64
+    if case == "empty":
+        bits = np.zeros(R ** 3 // 8, dtype=np.uint8)
+    elif case == "full":
+        bits = np.full(R ** 3 // 8, 0xFF, dtype=np.uint8)
+    else:
+        bits = np.random.default_rng(11).integers(0, 256, R ** 3 // 8, dtype=np.uint8)
+    host = pl.Svdag(bits, R)
+    dev = pl.Svdag.from_device(gpu_ctx, _device_bits(bits), R)
+    _same(host, dev)
+    if case == "full":
+        assert dev.level_sizes[2:] == [1] * (dev.root_level - 1)
+
+
+def test_device_svdag_feeds_the_driver(oracle, gpu_ctx):
+    """The device-built SVDAG gives the same .tdag as the host-built one (full driver)."""
+    import tdag_amd
+    from tdag_amd import pipeline as pl
+
+    bits = pl.scene_dense(1, 128, 5)
+    cfg = tdag_amd.TransformDAGConfig(True, True, True, 3)
+    a = pl.construct_transform_dag(gpu_ctx, pl.Svdag(bits, 128), cfg).tdag_bytes()
+    b = pl.construct_transform_dag(gpu_ctx, pl.Svdag.from_device(gpu_ctx, _device_bits(bits), 128), cfg).tdag_bytes()
+    assert a == b
