@@ -152,6 +152,8 @@ tdag_status tdag_context_destroy(tdag_context* ctx)
         (void)hipFree(ctx->scratch);
     if (ctx->io)
         (void)hipFree(ctx->io);
+    if (ctx->svdagScratch)
+        (void)hipFree(ctx->svdagScratch);
     if (ctx->pinned)
         (void)hipHostFree(ctx->pinned);
     if (ctx->stream)

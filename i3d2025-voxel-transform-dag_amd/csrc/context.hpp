@@ -26,6 +26,8 @@ struct tdag_context {
     uint32_t batchGrowth[6] = { 0, 0, 0, 0, 0, 0 };
     uint32_t bloomBits[6] = { 0, 0, 0, 0, 0, 0 }; // Bloom filter bits per table entry (0 = default)
     uint32_t childFrames[6] = { 0, 0, 0, 0, 0, 0 }; // levels 3..5: child-table permutation frames, 1 or 6 (0 = default)
+    void* svdagScratch = nullptr; // workspace of the device SVDAG build (grown on demand)
+    size_t svdagScratchBytes = 0;
 };
 
 namespace tdag {

@@ -84,7 +84,7 @@ tdag_status tdag_svdag_from_dense_device(tdag_context* ctx, const uint8_t* d_bit
         return setErr(TDAG_EDEVICE, std::string("hipSetDevice: ") + hipGetErrorName(e));
     const auto t0 = std::chrono::steady_clock::now();
     tdag_svdag* h = new tdag_svdag();
-    if (hipError_t e = tdag::svdagFromDenseDevice(d_bits, resolution, ctx->stream, h->s); e != hipSuccess) {
+    if (hipError_t e = tdag::svdagFromDenseDevice(d_bits, resolution, ctx->stream, ctx->svdagScratch, ctx->svdagScratchBytes, h->s); e != hipSuccess) {
         delete h;
         return setErr(e == hipErrorOutOfMemory ? TDAG_ENOMEM : TDAG_EDEVICE, std::string("device SVDAG build: ") + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ")");
     }

@@ -52,8 +52,9 @@ bool generateSceneDense(uint32_t kind, uint32_t R, uint64_t seed, uint8_t* bits,
 Svdag svdagFromDense(const uint8_t* bits, uint32_t R, int threads);
 
 // The same SVDAG built on the device from a dense grid in device memory (R^3 / 8 bytes), work
-// enqueued on `stream`; returns after the result has been copied into `out`.
-hipError_t svdagFromDenseDevice(const uint8_t* d_bits, uint32_t R, hipStream_t stream, Svdag& out);
+// enqueued on `stream` in a workspace (scratch, scratchBytes) that is grown as needed and kept
+// by the caller; returns after the result has been copied into `out`.
+hipError_t svdagFromDenseDevice(const uint8_t* d_bits, uint32_t R, hipStream_t stream, void*& scratch, size_t& scratchBytes, Svdag& out);
 
 // createLargeSubGrid for level-`level` nodes (level 2 = leaves): linear masks, W words each.
 void flattenNodes(const Svdag& s, uint32_t level, const uint32_t* nodes, uint64_t n, uint64_t* out, int threads);

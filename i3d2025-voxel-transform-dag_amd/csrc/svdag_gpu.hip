@@ -106,10 +106,10 @@ __device__ inline bool keyEq(const uint32_t* a, const uint32_t* b)
 // never empty again and only ever move to a smaller index with the same key, so comparing
 // against the key of whatever index a slot holds is always meaningful. At most cap probes.
 template <int K>
-__global__ __launch_bounds__(kThreads) void kInsertFirst(const uint32_t* __restrict__ keys, uint32_t n, uint32_t* table, uint32_t capMask)
+__global__ __launch_bounds__(kThreads) void kInsertFirst(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ d_n, uint32_t* table, uint32_t capMask)
 {
     const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= n)
+    if (i >= *d_n)
         return;
     const uint32_t* k = keys + (size_t)i * K;
     uint32_t h = keyHash<K>(k) & capMask;
@@ -129,13 +129,19 @@ __global__ __launch_bounds__(kThreads) void kInsertFirst(const uint32_t* __restr
     }
 }
 
+// Over the whole bound: items past the count get first = 0, so the scan of `first` numbers the
+// uniques only.
 template <int K>
-__global__ __launch_bounds__(kThreads) void kRepresentative(const uint32_t* __restrict__ keys, uint32_t n, const uint32_t* __restrict__ table, uint32_t capMask,
-    uint32_t* __restrict__ rep, uint32_t* __restrict__ first)
+__global__ __launch_bounds__(kThreads) void kRepresentative(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ d_n, uint32_t bound,
+    const uint32_t* __restrict__ table, uint32_t capMask, uint32_t* __restrict__ rep, uint32_t* __restrict__ first)
 {
     const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= n)
+    if (i >= bound)
         return;
+    if (i >= *d_n) {
+        first[i] = 0;
+        return;
+    }
     const uint32_t* k = keys + (size_t)i * K;
     uint32_t h = keyHash<K>(k) & capMask;
     uint32_t r = i; // every key was inserted, so the loop finds it; i is only a guard value
@@ -154,11 +160,11 @@ __global__ __launch_bounds__(kThreads) void kRepresentative(const uint32_t* __re
 }
 
 template <int K>
-__global__ __launch_bounds__(kThreads) void kAssignIds(const uint32_t* __restrict__ keys, uint32_t n, const uint32_t* __restrict__ rep, const uint32_t* __restrict__ uid,
-    uint32_t* __restrict__ id, uint32_t* __restrict__ uniqueKeys)
+__global__ __launch_bounds__(kThreads) void kAssignIds(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ d_n, const uint32_t* __restrict__ rep,
+    const uint32_t* __restrict__ uid, uint32_t* __restrict__ id, uint32_t* __restrict__ uniqueKeys)
 {
     const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= n)
+    if (i >= *d_n)
         return;
     const uint32_t r = rep[i];
     id[i] = uid[r];
@@ -167,20 +173,22 @@ __global__ __launch_bounds__(kThreads) void kAssignIds(const uint32_t* __restric
             uniqueKeys[(size_t)uid[i] * K + j] = keys[(size_t)i * K + j];
 }
 
-__global__ __launch_bounds__(kThreads) void kParentStarts(const uint32_t* __restrict__ codes, uint32_t n, uint32_t* __restrict__ start)
+// Over the whole bound (items past the count start nothing).
+__global__ __launch_bounds__(kThreads) void kParentStarts(const uint32_t* __restrict__ codes, const uint32_t* __restrict__ d_n, uint32_t bound,
+    uint32_t* __restrict__ start)
 {
     const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= n)
+    if (i >= bound)
         return;
-    start[i] = (i == 0 || (codes[i] >> 3) != (codes[i - 1] >> 3)) ? 1u : 0u;
+    start[i] = (i < *d_n && (i == 0 || (codes[i] >> 3) != (codes[i - 1] >> 3))) ? 1u : 0u;
 }
 
 // pstart = exclusive scan of start: the parent of item i is pstart[i] + start[i] - 1.
 __global__ __launch_bounds__(kThreads) void kParents(const uint32_t* __restrict__ codes, const uint32_t* __restrict__ id, const uint32_t* __restrict__ start,
-    const uint32_t* __restrict__ pstart, uint32_t n, uint32_t* __restrict__ pkeys, uint32_t* __restrict__ pcodes)
+    const uint32_t* __restrict__ pstart, const uint32_t* __restrict__ d_n, uint32_t* __restrict__ pkeys, uint32_t* __restrict__ pcodes)
 {
     const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= n)
+    if (i >= *d_n)
         return;
     const uint32_t p = pstart[i] + start[i] - 1;
     pkeys[(size_t)p * 8 + (codes[i] & 7u)] = id[i];
@@ -188,90 +196,125 @@ __global__ __launch_bounds__(kThreads) void kParents(const uint32_t* __restrict_
         pcodes[p] = codes[i] >> 3;
 }
 
-// Stream-ordered device allocations of one build, released together.
-struct Arena {
-    hipStream_t stream;
-    std::vector<void*> blocks;
-    hipError_t err = hipSuccess;
+// Carves the build's buffers out of one workspace, 256-byte aligned. With a null base it only
+// adds up the size, so the same code path first sizes the workspace, then runs.
+struct Carver {
+    char* base = nullptr;
+    size_t off = 0;
     template <typename T>
     T* get(size_t count)
     {
-        void* p = nullptr;
-        if (err == hipSuccess)
-            err = hipMallocAsync(&p, std::max<size_t>(count * sizeof(T), 256), stream);
-        if (err != hipSuccess)
-            return nullptr;
-        blocks.push_back(p);
-        return (T*)p;
-    }
-    void release(void* p)
-    {
-        auto it = std::find(blocks.begin(), blocks.end(), p);
-        if (it != blocks.end()) {
-            (void)hipFreeAsync(p, stream);
-            blocks.erase(it);
-        }
-    }
-    ~Arena()
-    {
-        for (void* p : blocks)
-            (void)hipFreeAsync(p, stream);
-        (void)hipStreamSynchronize(stream);
+        off = (off + 255) & ~size_t(255);
+        T* p = base ? (T*)(base + off) : nullptr;
+        off += std::max<size_t>(count * sizeof(T), 4);
+        return p;
     }
 };
 
 inline uint32_t gridFor(uint64_t n) { return (uint32_t)((n + kThreads - 1) / kThreads); }
 
-// First-occurrence dedup of n keys of K words: id[i] (unique number of item i) and the unique keys
-// in first-occurrence order, copied to `hostOut` (nu * K words).
+// First-occurrence dedup of the d_n[0] (<= bound) keys of K words: id[i] is the unique number of
+// item i, uniqueKeys receives the uniques in first-occurrence order and d_nu their count. No host
+// synchronisation: every launch covers the bound and reads the count on the device.
 template <int K>
-hipError_t dedupLevel(Arena& A, const uint32_t* keys, uint32_t n, uint32_t* id, std::vector<uint32_t>& hostOut, uint32_t* d_total, hipStream_t st)
+hipError_t dedupLevel(Carver& A, const uint32_t* keys, const uint32_t* d_n, uint32_t bound, uint32_t* id, uint32_t* uniqueKeys, uint32_t* d_nu, hipStream_t st)
 {
     uint32_t cap = 64;
-    while (cap < 2 * (uint64_t)n)
+    while (cap < 2 * (uint64_t)bound)
         cap <<= 1;
     uint32_t* table = A.get<uint32_t>(cap);
-    uint32_t* rep = A.get<uint32_t>(n);
-    uint32_t* first = A.get<uint32_t>(n);
-    uint32_t* uid = A.get<uint32_t>(n);
-    void* scan = A.get<uint8_t>(scanScratchBytes(n));
-    if (A.err != hipSuccess)
-        return A.err;
+    uint32_t* rep = A.get<uint32_t>(bound);
+    uint32_t* first = A.get<uint32_t>(bound);
+    uint32_t* uid = A.get<uint32_t>(bound);
+    void* scan = A.get<uint8_t>(scanScratchBytes(bound));
+    if (!A.base)
+        return hipSuccess;
     hipError_t e;
     if ((e = fillU32(table, cap, kNone, st)) != hipSuccess)
         return e;
-    hipLaunchKernelGGL(kInsertFirst<K>, dim3(gridFor(n)), dim3(kThreads), 0, st, keys, n, table, cap - 1);
-    hipLaunchKernelGGL(kRepresentative<K>, dim3(gridFor(n)), dim3(kThreads), 0, st, keys, n, table, cap - 1, rep, first);
+    hipLaunchKernelGGL(kInsertFirst<K>, dim3(gridFor(bound)), dim3(kThreads), 0, st, keys, d_n, table, cap - 1);
+    hipLaunchKernelGGL(kRepresentative<K>, dim3(gridFor(bound)), dim3(kThreads), 0, st, keys, d_n, bound, table, cap - 1, rep, first);
     if ((e = hipGetLastError()) != hipSuccess)
         return e;
-    if ((e = exclusiveScanU32(first, uid, n, d_total, scan, st)) != hipSuccess)
+    if ((e = exclusiveScanU32(first, uid, bound, d_nu, scan, st)) != hipSuccess)
         return e;
-    uint32_t nu = 0;
-    if ((e = hipMemcpyAsync(&nu, d_total, 4, hipMemcpyDeviceToHost, st)) != hipSuccess || (e = hipStreamSynchronize(st)) != hipSuccess)
-        return e;
-    uint32_t* uniqueKeys = A.get<uint32_t>((size_t)nu * K);
-    if (A.err != hipSuccess)
-        return A.err;
-    hipLaunchKernelGGL(kAssignIds<K>, dim3(gridFor(n)), dim3(kThreads), 0, st, keys, n, rep, uid, id, uniqueKeys);
-    if ((e = hipGetLastError()) != hipSuccess)
-        return e;
-    hostOut.resize((size_t)nu * K);
-    if ((e = hipMemcpyAsync(hostOut.data(), uniqueKeys, (size_t)nu * K * 4, hipMemcpyDeviceToHost, st)) != hipSuccess)
-        return e;
-    if ((e = hipStreamSynchronize(st)) != hipSuccess)
-        return e;
-    A.release(table);
-    A.release(rep);
-    A.release(first);
-    A.release(uid);
-    A.release(scan);
-    A.release(uniqueKeys);
+    hipLaunchKernelGGL(kAssignIds<K>, dim3(gridFor(bound)), dim3(kThreads), 0, st, keys, d_n, rep, uid, id, uniqueKeys);
+    return hipGetLastError();
+}
+
+// The whole build against the per-level bounds; with a null carver base nothing is launched and
+// the carver only sizes the workspace.
+hipError_t enqueueBuild(Carver& A, const uint8_t* d_bits, uint32_t R, uint32_t levels, hipStream_t st, uint32_t*& d_counts, std::vector<uint32_t*>& uniqueKeys)
+{
+    const uint64_t B = R / 4;
+    const uint64_t numBlocks = B * B * B;
+    // Level l holds at most one node per position: bound[l] = numBlocks / 8^(l - 2). The counts
+    // stay on the device until the end.
+    std::vector<uint32_t> bound(levels + 1, 0);
+    for (uint32_t l = 2; l <= levels; ++l)
+        bound[l] = (uint32_t)(numBlocks >> (3 * (l - 2)));
+    const bool run = A.base != nullptr;
+    d_counts = A.get<uint32_t>(2 * (levels + 1)); // [2l] items, [2l + 1] uniques of level l
+    uint64_t* masks = A.get<uint64_t>(numBlocks);
+    uint32_t* flags = A.get<uint32_t>(numBlocks);
+    uint32_t* pos = A.get<uint32_t>(numBlocks);
+    void* scan = A.get<uint8_t>(scanScratchBytes(numBlocks));
+    uint32_t* codes = A.get<uint32_t>(numBlocks);
+    uint32_t* keys = A.get<uint32_t>(numBlocks * 2);
+    hipError_t e;
+    if (run) {
+        if ((e = fillU32(d_counts, 2 * (levels + 1), 0u, st)) != hipSuccess)
+            return e;
+        hipLaunchKernelGGL(kLeafMasks, dim3(gridFor(numBlocks)), dim3(kThreads), 0, st, d_bits, R, numBlocks, masks, flags);
+        if ((e = hipGetLastError()) != hipSuccess)
+            return e;
+        if ((e = exclusiveScanU32(flags, pos, numBlocks, d_counts + 4, scan, st)) != hipSuccess)
+            return e;
+        hipLaunchKernelGGL(kCompactLeaves, dim3(gridFor(numBlocks)), dim3(kThreads), 0, st, masks, flags, pos, numBlocks, codes, keys);
+        if ((e = hipGetLastError()) != hipSuccess)
+            return e;
+    }
+    uniqueKeys.assign(levels + 1, nullptr);
+    for (uint32_t level = 2;; ++level) {
+        const uint32_t K = level == 2 ? 2 : 8;
+        const uint32_t* d_n = run ? d_counts + 2 * level : nullptr;
+        uint32_t* d_nu = run ? d_counts + 2 * level + 1 : nullptr;
+        uint32_t* id = A.get<uint32_t>(bound[level]);
+        uniqueKeys[level] = A.get<uint32_t>((size_t)bound[level] * K);
+        e = level == 2 ? dedupLevel<2>(A, keys, d_n, bound[level], id, uniqueKeys[level], d_nu, st)
+                       : dedupLevel<8>(A, keys, d_n, bound[level], id, uniqueKeys[level], d_nu, st);
+        if (e != hipSuccess)
+            return e;
+        if (level == levels)
+            break;
+        // Parents at level + 1: runs of equal code >> 3.
+        const uint32_t nb = bound[level], pb = bound[level + 1];
+        uint32_t* start = A.get<uint32_t>(nb);
+        uint32_t* pstart = A.get<uint32_t>(nb);
+        void* pscan = A.get<uint8_t>(scanScratchBytes(nb));
+        uint32_t* pkeys = A.get<uint32_t>((size_t)pb * 8);
+        uint32_t* pcodes = A.get<uint32_t>(pb);
+        if (run) {
+            hipLaunchKernelGGL(kParentStarts, dim3(gridFor(nb)), dim3(kThreads), 0, st, codes, d_n, nb, start);
+            if ((e = hipGetLastError()) != hipSuccess)
+                return e;
+            if ((e = exclusiveScanU32(start, pstart, nb, d_counts + 2 * (level + 1), pscan, st)) != hipSuccess)
+                return e;
+            if ((e = fillU32(pkeys, (size_t)pb * 8, kNone, st)) != hipSuccess)
+                return e;
+            hipLaunchKernelGGL(kParents, dim3(gridFor(nb)), dim3(kThreads), 0, st, codes, id, start, pstart, d_n, pkeys, pcodes);
+            if ((e = hipGetLastError()) != hipSuccess)
+                return e;
+        }
+        codes = pcodes;
+        keys = pkeys;
+    }
     return hipSuccess;
 }
 
 } // namespace
 
-hipError_t svdagFromDenseDevice(const uint8_t* d_bits, uint32_t R, hipStream_t st, Svdag& out)
+hipError_t svdagFromDenseDevice(const uint8_t* d_bits, uint32_t R, hipStream_t st, void*& scratch, size_t& scratchBytes, Svdag& out)
 {
     uint32_t levels = 0;
     while ((1u << levels) < R)
@@ -281,91 +324,41 @@ hipError_t svdagFromDenseDevice(const uint8_t* d_bits, uint32_t R, hipStream_t s
     out.rootLevel = levels;
     out.nodes.resize(levels + 1);
     out.structureType = 1;
-    Arena A { st };
-    const uint64_t B = R / 4;
-    const uint64_t numBlocks = B * B * B;
-    uint32_t* d_total = A.get<uint32_t>(1);
-    uint64_t* masks = A.get<uint64_t>(numBlocks);
-    uint32_t* flags = A.get<uint32_t>(numBlocks);
-    uint32_t* pos = A.get<uint32_t>(numBlocks);
-    void* scan = A.get<uint8_t>(scanScratchBytes(numBlocks));
-    if (A.err != hipSuccess)
-        return A.err;
-    hipError_t e;
-    hipLaunchKernelGGL(kLeafMasks, dim3(gridFor(numBlocks)), dim3(kThreads), 0, st, d_bits, R, numBlocks, masks, flags);
-    if ((e = hipGetLastError()) != hipSuccess)
+    uint32_t* d_counts = nullptr;
+    std::vector<uint32_t*> uniqueKeys;
+    Carver sizing;
+    hipError_t e = enqueueBuild(sizing, d_bits, R, levels, st, d_counts, uniqueKeys);
+    if (e != hipSuccess)
         return e;
-    if ((e = exclusiveScanU32(flags, pos, numBlocks, d_total, scan, st)) != hipSuccess)
-        return e;
-    uint32_t n = 0;
-    if ((e = hipMemcpyAsync(&n, d_total, 4, hipMemcpyDeviceToHost, st)) != hipSuccess || (e = hipStreamSynchronize(st)) != hipSuccess)
-        return e;
-    if (n == 0)
-        return hipSuccess;
-    uint32_t* codes = A.get<uint32_t>(n);
-    uint32_t* keys = A.get<uint32_t>((size_t)n * 2);
-    if (A.err != hipSuccess)
-        return A.err;
-    hipLaunchKernelGGL(kCompactLeaves, dim3(gridFor(numBlocks)), dim3(kThreads), 0, st, masks, flags, pos, numBlocks, codes, keys);
-    if ((e = hipGetLastError()) != hipSuccess)
-        return e;
-    A.release(masks);
-    A.release(flags);
-    A.release(pos);
-    A.release(scan);
-    std::vector<uint32_t> host;
-    for (uint32_t level = 2;; ++level) {
-        uint32_t* id = A.get<uint32_t>(n);
-        if (A.err != hipSuccess)
-            return A.err;
-        if (level == 2) {
-            if ((e = dedupLevel<2>(A, keys, n, id, host, d_total, st)) != hipSuccess)
+    if (scratchBytes < sizing.off || !scratch) {
+        if (scratch) {
+            if ((e = hipStreamSynchronize(st)) != hipSuccess)
                 return e;
-            out.leaves.resize(host.size() / 2);
-            std::copy((const uint8_t*)host.data(), (const uint8_t*)(host.data() + host.size()), (uint8_t*)out.leaves.data());
-        } else {
-            if ((e = dedupLevel<8>(A, keys, n, id, host, d_total, st)) != hipSuccess)
-                return e;
-            auto& nodes = out.nodes[level];
-            nodes.resize(host.size() / 8);
-            std::copy(host.begin(), host.end(), &nodes.data()[0][0]);
+            (void)hipFree(scratch);
+            scratch = nullptr;
+            scratchBytes = 0;
         }
-        if (level == levels)
-            break;
-        // Parents at level + 1: runs of equal code >> 3.
-        uint32_t* start = A.get<uint32_t>(n);
-        uint32_t* pstart = A.get<uint32_t>(n);
-        void* pscan = A.get<uint8_t>(scanScratchBytes(n));
-        if (A.err != hipSuccess)
-            return A.err;
-        hipLaunchKernelGGL(kParentStarts, dim3(gridFor(n)), dim3(kThreads), 0, st, codes, n, start);
-        if ((e = hipGetLastError()) != hipSuccess)
+        if ((e = hipMalloc(&scratch, sizing.off)) != hipSuccess)
             return e;
-        if ((e = exclusiveScanU32(start, pstart, n, d_total, pscan, st)) != hipSuccess)
-            return e;
-        uint32_t np = 0;
-        if ((e = hipMemcpyAsync(&np, d_total, 4, hipMemcpyDeviceToHost, st)) != hipSuccess || (e = hipStreamSynchronize(st)) != hipSuccess)
-            return e;
-        uint32_t* pkeys = A.get<uint32_t>((size_t)np * 8);
-        uint32_t* pcodes = A.get<uint32_t>(np);
-        if (A.err != hipSuccess)
-            return A.err;
-        if ((e = fillU32(pkeys, (size_t)np * 8, kNone, st)) != hipSuccess)
-            return e;
-        hipLaunchKernelGGL(kParents, dim3(gridFor(n)), dim3(kThreads), 0, st, codes, id, start, pstart, n, pkeys, pcodes);
-        if ((e = hipGetLastError()) != hipSuccess)
-            return e;
-        A.release(start);
-        A.release(pstart);
-        A.release(pscan);
-        A.release(id);
-        A.release(codes);
-        A.release(keys);
-        codes = pcodes;
-        keys = pkeys;
-        n = np;
+        scratchBytes = sizing.off;
     }
-    return hipSuccess;
+    Carver A { (char*)scratch, 0 };
+    if ((e = enqueueBuild(A, d_bits, R, levels, st, d_counts, uniqueKeys)) != hipSuccess)
+        return e;
+    // One readback of the counts, then the unique lists.
+    std::vector<uint32_t> counts(2 * (levels + 1));
+    if ((e = hipMemcpyAsync(counts.data(), d_counts, counts.size() * 4, hipMemcpyDeviceToHost, st)) != hipSuccess || (e = hipStreamSynchronize(st)) != hipSuccess)
+        return e;
+    out.leaves.resize(counts[5]);
+    if (counts[5] && (e = hipMemcpyAsync(out.leaves.data(), uniqueKeys[2], (size_t)counts[5] * 8, hipMemcpyDeviceToHost, st)) != hipSuccess)
+        return e;
+    for (uint32_t level = 3; level <= levels; ++level) {
+        const uint32_t nu = counts[2 * level + 1];
+        out.nodes[level].resize(nu);
+        if (nu && (e = hipMemcpyAsync(out.nodes[level].data(), uniqueKeys[level], (size_t)nu * 32, hipMemcpyDeviceToHost, st)) != hipSuccess)
+            return e;
+    }
+    return hipStreamSynchronize(st);
 }
 
 } // namespace tdag
