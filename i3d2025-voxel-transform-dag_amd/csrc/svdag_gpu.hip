@@ -67,6 +67,38 @@ __global__ __launch_bounds__(kThreads) void kLeafMasks(const uint8_t* __restrict
     flags[code] = mask != 0 ? 1u : 0u;
 }
 
+// One workgroup per 32^3-voxel cube = 8^3 blocks = the aligned Morton range [cube * 512, +512):
+// the cube's 32 x 32 rows of 32 voxels are 1024 aligned 32-bit words (bit k = voxel x0 + k,
+// little-endian bytes), staged in LDS, then each thread builds two blocks' masks from nibbles.
+// Needs R >= 32 and a 4-byte aligned grid.
+__global__ __launch_bounds__(kThreads) void kLeafMasksCube(const uint32_t* __restrict__ words, uint32_t R, uint64_t* __restrict__ masks,
+    uint32_t* __restrict__ flags)
+{
+    __shared__ uint32_t rows[32][33]; // [z][y], padded against bank conflicts
+    const uint32_t cube = blockIdx.x;
+    const uint32_t cx = compact3(cube), cy = compact3(cube >> 1), cz = compact3(cube >> 2);
+    const uint64_t wordsPerRow = R / 32;
+    for (uint32_t k = threadIdx.x; k < 1024; k += kThreads) {
+        const uint32_t y = k & 31u, z = k >> 5;
+        rows[z][y] = words[((uint64_t)(32 * cz + z) * R + (32 * cy + y)) * wordsPerRow + cx];
+    }
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < 512; j += kThreads) {
+        const uint32_t bx = compact3(j), by = compact3(j >> 1), bz = compact3(j >> 2);
+        uint64_t mask = 0;
+#pragma unroll
+        for (uint32_t z = 0; z < 4; ++z)
+#pragma unroll
+            for (uint32_t y = 0; y < 4; ++y) {
+                const uint32_t nib = (rows[4 * bz + z][4 * by + y] >> (4 * bx)) & 0xFu;
+                mask |= nibbleToMorton(nib) << (spread2(y) << 1 | spread2(z) << 2);
+            }
+        const uint64_t code = (uint64_t)cube << 9 | j;
+        masks[code] = mask;
+        flags[code] = mask != 0 ? 1u : 0u;
+    }
+}
+
 __global__ __launch_bounds__(kThreads) void kCompactLeaves(const uint64_t* __restrict__ masks, const uint32_t* __restrict__ flags,
     const uint32_t* __restrict__ pos, uint64_t numBlocks, uint32_t* __restrict__ codes, uint32_t* __restrict__ keys)
 {
@@ -216,34 +248,44 @@ inline uint32_t gridFor(uint64_t n) { return (uint32_t)((n + kThreads - 1) / kTh
 // First-occurrence dedup of the d_n[0] (<= bound) keys of K words: id[i] is the unique number of
 // item i, uniqueKeys receives the uniques in first-occurrence order and d_nu their count. No host
 // synchronisation: every launch covers the bound and reads the count on the device.
-template <int K>
-hipError_t dedupLevel(Carver& A, const uint32_t* keys, const uint32_t* d_n, uint32_t bound, uint32_t* id, uint32_t* uniqueKeys, uint32_t* d_nu, hipStream_t st)
+inline uint32_t tableCap(uint32_t items)
 {
     uint32_t cap = 64;
-    while (cap < 2 * (uint64_t)bound)
+    while (cap < 2 * (uint64_t)items)
         cap <<= 1;
-    uint32_t* table = A.get<uint32_t>(cap);
+    return cap;
+}
+
+// Buffers are carved for `bound` items; the launches, the scan and the table cover `work` items
+// (work <= bound: the level's count is at most work, which the host learned after the leaves).
+template <int K>
+hipError_t dedupLevel(Carver& A, const uint32_t* keys, const uint32_t* d_n, uint32_t bound, uint32_t work, uint32_t* id, uint32_t* uniqueKeys, uint32_t* d_nu,
+    hipStream_t st)
+{
+    uint32_t* table = A.get<uint32_t>(tableCap(bound));
     uint32_t* rep = A.get<uint32_t>(bound);
     uint32_t* first = A.get<uint32_t>(bound);
     uint32_t* uid = A.get<uint32_t>(bound);
     void* scan = A.get<uint8_t>(scanScratchBytes(bound));
-    if (!A.base)
+    if (!A.base || work == 0)
         return hipSuccess;
+    const uint32_t cap = tableCap(work);
     hipError_t e;
     if ((e = fillU32(table, cap, kNone, st)) != hipSuccess)
         return e;
-    hipLaunchKernelGGL(kInsertFirst<K>, dim3(gridFor(bound)), dim3(kThreads), 0, st, keys, d_n, table, cap - 1);
-    hipLaunchKernelGGL(kRepresentative<K>, dim3(gridFor(bound)), dim3(kThreads), 0, st, keys, d_n, bound, table, cap - 1, rep, first);
+    hipLaunchKernelGGL(kInsertFirst<K>, dim3(gridFor(work)), dim3(kThreads), 0, st, keys, d_n, table, cap - 1);
+    hipLaunchKernelGGL(kRepresentative<K>, dim3(gridFor(work)), dim3(kThreads), 0, st, keys, d_n, work, table, cap - 1, rep, first);
     if ((e = hipGetLastError()) != hipSuccess)
         return e;
-    if ((e = exclusiveScanU32(first, uid, bound, d_nu, scan, st)) != hipSuccess)
+    if ((e = exclusiveScanU32(first, uid, work, d_nu, scan, st)) != hipSuccess)
         return e;
-    hipLaunchKernelGGL(kAssignIds<K>, dim3(gridFor(bound)), dim3(kThreads), 0, st, keys, d_n, rep, uid, id, uniqueKeys);
+    hipLaunchKernelGGL(kAssignIds<K>, dim3(gridFor(work)), dim3(kThreads), 0, st, keys, d_n, rep, uid, id, uniqueKeys);
     return hipGetLastError();
 }
 
-// The whole build against the per-level bounds; with a null carver base nothing is launched and
-// the carver only sizes the workspace.
+// The whole build. Buffers are carved against the per-level position bounds; with a null carver
+// base nothing is launched and the carver only sizes the workspace. The launches cover at most the
+// number of non-empty leaves, read back once after the leaf compaction.
 hipError_t enqueueBuild(Carver& A, const uint8_t* d_bits, uint32_t R, uint32_t levels, hipStream_t st, uint32_t*& d_counts, std::vector<uint32_t*>& uniqueKeys)
 {
     const uint64_t B = R / 4;
@@ -262,16 +304,23 @@ hipError_t enqueueBuild(Carver& A, const uint8_t* d_bits, uint32_t R, uint32_t l
     uint32_t* codes = A.get<uint32_t>(numBlocks);
     uint32_t* keys = A.get<uint32_t>(numBlocks * 2);
     hipError_t e;
+    uint32_t leaves = 0; // non-empty blocks: every level has at most this many nodes
     if (run) {
         if ((e = fillU32(d_counts, 2 * (levels + 1), 0u, st)) != hipSuccess)
             return e;
-        hipLaunchKernelGGL(kLeafMasks, dim3(gridFor(numBlocks)), dim3(kThreads), 0, st, d_bits, R, numBlocks, masks, flags);
+        if (R >= 32 && ((uintptr_t)d_bits & 3u) == 0)
+            hipLaunchKernelGGL(kLeafMasksCube, dim3((uint32_t)(numBlocks / 512)), dim3(kThreads), 0, st, (const uint32_t*)d_bits, R, masks, flags);
+        else
+            hipLaunchKernelGGL(kLeafMasks, dim3(gridFor(numBlocks)), dim3(kThreads), 0, st, d_bits, R, numBlocks, masks, flags);
         if ((e = hipGetLastError()) != hipSuccess)
             return e;
         if ((e = exclusiveScanU32(flags, pos, numBlocks, d_counts + 4, scan, st)) != hipSuccess)
             return e;
         hipLaunchKernelGGL(kCompactLeaves, dim3(gridFor(numBlocks)), dim3(kThreads), 0, st, masks, flags, pos, numBlocks, codes, keys);
         if ((e = hipGetLastError()) != hipSuccess)
+            return e;
+        // The one mid-build readback: it sizes the work of every later launch.
+        if ((e = hipMemcpyAsync(&leaves, d_counts + 4, 4, hipMemcpyDeviceToHost, st)) != hipSuccess || (e = hipStreamSynchronize(st)) != hipSuccess)
             return e;
     }
     uniqueKeys.assign(levels + 1, nullptr);
@@ -281,28 +330,30 @@ hipError_t enqueueBuild(Carver& A, const uint8_t* d_bits, uint32_t R, uint32_t l
         uint32_t* d_nu = run ? d_counts + 2 * level + 1 : nullptr;
         uint32_t* id = A.get<uint32_t>(bound[level]);
         uniqueKeys[level] = A.get<uint32_t>((size_t)bound[level] * K);
-        e = level == 2 ? dedupLevel<2>(A, keys, d_n, bound[level], id, uniqueKeys[level], d_nu, st)
-                       : dedupLevel<8>(A, keys, d_n, bound[level], id, uniqueKeys[level], d_nu, st);
+        const uint32_t work = std::min(bound[level], leaves);
+        e = level == 2 ? dedupLevel<2>(A, keys, d_n, bound[level], work, id, uniqueKeys[level], d_nu, st)
+                       : dedupLevel<8>(A, keys, d_n, bound[level], work, id, uniqueKeys[level], d_nu, st);
         if (e != hipSuccess)
             return e;
         if (level == levels)
             break;
         // Parents at level + 1: runs of equal code >> 3.
         const uint32_t nb = bound[level], pb = bound[level + 1];
+        const uint32_t pwork = std::min(pb, leaves);
         uint32_t* start = A.get<uint32_t>(nb);
         uint32_t* pstart = A.get<uint32_t>(nb);
         void* pscan = A.get<uint8_t>(scanScratchBytes(nb));
         uint32_t* pkeys = A.get<uint32_t>((size_t)pb * 8);
         uint32_t* pcodes = A.get<uint32_t>(pb);
-        if (run) {
-            hipLaunchKernelGGL(kParentStarts, dim3(gridFor(nb)), dim3(kThreads), 0, st, codes, d_n, nb, start);
+        if (run && work > 0) {
+            hipLaunchKernelGGL(kParentStarts, dim3(gridFor(work)), dim3(kThreads), 0, st, codes, d_n, work, start);
             if ((e = hipGetLastError()) != hipSuccess)
                 return e;
-            if ((e = exclusiveScanU32(start, pstart, nb, d_counts + 2 * (level + 1), pscan, st)) != hipSuccess)
+            if ((e = exclusiveScanU32(start, pstart, work, d_counts + 2 * (level + 1), pscan, st)) != hipSuccess)
                 return e;
-            if ((e = fillU32(pkeys, (size_t)pb * 8, kNone, st)) != hipSuccess)
+            if ((e = fillU32(pkeys, (size_t)pwork * 8, kNone, st)) != hipSuccess)
                 return e;
-            hipLaunchKernelGGL(kParents, dim3(gridFor(nb)), dim3(kThreads), 0, st, codes, id, start, pstart, d_n, pkeys, pcodes);
+            hipLaunchKernelGGL(kParents, dim3(gridFor(work)), dim3(kThreads), 0, st, codes, id, start, pstart, d_n, pkeys, pcodes);
             if ((e = hipGetLastError()) != hipSuccess)
                 return e;
         }

@@ -2,7 +2,8 @@
 CPU tests pin node for node to the oracle's restatement of the reference's recursive
 EditStructure(VoxelGrid) constructor + toDAG (structure.h:658-763): same leaves, same nodes, same
 indices, at every level; edge cases are the empty grid, the full grid, the smallest resolution and
-a noise grid whose leaves are nearly all distinct."""
+a noise grid whose leaves are nearly all distinct. Grids of 32^3 and more with a 4-byte aligned
+pointer take the cube-tiled leaf kernel; 4^3..16^3 and unaligned pointers take the per-block one."""
 import numpy as np
 import pytest
 
@@ -32,18 +33,20 @@ def test_device_svdag_equals_host(gpu_ctx, kind, R, seed):
     _same(pl.Svdag(bits, R), pl.Svdag.from_device(gpu_ctx, _device_bits(bits), R))
 
 
-@pytest.mark.parametrize("case", ["empty", "full", "R4", "noise"])
+@pytest.mark.parametrize("case", ["empty", "full", "R4", "R16", "noise"])
 def test_device_svdag_edge_cases(gpu_ctx, case):
     from tdag_amd import pipeline as pl
 
     print(f"device svdag edge case {case}", flush=True)
-    R = 4 if case == "R4" else 64
+    R = {"R4": 4, "R16": 16}.get(case, 64)
     if case == "empty":
         bits = np.zeros(R ** 3 // 8, dtype=np.uint8)
     elif case == "full":
         bits = np.full(R ** 3 // 8, 0xFF, dtype=np.uint8)
     else:
         bits = np.random.default_rng(11).integers(0, 256, R ** 3 // 8, dtype=np.uint8)
+        if case == "R16":
+            bits &= np.uint8(0x5A)  # sparser, so some blocks stay empty
     host = pl.Svdag(bits, R)
     dev = pl.Svdag.from_device(gpu_ctx, _device_bits(bits), R)
     _same(host, dev)
@@ -61,3 +64,17 @@ def test_device_svdag_feeds_the_driver(oracle, gpu_ctx):
     a = pl.construct_transform_dag(gpu_ctx, pl.Svdag(bits, 128), cfg).tdag_bytes()
     b = pl.construct_transform_dag(gpu_ctx, pl.Svdag.from_device(gpu_ctx, _device_bits(bits), 128), cfg).tdag_bytes()
     assert a == b
+
+
+def test_device_svdag_unaligned_grid(gpu_ctx):
+    """A grid pointer that is not 4-byte aligned is read by the per-block kernel."""
+    import torch
+    from tdag_amd import pipeline as pl
+
+    print("device svdag unaligned grid", flush=True)
+    bits = pl.scene_dense(1, 128, 13)
+    buf = torch.zeros(bits.size + 1, dtype=torch.uint8, device="cuda")
+    buf[1:] = torch.from_numpy(bits).cuda()
+    view = buf[1:]
+    assert view.data_ptr() % 4 != 0
+    _same(pl.Svdag(bits, 128), pl.Svdag.from_device(gpu_ctx, view, 128))
