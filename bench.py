@@ -82,7 +82,9 @@ def build_workload(ctx, resolution: int, seed: int, max_level: int, kind: int = 
     host_build_ms = (time.perf_counter() - th) * 1e3
     svdag_build = device_svdag_build(ctx, bits, resolution, svdag, host_build_ms)
     cfg = tdag_amd.TransformDAGConfig(True, True, True, max_level)
+    tc = time.perf_counter()
     tdag = pl.construct_transform_dag(ctx, svdag, cfg, keep_level_inputs=True)
+    construct_ms = (time.perf_counter() - tc) * 1e3
     levels = {}
     for level in range(max_level, 1, -1):
         masks = tdag.level_masks(level)
@@ -95,6 +97,9 @@ def build_workload(ctx, resolution: int, seed: int, max_level: int, kind: int = 
         "tdag_level_sizes": tdag.level_sizes,
         "setup_s": round(time.time() - t0, 2),
         "svdag_build": svdag_build,
+        # Wall time of one whole constructTransformDAG call (host hierarchical pass, device
+        # flattening, one matcher call per level, pointer rewrites), first call of the process.
+        "construct_ms": round(construct_ms, 1),
     }
     return cfg, levels, info
 
@@ -462,6 +467,7 @@ def main():
             "tdag_bytes": info["tdag_bytes"],
             "matches_per_step": per_step_matches,
             **({"svdag_build": info["svdag_build"]} if info.get("svdag_build") else {}),
+            "construct_transform_dag_ms": info["construct_ms"],
             "levels": per_level,
             "search_variants": {l: ctx.get_variant(l) for l in order if l >= 3},
             "autotune": tuning,

@@ -15,10 +15,19 @@
 #include "context.hpp"
 #include "errors.hpp"
 #include "pipeline.hpp"
+#include "primitives.hpp"
 #include "tdag.h"
 
 struct tdag_svdag {
     tdag::Svdag s;
+    // Device copy of levels 2..5 for tdag_svdag_flatten_device and the driver, made on first use.
+    tdag::DeviceSvdag dev;
+    int devDevice = -1;
+    ~tdag_svdag()
+    {
+        if (devDevice >= 0 && hipSetDevice(devDevice) == hipSuccess)
+            tdag::freeDeviceSvdag(dev);
+    }
 };
 
 struct tdag_tdag {
@@ -44,6 +53,58 @@ std::array<uint64_t, 8> toTransformNode(const std::array<uint32_t, 8>& n) // tra
 }
 
 tdag_status setErr(tdag_status s, const std::string& m) { return tdag::setError(s, m); }
+
+tdag_status setHipErr(hipError_t e, const std::string& what)
+{
+    return tdag::setError(e == hipErrorOutOfMemory ? TDAG_ENOMEM : TDAG_EDEVICE, what + ": " + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ")");
+}
+
+// The SVDAG's device copy on ctx's device (levels 2..min(5, root)).
+tdag_status ensureDeviceCopy(tdag_context* ctx, const tdag_svdag* h)
+{
+    tdag_svdag* m = const_cast<tdag_svdag*>(h);
+    if (m->devDevice == ctx->device)
+        return TDAG_OK;
+    if (m->devDevice >= 0) {
+        if (hipError_t e = hipSetDevice(m->devDevice); e == hipSuccess)
+            tdag::freeDeviceSvdag(m->dev);
+        m->devDevice = -1;
+    }
+    if (hipError_t e = hipSetDevice(ctx->device); e != hipSuccess)
+        return setHipErr(e, "hipSetDevice");
+    if (hipError_t e = tdag::uploadSvdag(m->s, 5, m->dev); e != hipSuccess) {
+        tdag::freeDeviceSvdag(m->dev);
+        return setHipErr(e, "SVDAG upload");
+    }
+    m->devDevice = ctx->device;
+    return TDAG_OK;
+}
+
+// Device buffers of one driver level, grown on demand and freed with the driver call.
+struct LevelBuffers {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~LevelBuffers()
+    {
+        if (p)
+            (void)hipFree(p);
+    }
+    hipError_t ensure(size_t need)
+    {
+        if (need <= bytes && p)
+            return hipSuccess;
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        const hipError_t e = hipMalloc(&p, need);
+        if (e == hipSuccess)
+            bytes = need;
+        return e;
+    }
+};
+
+inline size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 
 } // namespace
 
@@ -137,10 +198,82 @@ tdag_status tdag_svdag_flatten(const tdag_svdag* h, uint32_t level, const uint32
     return TDAG_OK;
 }
 
+tdag_status tdag_svdag_flatten_device(tdag_context* ctx, const tdag_svdag* h, uint32_t level, const uint32_t* d_nodes, uint64_t n, uint64_t* d_out_masks)
+{
+    if (!ctx || !h || (n && (!d_nodes || !d_out_masks)))
+        return setErr(TDAG_EINVAL, "null argument");
+    if (level < 2 || level > h->s.rootLevel || level > 5)
+        return setErr(TDAG_EINVAL, "level must be in [2, min(5, rootLevel)]");
+    if (n == 0)
+        return TDAG_OK;
+    if (tdag_status s = ensureDeviceCopy(ctx, h); s != TDAG_OK)
+        return s;
+    const size_t temp = align256(tdag::flattenTempBytes(level, n));
+    if (tdag_status s = tdag::capi::ensureBuffer(ctx->svdagScratch, ctx->svdagScratchBytes, temp + 256, "flatten scratch"); s != TDAG_OK)
+        return s;
+    uint64_t* d_temp = (uint64_t*)ctx->svdagScratch;
+    uint32_t* d_bad = (uint32_t*)((char*)ctx->svdagScratch + temp);
+    hipError_t e = tdag::fillU32(d_bad, 1, 0u, ctx->stream);
+    if (e == hipSuccess)
+        e = tdag::flattenDevice(h->dev, level, d_nodes, n, d_out_masks, d_temp, ctx->stream, d_bad);
+    uint32_t bad = 0;
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(&bad, d_bad, 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess)
+        return setHipErr(e, "device flatten");
+    if (bad)
+        return setErr(TDAG_EINVAL, "node index out of range");
+    return TDAG_OK;
+}
+
 tdag_status tdag_svdag_free(tdag_svdag* h)
 {
     delete h;
     return TDAG_OK;
+}
+
+// One exhaustive matcher level of the driver with its input flattened on the device: `items` are
+// the referenced level-`level` nodes in ascending order. The matcher's mapping and unique list
+// come back to the host; the masks only when they are kept.
+static tdag_status matchLevelOnDevice(tdag_context* ctx, const tdag_svdag* in, uint32_t level, const tdag_config* cfg, const std::vector<uint32_t>& items,
+    uint64_t W, LevelBuffers& bufs, bool keepMasks, std::vector<uint64_t>& outMasks, std::vector<uint64_t>& mapping, std::vector<uint32_t>& uniq, uint64_t& nu,
+    tdag_match_stats& st)
+{
+    const uint64_t n = items.size();
+    if (n == 0)
+        return tdag_match_level_device(ctx, level, nullptr, 0, cfg, nullptr, nullptr, nullptr, &nu, nullptr, &st);
+    if (tdag_status s = ensureDeviceCopy(ctx, in); s != TDAG_OK)
+        return s;
+    const size_t oMasks = align256(n * 4), oMap = align256(oMasks + n * W * 8), oUniq = align256(oMap + n * 8), oTemp = align256(oUniq + n * 4);
+    const size_t total = oTemp + align256(tdag::flattenTempBytes(level, n));
+    if (hipError_t e = bufs.ensure(total); e != hipSuccess)
+        return setHipErr(e, "driver level buffers");
+    char* b = (char*)bufs.p;
+    uint32_t* d_items = (uint32_t*)b;
+    uint64_t* d_masks = (uint64_t*)(b + oMasks);
+    uint64_t* d_map = (uint64_t*)(b + oMap);
+    uint32_t* d_uniq = (uint32_t*)(b + oUniq);
+    hipError_t e = hipMemcpyAsync(d_items, items.data(), n * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = tdag::flattenDevice(in->dev, level, d_items, n, d_masks, (uint64_t*)(b + oTemp), ctx->stream);
+    if (e != hipSuccess)
+        return setHipErr(e, "device flatten");
+    if (tdag_status s = tdag_match_level_device(ctx, level, d_masks, n, cfg, nullptr, d_map, d_uniq, &nu, nullptr, &st); s != TDAG_OK)
+        return s;
+    mapping.resize(n);
+    uniq.resize(n);
+    e = hipMemcpyAsync(mapping.data(), d_map, n * 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(uniq.data(), d_uniq, nu * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess && keepMasks) {
+        outMasks.resize(n * W);
+        e = hipMemcpyAsync(outMasks.data(), d_masks, n * W * 8, hipMemcpyDeviceToHost, ctx->stream);
+    }
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? TDAG_OK : setHipErr(e, "driver readback");
 }
 
 // Both drivers: constructTransformDAG (transform_dag.cpp:138-240) and, with staticForm,
@@ -164,6 +297,7 @@ static tdag_status constructTransformDag(tdag_context* ctx, const tdag_svdag* in
     if (maxT + 1 > dag.rootLevel)
         return setErr(TDAG_EINVAL, "max_translation_level + 1 exceeds the root level");
     const int threads = hostThreads();
+    LevelBuffers bufs;
     tdag_tdag* res = new tdag_tdag();
     res->isStatic = staticForm;
     std::vector<uint32_t> translationParents;
@@ -196,14 +330,18 @@ static tdag_status constructTransformDag(tdag_context* ctx, const tdag_svdag* in
             tdag::LevelRecord rec;
             rec.level = level;
             rec.items = indices.size();
-            rec.masks.resize(indices.size() * W);
-            tdag::flattenNodes(dag, level, indices.data(), indices.size(), rec.masks.data(), threads);
             std::vector<uint64_t> mapping(indices.size());
             std::vector<uint32_t> uniq(indices.size());
             uint64_t nu = 0;
             tdag_match_stats st {};
             tdag_status s;
-            if (learned && level == 2) {
+            if (!learned || level != 2) {
+                // Flatten on the device straight into the matcher's input (FlatGridGenerator,
+                // transform_dag.cpp:196-204), match there, read back mapping and uniques.
+                s = matchLevelOnDevice(ctx, in, level, cfg, indices, W, bufs, (flags & TDAG_KEEP_LEVEL_INPUTS) != 0, rec.masks, mapping, uniq, nu, st);
+            } else {
+                rec.masks.resize(indices.size() * W);
+                tdag::flattenNodes(dag, level, indices.data(), indices.size(), rec.masks.data(), threads);
                 tdag_learned_stats ls {};
                 s = tdag_match_level_learned(ctx, mlp, level, rec.masks.data(), indices.size(), cfg, k, nullptr, mapping.data(), uniq.data(), &nu, &ls);
                 st.num_matched = ls.num_matched;
@@ -211,8 +349,6 @@ static tdag_status constructTransformDag(tdag_context* ctx, const tdag_svdag* in
                 rec.learned = true;
                 rec.proposalHits = ls.proposal_hits;
                 rec.fallback = ls.fallback_queries;
-            } else {
-                s = tdag_match_level(ctx, level, rec.masks.data(), indices.size(), cfg, nullptr, mapping.data(), uniq.data(), &nu, &st);
             }
             if (s != TDAG_OK) {
                 delete res;

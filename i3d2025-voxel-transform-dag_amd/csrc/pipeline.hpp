@@ -59,6 +59,23 @@ hipError_t svdagFromDenseDevice(const uint8_t* d_bits, uint32_t R, hipStream_t s
 // createLargeSubGrid for level-`level` nodes (level 2 = leaves): linear masks, W words each.
 void flattenNodes(const Svdag& s, uint32_t level, const uint32_t* nodes, uint64_t n, uint64_t* out, int threads);
 
+// Device copy of an SVDAG's leaves and node levels 3..maxLevel (maxLevel <= 5) for flattening.
+struct DeviceSvdag {
+    const uint64_t* leaves = nullptr;
+    const uint32_t* nodes[6] = { nullptr, nullptr, nullptr, nullptr, nullptr, nullptr }; // 8 children per node
+    uint32_t count[6] = { 0, 0, 0, 0, 0, 0 }; // leaves at [2], nodes at [3..5]
+    uint32_t maxLevel = 0;
+};
+hipError_t uploadSvdag(const Svdag& s, uint32_t maxLevel, DeviceSvdag& d);
+void freeDeviceSvdag(DeviceSvdag& d);
+
+// createLargeSubGrid on the device for the level-`level` nodes d_items[0..n) into d_out (n * W
+// words); d_temp holds flattenTempBytes(level, n) bytes. Enqueued on `stream`. An item index
+// outside the level reads as empty and sets *d_badItem (when given).
+size_t flattenTempBytes(uint32_t level, uint64_t n);
+hipError_t flattenDevice(const DeviceSvdag& d, uint32_t level, const uint32_t* d_items, uint64_t n, uint64_t* d_out, uint64_t* d_temp, hipStream_t stream,
+    uint32_t* d_badItem = nullptr);
+
 // constructTransformDAGHierarchical with midLevel; fills outMidInv (transform_dag_hierarchical.cpp:304-523).
 // With staticOffsets, groups are keyed as constructStaticTransformDAGHierarchical keys them
 // (transform_dag_hierarchical.cpp:531-743: children by output word offset), which changes the

@@ -57,6 +57,8 @@ def bind(L):
     L.tdag_svdag_from_dense_device.restype = ctypes.c_int
     L.tdag_svdag_copy_level.argtypes = [vp, ctypes.c_uint32, vp]
     L.tdag_svdag_flatten.argtypes = [vp, ctypes.c_uint32, _u32p, ctypes.c_uint64, _u64p]
+    L.tdag_svdag_flatten_device.argtypes = [vp, vp, ctypes.c_uint32, vp, ctypes.c_uint64, vp]
+    L.tdag_svdag_flatten_device.restype = ctypes.c_int
     L.tdag_svdag_free.argtypes = [vp]
     L.tdag_construct_transform_dag.argtypes = [vp, vp, ctypes.POINTER(TdagConfig), ctypes.c_uint32, ctypes.POINTER(vp)]
     L.tdag_construct_transform_dag_learned.argtypes = [vp, vp, ctypes.POINTER(TdagConfig), ctypes.c_uint32, vp, ctypes.c_uint32, ctypes.POINTER(vp)]
@@ -142,6 +144,19 @@ class Svdag:
         out = np.zeros((max(1, idx.size), W), dtype=np.uint64)
         check(lib().tdag_svdag_flatten(self._h, level, idx.ctypes.data_as(_u32p), idx.size, out.ctypes.data_as(_u64p)), "tdag_svdag_flatten")
         return out[: idx.size]
+
+    def flatten_device(self, ctx, level: int, d_nodes, d_out):
+        """createLargeSubGrid on the GPU: d_nodes (int32/uint32 device tensor) -> d_out (int64
+        device tensor of len(d_nodes) * W words)."""
+        n = d_nodes.numel()
+        W = max(1, (1 << (3 * level)) // 64)
+        if not (d_nodes.is_cuda and d_out.is_cuda and d_nodes.element_size() == 4 and d_out.element_size() == 8 and d_out.numel() >= n * W):
+            raise ValueError("d_nodes must be a 4-byte device tensor and d_out an 8-byte device tensor of n * W words")
+        import torch
+
+        torch.cuda.synchronize()
+        check(lib().tdag_svdag_flatten_device(ctx._ptr, self._h, level, ctypes.c_void_p(d_nodes.data_ptr()), n, ctypes.c_void_p(d_out.data_ptr())),
+              "tdag_svdag_flatten_device")
 
     def static_words(self, level: int) -> np.ndarray:
         """StaticStructure(EditStructure) packed words of `level` (uint32; level 2: uint64 leaves)."""

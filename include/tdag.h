@@ -305,6 +305,12 @@ tdag_status tdag_svdag_copy_level(const tdag_svdag* svdag, uint32_t level, void*
 /* FlatGridGenerator / createLargeSubGrid (transform_dag.cpp:44-116, large_sub_grid.h:127-141):
  * linear LargeSubGrid masks (W words each) of the given level-`level` nodes. */
 tdag_status tdag_svdag_flatten(const tdag_svdag* svdag, uint32_t level, const uint32_t* nodes, uint64_t n, uint64_t* out_masks);
+/* The same masks written on the device (d_nodes, d_out_masks in device memory; n * W words out),
+ * the input of tdag_match_level_device. The SVDAG's levels 2..5 are copied to the device on first
+ * use and kept with the handle. A node index outside the level gives TDAG_EINVAL. Returns after
+ * the work on ctx's stream is done. tdag_construct_transform_dag flattens its exhaustive levels
+ * this way. */
+tdag_status tdag_svdag_flatten_device(tdag_context* ctx, const tdag_svdag* svdag, uint32_t level, const uint32_t* d_nodes, uint64_t n, uint64_t* d_out_masks);
 tdag_status tdag_svdag_free(tdag_svdag* svdag);
 
 /* constructTransformDAG (transform_dag.h:161-162, transform_dag.cpp:138-240): hierarchical

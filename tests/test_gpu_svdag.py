@@ -78,3 +78,35 @@ def test_device_svdag_unaligned_grid(gpu_ctx):
     view = buf[1:]
     assert view.data_ptr() % 4 != 0
     _same(pl.Svdag(bits, 128), pl.Svdag.from_device(gpu_ctx, view, 128))
+
+
+@pytest.mark.parametrize("level", [2, 3, 4, 5])
+def test_device_flatten_equals_host(gpu_ctx, level):
+    """createLargeSubGrid on the device equals the host flattening (pinned to the oracle's
+    structure.get-based createLargeSubGrid by the CPU tests), word for word."""
+    import torch
+    from tdag_amd import pipeline as pl
+
+    print(f"device flatten level {level}", flush=True)
+    bits = pl.scene_dense(1, 256, 21)
+    s = pl.Svdag(bits, 256)
+    rng = np.random.default_rng(level)
+    idx = rng.choice(s.level_sizes[level], size=min(700, s.level_sizes[level]), replace=False).astype(np.uint32)
+    host = s.flatten(level, idx)
+    d_idx = torch.from_numpy(idx.view(np.int32)).cuda()
+    d_out = torch.full((len(idx) * host.shape[1],), -1, dtype=torch.int64, device="cuda")
+    s.flatten_device(gpu_ctx, level, d_idx, d_out)
+    assert np.array_equal(d_out.cpu().numpy().view(np.uint64).reshape(host.shape), host)
+
+
+def test_device_flatten_rejects_bad_index(gpu_ctx):
+    import torch
+    import tdag_amd
+    from tdag_amd import pipeline as pl
+
+    print("device flatten bad index", flush=True)
+    s = pl.Svdag(pl.scene_dense(0, 64, 2), 64)
+    d_idx = torch.tensor([0, s.level_sizes[3]], dtype=torch.int32, device="cuda")
+    d_out = torch.zeros(2 * 8, dtype=torch.int64, device="cuda")
+    with pytest.raises(tdag_amd.TdagError, match="EINVAL"):
+        s.flatten_device(gpu_ctx, 3, d_idx, d_out)
