@@ -228,6 +228,156 @@ __global__ __launch_bounds__(kThreads) void kParents(const uint32_t* __restrict_
         pcodes[p] = codes[i] >> 3;
 }
 
+// The upper levels, at most kTail nodes each, run in one workgroup: the dedup table lives in LDS,
+// block-wide scans number the uniques and the parents, and level after level follows without a
+// launch. The per-level counts land in the same places as for the multi-kernel levels.
+constexpr uint32_t kTail = 4096;
+constexpr uint32_t kTailThreads = 1024;
+constexpr uint32_t kTailCap = 8192; // >= 2 * kTail
+constexpr uint32_t kMaxLevels = 13;
+
+struct TailPlan {
+    uint32_t first, last; // levels first..last
+    const uint32_t* keys0; // keys and codes of level `first`
+    const uint32_t* codes0;
+    uint32_t* uniq[kMaxLevels]; // unique keys of each level
+    uint32_t* pkeys[kMaxLevels]; // keys / codes of level l + 1, written at level l
+    uint32_t* pcodes[kMaxLevels];
+    uint32_t* counts; // [2l] items, [2l + 1] uniques
+    uint32_t* rep; // kTail words each
+    uint32_t* uid;
+    uint32_t* pstart;
+};
+
+// Exclusive scan of one value per thread over the 1024-thread block; *total gets the sum.
+__device__ inline uint32_t blockExclusiveScan(uint32_t v, uint32_t* part, uint32_t* total)
+{
+    const uint32_t t = threadIdx.x;
+    part[t] = v;
+    __syncthreads();
+    for (uint32_t off = 1; off < kTailThreads; off <<= 1) {
+        const uint32_t add = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += add;
+        __syncthreads();
+    }
+    *total = part[kTailThreads - 1];
+    const uint32_t excl = part[t] - v;
+    __syncthreads();
+    return excl;
+}
+
+template <int K>
+__device__ inline void tailInsert(const uint32_t* keys, uint32_t i, uint32_t* table)
+{
+    const uint32_t* k = keys + (size_t)i * K;
+    uint32_t h = keyHash<K>(k) & (kTailCap - 1);
+    for (uint32_t probe = 0; probe < kTailCap; ++probe) {
+        uint32_t cur = __hip_atomic_load(&table[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (cur == kNone) {
+            cur = atomicCAS(&table[h], kNone, i);
+            if (cur == kNone)
+                return;
+        }
+        if (keyEq<K>(keys + (size_t)cur * K, k)) {
+            if (i < cur)
+                atomicMin(&table[h], i);
+            return;
+        }
+        h = (h + 1) & (kTailCap - 1);
+    }
+}
+
+template <int K>
+__device__ inline uint32_t tailLookup(const uint32_t* keys, uint32_t i, const uint32_t* table)
+{
+    const uint32_t* k = keys + (size_t)i * K;
+    uint32_t h = keyHash<K>(k) & (kTailCap - 1);
+    for (uint32_t probe = 0; probe < kTailCap; ++probe) {
+        const uint32_t cur = table[h];
+        if (cur == kNone)
+            break;
+        if (keyEq<K>(keys + (size_t)cur * K, k))
+            return cur;
+        h = (h + 1) & (kTailCap - 1);
+    }
+    return i; // not reached: every key was inserted
+}
+
+__global__ __launch_bounds__(kTailThreads) void kTailLevels(TailPlan P)
+{
+    __shared__ uint32_t table[kTailCap];
+    __shared__ uint32_t part[kTailThreads];
+    __shared__ uint32_t s_n;
+    const uint32_t t = threadIdx.x;
+    const uint32_t* keys = P.keys0;
+    const uint32_t* codes = P.codes0;
+    for (uint32_t level = P.first; level <= P.last; ++level) {
+        const bool leafLevel = level == 2;
+        const uint32_t K = leafLevel ? 2u : 8u;
+        if (t == 0)
+            s_n = P.counts[2 * level];
+        for (uint32_t i = t; i < kTailCap; i += kTailThreads)
+            table[i] = kNone;
+        __syncthreads();
+        const uint32_t n = s_n; // <= kTail: at most one node per position of this level
+        for (uint32_t i = t; i < n; i += kTailThreads)
+            leafLevel ? tailInsert<2>(keys, i, table) : tailInsert<8>(keys, i, table);
+        __syncthreads();
+        // Items 4t .. 4t + 3 belong to thread t, so the scans number them in index order.
+        uint32_t firstBits = 0, startBits = 0;
+        for (uint32_t j = 0; j < 4; ++j) {
+            const uint32_t i = 4 * t + j;
+            if (i >= n)
+                break;
+            const uint32_t r = leafLevel ? tailLookup<2>(keys, i, table) : tailLookup<8>(keys, i, table);
+            P.rep[i] = r;
+            firstBits |= (r == i ? 1u : 0u) << j;
+            startBits |= ((i == 0 || (codes[i] >> 3) != (codes[i - 1] >> 3)) ? 1u : 0u) << j;
+        }
+        uint32_t nu = 0, np = 0;
+        uint32_t ubase = blockExclusiveScan((uint32_t)__builtin_popcount(firstBits), part, &nu);
+        uint32_t pbase = blockExclusiveScan((uint32_t)__builtin_popcount(startBits), part, &np);
+        for (uint32_t j = 0; j < 4; ++j) {
+            const uint32_t i = 4 * t + j;
+            if (i >= n)
+                break;
+            P.uid[i] = ubase; // exclusive count of uniques before item i
+            P.pstart[i] = pbase; // exclusive count of parent starts before item i
+            ubase += (firstBits >> j) & 1u;
+            pbase += (startBits >> j) & 1u;
+        }
+        if (level < P.last)
+            for (uint32_t i = t; i < np * 8; i += kTailThreads)
+                P.pkeys[level][i] = kNone;
+        __syncthreads();
+        for (uint32_t i = t; i < n; i += kTailThreads) {
+            const uint32_t r = P.rep[i];
+            const uint32_t id = P.uid[r];
+            if (r == i)
+                for (uint32_t w = 0; w < K; ++w)
+                    P.uniq[level][(size_t)id * K + w] = keys[(size_t)i * K + w];
+            if (level < P.last) {
+                const uint32_t st = (i == 0 || (codes[i] >> 3) != (codes[i - 1] >> 3)) ? 1u : 0u;
+                const uint32_t p = P.pstart[i] + st - 1;
+                P.pkeys[level][(size_t)p * 8 + (codes[i] & 7u)] = id;
+                if (st)
+                    P.pcodes[level][p] = codes[i] >> 3;
+            }
+        }
+        if (t == 0) {
+            P.counts[2 * level + 1] = nu;
+            if (level < P.last)
+                P.counts[2 * (level + 1)] = np;
+        }
+        __syncthreads();
+        if (level < P.last) {
+            keys = P.pkeys[level];
+            codes = P.pcodes[level];
+        }
+    }
+}
+
 // Carves the build's buffers out of one workspace, 256-byte aligned. With a null base it only
 // adds up the size, so the same code path first sizes the workspace, then runs.
 struct Carver {
@@ -325,6 +475,33 @@ hipError_t enqueueBuild(Carver& A, const uint8_t* d_bits, uint32_t R, uint32_t l
     }
     uniqueKeys.assign(levels + 1, nullptr);
     for (uint32_t level = 2;; ++level) {
+        if (bound[level] <= kTail) {
+            // This level and all above it: one workgroup (the choice depends on the bounds only,
+            // so the sizing pass carves the same buffers).
+            TailPlan P {};
+            P.first = level;
+            P.last = levels;
+            P.keys0 = keys;
+            P.codes0 = codes;
+            P.counts = d_counts;
+            P.rep = A.get<uint32_t>(kTail);
+            P.uid = A.get<uint32_t>(kTail);
+            P.pstart = A.get<uint32_t>(kTail);
+            for (uint32_t l = level; l <= levels; ++l) {
+                uniqueKeys[l] = A.get<uint32_t>((size_t)bound[l] * (l == 2 ? 2 : 8));
+                P.uniq[l] = uniqueKeys[l];
+                if (l < levels) {
+                    P.pkeys[l] = A.get<uint32_t>((size_t)bound[l + 1] * 8);
+                    P.pcodes[l] = A.get<uint32_t>(bound[l + 1]);
+                }
+            }
+            if (run) {
+                hipLaunchKernelGGL(kTailLevels, dim3(1), dim3(kTailThreads), 0, st, P);
+                if ((e = hipGetLastError()) != hipSuccess)
+                    return e;
+            }
+            break;
+        }
         const uint32_t K = level == 2 ? 2 : 8;
         const uint32_t* d_n = run ? d_counts + 2 * level : nullptr;
         uint32_t* d_nu = run ? d_counts + 2 * level + 1 : nullptr;
