@@ -25,8 +25,14 @@ struct tdag_svdag {
     int devDevice = -1;
     ~tdag_svdag()
     {
-        if (devDevice >= 0 && hipSetDevice(devDevice) == hipSuccess)
+        if (devDevice < 0)
+            return;
+        int current = -1;
+        const bool haveCurrent = hipGetDevice(&current) == hipSuccess;
+        if (hipSetDevice(devDevice) == hipSuccess)
             tdag::freeDeviceSvdag(dev);
+        if (haveCurrent)
+            (void)hipSetDevice(current); // leave the caller's current device as it was
     }
 };
 
