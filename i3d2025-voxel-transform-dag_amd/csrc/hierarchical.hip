@@ -130,6 +130,59 @@ inline Node transformNode(const Node& n, uint32_t id, const Tables& T)
 
 inline void hashCombine(uint64_t& seed, uint64_t v) { seed ^= v + 0x9e3779b9ull + (seed << 6) + (seed >> 2); }
 
+// Runs f(begin, end) over [0, n) split into one contiguous range per thread.
+template <typename F>
+void parallelFor(size_t n, int threads, F&& f)
+{
+    const size_t t = (size_t)std::max(1, threads);
+    if (n < 8192 || t == 1) {
+        f(size_t(0), n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    const size_t chunk = (n + t - 1) / t;
+    for (size_t i = 0; i < t; ++i) {
+        const size_t b = std::min(n, chunk * i), e = std::min(n, chunk * (i + 1));
+        if (b < e)
+            pool.emplace_back([&f, b, e]() { f(b, e); });
+    }
+    for (auto& th : pool)
+        th.join();
+}
+
+// Sorts unique (key, index) pairs: sorted chunks, merged pairwise. Pairs are distinct, so the
+// result is the single ascending order whatever the split.
+void parallelSortPairs(std::vector<std::pair<uint64_t, uint32_t>>& v, int threads)
+{
+    const size_t n = v.size();
+    size_t parts = 1;
+    while (parts < (size_t)std::max(1, threads) && parts < 64)
+        parts <<= 1;
+    if (n < 16384 || parts == 1) {
+        std::sort(v.begin(), v.end());
+        return;
+    }
+    std::vector<size_t> bounds(parts + 1);
+    for (size_t i = 0; i <= parts; ++i)
+        bounds[i] = n * i / parts;
+    {
+        std::vector<std::thread> pool;
+        for (size_t i = 0; i < parts; ++i)
+            pool.emplace_back([&, i]() { std::sort(v.begin() + bounds[i], v.begin() + bounds[i + 1]); });
+        for (auto& th : pool)
+            th.join();
+    }
+    for (size_t width = 1; width < parts; width <<= 1) {
+        std::vector<std::thread> pool;
+        for (size_t i = 0; i + width < parts; i += 2 * width) {
+            const size_t b = bounds[i], m = bounds[i + width], e = bounds[std::min(parts, i + 2 * width)];
+            pool.emplace_back([&v, b, m, e]() { std::inplace_merge(v.begin() + b, v.begin() + m, v.begin() + e); });
+        }
+        for (auto& th : pool)
+            th.join();
+    }
+}
+
 uint64_t stateToTp(uint64_t ptr, uint32_t state)
 {
     const uint32_t* p = kPermTab[state >> 3];
@@ -166,7 +219,8 @@ void reduceLevel(const std::vector<Node>& nodes, uint32_t allowedIds, const std:
     const Tables& T = tables();
     const size_t n = nodes.size();
     std::vector<std::pair<uint64_t, uint32_t>> keys(n);
-    for (size_t i = 0; i < n; ++i) {
+    parallelFor(n, threads, [&](size_t kb, size_t ke) {
+    for (size_t i = kb; i < ke; ++i) {
         uint64_t ptrs[8];
         for (int c = 0; c < 8; ++c) {
             const uint64_t p = nodes[i].c[c].ptr;
@@ -178,7 +232,8 @@ void reduceLevel(const std::vector<Node>& nodes, uint32_t allowedIds, const std:
             hashCombine(seed, p);
         keys[i] = { seed, (uint32_t)i };
     }
-    std::sort(keys.begin(), keys.end());
+    });
+    parallelSortPairs(keys, threads);
     std::vector<size_t> groupStart;
     for (size_t j = 0; j < n; ++j)
         if (j == 0 || keys[j].first != keys[j - 1].first)
@@ -235,11 +290,15 @@ void reduceLevel(const std::vector<Node>& nodes, uint32_t allowedIds, const std:
             }
         }
     };
-    std::vector<std::thread> pool;
-    for (int t = 0; t < std::max(1, threads); ++t)
-        pool.emplace_back(worker);
-    for (auto& th : pool)
-        th.join();
+    if (n < 4096 || threads <= 1) {
+        worker(); // small levels: starting threads would cost more than the work
+    } else {
+        std::vector<std::thread> pool;
+        for (int t = 0; t < threads; ++t)
+            pool.emplace_back(worker);
+        for (auto& th : pool)
+            th.join();
+    }
     (void)allowedIds;
     mapping.assign(n, Mapping { 0, 0, 0 });
     kept.clear();
@@ -320,9 +379,11 @@ TDag hierarchicalDedup(const Svdag& in, uint32_t midLevel, std::vector<uint32_t>
         Child canon[256];
         canonical2x2x2(idList, canon);
         std::vector<Node> leaves(in.leaves.size());
-        for (size_t i = 0; i < in.leaves.size(); ++i)
-            for (uint32_t k = 0; k < 8; ++k)
-                leaves[i].c[k] = canon[(in.leaves[i] >> (8 * k)) & 0xFFu];
+        parallelFor(in.leaves.size(), threads, [&](size_t b, size_t e) {
+            for (size_t i = b; i < e; ++i)
+                for (uint32_t k = 0; k < 8; ++k)
+                    leaves[i].c[k] = canon[(in.leaves[i] >> (8 * k)) & 0xFFu];
+        });
         std::vector<uint32_t> kept;
         reduceLevel(leaves, 0, idList, threads, kept, prev);
         out.leaves.resize(kept.size());
@@ -334,11 +395,13 @@ TDag hierarchicalDedup(const Svdag& in, uint32_t midLevel, std::vector<uint32_t>
     for (uint32_t level = 3; level <= in.rootLevel; ++level) {
         const auto& inNodes = in.nodes[level];
         std::vector<Node> nodes(inNodes.size());
-        for (size_t i = 0; i < inNodes.size(); ++i)
-            for (int c = 0; c < 8; ++c) {
-                const uint32_t ch = inNodes[i][c];
-                nodes[i].c[c] = (ch == kEmptyChild) ? Child { kSentinelPtr, ~0ull, 0u } : Child { prev[ch].ptr, prev[ch].inv, prev[ch].state };
-            }
+        parallelFor(inNodes.size(), threads, [&](size_t b, size_t e) {
+            for (size_t i = b; i < e; ++i)
+                for (int c = 0; c < 8; ++c) {
+                    const uint32_t ch = inNodes[i][c];
+                    nodes[i].c[c] = (ch == kEmptyChild) ? Child { kSentinelPtr, ~0ull, 0u } : Child { prev[ch].ptr, prev[ch].inv, prev[ch].state };
+                }
+        });
         std::vector<uint32_t> kept;
         std::vector<Mapping> mapping;
         // constructStaticTransformDAGHierarchical (transform_dag_hierarchical.cpp:691-704) points
@@ -349,11 +412,13 @@ TDag hierarchicalDedup(const Svdag& in, uint32_t midLevel, std::vector<uint32_t>
         reduceLevel(nodes, 0, idList, threads, kept, mapping, (staticOffsets && level >= 4) ? &offsets : nullptr);
         auto& outNodes = out.nodes[level];
         outNodes.resize(kept.size());
-        for (size_t k = 0; k < kept.size(); ++k)
-            for (int c = 0; c < 8; ++c) {
-                const Child& ch = nodes[kept[k]].c[c];
-                outNodes[k][c] = stateToTp(ch.ptr, ch.state);
-            }
+        parallelFor(kept.size(), threads, [&](size_t b, size_t e) {
+            for (size_t k = b; k < e; ++k)
+                for (int c = 0; c < 8; ++c) {
+                    const Child& ch = nodes[kept[k]].c[c];
+                    outNodes[k][c] = stateToTp(ch.ptr, ch.state);
+                }
+        });
         prev.swap(mapping);
         if (level == midLevel)
             storeMid(outNodes.size(), prev);
