@@ -104,31 +104,30 @@ struct Node {
 
 constexpr uint64_t kSentinelPtr = 0xFFFFFFFFull;
 
+// Child equality of TaggedPointer::operator== (transform_dag_hierarchical.cpp:79-122): same
+// pointer, and the pointer's node is invariant under the transform that maps one state onto the
+// other.
 inline bool childEq(const Child& l, const Child& r, const Tables& T)
 {
     if (l.ptr != r.ptr)
         return false;
     return (l.inv >> T.required[l.state][r.state]) & 1ull;
 }
-inline bool nodeEq(const Node& a, const Node& b, const Tables& T)
+
+inline void hashCombine(uint64_t& seed, uint64_t v) { seed ^= v + 0x9e3779b9ull + (seed << 6) + (seed >> 2); }
+
+// nodeEq(transformNode(n, id), other) without building the transformed node: child i of n lands
+// at position childPos[id][i] with its state composed with id; stops at the first mismatch.
+inline bool transformedEq(const Node& n, uint32_t id, const Node& other, const Tables& T)
 {
-    for (int i = 0; i < 8; ++i)
-        if (!childEq(a.c[i], b.c[i], T))
-            return false;
-    return true;
-}
-inline Node transformNode(const Node& n, uint32_t id, const Tables& T)
-{
-    Node o;
     for (uint32_t i = 0; i < 8; ++i) {
         Child c = n.c[i];
         c.state = T.compose[id][c.state];
-        o.c[T.childPos[id][i]] = c;
+        if (!childEq(c, other.c[T.childPos[id][i]], T))
+            return false;
     }
-    return o;
+    return true;
 }
-
-inline void hashCombine(uint64_t& seed, uint64_t v) { seed ^= v + 0x9e3779b9ull + (seed << 6) + (seed >> 2); }
 
 // Runs f(begin, end) over [0, n) split into one contiguous range per thread.
 template <typename F>
@@ -258,15 +257,14 @@ void reduceLevel(const std::vector<Node>& nodes, uint32_t allowedIds, const std:
                 uint64_t inv = 0;
                 bool hit = false;
                 for (uint32_t id : idList) {
-                    const Node tn = transformNode(node, id, T);
                     if (e - b > 1) {
                         std::array<uint64_t, 8> k;
-                        for (int c = 0; c < 8; ++c)
-                            k[c] = tn.c[c].ptr;
+                        for (uint32_t i = 0; i < 8; ++i)
+                            k[T.childPos[id][i]] = node.c[i].ptr;
                         auto it = lut.find(k);
                         if (it != lut.end()) {
                             for (uint32_t cand : it->second)
-                                if (nodeEq(tn, nodes[cand], T)) {
+                                if (transformedEq(node, id, nodes[cand], T)) {
                                     hitIndex[index] = (int32_t)cand;
                                     hitId[index] = (uint8_t)id;
                                     inv = invMask[cand];
@@ -277,7 +275,7 @@ void reduceLevel(const std::vector<Node>& nodes, uint32_t allowedIds, const std:
                         if (hit)
                             break;
                     }
-                    if (nodeEq(tn, node, T))
+                    if (transformedEq(node, id, node, T))
                         inv |= 1ull << id;
                 }
                 invMask[index] = inv;
