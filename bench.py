@@ -169,50 +169,65 @@ def cpu_baseline(levels, cfg, budget_s: float, threads: int):
 
 
 def bench_batched(ctx, cfg, dev, order, args):
-    """The batched form on the same level inputs: every level's parent search in one persistent
-    kernel, the whole multi-level call captured once as a hipGraph and replayed per step. Its
-    outputs are compared with the per-level launches' on the device (bit-exact)."""
+    """The batched forms on the same level inputs, each captured once as a hipGraph and replayed
+    per step: (a) every level's parent search in one persistent kernel over a device work queue;
+    (b) concurrent: each level's own tuned matcher on its own graph branch. Outputs are compared
+    with the per-level launches' on the device (bit-exact)."""
     import torch
 
-    jobs, outs = [], {}
-    for level in order:
-        d = dev[level]
-        n = d["n"]
-        outs[level] = {"enc": torch.empty(max(1, n), dtype=torch.int64, device="cuda"),
-                       "map": torch.empty(max(1, n), dtype=torch.int64, device="cuda")}
-        jobs.append({"level": level, "n": n, "d_masks": d["masks"], "d_encoded": outs[level]["enc"], "d_mapping": outs[level]["map"],
-                     "d_unique": torch.empty(max(1, n), dtype=torch.int32, device="cuda")})
-    torch.cuda.synchronize()
-    b = ctx.batch(jobs, cfg, use_graph=True)
-    try:
-        grab = b.autotune(1)
-        for _ in range(max(1, args.warmup)):
-            b.run()
-        t0 = time.perf_counter()
-        matches = 0
-        for _ in range(args.steps):
-            stats, _ = b.run()
-            matches += sum(int(s.num_matched) for s in stats)
-        elapsed = time.perf_counter() - t0
-    finally:
-        b.close()
-    # Bit-exactness against the per-level launches on the full-size inputs.
-    exact = True
-    for level in order:
-        d = dev[level]
-        n = d["n"]
-        ref = torch.empty(max(1, n), dtype=torch.int64, device="cuda")
-        ref_map = torch.empty(max(1, n), dtype=torch.int64, device="cuda")
-        ctx.match_level_device(level, d["masks"], n, cfg, d_encoded=ref, d_mapping=ref_map, d_unique=d["unique"])
+    def run_form(concurrent):
+        jobs, outs = [], {}
+        for level in order:
+            d = dev[level]
+            n = d["n"]
+            outs[level] = {"enc": torch.empty(max(1, n), dtype=torch.int64, device="cuda"),
+                           "map": torch.empty(max(1, n), dtype=torch.int64, device="cuda")}
+            jobs.append({"level": level, "n": n, "d_masks": d["masks"], "d_encoded": outs[level]["enc"], "d_mapping": outs[level]["map"],
+                         "d_unique": torch.empty(max(1, n), dtype=torch.int32, device="cuda")})
         torch.cuda.synchronize()
-        exact = exact and bool(torch.equal(ref[:n], outs[level]["enc"][:n])) and bool(torch.equal(ref_map[:n], outs[level]["map"][:n]))
-    return {
+        b = ctx.batch(jobs, cfg, use_graph=True, concurrent=concurrent)
+        try:
+            grab = b.autotune(1)
+            for _ in range(max(1, args.warmup)):
+                b.run()
+            t0 = time.perf_counter()
+            matches = 0
+            for _ in range(args.steps):
+                stats, _ = b.run()
+                matches += sum(int(s.num_matched) for s in stats)
+            elapsed = time.perf_counter() - t0
+        finally:
+            b.close()
+        return outs, grab, matches, elapsed
+
+    def exact_vs_per_level(outs):
+        exact = True
+        for level in order:
+            d = dev[level]
+            n = d["n"]
+            ref = torch.empty(max(1, n), dtype=torch.int64, device="cuda")
+            ref_map = torch.empty(max(1, n), dtype=torch.int64, device="cuda")
+            ctx.match_level_device(level, d["masks"], n, cfg, d_encoded=ref, d_mapping=ref_map, d_unique=d["unique"])
+            torch.cuda.synchronize()
+            exact = exact and bool(torch.equal(ref[:n], outs[level]["enc"][:n])) and bool(torch.equal(ref_map[:n], outs[level]["map"][:n]))
+        return exact
+
+    outs, grab, matches, elapsed = run_form(False)
+    res = {
         "form": "one persistent kernel for all levels (device work queue, phases = popcount buckets of parents), hipGraph replay",
         "ms_per_step": round(elapsed * 1e3 / max(1, args.steps), 4),
         "matches_per_s": matches / elapsed if elapsed > 0 else None,
         "tiles_per_pop": grab,
-        "bit_exact_vs_per_level": exact,
+        "bit_exact_vs_per_level": exact_vs_per_level(outs),
     }
+    outs, _, matches, elapsed = run_form(True)
+    res["concurrent"] = {
+        "form": "each level's tuned per-level matcher on its own branch of one hipGraph (levels overlap), replayed per step",
+        "ms_per_step": round(elapsed * 1e3 / max(1, args.steps), 4),
+        "matches_per_s": matches / elapsed if elapsed > 0 else None,
+        "bit_exact_vs_per_level": exact_vs_per_level(outs),
+    }
+    return res
 
 
 def bench_learned(ctx, cfg, dev, levels, info, args):

@@ -89,6 +89,14 @@ struct tdag_batch {
     uint64_t spinTicks = 0;
     uint32_t cfgBits = 0;
     uint32_t grab = 1;
+    // Concurrent mode: every level's own tuned matcher (the per-level plan and kernels) on its own
+    // stream, forked from and joined back into the context's stream (graph branches when captured).
+    bool concurrent = false;
+    tdag::MatchPlan cplans[tdag::kBatchMaxLevels] {};
+    size_t cOff[tdag::kBatchMaxLevels] {};
+    hipStream_t aux[tdag::kBatchMaxLevels] {};
+    hipEvent_t evFork = nullptr;
+    hipEvent_t evJoin[tdag::kBatchMaxLevels] {};
 };
 
 namespace tdag {
@@ -474,8 +482,27 @@ tdag_status uploadBatchDesc(tdag_batch* b)
     return TDAG_OK;
 }
 
+tdag_status enqueueConcurrentCall(tdag_batch* b)
+{
+    const uint32_t k = (uint32_t)b->jobs.size();
+    hipStream_t main = b->ctx->stream;
+    CAPI_HIP(hipEventRecord(b->evFork, main), "fork event");
+    for (uint32_t i = 0; i < k; ++i) {
+        const tdag_level_job& j = b->jobs[i];
+        CAPI_HIP(hipStreamWaitEvent(b->aux[i], b->evFork, 0), "fork wait");
+        CAPI_HIP(tdag::enqueueMatch(b->cplans[i], j.d_masks, b->cfgBits, (char*)b->scratch + b->cOff[i], b->aux[i], j.d_encoded, j.d_mapping, j.d_unique, nullptr,
+                     nullptr),
+            "concurrent matcher launch");
+        CAPI_HIP(hipEventRecord(b->evJoin[i], b->aux[i]), "join event");
+        CAPI_HIP(hipStreamWaitEvent(main, b->evJoin[i], 0), "join wait");
+    }
+    return TDAG_OK;
+}
+
 tdag_status enqueueBatchCall(tdag_batch* b)
 {
+    if (b->concurrent)
+        return enqueueConcurrentCall(b);
     const uint32_t k = b->plan.count;
     std::vector<const uint64_t*> masks(k);
     std::vector<uint64_t*> enc(k), map(k);
@@ -497,6 +524,7 @@ extern "C" {
 
 tdag_status tdag_batch_create(tdag_context* ctx, const tdag_level_job* jobs, uint32_t count, const tdag_config* cfg, int use_graph, tdag_batch** out)
 {
+    const uint32_t flags = (uint32_t)use_graph;
     if (!ctx || !jobs || !cfg || !out)
         return fail(TDAG_EINVAL, "null argument");
     if (count == 0 || count > tdag::kBatchMaxLevels)
@@ -521,14 +549,39 @@ tdag_status tdag_batch_create(tdag_context* ctx, const tdag_level_job* jobs, uin
     auto* b = new tdag_batch();
     b->ctx = ctx;
     b->jobs.assign(jobs, jobs + count);
+    b->concurrent = (flags & TDAG_BATCH_CONCURRENT) != 0;
     auto cleanup = [&](tdag_status s) {
         tdag_batch_destroy(b);
         return s;
     };
-    if (hipError_t e = tdag::planBatch(count, levels, ns, first, bits, frames, ctx->device, &b->plan); e != hipSuccess)
-        return cleanup(failHip(e, "batched matcher plan"));
-    if (hipError_t e = hipMalloc(&b->scratch, b->plan.scratchBytes); e != hipSuccess)
-        return cleanup(failHip(e, "batched matcher scratch allocation"));
+    if (b->concurrent) {
+        // The same plans tdag_match_level_device would use for each level (tuned variant and
+        // schedule), each with its own scratch, stream and join event.
+        size_t off = 0;
+        for (uint32_t i = 0; i < count; ++i) {
+            const uint32_t l = jobs[i].level;
+            const int32_t v = ctx->variant[l];
+            b->cplans[i] = tdag::planMatch(l, jobs[i].n, v >= 0 ? (uint32_t)v : tdag::defaultVariant(l), ctx->firstBatch[l], ctx->batchGrowth[l], ctx->bloomBits[l],
+                ctx->childFrames[l]);
+            b->cOff[i] = off;
+            off += (b->cplans[i].scratchBytes + 255) & ~size_t(255);
+        }
+        if (hipError_t e = hipMalloc(&b->scratch, std::max<size_t>(off, 256)); e != hipSuccess)
+            return cleanup(failHip(e, "concurrent matcher scratch allocation"));
+        for (uint32_t i = 0; i < count; ++i) {
+            if (hipError_t e = hipStreamCreateWithFlags(&b->aux[i], hipStreamNonBlocking); e != hipSuccess) // default priority
+                return cleanup(failHip(e, "batch stream creation"));
+            if (hipError_t e = hipEventCreateWithFlags(&b->evJoin[i], hipEventDisableTiming); e != hipSuccess)
+                return cleanup(failHip(e, "batch event creation"));
+        }
+        if (hipError_t e = hipEventCreateWithFlags(&b->evFork, hipEventDisableTiming); e != hipSuccess)
+            return cleanup(failHip(e, "batch event creation"));
+    } else {
+        if (hipError_t e = tdag::planBatch(count, levels, ns, first, bits, frames, ctx->device, &b->plan); e != hipSuccess)
+            return cleanup(failHip(e, "batched matcher plan"));
+        if (hipError_t e = hipMalloc(&b->scratch, b->plan.scratchBytes); e != hipSuccess)
+            return cleanup(failHip(e, "batched matcher scratch allocation"));
+    }
     if (hipError_t e = hipHostMalloc((void**)&b->pinned, 8 * (tdag::kNumCounters * count + 1)); e != hipSuccess)
         return cleanup(failHip(e, "pinned readback allocation"));
     int rateKHz = 0;
@@ -537,9 +590,10 @@ tdag_status tdag_batch_create(tdag_context* ctx, const tdag_level_job* jobs, uin
     b->spinTicks = (uint64_t)std::max(rateKHz, 1) * 1000ull * 10ull; // 10 s
     b->cfgBits = cfgBitsOf(cfg) | (ctx->counters ? tdag::kCfgCount : 0u);
     b->grab = 4;
-    if (tdag_status s = uploadBatchDesc(b); s != TDAG_OK)
-        return cleanup(s);
-    if (use_graph) {
+    if (!b->concurrent)
+        if (tdag_status s = uploadBatchDesc(b); s != TDAG_OK)
+            return cleanup(s);
+    if (flags & TDAG_BATCH_GRAPH) {
         if (hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal); e != hipSuccess)
             return cleanup(failHip(e, "hipStreamBeginCapture"));
         const tdag_status s = enqueueBatchCall(b);
@@ -570,15 +624,19 @@ tdag_status tdag_batch_run(tdag_batch* b, tdag_match_stats* out_stats, float* ou
         return s;
     }
     CAPI_HIP(hipEventRecord(ctx->ev[3], ctx->stream), "hipEventRecord");
-    const uint32_t k = b->plan.count;
+    const uint32_t k = (uint32_t)b->jobs.size();
     for (uint32_t i = 0; i < k; ++i) {
-        const tdag::LevelView v = tdag::levelView(b->plan.plans[i], (char*)b->scratch + b->plan.scratchOff[i]);
-        CAPI_HIP(hipMemcpyAsync(b->pinned + tdag::kNumCounters * b->plan.job[i], v.counters, 8 * tdag::kNumCounters, hipMemcpyDeviceToHost, ctx->stream),
-            "counter readback");
+        const tdag::LevelView v = b->concurrent ? tdag::levelView(b->cplans[i], (char*)b->scratch + b->cOff[i])
+                                                : tdag::levelView(b->plan.plans[i], (char*)b->scratch + b->plan.scratchOff[i]);
+        const uint32_t job = b->concurrent ? i : b->plan.job[i];
+        CAPI_HIP(hipMemcpyAsync(b->pinned + tdag::kNumCounters * job, v.counters, 8 * tdag::kNumCounters, hipMemcpyDeviceToHost, ctx->stream), "counter readback");
     }
-    CAPI_HIP(hipMemcpyAsync(b->pinned + tdag::kNumCounters * k, (char*)b->scratch + b->plan.syncOff + offsetof(tdag::BatchSync, error), 4, hipMemcpyDeviceToHost,
-                 ctx->stream),
-        "barrier status readback");
+    if (b->concurrent)
+        b->pinned[tdag::kNumCounters * k] = 0; // no grid barrier in this mode
+    else
+        CAPI_HIP(hipMemcpyAsync(b->pinned + tdag::kNumCounters * k, (char*)b->scratch + b->plan.syncOff + offsetof(tdag::BatchSync, error), 4, hipMemcpyDeviceToHost,
+                     ctx->stream),
+            "barrier status readback");
     CAPI_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     if ((uint32_t)b->pinned[tdag::kNumCounters * k] != 0)
         return fail(TDAG_EINTERNAL, "batched matcher: a grid barrier timed out (not all blocks were resident); results are invalid");
@@ -610,6 +668,11 @@ tdag_status tdag_batch_autotune(tdag_batch* b, uint32_t repeats, uint32_t* out_g
 {
     if (!b)
         return fail(TDAG_EINVAL, "batch is null");
+    if (b->concurrent) { // no work queue to tune: each level runs its own tuned kernels
+        if (out_grab)
+            *out_grab = 0;
+        return TDAG_OK;
+    }
     repeats = std::max<uint32_t>(1, repeats);
     const uint32_t cands[5] = { 1, 2, 4, 8, 16 };
     uint32_t best = b->grab;
@@ -651,6 +714,14 @@ tdag_status tdag_batch_destroy(tdag_batch* b)
         (void)hipFree(b->scratch);
     if (b->pinned)
         (void)hipHostFree(b->pinned);
+    for (auto st : b->aux)
+        if (st)
+            (void)hipStreamDestroy(st);
+    for (auto ev : b->evJoin)
+        if (ev)
+            (void)hipEventDestroy(ev);
+    if (b->evFork)
+        (void)hipEventDestroy(b->evFork);
     delete b;
     return TDAG_OK;
 }

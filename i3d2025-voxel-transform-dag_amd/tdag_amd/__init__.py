@@ -190,14 +190,18 @@ def _jobs_array(jobs):
 class Batch:
     """A planned (and, with use_graph, captured) batched matcher call; see Context.batch."""
 
-    def __init__(self, ctx: "Context", jobs, config, use_graph: bool):
+    GRAPH = 1  # TDAG_BATCH_GRAPH
+    CONCURRENT = 2  # TDAG_BATCH_CONCURRENT
+
+    def __init__(self, ctx: "Context", jobs, config, use_graph: bool, concurrent: bool = False):
         self._ptr = ctypes.c_void_p()
         self._ctx = ctx
         self._jobs, self._keep = _jobs_array(jobs)
         self.count = len(jobs)
+        self.concurrent = bool(concurrent)
         cfg = (config or TransformDAGConfig()).to_c()
-        check(lib().tdag_batch_create(ctx._ptr, self._jobs, self.count, ctypes.byref(cfg), int(bool(use_graph)), ctypes.byref(self._ptr)),
-              "tdag_batch_create")
+        flags = (self.GRAPH if use_graph else 0) | (self.CONCURRENT if concurrent else 0)
+        check(lib().tdag_batch_create(ctx._ptr, self._jobs, self.count, ctypes.byref(cfg), flags, ctypes.byref(self._ptr)), "tdag_batch_create")
 
     def run(self):
         """Runs the call and waits; returns (per-job stats, milliseconds of the whole call)."""
@@ -285,12 +289,13 @@ class Context:
         return None, None
 
     # -- batched form ------------------------------------------------------------------------
-    def batch(self, jobs, config: TransformDAGConfig | None = None, use_graph: bool = True) -> "Batch":
-        """Batched matcher over several levels (one job per level): every level's parent search
-        runs in one persistent kernel; with use_graph the whole call is captured once as a
-        hipGraph. `jobs` holds dicts with keys level, n, d_masks and optional d_encoded,
-        d_mapping, d_unique (torch tensors or raw device pointers)."""
-        return Batch(self, jobs, config, use_graph)
+    def batch(self, jobs, config: TransformDAGConfig | None = None, use_graph: bool = True, concurrent: bool = False) -> "Batch":
+        """Batched matcher over several levels (one job per level). By default every level's
+        parent search runs in one persistent kernel; with concurrent=True each level runs its own
+        tuned matcher on its own stream and the levels overlap. With use_graph the whole call is
+        captured once as a hipGraph. `jobs` holds dicts with keys level, n, d_masks and optional
+        d_encoded, d_mapping, d_unique (torch tensors or raw device pointers)."""
+        return Batch(self, jobs, config, use_graph, concurrent)
 
     def match_levels_batched_device(self, jobs, config: TransformDAGConfig | None = None):
         """One-shot batched call (no graph); returns per-job stats."""

@@ -181,10 +181,19 @@ typedef struct tdag_level_job {
 
 typedef struct tdag_batch tdag_batch;
 
-/* Plans the batch (device scratch owned by the batch), uploads its descriptor and, when
- * use_graph != 0, captures the whole call (all levels' preparation, the persistent search, all
- * levels' post-processing) into a hipGraph on the context's stream. The jobs' device buffers
- * must stay valid until tdag_batch_destroy. */
+/* Flags of tdag_batch_create (its `use_graph` argument; 0 and 1 keep their old meaning).
+ * TDAG_BATCH_GRAPH: capture the whole call into a hipGraph and replay it on every run.
+ * TDAG_BATCH_CONCURRENT: instead of the persistent kernel, each level runs the matcher that
+ * tdag_match_level_device would run for it (the context's tuned variant and schedule at creation
+ * time), on its own stream forked from and joined back into the context's stream; the levels
+ * overlap (graph branches when captured). Same results. */
+#define TDAG_BATCH_GRAPH 1u
+#define TDAG_BATCH_CONCURRENT 2u
+
+/* Plans the batch (device scratch owned by the batch) and, with TDAG_BATCH_GRAPH, captures the
+ * whole call into a hipGraph on the context's stream. Persistent mode (default): all levels'
+ * preparation, one persistent search kernel over a device work queue, all levels'
+ * post-processing. The jobs' device buffers must stay valid until tdag_batch_destroy. */
 tdag_status tdag_batch_create(tdag_context* ctx, const tdag_level_job* jobs, uint32_t count, const tdag_config* cfg, int use_graph, tdag_batch** out);
 /* Runs the batch on the context's stream (graph replay or eager launches) and waits. out_stats
  * (count entries, job order, may be NULL) receives per-level results; ms_total and ms_search
@@ -192,7 +201,7 @@ tdag_status tdag_batch_create(tdag_context* ctx, const tdag_level_job* jobs, uin
 tdag_status tdag_batch_run(tdag_batch* b, tdag_match_stats* out_stats, float* out_ms);
 /* Tunes the batch's queue granularity (consecutive tiles per pop: 1, 2, 4, 8, 16) by running it
  * (1 warm-up + `repeats` timed runs each) and keeps the fastest; the captured graph stays valid
- * (it reads the descriptor at run time). */
+ * (it reads the descriptor at run time). A concurrent batch has no queue: *out_grab = 0. */
 tdag_status tdag_batch_autotune(tdag_batch* b, uint32_t repeats, uint32_t* out_grab);
 tdag_status tdag_batch_destroy(tdag_batch* b);
 /* One-shot batched call without a graph: create, run, destroy. */

@@ -1,4 +1,6 @@
-"""GPU parity of the batched form (persistent multi-level kernel, device work queue, hipGraph).
+"""GPU parity of the batched form: the persistent multi-level kernel with its device work queue,
+and the concurrent form (each level's own matcher on its own stream / graph branch), with and
+without a hipGraph.
 
 Every level of a batch must produce the same encodedParents keys, mapping and unique list as the
 CPU oracle (and hence as the per-level launches), whatever the phase schedule: the tests force
@@ -56,14 +58,14 @@ def _check_outputs(oracle, level_masks, bufs, stats, cfg_tuple=(True, True, True
         assert nu == len(ref_uniq) and np.array_equal(uniq, ref_uniq), f"level {lv}: unique list differs"
 
 
-@pytest.mark.parametrize("use_graph", [False, True])
-def test_batched_all_levels(oracle, gpu_ctx, use_graph):
+@pytest.mark.parametrize("use_graph,concurrent", [(False, False), (True, False), (False, True), (True, True)])
+def test_batched_all_levels(oracle, gpu_ctx, use_graph, concurrent):
     import tdag_amd
 
     level_masks = _level_sets(oracle, 7)
     jobs, bufs = _device_jobs(level_masks)
     print(f"batched use_graph={use_graph} sizes={ {lv: len(m) for lv, m in level_masks.items()} }", flush=True)
-    b = gpu_ctx.batch(jobs, tdag_amd.TransformDAGConfig(True, True, True), use_graph=use_graph)
+    b = gpu_ctx.batch(jobs, tdag_amd.TransformDAGConfig(True, True, True), use_graph=use_graph, concurrent=concurrent)
     try:
         stats, ms = b.run()
     finally:
@@ -94,21 +96,23 @@ def test_batched_many_phases(oracle, gpu_ctx, first_batch):
     _check_outputs(oracle, level_masks, bufs, stats)
 
 
-def test_batched_graph_replays(oracle, gpu_ctx):
+@pytest.mark.parametrize("concurrent", [False, True])
+def test_batched_graph_replays(oracle, gpu_ctx, concurrent):
     """One captured graph replayed several times, with other GPU work (a plain per-level call and
-    torch kernels) between the replays; every replay equals the oracle."""
+    torch kernels) between the replays; every replay equals the oracle. Concurrent mode: the graph
+    holds one branch per level."""
     import torch
 
     import tdag_amd
 
     level_masks = _level_sets(oracle, 23)
     jobs, bufs = _device_jobs(level_masks)
-    b = gpu_ctx.batch(jobs, tdag_amd.TransformDAGConfig(True, True, True), use_graph=True)
+    b = gpu_ctx.batch(jobs, tdag_amd.TransformDAGConfig(True, True, True), use_graph=True, concurrent=concurrent)
     try:
         grab = b.autotune(1)  # rewrites the descriptor the graph reads; replays must stay exact
-        assert grab in (1, 2, 4, 8, 16)
+        assert grab in ((0,) if concurrent else (1, 2, 4, 8, 16))
         for rep in range(4):
-            print(f"replay {rep}", flush=True)
+            print(f"replay {rep} concurrent={concurrent}", flush=True)
             for d in bufs.values():
                 d["enc"].fill_(-1)
                 d["map"].fill_(-1)
@@ -132,6 +136,25 @@ def test_batched_config_switches(oracle, gpu_ctx, cfg):
     print(f"batched cfg={cfg}", flush=True)
     stats = gpu_ctx.match_levels_batched_device(jobs, tdag_amd.TransformDAGConfig(*cfg))
     _check_outputs(oracle, level_masks, bufs, stats, cfg)
+
+
+def test_batched_concurrent_subset_with_empty_level(oracle, gpu_ctx):
+    import tdag_amd
+
+    rng = np.random.default_rng(43)
+    level_masks = {
+        5: sets.planted_set(rng, 5, 6, 6, oracle.transform),
+        3: np.zeros((0, sets.words(3)), dtype=np.uint64),
+        2: sets.random_masks(rng, 2, 40, 0.3),
+    }
+    jobs, bufs = _device_jobs(level_masks)
+    print("concurrent batch with an empty level", flush=True)
+    b = gpu_ctx.batch(jobs, tdag_amd.TransformDAGConfig(True, True, True), use_graph=True, concurrent=True)
+    try:
+        stats, _ = b.run()
+    finally:
+        b.close()
+    _check_outputs(oracle, level_masks, bufs, stats)
 
 
 def test_batched_subsets_and_empty(oracle, gpu_ctx):
