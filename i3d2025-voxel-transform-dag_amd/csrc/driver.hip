@@ -307,7 +307,19 @@ static tdag_status constructTransformDag(tdag_context* ctx, const tdag_svdag* in
     tdag_tdag* res = new tdag_tdag();
     res->isStatic = staticForm;
     std::vector<uint32_t> translationParents;
-    res->t = tdag::hierarchicalDedup(dag, maxT + 1, translationParents, *cfg, threads, staticForm);
+    if (ctx) {
+        // The hierarchical pass on the device (same output as the host pass).
+        if (hipError_t e = hipSetDevice(ctx->device); e != hipSuccess) {
+            delete res;
+            return setHipErr(e, "hipSetDevice");
+        }
+        if (hipError_t e = tdag::hierarchicalDedupDevice(dag, maxT + 1, translationParents, *cfg, staticForm, ctx->stream, res->t); e != hipSuccess) {
+            delete res;
+            return setHipErr(e, "device hierarchical pass");
+        }
+    } else {
+        res->t = tdag::hierarchicalDedup(dag, maxT + 1, translationParents, *cfg, threads, staticForm);
+    }
     if (cfg->translation) {
         tdag::TDag& t = res->t;
         t.leaves.clear();

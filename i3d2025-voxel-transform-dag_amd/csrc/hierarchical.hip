@@ -348,6 +348,32 @@ void canonical2x2x2(const std::vector<uint32_t>& idList, Child out[256])
 
 } // namespace
 
+void hierarchicalTables(const tdag_config& cfg, HierTables& out)
+{
+    const Tables& T = tables();
+    std::memcpy(out.compose, T.compose, sizeof(out.compose));
+    std::memcpy(out.childPos, T.childPos, sizeof(out.childPos));
+    std::memcpy(out.required, T.required, sizeof(out.required));
+    std::memcpy(out.inverseId, T.inverseId, sizeof(out.inverseId));
+    std::vector<uint32_t> idList;
+    out.numIds = 0;
+    for (uint32_t id = 0; id < 48; ++id) {
+        if (!cfg.axis_permutation && (id >> 3) != 0)
+            continue;
+        if (!cfg.symmetry && (id & 7u) != 0)
+            continue;
+        idList.push_back(id);
+        out.ids[out.numIds++] = (uint8_t)id;
+    }
+    Child canon[256];
+    canonical2x2x2(idList, canon);
+    for (int b = 0; b < 256; ++b) {
+        out.leafPtr[b] = (uint32_t)canon[b].ptr;
+        out.leafInv[b] = canon[b].inv;
+        out.leafState[b] = (uint8_t)canon[b].state;
+    }
+}
+
 TDag hierarchicalDedup(const Svdag& in, uint32_t midLevel, std::vector<uint32_t>& outMidInv, const tdag_config& cfg, int threads, bool staticOffsets)
 {
     std::vector<uint32_t> idList;

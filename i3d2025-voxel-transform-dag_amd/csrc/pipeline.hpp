@@ -83,6 +83,26 @@ hipError_t flattenDevice(const DeviceSvdag& d, uint32_t level, const uint32_t* d
 TDag hierarchicalDedup(const Svdag& in, uint32_t midLevel, std::vector<uint32_t>& outMidInv, const tdag_config& cfg, int threads,
     bool staticOffsets = false);
 
+// Transform tables of the hierarchical pass (state = mirror bits | permutation code << 3, the
+// transform-ID numbering) and its 256-entry canonical 2x2x2 leaf table, for the device form.
+struct HierTables {
+    uint8_t compose[48][48]; // child state after applying transform id to state
+    uint8_t childPos[48][8]; // where child i lands under transform id
+    uint8_t required[48][48]; // invariance ID needed for state l == state r
+    uint8_t inverseId[48]; // pointer state stored on a hit with transform id
+    uint8_t ids[48]; // the allowed transform IDs, ascending
+    uint32_t numIds;
+    uint32_t leafPtr[256]; // canonical 2x2x2 table: representative byte, invariance, state
+    uint64_t leafInv[256];
+    uint8_t leafState[256];
+};
+void hierarchicalTables(const tdag_config& cfg, HierTables& out);
+
+// hierarchicalDedup computed on the device (same TDag and outMidInv, bit for bit) with work on
+// `stream`; workspace grown in (scratch, scratchBytes).
+hipError_t hierarchicalDedupDevice(const Svdag& in, uint32_t midLevel, std::vector<uint32_t>& outMidInv, const tdag_config& cfg, bool staticOffsets,
+    hipStream_t stream, TDag& out);
+
 // .tdag byte stream (voxelizer/src/main.cpp:147-150).
 std::vector<uint8_t> serializeTdag(const TDag& t, const tdag_config& cfg);
 

@@ -120,3 +120,24 @@ def test_static_config_switches(oracle, gpu_ctx, sym, axis):
     cfg = tdag_amd.TransformDAGConfig(sym, axis, True, 3)
     t = pl.construct_static_transform_dag(gpu_ctx, pl.StaticSvdag.from_svdag(pl.Svdag(bits, 128)), cfg)
     assert t.static_bytes() == _oracle_static_bytes(oracle, bits, 128, oracle.config(sym, axis, True, 3))
+
+
+@pytest.mark.parametrize("kind,R,seed", [(0, 64, 1), (1, 128, 3), (2, 256, 4), (1, 512, 7)])
+@pytest.mark.parametrize("sym,axis", [(True, True), (True, False), (False, True), (False, False)])
+def test_device_hierarchical_equals_host(gpu_ctx, kind, R, seed, sym, axis):
+    """The hierarchical pass on the device (canonical keys, min-index classes, sorted emission)
+    gives the host pass's output byte for byte, in the EditStructure and the StaticStructure form
+    (translation off: the driver is the hierarchical pass alone). The host pass is pinned to the
+    oracle's literal restatement by the CPU tests."""
+    import tdag_amd
+    from tdag_amd import pipeline as pl
+
+    print(f"device hierarchical kind={kind} R={R} seed={seed} sym={sym} axis={axis}", flush=True)
+    bits = pl.scene_dense(kind, R, seed)
+    svdag = pl.Svdag(bits, R)
+    cfg = tdag_amd.TransformDAGConfig(sym, axis, False, 3)
+    host = pl.construct_transform_dag(None, svdag, cfg).tdag_bytes()
+    dev = pl.construct_transform_dag(gpu_ctx, svdag, cfg).tdag_bytes()
+    assert dev == host
+    st = pl.StaticSvdag.from_svdag(svdag)
+    assert pl.construct_static_transform_dag(gpu_ctx, st, cfg).static_bytes() == pl.construct_static_transform_dag(None, st, cfg).static_bytes()
