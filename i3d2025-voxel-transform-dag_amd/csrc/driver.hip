@@ -86,30 +86,6 @@ tdag_status ensureDeviceCopy(tdag_context* ctx, const tdag_svdag* h)
     return TDAG_OK;
 }
 
-// Device buffers of one driver level, grown on demand and freed with the driver call.
-struct LevelBuffers {
-    void* p = nullptr;
-    size_t bytes = 0;
-    ~LevelBuffers()
-    {
-        if (p)
-            (void)hipFree(p);
-    }
-    hipError_t ensure(size_t need)
-    {
-        if (need <= bytes && p)
-            return hipSuccess;
-        if (p)
-            (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-        const hipError_t e = hipMalloc(&p, need);
-        if (e == hipSuccess)
-            bytes = need;
-        return e;
-    }
-};
-
 inline size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 
 } // namespace
@@ -241,10 +217,11 @@ tdag_status tdag_svdag_free(tdag_svdag* h)
 }
 
 // One exhaustive matcher level of the driver with its input flattened on the device: `items` are
-// the referenced level-`level` nodes in ascending order. The matcher's mapping and unique list
-// come back to the host; the masks only when they are kept.
+// the referenced level-`level` nodes in ascending order. Its device buffers live in the context's
+// staging buffer. The matcher's mapping and unique list come back to the host; the masks only when
+// they are kept.
 static tdag_status matchLevelOnDevice(tdag_context* ctx, const tdag_svdag* in, uint32_t level, const tdag_config* cfg, const std::vector<uint32_t>& items,
-    uint64_t W, LevelBuffers& bufs, bool keepMasks, std::vector<uint64_t>& outMasks, std::vector<uint64_t>& mapping, std::vector<uint32_t>& uniq, uint64_t& nu,
+    uint64_t W, bool keepMasks, std::vector<uint64_t>& outMasks, std::vector<uint64_t>& mapping, std::vector<uint32_t>& uniq, uint64_t& nu,
     tdag_match_stats& st)
 {
     const uint64_t n = items.size();
@@ -254,9 +231,9 @@ static tdag_status matchLevelOnDevice(tdag_context* ctx, const tdag_svdag* in, u
         return s;
     const size_t oMasks = align256(n * 4), oMap = align256(oMasks + n * W * 8), oUniq = align256(oMap + n * 8), oTemp = align256(oUniq + n * 4);
     const size_t total = oTemp + align256(tdag::flattenTempBytes(level, n));
-    if (hipError_t e = bufs.ensure(total); e != hipSuccess)
-        return setHipErr(e, "driver level buffers");
-    char* b = (char*)bufs.p;
+    if (tdag_status s = tdag::capi::ensureBuffer(ctx->io, ctx->ioBytes, total, "driver level buffers"); s != TDAG_OK)
+        return s;
+    char* b = (char*)ctx->io;
     uint32_t* d_items = (uint32_t*)b;
     uint64_t* d_masks = (uint64_t*)(b + oMasks);
     uint64_t* d_map = (uint64_t*)(b + oMap);
@@ -303,7 +280,6 @@ static tdag_status constructTransformDag(tdag_context* ctx, const tdag_svdag* in
     if (maxT + 1 > dag.rootLevel)
         return setErr(TDAG_EINVAL, "max_translation_level + 1 exceeds the root level");
     const int threads = hostThreads();
-    LevelBuffers bufs;
     tdag_tdag* res = new tdag_tdag();
     res->isStatic = staticForm;
     std::vector<uint32_t> translationParents;
@@ -313,7 +289,7 @@ static tdag_status constructTransformDag(tdag_context* ctx, const tdag_svdag* in
             delete res;
             return setHipErr(e, "hipSetDevice");
         }
-        if (hipError_t e = tdag::hierarchicalDedupDevice(dag, maxT + 1, translationParents, *cfg, staticForm, ctx->stream, res->t); e != hipSuccess) {
+        if (hipError_t e = tdag::hierarchicalDedupDevice(dag, maxT + 1, translationParents, *cfg, staticForm, ctx->stream, ctx->svdagScratch, ctx->svdagScratchBytes, res->t); e != hipSuccess) {
             delete res;
             return setHipErr(e, "device hierarchical pass");
         }
@@ -356,7 +332,7 @@ static tdag_status constructTransformDag(tdag_context* ctx, const tdag_svdag* in
             if (!learned || level != 2) {
                 // Flatten on the device straight into the matcher's input (FlatGridGenerator,
                 // transform_dag.cpp:196-204), match there, read back mapping and uniques.
-                s = matchLevelOnDevice(ctx, in, level, cfg, indices, W, bufs, (flags & TDAG_KEEP_LEVEL_INPUTS) != 0, rec.masks, mapping, uniq, nu, st);
+                s = matchLevelOnDevice(ctx, in, level, cfg, indices, W, (flags & TDAG_KEEP_LEVEL_INPUTS) != 0, rec.masks, mapping, uniq, nu, st);
             } else {
                 rec.masks.resize(indices.size() * W);
                 tdag::flattenNodes(dag, level, indices.data(), indices.size(), rec.masks.data(), threads);

@@ -131,7 +131,7 @@ __global__ __launch_bounds__(kHT) void kSeeds(const uint32_t* __restrict__ cptr,
 // Canonical key: 8 words ptr | normalised state << 40, smallest over the allowed IDs. The
 // transformed copy is assembled in a per-thread LDS row (child c goes to slot childPos[id][c]);
 // private arrays are only indexed by unrolled loop counters.
-__global__ __launch_bounds__(kHT) void kCanonKeys(const DevTables* __restrict__ gt, const uint32_t* __restrict__ cptr, const uint64_t* __restrict__ cinv,
+__global__ __launch_bounds__(kHT) void kCanonKeys(const DevTables* __restrict__ gt, const uint32_t* __restrict__ cptr, const uint8_t* __restrict__ normTab,
     const uint8_t* __restrict__ cst, uint32_t n, uint32_t numIds, uint64_t* __restrict__ keys)
 {
     __shared__ LdsTables T;
@@ -142,12 +142,10 @@ __global__ __launch_bounds__(kHT) void kCanonKeys(const DevTables* __restrict__ 
         return;
     uint64_t* row = rows + threadIdx.x * 8;
     uint32_t p[8];
-    uint64_t inv[8];
     uint32_t st[8];
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         p[c] = cptr[(size_t)i * 8 + c];
-        inv[c] = cinv[(size_t)i * 8 + c];
         st[c] = cst[(size_t)i * 8 + c];
     }
     uint64_t best[8];
@@ -159,13 +157,8 @@ __global__ __launch_bounds__(kHT) void kCanonKeys(const DevTables* __restrict__ 
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
             const uint32_t s = T.compose[id * 48 + st[c]];
-            uint32_t m = s; // smallest state of s's coset: s composed with each element of H
-            uint64_t h = inv[c] & ((1ull << 48) - 1);
-            while (h) {
-                const uint32_t e = (uint32_t)__builtin_ctzll(h);
-                h &= h - 1;
-                m = min(m, (uint32_t)T.cosetOf[s * 48 + e]);
-            }
+            // smallest state of s's coset under the child's invariance subgroup (per pointer)
+            const uint32_t m = p[c] == kEmptyPtr ? 0u : normTab[(size_t)p[c] * 48 + s];
             row[T.childPos[id * 8 + c]] = (uint64_t)p[c] | (uint64_t)m << 40;
         }
         int order = 0; // -1: row < best, 1: row > best
@@ -184,6 +177,24 @@ __global__ __launch_bounds__(kHT) void kCanonKeys(const DevTables* __restrict__ 
 #pragma unroll
     for (int c = 0; c < 8; ++c)
         keys[(size_t)i * 8 + c] = best[c];
+}
+
+// norm[h * 48 + s]: the smallest state of s's coset under the invariance subgroup inv[h] (the
+// states t with required[s][t] in inv[h]).
+__global__ __launch_bounds__(kHT) void kNormTable(const DevTables* __restrict__ gt, const uint64_t* __restrict__ inv, uint32_t count, uint8_t* __restrict__ norm)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * kHT + threadIdx.x;
+    if (t >= (uint64_t)count * 48)
+        return;
+    const uint32_t h = (uint32_t)(t / 48), s = (uint32_t)(t % 48);
+    uint64_t bits = inv[h] & ((1ull << 48) - 1);
+    uint32_t m = s;
+    while (bits) {
+        const uint32_t e = (uint32_t)__builtin_ctzll(bits);
+        bits &= bits - 1;
+        m = min(m, (uint32_t)gt->cosetOf[s * 48 + e]);
+    }
+    norm[t] = (uint8_t)m;
 }
 
 __device__ inline uint32_t hashKey(const uint64_t* k)
@@ -353,13 +364,14 @@ __device__ inline uint64_t stateToTpDev(uint32_t ptr, uint32_t state)
 // Kept node i is written at its handle: its 8 children as TransformPointers (inner levels) or its
 // leaf mask; `size` gets 1 + its child count (the packed node size of the static form).
 __global__ __launch_bounds__(kHT) void kOutput(const uint32_t* __restrict__ rep, const uint32_t* __restrict__ handle, uint32_t n, const uint32_t* __restrict__ cptr,
-    const uint8_t* __restrict__ cst, const uint64_t* __restrict__ leaves, uint64_t* __restrict__ outNodes, uint64_t* __restrict__ outLeaves,
-    uint32_t* __restrict__ size)
+    const uint8_t* __restrict__ cst, const uint64_t* __restrict__ leaves, const uint64_t* __restrict__ nodeInv, uint64_t* __restrict__ outNodes,
+    uint64_t* __restrict__ outLeaves, uint32_t* __restrict__ size, uint64_t* __restrict__ invByHandle)
 {
     const uint32_t i = blockIdx.x * kHT + threadIdx.x;
     if (i >= n || rep[i] != i)
         return;
     const uint32_t h = handle[i];
+    invByHandle[h] = nodeInv[i];
     if (leaves) {
         outLeaves[h] = leaves[i];
         return;
@@ -383,31 +395,25 @@ __global__ __launch_bounds__(kHT) void kMid(const uint32_t* __restrict__ mptr, c
     atomicMax(&mid[mptr[i]], i);
 }
 
-struct DevBuffers {
-    std::vector<void*> blocks;
-    hipError_t err = hipSuccess;
+// Carves the pass's buffers out of one workspace, 256-byte aligned; with a null base it only adds
+// up the size.
+struct Carve {
+    char* base = nullptr;
+    size_t off = 0;
     template <typename T>
     T* get(size_t count)
     {
-        void* p = nullptr;
-        if (err == hipSuccess)
-            err = hipMalloc(&p, std::max<size_t>(count * sizeof(T), 256));
-        if (err != hipSuccess)
-            return nullptr;
-        blocks.push_back(p);
-        return (T*)p;
-    }
-    ~DevBuffers()
-    {
-        for (void* p : blocks)
-            (void)hipFree(p);
+        off = (off + 255) & ~size_t(255);
+        T* p = base ? (T*)(base + off) : nullptr;
+        off += std::max<size_t>(count * sizeof(T), 8);
+        return p;
     }
 };
 
 } // namespace
 
 hipError_t hierarchicalDedupDevice(const Svdag& in, uint32_t midLevel, std::vector<uint32_t>& outMidInv, const tdag_config& cfg, bool staticOffsets,
-    hipStream_t st, TDag& out)
+    hipStream_t st, void*& work, size_t& workBytes, TDag& out)
 {
     out = TDag {};
     out.resolution = in.resolution;
@@ -434,38 +440,69 @@ hipError_t hierarchicalDedupDevice(const Svdag& in, uint32_t midLevel, std::vect
     uint32_t cap = 64;
     while (cap < 2 * maxN)
         cap <<= 1;
-    DevBuffers B;
-    DevTables* dT = B.get<DevTables>(1);
-    uint32_t* lptr = B.get<uint32_t>(256);
-    uint64_t* linv = B.get<uint64_t>(256);
-    uint8_t* lst = B.get<uint8_t>(256);
-    uint64_t* leaves = B.get<uint64_t>(in.leaves.size());
-    uint32_t* child = B.get<uint32_t>(maxN * 8);
-    uint32_t* cptr = B.get<uint32_t>(maxN * 8);
-    uint64_t* cinv = B.get<uint64_t>(maxN * 8);
-    uint8_t* cst = B.get<uint8_t>(maxN * 8);
-    uint64_t* seed = B.get<uint64_t>(maxN);
-    uint32_t* idx = B.get<uint32_t>(maxN);
-    void* sortScratch = B.get<uint8_t>(radixScratchBytes(maxN, 8));
-    uint64_t* keys = B.get<uint64_t>(maxN * 8);
-    uint32_t* table = B.get<uint32_t>(cap);
-    uint32_t* rep = B.get<uint32_t>(maxN);
-    uint8_t* hitId = B.get<uint8_t>(maxN);
-    uint64_t* nodeInv = B.get<uint64_t>(maxN);
-    uint32_t* kf = B.get<uint32_t>(maxN);
-    uint32_t* pos = B.get<uint32_t>(maxN);
-    void* scanScratch = B.get<uint8_t>(scanScratchBytes(maxN));
-    uint32_t* handle = B.get<uint32_t>(maxN);
-    uint32_t* mptr[2] = { B.get<uint32_t>(maxN), B.get<uint32_t>(maxN) };
-    uint64_t* minv[2] = { B.get<uint64_t>(maxN), B.get<uint64_t>(maxN) };
-    uint8_t* mst[2] = { B.get<uint8_t>(maxN), B.get<uint8_t>(maxN) };
-    uint64_t* outBuf = B.get<uint64_t>(maxN * 8);
-    uint32_t* sizes = B.get<uint32_t>(maxN);
-    uint32_t* offsets = B.get<uint32_t>(maxN);
-    uint32_t* mid = B.get<uint32_t>(maxN);
-    uint32_t* flags = B.get<uint32_t>(4); // [0] scan total, [1] bad
-    if (B.err != hipSuccess)
-        return B.err;
+    uint32_t* lptr;
+    uint64_t* linv;
+    uint8_t* lst;
+    uint64_t* leaves;
+    uint32_t *child, *cptr, *idx, *table, *rep, *kf, *pos, *handle, *sizes, *offsets, *mid, *flags;
+    uint64_t *cinv, *seed, *keys, *nodeInv, *outBuf, *invByHandle;
+    uint8_t *cst, *hitId, *normTab;
+    void *sortScratch, *scanScratch;
+    DevTables* dT;
+    uint32_t* mptr[2];
+    uint64_t* minv[2];
+    uint8_t* mst[2];
+    auto carveAll = [&](Carve& B) {
+        dT = B.get<DevTables>(1);
+        lptr = B.get<uint32_t>(256);
+        linv = B.get<uint64_t>(256);
+        lst = B.get<uint8_t>(256);
+        leaves = B.get<uint64_t>(in.leaves.size());
+        child = B.get<uint32_t>(maxN * 8);
+        cptr = B.get<uint32_t>(maxN * 8);
+        cinv = B.get<uint64_t>(maxN * 8);
+        cst = B.get<uint8_t>(maxN * 8);
+        seed = B.get<uint64_t>(maxN);
+        idx = B.get<uint32_t>(maxN);
+        sortScratch = B.get<uint8_t>(radixScratchBytes(maxN, 8));
+        keys = B.get<uint64_t>(maxN * 8);
+        table = B.get<uint32_t>(cap);
+        rep = B.get<uint32_t>(maxN);
+        hitId = B.get<uint8_t>(maxN);
+        nodeInv = B.get<uint64_t>(maxN);
+        kf = B.get<uint32_t>(maxN);
+        pos = B.get<uint32_t>(maxN);
+        scanScratch = B.get<uint8_t>(scanScratchBytes(maxN));
+        handle = B.get<uint32_t>(maxN);
+        for (int q = 0; q < 2; ++q) {
+            mptr[q] = B.get<uint32_t>(maxN);
+            minv[q] = B.get<uint64_t>(maxN);
+            mst[q] = B.get<uint8_t>(maxN);
+        }
+        outBuf = B.get<uint64_t>(maxN * 8);
+        sizes = B.get<uint32_t>(maxN);
+        offsets = B.get<uint32_t>(maxN);
+        mid = B.get<uint32_t>(maxN);
+        normTab = B.get<uint8_t>(std::max<size_t>(maxN, 256) * 48);
+        invByHandle = B.get<uint64_t>(maxN);
+        flags = B.get<uint32_t>(4); // [0] scan total, [1] inconsistent-class flag, [2] offsets total
+    };
+    Carve sizing;
+    carveAll(sizing);
+    if (workBytes < sizing.off || !work) {
+        if (work) {
+            if (hipError_t e0 = hipStreamSynchronize(st); e0 != hipSuccess)
+                return e0;
+            (void)hipFree(work);
+            work = nullptr;
+            workBytes = 0;
+        }
+        if (hipError_t e0 = hipMalloc(&work, sizing.off); e0 != hipSuccess)
+            return e0;
+        workBytes = sizing.off;
+    }
+    Carve B { (char*)work, 0 };
+    carveAll(B);
     hipError_t e;
 #define HD_TRY(x)                                                                                                                                               \
     do {                                                                                                                                                        \
@@ -479,6 +516,18 @@ hipError_t hierarchicalDedupDevice(const Svdag& in, uint32_t midLevel, std::vect
     if (!in.leaves.empty())
         HD_TRY(hipMemcpyAsync(leaves, in.leaves.data(), in.leaves.size() * 8, hipMemcpyHostToDevice, st));
     HD_TRY(fillU32(flags, 4, 0u, st));
+    // Leaf children point at canonical 2x2x2 bytes: their normalisation table comes from the
+    // 256-entry table's invariance masks.
+    std::vector<uint8_t> leafNorm(256 * 48);
+    for (uint32_t b = 0; b < 256; ++b)
+        for (uint32_t s0 = 0; s0 < 48; ++s0) {
+            uint32_t m = s0;
+            for (uint32_t e0 = 0; e0 < 48; ++e0)
+                if ((H.leafInv[b] >> e0) & 1ull)
+                    m = std::min<uint32_t>(m, t.cosetOf[s0 * 48 + e0]);
+            leafNorm[b * 48 + s0] = (uint8_t)m;
+        }
+    HD_TRY(hipMemcpyAsync(normTab, leafNorm.data(), leafNorm.size(), hipMemcpyHostToDevice, st));
     bool haveOffsets = false; // offsets of the previous level's output nodes are valid
     int cur = 0; // mapping buffers of the level just processed
     for (uint32_t level = 2; level <= std::max(2u, in.rootLevel); ++level) {
@@ -496,7 +545,7 @@ hipError_t hierarchicalDedupDevice(const Svdag& in, uint32_t midLevel, std::vect
             hipLaunchKernelGGL(kSeeds, dim3(gridOf(n)), dim3(kHT), 0, st, cptr, n, hashOf, seed, idx);
             HD_TRY(hipGetLastError());
             HD_TRY(radixSortPairsU64(seed, idx, n, 0, 64, sortScratch, st));
-            hipLaunchKernelGGL(kCanonKeys, dim3(gridOf(n)), dim3(kHT), 0, st, dT, cptr, cinv, cst, n, t.numIds, keys);
+            hipLaunchKernelGGL(kCanonKeys, dim3(gridOf(n)), dim3(kHT), 0, st, dT, cptr, normTab, cst, n, t.numIds, keys);
             uint32_t lcap = 64;
             while (lcap < 2 * (uint64_t)n)
                 lcap <<= 1;
@@ -509,7 +558,8 @@ hipError_t hierarchicalDedupDevice(const Svdag& in, uint32_t midLevel, std::vect
             HD_TRY(exclusiveScanU32(kf, pos, n, flags, scanScratch, st));
             hipLaunchKernelGGL(kHandles, dim3(gridOf(n)), dim3(kHT), 0, st, idx, kf, pos, n, handle);
             hipLaunchKernelGGL(kMapping, dim3(gridOf(n)), dim3(kHT), 0, st, dT, rep, hitId, nodeInv, handle, n, mptr[nxt], minv[nxt], mst[nxt]);
-            hipLaunchKernelGGL(kOutput, dim3(gridOf(n)), dim3(kHT), 0, st, rep, handle, n, cptr, cst, level == 2 ? leaves : nullptr, outBuf, outBuf, sizes);
+            hipLaunchKernelGGL(kOutput, dim3(gridOf(n)), dim3(kHT), 0, st, rep, handle, n, cptr, cst, level == 2 ? leaves : nullptr, nodeInv, outBuf, outBuf, sizes,
+                invByHandle);
             HD_TRY(hipGetLastError());
             uint32_t hk[2] = { 0, 0 };
             HD_TRY(hipMemcpyAsync(hk, flags, 8, hipMemcpyDeviceToHost, st));
@@ -517,6 +567,9 @@ hipError_t hierarchicalDedupDevice(const Svdag& in, uint32_t midLevel, std::vect
             if (hk[1])
                 return hipErrorUnknown; // a non-kept node without a mapping transform: the keys rule it out
             nk = hk[0];
+            // The next level's children point at these nk kept nodes.
+            hipLaunchKernelGGL(kNormTable, dim3(gridOf((uint64_t)nk * 48)), dim3(kHT), 0, st, dT, invByHandle, nk, normTab);
+            HD_TRY(hipGetLastError());
             if (level == 2) {
                 out.leaves.resize(nk);
                 HD_TRY(hipMemcpyAsync(out.leaves.data(), outBuf, (size_t)nk * 8, hipMemcpyDeviceToHost, st));
