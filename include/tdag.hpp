@@ -104,6 +104,22 @@ inline Svdag svdagFromDense(const uint8_t* bits, uint32_t resolution)
     return Svdag(s);
 }
 
+// EditStructure(VoxelGrid) + toDAG built on the device from a dense grid already in device
+// memory (resolution^3 / 8 bytes, bit i = x + R*y + R*R*z); the same SVDAG as svdagFromDense.
+inline Svdag svdagFromDenseDevice(const uint8_t* d_bits, uint32_t resolution, Context& ctx = Context::instance(), float* out_ms = nullptr)
+{
+    tdag_svdag* s = nullptr;
+    check(tdag_svdag_from_dense_device(ctx.get(), d_bits, resolution, &s, out_ms), "tdag_svdag_from_dense_device");
+    return Svdag(s);
+}
+
+// FlatGridGenerator / createLargeSubGrid on the device: d_nodes (n level-`level` node indices)
+// -> d_out (n * W words, the LargeSubGrid layout tdag_match_level_device reads).
+inline void flattenDevice(const tdag_svdag* svdag, uint32_t level, const uint32_t* d_nodes, uint64_t n, uint64_t* d_out, Context& ctx = Context::instance())
+{
+    check(tdag_svdag_flatten_device(ctx.get(), svdag, level, d_nodes, n, d_out), "tdag_svdag_flatten_device");
+}
+
 // constructTransformDAG (transform_dag.h:161-162) on the GPU matcher.
 inline TDag constructTransformDAG(const tdag_svdag* inDag, const tdag_config& config, Context& ctx = Context::instance())
 {
