@@ -222,11 +222,21 @@ tdag_status tdag_svdag_free(tdag_svdag* h)
 // they are kept.
 static tdag_status matchLevelOnDevice(tdag_context* ctx, const tdag_svdag* in, uint32_t level, const tdag_config* cfg, const std::vector<uint32_t>& items,
     uint64_t W, bool keepMasks, std::vector<uint64_t>& outMasks, std::vector<uint64_t>& mapping, std::vector<uint32_t>& uniq, uint64_t& nu,
-    tdag_match_stats& st)
+    tdag_match_stats& st, tdag_mlp* mlp = nullptr, uint32_t k = 0, tdag_learned_stats* ls = nullptr)
 {
     const uint64_t n = items.size();
+    // With mlp: the learned level-2 matcher (proposals, verification, exhaustive fallback).
+    auto match = [&](const uint64_t* d_masks, uint64_t* d_map, uint32_t* d_uniq) -> tdag_status {
+        if (!mlp)
+            return tdag_match_level_device(ctx, level, d_masks, n, cfg, nullptr, d_map, d_uniq, &nu, nullptr, &st);
+        const tdag_status s = tdag_match_level_learned_device(ctx, mlp, level, d_masks, n, cfg, k, nullptr, d_map, d_uniq, ls);
+        nu = ls->num_unique;
+        st.num_matched = ls->num_matched;
+        st.ms_total = ls->ms_total;
+        return s;
+    };
     if (n == 0)
-        return tdag_match_level_device(ctx, level, nullptr, 0, cfg, nullptr, nullptr, nullptr, &nu, nullptr, &st);
+        return match(nullptr, nullptr, nullptr);
     if (tdag_status s = ensureDeviceCopy(ctx, in); s != TDAG_OK)
         return s;
     const size_t oMasks = align256(n * 4), oMap = align256(oMasks + n * W * 8), oUniq = align256(oMap + n * 8), oTemp = align256(oUniq + n * 4);
@@ -243,7 +253,7 @@ static tdag_status matchLevelOnDevice(tdag_context* ctx, const tdag_svdag* in, u
         e = tdag::flattenDevice(in->dev, level, d_items, n, d_masks, (uint64_t*)(b + oTemp), ctx->stream);
     if (e != hipSuccess)
         return setHipErr(e, "device flatten");
-    if (tdag_status s = tdag_match_level_device(ctx, level, d_masks, n, cfg, nullptr, d_map, d_uniq, &nu, nullptr, &st); s != TDAG_OK)
+    if (tdag_status s = match(d_masks, d_map, d_uniq); s != TDAG_OK)
         return s;
     mapping.resize(n);
     uniq.resize(n);
@@ -329,17 +339,14 @@ static tdag_status constructTransformDag(tdag_context* ctx, const tdag_svdag* in
             uint64_t nu = 0;
             tdag_match_stats st {};
             tdag_status s;
-            if (!learned || level != 2) {
-                // Flatten on the device straight into the matcher's input (FlatGridGenerator,
-                // transform_dag.cpp:196-204), match there, read back mapping and uniques.
-                s = matchLevelOnDevice(ctx, in, level, cfg, indices, W, (flags & TDAG_KEEP_LEVEL_INPUTS) != 0, rec.masks, mapping, uniq, nu, st);
-            } else {
-                rec.masks.resize(indices.size() * W);
-                tdag::flattenNodes(dag, level, indices.data(), indices.size(), rec.masks.data(), threads);
-                tdag_learned_stats ls {};
-                s = tdag_match_level_learned(ctx, mlp, level, rec.masks.data(), indices.size(), cfg, k, nullptr, mapping.data(), uniq.data(), &nu, &ls);
-                st.num_matched = ls.num_matched;
-                st.ms_total = ls.ms_total;
+            // Flatten on the device straight into the matcher's input (FlatGridGenerator,
+            // transform_dag.cpp:196-204), match there (the learned matcher at level 2 of the
+            // learned driver), read back mapping and uniques.
+            const bool useLearned = learned && level == 2;
+            tdag_learned_stats ls {};
+            s = matchLevelOnDevice(ctx, in, level, cfg, indices, W, (flags & TDAG_KEEP_LEVEL_INPUTS) != 0, rec.masks, mapping, uniq, nu, st,
+                useLearned ? mlp : nullptr, k, &ls);
+            if (useLearned) {
                 rec.learned = true;
                 rec.proposalHits = ls.proposal_hits;
                 rec.fallback = ls.fallback_queries;
