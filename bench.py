@@ -88,6 +88,7 @@ def build_workload(ctx, resolution: int, seed: int, max_level: int, kind: int = 
     tc = time.perf_counter()
     pl.construct_transform_dag(ctx, svdag, cfg).close()  # again, with the context's buffers in place
     construct_warm_ms = (time.perf_counter() - tc) * 1e3
+    tdag16 = encode16_and_decode(ctx, tdag, bits, resolution)
     levels = {}
     for level in range(max_level, 1, -1):
         masks = tdag.level_masks(level)
@@ -105,8 +106,40 @@ def build_workload(ctx, resolution: int, seed: int, max_level: int, kind: int = 
         # the process (it also allocates the context's buffers), then a second one.
         "construct_ms": round(construct_ms, 1),
         "construct_warm_ms": round(construct_warm_ms, 1),
+        "tdag16": tdag16,
     }
     return cfg, levels, info
+
+
+def encode16_and_decode(ctx, tdag, bits, resolution: int):
+    """constructTransformDAG16 of the result on the host (pointer tables + Huffman), then every voxel
+    looked up in the encoded DAG on the device and compared with the input grid."""
+    import numpy as np
+    import torch
+
+    plain = tdag.encode16(pointer_tables=False, huffman=False)
+    bytes_plain = len(plain.to_bytes())
+    plain.close()
+    t0 = time.perf_counter()
+    e = tdag.encode16(pointer_tables=True, huffman=True)
+    encode_ms = (time.perf_counter() - t0) * 1e3
+    data = e.to_bytes()
+    d_out, _ = e.decode_device(ctx)  # first call copies the encoded DAG to the device
+    ms = [e.decode_device(ctx, d_out)[1] for _ in range(3)]
+    expect = torch.from_numpy(np.ascontiguousarray(bits).view(np.int32)).to(d_out.device)
+    equal = bool(torch.equal(d_out, expect))
+    e.close()
+    avg = sum(ms) / len(ms)
+    return {
+        "bytes": len(data),
+        "bytes_without_tables_or_huffman": bytes_plain,
+        "encode_ms_host": round(encode_ms, 1),
+        "decode_ms": round(avg, 3),
+        "decode_Gvoxels_per_s": round(resolution ** 3 / (avg * 1e-3) / 1e9, 2) if avg > 0 else None,
+        "decoded_equal_to_input": equal,
+        "note": "decode = the reference's per-voxel lookup (transform_dag_encoding.cpp:897-942) for all "
+                "resolution^3 voxels on the device, kernel time from hipEvents",
+    }
 
 
 def device_svdag_build(ctx, bits, resolution: int, host_svdag, host_build_ms: float):
@@ -488,6 +521,7 @@ def main():
             "matches_per_step": per_step_matches,
             **({"svdag_build": info["svdag_build"]} if info.get("svdag_build") else {}),
             "construct_transform_dag_ms": {"first_call": info["construct_ms"], "second_call": info["construct_warm_ms"]},
+            "tdag16": info["tdag16"],
             "levels": per_level,
             "search_variants": {l: ctx.get_variant(l) for l in order if l >= 3},
             "autotune": tuning,

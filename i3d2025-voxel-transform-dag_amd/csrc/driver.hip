@@ -541,6 +541,24 @@ tdag_status tdag_tdag_info(const tdag_tdag* t, uint32_t* root_level, uint64_t* l
     return TDAG_OK;
 }
 
+tdag_status tdag_tdag16_encode(const tdag_tdag* t, const tdag_config* cfg, uint32_t flags, tdag_tdag16** out)
+{
+    if (!t || !cfg || !out)
+        return setErr(TDAG_EINVAL, "null argument");
+    if (t->isStatic)
+        return setErr(TDAG_EINVAL, "encode the result of tdag_construct_transform_dag (edit-structure form)");
+    if (flags & ~(TDAG16_POINTER_TABLES | TDAG16_HUFFMAN))
+        return setErr(TDAG_EINVAL, "unknown flag");
+    try {
+        *out = tdag::newTdag16Handle(tdag::encodeTdag16(t->t, *cfg, (flags & TDAG16_POINTER_TABLES) != 0, (flags & TDAG16_HUFFMAN) != 0));
+    } catch (const std::bad_alloc&) {
+        return setErr(TDAG_ENOMEM, "out of host memory");
+    } catch (const std::exception& e) {
+        return setErr(TDAG_EINVAL, e.what());
+    }
+    return TDAG_OK;
+}
+
 tdag_status tdag_tdag_free(tdag_tdag* t)
 {
     delete t;

@@ -34,6 +34,28 @@ struct TDag {
     std::vector<uint64_t> leaves;
 };
 
+// The .tdag16 compact encoding (TransformDAG16, transform_dag_encoding.h:138-153): 16-bit node
+// words per level, leaves, per-level pointer tables and Huffman code words (LSB first).
+struct HuffCode {
+    uint64_t code;
+    uint32_t bits;
+};
+struct Tdag16 {
+    tdag_config cfg {};
+    bool tables = false, huffman = false;
+    uint32_t resolution = 0, rootLevel = 0;
+    std::vector<std::vector<uint16_t>> levels;
+    std::vector<uint64_t> subgrids;
+    std::vector<std::vector<uint64_t>> ptrTables;
+    std::vector<std::vector<HuffCode>> codes; // empty when the Huffman code is off
+};
+std::vector<uint32_t> huffmanCodeLengths(const std::vector<uint64_t>& freq, uint32_t maxLen);
+std::vector<HuffCode> huffmanCodeWords(const std::vector<uint32_t>& len, bool lsbFirst);
+Tdag16 encodeTdag16(const TDag& in, const tdag_config& cfg, bool tables, bool huffman);
+std::vector<uint8_t> serializeTdag16(const Tdag16& d);
+bool parseTdag16(const uint8_t* bytes, size_t len, Tdag16& out, std::string& err);
+tdag_tdag16* newTdag16Handle(Tdag16&& e);
+
 // Per-level record of one constructTransformDAG run.
 struct LevelRecord {
     uint32_t level = 0;

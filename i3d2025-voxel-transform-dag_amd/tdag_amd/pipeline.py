@@ -7,6 +7,8 @@ Mirrors the reference's construction API:
 * :meth:`TDag.tdag_bytes` ~ the voxelizer's ``.tdag`` output (voxelizer/src/main.cpp:144-151)
 * :class:`StaticSvdag` ~ ``voxcom::StaticStructure<void, uint32_t>`` (structure.h:299-320)
 * :func:`construct_static_transform_dag` ~ ``constructStaticTransformDAG`` (transform_dag.h:163)
+* :class:`TDag16` ~ ``TransformDAG16`` and ``constructTransformDAG16``
+  (tools/svdag_encoding/src/transform_dag_encoding.h:138-153, transform_dag_encoding.cpp:241-478)
 """
 from __future__ import annotations
 
@@ -21,6 +23,8 @@ _u32p = ctypes.POINTER(ctypes.c_uint32)
 _u8p = ctypes.POINTER(ctypes.c_uint8)
 
 KEEP_LEVEL_INPUTS = 1
+POINTER_TABLES = 1  # TDAG16_POINTER_TABLES
+HUFFMAN = 2  # TDAG16_HUFFMAN
 
 
 class StaticSvdagView(ctypes.Structure):
@@ -72,6 +76,14 @@ def bind(L):
     L.tdag_construct_static_transform_dag.argtypes = [vp, ctypes.POINTER(StaticSvdagView), ctypes.POINTER(TdagConfig), ctypes.c_uint32, ctypes.POINTER(vp)]
     L.tdag_tdag_static_words.argtypes = [vp, ctypes.c_uint32, _u64p, ctypes.c_uint64, _u64p]
     L.tdag_tdag_static_serialize.argtypes = [vp, _u8p, ctypes.c_uint64, _u64p]
+    L.tdag_tdag16_encode.argtypes = [vp, ctypes.POINTER(TdagConfig), ctypes.c_uint32, ctypes.POINTER(vp)]
+    L.tdag_tdag16_from_bytes.argtypes = [ctypes.c_char_p, ctypes.c_uint64, ctypes.POINTER(vp)]
+    L.tdag_tdag16_serialize.argtypes = [vp, _u8p, ctypes.c_uint64, _u64p]
+    L.tdag_tdag16_info.argtypes = [vp, _u32p, _u32p, _u64p, _u64p, ctypes.c_uint32]
+    L.tdag_tdag16_decode_device.argtypes = [vp, vp, vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_float)]
+    L.tdag_tdag16_free.argtypes = [vp]
+    L.tdag_huffman_code_lengths.argtypes = [_u64p, ctypes.c_uint32, ctypes.c_uint32, _u32p]
+    L.tdag_huffman_code_words.argtypes = [_u32p, ctypes.c_uint32, ctypes.c_int, _u64p]
     for name in ("tdag_scene_dense", "tdag_svdag_from_dense", "tdag_svdag_info", "tdag_svdag_copy_level", "tdag_svdag_flatten",
                  "tdag_svdag_free", "tdag_construct_transform_dag", "tdag_tdag_serialize", "tdag_tdag_info",
                  "tdag_tdag_level_record", "tdag_tdag_level_masks", "tdag_tdag_free", "tdag_svdag_static_words",
@@ -227,6 +239,14 @@ class TDag:
         check(lib().tdag_tdag_level_masks(self._h, level, out.ctypes.data_as(_u64p), out.size), "tdag_tdag_level_masks")
         return out[: rec.items]
 
+    def encode16(self, pointer_tables: bool = True, huffman: bool = True) -> "TDag16":
+        """constructTransformDAG16 of this result (transform_dag_encoding.cpp:241-478)."""
+        cfg = self.config.to_c()
+        h = ctypes.c_void_p()
+        flags = (POINTER_TABLES if pointer_tables else 0) | (HUFFMAN if huffman else 0)
+        check(lib().tdag_tdag16_encode(self._h, ctypes.byref(cfg), flags, ctypes.byref(h)), "tdag_tdag16_encode")
+        return TDag16(h)
+
     def close(self):
         if self._h:
             lib().tdag_tdag_free(self._h)
@@ -296,3 +316,79 @@ def construct_static_transform_dag(ctx, static: StaticSvdag, config: TransformDA
           "tdag_construct_static_transform_dag")
     del keep
     return TDag(h, config)
+
+
+class TDag16:
+    """TransformDAG16: the .tdag16 compact encoding (transform_dag_encoding.h:138-153)."""
+
+    def __init__(self, handle):
+        self._h = handle
+        res, root = ctypes.c_uint32(), ctypes.c_uint32()
+        words = np.zeros(32, dtype=np.uint64)
+        tables = np.zeros(32, dtype=np.uint64)
+        check(lib().tdag_tdag16_info(self._h, ctypes.byref(res), ctypes.byref(root), words.ctypes.data_as(_u64p),
+                                     tables.ctypes.data_as(_u64p), 32), "tdag_tdag16_info")
+        self.resolution = int(res.value)
+        self.root_level = int(root.value)
+        self.level_words = [int(v) for v in words[: self.root_level + 1]]
+        self.table_entries = [int(v) for v in tables[: self.root_level + 1]]
+
+    @classmethod
+    def from_bytes(cls, data: bytes) -> "TDag16":
+        """TransformDAG16::readFrom (transform_dag_encoding.cpp:1029-1036)."""
+        h = ctypes.c_void_p()
+        check(lib().tdag_tdag16_from_bytes(data, len(data), ctypes.byref(h)), "tdag_tdag16_from_bytes")
+        return cls(h)
+
+    def to_bytes(self) -> bytes:
+        """TransformDAG16::writeTo (transform_dag_encoding.cpp:1020-1027)."""
+        size = ctypes.c_uint64()
+        check(lib().tdag_tdag16_serialize(self._h, None, 0, ctypes.byref(size)), "tdag_tdag16_serialize")
+        buf = np.zeros(max(1, int(size.value)), dtype=np.uint8)
+        check(lib().tdag_tdag16_serialize(self._h, buf.ctypes.data_as(_u8p), buf.size, ctypes.byref(size)), "tdag_tdag16_serialize")
+        return buf[: int(size.value)].tobytes()
+
+    def decode_device(self, ctx, d_out=None):
+        """Every voxel through TranslateTransformBaseDAG::get on the GPU (transform_dag_encoding.cpp:
+        897-942). Returns (d_out, ms): d_out an int32 device tensor of R^3/32 words, bit i = x + R*y
+        + R*R*z; ms the device time of the decode kernel."""
+        import torch
+
+        R = self.resolution
+        n = max(1, R * R * R // 32)
+        if d_out is None:
+            d_out = torch.empty(n, dtype=torch.int32, device=f"cuda:{ctx.device}")
+        if not (d_out.is_cuda and d_out.element_size() == 4 and d_out.is_contiguous() and d_out.numel() >= n):
+            raise ValueError("d_out must be a contiguous 4-byte device tensor of resolution^3 / 32 words")
+        torch.cuda.synchronize()
+        ms = ctypes.c_float()
+        check(lib().tdag_tdag16_decode_device(ctx._ptr, self._h, ctypes.c_void_p(d_out.data_ptr()), d_out.numel(), ctypes.byref(ms)),
+              "tdag_tdag16_decode_device")
+        return d_out, float(ms.value)
+
+    def close(self):
+        if self._h:
+            lib().tdag_tdag16_free(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def huffman_code_lengths(freq, max_len: int) -> np.ndarray:
+    """createLengthLimitedHuffmanCodeLengths (huffman.cpp:113-183)."""
+    f = np.ascontiguousarray(freq, dtype=np.uint64)
+    out = np.zeros(len(f), dtype=np.uint32)
+    check(lib().tdag_huffman_code_lengths(f.ctypes.data_as(_u64p), len(f), max_len, out.ctypes.data_as(_u32p)), "tdag_huffman_code_lengths")
+    return out
+
+
+def huffman_code_words(lengths, lsb_first: bool = False) -> np.ndarray:
+    """assignCodeWords (huffman.cpp:93-111), optionally convertHuffmanToLSB (192-202)."""
+    ln = np.ascontiguousarray(lengths, dtype=np.uint32)
+    out = np.zeros(len(ln), dtype=np.uint64)
+    check(lib().tdag_huffman_code_words(ln.ctypes.data_as(_u32p), len(ln), int(lsb_first), out.ctypes.data_as(_u64p)), "tdag_huffman_code_words")
+    return out

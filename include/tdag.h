@@ -343,6 +343,40 @@ tdag_status tdag_tdag_level_masks(const tdag_tdag* tdag, uint32_t level, uint64_
 tdag_status tdag_tdag_free(tdag_tdag* tdag);
 
 /* ------------------------------------------------------------------------------------------
+ * .tdag16 compact encoding (tools/svdag_encoding: TransformDAG16, transform_dag_encoding.h:138-153).
+ * ------------------------------------------------------------------------------------------ */
+typedef struct tdag_tdag16 tdag_tdag16;
+#define TDAG16_POINTER_TABLES 1u /* TransformEncodingConfig::pointerTables */
+#define TDAG16_HUFFMAN 2u /* TransformEncodingConfig::huffmanCode */
+/* constructTransformDAG16(EditStructure<void, TransformPointer>, ...) (transform_dag_encoding.cpp:
+ * 241-478): nodes ordered by reference count with unreferenced nodes pruned, 16-bit node words,
+ * optional pointer tables and per-level Huffman code of the pointer types. Pointers of equal
+ * reference count keep the order of their first use (the reference takes an unordered_map's
+ * order there), so the bytes can differ from the reference's while the volume is the same.
+ * Takes the result of tdag_construct_transform_dag (not of the static driver). */
+tdag_status tdag_tdag16_encode(const tdag_tdag* tdag, const tdag_config* cfg, uint32_t flags, tdag_tdag16** out);
+/* TransformDAG16::readFrom of a .tdag16 byte stream (transform_dag_encoding.cpp:1029-1036). */
+tdag_status tdag_tdag16_from_bytes(const uint8_t* bytes, uint64_t len, tdag_tdag16** out);
+/* TransformDAG16::writeTo (transform_dag_encoding.cpp:1020-1027): pass buf = NULL to query the size. */
+tdag_status tdag_tdag16_serialize(const tdag_tdag16* h, uint8_t* buf, uint64_t cap, uint64_t* out_size);
+/* Resolution, root level, and for levels [0, cap): 16-bit words per level (level 2: 4 per leaf)
+ * and pointer-table entries per child level. Any output pointer may be NULL. */
+tdag_status tdag_tdag16_info(const tdag_tdag16* h, uint32_t* resolution, uint32_t* root_level, uint64_t* level_words, uint64_t* table_entries, uint32_t cap);
+/* The voxel lookup TranslateTransformBaseDAG::get (transform_dag_encoding.cpp:897-942) for every
+ * voxel, on the device: writes the dense grid (bit i = x + R*y + R*R*z; resolution^3 / 32 uint32
+ * words, d_out_words in device memory). The encoded DAG is copied to the device on first use and
+ * kept with the handle. Malformed data (an index outside its level, table or leaf array) gives
+ * TDAG_EINVAL. out_ms (optional): device time of the decode kernel. Returns after the work on
+ * ctx's stream is done. */
+tdag_status tdag_tdag16_decode_device(tdag_context* ctx, const tdag_tdag16* h, uint32_t* d_out_words, uint64_t out_words, float* out_ms);
+tdag_status tdag_tdag16_free(tdag_tdag16* h);
+/* createLengthLimitedHuffmanCodeLengths (huffman.cpp:113-183): package-merge code lengths of at
+ * most max_len bits for n symbols. */
+tdag_status tdag_huffman_code_lengths(const uint64_t* freq, uint32_t n, uint32_t max_len, uint32_t* out_len);
+/* assignCodeWords (huffman.cpp:93-111), then convertHuffmanToLSB (192-202) when lsb_first. */
+tdag_status tdag_huffman_code_words(const uint32_t* len, uint32_t n, int lsb_first, uint64_t* out_code);
+
+/* ------------------------------------------------------------------------------------------
  * StaticStructure form: constructStaticTransformDAG (transform_dag.h:163).
  * ---------------------------------------------------------------------------------------- */
 

@@ -26,6 +26,11 @@ int orc_corner_hash(uint32_t level, const uint64_t* in, uint32_t dirBits, uint64
 int orc_match_encoded(uint32_t level, const uint64_t* masks, uint64_t n, const orc_config* cfg, uint64_t* outEncoded, int nthreads, int brute);
 int orc_postprocess(uint32_t level, const uint64_t* masks, uint64_t n, const uint64_t* encoded, uint64_t* outMapping, uint32_t* outUnique, uint64_t* outNumUnique);
 
+/* .tdag16 compact encoding (oracle_tdag16.cpp). */
+int orc_huffman_code_lengths(const uint64_t* freq, uint32_t n, uint32_t maxLen, uint32_t* outLen);
+int orc_huffman_code_words(const uint32_t* len, uint32_t n, int lsbFirst, uint64_t* outCode);
+int64_t orc_tdag16_decode(const uint8_t* bytes, uint64_t len, uint8_t* outBits, uint64_t outCap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -57,6 +57,10 @@ def lib():
         L.orc_postprocess.argtypes = [ctypes.c_uint32, _u64p, ctypes.c_uint64, _u64p, _u64p, _u32p, _u64p]
         if hasattr(L, "orc_svdag_build"):
             _bind_svdag(L)
+        L.orc_huffman_code_lengths.argtypes = [_u64p, ctypes.c_uint32, ctypes.c_uint32, _u32p]
+        L.orc_huffman_code_words.argtypes = [_u32p, ctypes.c_uint32, ctypes.c_int, _u64p]
+        L.orc_tdag16_decode.argtypes = [ctypes.c_char_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64]
+        L.orc_tdag16_decode.restype = ctypes.c_int64
         _lib = L
     return _lib
 
@@ -81,6 +85,36 @@ def _bind_svdag(L):
     L.orc_svdag_static_level.restype = ctypes.c_int64
     L.orc_construct_static_tdag.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrcConfig), ctypes.c_int, ctypes.c_char_p]
     L.orc_construct_static_tdag.restype = ctypes.c_int64
+
+
+def huffman_code_lengths(freq, max_len: int) -> np.ndarray:
+    """Length-limited Huffman code lengths by package-merge (huffman.cpp:113-183)."""
+    f = np.ascontiguousarray(freq, dtype=np.uint64)
+    out = np.zeros(len(f), dtype=np.uint32)
+    if lib().orc_huffman_code_lengths(_p64(f), len(f), max_len, out.ctypes.data_as(_u32p)) != 0:
+        raise ValueError("package-merge failed (too few symbols for the length limit?)")
+    return out
+
+
+def huffman_code_words(lengths, lsb_first: bool = False) -> np.ndarray:
+    """Canonical code words for the given lengths (huffman.cpp:93-111), optionally bit-reversed
+    to LSB-first order (huffman.cpp:192-202)."""
+    ln = np.ascontiguousarray(lengths, dtype=np.uint32)
+    out = np.zeros(len(ln), dtype=np.uint64)
+    lib().orc_huffman_code_words(ln.ctypes.data_as(_u32p), len(ln), int(lsb_first), _p64(out))
+    return out
+
+
+def tdag16_decode(data: bytes) -> np.ndarray:
+    """Decodes a .tdag16 stream voxel by voxel into a dense (R, R, R) bool grid indexed [z, y, x]."""
+    L = lib()
+    R = int(L.orc_tdag16_decode(data, len(data), None, 0))
+    if R < 0:
+        raise ValueError("malformed .tdag16 stream")
+    bits = np.zeros(R * R * R // 8, dtype=np.uint8)
+    if L.orc_tdag16_decode(data, len(data), bits.ctypes.data, bits.nbytes) < 0:
+        raise ValueError("malformed .tdag16 stream")
+    return np.unpackbits(bits, bitorder="little").reshape(R, R, R).astype(bool)
 
 
 def _p64(a):

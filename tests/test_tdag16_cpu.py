@@ -1,0 +1,133 @@
+""".tdag16 compact encoding on the CPU (no GPU).
+
+* Huffman: the library's and the oracle's package-merge code lengths and canonical code words
+  equal the reference's own known-answer vectors (huffman.cpp:246-365); on random histograms the
+  two agree, satisfy Kraft's inequality, respect the length limit, and, where the limit does not
+  bind, cost exactly as much as an unconstrained Huffman code (heapq construction).
+* Encoder: with translation off (no matcher call), every flag combination of the encoder decodes,
+  voxel for voxel in the oracle's restatement of the reference's lookup, to the grid it was built
+  from; a parsed stream re-serialises to the same bytes; a truncated stream is rejected.
+"""
+import heapq
+
+import numpy as np
+import pytest
+
+FLAGS = [(False, False), (True, False), (False, True), (True, True)]
+
+
+@pytest.fixture(scope="module")
+def pl():
+    import tdag_amd
+    from tdag_amd import pipeline
+
+    tdag_amd.lib()
+    return pipeline
+
+
+def _impls(oracle, pl):
+    return [("oracle", oracle.huffman_code_lengths, oracle.huffman_code_words),
+            ("library", pl.huffman_code_lengths, pl.huffman_code_words)]
+
+
+def test_huffman_known_answers(oracle, pl):
+    for name, lengths, words in _impls(oracle, pl):
+        # huffman.cpp:267-283 (Moffat 2019, fig. 6): ties decide that symbol 9 gets 4 bits.
+        assert lengths([20, 17, 6, 3, 2, 2, 2, 1, 1, 1], 5).tolist() == [2, 2, 3, 4, 4, 4, 4, 5, 5, 4], name
+        ln = [1, 2, 4, 5, 5, 5, 5, 5, 6, 6]
+        # huffman.cpp:285-305 (table 1)
+        assert words(ln).tolist() == [0b0, 0b10, 0b1100, 0b11010, 0b11011, 0b11100, 0b11101, 0b11110, 0b111110, 0b111111], name
+        # huffman.cpp:314-332
+        lsb = words(ln, lsb_first=True)
+        assert lsb.tolist() == [0b0, 0b01, 0b0011, 0b01011, 0b11011, 0b00111, 0b10111, 0b01111, 0b011111, 0b111111], name
+        # huffman.cpp:307-312: Kraft sum 1.
+        assert sum(2.0 ** -l for l in ln) == 1.0
+        # huffman.cpp:334-365: a table indexed by the low 6 bits resolves every key to the one
+        # code word that is a suffix of it.
+        def decode(key):
+            hits = [(s, l) for s, (c, l) in enumerate(zip(lsb.tolist(), ln)) if (key & ((1 << l) - 1)) == c]
+            assert len(hits) == 1
+            return hits[0]
+        assert decode(0b0) == (0, 1) and decode(0b1101010) == (0, 1)
+        assert decode(0b01) == (1, 2) and decode(0b0011) == (2, 4) and decode(0b1010011) == (2, 4)
+
+
+def _huffman_cost(freq):
+    h = [int(f) for f in freq]
+    heapq.heapify(h)
+    cost = 0
+    while len(h) > 1:
+        a, b = heapq.heappop(h), heapq.heappop(h)
+        cost += a + b
+        heapq.heappush(h, a + b)
+    return cost
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_huffman_random_histograms(oracle, pl, seed):
+    rng = np.random.default_rng(seed)
+    for n, limit in [(2, 1), (3, 2), (17, 5), (48, 6), (64, 16), (384, 16), (384, 9)]:
+        if (1 << limit) < n:
+            continue
+        freq = rng.integers(0, 1000, n) * (rng.random(n) < 0.7)
+        a = oracle.huffman_code_lengths(freq, limit)
+        b = pl.huffman_code_lengths(freq, limit)
+        assert a.tolist() == b.tolist()
+        assert b.max() <= limit and b.min() >= 1
+        assert sum(2.0 ** -int(l) for l in b) <= 1.0 + 1e-12
+        cost = int(np.dot(b.astype(np.int64), freq.astype(np.int64)))
+        assert cost >= _huffman_cost(freq)
+        if limit == 16 and n <= 64:
+            assert cost == _huffman_cost(freq)
+        wa = oracle.huffman_code_words(a, lsb_first=True)
+        wb = pl.huffman_code_words(b, lsb_first=True)
+        assert wa.tolist() == wb.tolist()
+        # Prefix-free: no LSB-first code word is a suffix-aligned prefix of another.
+        for i in range(n):
+            for j in range(n):
+                if i != j and b[i] <= b[j]:
+                    assert (int(wb[j]) & ((1 << int(b[i])) - 1)) != int(wb[i])
+
+
+def _dense(bits, R):
+    return np.unpackbits(bits, bitorder="little").reshape(R, R, R).astype(bool)
+
+
+@pytest.mark.parametrize("kind,R,seed", [(0, 64, 1), (1, 128, 3), (0, 128, 4)])
+@pytest.mark.parametrize("sym,axis", [(True, True), (True, False), (False, True)])
+def test_encode_decodes_to_the_grid(oracle, pl, kind, R, seed, sym, axis):
+    import tdag_amd
+
+    bits = pl.scene_dense(kind, R, seed)
+    cfg = tdag_amd.TransformDAGConfig(sym, axis, False, 3)
+    t = pl.construct_transform_dag(None, pl.Svdag(bits, R), cfg)
+    expect = _dense(bits, R)
+    for tables, huffman in FLAGS:
+        e = t.encode16(pointer_tables=tables, huffman=huffman)
+        data = e.to_bytes()
+        assert e.resolution == R
+        got = oracle.tdag16_decode(data)
+        assert np.array_equal(got, expect), (tables, huffman)
+        assert pl.TDag16.from_bytes(data).to_bytes() == data
+
+
+def test_no_transform_config(oracle, pl):
+    import tdag_amd
+
+    bits = pl.scene_dense(1, 64, 7)
+    cfg = tdag_amd.TransformDAGConfig(False, False, False, 3)
+    t = pl.construct_transform_dag(None, pl.Svdag(bits, 64), cfg)
+    for tables, huffman in FLAGS:
+        data = t.encode16(pointer_tables=tables, huffman=huffman).to_bytes()
+        assert np.array_equal(oracle.tdag16_decode(data), _dense(bits, 64))
+
+
+def test_truncated_stream_is_rejected(pl):
+    import tdag_amd
+
+    bits = pl.scene_dense(0, 64, 1)
+    t = pl.construct_transform_dag(None, pl.Svdag(bits, 64), tdag_amd.TransformDAGConfig(True, True, False, 3))
+    data = t.encode16().to_bytes()
+    for cut in (0, 9, len(data) // 2, len(data) - 1):
+        with pytest.raises(RuntimeError):
+            pl.TDag16.from_bytes(data[:cut])
