@@ -3,6 +3,7 @@
 //   findDuplicatesAmongFlattenedItems_gpu<Level, Item>   transform_dag.h:310-311
 //   constructTransformDAG(structure, config)              transform_dag.h:161-162
 //   constructStaticTransformDAG(structure, config)        transform_dag.h:163
+//   constructTransformDAG16(structure, config, encoding)  svdag_encoding/src/transform_dag_encoding.cpp:241-478
 //
 // Header-only; link against libtdag.so. Errors are rethrown as std::runtime_error, which is
 // what the reference's assert_always does (error_handling.h:17-45).
@@ -194,6 +195,47 @@ inline StaticStructure<uint64_t> constructStaticTransformDAG(const StaticStructu
         check(tdag_tdag_static_words(t.get(), level, dst.data(), n, &n), "tdag_tdag_static_words");
     }
     return out;
+}
+
+struct Tdag16Deleter {
+    void operator()(tdag_tdag16* h) const { tdag_tdag16_free(h); }
+};
+using TDag16 = std::unique_ptr<tdag_tdag16, Tdag16Deleter>;
+
+// constructTransformDAG16 (transform_dag_encoding.cpp:241-478) of a constructTransformDAG result;
+// TransformEncodingConfig{pointerTables, huffmanCode} (transform_dag_encoding.h:118-120).
+inline TDag16 constructTransformDAG16(const tdag_tdag* t, const tdag_config& config, bool pointerTables, bool huffmanCode)
+{
+    tdag_tdag16* h = nullptr;
+    check(tdag_tdag16_encode(t, &config, (pointerTables ? TDAG16_POINTER_TABLES : 0u) | (huffmanCode ? TDAG16_HUFFMAN : 0u), &h), "tdag_tdag16_encode");
+    return TDag16(h);
+}
+
+// TransformDAG16::writeTo (transform_dag_encoding.cpp:1020-1027).
+inline std::vector<uint8_t> tdag16Bytes(const tdag_tdag16* h)
+{
+    uint64_t n = 0;
+    check(tdag_tdag16_serialize(h, nullptr, 0, &n), "tdag_tdag16_serialize");
+    std::vector<uint8_t> out(n);
+    check(tdag_tdag16_serialize(h, out.data(), n, &n), "tdag_tdag16_serialize");
+    return out;
+}
+
+// TransformDAG16::readFrom (transform_dag_encoding.cpp:1029-1036).
+inline TDag16 tdag16FromBytes(const std::vector<uint8_t>& bytes)
+{
+    tdag_tdag16* h = nullptr;
+    check(tdag_tdag16_from_bytes(bytes.data(), bytes.size(), &h), "tdag_tdag16_from_bytes");
+    return TDag16(h);
+}
+
+// TranslateTransformBaseDAG::get (transform_dag_encoding.cpp:897-942) for every voxel, into a dense
+// grid in device memory (resolution^3 / 32 words, bit i = x + R*y + R*R*z).
+inline float decodeTdag16Device(const tdag_tdag16* h, uint32_t* d_words, uint64_t numWords, Context& ctx = Context::instance())
+{
+    float ms = 0.f;
+    check(tdag_tdag16_decode_device(ctx.get(), h, d_words, numWords, &ms), "tdag_tdag16_decode_device");
+    return ms;
 }
 
 } // namespace tdag_cpp
