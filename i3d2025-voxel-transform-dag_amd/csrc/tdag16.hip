@@ -636,6 +636,213 @@ __global__ __launch_bounds__(256) void kDecodeVolume(Dev16 d, uint32_t* out, uin
     out[w] = word;
 }
 
+// The same lookup for a whole run of 32 voxels along x (R >= 32, so a run shares y and z). Voxel
+// x+1 follows voxel x's walk down to the first level where x, or x minus that level's translation,
+// crosses a cell boundary of the level's children; only the levels below are walked again. The
+// walk state of each level is kept in registers: the loops over levels are unrolled for the
+// root level ROOT, so every state index is a compile-time constant.
+template <int ROOT>
+__global__ __launch_bounds__(256) void kDecodeRuns(Dev16 d, uint32_t* out, uint64_t numWords)
+{
+    __shared__ uint8_t xf[kXfBytes];
+    for (uint32_t i = threadIdx.x; i < kXfBytes; i += blockDim.x)
+        xf[i] = d.xf[i];
+    __syncthreads();
+    const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= numWords)
+        return;
+    constexpr int N = ROOT - 2; // inner levels ROOT..3 at indices 0..N-1; index N is the leaf stage
+    const uint64_t rmask = (1ull << ROOT) - 1ull;
+    const uint64_t i0 = w * 32;
+    const int x0 = (int)(i0 & rmask);
+    // Entry state of the level at index li: x offset (translations subtracted so far), y, z, the
+    // accumulated transform and the node; txA[li] is the x translation (global frame) applied there.
+    int offX[N + 1], vyA[N + 1], vzA[N + 1], txA[N + 1];
+    uint32_t tidA[N + 1];
+    uint64_t nodeA[N + 1];
+    offX[0] = 0;
+    vyA[0] = (int)((i0 >> ROOT) & rmask);
+    vzA[0] = (int)(i0 >> (2 * ROOT));
+    tidA[0] = 0;
+    nodeA[0] = 0;
+#pragma unroll
+    for (int li = 0; li <= N; ++li)
+        txA[li] = 0;
+    int stop = -1; // level index where the last walk ended with an empty result (-1: reached a leaf)
+    uint32_t word = 0;
+    for (int b = 0; b < 32; ++b) {
+        const int x = x0 + b;
+        int from = 0;
+        if (b > 0) {
+            // First level whose decision differs between voxel x-1 and x; levels below the one the
+            // last walk stopped at hold no state for x-1.
+            const int limit = stop >= 0 ? stop : N - 1;
+            from = -1;
+#pragma unroll
+            for (int li = 0; li < N; ++li) {
+                const int cl = ROOT - li - 1;
+                const int e = x - offX[li];
+                const bool moved = (e >> cl) != ((e - 1) >> cl) || (((e - txA[li]) >> cl) != ((e - 1 - txA[li]) >> cl));
+                if (from < 0 && li <= limit && moved)
+                    from = li;
+            }
+            if (from < 0)
+                from = stop >= 0 ? -2 : N; // -2: the same empty result as voxel x-1
+        }
+        uint32_t bit = 0;
+        if (from != -2) {
+            stop = -1;
+#pragma unroll
+            for (int li = 0; li < N; ++li) {
+                if (li < from || stop >= 0)
+                    continue;
+                const int level = ROOT - li;
+                const uint32_t cl = (uint32_t)level - 1u;
+                int v[3] = { x - offX[li], vyA[li], vzA[li] };
+                const uint32_t tid = tidA[li];
+                const uint64_t node = nodeA[li];
+                txA[li] = 0;
+                const uint64_t size = d.levelSize[level];
+                const uint16_t* wds = d.words + d.levelOff[level];
+                if (node >= size) {
+                    flagBad(d);
+                    stop = li;
+                    continue;
+                }
+                const uint32_t c0 = (((uint32_t)v[0] >> cl) & 1u) | ((((uint32_t)v[1] >> cl) & 1u) << 1) | ((((uint32_t)v[2] >> cl) & 1u) << 2);
+                const uint32_t c = xf[kXfChild + tid * 8 + c0];
+                const uint32_t mask = wds[node];
+                const uint32_t sz = (mask >> (2 * c)) & 3u;
+                if (!sz) {
+                    stop = li;
+                    continue;
+                }
+                const uint32_t below = mask & ((1u << (2 * c)) - 1u);
+                const uint64_t at = node + 1 + __popc(below & 0x5555u) + 2 * __popc(below & 0xAAAAu);
+                if (at + sz > size) {
+                    flagBad(d);
+                    stop = li;
+                    continue;
+                }
+                uint64_t e = 0;
+                for (uint32_t i = 0; i < sz; ++i)
+                    e |= (uint64_t)wds[at + i] << (16 * i);
+                bool ok = true;
+                if (d.useTables && (sz == 1 || (e & 1ull))) {
+                    const uint64_t idx = sz == 1 ? e : e >> 1;
+                    if (idx >= d.tableSize[cl])
+                        ok = false;
+                    else
+                        e = d.tables[d.tableOff[cl] + idx];
+                } else if (d.useTables) {
+                    e >>= 1;
+                }
+                const bool hasT = d.trans && cl <= d.maxT;
+                const uint32_t tb = cl + 1;
+                const uint64_t tm = (1ull << tb) - 1ull;
+                const int bias = 1 << cl;
+                int t[3] = { 0, 0, 0 };
+                uint32_t xid = 0;
+                if (ok && d.useHuffman) {
+                    if (!((d.hasLut >> cl) & 1u)) {
+                        ok = false;
+                    } else {
+                        const uint32_t ent = d.lut[d.lutOff[cl] + ((uint32_t)e & d.lutMask[cl])];
+                        uint32_t type = ent & 0xFFFFu;
+                        e >>= ent >> 16;
+                        uint32_t axes = 0;
+                        if (hasT) {
+                            axes = type & 7u;
+                            type >>= 3;
+                        }
+                        if (axes & 4u) {
+                            t[2] = (int)(e & tm) - bias;
+                            e >>= tb;
+                        }
+                        if (axes & 2u) {
+                            t[1] = (int)(e & tm) - bias;
+                            e >>= tb;
+                        }
+                        if (axes & 1u) {
+                            t[0] = (int)(e & tm) - bias;
+                            e >>= tb;
+                        }
+                        if (d.sym) {
+                            xid |= type & 7u;
+                            type >>= 3;
+                        }
+                        if (d.perm)
+                            xid |= (type & 7u) << 3;
+                    }
+                } else if (ok && (d.sym || d.perm || d.trans)) {
+                    const bool tr = e & 1ull;
+                    e >>= 1;
+                    if (tr) {
+                        if (hasT) {
+                            t[0] = (int)(e & tm) - bias;
+                            e >>= tb;
+                            t[1] = (int)(e & tm) - bias;
+                            e >>= tb;
+                            t[2] = (int)(e & tm) - bias;
+                            e >>= tb;
+                        }
+                        if (d.sym) {
+                            xid |= (uint32_t)e & 7u;
+                            e >>= 3;
+                        }
+                        if (d.perm) {
+                            xid |= ((uint32_t)e & 7u) << 3;
+                            e >>= 3;
+                        }
+                    }
+                }
+                if (!ok || xid >= 48) {
+                    flagBad(d);
+                    stop = li;
+                    continue;
+                }
+                // The translation in the global frame: permuted and mirrored by the transform
+                // accumulated so far.
+                int tg[3];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const uint32_t src = xf[kXfPerm + tid * 3 + a];
+                    int ta = src == 0 ? t[0] : (src == 1 ? t[1] : t[2]);
+                    tg[a] = ((tid >> a) & 1u) ? -ta : ta;
+                }
+                txA[li] = tg[0];
+                bool exits = false;
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+                    exits |= (((uint32_t)v[a] >> cl) != ((uint32_t)(v[a] - tg[a]) >> cl));
+                if (exits) {
+                    stop = li;
+                    continue;
+                }
+                offX[li + 1] = offX[li] + tg[0];
+                vyA[li + 1] = v[1] - tg[1];
+                vzA[li + 1] = v[2] - tg[2];
+                tidA[li + 1] = xf[kXfCombine + tid * 48 + xid];
+                nodeA[li + 1] = (uint32_t)e;
+            }
+            if (stop < 0) {
+                const uint64_t node = nodeA[N];
+                if (node >= d.numSubgrids) {
+                    flagBad(d);
+                } else {
+                    const int v0 = x - offX[N], v1 = vyA[N], v2 = vzA[N];
+                    const uint32_t tid = tidA[N];
+                    const uint32_t lo = ((uint32_t)v0 & 1u) | (((uint32_t)v1 & 1u) << 1) | (((uint32_t)v2 & 1u) << 2);
+                    const uint32_t hi = (((uint32_t)v0 >> 1) & 1u) | ((((uint32_t)v1 >> 1) & 1u) << 1) | ((((uint32_t)v2 >> 1) & 1u) << 2);
+                    bit = (uint32_t)(d.subgrids[node] >> (xf[kXfChild + tid * 8 + hi] * 8u + xf[kXfChild + tid * 8 + lo])) & 1u;
+                }
+            }
+        }
+        word |= bit << b;
+    }
+    out[w] = word;
+}
+
 } // namespace
 } // namespace tdag
 
@@ -877,7 +1084,19 @@ tdag_status tdag_tdag16_decode_device(tdag_context* ctx, const tdag_tdag16* hc, 
     hipError_t x = hipMemcpyAsync(h->d.bad, &zero, 4, hipMemcpyHostToDevice, st);
     if (x == hipSuccess) {
         const uint64_t blocks = (need + 255) / 256;
-        hipLaunchKernelGGL(tdag::kDecodeVolume, dim3((uint32_t)blocks), dim3(256), 0, st, h->d, d_out_words, need);
+        const dim3 grid((uint32_t)blocks), block(256);
+        switch (h->e.rootLevel) {
+        case 5: hipLaunchKernelGGL(tdag::kDecodeRuns<5>, grid, block, 0, st, h->d, d_out_words, need); break;
+        case 6: hipLaunchKernelGGL(tdag::kDecodeRuns<6>, grid, block, 0, st, h->d, d_out_words, need); break;
+        case 7: hipLaunchKernelGGL(tdag::kDecodeRuns<7>, grid, block, 0, st, h->d, d_out_words, need); break;
+        case 8: hipLaunchKernelGGL(tdag::kDecodeRuns<8>, grid, block, 0, st, h->d, d_out_words, need); break;
+        case 9: hipLaunchKernelGGL(tdag::kDecodeRuns<9>, grid, block, 0, st, h->d, d_out_words, need); break;
+        case 10: hipLaunchKernelGGL(tdag::kDecodeRuns<10>, grid, block, 0, st, h->d, d_out_words, need); break;
+        case 11: hipLaunchKernelGGL(tdag::kDecodeRuns<11>, grid, block, 0, st, h->d, d_out_words, need); break;
+        case 12: hipLaunchKernelGGL(tdag::kDecodeRuns<12>, grid, block, 0, st, h->d, d_out_words, need); break;
+        default: // resolution below 32: a word spans several rows
+            hipLaunchKernelGGL(tdag::kDecodeVolume, grid, block, 0, st, h->d, d_out_words, need);
+        }
         x = hipGetLastError();
     }
     uint32_t bad = 0;
