@@ -423,7 +423,9 @@ bool parseTdag16(const uint8_t* p, size_t len, Tdag16& d, std::string& err)
             take(&c.code, 8);
             take(&c.bits, 4);
             take(&pad, 4);
-            if (ok && c.bits > 16)
+            // A code word must fit its length: the decode table indexes (suffix << bits) | code
+            // over 2^maxLen entries.
+            if (ok && (c.bits > 16 || (c.code >> c.bits) != 0))
                 ok = false;
         }
     }
@@ -921,8 +923,11 @@ tdag_status upload(tdag_context* ctx, tdag_tdag16* h)
         if (cs.empty())
             continue;
         uint32_t maxLen = 0;
-        for (const auto& c : cs)
+        for (const auto& c : cs) {
+            if (c.bits > 16 || (c.code >> c.bits) != 0)
+                return hipErr(hipErrorInvalidValue, "Huffman code word longer than its length");
             maxLen = std::max(maxLen, c.bits);
+        }
         d.hasLut |= 1u << l;
         d.lutOff[l] = lut.size();
         d.lutMask[l] = (1u << maxLen) - 1u;

@@ -131,3 +131,46 @@ def test_truncated_stream_is_rejected(pl):
     for cut in (0, 9, len(data) // 2, len(data) - 1):
         with pytest.raises(RuntimeError):
             pl.TDag16.from_bytes(data[:cut])
+
+
+def test_code_word_longer_than_its_length_is_rejected(pl):
+    """A Huffman code record whose word does not fit its bit length would index past the decode
+    table built from it; the reader refuses the stream with TDAG_EINVAL."""
+    import struct
+
+    import tdag_amd
+
+    bits = pl.scene_dense(1, 64, 2)
+    t = pl.construct_transform_dag(None, pl.Svdag(bits, 64), tdag_amd.TransformDAGConfig(True, True, False, 3))
+    data = bytearray(t.encode16(pointer_tables=True, huffman=True).to_bytes())
+    # Walk the stream (TransformDAG16::writeTo order) to the first non-empty code record.
+    off = 10  # TransformDAGConfig (8 bytes) + the two encoding flags
+
+    def u64():
+        nonlocal off
+        v = struct.unpack_from("<Q", data, off)[0]
+        off += 8
+        return v
+
+    for _ in range(u64()):
+        words = u64()  # 16-bit node words per level
+        off += 2 * words
+    leaves = u64()
+    off += 8 * leaves + 8  # leaf masks, resolution, root level
+    for _ in range(u64()):
+        entries = u64()  # pointer tables
+        off += 8 * entries
+    rec = None
+    for _ in range(u64()):
+        for _ in range(u64()):
+            code, nbits, pad = struct.unpack_from("<QII", data, off)
+            if rec is None and nbits > 0:
+                rec = off
+            off += 16
+    assert off == len(data) and rec is not None
+    code, nbits, pad = struct.unpack_from("<QII", data, rec)
+    assert 1 <= nbits <= 16 and code >> nbits == 0 and pad == 0
+    pl.TDag16.from_bytes(bytes(data)).close()
+    struct.pack_into("<Q", data, rec, code | (1 << nbits))
+    with pytest.raises(tdag_amd.TdagError, match="EINVAL"):
+        pl.TDag16.from_bytes(bytes(data))
