@@ -138,6 +138,10 @@ tdag_status tdag_context_create(int device, tdag_context** out_ctx)
         e = hipHostMalloc((void**)&ctx->pinned, 64, hipHostMallocDefault);
     for (int i = 0; i < 4 && e == hipSuccess; ++i)
         e = hipEventCreate(&ctx->ev[i]);
+    // Set-up of the allocation-free matcher's scratch on this device (once per device). A
+    // failure here only disables tdag_match_level_device_noalloc, which then reports it.
+    if (e == hipSuccess)
+        (void)tdag::noallocScratchInit(device);
     if (e != hipSuccess) {
         tdag_context_destroy(ctx);
         return failHip(e, "context creation");

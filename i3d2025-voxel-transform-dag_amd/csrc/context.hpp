@@ -31,6 +31,11 @@ struct tdag_context {
 };
 
 namespace tdag {
+
+// Looks up, once per device, the device address of the allocation-free matcher's scratch array
+// (device must be current). Later calls return the first call's result.
+hipError_t noallocScratchInit(int device);
+
 namespace capi {
 
 // Records `msg` as the calling thread's last error and returns `s`.

@@ -652,8 +652,12 @@ hipError_t enqueueSearch(const MatchPlan& p, uint32_t cfgBitsIn, void* d_scratch
     const LevelView v = levelView(p, d_scratch);
     const uint32_t n = v.n;
     const uint64_t cap = 1ull << p.logCap;
+    // The batches of batchEnds(p), walked in place (no host allocation on this path).
     uint32_t r0 = 0;
-    for (const uint32_t r1 : batchEnds(p)) {
+    uint32_t batch = std::max<uint32_t>(1u, std::min<uint32_t>(n, p.firstBatch));
+    while (r0 < n) {
+        const uint32_t r1 = (uint32_t)std::min<uint64_t>(n, (uint64_t)r0 + batch);
+        batch = (uint32_t)std::min<uint64_t>((uint64_t)batch * p.batchGrowth, n);
         switch (p.level) {
         case 2:
             hipLaunchKernelGGL(kSearch2, dim3((unsigned)ceilDiv(r1 - r0, 4)), dim3(256), 0, stream, v.sortedMasks, r1, v.tabKeys, v.tabVals, p.logCap, v.bloom,

@@ -117,6 +117,10 @@ def lib() -> ctypes.CDLL:
         ctypes.c_void_p, ctypes.c_uint32, _u64p, ctypes.c_uint64, ctypes.POINTER(TdagConfig),
         _u64p, _u64p, _u32p, _u64p, ctypes.POINTER(TdagMatchStats),
     ]
+    L.tdag_match_level_device_noalloc.argtypes = [
+        ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(TdagConfig),
+        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+    ]
     L.tdag_last_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(TdagMatchStats)]
     L.tdag_context_set_variant.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int32]
     L.tdag_context_set_counters.argtypes = [ctypes.c_void_p, ctypes.c_int]
@@ -138,7 +142,7 @@ def lib() -> ctypes.CDLL:
                                                    ctypes.POINTER(TdagMatchStats)]
     for name in ("tdag_batch_create", "tdag_batch_run", "tdag_batch_destroy", "tdag_batch_autotune", "tdag_match_levels_batched_device"):
         getattr(L, name).restype = ctypes.c_int
-    for name in ("tdag_context_create", "tdag_context_destroy", "tdag_match_level_device", "tdag_match_level", "tdag_last_stats",
+    for name in ("tdag_context_create", "tdag_context_destroy", "tdag_match_level_device", "tdag_match_level_device_noalloc", "tdag_match_level", "tdag_last_stats",
                  "tdag_context_set_variant", "tdag_context_get_variant", "tdag_autotune_level", "tdag_context_set_counters",
                  "tdag_context_set_batching", "tdag_context_get_batching", "tdag_context_set_filter_bits", "tdag_context_get_filter_bits",
                  "tdag_context_set_child_frames", "tdag_context_get_child_frames"):
@@ -287,6 +291,16 @@ class Context:
         if want_stats:
             return int(nu.value), st
         return None, None
+
+    def match_level_device_noalloc(self, level: int, d_masks, n: int, config: TransformDAGConfig | None = None,
+                                   d_encoded=None, d_mapping=None, d_unique=None, d_counts=None, stream=None):
+        """The allocation-free matcher (tdag_match_level_device_noalloc): same results as
+        match_level_device, scratch in the library's fixed per-GPU array, no synchronisation."""
+        cfg = (config or TransformDAGConfig()).to_c()
+        s = None if stream is None else ctypes.c_void_p(stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream))
+        check(lib().tdag_match_level_device_noalloc(
+            self._ptr, int(level), _dev_ptr(d_masks), int(n), ctypes.byref(cfg), _dev_ptr(d_encoded), _dev_ptr(d_mapping), _dev_ptr(d_unique),
+            _dev_ptr(d_counts), s), "tdag_match_level_device_noalloc")
 
     # -- batched form ------------------------------------------------------------------------
     def batch(self, jobs, config: TransformDAGConfig | None = None, use_graph: bool = True, concurrent: bool = False) -> "Batch":

@@ -102,6 +102,25 @@ tdag_status tdag_match_level(tdag_context* ctx, uint32_t level, const uint64_t* 
     const tdag_config* cfg, uint64_t* out_encoded, uint64_t* out_mapping, uint32_t* out_unique,
     uint64_t* out_num_unique, tdag_match_stats* stats);
 
+/* Device scratch of tdag_match_level_device_noalloc on each GPU: 64 MiB. */
+#define TDAG_NOALLOC_SCRATCH_BYTES (64u << 20)
+
+/*
+ * Allocation-free form of tdag_match_level_device. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory during the call (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc). Same kernels, arguments and results as
+ * tdag_match_level_device with the context's tuning; the call allocates nothing and does not
+ * synchronise. Its scratch is one fixed array of TDAG_NOALLOC_SCRATCH_BYTES per GPU, held from
+ * the first context created on that GPU until the process ends; a level / item count whose
+ * scratch needs more returns TDAG_EINVAL (at default tuning about 100 K items fit at levels 2
+ * and 3, 17 K at level 4 and 4.6 K at level 5). Every call on one GPU shares that array, so calls that overlap on
+ * one GPU (any contexts, any streams) race; consecutive calls are independent.
+ *   d_counts  optional device output, 2 uint64: verified matches, number of unique items
+ *   stream    hipStream_t (NULL = the context's stream)
+ */
+tdag_status tdag_match_level_device_noalloc(tdag_context* ctx, uint32_t level, const uint64_t* d_masks, uint64_t n,
+    const tdag_config* cfg, uint64_t* d_encoded, uint64_t* d_mapping, uint32_t* d_unique, uint64_t* d_counts, void* stream);
+
 /* Statistics of the last matcher call on ctx. */
 tdag_status tdag_last_stats(const tdag_context* ctx, tdag_match_stats* out);
 
