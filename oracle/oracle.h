@@ -24,6 +24,8 @@ uint64_t orc_decode_fixed64(uint64_t key, uint32_t level);
 int orc_transform(uint32_t level, const uint64_t* in, uint32_t permCode, uint32_t symBits, int tx, int ty, int tz, uint64_t* out);
 int orc_corner_hash(uint32_t level, const uint64_t* in, uint32_t dirBits, uint64_t* outHash);
 int orc_match_encoded(uint32_t level, const uint64_t* masks, uint64_t n, const orc_config* cfg, uint64_t* outEncoded, int nthreads, int brute);
+int orc_match_timed(uint32_t level, const uint64_t* masks, uint64_t n, const orc_config* cfg, int nthreads, uint32_t queryStride, double* outSetupS,
+    double* outSearchS, uint64_t* outQueries);
 int orc_postprocess(uint32_t level, const uint64_t* masks, uint64_t n, const uint64_t* encoded, uint64_t* outMapping, uint32_t* outUnique, uint64_t* outNumUnique);
 
 /* .tdag16 compact encoding (oracle_tdag16.cpp). */

@@ -54,6 +54,8 @@ def lib():
         L.orc_transform.argtypes = [ctypes.c_uint32, _u64p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_int, _u64p]
         L.orc_corner_hash.argtypes = [ctypes.c_uint32, _u64p, ctypes.c_uint32, _u64p]
         L.orc_match_encoded.argtypes = [ctypes.c_uint32, _u64p, ctypes.c_uint64, ctypes.POINTER(OrcConfig), _u64p, ctypes.c_int, ctypes.c_int]
+        L.orc_match_timed.argtypes = [ctypes.c_uint32, _u64p, ctypes.c_uint64, ctypes.POINTER(OrcConfig), ctypes.c_int, ctypes.c_uint32,
+                                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), _u64p]
         L.orc_postprocess.argtypes = [ctypes.c_uint32, _u64p, ctypes.c_uint64, _u64p, _u64p, _u32p, _u64p]
         if hasattr(L, "orc_svdag_build"):
             _bind_svdag(L)
@@ -173,6 +175,20 @@ def match_encoded(level: int, masks: np.ndarray, cfg: OrcConfig | None = None, n
     rc = lib().orc_match_encoded(level, _p64(m), n, ctypes.byref(cfg), _p64(out), int(nthreads), int(brute))
     assert rc == 0
     return out[:n]
+
+
+def match_timed(level: int, masks: np.ndarray, cfg: OrcConfig | None = None, nthreads: int | None = None, query_stride: int = 1):
+    """Wall times of the hash search with the full table and every query_stride-th query rank:
+    (table build seconds, search seconds, queries searched)."""
+    cfg = cfg or config()
+    W = words_per_mask(level)
+    m = np.ascontiguousarray(masks, dtype=np.uint64).reshape(-1, W)
+    setup, search = ctypes.c_double(0.0), ctypes.c_double(0.0)
+    q = np.zeros(1, dtype=np.uint64)
+    rc = lib().orc_match_timed(level, _p64(m), m.shape[0], ctypes.byref(cfg), int(nthreads or os.cpu_count() or 1), int(query_stride),
+                               ctypes.byref(setup), ctypes.byref(search), _p64(q))
+    assert rc == 0
+    return setup.value, search.value, int(q[0])
 
 
 def postprocess(level: int, masks: np.ndarray, encoded: np.ndarray):

@@ -16,6 +16,7 @@
 // no-smaller-key invariants of every chosen parent.
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cstdint>
 #include <cstring>
 #include <numeric>
@@ -370,8 +371,15 @@ static std::vector<Grid> loadGrids(uint32_t level, const uint64_t* masks, uint64
 
 // findDuplicatesAmongFlattenedItems_cpp2 search phase (find_translation.cpp:616-921), returning
 // encodedParents indexed by the original item index (key ptr = popcount-sorted rank of the parent).
-static void matchHash(uint32_t level, const std::vector<Grid>& grids, const orc_config& cfg, uint64_t* outEncoded, int nthreads)
+//
+// Timing form (cpu_baseline): with queryStride > 1 only the queries of rank 0, s, 2s, ... are
+// searched (the table still holds every item); setupS / searchS receive the wall time of the
+// table build (sort, corner hashes, ranges) and of the search loop.
+static void matchHash(uint32_t level, const std::vector<Grid>& grids, const orc_config& cfg, uint64_t* outEncoded, int nthreads, uint32_t queryStride = 1,
+    double* setupS = nullptr, double* searchS = nullptr, uint64_t* numQueries = nullptr)
 {
+    const auto tSetup = std::chrono::steady_clock::now();
+    queryStride = std::max<uint32_t>(1u, queryStride);
     const uint32_t R = 1u << level;
     const uint32_t R3 = R * R * R;
     const size_t N = grids.size();
@@ -435,13 +443,19 @@ static void matchHash(uint32_t level, const std::vector<Grid>& grids, const orc_
             lut[rangeHashes[i]] = i;
     }
     hashAndPointers.clear();
+    const auto tSearch = std::chrono::steady_clock::now();
+    uint64_t searched = 0;
 
     // Search in batches of queries in sorted order (:769-918).
     const size_t maxBatch = std::max<size_t>(N / 100, (size_t)100000);
     for (size_t batchStart = 0; batchStart < N; batchStart += maxBatch) {
         const size_t batchEnd = std::min(batchStart + maxBatch, N);
-        parallelFor(batchEnd - batchStart, nthreads, [&](size_t off) {
-            const uint32_t itemIdx = (uint32_t)(batchStart + off);
+        // Sampled ranks of this batch: the multiples of queryStride in [batchStart, batchEnd).
+        const size_t first = (batchStart + queryStride - 1) / queryStride * queryStride;
+        const size_t count = first < batchEnd ? (batchEnd - first + queryStride - 1) / queryStride : 0;
+        searched += count;
+        parallelFor(count, nthreads, [&](size_t off) {
+            const uint32_t itemIdx = (uint32_t)(first + off * queryStride);
             const uint64_t minKey = encodeFixed64(tpCreate(itemIdx, 0, 0, 1 - (int)R, 1 - (int)R, 1 - (int)R), level);
             const Grid& grid = grids[byCount[itemIdx]];
             const HashGrid initial = initialHashGrid(grid);
@@ -506,9 +520,15 @@ static void matchHash(uint32_t level, const std::vector<Grid>& grids, const orc_
             range.second = newEnd;
         }
     }
+    const auto tEnd = std::chrono::steady_clock::now();
     for (size_t i = 0; i < N; ++i)
         outEncoded[i] = encodedParents[i].load();
-    (void)byCount;
+    if (setupS)
+        *setupS = std::chrono::duration<double>(tSearch - tSetup).count();
+    if (searchS)
+        *searchS = std::chrono::duration<double>(tEnd - tSearch).count();
+    if (numQueries)
+        *numQueries = searched;
 }
 
 // Independent brute force over the candidate set: all refs (in sorted rank order), all 48
@@ -674,6 +694,20 @@ int orc_match_encoded(uint32_t level, const uint64_t* masks, uint64_t n, const o
         matchBrute(level, grids, *cfg, outEncoded, nthreads);
     else
         matchHash(level, grids, *cfg, outEncoded, nthreads);
+    return 0;
+}
+
+// The hash search of orc_match_encoded on every item's table but only the queries of rank
+// 0, stride, 2*stride, ...; reports the table-build and search wall times and the number of
+// queries searched (timing only: the keys of a sampled run are not the full run's).
+int orc_match_timed(uint32_t level, const uint64_t* masks, uint64_t n, const orc_config* cfg, int nthreads, uint32_t queryStride, double* outSetupS,
+    double* outSearchS, uint64_t* outQueries)
+{
+    if (level < 2 || level > 5 || cfg == nullptr || queryStride == 0)
+        return 1;
+    const std::vector<Grid> grids = loadGrids(level, masks, n);
+    std::vector<uint64_t> enc(std::max<uint64_t>(n, 1));
+    matchHash(level, grids, *cfg, enc.data(), nthreads, queryStride, outSetupS, outSearchS, outQueries);
     return 0;
 }
 

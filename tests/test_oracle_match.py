@@ -104,3 +104,16 @@ def test_empty_and_single(oracle):
         one = np.zeros((1, W), dtype=np.uint64)
         one[0, 0] = 1
         assert int(oracle.match_encoded(level, one)[0]) == oracle.encode_fixed64(oracle.tp_create(0), level)
+
+
+def test_timed_sampling_counts_queries(oracle):
+    """The cpu_baseline timing entry searches exactly the ranks 0, s, 2s, ... and leaves the
+    keys of a full (stride 1) run unchanged."""
+    rng = np.random.default_rng(5)
+    masks = sets.planted_set(rng, 3, 150, 150, oracle.transform)
+    n = len(masks)
+    for stride in (1, 3, 7, n + 5):
+        setup, search, q = oracle.match_timed(3, masks, oracle.config(), nthreads=2, query_stride=stride)
+        assert q == (n + stride - 1) // stride
+        assert setup >= 0.0 and search >= 0.0
+    assert np.array_equal(oracle.match_encoded(3, masks), oracle.match_encoded(3, masks, brute=True))
