@@ -1,18 +1,22 @@
 #!/usr/bin/env python3
 """Benchmark: verified matches/s of the exhaustive GPU matcher on one MI355X.
 
-Workload (the north-star configuration that fits one GPU, plain exhaustive matcher):
-a 1024^3 grid of 200 sphere shells + 100 box shells + a 6-octave value-noise heightfield shell
-(seed 0), built into an SVDAG and run through constructTransformDAG with
-maxTranslationLevel = 5. The per-level matcher inputs that build produces (32^3, 16^3, 8^3 and
-4^3 subgrids) are uploaded to HBM once; one step is one matcher launch per level over those
-inputs, in the driver's top-down order. Verified matches = items whose minimum-key parent is
-another item, each accepted only after the bitwise check child == T(parent).
+Workload (the largest configuration, the one the metric is quoted on; it fits one GPU): the
+seeded 4096^3 city (box-shell buildings with window voids on 64-voxel lots, half of the
+256-voxel blocks mirrored / axis-swapped copies, noise ground; seed 0), built into an SVDAG and
+run through constructTransformDAG with maxTranslationLevel = 5. The per-level matcher inputs that
+build produces (32^3, 16^3, 8^3 and 4^3 subgrids) are uploaded to HBM once; one step is one
+exhaustive matcher launch per level over those inputs, in the driver's top-down order. Verified
+matches = items whose minimum-key parent is another item, each accepted only after the bitwise
+check child == T(parent). The same line reports the batched and learned forms on the same
+inputs, the host-buffer (PCIe-inclusive) rate, and the 1024^3 sphere+box+heightfield scene.
 
-Prints one JSON line. `roofline` reports the dominant search kernel against HBM bandwidth using
-the reference's own matcher unit (8 directions x 6 permutations x R^3 voxels x 8 bytes per
-query, find_translation.cpp:879-885); `cpu_baseline` times the CPU oracle (a restatement of
-the reference's CPU matcher) on a bounded sample of the same level inputs.
+Prints one JSON line. `roofline` reports the dominant search kernel against VALU issue (its
+binding ceiling) from committed rocprofv3 counters of the same workload and tuning, with the
+counter-measured HBM fraction and the reference's own throughput unit (8 directions x 6
+permutations x R^3 voxels x 8 bytes per query, find_translation.cpp:879-885) labelled
+reference-equivalent; `cpu_baseline` times the CPU oracle (a restatement of the reference's CPU
+matcher) on this host.
 """
 from __future__ import annotations
 
@@ -71,7 +75,9 @@ def load_tuning(ctx, path: str, levels) -> dict:
     return out
 
 
-def build_workload(ctx, resolution: int, seed: int, max_level: int, kind: int = 1):
+def build_workload(ctx, resolution: int, seed: int, max_level: int, kind: int = 1, extras: bool = True):
+    """Scene -> SVDAG -> constructTransformDAG (the matcher inputs of every level are kept). With
+    extras: the device SVDAG build, a second (warm) driver call and the .tdag16 encode/decode."""
     import tdag_amd
     from tdag_amd import pipeline as pl
 
@@ -80,15 +86,10 @@ def build_workload(ctx, resolution: int, seed: int, max_level: int, kind: int = 
     th = time.perf_counter()
     svdag = pl.Svdag(bits, resolution)
     host_build_ms = (time.perf_counter() - th) * 1e3
-    svdag_build = device_svdag_build(ctx, bits, resolution, svdag, host_build_ms)
     cfg = tdag_amd.TransformDAGConfig(True, True, True, max_level)
     tc = time.perf_counter()
     tdag = pl.construct_transform_dag(ctx, svdag, cfg, keep_level_inputs=True)
     construct_ms = (time.perf_counter() - tc) * 1e3
-    tc = time.perf_counter()
-    pl.construct_transform_dag(ctx, svdag, cfg).close()  # again, with the context's buffers in place
-    construct_warm_ms = (time.perf_counter() - tc) * 1e3
-    tdag16 = encode16_and_decode(ctx, tdag, bits, resolution)
     levels = {}
     for level in range(max_level, 1, -1):
         masks = tdag.level_masks(level)
@@ -99,15 +100,19 @@ def build_workload(ctx, resolution: int, seed: int, max_level: int, kind: int = 
         "tdag_bytes": len(tdag.tdag_bytes()),
         "svdag_level_sizes": svdag.level_sizes,
         "tdag_level_sizes": tdag.level_sizes,
-        "setup_s": round(time.time() - t0, 2),
-        "svdag_build": svdag_build,
+    }
+    if extras:
+        info["svdag_build"] = device_svdag_build(ctx, bits, resolution, svdag, host_build_ms)
+        tc = time.perf_counter()
+        pl.construct_transform_dag(ctx, svdag, cfg).close()  # again, with the context's buffers in place
         # Wall time of one whole constructTransformDAG call (hierarchical pass, flattening and one
         # matcher call per level on the device, pointer rewrites on the host): the first call of
         # the process (it also allocates the context's buffers), then a second one.
-        "construct_ms": round(construct_ms, 1),
-        "construct_warm_ms": round(construct_warm_ms, 1),
-        "tdag16": tdag16,
-    }
+        info["construct_ms"] = round(construct_ms, 1)
+        info["construct_warm_ms"] = round((time.perf_counter() - tc) * 1e3, 1)
+        info["tdag16"] = encode16_and_decode(ctx, tdag, bits, resolution)
+    tdag.close()
+    info["setup_s"] = round(time.time() - t0, 2)
     return cfg, levels, info
 
 
@@ -166,43 +171,56 @@ def device_svdag_build(ctx, bits, resolution: int, host_svdag, host_build_ms: fl
             "note": "dense grid resident in HBM; device_ms is the wall time of the call including the copy of the SVDAG to the host"}
 
 
-def cpu_baseline(levels, cfg, budget_s: float, threads: int):
-    """Times the CPU oracle on a prefix sample of every level's input and scales each level's
-    time by items / sample (per-query work of the hash search does not depend on N)."""
+def host_cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(levels, cfg, l5_queries: int, threads: int):
+    """The CPU oracle (a restatement of the reference's findDuplicatesAmongFlattenedItems_cpp2)
+    timed on this host: levels below 5 on their full inputs, level 5 with its full table and a
+    uniform sample of its query ranks (every s-th rank), after one untimed warm-up call. Per
+    level: table-build seconds + search seconds (level 5: search seconds x queries / sampled)."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle_ctypes as oc
 
     ocfg = oc.config(cfg.symmetry, cfg.axis_permutation, cfg.translation, cfg.max_translation_level)
-    per_level_budget = budget_s / max(1, len(levels))
-    est_total_s = 0.0
+    lo = min(levels)
+    warm = levels[lo]["masks"]
+    oc.match_timed(lo, warm, ocfg, nthreads=threads, query_stride=max(1, len(warm) // 64))  # warm-up (threads, pages)
+    per_level = {}
+    total_s = 0.0
     total_matches = 0
-    samples = {}
     for level, lv in sorted(levels.items()):
         masks = lv["masks"]
         n = len(masks)
         total_matches += lv["matched"]
         if n == 0:
             continue
-        # grow the sample until it costs about the per-level budget
-        k = min(n, 64)
-        while True:
-            t0 = time.perf_counter()
-            oc.match_encoded(level, masks[:k], ocfg, nthreads=threads)
-            dt = time.perf_counter() - t0
-            if k == n or dt > per_level_budget / 4:
-                break
-            k = min(n, max(k * 2, int(k * (per_level_budget / 4) / max(dt, 1e-4))))
-        est = dt * n / k
-        est_total_s += est
-        samples[level] = f"{k}/{n} items in {dt:.2f}s"
-    value = total_matches / est_total_s if est_total_s > 0 else None
+        stride = 1 if level < 5 else max(1, n // max(1, l5_queries))
+        setup, search, q = oc.match_timed(level, masks, ocfg, nthreads=threads, query_stride=stride)
+        est = setup + search * n / max(1, q)
+        total_s += est
+        per_level[level] = {"items": n, "queries_searched": q, "table_build_s": round(setup, 3), "search_s": round(search, 3),
+                            "level_s": round(est, 3), "measured_s": round(setup + search, 3)}
+    value = total_matches / total_s if total_s > 0 else None
     return {
         "value": value,
         "unit": "matches/s",
         "cores": threads,
         "kind": "port",
-        "sample": "oracle hash-search matcher on a prefix of each level's input (" + "; ".join(f"L{l}: {s}" for l, s in sorted(samples.items()))
-                  + "), each level's time scaled by items/sample; matches from the full GPU run",
+        "cpu_model": host_cpu_model(),
+        "host_nproc": os.cpu_count(),
+        "levels": per_level,
+        "sample": "oracle hash-search matcher (find_translation.cpp:616-921 restated) with std::thread workers over queries: levels "
+                  + ", ".join(str(l) for l in sorted(per_level) if l < 5) + " on their full inputs; level 5 on its full table with every "
+                  f"s-th query rank searched and its search time scaled by queries/sampled; one untimed warm-up call first; matches from the GPU run",
     }
 
 
@@ -347,18 +365,201 @@ def bench_learned(ctx, cfg, dev, levels, info, args):
     }
 
 
+VALU_PEAK_GINST = 256 * 2.4  # 256 CUs x (4 SIMDs x one wave64 VALU instruction per 4 cycles) x 2.4 GHz, G wave-instructions/s
+
+
+def counters_path(args, level: int) -> str:
+    return os.path.join(REPO, "profiles", f"counters_search_L{level}_{args.workload}_{args.resolution}.json")
+
+
+def load_counters(args, level: int, tuning: dict):
+    """Per-call hardware counters of the level's search (rocprofv3 --pmc on this workload), if they
+    were taken with the tuning in use; else None."""
+    path = counters_path(args, level)
+    if not os.path.exists(path):
+        return None
+    try:
+        prof = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    if prof.get("match") != workload_key(args) or level not in tuning or prof.get("tuning") != tuning_key(tuning[level]):
+        return None
+    return prof
+
+
+def roofline(args, dom: int, dom_ms: float, items: int, tuning: dict, variant: int):
+    """The dominant search kernel against the ceiling that binds it. The level-3..5 search is integer
+    hashing and table probing: its bound is VALU issue (wave64 instructions/s), its HBM traffic is a
+    few percent of peak. achieved = the call's VALU instructions (rocprofv3 SQ_INSTS_VALU, per call,
+    same workload and tuning) / the call's search time from hipEvents in this run."""
+    kernel = ("kSearch2" if dom == 2 else f"kSearch<{dom}> variant {variant}") + f" (level {dom})"
+    ref_equiv = items * bytes_per_query(dom) / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else None
+    out = {"bound": "valu", "kernel": kernel, "achieved": None, "peak": VALU_PEAK_GINST, "unit": "G wave64 VALU instructions/s", "frac": None,
+           "traffic": None}
+    prof = load_counters(args, dom, tuning)
+    if prof and dom_ms > 0:
+        pc = prof["per_call"]
+        valu = pc["SQ_INSTS_VALU"] / (dom_ms * 1e-3) / 1e9
+        traffic = prof["bytes_per_launch"]
+        out.update({
+            "achieved": round(valu, 2),
+            "frac": round(valu / VALU_PEAK_GINST, 4),
+            "traffic": traffic,
+            "hbm": {"achieved": round(traffic / (dom_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(traffic / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+            "wait_any_per_wave_cycle": round(pc["SQ_WAIT_ANY"] / pc["SQ_WAVE_CYCLES"], 4) if pc.get("SQ_WAVE_CYCLES") else None,
+            "counters": os.path.relpath(counters_path(args, dom), REPO),
+            "profiled_search_ms_per_call": prof.get("kernel_ms_per_call"),
+        })
+    out["reference_equivalent"] = {
+        "achieved": round(ref_equiv, 1) if ref_equiv else None, "unit": "GB/s",
+        "note": "items x 384 R^3 B (the reference matcher's own throughput unit, find_translation.cpp:879-885) / search time: bytes of "
+                "nested hash grids the reference writes per query and this kernel never materialises; can exceed HBM peak"}
+    out["note"] = ("peak = 256 CUs x 4 SIMDs x 1/4 wave64 VALU instruction per cycle x 2.4 GHz; traffic = HBM bytes of one search call from "
+                   "rocprofv3 FETCH_SIZE/WRITE_SIZE, scaled by a device copy of known size; achieved/frac are null when the committed "
+                   "counters were not taken on this workload with this tuning")
+    return out
+
+
+def time_plain(ctx, cfg, dev, order, steps: int, warmup: int):
+    """One step = the exhaustive matcher launched once per level (device-resident inputs)."""
+    import torch
+
+    def step(record):
+        total = 0
+        for level in order:
+            d = dev[level]
+            _, st = ctx.match_level_device(level, d["masks"], d["n"], cfg, d_mapping=d["mapping"], d_unique=d["unique"])
+            total += st.num_matched
+            if record is not None:
+                record[level]["search_ms"].append(st.ms_search)
+                record[level]["total_ms"].append(st.ms_total)
+        return total
+
+    for _ in range(warmup):
+        step(None)
+    rec = {level: {"search_ms": [], "total_ms": []} for level in order}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    matches = 0
+    for _ in range(steps):
+        matches += step(rec)
+    torch.cuda.synchronize()
+    return matches, time.perf_counter() - t0, rec
+
+
+def time_host_buffers(ctx, cfg, levels, order, reps: int):
+    """The host-buffer drop-in (tdag_match_level: masks host->device, mapping and unique list
+    device->host, pageable host memory) per level; whole-build wall time."""
+    best = None
+    matches = 0
+    for _ in range(max(1, reps)):
+        t0 = time.perf_counter()
+        m = 0
+        for level in order:
+            _, _, _, st = ctx.match_level(level, levels[level]["masks"], cfg, want_encoded=False)
+            m += st.num_matched
+        dt = time.perf_counter() - t0
+        if best is None or dt < best:
+            best, matches = dt, m
+    return {"ms_per_build": round(best * 1e3, 3), "matches_per_s": matches / best if best else None,
+            "note": "host buffers in and out (tdag_match_level, the findDuplicatesAmongFlattenedItems_gpu drop-in), PCIe included, best of "
+                    f"{max(1, reps)}"}
+
+
+def per_level_summary(levels, order, rec):
+    out = {}
+    for level in order:
+        s = rec[level]["search_ms"]
+        avg = sum(s) / len(s) if s else 0.0
+        out[level] = {
+            "items": levels[level]["items"],
+            "matched": levels[level]["matched"],
+            "search_ms": round(avg, 4),
+            "call_ms": round(sum(rec[level]["total_ms"]) / max(1, len(rec[level]["total_ms"])), 4),
+            "reference_equivalent_GBs": round(levels[level]["items"] * bytes_per_query(level) / (avg * 1e-3) / 1e9, 1) if avg > 0 else None,
+        }
+    return out
+
+
+def upload_levels(levels):
+    import numpy as np
+    import torch
+
+    dev = {}
+    for level, lv in levels.items():
+        m = np.ascontiguousarray(lv["masks"]).view(np.int64)
+        n = lv["items"]
+        dev[level] = {
+            "masks": torch.from_numpy(m).cuda(),
+            "mapping": torch.empty(max(1, n), dtype=torch.int64, device="cuda"),
+            "unique": torch.empty(max(1, n), dtype=torch.int32, device="cuda"),
+            "n": n,
+        }
+    return dev
+
+
+def apply_tuning(ctx, args, order_levels):
+    """Committed tuning file of the workload (the autotuner's output), else autotune later."""
+    tuning_path = args.tuning or default_tuning_path(args)
+    if not args.no_autotune and not args.autotune and os.path.exists(tuning_path):
+        tuning = load_tuning(ctx, tuning_path, order_levels)
+        print(f"[bench] tuning from {tuning_path}: { {l: tuning_key(t) for l, t in tuning.items()} }", file=sys.stderr, flush=True)
+        return tuning, f"autotuner output committed in {os.path.relpath(tuning_path, REPO)}"
+    return {}, "library defaults"
+
+
+def workload_name(args) -> str:
+    kind = f"{args.resolution}^3 sphere+box+heightfield shells" if args.workload == "scene" else f"{args.resolution}^3 city"
+    return (kind + f" seed {args.seed}, constructTransformDAG maxTranslationLevel={args.max_level}, exhaustive GPU matcher one launch per level "
+            "(32^3,16^3,8^3,4^3)")
+
+
+def secondary_config1(args):
+    """The 1024^3 sphere+box+heightfield scene (the plain-matcher configuration with one launch per
+    level), on its own context with its own committed tuning: per-level steps only."""
+    import tdag_amd
+
+    sub = argparse.Namespace(**vars(args))
+    sub.workload, sub.resolution, sub.seed, sub.tuning = "scene", 1024, 0, ""
+    ctx = tdag_amd.Context(0)
+    try:
+        tuning, source = apply_tuning(ctx, sub, range(sub.max_level, 1, -1))
+        cfg, levels, info = build_workload(ctx, sub.resolution, sub.seed, sub.max_level, SCENE_KINDS[sub.workload], extras=False)
+        dev = upload_levels(levels)
+        order = sorted(levels, reverse=True)
+        matches, elapsed, rec = time_plain(ctx, cfg, dev, order, args.steps, args.warmup)
+        per_level = per_level_summary(levels, order, rec)
+        dom = max(order, key=lambda l: per_level[l]["search_ms"])
+        return {
+            "workload": workload_name(sub),
+            "value": matches / elapsed,
+            "unit": "matches/s",
+            "ms_per_step": elapsed * 1e3 / max(1, args.steps),
+            "matches_per_step": matches // max(1, args.steps),
+            "tdag_bytes": info["tdag_bytes"],
+            "levels": per_level,
+            "tuning_source": source,
+            "roofline": roofline(sub, dom, per_level[dom]["search_ms"], levels[dom]["items"], tuning, ctx.get_variant(dom)),
+        }
+    finally:
+        ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--resolution", type=int, default=1024)
-    ap.add_argument("--workload", choices=sorted(SCENE_KINDS), default="scene",
-                    help="scene: spheres + boxes + heightfield (the default workload); city: the seeded city of the largest configuration")
+    ap.add_argument("--resolution", type=int, default=4096)
+    ap.add_argument("--workload", choices=sorted(SCENE_KINDS), default="city",
+                    help="city: the seeded 4096^3 city (the default workload); scene: spheres + boxes + heightfield")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--max-level", type=int, default=5)
-    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU oracle time for cpu_baseline")
+    ap.add_argument("--l5-cpu-queries", type=int, default=256, help="level-5 query ranks searched by the CPU baseline (uniform sample)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true", help="skip the 1024^3 scene block")
+    ap.add_argument("--no-host-buffers", action="store_true", help="skip the PCIe-inclusive host-buffer timing")
     ap.add_argument("--levels", default="", help="comma-separated subset of levels to time (default: all)")
     ap.add_argument("--no-autotune", action="store_true", help="use the library's default search variants instead of tuning them first")
     ap.add_argument("--autotune", action="store_true",
@@ -368,8 +569,9 @@ def main():
                          "default: configs/tuning_<workload>_<resolution>_seed<seed>_maxT<max level>.json when it exists")
     ap.add_argument("--save-tuning", default="", help="write the tuning in use to this file")
     ap.add_argument("--counters", action="store_true", help="also run each level once with per-stage probe counters (diagnostics, untimed)")
-    ap.add_argument("--no-batched", action="store_true", help="skip the batched form (one persistent kernel for all levels, replayed hipGraph)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched forms")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal measurement at level 2")
+    ap.add_argument("--no-extras", action="store_true", help="skip the .tdag16 encode/decode and the whole-driver timings of the workload build")
     ap.add_argument("--learned-steps", type=int, default=1500, help="training steps of the learned proposal")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB three times (reference kernel for HBM byte counters)")
@@ -379,7 +581,6 @@ def main():
         print(f"bench.py runs on exactly one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import torch
 
     import tdag_amd
@@ -397,42 +598,15 @@ def main():
     ctx = tdag_amd.Context(0)
     # A committed tuning file is applied before the workload build, so every matcher call of the
     # run (the build's and the timed ones) uses the same kernels and schedule.
-    tuning = {}
-    tuning_path = args.tuning or default_tuning_path(args)
-    tuning_source = "library defaults"
-    if not args.no_autotune and not args.autotune and os.path.exists(tuning_path):
-        tuning = load_tuning(ctx, tuning_path, range(args.max_level, 1, -1))
-        tuning_source = f"autotuner output committed in {os.path.relpath(tuning_path, REPO)}"
-        print(f"[bench] tuning from {tuning_path}: { {l: tuning_key(t) for l, t in tuning.items()} }", file=sys.stderr, flush=True)
-    cfg, levels, info = build_workload(ctx, args.resolution, args.seed, args.max_level, SCENE_KINDS[args.workload])
+    tuning, tuning_source = apply_tuning(ctx, args, range(args.max_level, 1, -1))
+    cfg, levels, info = build_workload(ctx, args.resolution, args.seed, args.max_level, SCENE_KINDS[args.workload], extras=not args.no_extras)
     print(f"[bench] workload ready: { {k: v for k, v in info.items() if k != 'svdag'} } items={ {l: v['items'] for l, v in levels.items()} }", file=sys.stderr,
           flush=True)
-
-    dev = {}
-    for level, lv in levels.items():
-        m = np.ascontiguousarray(lv["masks"]).view(np.int64)
-        n = lv["items"]
-        dev[level] = {
-            "masks": torch.from_numpy(m).cuda(),
-            "mapping": torch.empty(max(1, n), dtype=torch.int64, device="cuda"),
-            "unique": torch.empty(max(1, n), dtype=torch.int32, device="cuda"),
-            "n": n,
-        }
+    dev = upload_levels(levels)
     order = sorted(levels, reverse=True)
     if args.levels:
         keep = {int(v) for v in args.levels.split(",")}
         order = [l for l in order if l in keep]
-
-    def step(record):
-        total = 0
-        for level in order:
-            d = dev[level]
-            _, st = ctx.match_level_device(level, d["masks"], d["n"], cfg, d_mapping=d["mapping"], d_unique=d["unique"])
-            total += st.num_matched
-            if record is not None:
-                record[level]["search_ms"].append(st.ms_search)
-                record[level]["total_ms"].append(st.ms_total)
-        return total
 
     if not tuning and not args.no_autotune:
         for level in order:
@@ -454,17 +628,11 @@ def main():
             counters[level] = st.counters()
         ctx.set_counters(False)
         print(f"[bench] probe counters: {counters}", file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
-        step(None)
-    rec = {level: {"search_ms": [], "total_ms": []} for level in order}
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    matches = 0
-    for _ in range(args.steps):
-        matches += step(rec)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
 
+    matches, elapsed, rec = time_plain(ctx, cfg, dev, order, args.steps, args.warmup)
+    print(f"[bench] timed {args.steps} steps: {elapsed * 1e3 / max(1, args.steps):.3f} ms/step", file=sys.stderr, flush=True)
+
+    host_buffers = None if args.no_host_buffers else time_host_buffers(ctx, cfg, levels, order, 2)
     batched = None if args.no_batched else bench_batched(ctx, cfg, dev, order, args)
     learned_res = None if (args.no_learned or 2 not in dev) else bench_learned(ctx, cfg, dev, levels, info, args)
 
@@ -472,33 +640,8 @@ def main():
     expected = sum(levels[l]["matched"] for l in order)
     if per_step_matches != expected:
         print(f"[bench] WARNING: matches per step {per_step_matches} != driver's {expected}", file=sys.stderr)
-
-    per_level = {}
-    for level in order:
-        s = rec[level]["search_ms"]
-        avg = sum(s) / len(s) if s else 0.0
-        alg = levels[level]["items"] * bytes_per_query(level)
-        per_level[level] = {
-            "items": levels[level]["items"],
-            "matched": levels[level]["matched"],
-            "search_ms": round(avg, 4),
-            "call_ms": round(sum(rec[level]["total_ms"]) / max(1, len(rec[level]["total_ms"])), 4),
-            "alg_GBs": round(alg / (avg * 1e-3) / 1e9, 1) if avg > 0 else None,
-        }
+    per_level = per_level_summary(levels, order, rec)
     dom = max(order, key=lambda l: per_level[l]["search_ms"])
-    dom_ms = per_level[dom]["search_ms"]
-    achieved = levels[dom]["items"] * bytes_per_query(dom) / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
-    traffic = None
-    tpath = os.path.join(REPO, "profiles", f"traffic_search_L{dom}.json")
-    if os.path.exists(tpath):
-        try:
-            prof = json.load(open(tpath))
-            # The counters count for this run only if they were taken on this exact workload and
-            # with the same search variant and schedule of the dominant level.
-            if prof.get("match") == workload_key(args) and dom in tuning and prof.get("tuning") == tuning_key(tuning[dom]):
-                traffic = prof.get("bytes_per_launch")
-        except (OSError, ValueError):
-            traffic = None
 
     out = {
         "metric": METRIC,
@@ -514,15 +657,14 @@ def main():
         "dtype": "u64",
         "data": "synthetic (seeded scene generator, no external data)",
         "config": {
-            "workload": (f"{args.resolution}^3 sphere+box+heightfield shells" if args.workload == "scene" else f"{args.resolution}^3 city")
-                        + f" seed {args.seed}, constructTransformDAG maxTranslationLevel={args.max_level}, "
-                        "exhaustive GPU matcher one launch per level (32^3,16^3,8^3,4^3)",
+            "workload": workload_name(args),
             "tdag_bytes": info["tdag_bytes"],
             "matches_per_step": per_step_matches,
             **({"svdag_build": info["svdag_build"]} if info.get("svdag_build") else {}),
-            "construct_transform_dag_ms": {"first_call": info["construct_ms"], "second_call": info["construct_warm_ms"]},
-            "tdag16": info["tdag16"],
+            **({"construct_transform_dag_ms": {"first_call": info["construct_ms"], "second_call": info["construct_warm_ms"]}} if "construct_ms" in info else {}),
+            **({"tdag16": info["tdag16"]} if info.get("tdag16") else {}),
             "levels": per_level,
+            **({"host_buffers": host_buffers} if host_buffers else {}),
             "search_variants": {l: ctx.get_variant(l) for l in order if l >= 3},
             "autotune": tuning,
             "tuning_source": tuning_source,
@@ -531,25 +673,13 @@ def main():
             **({"learned": learned_res} if learned_res else {}),
             "parallelism": "single GPU",
         },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": ("kSearch2" if dom == 2 else f"kSearch<{dom}> variant {ctx.get_variant(dom)}") + f" (level {dom})",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
-            **({"traffic_GBs": round(traffic / (dom_ms * 1e-3) / 1e9, 1), "traffic_frac": round(traffic / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-               if traffic and dom_ms > 0 else {}),
-            "note": "achieved = items x 384 R^3 B (the reference matcher's own throughput unit, find_translation.cpp:879-885) / the level's "
-                    "search time from hipEvents on the call stream; it exceeds HBM peak because the nested hash grids that unit counts are "
-                    "never written to memory here. traffic = measured HBM bytes of one search call (rocprofv3 FETCH_SIZE/WRITE_SIZE, "
-                    "copy-calibrated); traffic_frac is the real HBM utilisation: the search is bound by integer VALU issue and table-walk latency",
-        },
+        "roofline": roofline(args, dom, per_level[dom]["search_ms"], levels[dom]["items"], tuning, ctx.get_variant(dom)),
     }
+    if not args.no_secondary and not (args.workload == "scene" and args.resolution == 1024):
+        out["config"]["config1_scene_1024"] = secondary_config1(args)
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(levels, cfg, args.cpu_budget, threads)
+        out["cpu_baseline"] = cpu_baseline({l: levels[l] for l in order}, cfg, args.l5_cpu_queries, threads)
     print(json.dumps(out), flush=True)
 
 

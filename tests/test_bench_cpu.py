@@ -54,6 +54,33 @@ def test_traffic_profiles_match_committed_tuning():
         assert prof["fetch_scale"] > 0 and prof["write_scale"] > 0
 
 
+def test_counter_profiles_match_committed_tuning_and_give_a_bounded_roofline():
+    """Each committed counter profile was taken with the committed tuning of its workload's level,
+    and the roofline the bench derives from it, at the profiled kernel time, stays within its
+    peaks (VALU issue and HBM bandwidth)."""
+    import argparse
+
+    profiles = sorted(glob.glob(os.path.join(REPO, "profiles", "counters_search_L*_*.json")))
+    assert profiles
+    for path in profiles:
+        prof = json.load(open(path))
+        level = int(os.path.basename(path)[len("counters_search_L")])
+        wk = prof["match"]
+        tuning = json.load(open(bench.default_tuning_path(type("A", (), wk))))
+        assert prof["tuning"] == bench.tuning_key(tuning["levels"][str(level)]), path
+        for k in ("SQ_INSTS_VALU", "SQ_WAVE_CYCLES", "SQ_WAIT_ANY"):
+            assert prof["per_call"][k] > 0
+        args = argparse.Namespace(workload=wk["workload"], resolution=wk["resolution"], seed=wk["seed"], max_level=wk["max_level"])
+        assert bench.counters_path(args, level) == path
+        tun = {level: tuning["levels"][str(level)]}
+        r = bench.roofline(args, level, prof["kernel_ms_per_call"], 1000, tun, tun[level]["variant"])
+        assert r["bound"] == "valu" and r["traffic"] == prof["bytes_per_launch"]
+        assert 0 < r["frac"] <= 1 and 0 < r["hbm"]["frac"] <= 1
+        # a different tuning disables the figures instead of reporting stale ones
+        other = {level: dict(tun[level], first_batch=tun[level]["first_batch"] + 1)}
+        assert bench.roofline(args, level, prof["kernel_ms_per_call"], 1000, other, 0)["frac"] is None
+
+
 class _FakeCtx:
     def __init__(self):
         self.calls = []
