@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <thread>
 
@@ -134,6 +135,52 @@ tdag_status tdag_svdag_from_dense_device(tdag_context* ctx, const uint8_t* d_bit
     if (out_ms)
         *out_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
     *out = h;
+    return TDAG_OK;
+}
+
+tdag_status tdag_svdag_from_view(const tdag_svdag_view* v, tdag_svdag** out)
+{
+    if (!v || !out)
+        return setErr(TDAG_EINVAL, "null argument");
+    *out = nullptr;
+    if (v->root_level < 3 || v->root_level > 12 || v->resolution != (1u << v->root_level))
+        return setErr(TDAG_EINVAL, "resolution must be 2^root_level with root_level in [3, 12]");
+    if (v->structure_type != 0 && v->structure_type != 1)
+        return setErr(TDAG_EINVAL, "structure_type " + std::to_string(v->structure_type) + " is neither Tree (0) nor DAG (1)");
+    if (!v->nodes_per_level || !v->node_counts)
+        return setErr(TDAG_EINVAL, "nodes_per_level / node_counts are null");
+    if (v->num_subgrids && !v->subgrids)
+        return setErr(TDAG_EINVAL, "subgrids is null");
+    if (v->num_subgrids >= tdag::kEmptyChild)
+        return setErr(TDAG_EINVAL, "too many subgrids for 32-bit child indices");
+    std::unique_ptr<tdag_svdag> h(new tdag_svdag());
+    tdag::Svdag& s = h->s;
+    s.resolution = v->resolution;
+    s.rootLevel = v->root_level;
+    s.structureType = v->structure_type;
+    s.nodes.resize(v->root_level + 1);
+    s.leaves.assign(v->subgrids, v->subgrids + v->num_subgrids);
+    uint64_t below = v->num_subgrids;
+    for (uint32_t level = 3; level <= v->root_level; ++level) {
+        const uint64_t count = v->node_counts[level];
+        const uint32_t* nodes = v->nodes_per_level[level];
+        if (count && !nodes)
+            return setErr(TDAG_EINVAL, "level " + std::to_string(level) + " has " + std::to_string(count) + " nodes but a null array");
+        if (count >= tdag::kEmptyChild)
+            return setErr(TDAG_EINVAL, "level " + std::to_string(level) + " has too many nodes for 32-bit child indices");
+        auto& dst = s.nodes[level];
+        dst.resize(count);
+        for (uint64_t i = 0; i < count; ++i)
+            for (int c = 0; c < 8; ++c) {
+                const uint32_t child = nodes[i * 8 + c];
+                if (child != tdag::kEmptyChild && child >= below)
+                    return setErr(TDAG_EINVAL, "level " + std::to_string(level) + " node " + std::to_string(i) + " child " + std::to_string(c) + ": index " +
+                            std::to_string(child) + " outside level " + std::to_string(level - 1) + " (" + std::to_string(below) + " entries)");
+                dst[i][c] = child;
+            }
+        below = count;
+    }
+    *out = h.release();
     return TDAG_OK;
 }
 
@@ -513,6 +560,27 @@ tdag_status tdag_tdag_level_record(const tdag_tdag* t, uint32_t level, tdag_leve
             return TDAG_OK;
         }
     return setErr(TDAG_EINVAL, "no matcher record for level " + std::to_string(level));
+}
+
+tdag_status tdag_tdag_copy_level(const tdag_tdag* t, uint32_t level, uint64_t* out, uint64_t cap, uint64_t* out_count)
+{
+    if (!t || !out_count)
+        return setErr(TDAG_EINVAL, "null argument");
+    if (t->isStatic)
+        return setErr(TDAG_EINVAL, "a static-structure result is read with tdag_tdag_static_words");
+    if (level < 2 || level > t->t.rootLevel)
+        return setErr(TDAG_EINVAL, "level " + std::to_string(level) + " outside [2, root level]");
+    const uint64_t words = level == 2 ? t->t.leaves.size() : t->t.nodes[level].size() * 8;
+    *out_count = words;
+    if (!out)
+        return TDAG_OK;
+    if (cap < words)
+        return setErr(TDAG_EINVAL, "buffer too small");
+    if (level == 2)
+        std::memcpy(out, t->t.leaves.data(), words * 8);
+    else
+        std::memcpy(out, t->t.nodes[level].data(), words * 8);
+    return TDAG_OK;
 }
 
 tdag_status tdag_tdag_level_masks(const tdag_tdag* t, uint32_t level, uint64_t* out, uint64_t cap_words)

@@ -52,6 +52,21 @@ class LevelRecord(ctypes.Structure):
     ]
 
 
+class SvdagView(ctypes.Structure):
+    """tdag_svdag_view: EditStructure<void, uint32_t> as borrowed arrays (include/tdag.h)."""
+
+    _fields_ = [
+        ("resolution", ctypes.c_uint32),
+        ("root_level", ctypes.c_uint32),
+        ("nodes_per_level", ctypes.POINTER(ctypes.POINTER(ctypes.c_uint32))),
+        ("node_counts", ctypes.POINTER(ctypes.c_uint64)),
+        ("subgrids", ctypes.POINTER(ctypes.c_uint64)),
+        ("num_subgrids", ctypes.c_uint64),
+        ("structure_type", ctypes.c_int32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
 def bind(L):
     vp = ctypes.c_void_p
     L.tdag_scene_dense.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, _u8p, ctypes.c_uint64]
@@ -82,6 +97,10 @@ def bind(L):
     L.tdag_tdag16_info.argtypes = [vp, _u32p, _u32p, _u64p, _u64p, ctypes.c_uint32]
     L.tdag_tdag16_decode_device.argtypes = [vp, vp, vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_float)]
     L.tdag_tdag16_free.argtypes = [vp]
+    L.tdag_svdag_from_view.argtypes = [ctypes.POINTER(SvdagView), ctypes.POINTER(vp)]
+    L.tdag_svdag_from_view.restype = ctypes.c_int
+    L.tdag_tdag_copy_level.argtypes = [vp, ctypes.c_uint32, _u64p, ctypes.c_uint64, _u64p]
+    L.tdag_tdag_copy_level.restype = ctypes.c_int
     L.tdag_huffman_code_lengths.argtypes = [_u64p, ctypes.c_uint32, ctypes.c_uint32, _u32p]
     L.tdag_huffman_code_words.argtypes = [_u32p, ctypes.c_uint32, ctypes.c_int, _u64p]
     for name in ("tdag_scene_dense", "tdag_svdag_from_dense", "tdag_svdag_info", "tdag_svdag_copy_level", "tdag_svdag_flatten",
@@ -123,6 +142,26 @@ class Svdag:
         check(lib().tdag_svdag_from_dense_device(ctx._ptr, ctypes.c_void_p(d_bits.data_ptr()), resolution, ctypes.byref(self._h), ctypes.byref(ms)),
               "tdag_svdag_from_dense_device")
         self.build_ms = float(ms.value)
+        self._load_info()
+        return self
+
+    @classmethod
+    def from_view(cls, resolution: int, root_level: int, nodes: dict, subgrids, structure_type: int = 1):
+        """tdag_svdag_from_view of an EditStructure<void, uint32_t> given as arrays: nodes[level]
+        (level 3..root) of shape (count, 8) uint32 child indices, subgrids the uint64 leaf masks."""
+        keep = {}
+        ptrs = (ctypes.POINTER(ctypes.c_uint32) * (root_level + 1))()
+        counts = (ctypes.c_uint64 * (root_level + 1))()
+        for level in range(3, root_level + 1):
+            a = np.ascontiguousarray(nodes.get(level, np.zeros((0, 8), dtype=np.uint32)), dtype=np.uint32).reshape(-1, 8)
+            keep[level] = a
+            ptrs[level] = a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+            counts[level] = a.shape[0]
+        sg = np.ascontiguousarray(subgrids, dtype=np.uint64).reshape(-1)
+        view = SvdagView(resolution, root_level, ptrs, counts, sg.ctypes.data_as(_u64p), sg.size, structure_type, 0)
+        self = cls.__new__(cls)
+        self._h = ctypes.c_void_p()
+        check(lib().tdag_svdag_from_view(ctypes.byref(view), ctypes.byref(self._h)), "tdag_svdag_from_view")
         self._load_info()
         return self
 
@@ -231,6 +270,16 @@ class TDag:
         if lib().tdag_tdag_level_record(self._h, level, ctypes.byref(rec)) != 0:
             return None
         return rec
+
+    def nodes(self, level: int) -> np.ndarray:
+        """Level `level` of the EditStructure<void, TransformPointer>: (count, 8) uint64 TransformPointer
+        bit patterns (level >= 3) or the uint64 leaf masks (level 2)."""
+        n = ctypes.c_uint64()
+        check(lib().tdag_tdag_copy_level(self._h, level, None, 0, ctypes.byref(n)), "tdag_tdag_copy_level")
+        out = np.zeros(max(1, n.value), dtype=np.uint64)
+        check(lib().tdag_tdag_copy_level(self._h, level, out.ctypes.data_as(_u64p), out.size, ctypes.byref(n)), "tdag_tdag_copy_level")
+        out = out[: n.value]
+        return out if level == 2 else out.reshape(-1, 8)
 
     def level_masks(self, level: int) -> np.ndarray:
         rec = self.record(level)

@@ -341,6 +341,30 @@ tdag_status tdag_svdag_flatten(const tdag_svdag* svdag, uint32_t level, const ui
 tdag_status tdag_svdag_flatten_device(tdag_context* ctx, const tdag_svdag* svdag, uint32_t level, const uint32_t* d_nodes, uint64_t n, uint64_t* d_out_masks);
 tdag_status tdag_svdag_free(tdag_svdag* svdag);
 
+/* voxcom::EditStructure<void, uint32_t> / EditStructureOOC<void, uint32_t> (structure.h:133-171), the
+ * input type of constructTransformDAG (transform_dag.h:161-162), as borrowed arrays. For level l in
+ * [3, root_level], nodes_per_level[l] holds node_counts[l] EditNode<uint32_t> (8 uint32 child
+ * indices into level l - 1, 0xFFFFFFFF = EmptyChild; leaf indices at level 3); entries 0..2 are
+ * not read. subgrids: num_subgrids EditSubGrid<void> bitmasks (4x4x4, Morton bit order,
+ * structure.h:116-125). structure_type: voxcom::StructureType (0 Tree, 1 DAG). */
+typedef struct tdag_svdag_view {
+    uint32_t resolution;
+    uint32_t root_level;
+    const uint32_t* const* nodes_per_level;
+    const uint64_t* node_counts;
+    const uint64_t* subgrids;
+    uint64_t num_subgrids;
+    int32_t structure_type;
+    uint32_t reserved;
+} tdag_svdag_view;
+
+/* Copies an EditStructure view into a tdag_svdag (the handle tdag_construct_transform_dag takes).
+ * Validated: resolution == 2^root_level with root_level in [3, 12], structure_type 0 or 1, every
+ * child index EmptyChild or inside the level below, no null array with a non-zero count;
+ * TDAG_EINVAL otherwise. The driver then requires structure_type == 1 (DAG), as the reference
+ * asserts (transform_dag.cpp:141). */
+tdag_status tdag_svdag_from_view(const tdag_svdag_view* view, tdag_svdag** out);
+
 /* constructTransformDAG (transform_dag.h:161-162, transform_dag.cpp:138-240): hierarchical
  * sym+perm dedup above max_translation_level, then one GPU matcher call per level from
  * max_translation_level down to 2. max_translation_level must lie in [2, 5]. ctx may be NULL
@@ -357,6 +381,11 @@ tdag_status tdag_construct_transform_dag_learned(tdag_context* ctx, const tdag_s
 tdag_status tdag_tdag_serialize(const tdag_tdag* tdag, const tdag_config* cfg, uint8_t* buf, uint64_t cap, uint64_t* out_size);
 tdag_status tdag_tdag_info(const tdag_tdag* tdag, uint32_t* root_level, uint64_t* level_sizes, uint32_t cap);
 tdag_status tdag_tdag_level_record(const tdag_tdag* tdag, uint32_t level, tdag_level_record* out);
+/* The nodes of `level` of a constructTransformDAG result, the EditStructure<void, TransformPointer>
+ * the reference returns: for level >= 3, 8 TransformPointer bit patterns per node (sentinel
+ * 0x82082400FFFFFFFF = empty child); for level 2, the uint64 leaf masks. cap in uint64 words;
+ * *out_count receives the word count (out may be NULL to query it). */
+tdag_status tdag_tdag_copy_level(const tdag_tdag* tdag, uint32_t level, uint64_t* out, uint64_t cap, uint64_t* out_count);
 /* Copies the kept matcher input masks of `level` (items * W words). */
 tdag_status tdag_tdag_level_masks(const tdag_tdag* tdag, uint32_t level, uint64_t* out, uint64_t cap_words);
 tdag_status tdag_tdag_free(tdag_tdag* tdag);

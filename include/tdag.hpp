@@ -15,6 +15,7 @@
 #include <span>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "tdag.h"
@@ -59,28 +60,55 @@ inline tdag_config toC(bool symmetry, bool axisPermutation, bool translation, ui
     return c;
 }
 
-// findDuplicatesAmongFlattenedItems_gpu<Level, Item> (transform_dag.h:310-311).
-//   items    one payload per grid (node index, leaf, ...)
-//   grids    n * W uint64 words in the LargeSubGrid<2^Level> layout (large_sub_grid.h:80-97);
-//            a std::span<const LargeSubGrid<R>> can be passed as
-//            {reinterpret_cast<const uint64_t*>(grids.data()), grids.size() * W}
-//   outPrevLevelMapping  resized to n; TransformPointer bit patterns (transform_dag.h:24-33)
+namespace detail {
+
+// TransformDAGConfig (transform_dag.h:157-160) or tdag_config: 3 bools, padding, uint32.
+template <typename Config>
+tdag_config toConfig(const Config& config)
+{
+    static_assert(sizeof(Config) == sizeof(tdag_config) && std::is_trivially_copyable_v<Config>,
+        "config must have the byte layout of voxcom::TransformDAGConfig");
+    tdag_config c;
+    std::memcpy(&c, &config, sizeof(c));
+    return c;
+}
+
+} // namespace detail
+
+// findDuplicatesAmongFlattenedItems_gpu<Level, Item> (transform_dag.h:310-311), typed like the
+// reference's declaration and called the same way (transform_dag.cpp:201):
+//   findDuplicatesAmongFlattenedItems_gpu<Level, uint32_t>(indices, flattenedGrids, nodeMapping, config)
+//   items                one payload per grid (node index, leaf, ...)
+//   grids                a contiguous range (std::vector, std::span) of voxcom::LargeSubGrid<2^Level>
+//                        (large_sub_grid.h:80-97: W uint64 words each), or of n * W uint64 words
+//   outPrevLevelMapping  std::vector of voxcom::TransformPointer (or uint64_t): resized to n, filled
+//                        with the TransformPointer bit patterns (transform_dag.h:24-33)
+//   config               voxcom::TransformDAGConfig (or tdag_config)
 // Returns the unique items in original order, as the reference does.
-template <uint32_t Level, typename Item>
-std::vector<Item> findDuplicatesAmongFlattenedItems_gpu(std::span<const Item> items, std::span<const uint64_t> grids,
-    std::vector<uint64_t>& outPrevLevelMapping, const tdag_config& config, Context& ctx = Context::instance())
+template <size_t Level, typename Item, typename Grids, typename Pointer, typename Config>
+std::vector<Item> findDuplicatesAmongFlattenedItems_gpu(std::span<const Item> items, const Grids& grids, std::vector<Pointer>& outPrevLevelMapping,
+    const Config& config, Context& ctx = Context::instance())
 {
     static_assert(Level >= 2 && Level <= 5, "levels 2..5 are supported");
+    using Grid = std::remove_cv_t<std::remove_reference_t<decltype(*std::data(grids))>>;
+    static_assert(std::is_trivially_copyable_v<Grid> && sizeof(Grid) % 8 == 0, "grids must be LargeSubGrid<R> or uint64 words");
+    static_assert(sizeof(Pointer) == 8 && std::is_trivially_copyable_v<Pointer>, "the mapping holds 64-bit TransformPointers");
     constexpr size_t R = size_t(1) << Level;
     constexpr size_t W = R * R * R / 64 > 0 ? R * R * R / 64 : 1;
-    if (grids.size() != items.size() * W)
-        throw std::runtime_error("findDuplicatesAmongFlattenedItems_gpu: grids.size() != items.size() * W");
+    static_assert(sizeof(Grid) == 8 || sizeof(Grid) == W * 8, "a grid must be W = R^3/64 uint64 words (LargeSubGrid<R>)");
+    const uint64_t* words = reinterpret_cast<const uint64_t*>(std::data(grids));
+    const size_t numWords = std::size(grids) * (sizeof(Grid) / 8);
+    if (numWords != items.size() * W)
+        throw std::runtime_error("findDuplicatesAmongFlattenedItems_gpu: the grids do not hold W words per item");
     const uint64_t n = items.size();
-    outPrevLevelMapping.resize(n);
+    const tdag_config cfg = detail::toConfig(config);
+    std::vector<uint64_t> mapping(n);
     std::vector<uint32_t> unique(n);
     uint64_t numUnique = 0;
-    check(tdag_match_level(ctx.get(), Level, grids.data(), n, &config, nullptr, outPrevLevelMapping.data(), unique.data(), &numUnique, nullptr),
-        "tdag_match_level");
+    check(tdag_match_level(ctx.get(), (uint32_t)Level, words, n, &cfg, nullptr, mapping.data(), unique.data(), &numUnique, nullptr), "tdag_match_level");
+    outPrevLevelMapping.resize(n);
+    if (n)
+        std::memcpy(static_cast<void*>(outPrevLevelMapping.data()), mapping.data(), n * 8);
     std::vector<Item> out;
     out.reserve(numUnique);
     for (uint64_t k = 0; k < numUnique; ++k)
@@ -127,6 +155,74 @@ inline TDag constructTransformDAG(const tdag_svdag* inDag, const tdag_config& co
     tdag_tdag* t = nullptr;
     check(tdag_construct_transform_dag(ctx.get(), inDag, &config, 0, &t), "tdag_construct_transform_dag");
     return TDag(t);
+}
+
+// An EditStructure<void, uint32_t> / EditStructureOOC<void, uint32_t> (structure.h:133-171) as
+// the C-ABI view: any type with the reference's members nodesPerLevel (per level a contiguous
+// range of EditNode<uint32_t>: 8 uint32 children), subGrids (EditSubGrid<void>: one uint64),
+// resolution, rootLevel and structureType. Validated and copied by the library.
+template <typename InStructure>
+Svdag svdagFromStructure(const InStructure& inDag)
+{
+    const uint32_t root = (uint32_t)inDag.rootLevel;
+    std::vector<const uint32_t*> nodes(root + 1, nullptr);
+    std::vector<uint64_t> counts(root + 1, 0);
+    for (uint32_t level = 3; level <= root && level < std::size(inDag.nodesPerLevel); ++level) {
+        const auto& lv = inDag.nodesPerLevel[level];
+        using Node = std::remove_cv_t<std::remove_reference_t<decltype(*std::data(lv))>>;
+        static_assert(sizeof(Node) == 32 && std::is_trivially_copyable_v<Node>, "EditNode<uint32_t>: 8 uint32 children");
+        nodes[level] = reinterpret_cast<const uint32_t*>(std::data(lv));
+        counts[level] = std::size(lv);
+    }
+    using Sub = std::remove_cv_t<std::remove_reference_t<decltype(*std::data(inDag.subGrids))>>;
+    static_assert(sizeof(Sub) == 8 && std::is_trivially_copyable_v<Sub>, "EditSubGrid<void>: one uint64 bitmask");
+    const tdag_svdag_view view { (uint32_t)inDag.resolution, root, nodes.data(), counts.data(), reinterpret_cast<const uint64_t*>(std::data(inDag.subGrids)),
+        (uint64_t)std::size(inDag.subGrids), (int32_t)inDag.structureType, 0u };
+    tdag_svdag* s = nullptr;
+    check(tdag_svdag_from_view(&view, &s), "tdag_svdag_from_view");
+    return Svdag(s);
+}
+
+// constructTransformDAG (transform_dag.h:161-162) typed like the reference:
+//   EditStructure<void, TransformPointer> constructTransformDAG(const Structure<void, uint32_t>&, const TransformDAGConfig&)
+// OutStructure is the reference's EditStructure<void, TransformPointer> (or any type with its
+// members: nodesPerLevel as std::vector<std::vector<EditNode<TransformPointer>>>, subGrids as a
+// std::vector of EditSubGrid<void>, resolution, rootLevel, structureType); it comes back with the
+// reference's output: every level filled, structureType Tree (transform_dag_hierarchical.cpp:307).
+// A voxcom build binds its own template to this one:
+//   template <template <typename, typename> typename Structure>
+//   EditStructure<void, TransformPointer> constructTransformDAG(const Structure<void, uint32_t>& in, const TransformDAGConfig& c)
+//   { return tdag_cpp::constructTransformDAG<EditStructure<void, TransformPointer>>(in, c); }
+template <typename OutStructure, typename InStructure, typename Config>
+OutStructure constructTransformDAG(const InStructure& inDag, const Config& config, Context& ctx = Context::instance())
+{
+    const Svdag in = svdagFromStructure(inDag);
+    const tdag_config cfg = detail::toConfig(config);
+    tdag_tdag* raw = nullptr;
+    check(tdag_construct_transform_dag(ctx.get(), in.get(), &cfg, 0, &raw), "tdag_construct_transform_dag");
+    const TDag t(raw);
+    OutStructure out {};
+    uint32_t root = 0;
+    check(tdag_tdag_info(t.get(), &root, nullptr, 0), "tdag_tdag_info");
+    out.resolution = inDag.resolution;
+    out.rootLevel = root;
+    out.structureType = static_cast<decltype(out.structureType)>(0);
+    out.nodesPerLevel.resize(root + 1);
+    for (uint32_t level = 2; level <= root; ++level) {
+        uint64_t words = 0;
+        check(tdag_tdag_copy_level(t.get(), level, nullptr, 0, &words), "tdag_tdag_copy_level");
+        if (level == 2) {
+            static_assert(sizeof(*std::data(out.subGrids)) == 8, "EditSubGrid<void>: one uint64 bitmask");
+            out.subGrids.resize(words);
+            check(tdag_tdag_copy_level(t.get(), level, reinterpret_cast<uint64_t*>(out.subGrids.data()), words, &words), "tdag_tdag_copy_level");
+        } else {
+            auto& lv = out.nodesPerLevel[level];
+            static_assert(sizeof(*std::data(lv)) == 64, "EditNode<TransformPointer>: 8 64-bit TransformPointers");
+            lv.resize(words / 8);
+            check(tdag_tdag_copy_level(t.get(), level, reinterpret_cast<uint64_t*>(lv.data()), words, &words), "tdag_tdag_copy_level");
+        }
+    }
+    return out;
 }
 
 // The .tdag byte stream the voxelizer writes (voxelizer/src/main.cpp:144-151).

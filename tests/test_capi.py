@@ -54,18 +54,10 @@ def test_tp_helpers_closed_forms():
     assert u == {"ptr": 9, "sym": 5, "perm": (2, 0, 1), "t": (-3, 0, 31)}
 
 
-def test_cpp_dropin_header_compiles(tmp_path):
-    """include/tdag.hpp (the C++ drop-ins over the C-ABI) compiles as C++20."""
-    import subprocess
+def test_cpp_dropins_compile_and_link(tmp_path):
+    """include/tdag.hpp's typed drop-ins, called as the reference calls its own entry points on
+    stand-in types with the reference's layouts (tests/cpp/dropin_check.cpp), compile as C++20
+    and link against libtdag.so. The program is run by tests/test_gpu_dropin.py."""
+    import dropin_build
 
-    src = tmp_path / "use.cpp"
-    src.write_text(
-        '#include "tdag.hpp"\n'
-        "int main() {\n"
-        "  tdag_config cfg = tdag_cpp::toC(true, true, true, 4);\n"
-        "  std::vector<uint32_t> items(2); std::vector<uint64_t> grids(2), mapping;\n"
-        "  auto u = tdag_cpp::findDuplicatesAmongFlattenedItems_gpu<2, uint32_t>(items, grids, mapping, cfg);\n"
-        "  return (int)u.size();\n"
-        "}\n")
-    proc = subprocess.run(["g++", "-std=c++20", "-fsyntax-only", "-I", os.path.join(REPO, "include"), str(src)], capture_output=True, text=True)
-    assert proc.returncode == 0, proc.stderr
+    assert os.path.exists(dropin_build.build(str(tmp_path)))

@@ -141,3 +141,19 @@ def test_device_hierarchical_equals_host(gpu_ctx, kind, R, seed, sym, axis):
     assert dev == host
     st = pl.StaticSvdag.from_svdag(svdag)
     assert pl.construct_static_transform_dag(gpu_ctx, st, cfg).static_bytes() == pl.construct_static_transform_dag(None, st, cfg).static_bytes()
+
+
+@pytest.mark.parametrize("kind,R,seed,maxT", [(0, 64, 2, 5), (2, 256, 5, 4)])
+def test_edit_structure_view_bytes_match_oracle(oracle, gpu_ctx, kind, R, seed, maxT):
+    """constructTransformDAG on the oracle's SVDAG handed over as an EditStructure view (the
+    reference's input type), not built from a dense grid: the same .tdag bytes as the oracle."""
+    import svdag_arrays
+    import tdag_amd
+    from tdag_amd import pipeline as pl
+
+    print(f"view scene kind={kind} R={R} seed={seed} maxT={maxT}", flush=True)
+    bits = pl.scene_dense(kind, R, seed)
+    root, nodes, leaves = svdag_arrays.oracle_svdag_arrays(oracle, bits, R)
+    s = pl.Svdag.from_view(R, root, nodes, leaves)
+    mine = pl.construct_transform_dag(gpu_ctx, s, tdag_amd.TransformDAGConfig(True, True, True, maxT)).tdag_bytes()
+    assert mine == _oracle_bytes(oracle, bits, R, oracle.config(True, True, True, maxT))

@@ -153,3 +153,51 @@ def test_city_scene_deterministic_with_transformed_copies(oracle, pl):
     perm = (enc >> np.uint64(3 * b + 3)) & np.uint64(7)
     assert matched.sum() > 0
     assert np.sum(matched & ((sym != 0) | (perm != 0))) > 0
+
+
+@pytest.mark.parametrize("kind,R,seed", [(0, 64, 1), (1, 128, 3), (2, 256, 4)])
+def test_edit_structure_view_matches_oracle(oracle, pl, kind, R, seed):
+    """constructTransformDAG on an EditStructure<void, uint32_t> handed over as arrays (the
+    reference's input type, transform_dag.h:161-162), built by the oracle rather than from a
+    dense grid: with translation off the .tdag bytes equal the oracle's driver, and the result's
+    levels read back through tdag_tdag_copy_level are the nodes those bytes hold."""
+    import svdag_arrays
+    import tdag_amd
+
+    bits = pl.scene_dense(kind, R, seed)
+    root, nodes, leaves = svdag_arrays.oracle_svdag_arrays(oracle, bits, R)
+    s = pl.Svdag.from_view(R, root, nodes, leaves)
+    assert s.level_sizes[2] == len(leaves) and all(s.level_sizes[l] == len(nodes[l]) for l in nodes)
+    for maxT in (2, 4):
+        cfg = tdag_amd.TransformDAGConfig(True, True, False, maxT)
+        t = pl.construct_transform_dag(None, s, cfg)
+        mine = t.tdag_bytes()
+        assert mine == _oracle_tdag_bytes(oracle, bits, R, oracle.config(True, True, False, maxT))
+        d = tdag_decode.parse(mine)
+        assert np.array_equal(t.nodes(2), d["leaves"])
+        for level in range(3, root + 1):
+            assert np.array_equal(t.nodes(level), d["levels"][level])
+
+
+def test_edit_structure_view_is_validated(oracle, pl):
+    import svdag_arrays
+    import tdag_amd
+
+    bits = pl.scene_dense(0, 64, 1)
+    root, nodes, leaves = svdag_arrays.oracle_svdag_arrays(oracle, bits, 64)
+    bad = {l: a.copy() for l, a in nodes.items()}
+    bad[4][0, int(np.argmax(bad[4][0] != 0xFFFFFFFF))] = len(nodes[3])  # one past the end of level 3
+    with pytest.raises(tdag_amd.TdagError, match="EINVAL.*outside level 3"):
+        pl.Svdag.from_view(64, root, bad, leaves)
+    bad = {l: a.copy() for l, a in nodes.items()}
+    bad[3][0, int(np.argmax(bad[3][0] != 0xFFFFFFFF))] = len(leaves)
+    with pytest.raises(tdag_amd.TdagError, match="EINVAL"):
+        pl.Svdag.from_view(64, root, bad, leaves)
+    with pytest.raises(tdag_amd.TdagError, match="EINVAL"):
+        pl.Svdag.from_view(128, root, nodes, leaves)  # resolution != 2^root_level
+    with pytest.raises(tdag_amd.TdagError, match="EINVAL"):
+        pl.Svdag.from_view(64, root, nodes, leaves, structure_type=7)
+    # A Tree is accepted as a structure but refused by the driver (transform_dag.cpp:141).
+    tree = pl.Svdag.from_view(64, root, nodes, leaves, structure_type=0)
+    with pytest.raises(tdag_amd.TdagError, match="EINVAL.*DAG"):
+        pl.construct_transform_dag(None, tree, tdag_amd.TransformDAGConfig(True, True, False, 3))
