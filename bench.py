@@ -286,11 +286,27 @@ def bench_batched(ctx, cfg, dev, order, args):
     return res
 
 
-def bench_learned(ctx, cfg, dev, levels, info, args):
-    """Learned proposal at level 2: trained on the exact matcher's (leaf, parent) pairs from three
-    other seeded scenes of the same size, then run on this workload's level-2 input. Reports
-    recall of the verified proposals, the matcher time against the exhaustive call and the .tdag
-    size against the exhaustive build (every accepted match is exact either way)."""
+def ranks_of(masks):
+    """Rank of every item in the stable popcount-descending order (find_translation.cpp:631-643)."""
+    import numpy as np
+
+    m = np.ascontiguousarray(masks, dtype=np.uint64)
+    pop = np.bitwise_count(m.reshape(len(m), -1)).sum(axis=1).astype(np.int64)
+    order = np.argsort(-pop, kind="stable")
+    rank = np.empty(len(m), dtype=np.int64)
+    rank[order] = np.arange(len(m))
+    return rank
+
+
+def bench_combined(ctx, cfg, dev, order, levels, info, args, plain_ms: float):
+    """The combined form of the largest configuration: the batched matcher with the learned
+    proposal at level 2 (the network pre-trained on the exact matcher's level-2 pairs of three
+    seeded 1024^3 sphere+box+heightfield scenes, then fine-tuned on level-2 pairs of three seeded
+    scenes of this workload's kind: level-local pairs), proposals verified bitwise, exhaustive
+    fallback, the whole call one captured hipGraph. Reported against the plain per-level matcher
+    on the same inputs, with recall, the .tdag size against the exhaustive build, and per level
+    the time of the exhaustive search restricted to the items that have no match at all: the
+    least any proposal scheme with an exhaustive fallback can take there."""
     import numpy as np
     import torch
 
@@ -298,69 +314,121 @@ def bench_learned(ctx, cfg, dev, levels, info, args):
     from tdag_amd import pipeline as pl
 
     t0 = time.time()
-    qs, ps = [], []
     train_res = min(args.resolution, 1024)
-    for seed in (args.seed + 101, args.seed + 102, args.seed + 103):
-        bits = pl.scene_dense(SCENE_KINDS[args.workload], train_res, seed)
-        t = pl.construct_transform_dag(ctx, pl.Svdag(bits, train_res), cfg, keep_level_inputs=True)
-        m2 = np.ascontiguousarray(t.level_masks(2).reshape(-1))
-        d_m = torch.from_numpy(m2.view(np.int64)).cuda()
-        d_enc = torch.empty(max(1, len(m2)), dtype=torch.int64, device="cuda")
-        torch.cuda.synchronize()
-        ctx.match_level_device(2, d_m, len(m2), cfg, d_encoded=d_enc)
-        enc = d_enc.cpu().numpy().view(np.uint64)[: len(m2)]
-        q, p = learned.pairs_from_encoded(m2, enc)
-        qs.append(q)
-        ps.append(p)
-        t.close()
-    q, p = np.concatenate(qs), np.concatenate(ps)
-    d_q = torch.from_numpy(q.view(np.int64).copy()).cuda()
-    d_p = torch.from_numpy(p.view(np.int64).copy()).cuda()
-    torch.cuda.synchronize()
+    seeds = [args.seed + 101, args.seed + 102, args.seed + 103]
     mlp = learned.MLP(ctx, seed=args.seed)
-    t1 = time.time()
-    losses = mlp.train(d_q, d_p, len(q), steps=args.learned_steps, batch=512, lr=1e-3, tau=0.1, seed=args.seed + 7)
-    train_s = time.time() - t1
+    d_q, d_p, npre = learned.training_pairs(ctx, cfg, SCENE_KINDS["scene"], train_res, seeds)
+    tp = time.time()
+    losses_pre = mlp.train(d_q, d_p, npre, steps=args.learned_steps, batch=512, lr=1e-3, tau=0.1, seed=args.seed + 7)
+    pre_s = time.time() - tp
     d = dev[2]
-    n = d["n"]
-    d_enc = torch.empty(max(1, n), dtype=torch.int64, device="cuda")
-    ref_enc = torch.empty(max(1, n), dtype=torch.int64, device="cuda")
-    reps = max(1, args.steps)
-    ms_ex = 0.0
-    for _ in range(reps):
-        _, stp = ctx.match_level_device(2, d["masks"], n, cfg, d_encoded=ref_enc)
-        ms_ex += stp.ms_total
-    learned.match_level_learned_device(ctx, mlp, d["masks"], n, cfg, 64, d_encoded=d_enc)  # warm-up
-    ms, st = 0.0, None
-    for _ in range(reps):
-        st = learned.match_level_learned_device(ctx, mlp, d["masks"], n, cfg, 64, d_encoded=d_enc)
-        ms += st.ms_total
-    base = learned.match_level_learned_device(ctx, None, d["masks"], n, cfg, 64)
+    n2 = d["n"]
+    ref_enc = torch.empty(max(1, n2), dtype=torch.int64, device="cuda")
+    ctx.match_level_device(2, d["masks"], n2, cfg, d_encoded=ref_enc)
+    base = learned.match_level_learned_device(ctx, None, d["masks"], n2, cfg, 64)
+    st_pre = learned.match_level_learned_device(ctx, mlp, d["masks"], n2, cfg, 64)
+    fine = {"pairs": 0, "steps": 0}
+    losses_ft = None
+    if args.workload != "scene":
+        d_q, d_p, nft = learned.training_pairs(ctx, cfg, SCENE_KINDS[args.workload], train_res, seeds)
+        tf = time.time()
+        losses_ft = mlp.train(d_q, d_p, nft, steps=args.finetune_steps, batch=512, lr=3e-4, tau=0.1, seed=args.seed + 8)
+        fine = {"pairs": int(nft), "steps": args.finetune_steps, "train_s": round(time.time() - tf, 2),
+                "loss_first_last": [round(float(losses_ft[:20].mean()), 4), round(float(losses_ft[-20:].mean()), 4)],
+                "scenes": {"workload": args.workload, "resolution": train_res, "seeds": seeds}}
+    d_enc = torch.empty(max(1, n2), dtype=torch.int64, device="cuda")
+    st = learned.match_level_learned_device(ctx, mlp, d["masks"], n2, cfg, 64, d_encoded=d_enc)  # standalone learned call: the reference
     torch.cuda.synchronize()
-    same_key = int((d_enc[:n] == ref_enc[:n]).sum().item())
+    matched2 = int(st.num_matched)
+
+    def run_combined(concurrent: bool):
+        jobs, outs = [], {}
+        for level in order:
+            dl = dev[level]
+            outs[level] = {"enc": torch.empty(max(1, dl["n"]), dtype=torch.int64, device="cuda"),
+                           "map": torch.empty(max(1, dl["n"]), dtype=torch.int64, device="cuda")}
+            jobs.append({"level": level, "n": dl["n"], "d_masks": dl["masks"], "d_encoded": outs[level]["enc"], "d_mapping": outs[level]["map"],
+                         "d_unique": torch.empty(max(1, dl["n"]), dtype=torch.int32, device="cuda")})
+        torch.cuda.synchronize()
+        b = ctx.batch(jobs, cfg, use_graph=True, concurrent=concurrent, mlp=mlp, k=64)
+        try:
+            if not concurrent:
+                b.autotune(1)
+            for _ in range(max(1, args.warmup)):
+                b.run()
+            torch.cuda.synchronize()
+            tt = time.perf_counter()
+            matches = 0
+            for _ in range(args.steps):
+                stats, _ = b.run()
+                matches += sum(int(x.num_matched) for x in stats)
+            elapsed = time.perf_counter() - tt
+            hits, fb = b.learned_stats()
+        finally:
+            b.close()
+        exact = True
+        for level in order:
+            dl = dev[level]
+            n = dl["n"]
+            if level == 2:
+                exact = exact and bool(torch.equal(outs[2]["enc"][:n], d_enc[:n]))
+                continue
+            ref = torch.empty(max(1, n), dtype=torch.int64, device="cuda")
+            ctx.match_level_device(level, dl["masks"], n, cfg, d_encoded=ref)
+            torch.cuda.synchronize()
+            exact = exact and bool(torch.equal(ref[:n], outs[level]["enc"][:n]))
+        ms = elapsed * 1e3 / max(1, args.steps)
+        return {"ms_per_step": round(ms, 4), "matches_per_s": matches / elapsed if elapsed > 0 else None,
+                "vs_plain": round(plain_ms / ms, 4) if ms > 0 else None, "matches_per_step": matches // max(1, args.steps),
+                "levels_3_5_bit_exact_vs_per_level_and_level_2_equal_to_the_learned_matcher": exact,
+                "level2_proposal_hits": hits, "level2_fallback_queries": fb}
+
+    persistent = run_combined(False)
+    concurrent = run_combined(True)
+    # Every accepted level-2 match is exact (verified on the device) and never below the exhaustive key.
+    never_below = bool(np.all(d_enc[:n2].cpu().numpy().view(np.uint64) >= ref_enc[:n2].cpu().numpy().view(np.uint64)))
     tdag_l = pl.construct_transform_dag_learned(ctx, info["svdag"], cfg, mlp, 64)
     learned_bytes = len(tdag_l.tdag_bytes())
     tdag_l.close()
+    # Least time of any proposal scheme with an exhaustive fallback: the search with only the
+    # items that have no match in its table.
+    bounds = {}
+    for level in order:
+        dl = dev[level]
+        n = dl["n"]
+        enc = torch.empty(max(1, n), dtype=torch.int64, device="cuda")
+        _, full = ctx.match_level_device(level, dl["masks"], n, cfg, d_encoded=enc)
+        keys = enc[:n].cpu().numpy().view(np.uint64)
+        ptr = (keys >> np.uint64(6 + 3 * (level + 1))).astype(np.int64)
+        rank = ranks_of(levels[level]["masks"])
+        unmatched = (ptr == rank) | (ptr == 0xFFFFFFFF)
+        flags = torch.from_numpy(unmatched.astype(np.int32)).cuda()
+        sub = ctx.match_level_subset_device(level, dl["masks"], n, flags, cfg)
+        bounds[level] = {"unmatched_items": int(unmatched.sum()), "items": n, "search_ms_all_children": round(full.ms_search, 4),
+                         "search_ms_unmatched_children_only": round(sub.ms_search, 4)}
     mlp.close()
-    matched = int(st.num_matched)
     return {
-        "level": 2,
-        "train_pairs": int(len(q)),
-        "train_scenes": {"workload": args.workload, "resolution": train_res, "seeds": [args.seed + 101, args.seed + 102, args.seed + 103]},
-        "train_steps": args.learned_steps,
-        "train_s": round(train_s, 2),
-        "loss_first_last": [round(float(losses[:20].mean()), 4), round(float(losses[-20:].mean()), 4)],
-        "matcher_ms": round(ms / reps, 4),
-        "exhaustive_matcher_ms": round(ms_ex / reps, 4),
-        "ms_split": {"embed": round(st.ms_embed, 4), "propose": round(st.ms_propose, 4), "verify": round(st.ms_verify, 4), "fallback": round(st.ms_fallback, 4)},
-        "matched": matched,
-        "recall_at_64": round(st.proposal_hits / matched, 4) if matched else None,
-        "recall_at_64_popcount_neighbours": round(base.proposal_hits / matched, 4) if matched else None,
-        "fallback_queries": int(st.fallback_queries),
-        "same_parent_as_exhaustive": round(same_key / n, 4) if n else None,
+        "form": "batched matcher with the learned level-2 proposal (verified, exhaustive fallback, parent chase) in one captured hipGraph",
+        "persistent": persistent,
+        "concurrent": concurrent,
+        "pretrain": {"pairs": int(npre), "steps": args.learned_steps, "train_s": round(pre_s, 2),
+                     "loss_first_last": [round(float(losses_pre[:20].mean()), 4), round(float(losses_pre[-20:].mean()), 4)],
+                     "scenes": {"workload": "scene", "resolution": train_res, "seeds": seeds}},
+        "finetune_level_local": fine,
+        "level2": {
+            "items": n2, "matched": matched2,
+            "recall_at_64": round(st.proposal_hits / matched2, 4) if matched2 else None,
+            "recall_at_64_pretrained_only": round(st_pre.proposal_hits / matched2, 4) if matched2 else None,
+            "recall_at_64_popcount_neighbours": round(base.proposal_hits / matched2, 4) if matched2 else None,
+            "same_parent_as_exhaustive": round(int((d_enc[:n2] == ref_enc[:n2]).sum().item()) / n2, 4) if n2 else None,
+            "never_below_exhaustive_key": never_below,
+            "learned_matcher_ms": round(st.ms_total, 4),
+            "ms_split": {"embed": round(st.ms_embed, 4), "propose": round(st.ms_propose, 4), "verify": round(st.ms_verify, 4), "fallback": round(st.ms_fallback, 4)},
+        },
         "tdag_bytes": learned_bytes,
         "tdag_bytes_exhaustive": info["tdag_bytes"],
         "tdag_size_delta": round(learned_bytes / info["tdag_bytes"] - 1.0, 5),
+        "fallback_bound_per_level": bounds,
         "setup_s": round(time.time() - t0, 2),
     }
 
@@ -570,9 +638,10 @@ def main():
     ap.add_argument("--save-tuning", default="", help="write the tuning in use to this file")
     ap.add_argument("--counters", action="store_true", help="also run each level once with per-stage probe counters (diagnostics, untimed)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched forms")
-    ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal measurement at level 2")
+    ap.add_argument("--no-learned", action="store_true", help="skip the combined form (batched + learned level-2 proposal + fallback)")
     ap.add_argument("--no-extras", action="store_true", help="skip the .tdag16 encode/decode and the whole-driver timings of the workload build")
-    ap.add_argument("--learned-steps", type=int, default=1500, help="training steps of the learned proposal")
+    ap.add_argument("--learned-steps", type=int, default=1500, help="pre-training steps of the learned proposal")
+    ap.add_argument("--finetune-steps", type=int, default=500, help="level-local fine-tuning steps of the learned proposal")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB three times (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -634,7 +703,7 @@ def main():
 
     host_buffers = None if args.no_host_buffers else time_host_buffers(ctx, cfg, levels, order, 2)
     batched = None if args.no_batched else bench_batched(ctx, cfg, dev, order, args)
-    learned_res = None if (args.no_learned or 2 not in dev) else bench_learned(ctx, cfg, dev, levels, info, args)
+    learned_res = None if (args.no_learned or 2 not in dev) else bench_combined(ctx, cfg, dev, order, levels, info, args, elapsed * 1e3 / max(1, args.steps))
 
     per_step_matches = matches // max(1, args.steps)
     expected = sum(levels[l]["matched"] for l in order)
@@ -670,7 +739,7 @@ def main():
             "tuning_source": tuning_source,
             **({"probe_counters": counters} if counters else {}),
             **({"batched": batched} if batched else {}),
-            **({"learned": learned_res} if learned_res else {}),
+            **({"combined": learned_res} if learned_res else {}),
             "parallelism": "single GPU",
         },
         "roofline": roofline(args, dom, per_level[dom]["search_ms"], levels[dom]["items"], tuning, ctx.get_variant(dom)),

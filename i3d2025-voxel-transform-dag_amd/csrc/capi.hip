@@ -12,6 +12,7 @@
 #include "batched.hpp"
 #include "context.hpp"
 #include "errors.hpp"
+#include "learned.hpp"
 #include "matcher.hpp"
 #include "tdag.h"
 
@@ -97,6 +98,14 @@ struct tdag_batch {
     hipStream_t aux[tdag::kBatchMaxLevels] {};
     hipEvent_t evFork = nullptr;
     hipEvent_t evJoin[tdag::kBatchMaxLevels] {};
+    // Learned level-2 job (tdag_batch_create_learned): the proposal matcher on its own branch.
+    tdag_mlp* mlp = nullptr;
+    uint32_t k = 0;
+    int learned = -1; // job index, -1: none
+    tdag::MatchPlan lplan {};
+    size_t lOff = 0, lWorkOff = 0; // its scratch and proposal workspace inside `scratch`
+    hipStream_t lStream = nullptr;
+    hipEvent_t lFork = nullptr, lJoin = nullptr;
 };
 
 namespace tdag {
@@ -217,6 +226,70 @@ tdag_status tdag_match_level_device(tdag_context* ctx, uint32_t level, const uin
             *out_num_unique = s.num_unique;
         if (stats)
             *stats = s;
+    }
+    return TDAG_OK;
+}
+
+} // extern "C"
+
+namespace {
+
+// Child flags by rank: rankFlags[r] = flags[order[r]] (order = rank -> original index).
+__global__ void kRankFlags(const uint32_t* __restrict__ order, uint32_t n, const uint32_t* __restrict__ flags, uint32_t* __restrict__ rankFlags)
+{
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n)
+        rankFlags[r] = flags[order[r]] != 0u ? 1u : 0u;
+}
+
+} // namespace
+
+extern "C" {
+
+tdag_status tdag_match_level_subset_device(tdag_context* ctx, uint32_t level, const uint64_t* d_masks, uint64_t n, const tdag_config* cfg,
+    const uint32_t* d_child_flags, uint64_t* d_encoded, tdag_match_stats* stats)
+{
+    if (tdag_status s = checkArgs(ctx, level, n, cfg); s != TDAG_OK)
+        return s;
+    if (n > 0 && (!d_masks || !d_child_flags))
+        return fail(TDAG_EINVAL, "d_masks / d_child_flags is null");
+    CAPI_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    hipStream_t st = ctx->stream;
+    const int32_t v = ctx->variant[level];
+    const tdag::MatchPlan plan = tdag::planMatch(level, n, v >= 0 ? (uint32_t)v : tdag::defaultVariant(level), ctx->firstBatch[level], ctx->batchGrowth[level],
+        ctx->bloomBits[level], ctx->childFrames[level]);
+    const size_t flagsOff = (plan.scratchBytes + 255) & ~size_t(255);
+    if (tdag_status s = ensureBuffer(ctx->scratch, ctx->scratchBytes, flagsOff + std::max<size_t>(256, n * 4), "matcher scratch allocation"); s != TDAG_OK)
+        return s;
+    uint32_t* rankFlags = (uint32_t*)((char*)ctx->scratch + flagsOff);
+    const uint32_t cfgBits = cfgBitsOf(cfg) | (ctx->counters ? tdag::kCfgCount : 0u);
+    CAPI_HIP(hipEventRecord(ctx->ev[0], st), "hipEventRecord");
+    CAPI_HIP(tdag::enqueuePrepareOrder(plan, d_masks, ctx->scratch, st), "matcher launch");
+    if (n > 0) {
+        hipLaunchKernelGGL(kRankFlags, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, (const uint32_t*)((char*)ctx->scratch + plan.offOrder), (uint32_t)n,
+            d_child_flags, rankFlags);
+        CAPI_HIP(hipGetLastError(), "child flags");
+        CAPI_HIP(tdag::enqueueBuildTable(plan, ctx->scratch, st, rankFlags), "matcher launch");
+    }
+    CAPI_HIP(hipEventRecord(ctx->ev[1], st), "hipEventRecord");
+    if (n > 0)
+        CAPI_HIP(tdag::enqueueSearch(plan, cfgBits, ctx->scratch, st), "matcher launch");
+    CAPI_HIP(hipEventRecord(ctx->ev[2], st), "hipEventRecord");
+    CAPI_HIP(tdag::enqueueFinish(plan, ctx->scratch, st, d_encoded, nullptr, nullptr), "matcher launch");
+    CAPI_HIP(hipEventRecord(ctx->ev[3], st), "hipEventRecord");
+    CAPI_HIP(hipMemcpyAsync(ctx->pinned, (char*)ctx->scratch + plan.offCounters, 16, hipMemcpyDeviceToHost, st), "counter readback");
+    CAPI_HIP(hipStreamSynchronize(st), "hipStreamSynchronize");
+    if (stats) {
+        tdag_match_stats s {};
+        s.num_items = n;
+        s.num_matched = ctx->pinned[0];
+        s.num_unique = n > 0 ? (uint32_t)(ctx->pinned[1] & 0xFFFFFFFFull) : 0;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[3]) == hipSuccess)
+            s.ms_total = ms;
+        if (hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]) == hipSuccess)
+            s.ms_search = ms;
+        *stats = s;
     }
     return TDAG_OK;
 }
@@ -494,9 +567,14 @@ tdag_status enqueueConcurrentCall(tdag_batch* b)
     for (uint32_t i = 0; i < k; ++i) {
         const tdag_level_job& j = b->jobs[i];
         CAPI_HIP(hipStreamWaitEvent(b->aux[i], b->evFork, 0), "fork wait");
-        CAPI_HIP(tdag::enqueueMatch(b->cplans[i], j.d_masks, b->cfgBits, (char*)b->scratch + b->cOff[i], b->aux[i], j.d_encoded, j.d_mapping, j.d_unique, nullptr,
-                     nullptr),
-            "concurrent matcher launch");
+        if ((int)i == b->learned)
+            CAPI_HIP(tdag::learn::enqueueLearned2(b->cplans[i], &b->mlp->mlp, b->k, j.d_masks, b->cfgBits & ~tdag::kCfgCount, (char*)b->scratch + b->cOff[i],
+                         (char*)b->scratch + b->lWorkOff, b->aux[i], j.d_encoded, j.d_mapping, j.d_unique, nullptr),
+                "learned matcher launch");
+        else
+            CAPI_HIP(tdag::enqueueMatch(b->cplans[i], j.d_masks, b->cfgBits, (char*)b->scratch + b->cOff[i], b->aux[i], j.d_encoded, j.d_mapping, j.d_unique, nullptr,
+                         nullptr),
+                "concurrent matcher launch");
         CAPI_HIP(hipEventRecord(b->evJoin[i], b->aux[i]), "join event");
         CAPI_HIP(hipStreamWaitEvent(main, b->evJoin[i], 0), "join wait");
     }
@@ -507,32 +585,72 @@ tdag_status enqueueBatchCall(tdag_batch* b)
 {
     if (b->concurrent)
         return enqueueConcurrentCall(b);
-    const uint32_t k = b->plan.count;
-    std::vector<const uint64_t*> masks(k);
-    std::vector<uint64_t*> enc(k), map(k);
-    std::vector<uint32_t*> uniq(k);
-    for (uint32_t i = 0; i < k; ++i) {
-        masks[i] = b->jobs[i].d_masks;
-        enc[i] = b->jobs[i].d_encoded;
-        map[i] = b->jobs[i].d_mapping;
-        uniq[i] = b->jobs[i].d_unique;
+    hipStream_t main = b->ctx->stream;
+    if (b->learned >= 0) { // the learned level-2 matcher on its own branch beside the persistent kernel
+        const tdag_level_job& j = b->jobs[b->learned];
+        CAPI_HIP(hipEventRecord(b->lFork, main), "fork event");
+        CAPI_HIP(hipStreamWaitEvent(b->lStream, b->lFork, 0), "fork wait");
+        CAPI_HIP(tdag::learn::enqueueLearned2(b->lplan, &b->mlp->mlp, b->k, j.d_masks, b->cfgBits & ~tdag::kCfgCount, (char*)b->scratch + b->lOff,
+                     (char*)b->scratch + b->lWorkOff, b->lStream, j.d_encoded, j.d_mapping, j.d_unique, nullptr),
+            "learned matcher launch");
+        CAPI_HIP(hipEventRecord(b->lJoin, b->lStream), "join event");
     }
-    CAPI_HIP(tdag::enqueueBatch(b->plan, masks.data(), b->scratch, b->ctx->stream, enc.data(), map.data(), uniq.data(), nullptr, nullptr, b->spinTicks),
-        "batched matcher launch");
+    const uint32_t k = b->plan.count;
+    if (k > 0) {
+        // planBatch's job indices refer to the jobs without the learned one; map them back.
+        std::vector<const uint64_t*> masks(b->jobs.size());
+        std::vector<uint64_t*> enc(b->jobs.size()), map(b->jobs.size());
+        std::vector<uint32_t*> uniq(b->jobs.size());
+        uint32_t c = 0;
+        for (uint32_t i = 0; i < b->jobs.size(); ++i) {
+            if ((int)i == b->learned)
+                continue;
+            masks[c] = b->jobs[i].d_masks;
+            enc[c] = b->jobs[i].d_encoded;
+            map[c] = b->jobs[i].d_mapping;
+            uniq[c] = b->jobs[i].d_unique;
+            ++c;
+        }
+        CAPI_HIP(tdag::enqueueBatch(b->plan, masks.data(), b->scratch, main, enc.data(), map.data(), uniq.data(), nullptr, nullptr, b->spinTicks),
+            "batched matcher launch");
+    }
+    if (b->learned >= 0)
+        CAPI_HIP(hipStreamWaitEvent(main, b->lJoin, 0), "join wait");
     return TDAG_OK;
+}
+
+// Caller's job index of the persistent plan's entry i (the learned job is not in the plan).
+uint32_t persistentJob(const tdag_batch* b, uint32_t i)
+{
+    uint32_t j = b->plan.job[i];
+    if (b->learned >= 0 && j >= (uint32_t)b->learned)
+        ++j;
+    return j;
 }
 
 } // namespace
 
 extern "C" {
 
-tdag_status tdag_batch_create(tdag_context* ctx, const tdag_level_job* jobs, uint32_t count, const tdag_config* cfg, int use_graph, tdag_batch** out)
+static tdag_status createBatch(tdag_context* ctx, const tdag_level_job* jobs, uint32_t count, const tdag_config* cfg, uint32_t flags, tdag_mlp* mlp, uint32_t kProp,
+    tdag_batch** out)
 {
-    const uint32_t flags = (uint32_t)use_graph;
     if (!ctx || !jobs || !cfg || !out)
         return fail(TDAG_EINVAL, "null argument");
     if (count == 0 || count > tdag::kBatchMaxLevels)
         return fail(TDAG_EINVAL, "a batch holds 1 to 4 levels");
+    int learned = -1;
+    if (mlp) {
+        if (mlp->ctx != ctx)
+            return fail(TDAG_EINVAL, "the network belongs to another context");
+        if (kProp == 0 || kProp > tdag::learn::kMaxProposals)
+            return fail(TDAG_EINVAL, "k must be in [1, 64]");
+        for (uint32_t i = 0; i < count; ++i)
+            if (jobs[i].level == 2)
+                learned = (int)i;
+        if (learned < 0)
+            return fail(TDAG_EINVAL, "the learned proposal needs a level-2 job");
+    }
     uint32_t levels[tdag::kBatchMaxLevels], first[tdag::kBatchMaxLevels], bits[tdag::kBatchMaxLevels], frames[tdag::kBatchMaxLevels];
     uint64_t ns[tdag::kBatchMaxLevels];
     for (uint32_t i = 0; i < count; ++i) {
@@ -554,10 +672,18 @@ tdag_status tdag_batch_create(tdag_context* ctx, const tdag_level_job* jobs, uin
     b->ctx = ctx;
     b->jobs.assign(jobs, jobs + count);
     b->concurrent = (flags & TDAG_BATCH_CONCURRENT) != 0;
+    b->mlp = mlp;
+    b->k = kProp;
+    b->learned = learned;
     auto cleanup = [&](tdag_status s) {
         tdag_batch_destroy(b);
         return s;
     };
+    // The learned job's plan: the one tdag_match_level_learned_device uses.
+    const auto learnedPlan = [&](uint64_t n) { return tdag::planMatch(2, n, 0, ctx->firstBatch[2], ctx->batchGrowth[2], ctx->bloomBits[2]); };
+    if (learned >= 0)
+        if (hipError_t e = tdag::learn::mlpPrepareEmbed(mlp->mlp, jobs[learned].n); e != hipSuccess)
+            return cleanup(failHip(e, "network workspace"));
     if (b->concurrent) {
         // The same plans tdag_match_level_device would use for each level (tuned variant and
         // schedule), each with its own scratch, stream and join event.
@@ -565,10 +691,15 @@ tdag_status tdag_batch_create(tdag_context* ctx, const tdag_level_job* jobs, uin
         for (uint32_t i = 0; i < count; ++i) {
             const uint32_t l = jobs[i].level;
             const int32_t v = ctx->variant[l];
-            b->cplans[i] = tdag::planMatch(l, jobs[i].n, v >= 0 ? (uint32_t)v : tdag::defaultVariant(l), ctx->firstBatch[l], ctx->batchGrowth[l], ctx->bloomBits[l],
-                ctx->childFrames[l]);
+            b->cplans[i] = (int)i == learned ? learnedPlan(jobs[i].n)
+                                             : tdag::planMatch(l, jobs[i].n, v >= 0 ? (uint32_t)v : tdag::defaultVariant(l), ctx->firstBatch[l], ctx->batchGrowth[l],
+                                                   ctx->bloomBits[l], ctx->childFrames[l]);
             b->cOff[i] = off;
             off += (b->cplans[i].scratchBytes + 255) & ~size_t(255);
+        }
+        if (learned >= 0) {
+            b->lWorkOff = off;
+            off += (tdag::learn::learnedWorkBytes(jobs[learned].n) + 255) & ~size_t(255);
         }
         if (hipError_t e = hipMalloc(&b->scratch, std::max<size_t>(off, 256)); e != hipSuccess)
             return cleanup(failHip(e, "concurrent matcher scratch allocation"));
@@ -581,12 +712,43 @@ tdag_status tdag_batch_create(tdag_context* ctx, const tdag_level_job* jobs, uin
         if (hipError_t e = hipEventCreateWithFlags(&b->evFork, hipEventDisableTiming); e != hipSuccess)
             return cleanup(failHip(e, "batch event creation"));
     } else {
-        if (hipError_t e = tdag::planBatch(count, levels, ns, first, bits, frames, ctx->device, &b->plan); e != hipSuccess)
-            return cleanup(failHip(e, "batched matcher plan"));
-        if (hipError_t e = hipMalloc(&b->scratch, b->plan.scratchBytes); e != hipSuccess)
+        // The persistent kernel takes every job but the learned one.
+        uint32_t pl[tdag::kBatchMaxLevels], pf[tdag::kBatchMaxLevels], pb[tdag::kBatchMaxLevels], pr[tdag::kBatchMaxLevels];
+        uint64_t pn[tdag::kBatchMaxLevels];
+        uint32_t c = 0;
+        for (uint32_t i = 0; i < count; ++i) {
+            if ((int)i == learned)
+                continue;
+            pl[c] = levels[i];
+            pn[c] = ns[i];
+            pf[c] = first[i];
+            pb[c] = bits[i];
+            pr[c] = frames[i];
+            ++c;
+        }
+        size_t total = 0;
+        if (c > 0) {
+            if (hipError_t e = tdag::planBatch(c, pl, pn, pf, pb, pr, ctx->device, &b->plan); e != hipSuccess)
+                return cleanup(failHip(e, "batched matcher plan"));
+            total = (b->plan.scratchBytes + 255) & ~size_t(255);
+        }
+        if (learned >= 0) {
+            b->lplan = learnedPlan(jobs[learned].n);
+            b->lOff = total;
+            total += (b->lplan.scratchBytes + 255) & ~size_t(255);
+            b->lWorkOff = total;
+            total += (tdag::learn::learnedWorkBytes(jobs[learned].n) + 255) & ~size_t(255);
+            if (hipError_t e = hipStreamCreateWithFlags(&b->lStream, hipStreamNonBlocking); e != hipSuccess) // default priority
+                return cleanup(failHip(e, "batch stream creation"));
+            if (hipError_t e = hipEventCreateWithFlags(&b->lFork, hipEventDisableTiming); e != hipSuccess)
+                return cleanup(failHip(e, "batch event creation"));
+            if (hipError_t e = hipEventCreateWithFlags(&b->lJoin, hipEventDisableTiming); e != hipSuccess)
+                return cleanup(failHip(e, "batch event creation"));
+        }
+        if (hipError_t e = hipMalloc(&b->scratch, std::max<size_t>(total, 256)); e != hipSuccess)
             return cleanup(failHip(e, "batched matcher scratch allocation"));
     }
-    if (hipError_t e = hipHostMalloc((void**)&b->pinned, 8 * (tdag::kNumCounters * count + 1)); e != hipSuccess)
+    if (hipError_t e = hipHostMalloc((void**)&b->pinned, 8 * (tdag::kNumCounters * count + 3)); e != hipSuccess)
         return cleanup(failHip(e, "pinned readback allocation"));
     int rateKHz = 0;
     if (hipError_t e = hipDeviceGetAttribute(&rateKHz, hipDeviceAttributeWallClockRate, ctx->device); e != hipSuccess)
@@ -594,7 +756,7 @@ tdag_status tdag_batch_create(tdag_context* ctx, const tdag_level_job* jobs, uin
     b->spinTicks = (uint64_t)std::max(rateKHz, 1) * 1000ull * 10ull; // 10 s
     b->cfgBits = cfgBitsOf(cfg) | (ctx->counters ? tdag::kCfgCount : 0u);
     b->grab = 4;
-    if (!b->concurrent)
+    if (!b->concurrent && b->plan.count > 0)
         if (tdag_status s = uploadBatchDesc(b); s != TDAG_OK)
             return cleanup(s);
     if (flags & TDAG_BATCH_GRAPH) {
@@ -615,6 +777,19 @@ tdag_status tdag_batch_create(tdag_context* ctx, const tdag_level_job* jobs, uin
     return TDAG_OK;
 }
 
+tdag_status tdag_batch_create(tdag_context* ctx, const tdag_level_job* jobs, uint32_t count, const tdag_config* cfg, int use_graph, tdag_batch** out)
+{
+    return createBatch(ctx, jobs, count, cfg, (uint32_t)use_graph, nullptr, 0, out);
+}
+
+tdag_status tdag_batch_create_learned(tdag_context* ctx, const tdag_level_job* jobs, uint32_t count, const tdag_config* cfg, uint32_t flags, tdag_mlp* mlp, uint32_t k,
+    tdag_batch** out)
+{
+    if (!mlp)
+        return fail(TDAG_EINVAL, "mlp is null");
+    return createBatch(ctx, jobs, count, cfg, flags, mlp, k, out);
+}
+
 tdag_status tdag_batch_run(tdag_batch* b, tdag_match_stats* out_stats, float* out_ms)
 {
     if (!b)
@@ -629,13 +804,26 @@ tdag_status tdag_batch_run(tdag_batch* b, tdag_match_stats* out_stats, float* ou
     }
     CAPI_HIP(hipEventRecord(ctx->ev[3], ctx->stream), "hipEventRecord");
     const uint32_t k = (uint32_t)b->jobs.size();
-    for (uint32_t i = 0; i < k; ++i) {
+    const uint32_t numLaunched = b->concurrent ? k : b->plan.count;
+    for (uint32_t i = 0; i < numLaunched; ++i) {
         const tdag::LevelView v = b->concurrent ? tdag::levelView(b->cplans[i], (char*)b->scratch + b->cOff[i])
                                                 : tdag::levelView(b->plan.plans[i], (char*)b->scratch + b->plan.scratchOff[i]);
-        const uint32_t job = b->concurrent ? i : b->plan.job[i];
+        const uint32_t job = b->concurrent ? i : persistentJob(b, i);
         CAPI_HIP(hipMemcpyAsync(b->pinned + tdag::kNumCounters * job, v.counters, 8 * tdag::kNumCounters, hipMemcpyDeviceToHost, ctx->stream), "counter readback");
     }
-    if (b->concurrent)
+    if (b->learned >= 0) {
+        const tdag::MatchPlan& lp = b->concurrent ? b->cplans[b->learned] : b->lplan;
+        char* base = (char*)b->scratch + (b->concurrent ? b->cOff[b->learned] : b->lOff);
+        if (!b->concurrent) {
+            const tdag::LevelView v = tdag::levelView(lp, base);
+            CAPI_HIP(hipMemcpyAsync(b->pinned + tdag::kNumCounters * b->learned, v.counters, 8 * tdag::kNumCounters, hipMemcpyDeviceToHost, ctx->stream),
+                "counter readback");
+        }
+        CAPI_HIP(hipMemcpyAsync(b->pinned + tdag::kNumCounters * k + 1, tdag::learn::learnedCounters(lp, (char*)b->scratch + b->lWorkOff), 16,
+                     hipMemcpyDeviceToHost, ctx->stream),
+            "learned counter readback");
+    }
+    if (b->concurrent || b->plan.count == 0)
         b->pinned[tdag::kNumCounters * k] = 0; // no grid barrier in this mode
     else
         CAPI_HIP(hipMemcpyAsync(b->pinned + tdag::kNumCounters * k, (char*)b->scratch + b->plan.syncOff + offsetof(tdag::BatchSync, error), 4, hipMemcpyDeviceToHost,
@@ -672,7 +860,7 @@ tdag_status tdag_batch_autotune(tdag_batch* b, uint32_t repeats, uint32_t* out_g
 {
     if (!b)
         return fail(TDAG_EINVAL, "batch is null");
-    if (b->concurrent) { // no work queue to tune: each level runs its own tuned kernels
+    if (b->concurrent || b->plan.count == 0) { // no work queue to tune: each level runs its own tuned kernels
         if (out_grab)
             *out_grab = 0;
         return TDAG_OK;
@@ -706,6 +894,20 @@ tdag_status tdag_batch_autotune(tdag_batch* b, uint32_t repeats, uint32_t* out_g
     return TDAG_OK;
 }
 
+tdag_status tdag_batch_learned_stats(const tdag_batch* b, uint64_t* out_proposal_hits, uint64_t* out_fallback_queries)
+{
+    if (!b)
+        return fail(TDAG_EINVAL, "batch is null");
+    if (b->learned < 0)
+        return fail(TDAG_EINVAL, "the batch has no learned job");
+    const uint32_t k = (uint32_t)b->jobs.size();
+    if (out_proposal_hits)
+        *out_proposal_hits = b->pinned[tdag::kNumCounters * k + 1];
+    if (out_fallback_queries)
+        *out_fallback_queries = b->pinned[tdag::kNumCounters * k + 2];
+    return TDAG_OK;
+}
+
 tdag_status tdag_batch_destroy(tdag_batch* b)
 {
     if (!b)
@@ -726,6 +928,12 @@ tdag_status tdag_batch_destroy(tdag_batch* b)
             (void)hipEventDestroy(ev);
     if (b->evFork)
         (void)hipEventDestroy(b->evFork);
+    if (b->lStream)
+        (void)hipStreamDestroy(b->lStream);
+    if (b->lFork)
+        (void)hipEventDestroy(b->lFork);
+    if (b->lJoin)
+        (void)hipEventDestroy(b->lJoin);
     delete b;
     return TDAG_OK;
 }

@@ -12,10 +12,6 @@
 
 using namespace tdag::capi;
 
-struct tdag_mlp {
-    tdag_context* ctx = nullptr;
-    tdag::learn::Mlp mlp {};
-};
 
 extern "C" {
 

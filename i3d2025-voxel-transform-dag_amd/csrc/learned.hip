@@ -1150,7 +1150,12 @@ size_t learnedWorkBytes(uint64_t n)
     // proposals, flags, bf16 embeddings, counters, 48 images and boxes per leaf, the retrieval's
     // per-chunk lists (the embedding workspace itself lives in the Mlp)
     return align256(n * kMaxProposals * 4) + align256(n * 4) + align256((n + kQ) * kEmb * 2) + 256 + align256(n * 48 * 8) + align256(n * 48 * 2) +
-        align256(n * 8) + topkWorkBytes(n);
+        align256(n * 8) + align256(n * kEmb * 4) + topkWorkBytes(n);
+}
+
+hipError_t mlpPrepareEmbed(Mlp& mlp, uint64_t n)
+{
+    return n == 0 ? hipSuccess : ensureWork(mlp, std::min<uint64_t>(n, 65536), 2);
 }
 
 hipError_t gemmBf16(bool ta, bool tb, const uint16_t* A, int lda, const uint16_t* B, int ldb, float* C, int ldc, int M, int N, int K, hipStream_t stream)
@@ -1313,8 +1318,8 @@ hipError_t mlpEmbed(Mlp& mlp, const uint64_t* d_masks, uint64_t n, float* d_out,
     return hipSuccess;
 }
 
-hipError_t matchLearned2(const MatchPlan& p, Mlp* mlp, uint32_t k, const uint64_t* d_masks, uint32_t cfgBits, void* d_scratch, void* d_work, hipStream_t s,
-    uint64_t* d_encoded, uint64_t* d_mapping, uint32_t* d_unique, LearnedStats* stats)
+hipError_t enqueueLearned2(const MatchPlan& p, Mlp* mlp, uint32_t k, const uint64_t* d_masks, uint32_t cfgBits, void* d_scratch, void* d_work, hipStream_t s,
+    uint64_t* d_encoded, uint64_t* d_mapping, uint32_t* d_unique, hipEvent_t* ev)
 {
     if (p.level != 2 || k == 0 || k > kMaxProposals)
         return hipErrorInvalidValue;
@@ -1328,8 +1333,64 @@ hipError_t matchLearned2(const MatchPlan& p, Mlp* mlp, uint32_t k, const uint64_
     uint64_t* images = (uint64_t*)((char*)cnt + 256);
     uint16_t* boxes = (uint16_t*)((char*)images + align256((size_t)n * 48 * 8));
     uint64_t* prevBest = (uint64_t*)((char*)boxes + align256((size_t)n * 48 * 2));
-    void* topkWork = (char*)prevBest + align256((size_t)n * 8);
-    hipEvent_t ev[5];
+    float* E = (float*)((char*)prevBest + align256((size_t)n * 8));
+    void* topkWork = (char*)E + align256((size_t)n * kEmb * 4);
+    auto mark = [&](int i) {
+        if (ev)
+            (void)hipEventRecord(ev[i], s);
+    };
+    LEARN_TRY(fillU64((uint64_t*)cnt, 2, 0ull, s));
+    LEARN_TRY(enqueuePrepareOrder(p, d_masks, d_scratch, s));
+    mark(0);
+    if (n > 0) {
+        // 1. Proposals.
+        if (mlp) {
+            if (mlp->wsBytes < workBytes(std::min<uint64_t>(n, 65536), 2))
+                return hipErrorInvalidValue; // mlpPrepareEmbed was not called for this size
+            LEARN_TRY(mlpEmbed(*mlp, v.sortedMasks, n, E, s));
+            hipLaunchKernelGGL(kCastBf, dim3((unsigned)cdiv((size_t)n * kEmb, 256)), dim3(256), 0, s, E, (size_t)n * kEmb, Ebf);
+            LEARN_TRY(hipGetLastError());
+            mark(1);
+            LEARN_TRY(topkPropose(Ebf, n, k, cand, topkWork, s));
+        } else {
+            mark(1);
+            hipLaunchKernelGGL(kNeighbourProposals, dim3((unsigned)cdiv((size_t)n * k, 256)), dim3(256), 0, s, n, k, cand);
+        }
+        LEARN_TRY(hipGetLastError());
+        mark(2);
+        // 2. Bitwise verification of every proposal against the 48 precomputed images.
+        hipLaunchKernelGGL(kTransforms48, dim3((unsigned)cdiv((size_t)n * 48, 256)), dim3(256), 0, s, v.sortedMasks, n, images, boxes);
+        hipLaunchKernelGGL(kVerifyProposals, dim3((unsigned)cdiv(n, 4)), dim3(256), 0, s, v.sortedMasks, images, boxes, n, k, cand, cfgBits, v.best, cnt);
+        hipLaunchKernelGGL(kFallbackFlags, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, s, v.sortedMasks, v.best, n, flags, cnt + 1);
+        LEARN_TRY(hipGetLastError());
+        mark(3);
+        // 3. Exhaustive search for the queries without a verified proposal.
+        LEARN_TRY(enqueueBuildTable(p, d_scratch, s, flags));
+        LEARN_TRY(enqueueSearch(p, cfgBits, d_scratch, s));
+        // 4. Parent chase (pointer jumping along verified parent chains).
+        for (int pass = 0; pass < kChasePasses; ++pass) {
+            LEARN_TRY(hipMemcpyAsync(prevBest, v.best, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
+            hipLaunchKernelGGL(kChaseParents, dim3((unsigned)cdiv(n, 16)), dim3(256), 0, s, v.sortedMasks, images, boxes, n, cfgBits, prevBest, v.best);
+            LEARN_TRY(hipGetLastError());
+        }
+        mark(4);
+    }
+    return enqueueFinish(p, d_scratch, s, d_encoded, d_mapping, d_unique);
+}
+
+const unsigned long long* learnedCounters(const MatchPlan& p, void* d_work)
+{
+    const size_t n = p.n;
+    const size_t off = align256(n * kMaxProposals * 4) + align256(n * 4) + align256((n + kQ) * kEmb * 2);
+    return (const unsigned long long*)((char*)d_work + off);
+}
+
+hipError_t matchLearned2(const MatchPlan& p, Mlp* mlp, uint32_t k, const uint64_t* d_masks, uint32_t cfgBits, void* d_scratch, void* d_work, hipStream_t s,
+    uint64_t* d_encoded, uint64_t* d_mapping, uint32_t* d_unique, LearnedStats* stats)
+{
+    if (mlp)
+        LEARN_TRY(mlpPrepareEmbed(*mlp, p.n));
+    hipEvent_t ev[5] = {};
     for (auto& e : ev)
         LEARN_TRY(hipEventCreate(&e));
     auto cleanup = [&](hipError_t e) {
@@ -1337,69 +1398,18 @@ hipError_t matchLearned2(const MatchPlan& p, Mlp* mlp, uint32_t k, const uint64_
             (void)hipEventDestroy(x);
         return e;
     };
-    hipError_t e;
-    if ((e = fillU64((uint64_t*)cnt, 2, 0ull, s)) != hipSuccess)
-        return cleanup(e);
-    if ((e = enqueuePrepareOrder(p, d_masks, d_scratch, s)) != hipSuccess)
-        return cleanup(e);
-    (void)hipEventRecord(ev[0], s);
-    if (n > 0) {
-        // 1. Proposals.
-        if (mlp) {
-            float* E = nullptr;
-            if ((e = hipMallocAsync((void**)&E, (size_t)n * kEmb * 4, s)) != hipSuccess)
-                return cleanup(e);
-            e = mlpEmbed(*mlp, v.sortedMasks, n, E, s);
-            if (e == hipSuccess) {
-                hipLaunchKernelGGL(kCastBf, dim3((unsigned)cdiv((size_t)n * kEmb, 256)), dim3(256), 0, s, E, (size_t)n * kEmb, Ebf);
-                e = hipGetLastError();
-            }
-            (void)hipFreeAsync(E, s);
-            if (e != hipSuccess)
-                return cleanup(e);
-            (void)hipEventRecord(ev[1], s);
-            if ((e = topkPropose(Ebf, n, k, cand, topkWork, s)) != hipSuccess)
-                return cleanup(e);
-        } else {
-            (void)hipEventRecord(ev[1], s);
-            hipLaunchKernelGGL(kNeighbourProposals, dim3((unsigned)cdiv((size_t)n * k, 256)), dim3(256), 0, s, n, k, cand);
-        }
-        if ((e = hipGetLastError()) != hipSuccess)
-            return cleanup(e);
-        (void)hipEventRecord(ev[2], s);
-        // 2. Bitwise verification of every proposal against the 48 precomputed images.
-        hipLaunchKernelGGL(kTransforms48, dim3((unsigned)cdiv((size_t)n * 48, 256)), dim3(256), 0, s, v.sortedMasks, n, images, boxes);
-        hipLaunchKernelGGL(kVerifyProposals, dim3((unsigned)cdiv(n, 4)), dim3(256), 0, s, v.sortedMasks, images, boxes, n, k, cand, cfgBits, v.best, cnt);
-        hipLaunchKernelGGL(kFallbackFlags, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, s, v.sortedMasks, v.best, n, flags, cnt + 1);
-        if ((e = hipGetLastError()) != hipSuccess)
-            return cleanup(e);
-        (void)hipEventRecord(ev[3], s);
-        // 3. Exhaustive search for the queries without a verified proposal.
-        if ((e = enqueueBuildTable(p, d_scratch, s, flags)) != hipSuccess)
-            return cleanup(e);
-        if ((e = enqueueSearch(p, cfgBits, d_scratch, s)) != hipSuccess)
-            return cleanup(e);
-        // 4. Parent chase (pointer jumping along verified parent chains).
-        for (int pass = 0; pass < kChasePasses; ++pass) {
-            if ((e = hipMemcpyAsync(prevBest, v.best, (size_t)n * 8, hipMemcpyDeviceToDevice, s)) != hipSuccess)
-                return cleanup(e);
-            hipLaunchKernelGGL(kChaseParents, dim3((unsigned)cdiv(n, 16)), dim3(256), 0, s, v.sortedMasks, images, boxes, n, cfgBits, prevBest, v.best);
-            if ((e = hipGetLastError()) != hipSuccess)
-                return cleanup(e);
-        }
-        (void)hipEventRecord(ev[4], s);
-    }
-    if ((e = enqueueFinish(p, d_scratch, s, d_encoded, d_mapping, d_unique)) != hipSuccess)
-        return cleanup(e);
-    if ((e = hipStreamSynchronize(s)) != hipSuccess)
+    hipError_t e = enqueueLearned2(p, mlp, k, d_masks, cfgBits, d_scratch, d_work, s, d_encoded, d_mapping, d_unique, ev);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(s);
+    if (e != hipSuccess)
         return cleanup(e);
     if (stats) {
         unsigned long long c[2] = { 0, 0 };
-        if ((e = hipMemcpy(c, cnt, 16, hipMemcpyDeviceToHost)) != hipSuccess)
+        if ((e = hipMemcpy(c, learnedCounters(p, d_work), 16, hipMemcpyDeviceToHost)) != hipSuccess)
             return cleanup(e);
         stats->proposalHits = c[0];
         stats->fallback = c[1];
-        if (n > 0) {
+        if (p.n > 0) {
             (void)hipEventElapsedTime(&stats->msEmbed, ev[0], ev[1]);
             (void)hipEventElapsedTime(&stats->msPropose, ev[1], ev[2]);
             (void)hipEventElapsedTime(&stats->msVerify, ev[2], ev[3]);

@@ -9,6 +9,7 @@
 #include <stdint.h>
 
 #include "matcher.hpp"
+#include "tdag.h"
 
 namespace tdag {
 namespace learn {
@@ -73,6 +74,14 @@ hipError_t matchLearned2(const MatchPlan& plan, Mlp* mlp, uint32_t k, const uint
     uint64_t* d_encoded, uint64_t* d_mapping, uint32_t* d_unique, LearnedStats* stats);
 // Device workspace bytes matchLearned2 needs for n leaves (besides the plan's scratch).
 size_t learnedWorkBytes(uint64_t n);
+// Grows the network's embedding workspace for n leaves (allocates; call before enqueueLearned2).
+hipError_t mlpPrepareEmbed(Mlp& mlp, uint64_t n);
+// matchLearned2 without allocation or synchronisation, so a stream capture can record it: the
+// network's workspace must have been prepared for p.n (mlpPrepareEmbed). ev (optional, 5 events)
+// marks the stage boundaries. The proposal-hit and fallback counters are at learnedCounters().
+hipError_t enqueueLearned2(const MatchPlan& plan, Mlp* mlp, uint32_t k, const uint64_t* d_masks, uint32_t cfgBits, void* d_scratch, void* d_work, hipStream_t stream,
+    uint64_t* d_encoded, uint64_t* d_mapping, uint32_t* d_unique, hipEvent_t* ev);
+const unsigned long long* learnedCounters(const MatchPlan& plan, void* d_work);
 
 // Retrieval of the learned proposal: for each of the n leaves (rank order, bf16 embeddings n x 128
 // row-major) the k <= 64 best earlier leaves under "higher score first, then lower rank", best
@@ -85,3 +94,9 @@ hipError_t gemmBf16(bool ta, bool tb, const uint16_t* A, int lda, const uint16_t
 
 } // namespace learn
 } // namespace tdag
+
+// The C-ABI handle of a network (tdag_mlp in tdag.h): the context it belongs to and its state.
+struct tdag_mlp {
+    tdag_context* ctx = nullptr;
+    tdag::learn::Mlp mlp {};
+};

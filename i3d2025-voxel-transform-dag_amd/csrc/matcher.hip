@@ -191,7 +191,7 @@ __global__ __launch_bounds__(256) void kLayouts(uint64_t* __restrict__ layouts, 
 // for one coordinate along sigma_2 each; the corner's first lane folds pass 3.
 template <int L>
 __global__ __launch_bounds__(64) void kCornerTable(const uint64_t* __restrict__ layouts, uint32_t n, uint64_t* __restrict__ table, uint32_t logCap,
-    uint32_t* __restrict__ bloom, uint32_t logBloom)
+    uint32_t* __restrict__ bloom, uint32_t logBloom, const uint32_t* __restrict__ childFlags)
 {
     using G = Geo<L>;
     using Row = typename G::Row;
@@ -199,6 +199,8 @@ __global__ __launch_bounds__(64) void kCornerTable(const uint64_t* __restrict__ 
     __shared__ Row s_rows[G::Rows];
     const uint32_t r = blockIdx.x;
     const uint32_t q = blockIdx.y;
+    if (childFlags && childFlags[r] == 0u) // only flagged ranks enter the table as children
+        return;
     // sigma = q^-1: pass a reads the child axis b with permAxis(q, b) == a.
     uint32_t sigma[3];
 #pragma unroll
@@ -612,8 +614,6 @@ hipError_t enqueueBuildTable(const MatchPlan& p, void* d_scratch, hipStream_t st
     const uint64_t cap = 1ull << p.logCap;
     if (n == 0)
         return hipSuccess;
-    if (childFlags && p.level != 2)
-        return hipErrorInvalidValue;
     // 3. Child table and its Bloom filter (levels >= 3 first build the transposed bit layouts).
     TDAG_TRY(fillU64(v.tabKeys, cap, 0ull, stream));
     TDAG_TRY(fillU32(v.bloom, 1ull << p.logBloom, 0u, stream));
@@ -624,15 +624,18 @@ hipError_t enqueueBuildTable(const MatchPlan& p, void* d_scratch, hipStream_t st
         break;
     case 3:
         hipLaunchKernelGGL(kLayouts<3>, dim3((unsigned)ceilDiv((size_t)n * 2 * 8, 256)), dim3(256), 0, stream, v.sortedMasks, n);
-        hipLaunchKernelGGL(kCornerTable<3>, dim3(n, p.childFrames), dim3(64), 0, stream, v.sortedMasks, n, v.tabKeys, p.logCap, v.bloom, p.logBloom);
+        hipLaunchKernelGGL(kCornerTable<3>, dim3(n, p.childFrames), dim3(64), 0, stream, v.sortedMasks, n, v.tabKeys, p.logCap, v.bloom, p.logBloom,
+            childFlags);
         break;
     case 4:
         hipLaunchKernelGGL(kLayouts<4>, dim3((unsigned)ceilDiv((size_t)n * 2 * 16, 256)), dim3(256), 0, stream, v.sortedMasks, n);
-        hipLaunchKernelGGL(kCornerTable<4>, dim3(n, p.childFrames), dim3(64), 0, stream, v.sortedMasks, n, v.tabKeys, p.logCap, v.bloom, p.logBloom);
+        hipLaunchKernelGGL(kCornerTable<4>, dim3(n, p.childFrames), dim3(64), 0, stream, v.sortedMasks, n, v.tabKeys, p.logCap, v.bloom, p.logBloom,
+            childFlags);
         break;
     case 5:
         hipLaunchKernelGGL(kLayouts<5>, dim3((unsigned)ceilDiv((size_t)n * 2 * 32, 256)), dim3(256), 0, stream, v.sortedMasks, n);
-        hipLaunchKernelGGL(kCornerTable<5>, dim3(n, p.childFrames), dim3(64), 0, stream, v.sortedMasks, n, v.tabKeys, p.logCap, v.bloom, p.logBloom);
+        hipLaunchKernelGGL(kCornerTable<5>, dim3(n, p.childFrames), dim3(64), 0, stream, v.sortedMasks, n, v.tabKeys, p.logCap, v.bloom, p.logBloom,
+            childFlags);
         break;
     default:
         return hipErrorInvalidValue;

@@ -69,7 +69,7 @@ hipError_t enqueueMatch(const MatchPlan& plan, const uint64_t* d_masks, uint32_t
 // parents, unique flags and the mapping.
 hipError_t enqueuePrepare(const MatchPlan& plan, const uint64_t* d_masks, void* d_scratch, hipStream_t stream);
 // Prepare in two parts: rank order, sorted masks and best[] seeds; then the child table. With
-// childFlags (level 2 only) only ranks whose flag is non-zero enter the table.
+// childFlags (indexed by rank) only ranks whose flag is non-zero enter the table.
 hipError_t enqueuePrepareOrder(const MatchPlan& plan, const uint64_t* d_masks, void* d_scratch, hipStream_t stream);
 hipError_t enqueueBuildTable(const MatchPlan& plan, void* d_scratch, hipStream_t stream, const uint32_t* childFlags);
 hipError_t enqueueSearch(const MatchPlan& plan, uint32_t cfgBits, void* d_scratch, hipStream_t stream);

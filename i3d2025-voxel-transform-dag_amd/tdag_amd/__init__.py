@@ -121,6 +121,10 @@ def lib() -> ctypes.CDLL:
         ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(TdagConfig),
         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
     ]
+    L.tdag_match_level_subset_device.argtypes = [
+        ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(TdagConfig), ctypes.c_void_p, ctypes.c_void_p,
+        ctypes.POINTER(TdagMatchStats),
+    ]
     L.tdag_last_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(TdagMatchStats)]
     L.tdag_context_set_variant.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int32]
     L.tdag_context_set_counters.argtypes = [ctypes.c_void_p, ctypes.c_int]
@@ -140,9 +144,13 @@ def lib() -> ctypes.CDLL:
     L.tdag_batch_autotune.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
     L.tdag_match_levels_batched_device.argtypes = [ctypes.c_void_p, ctypes.POINTER(TdagLevelJob), ctypes.c_uint32, ctypes.POINTER(TdagConfig),
                                                    ctypes.POINTER(TdagMatchStats)]
-    for name in ("tdag_batch_create", "tdag_batch_run", "tdag_batch_destroy", "tdag_batch_autotune", "tdag_match_levels_batched_device"):
+    L.tdag_batch_create_learned.argtypes = [ctypes.c_void_p, ctypes.POINTER(TdagLevelJob), ctypes.c_uint32, ctypes.POINTER(TdagConfig), ctypes.c_uint32,
+                                            ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)]
+    L.tdag_batch_learned_stats.argtypes = [ctypes.c_void_p, _u64p, _u64p]
+    for name in ("tdag_batch_create", "tdag_batch_run", "tdag_batch_destroy", "tdag_batch_autotune", "tdag_match_levels_batched_device",
+                 "tdag_batch_create_learned", "tdag_batch_learned_stats"):
         getattr(L, name).restype = ctypes.c_int
-    for name in ("tdag_context_create", "tdag_context_destroy", "tdag_match_level_device", "tdag_match_level_device_noalloc", "tdag_match_level", "tdag_last_stats",
+    for name in ("tdag_context_create", "tdag_context_destroy", "tdag_match_level_device", "tdag_match_level_device_noalloc", "tdag_match_level_subset_device", "tdag_match_level", "tdag_last_stats",
                  "tdag_context_set_variant", "tdag_context_get_variant", "tdag_autotune_level", "tdag_context_set_counters",
                  "tdag_context_set_batching", "tdag_context_get_batching", "tdag_context_set_filter_bits", "tdag_context_get_filter_bits",
                  "tdag_context_set_child_frames", "tdag_context_get_child_frames"):
@@ -197,15 +205,27 @@ class Batch:
     GRAPH = 1  # TDAG_BATCH_GRAPH
     CONCURRENT = 2  # TDAG_BATCH_CONCURRENT
 
-    def __init__(self, ctx: "Context", jobs, config, use_graph: bool, concurrent: bool = False):
+    def __init__(self, ctx: "Context", jobs, config, use_graph: bool, concurrent: bool = False, mlp=None, k: int = 64):
         self._ptr = ctypes.c_void_p()
         self._ctx = ctx
         self._jobs, self._keep = _jobs_array(jobs)
         self.count = len(jobs)
         self.concurrent = bool(concurrent)
+        self.learned = mlp is not None
         cfg = (config or TransformDAGConfig()).to_c()
         flags = (self.GRAPH if use_graph else 0) | (self.CONCURRENT if concurrent else 0)
-        check(lib().tdag_batch_create(ctx._ptr, self._jobs, self.count, ctypes.byref(cfg), flags, ctypes.byref(self._ptr)), "tdag_batch_create")
+        if mlp is None:
+            check(lib().tdag_batch_create(ctx._ptr, self._jobs, self.count, ctypes.byref(cfg), flags, ctypes.byref(self._ptr)), "tdag_batch_create")
+        else:
+            self._mlp = mlp  # keeps the network alive as long as the batch
+            check(lib().tdag_batch_create_learned(ctx._ptr, self._jobs, self.count, ctypes.byref(cfg), flags, mlp._ptr, int(k), ctypes.byref(self._ptr)),
+                  "tdag_batch_create_learned")
+
+    def learned_stats(self):
+        """(proposal hits, fallback queries) of the learned level-2 job in the last run."""
+        hits, fb = ctypes.c_uint64(), ctypes.c_uint64()
+        check(lib().tdag_batch_learned_stats(self._ptr, ctypes.byref(hits), ctypes.byref(fb)), "tdag_batch_learned_stats")
+        return int(hits.value), int(fb.value)
 
     def run(self):
         """Runs the call and waits; returns (per-job stats, milliseconds of the whole call)."""
@@ -302,14 +322,25 @@ class Context:
             self._ptr, int(level), _dev_ptr(d_masks), int(n), ctypes.byref(cfg), _dev_ptr(d_encoded), _dev_ptr(d_mapping), _dev_ptr(d_unique),
             _dev_ptr(d_counts), s), "tdag_match_level_device_noalloc")
 
+    def match_level_subset_device(self, level: int, d_masks, n: int, d_child_flags, config: TransformDAGConfig | None = None, d_encoded=None):
+        """The exhaustive search with only the flagged children (uint32 device tensor, original
+        order) in the table (tdag_match_level_subset_device); returns the stats."""
+        cfg = (config or TransformDAGConfig()).to_c()
+        st = TdagMatchStats()
+        check(lib().tdag_match_level_subset_device(self._ptr, int(level), _dev_ptr(d_masks), int(n), ctypes.byref(cfg), _dev_ptr(d_child_flags),
+                                                   _dev_ptr(d_encoded), ctypes.byref(st)), "tdag_match_level_subset_device")
+        return st
+
     # -- batched form ------------------------------------------------------------------------
-    def batch(self, jobs, config: TransformDAGConfig | None = None, use_graph: bool = True, concurrent: bool = False) -> "Batch":
+    def batch(self, jobs, config: TransformDAGConfig | None = None, use_graph: bool = True, concurrent: bool = False, mlp=None, k: int = 64) -> "Batch":
         """Batched matcher over several levels (one job per level). By default every level's
         parent search runs in one persistent kernel; with concurrent=True each level runs its own
         tuned matcher on its own stream and the levels overlap. With use_graph the whole call is
         captured once as a hipGraph. `jobs` holds dicts with keys level, n, d_masks and optional
-        d_encoded, d_mapping, d_unique (torch tensors or raw device pointers)."""
-        return Batch(self, jobs, config, use_graph, concurrent)
+        d_encoded, d_mapping, d_unique (torch tensors or raw device pointers). With `mlp` (a
+        learned.MLP of this context) the level-2 job runs the learned proposal matcher (k
+        proposals per query, exhaustive fallback) inside the same call."""
+        return Batch(self, jobs, config, use_graph, concurrent, mlp, k)
 
     def match_levels_batched_device(self, jobs, config: TransformDAGConfig | None = None):
         """One-shot batched call (no graph); returns per-job stats."""

@@ -166,3 +166,27 @@ def pairs_from_encoded(masks: np.ndarray, encoded: np.ndarray):
     q = m[sel]
     p = m[order[ptr[sel]]]
     return q, p
+
+
+def training_pairs(ctx, config: TransformDAGConfig, kind: int, resolution: int, seeds):
+    """Training data of the network: the (query, parent) 4^3 pairs the exact GPU matcher emits at
+    level 2 of constructTransformDAG on seeded scenes. Returns (d_q, d_p, n) device tensors."""
+    import torch
+
+    from . import pipeline as pl
+
+    qs, ps = [], []
+    for seed in seeds:
+        bits = pl.scene_dense(kind, resolution, seed)
+        t = pl.construct_transform_dag(ctx, pl.Svdag(bits, resolution), config, keep_level_inputs=True)
+        m2 = np.ascontiguousarray(t.level_masks(2).reshape(-1))
+        t.close()
+        d_m = torch.from_numpy(m2.view(np.int64)).cuda()
+        d_enc = torch.empty(max(1, len(m2)), dtype=torch.int64, device="cuda")
+        torch.cuda.synchronize()
+        ctx.match_level_device(2, d_m, len(m2), config, d_encoded=d_enc)
+        q, p = pairs_from_encoded(m2, d_enc.cpu().numpy().view(np.uint64)[: len(m2)])
+        qs.append(q)
+        ps.append(p)
+    q, p = np.concatenate(qs), np.concatenate(ps)
+    return torch.from_numpy(q.view(np.int64).copy()).cuda(), torch.from_numpy(p.view(np.int64).copy()).cuda(), len(q)

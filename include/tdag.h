@@ -121,6 +121,18 @@ tdag_status tdag_match_level(tdag_context* ctx, uint32_t level, const uint64_t* 
 tdag_status tdag_match_level_device_noalloc(tdag_context* ctx, uint32_t level, const uint64_t* d_masks, uint64_t n,
     const tdag_config* cfg, uint64_t* d_encoded, uint64_t* d_mapping, uint32_t* d_unique, uint64_t* d_counts, void* stream);
 
+/*
+ * The exhaustive search restricted to a subset of children: only items whose flag is non-zero
+ * (d_child_flags: n uint32 in original order, device) are looked up; the others keep their
+ * identity key and are reported unmatched. This is the fallback stage of a proposal scheme
+ * (queries without a verified proposal are searched exhaustively); with the flags set to the
+ * items that have no match at all it gives the least time any proposal scheme with an
+ * exhaustive fallback can reach at this level. Same parents, scratch and tuning as
+ * tdag_match_level_device; d_encoded (optional) as there. Synchronises. No reference counterpart.
+ */
+tdag_status tdag_match_level_subset_device(tdag_context* ctx, uint32_t level, const uint64_t* d_masks, uint64_t n, const tdag_config* cfg,
+    const uint32_t* d_child_flags, uint64_t* d_encoded, tdag_match_stats* stats);
+
 /* Statistics of the last matcher call on ctx. */
 tdag_status tdag_last_stats(const tdag_context* ctx, tdag_match_stats* out);
 
@@ -199,6 +211,8 @@ typedef struct tdag_level_job {
 } tdag_level_job;
 
 typedef struct tdag_batch tdag_batch;
+/* A network of the learned proposal (below). */
+typedef struct tdag_mlp tdag_mlp;
 
 /* Flags of tdag_batch_create (its `use_graph` argument; 0 and 1 keep their old meaning).
  * TDAG_BATCH_GRAPH: capture the whole call into a hipGraph and replay it on every run.
@@ -223,6 +237,18 @@ tdag_status tdag_batch_run(tdag_batch* b, tdag_match_stats* out_stats, float* ou
  * (it reads the descriptor at run time). A concurrent batch has no queue: *out_grab = 0. */
 tdag_status tdag_batch_autotune(tdag_batch* b, uint32_t repeats, uint32_t* out_grab);
 tdag_status tdag_batch_destroy(tdag_batch* b);
+/* The batched form with the learned proposal at level 2 (the combined form): the batch's level-2
+ * job runs the learned matcher (network `mlp` of the same context, k <= 64 proposals per query,
+ * bitwise verification, exhaustive fallback for the queries without a verified proposal, parent
+ * chase), the other jobs run as in tdag_batch_create with the same flags. Persistent mode: the
+ * learned matcher runs on its own graph branch beside the persistent kernel; concurrent mode: on
+ * the level-2 branch. With TDAG_BATCH_GRAPH the whole call, learned stage included, is one
+ * captured hipGraph. Every accepted match is exact; the level-2 parents can differ from the
+ * exhaustive minimum where it was not proposed. No reference counterpart. */
+tdag_status tdag_batch_create_learned(tdag_context* ctx, const tdag_level_job* jobs, uint32_t count, const tdag_config* cfg, uint32_t flags, tdag_mlp* mlp, uint32_t k,
+    tdag_batch** out);
+/* Proposal hits and fallback queries of the learned job in the last tdag_batch_run. */
+tdag_status tdag_batch_learned_stats(const tdag_batch* b, uint64_t* out_proposal_hits, uint64_t* out_fallback_queries);
 /* One-shot batched call without a graph: create, run, destroy. */
 tdag_status tdag_match_levels_batched_device(tdag_context* ctx, const tdag_level_job* jobs, uint32_t count, const tdag_config* cfg, tdag_match_stats* out_stats);
 
@@ -239,7 +265,6 @@ tdag_status tdag_match_levels_batched_device(tdag_context* ctx, const tdag_level
  * chosen parent can differ from the exhaustive minimum when the minimum is not proposed. No
  * reference counterpart.
  * ---------------------------------------------------------------------------------------- */
-typedef struct tdag_mlp tdag_mlp;
 
 typedef struct tdag_learned_stats {
     uint64_t num_items;

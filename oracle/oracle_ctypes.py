@@ -83,6 +83,8 @@ def _bind_svdag(L):
     L.orc_construct_tdag.restype = ctypes.c_int64
     L.orc_last_level_inputs.argtypes = [ctypes.c_uint32, _u64p, ctypes.c_uint64]
     L.orc_last_level_inputs.restype = ctypes.c_int64
+    L.orc_last_level_encoded.argtypes = [ctypes.c_uint32, _u64p, ctypes.c_uint64]
+    L.orc_last_level_encoded.restype = ctypes.c_int64
     L.orc_svdag_static_level.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64]
     L.orc_svdag_static_level.restype = ctypes.c_int64
     L.orc_construct_static_tdag.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrcConfig), ctypes.c_int, ctypes.c_char_p]

@@ -536,6 +536,7 @@ static std::array<uint64_t, 8> toTransformNode(const std::array<uint32_t, 8>& n)
 // Matcher inputs recorded per level by the last orc_construct_tdag call (for tests that
 // drive the GPU matcher with exactly the driver's inputs).
 static std::vector<std::vector<uint64_t>> g_levelInputs(8);
+static std::vector<std::vector<uint64_t>> g_levelEncoded(8); // encodedParents of each level's matcher call
 
 // constructTransformDAG (transform_dag.cpp:138-240), with the oracle matcher.
 static TDag construct(const Svdag& in, const orc_config& cfg, int nthreads)
@@ -546,6 +547,8 @@ static TDag construct(const Svdag& in, const orc_config& cfg, int nthreads)
     const uint32_t maxT = cfg.maxTranslationLevel;
     TDag out = hierarchical(in, maxT + 1, translationParents, cfg);
     for (auto& v : g_levelInputs)
+        v.clear();
+    for (auto& v : g_levelEncoded)
         v.clear();
     if (!cfg.translation)
         return out;
@@ -578,6 +581,7 @@ static TDag construct(const Svdag& in, const orc_config& cfg, int nthreads)
         std::vector<uint32_t> uniq(indices.size());
         uint64_t nu = 0;
         orc_match_encoded(level, masks.data(), indices.size(), &cfg, enc.data(), nthreads, 0);
+        g_levelEncoded[level] = enc;
         orc_postprocess(level, masks.data(), indices.size(), enc.data(), mapping.data(), uniq.data(), &nu);
         if (level == 2) {
             out.subGrids.resize(nu);
@@ -1051,6 +1055,18 @@ int64_t orc_last_level_inputs(uint32_t level, uint64_t* out, uint64_t cap)
     if (level >= g_levelInputs.size())
         return -1;
     const auto& v = g_levelInputs[level];
+    if (out && cap >= v.size())
+        std::memcpy(out, v.data(), v.size() * 8);
+    return (int64_t)v.size();
+}
+
+// Copies the encodedParents (minimum keys) of the level-`level` matcher call of the last
+// orc_construct_tdag call.
+int64_t orc_last_level_encoded(uint32_t level, uint64_t* out, uint64_t cap)
+{
+    if (level >= g_levelEncoded.size())
+        return -1;
+    const auto& v = g_levelEncoded[level];
     if (out && cap >= v.size())
         std::memcpy(out, v.data(), v.size() * 8);
     return (int64_t)v.size();

@@ -182,3 +182,45 @@ def test_batched_rejects_duplicate_levels(gpu_ctx):
     jobs = [{"level": 2, "n": 4, "d_masks": m}, {"level": 2, "n": 4, "d_masks": m}]
     with pytest.raises(tdag_amd.TdagError, match="one job per level"):
         gpu_ctx.match_levels_batched_device(jobs, tdag_amd.TransformDAGConfig())
+
+
+@pytest.mark.parametrize("concurrent", [False, True])
+def test_combined_batch_with_learned_level2(oracle, gpu_ctx, concurrent):
+    """The combined form (tdag_batch_create_learned): levels 3..5 equal the oracle, level 2 equals
+    the standalone learned matcher (same network, bit for bit), over several graph replays with
+    other GPU work in between."""
+    import torch
+
+    import tdag_amd
+    from tdag_amd import learned
+
+    level_masks = _level_sets(oracle, 31)
+    jobs, bufs = _device_jobs(level_masks)
+    mlp = learned.MLP(gpu_ctx, seed=5)
+    m2 = level_masks[2]
+    n2 = len(m2)
+    cfg = tdag_amd.TransformDAGConfig(True, True, True, 5)
+    ref2 = torch.zeros(max(1, n2), dtype=torch.int64, device="cuda")
+    learned.match_level_learned_device(gpu_ctx, mlp, bufs[2]["masks"], n2, cfg, 64, d_encoded=ref2)
+    torch.cuda.synchronize()
+    ref2 = ref2.cpu().numpy().view(np.uint64)[:n2].copy()
+    exact2 = oracle.match_encoded(2, m2, oracle.config(True, True, True))
+    print(f"combined batch concurrent={concurrent}", flush=True)
+    b = gpu_ctx.batch(jobs, cfg, use_graph=True, concurrent=concurrent, mlp=mlp, k=64)
+    try:
+        for replay in range(3):
+            for d in bufs.values():
+                d["enc"].zero_()
+            torch.cuda.synchronize()
+            stats, _ = b.run()
+            hits, fb = b.learned_stats()
+            assert hits + fb <= n2
+            rest = {lv: m for lv, m in level_masks.items() if lv != 2}
+            _check_outputs(oracle, rest, bufs, [s for lv, s in zip(level_masks, stats) if lv != 2])
+            got2 = bufs[2]["enc"].cpu().numpy().view(np.uint64)[:n2]
+            assert np.array_equal(got2, ref2), f"replay {replay}: level 2 differs from the standalone learned matcher"
+            assert np.all(got2 >= exact2)  # never below the exhaustive minimum
+            _ = torch.ones(1 << 20, device="cuda").sum().item()  # other GPU work between replays
+    finally:
+        b.close()
+        mlp.close()

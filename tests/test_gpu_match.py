@@ -166,3 +166,40 @@ def test_small_filters(oracle, gpu_ctx, level, bits):
         _check(oracle, gpu_ctx, level, masks)
     finally:
         gpu_ctx.set_filter_bits(level, 0)
+
+
+@pytest.mark.parametrize("level,n", [(2, 400), (3, 150), (5, 16)])
+def test_subset_search_keeps_the_flagged_minima(oracle, gpu_ctx, level, n):
+    """tdag_match_level_subset_device: flagged children get exactly the full search's minimum key,
+    the others keep their identity key (empty ones the sentinel)."""
+    import torch
+
+    import tdag_amd
+
+    rng = np.random.default_rng(90 + level)
+    masks = sets.planted_set(rng, level, n // 2, n - n // 2, oracle.transform)
+    m = len(masks)
+    full = oracle.match_encoded(level, masks, oracle.config(True, True, True))
+    flags = (rng.random(m) < 0.4).astype(np.int32)
+    d_m = torch.from_numpy(np.ascontiguousarray(masks).view(np.int64).reshape(-1).copy()).cuda()
+    d_f = torch.from_numpy(flags).cuda()
+    d_e = torch.zeros(max(1, m), dtype=torch.int64, device="cuda")
+    torch.cuda.synchronize()
+    print(f"subset level={level} n={m}", flush=True)
+    gpu_ctx.match_level_subset_device(level, d_m, m, d_f, tdag_amd.TransformDAGConfig(True, True, True), d_encoded=d_e)
+    got = d_e.cpu().numpy().view(np.uint64)[:m]
+    # identity key of the item's own rank (encodeFixed64 of TransformPointer::create(rank)); empty items the sentinel's
+    pop = np.bitwise_count(np.ascontiguousarray(masks, dtype=np.uint64).reshape(m, -1)).sum(axis=1).astype(np.int64)
+    rank = np.empty(m, dtype=np.int64)
+    rank[np.argsort(-pop, kind="stable")] = np.arange(m)
+    b, bias = level + 1, 1 << level
+
+    def ident(ptr):
+        k = ptr << 6
+        for _ in range(3):
+            k = (k << b) | bias
+        return k
+
+    expect = np.array([ident(0xFFFFFFFF) if pop[i] == 0 else ident(int(rank[i])) for i in range(m)], dtype=np.uint64)
+    assert np.array_equal(got[flags == 1], full[flags == 1])
+    assert np.array_equal(got[flags == 0], expect[flags == 0])

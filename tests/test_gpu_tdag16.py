@@ -30,7 +30,8 @@ def _device_words(e, gpu_ctx):
     return d_out.cpu().numpy().view(np.uint8), ms
 
 
-@pytest.mark.parametrize("kind,R,seed,maxT", [(0, 128, 2, 3), (1, 128, 5, 4), (0, 256, 0, 5), (1, 256, 11, 4)])
+@pytest.mark.parametrize("kind,R,seed,maxT", [(0, 16, 1, 3), (1, 16, 4, 2), (0, 32, 3, 4), (1, 32, 6, 3), (0, 128, 2, 3), (1, 128, 5, 4), (0, 256, 0, 5),
+                                               (1, 256, 11, 4)])
 def test_encode_decode_with_translation(oracle, pl, gpu_ctx, kind, R, seed, maxT):
     import tdag_amd
 
@@ -51,11 +52,12 @@ def test_encode_decode_with_translation(oracle, pl, gpu_ctx, kind, R, seed, maxT
 
 
 def test_decode_1024_round_trip(pl, gpu_ctx):
+    """The 1024^3 sphere+box+heightfield scene of the plain-matcher configuration (kind 1, seed 0)."""
     import tdag_amd
 
-    print("tdag16 decode 1024^3 scene 0 seed 0", flush=True)
+    print("tdag16 decode 1024^3 scene 1 seed 0", flush=True)
     R = 1024
-    bits = pl.scene_dense(0, R, 0)
+    bits = pl.scene_dense(1, R, 0)
     cfg = tdag_amd.TransformDAGConfig(True, True, True, 5)
     t = pl.construct_transform_dag(gpu_ctx, pl.Svdag(bits, R), cfg)
     e = t.encode16()
