@@ -104,8 +104,6 @@ struct tdag_batch {
     int learned = -1; // job index, -1: none
     tdag::MatchPlan lplan {};
     size_t lOff = 0, lWorkOff = 0; // its scratch and proposal workspace inside `scratch`
-    hipStream_t lStream = nullptr;
-    hipEvent_t lFork = nullptr, lJoin = nullptr;
 };
 
 namespace tdag {
@@ -586,14 +584,14 @@ tdag_status enqueueBatchCall(tdag_batch* b)
     if (b->concurrent)
         return enqueueConcurrentCall(b);
     hipStream_t main = b->ctx->stream;
-    if (b->learned >= 0) { // the learned level-2 matcher on its own branch beside the persistent kernel
+    if (b->learned >= 0) {
+        // The learned level-2 matcher runs on the call's stream ahead of the persistent kernel,
+        // never beside it: the persistent grid is sized to the blocks the GPU holds at once, and
+        // its grid barriers wait for all of them, so no other kernel may take CUs from it.
         const tdag_level_job& j = b->jobs[b->learned];
-        CAPI_HIP(hipEventRecord(b->lFork, main), "fork event");
-        CAPI_HIP(hipStreamWaitEvent(b->lStream, b->lFork, 0), "fork wait");
         CAPI_HIP(tdag::learn::enqueueLearned2(b->lplan, &b->mlp->mlp, b->k, j.d_masks, b->cfgBits & ~tdag::kCfgCount, (char*)b->scratch + b->lOff,
-                     (char*)b->scratch + b->lWorkOff, b->lStream, j.d_encoded, j.d_mapping, j.d_unique, nullptr),
+                     (char*)b->scratch + b->lWorkOff, main, j.d_encoded, j.d_mapping, j.d_unique, nullptr),
             "learned matcher launch");
-        CAPI_HIP(hipEventRecord(b->lJoin, b->lStream), "join event");
     }
     const uint32_t k = b->plan.count;
     if (k > 0) {
@@ -614,8 +612,6 @@ tdag_status enqueueBatchCall(tdag_batch* b)
         CAPI_HIP(tdag::enqueueBatch(b->plan, masks.data(), b->scratch, main, enc.data(), map.data(), uniq.data(), nullptr, nullptr, b->spinTicks),
             "batched matcher launch");
     }
-    if (b->learned >= 0)
-        CAPI_HIP(hipStreamWaitEvent(main, b->lJoin, 0), "join wait");
     return TDAG_OK;
 }
 
@@ -738,12 +734,6 @@ static tdag_status createBatch(tdag_context* ctx, const tdag_level_job* jobs, ui
             total += (b->lplan.scratchBytes + 255) & ~size_t(255);
             b->lWorkOff = total;
             total += (tdag::learn::learnedWorkBytes(jobs[learned].n) + 255) & ~size_t(255);
-            if (hipError_t e = hipStreamCreateWithFlags(&b->lStream, hipStreamNonBlocking); e != hipSuccess) // default priority
-                return cleanup(failHip(e, "batch stream creation"));
-            if (hipError_t e = hipEventCreateWithFlags(&b->lFork, hipEventDisableTiming); e != hipSuccess)
-                return cleanup(failHip(e, "batch event creation"));
-            if (hipError_t e = hipEventCreateWithFlags(&b->lJoin, hipEventDisableTiming); e != hipSuccess)
-                return cleanup(failHip(e, "batch event creation"));
         }
         if (hipError_t e = hipMalloc(&b->scratch, std::max<size_t>(total, 256)); e != hipSuccess)
             return cleanup(failHip(e, "batched matcher scratch allocation"));
@@ -928,12 +918,6 @@ tdag_status tdag_batch_destroy(tdag_batch* b)
             (void)hipEventDestroy(ev);
     if (b->evFork)
         (void)hipEventDestroy(b->evFork);
-    if (b->lStream)
-        (void)hipStreamDestroy(b->lStream);
-    if (b->lFork)
-        (void)hipEventDestroy(b->lFork);
-    if (b->lJoin)
-        (void)hipEventDestroy(b->lJoin);
     delete b;
     return TDAG_OK;
 }

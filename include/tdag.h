@@ -241,8 +241,8 @@ tdag_status tdag_batch_destroy(tdag_batch* b);
  * job runs the learned matcher (network `mlp` of the same context, k <= 64 proposals per query,
  * bitwise verification, exhaustive fallback for the queries without a verified proposal, parent
  * chase), the other jobs run as in tdag_batch_create with the same flags. Persistent mode: the
- * learned matcher runs on its own graph branch beside the persistent kernel; concurrent mode: on
- * the level-2 branch. With TDAG_BATCH_GRAPH the whole call, learned stage included, is one
+ * learned matcher runs ahead of the persistent kernel on the call's stream (never beside it: the
+ * persistent grid needs every CU for its grid barriers); concurrent mode: on the level-2 branch. With TDAG_BATCH_GRAPH the whole call, learned stage included, is one
  * captured hipGraph. Every accepted match is exact; the level-2 parents can differ from the
  * exhaustive minimum where it was not proposed. No reference counterpart. */
 tdag_status tdag_batch_create_learned(tdag_context* ctx, const tdag_level_job* jobs, uint32_t count, const tdag_config* cfg, uint32_t flags, tdag_mlp* mlp, uint32_t k,
