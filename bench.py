@@ -133,9 +133,12 @@ def encode16_and_decode(ctx, tdag, bits, resolution: int):
     ms = [e.decode_device(ctx, d_out)[1] for _ in range(3)]
     expect = torch.from_numpy(np.ascontiguousarray(bits).view(np.int32)).to(d_out.device)
     equal = bool(torch.equal(d_out, expect))
+    del d_out, expect
+    trace = trace16(ctx, e, resolution)
     e.close()
     avg = sum(ms) / len(ms)
     return {
+        "trace": trace,
         "bytes": len(data),
         "bytes_without_tables_or_huffman": bytes_plain,
         "encode_ms_host": round(encode_ms, 1),
@@ -145,6 +148,48 @@ def encode16_and_decode(ctx, tdag, bits, resolution: int):
         "note": "decode = the reference's per-voxel lookup (transform_dag_encoding.cpp:897-942) for all "
                 "resolution^3 voxels on the device, kernel time from hipEvents",
     }
+
+
+def camera_rays(resolution: int, width: int = 1024, height: int = 1024):
+    """Primary rays of a pinhole camera (60 degree field of view) above a corner of the volume,
+    looking at a point near the middle of its ground (y is up): n x 8 float32 (origin, tmin,
+    direction, tmax)."""
+    import numpy as np
+
+    R = float(resolution)
+    eye = np.array([-0.1 * R, 0.8 * R, -0.1 * R])
+    look = np.array([0.5 * R, 0.05 * R, 0.5 * R])
+    fwd = look - eye
+    fwd /= np.linalg.norm(fwd)
+    right = np.cross(fwd, [0.0, 1.0, 0.0])
+    right /= np.linalg.norm(right)
+    up = np.cross(right, fwd)
+    th = np.tan(np.radians(30.0))
+    u = (np.arange(width) + 0.5) / width * 2 - 1
+    v = (np.arange(height) + 0.5) / height * 2 - 1
+    uu, vv = np.meshgrid(u * th, v * th * height / width)
+    d = fwd[None, None, :] + uu[..., None] * right[None, None, :] + vv[..., None] * up[None, None, :]
+    d /= np.linalg.norm(d, axis=2, keepdims=True)
+    rays = np.zeros((width * height, 8), dtype=np.float32)
+    rays[:, 0:3] = eye
+    rays[:, 4:7] = d.reshape(-1, 3)
+    rays[:, 7] = 1e9
+    return rays
+
+
+def trace16(ctx, e, resolution: int):
+    """First-hit ray casting through the encoded DAG on the device (the renderer's use of the
+    format): 1024 x 1024 camera rays, kernel time from hipEvents, best of three."""
+    import torch
+
+    rays = camera_rays(resolution)
+    d_rays = torch.from_numpy(rays).cuda()
+    hits, ts, _ = e.trace_device(ctx, d_rays, len(rays))  # warm-up
+    ms = min(e.trace_device(ctx, d_rays, len(rays), hits, ts)[2] for _ in range(3))
+    h = hits.cpu().numpy()
+    return {"rays": len(rays), "ms": round(ms, 3), "Mrays_per_s": round(len(rays) / (ms * 1e-3) / 1e6, 1) if ms > 0 else None,
+            "hit_fraction": round(float((h[:, 0] >= 0).mean()), 4), "mean_steps": round(float(h[:, 3].mean()), 1),
+            "camera": "1024 x 1024 pinhole, 60 degrees, from above a corner toward the middle of the ground"}
 
 
 def device_svdag_build(ctx, bits, resolution: int, host_svdag, host_build_ms: float):

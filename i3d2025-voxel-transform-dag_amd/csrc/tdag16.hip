@@ -497,6 +497,108 @@ std::vector<uint8_t> transformTables()
 
 __device__ __forceinline__ void flagBad(const Dev16& d) { atomicOr(d.bad, 1u); }
 
+// Child pointer of stored child `c` of node `node` at `level` (transform_dag_encoding.cpp:777-811,
+// 138-216): 0 for an empty child; 1 with the child's index, transform id and translation (in the
+// child's own frame) set; 2 for malformed data (flagged).
+__device__ __forceinline__ int readChild(const Dev16& d, uint32_t level, uint64_t node, uint32_t c, uint64_t* child, uint32_t* outXid, int t[3])
+{
+    const uint32_t cl = level - 1;
+    const uint64_t size = d.levelSize[level];
+    const uint16_t* w = d.words + d.levelOff[level];
+    if (node >= size) {
+        flagBad(d);
+        return 2;
+    }
+    const uint32_t mask = w[node];
+    const uint32_t sz = (mask >> (2 * c)) & 3u;
+    if (!sz)
+        return 0;
+    // Words before child c: the sum of the lower 2-bit size fields.
+    const uint32_t below = mask & ((1u << (2 * c)) - 1u);
+    const uint64_t at = node + 1 + __popc(below & 0x5555u) + 2 * __popc(below & 0xAAAAu);
+    if (at + sz > size) {
+        flagBad(d);
+        return 2;
+    }
+    uint64_t e = 0;
+    for (uint32_t i = 0; i < sz; ++i)
+        e |= (uint64_t)w[at + i] << (16 * i);
+    if (d.useTables && (sz == 1 || (e & 1ull))) {
+        const uint64_t idx = sz == 1 ? e : e >> 1;
+        if (idx >= d.tableSize[cl]) {
+            flagBad(d);
+            return 2;
+        }
+        e = d.tables[d.tableOff[cl] + idx];
+    } else if (d.useTables) {
+        e >>= 1;
+    }
+    const bool hasT = d.trans && cl <= d.maxT;
+    const uint32_t tb = cl + 1;
+    const uint64_t tm = (1ull << tb) - 1ull;
+    const int bias = 1 << cl;
+    t[0] = t[1] = t[2] = 0;
+    uint32_t xid = 0;
+    if (d.useHuffman) {
+        if (!((d.hasLut >> cl) & 1u)) {
+            flagBad(d);
+            return 2;
+        }
+        const uint32_t ent = d.lut[d.lutOff[cl] + ((uint32_t)e & d.lutMask[cl])];
+        uint32_t type = ent & 0xFFFFu;
+        e >>= ent >> 16;
+        uint32_t axes = 0;
+        if (hasT) {
+            axes = type & 7u;
+            type >>= 3;
+        }
+        for (int a = 2; a >= 0; --a)
+            if (axes & (1u << a)) {
+                t[a] = (int)(e & tm) - bias;
+                e >>= tb;
+            }
+        if (d.sym) {
+            xid |= type & 7u;
+            type >>= 3;
+        }
+        if (d.perm)
+            xid |= (type & 7u) << 3;
+    } else if (d.sym || d.perm || d.trans) {
+        const bool tr = e & 1ull;
+        e >>= 1;
+        if (tr) {
+            if (hasT)
+                for (int a = 0; a < 3; ++a) {
+                    t[a] = (int)(e & tm) - bias;
+                    e >>= tb;
+                }
+            if (d.sym) {
+                xid |= (uint32_t)e & 7u;
+                e >>= 3;
+            }
+            if (d.perm) {
+                xid |= ((uint32_t)e & 7u) << 3;
+                e >>= 3;
+            }
+        }
+    }
+    if (xid >= 48) {
+        flagBad(d);
+        return 2;
+    }
+    *child = (uint32_t)e;
+    *outXid = xid;
+    return 1;
+}
+
+// The translation of a pointer in the frame of the transform `tid` accumulated above it.
+__device__ __forceinline__ int frameTranslation(const uint8_t* xf, uint32_t tid, const int t[3], int a)
+{
+    const uint32_t src = xf[kXfPerm + tid * 3 + a];
+    const int ta = src == 0 ? t[0] : (src == 1 ? t[1] : t[2]);
+    return ((tid >> a) & 1u) ? -ta : ta;
+}
+
 // Value of one voxel (transform_dag_encoding.cpp:897-942).
 __device__ uint32_t voxelValue(const Dev16& d, const uint8_t* xf, int vx, int vy, int vz)
 {
@@ -505,102 +607,20 @@ __device__ uint32_t voxelValue(const Dev16& d, const uint8_t* xf, int vx, int vy
     uint64_t node = 0;
     for (uint32_t level = d.rootLevel; level > 2; --level) {
         const uint32_t cl = level - 1;
-        const uint64_t size = d.levelSize[level];
-        const uint16_t* w = d.words + d.levelOff[level];
-        if (node >= size) {
-            flagBad(d);
-            return 0;
-        }
         const uint32_t c0 = (((uint32_t)v[0] >> cl) & 1u) | ((((uint32_t)v[1] >> cl) & 1u) << 1) | ((((uint32_t)v[2] >> cl) & 1u) << 2);
         const uint32_t c = xf[kXfChild + tid * 8 + c0];
-        const uint32_t mask = w[node];
-        const uint32_t sz = (mask >> (2 * c)) & 3u;
-        if (!sz)
-            return 0;
-        // Words before child c: the sum of the lower 2-bit size fields.
-        const uint32_t below = mask & ((1u << (2 * c)) - 1u);
-        const uint64_t at = node + 1 + __popc(below & 0x5555u) + 2 * __popc(below & 0xAAAAu);
-        if (at + sz > size) {
-            flagBad(d);
-            return 0;
-        }
-        uint64_t e = 0;
-        for (uint32_t i = 0; i < sz; ++i)
-            e |= (uint64_t)w[at + i] << (16 * i);
-        if (d.useTables && (sz == 1 || (e & 1ull))) {
-            const uint64_t idx = sz == 1 ? e : e >> 1;
-            if (idx >= d.tableSize[cl]) {
-                flagBad(d);
-                return 0;
-            }
-            e = d.tables[d.tableOff[cl] + idx];
-        } else if (d.useTables) {
-            e >>= 1;
-        }
-        const bool hasT = d.trans && cl <= d.maxT;
-        const uint32_t tb = cl + 1;
-        const uint64_t tm = (1ull << tb) - 1ull;
-        const int bias = 1 << cl;
-        int t[3] = { 0, 0, 0 };
+        uint64_t child = 0;
         uint32_t xid = 0;
-        if (d.useHuffman) {
-            if (!((d.hasLut >> cl) & 1u)) {
-                flagBad(d);
-                return 0;
-            }
-            const uint32_t ent = d.lut[d.lutOff[cl] + ((uint32_t)e & d.lutMask[cl])];
-            uint32_t type = ent & 0xFFFFu;
-            e >>= ent >> 16;
-            uint32_t axes = 0;
-            if (hasT) {
-                axes = type & 7u;
-                type >>= 3;
-            }
-            for (int a = 2; a >= 0; --a)
-                if (axes & (1u << a)) {
-                    t[a] = (int)(e & tm) - bias;
-                    e >>= tb;
-                }
-            if (d.sym) {
-                xid |= type & 7u;
-                type >>= 3;
-            }
-            if (d.perm)
-                xid |= (type & 7u) << 3;
-        } else if (d.sym || d.perm || d.trans) {
-            const bool tr = e & 1ull;
-            e >>= 1;
-            if (tr) {
-                if (hasT)
-                    for (int a = 0; a < 3; ++a) {
-                        t[a] = (int)(e & tm) - bias;
-                        e >>= tb;
-                    }
-                if (d.sym) {
-                    xid |= (uint32_t)e & 7u;
-                    e >>= 3;
-                }
-                if (d.perm) {
-                    xid |= ((uint32_t)e & 7u) << 3;
-                    e >>= 3;
-                }
-            }
-        }
-        if (xid >= 48) {
-            flagBad(d);
+        int t[3];
+        if (readChild(d, level, node, c, &child, &xid, t) != 1)
             return 0;
-        }
-        node = (uint32_t)e;
+        node = child;
         if (t[0] | t[1] | t[2]) {
             // The translation in the frame of the transform accumulated so far; the voxel must stay
             // inside the same child cell.
             for (int a = 0; a < 3; ++a) {
-                const uint32_t src = xf[kXfPerm + tid * 3 + a];
-                int ta = src == 0 ? t[0] : (src == 1 ? t[1] : t[2]);
-                if ((tid >> a) & 1u)
-                    ta = -ta;
                 const uint32_t before = (uint32_t)v[a] >> cl;
-                v[a] -= ta;
+                v[a] -= frameTranslation(xf, tid, t, a);
                 if (((uint32_t)v[a] >> cl) != before)
                     return 0;
             }
@@ -845,6 +865,237 @@ __global__ __launch_bounds__(256) void kDecodeRuns(Dev16 d, uint32_t* out, uint6
     out[w] = word;
 }
 
+// ---- ray casting ------------------------------------------------------------------------------
+// First voxel a ray enters, through the same lookup as the decoder (the renderer's use of the
+// format: renderer/src/dags/transform_dag/transform_dag.cu:5-41, tracer_impl.h:874-1010). This
+// traversal is its own design: every step walks from the root to the voxel the ray is in, the
+// way voxelValue does, and stops at the first level where that voxel is empty: an empty child
+// cell, or a translated child whose shifted content does not reach the voxel. The ray then jumps
+// to the exit of that empty region (the whole child cell, or up to the part of the cell the
+// shifted content covers), so empty space is skipped a cell at a time; inside a 4^3 leaf it steps
+// voxel by voxel. Coordinates are integer voxels; only the jump times are floating point.
+
+struct RayState {
+    float o[3], dir[3], inv[3];
+    float t; // current entry time
+    int v[3]; // current voxel (world coordinates)
+};
+
+// Moves the ray to where it leaves the world box [lo, hi) (it is inside the box now): the new
+// voxel is one past the box on every axis whose face is crossed first.
+__device__ __forceinline__ void exitBox(RayState& r, const int lo[3], const int hi[3])
+{
+    float te[3];
+    float tx = 3.0e38f;
+    for (int a = 0; a < 3; ++a) {
+        te[a] = r.dir[a] > 0.f ? ((float)hi[a] - r.o[a]) * r.inv[a] : (r.dir[a] < 0.f ? ((float)lo[a] - r.o[a]) * r.inv[a] : 3.0e38f);
+        tx = fminf(tx, te[a]);
+    }
+    r.t = fmaxf(r.t, tx);
+    for (int a = 0; a < 3; ++a) {
+        if (te[a] == tx && r.dir[a] != 0.f) {
+            r.v[a] = r.dir[a] > 0.f ? hi[a] : lo[a] - 1;
+        } else {
+            const int f = (int)floorf(r.o[a] + r.t * r.dir[a]);
+            r.v[a] = min(max(f, lo[a]), hi[a] - 1);
+        }
+    }
+}
+
+// Moves the ray to where it enters the world box [lo, hi) ahead of it (entry time te > r.t).
+__device__ __forceinline__ void enterBox(RayState& r, const int lo[3], const int hi[3], float te, const float ta[3])
+{
+    r.t = te;
+    for (int a = 0; a < 3; ++a) {
+        if (ta[a] == te && r.dir[a] != 0.f) {
+            r.v[a] = r.dir[a] > 0.f ? lo[a] : hi[a] - 1;
+        } else {
+            const int f = (int)floorf(r.o[a] + r.t * r.dir[a]);
+            r.v[a] = min(max(f, lo[a]), hi[a] - 1);
+        }
+    }
+}
+
+// Parametric interval of the ray inside the world box [lo, hi); per-axis entry times in ta.
+__device__ __forceinline__ void boxInterval(const RayState& r, const int lo[3], const int hi[3], float* t0, float* t1, float ta[3])
+{
+    float a0 = -3.0e38f, a1 = 3.0e38f;
+    for (int a = 0; a < 3; ++a) {
+        if (r.dir[a] == 0.f) {
+            ta[a] = -3.0e38f;
+            if (r.o[a] < (float)lo[a] || r.o[a] >= (float)hi[a])
+                a1 = -3.0e38f;
+            continue;
+        }
+        float x0 = ((float)lo[a] - r.o[a]) * r.inv[a];
+        float x1 = ((float)hi[a] - r.o[a]) * r.inv[a];
+        if (x0 > x1) {
+            const float tmp = x0;
+            x0 = x1;
+            x1 = tmp;
+        }
+        ta[a] = x0;
+        a0 = fmaxf(a0, x0);
+        a1 = fminf(a1, x1);
+    }
+    *t0 = a0;
+    *t1 = a1;
+}
+
+// One thread per ray. rays: 8 floats each (origin xyz, tmin, direction xyz, tmax), voxel units,
+// the volume is [0, R)^3. hits: 4 int32 (voxel x, y, z, steps taken) with x = -1 for no hit; ts:
+// the entry distance of the hit voxel. A ray that runs out of its step budget (more than 4R + 64
+// walks, which a correct traversal never needs) is reported as a miss with steps = -1.
+__global__ __launch_bounds__(256) void kTrace(Dev16 d, const float* __restrict__ rays, uint64_t n, int32_t* __restrict__ hits, float* __restrict__ ts)
+{
+    __shared__ uint8_t xf[kXfBytes];
+    for (uint32_t i = threadIdx.x; i < kXfBytes; i += blockDim.x)
+        xf[i] = d.xf[i];
+    __syncthreads();
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const float* in = rays + i * 8;
+    RayState r;
+    for (int a = 0; a < 3; ++a) {
+        r.o[a] = in[a];
+        r.dir[a] = in[4 + a];
+        r.inv[a] = r.dir[a] != 0.f ? 1.0f / r.dir[a] : 0.f;
+    }
+    const int R = 1 << d.log2R;
+    int32_t* h = hits + i * 4;
+    h[0] = h[1] = h[2] = -1;
+    h[3] = 0;
+    ts[i] = 0.f;
+    const int zero[3] = { 0, 0, 0 }, full[3] = { R, R, R };
+    float t0, t1, ta[3];
+    boxInterval(r, zero, full, &t0, &t1, ta);
+    t0 = fmaxf(t0, in[3]);
+    t1 = fminf(t1, in[7]);
+    if (!(t0 < t1) || (r.dir[0] == 0.f && r.dir[1] == 0.f && r.dir[2] == 0.f))
+        return; // the ray does not cross the volume
+    r.t = t0;
+    for (int a = 0; a < 3; ++a)
+        r.v[a] = min(max((int)floorf(r.o[a] + t0 * r.dir[a]), 0), R - 1);
+    const int budget = 4 * R + 64;
+    for (int step = 0; step < budget; ++step) {
+        if (r.t > t1 || r.v[0] < 0 || r.v[1] < 0 || r.v[2] < 0 || r.v[0] >= R || r.v[1] >= R || r.v[2] >= R) {
+            h[3] = step;
+            return; // left the volume or the ray's interval: no hit
+        }
+        // Walk from the root to the voxel, as voxelValue does, tracking the accumulated translation
+        // and the world box in which every choice of this walk stays the same: the intersection of
+        // the child cells taken and, below each translated pointer, of the part of its cell that
+        // the shifted content covers.
+        int vv[3] = { r.v[0], r.v[1], r.v[2] }; // virtual coordinates = world - shift
+        int shift[3] = { 0, 0, 0 };
+        int blo[3] = { 0, 0, 0 }, bhi[3] = { R, R, R }; // world box where this walk applies
+        uint32_t tid = 0;
+        uint64_t node = 0;
+        int stopLevel = -1; // child level of an empty cell (-1: reached the leaf)
+        bool clipped = false;
+        int tt[3] = { 0, 0, 0 };
+        int clo[3] = { 0, 0, 0 }; // world corner of the current child cell (its size is 2^cl)
+        uint32_t cellLevel = 0;
+        for (uint32_t level = d.rootLevel; level > 2; --level) {
+            const uint32_t cl = level - 1;
+            cellLevel = cl;
+            const uint32_t c0 = (((uint32_t)vv[0] >> cl) & 1u) | ((((uint32_t)vv[1] >> cl) & 1u) << 1) | ((((uint32_t)vv[2] >> cl) & 1u) << 2);
+            const uint32_t c = xf[kXfChild + tid * 8 + c0];
+            for (int a = 0; a < 3; ++a) { // the child cell, in world coordinates
+                clo[a] = ((vv[a] >> cl) << cl) + shift[a];
+                blo[a] = max(blo[a], clo[a]);
+                bhi[a] = min(bhi[a], clo[a] + (1 << cl));
+            }
+            uint64_t child = 0;
+            uint32_t xid = 0;
+            int t[3];
+            const int rc = readChild(d, level, node, c, &child, &xid, t);
+            if (rc == 2)
+                return; // malformed data (flagged)
+            if (rc == 0) {
+                stopLevel = (int)cl;
+                break;
+            }
+            if (t[0] | t[1] | t[2]) {
+                for (int a = 0; a < 3; ++a)
+                    tt[a] = frameTranslation(xf, tid, t, a);
+                if ((((uint32_t)(vv[0] - tt[0])) >> cl) != ((uint32_t)vv[0] >> cl) || (((uint32_t)(vv[1] - tt[1])) >> cl) != ((uint32_t)vv[1] >> cl) ||
+                    (((uint32_t)(vv[2] - tt[2])) >> cl) != ((uint32_t)vv[2] >> cl)) {
+                    stopLevel = (int)cl;
+                    clipped = true;
+                    break;
+                }
+                for (int a = 0; a < 3; ++a) { // the part of the cell the shifted content covers
+                    blo[a] = max(blo[a], clo[a] + tt[a]);
+                    bhi[a] = min(bhi[a], clo[a] + (1 << cl) + tt[a]);
+                    vv[a] -= tt[a];
+                    shift[a] += tt[a];
+                }
+            }
+            tid = xf[kXfCombine + tid * 48 + xid];
+            node = child;
+        }
+        if (stopLevel >= 0) {
+            if (!clipped) {
+                exitBox(r, blo, bhi); // an empty child cell (within the walk's box)
+                continue;
+            }
+            // Translated child whose content does not reach the voxel: its content covers the
+            // part of the box that is also inside the box shifted by the translation. Jump to
+            // where the ray enters that part if it is ahead, else leave the box (one voxel at a
+            // time when float rounding disagrees with the integer test).
+            int olo[3], ohi[3];
+            for (int a = 0; a < 3; ++a) {
+                olo[a] = max(blo[a], clo[a] + tt[a]);
+                ohi[a] = min(bhi[a], clo[a] + (1 << cellLevel) + tt[a]);
+            }
+            float e0, e1, ea[3];
+            boxInterval(r, olo, ohi, &e0, &e1, ea);
+            if (olo[0] >= ohi[0] || olo[1] >= ohi[1] || olo[2] >= ohi[2] || e0 > e1 || e1 < r.t) {
+                exitBox(r, blo, bhi);
+            } else if (e0 > r.t) {
+                enterBox(r, olo, ohi, e0, ea);
+            } else {
+                const int vlo[3] = { r.v[0], r.v[1], r.v[2] }, vhi[3] = { r.v[0] + 1, r.v[1] + 1, r.v[2] + 1 };
+                exitBox(r, vlo, vhi);
+            }
+            continue;
+        }
+        // Leaf: step voxel by voxel inside its 4^3 box (within the walk's box).
+        if (node >= d.numSubgrids) {
+            flagBad(d);
+            return;
+        }
+        const uint64_t leaf = d.subgrids[node];
+        for (int a = 0; a < 3; ++a) {
+            const int lo = ((vv[a] >> 2) << 2) + shift[a];
+            blo[a] = max(blo[a], lo);
+            bhi[a] = min(bhi[a], lo + 4);
+        }
+        for (;;) {
+            const int w0 = r.v[0] - shift[0], w1 = r.v[1] - shift[1], w2 = r.v[2] - shift[2];
+            const uint32_t lo = ((uint32_t)w0 & 1u) | (((uint32_t)w1 & 1u) << 1) | (((uint32_t)w2 & 1u) << 2);
+            const uint32_t hi = (((uint32_t)w0 >> 1) & 1u) | ((((uint32_t)w1 >> 1) & 1u) << 1) | ((((uint32_t)w2 >> 1) & 1u) << 2);
+            if ((leaf >> (xf[kXfChild + tid * 8 + hi] * 8u + xf[kXfChild + tid * 8 + lo])) & 1ull) {
+                h[0] = r.v[0];
+                h[1] = r.v[1];
+                h[2] = r.v[2];
+                h[3] = step;
+                ts[i] = r.t;
+                return;
+            }
+            const int vlo[3] = { r.v[0], r.v[1], r.v[2] }, vhi[3] = { r.v[0] + 1, r.v[1] + 1, r.v[2] + 1 };
+            exitBox(r, vlo, vhi);
+            if (++step >= budget || r.t > t1)
+                break;
+            if (r.v[0] < blo[0] || r.v[1] < blo[1] || r.v[2] < blo[2] || r.v[0] >= bhi[0] || r.v[1] >= bhi[1] || r.v[2] >= bhi[2])
+                break;
+        }
+    }
+    h[3] = -1; // step budget exhausted
+}
+
 } // namespace
 } // namespace tdag
 
@@ -981,6 +1232,24 @@ tdag_status upload(tdag_context* ctx, tdag_tdag16* h)
     return TDAG_OK;
 }
 
+// The device copy of the encoded DAG on ctx's device (made on first use; ctx's device current).
+tdag_status ensureDeviceCopy(tdag_context* ctx, tdag_tdag16* h)
+{
+    if (hipError_t x = hipSetDevice(ctx->device); x != hipSuccess)
+        return hipErr(x, "hipSetDevice");
+    if (h->devDevice == ctx->device)
+        return TDAG_OK;
+    if (h->dev) {
+        const int cur = ctx->device;
+        if (hipSetDevice(h->devDevice) == hipSuccess)
+            (void)hipFree(h->dev);
+        (void)hipSetDevice(cur);
+        h->dev = nullptr;
+        h->devDevice = -1;
+    }
+    return upload(ctx, h);
+}
+
 } // namespace
 } // namespace tdag
 
@@ -1064,20 +1333,8 @@ tdag_status tdag_tdag16_decode_device(tdag_context* ctx, const tdag_tdag16* hc, 
     const uint64_t need = std::max<uint64_t>(1, R * R * R / 32);
     if (R < 4 || out_words < need)
         return tdag::err(TDAG_EINVAL, "output buffer smaller than resolution^3 / 32 words");
-    if (hipError_t x = hipSetDevice(ctx->device); x != hipSuccess)
-        return tdag::hipErr(x, "hipSetDevice");
-    if (h->devDevice != ctx->device) {
-        if (h->dev) {
-            const int cur = ctx->device;
-            if (hipSetDevice(h->devDevice) == hipSuccess)
-                (void)hipFree(h->dev);
-            (void)hipSetDevice(cur);
-            h->dev = nullptr;
-            h->devDevice = -1;
-        }
-        if (tdag_status s = tdag::upload(ctx, h); s != TDAG_OK)
-            return s;
-    }
+    if (tdag_status s = tdag::ensureDeviceCopy(ctx, h); s != TDAG_OK)
+        return s;
     hipStream_t st = ctx->stream;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (out_ms) {
@@ -1119,6 +1376,42 @@ tdag_status tdag_tdag16_decode_device(tdag_context* ctx, const tdag_tdag16* hc, 
     }
     if (x != hipSuccess)
         return tdag::hipErr(x, ".tdag16 device decode");
+    if (bad)
+        return tdag::err(TDAG_EINVAL, "malformed .tdag16 data (an index outside its level, table or leaf array)");
+    return TDAG_OK;
+}
+
+tdag_status tdag_tdag16_trace_device(tdag_context* ctx, const tdag_tdag16* hc, const float* d_rays, uint64_t n, int32_t* d_hits, float* d_t, float* out_ms)
+{
+    if (!ctx || !hc || (n && (!d_rays || !d_hits || !d_t)))
+        return tdag::err(TDAG_EINVAL, "null argument");
+    auto* h = const_cast<tdag_tdag16*>(hc);
+    if (h->e.resolution < 4)
+        return tdag::err(TDAG_EINVAL, "resolution below 4");
+    if (tdag_status s = tdag::ensureDeviceCopy(ctx, h); s != TDAG_OK)
+        return s;
+    if (n == 0)
+        return TDAG_OK;
+    hipStream_t st = ctx->stream;
+    hipError_t x = hipEventRecord(ctx->ev[0], st);
+    const uint32_t zero = 0;
+    if (x == hipSuccess)
+        x = hipMemcpyAsync(h->d.bad, &zero, 4, hipMemcpyHostToDevice, st);
+    if (x == hipSuccess) {
+        hipLaunchKernelGGL(tdag::kTrace, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, h->d, d_rays, n, d_hits, d_t);
+        x = hipGetLastError();
+    }
+    if (x == hipSuccess)
+        x = hipEventRecord(ctx->ev[3], st);
+    uint32_t bad = 0;
+    if (x == hipSuccess)
+        x = hipMemcpyAsync(&bad, h->d.bad, 4, hipMemcpyDeviceToHost, st);
+    if (x == hipSuccess)
+        x = hipStreamSynchronize(st);
+    if (x != hipSuccess)
+        return tdag::hipErr(x, ".tdag16 ray casting");
+    if (out_ms)
+        (void)hipEventElapsedTime(out_ms, ctx->ev[0], ctx->ev[3]);
     if (bad)
         return tdag::err(TDAG_EINVAL, "malformed .tdag16 data (an index outside its level, table or leaf array)");
     return TDAG_OK;

@@ -97,6 +97,8 @@ def bind(L):
     L.tdag_tdag16_info.argtypes = [vp, _u32p, _u32p, _u64p, _u64p, ctypes.c_uint32]
     L.tdag_tdag16_decode_device.argtypes = [vp, vp, vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_float)]
     L.tdag_tdag16_free.argtypes = [vp]
+    L.tdag_tdag16_trace_device.argtypes = [vp, vp, vp, ctypes.c_uint64, vp, vp, ctypes.POINTER(ctypes.c_float)]
+    L.tdag_tdag16_trace_device.restype = ctypes.c_int
     L.tdag_svdag_from_view.argtypes = [ctypes.POINTER(SvdagView), ctypes.POINTER(vp)]
     L.tdag_svdag_from_view.restype = ctypes.c_int
     L.tdag_tdag_copy_level.argtypes = [vp, ctypes.c_uint32, _u64p, ctypes.c_uint64, _u64p]
@@ -414,6 +416,22 @@ class TDag16:
         check(lib().tdag_tdag16_decode_device(ctx._ptr, self._h, ctypes.c_void_p(d_out.data_ptr()), d_out.numel(), ctypes.byref(ms)),
               "tdag_tdag16_decode_device")
         return d_out, float(ms.value)
+
+    def trace_device(self, ctx, d_rays, n: int, d_hits=None, d_t=None):
+        """First-hit ray casting on the device (tdag_tdag16_trace_device). d_rays: float32 device
+        tensor of n x 8 (origin, tmin, direction, tmax). Returns (hits int32 n x 4: voxel x y z or
+        -1, steps; t float32 n; device ms)."""
+        import torch
+
+        if d_hits is None:
+            d_hits = torch.empty((max(1, n), 4), dtype=torch.int32, device=d_rays.device)
+        if d_t is None:
+            d_t = torch.empty(max(1, n), dtype=torch.float32, device=d_rays.device)
+        torch.cuda.synchronize()
+        ms = ctypes.c_float()
+        check(lib().tdag_tdag16_trace_device(ctx._ptr, self._h, ctypes.c_void_p(d_rays.data_ptr()), int(n), ctypes.c_void_p(d_hits.data_ptr()),
+                                             ctypes.c_void_p(d_t.data_ptr()), ctypes.byref(ms)), "tdag_tdag16_trace_device")
+        return d_hits, d_t, float(ms.value)
 
     def close(self):
         if self._h:

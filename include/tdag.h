@@ -442,6 +442,16 @@ tdag_status tdag_tdag16_info(const tdag_tdag16* h, uint32_t* resolution, uint32_
  * TDAG_EINVAL. out_ms (optional): device time of the decode kernel. Returns after the work on
  * ctx's stream is done. */
 tdag_status tdag_tdag16_decode_device(tdag_context* ctx, const tdag_tdag16* h, uint32_t* d_out_words, uint64_t out_words, float* out_ms);
+/* Ray casting through the encoded DAG on the device, the renderer's use of the format
+ * (renderer/src/dags/transform_dag/transform_dag.cu:5-41, tracer_impl.h:874-1010): for each ray the
+ * first voxel it enters, found with the same voxel lookup as tdag_tdag16_decode_device.
+ *   d_rays  device, n x 8 floats: origin x y z, tmin, direction x y z, tmax (voxel units; the
+ *           volume is [0, resolution)^3)
+ *   d_hits  device, n x 4 int32: the hit voxel x y z (x = -1: no voxel in [tmin, tmax]), then the
+ *           number of steps the ray took (-1 if it exceeded its budget of 4 * resolution + 64)
+ *   d_t     device, n floats: the distance at which the ray enters the hit voxel
+ * Malformed data gives TDAG_EINVAL. out_ms (optional): device time of the call. Synchronises. */
+tdag_status tdag_tdag16_trace_device(tdag_context* ctx, const tdag_tdag16* h, const float* d_rays, uint64_t n, int32_t* d_hits, float* d_t, float* out_ms);
 tdag_status tdag_tdag16_free(tdag_tdag16* h);
 /* createLengthLimitedHuffmanCodeLengths (huffman.cpp:113-183): package-merge code lengths of at
  * most max_len bits for n symbols. */

@@ -334,4 +334,15 @@ inline float decodeTdag16Device(const tdag_tdag16* h, uint32_t* d_words, uint64_
     return ms;
 }
 
+// First-hit ray casting through the encoded DAG on the device (the renderer's use of the format,
+// renderer/src/dags/transform_dag/transform_dag.cu:5-41): d_rays n x 8 floats (origin, tmin,
+// direction, tmax), d_hits n x 4 int32 (voxel x y z or -1, steps), d_t n floats. Returns the
+// device time in milliseconds.
+inline float traceTdag16Device(const tdag_tdag16* h, const float* d_rays, uint64_t n, int32_t* d_hits, float* d_t, Context& ctx = Context::instance())
+{
+    float ms = 0.f;
+    check(tdag_tdag16_trace_device(ctx.get(), h, d_rays, n, d_hits, d_t, &ms), "tdag_tdag16_trace_device");
+    return ms;
+}
+
 } // namespace tdag_cpp

@@ -31,6 +31,9 @@ int orc_postprocess(uint32_t level, const uint64_t* masks, uint64_t n, const uin
 /* .tdag16 compact encoding (oracle_tdag16.cpp). */
 int orc_huffman_code_lengths(const uint64_t* freq, uint32_t n, uint32_t maxLen, uint32_t* outLen);
 int orc_huffman_code_words(const uint32_t* len, uint32_t n, int lsbFirst, uint64_t* outCode);
+/* First-hit ray casting through a dense grid (oracle_rays.cpp): rays n x 8 floats (origin, tmin,
+ * direction, tmax), hits n x 3 int32 (-1: none), t the entry distance (double). */
+int orc_ray_march(const uint8_t* bits, uint32_t R, const float* rays, uint64_t n, int32_t* outHit, double* outT);
 int64_t orc_tdag16_decode(const uint8_t* bytes, uint64_t len, uint8_t* outBits, uint64_t outCap);
 
 #ifdef __cplusplus

@@ -61,6 +61,7 @@ def lib():
             _bind_svdag(L)
         L.orc_huffman_code_lengths.argtypes = [_u64p, ctypes.c_uint32, ctypes.c_uint32, _u32p]
         L.orc_huffman_code_words.argtypes = [_u32p, ctypes.c_uint32, ctypes.c_int, _u64p]
+        L.orc_ray_march.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_tdag16_decode.argtypes = [ctypes.c_char_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64]
         L.orc_tdag16_decode.restype = ctypes.c_int64
         _lib = L
@@ -205,3 +206,14 @@ def postprocess(level: int, masks: np.ndarray, encoded: np.ndarray):
     rc = lib().orc_postprocess(level, _p64(m), n, _p64(enc), _p64(mapping), uniq.ctypes.data_as(_u32p), _p64(nu))
     assert rc == 0
     return mapping[:n], uniq[: int(nu[0])]
+
+
+def ray_march(bits: np.ndarray, R: int, rays: np.ndarray):
+    """First occupied voxel along each ray (n x 8 float32: origin, tmin, direction, tmax) through
+    the dense grid bits (bit i = x + R*y + R*R*z): (hits n x 3 int32, -1 for none; t float64)."""
+    b = np.ascontiguousarray(bits, dtype=np.uint8)
+    r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+    hits = np.zeros((max(1, len(r)), 3), dtype=np.int32)
+    t = np.zeros(max(1, len(r)), dtype=np.float64)
+    assert lib().orc_ray_march(b.ctypes.data, R, r.ctypes.data, len(r), hits.ctypes.data, t.ctypes.data) == 0
+    return hits[: len(r)], t[: len(r)]
