@@ -110,3 +110,20 @@ def test_device_flatten_rejects_bad_index(gpu_ctx):
     d_out = torch.zeros(2 * 8, dtype=torch.int64, device="cuda")
     with pytest.raises(tdag_amd.TdagError, match="EINVAL"):
         s.flatten_device(gpu_ctx, 3, d_idx, d_out)
+
+
+@pytest.mark.parametrize("kind,R,seed", [(0, 64, 2), (1, 128, 5), (2, 256, 6)])
+def test_device_svdag_equals_oracle(oracle, gpu_ctx, kind, R, seed):
+    """The device build against the oracle's restatement of the recursive constructor + toDAG
+    directly (not through the host build): same leaves and nodes, index for index."""
+    import svdag_arrays
+    from tdag_amd import pipeline as pl
+
+    print(f"device svdag vs oracle kind={kind} R={R} seed={seed}", flush=True)
+    bits = pl.scene_dense(kind, R, seed)
+    dev = pl.Svdag.from_device(gpu_ctx, _device_bits(bits), R)
+    root, nodes, leaves = svdag_arrays.oracle_svdag_arrays(oracle, bits, R)
+    assert dev.root_level == root
+    assert np.array_equal(dev.leaves(), leaves)
+    for level in range(3, root + 1):
+        assert np.array_equal(dev.nodes(level), nodes[level]), level

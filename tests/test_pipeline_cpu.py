@@ -100,6 +100,20 @@ def test_hierarchical_tdag_bytes(oracle, pl, kind, R, seed, sym, axis):
     # and never marked a DAG, so the header holds StructureType::Tree = 0 (structure.h:127-131, 171).
     assert d["structure_type"] == 0
     assert np.array_equal(tdag_decode.expand(d), tdag_decode.dense_to_cube(bits, R))
+    # Every level comes out in the order of its group key: hash_combine over the node's 8 child
+    # pointers sorted by ptr, an empty child counting as the sentinel's ptr 0xFFFFFFFF
+    # (transform_dag_hierarchical.cpp:365-378 groups, :417-430 emits in sorted key order).
+    for level in range(3, d["root_level"] + 1):
+        keys = [_edit_group_key(node) for node in d["levels"][level]]
+        assert all(a <= b for a, b in zip(keys, keys[1:])), f"level {level} not in group-key order"
+
+
+def _edit_group_key(node):
+    m64 = (1 << 64) - 1
+    seed = 0
+    for p in sorted(int(c) & ((1 << 37) - 1) for c in node):
+        seed = (seed ^ ((p + 0x9E3779B9 + (seed << 6) + (seed >> 2)) & m64)) & m64
+    return seed
 
 
 def test_canonical_leaf_classes():
