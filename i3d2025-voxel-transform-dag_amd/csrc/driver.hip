@@ -41,7 +41,19 @@ struct tdag_tdag {
     tdag::TDag t;
     std::vector<tdag::LevelRecord> records;
     bool isStatic = false; // built by constructStaticTransformDAG: read back in the packed form
+    tdag_config cfg {}; // the TransformDAGConfig it was built with
 };
+
+namespace {
+// The serializers and the encoder take the config again (as the reference's writers do); it must
+// be the one the structure was built with, or the pointer types they write would not describe the
+// pointers (e.g. translation bits for a level the build did not translate).
+bool sameConfig(const tdag_config& a, const tdag_config& b)
+{
+    return a.symmetry == b.symmetry && a.axis_permutation == b.axis_permutation && a.translation == b.translation &&
+        a.max_translation_level == b.max_translation_level;
+}
+} // namespace
 
 namespace {
 
@@ -339,6 +351,7 @@ static tdag_status constructTransformDag(tdag_context* ctx, const tdag_svdag* in
     const int threads = hostThreads();
     tdag_tdag* res = new tdag_tdag();
     res->isStatic = staticForm;
+    res->cfg = *cfg;
     std::vector<uint32_t> translationParents;
     if (ctx) {
         // The hierarchical pass on the device (same output as the host pass).
@@ -498,6 +511,8 @@ tdag_status tdag_tdag_serialize(const tdag_tdag* t, const tdag_config* cfg, uint
         return setErr(TDAG_EINVAL, "null argument");
     if (t->isStatic)
         return setErr(TDAG_EINVAL, "a static-structure result is written by tdag_tdag_static_serialize");
+    if (!sameConfig(*cfg, t->cfg))
+        return setErr(TDAG_EINVAL, "config differs from the one the structure was built with");
     const std::vector<uint8_t> bytes = tdag::serializeTdag(t->t, *cfg);
     *out_size = bytes.size();
     if (buf) {
@@ -615,6 +630,8 @@ tdag_status tdag_tdag16_encode(const tdag_tdag* t, const tdag_config* cfg, uint3
         return setErr(TDAG_EINVAL, "null argument");
     if (t->isStatic)
         return setErr(TDAG_EINVAL, "encode the result of tdag_construct_transform_dag (edit-structure form)");
+    if (!sameConfig(*cfg, t->cfg))
+        return setErr(TDAG_EINVAL, "config differs from the one the structure was built with");
     if (flags & ~(TDAG16_POINTER_TABLES | TDAG16_HUFFMAN))
         return setErr(TDAG_EINVAL, "unknown flag");
     try {

@@ -400,7 +400,8 @@ tdag_status tdag_construct_transform_dag(tdag_context* ctx, const tdag_svdag* in
  * tdag_construct_transform_dag where a query's minimum-key parent was not proposed. */
 tdag_status tdag_construct_transform_dag_learned(tdag_context* ctx, const tdag_svdag* in, const tdag_config* cfg, uint32_t flags, tdag_mlp* mlp, uint32_t k,
     tdag_tdag** out);
-/* The .tdag byte stream (voxelizer/src/main.cpp:144-151): pass buf = NULL to query the size. The
+/* The .tdag byte stream (voxelizer/src/main.cpp:144-151): pass buf = NULL to query the size; cfg
+ * must be the config the structure was built with (TDAG_EINVAL otherwise). The
  * header's structureType is 0 (Tree), as in the reference, whose output structure is
  * default-constructed (transform_dag_hierarchical.cpp:307, structure.h:171). */
 tdag_status tdag_tdag_serialize(const tdag_tdag* tdag, const tdag_config* cfg, uint8_t* buf, uint64_t cap, uint64_t* out_size);
@@ -426,7 +427,9 @@ typedef struct tdag_tdag16 tdag_tdag16;
  * optional pointer tables and per-level Huffman code of the pointer types. Pointers of equal
  * reference count keep the order of their first use (the reference takes an unordered_map's
  * order there), so the bytes can differ from the reference's while the volume is the same.
- * Takes the result of tdag_construct_transform_dag (not of the static driver). */
+ * Takes the result of tdag_construct_transform_dag (not of the static driver) and the config it
+ * was built with (another config gives TDAG_EINVAL: the pointer types would not describe the
+ * pointers). */
 tdag_status tdag_tdag16_encode(const tdag_tdag* tdag, const tdag_config* cfg, uint32_t flags, tdag_tdag16** out);
 /* TransformDAG16::readFrom of a .tdag16 byte stream (transform_dag_encoding.cpp:1029-1036). */
 tdag_status tdag_tdag16_from_bytes(const uint8_t* bytes, uint64_t len, tdag_tdag16** out);

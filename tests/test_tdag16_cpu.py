@@ -174,3 +174,21 @@ def test_code_word_longer_than_its_length_is_rejected(pl):
     struct.pack_into("<Q", data, rec, code | (1 << nbits))
     with pytest.raises(tdag_amd.TdagError, match="EINVAL"):
         pl.TDag16.from_bytes(bytes(data))
+
+
+def test_encoder_and_writer_refuse_another_config(pl):
+    """The encoder and the .tdag writer take the config again; one that differs from the build's
+    would write pointer types that do not describe the pointers, so it is refused."""
+    import ctypes
+
+    import tdag_amd
+
+    bits = pl.scene_dense(1, 64, 3)
+    t = pl.construct_transform_dag(None, pl.Svdag(bits, 64), tdag_amd.TransformDAGConfig(True, True, False, 3))
+    other = tdag_amd.TransformDAGConfig(True, False, False, 3).to_c()
+    h = ctypes.c_void_p()
+    assert tdag_amd.lib().tdag_tdag16_encode(t._h, ctypes.byref(other), 3, ctypes.byref(h)) == 1  # TDAG_EINVAL
+    assert "config differs" in tdag_amd.last_error()
+    size = ctypes.c_uint64()
+    assert tdag_amd.lib().tdag_tdag_serialize(t._h, ctypes.byref(other), None, 0, ctypes.byref(size)) == 1
+    t.encode16().close()  # the build's own config is accepted
