@@ -359,8 +359,13 @@ static tdag_status constructTransformDag(tdag_context* ctx, const tdag_svdag* in
             delete res;
             return setHipErr(e, "hipSetDevice");
         }
-        if (hipError_t e = tdag::hierarchicalDedupDevice(dag, maxT + 1, translationParents, *cfg, staticForm, ctx->stream, ctx->svdagScratch, ctx->svdagScratchBytes, res->t); e != hipSuccess) {
+        std::string failure;
+        if (hipError_t e = tdag::hierarchicalDedupDevice(dag, maxT + 1, translationParents, *cfg, staticForm, ctx->stream, ctx->svdagScratch, ctx->svdagScratchBytes, res->t,
+                &failure);
+            e != hipSuccess) {
             delete res;
+            if (!failure.empty())
+                return setErr(TDAG_EINTERNAL, "device hierarchical pass: " + failure);
             return setHipErr(e, "device hierarchical pass");
         }
     } else {

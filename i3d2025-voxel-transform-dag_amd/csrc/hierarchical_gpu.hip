@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "pipeline.hpp"
@@ -80,13 +81,21 @@ __global__ __launch_bounds__(kHT) void kLeafNodes(const uint64_t* __restrict__ l
     cst[i] = lst[b];
 }
 
-__global__ __launch_bounds__(kHT) void kInnerNodes(const uint32_t* __restrict__ child, uint32_t n, const uint32_t* __restrict__ mptr,
-    const uint64_t* __restrict__ minv, const uint8_t* __restrict__ mst, uint32_t* __restrict__ cptr, uint64_t* __restrict__ cinv, uint8_t* __restrict__ cst)
+__global__ __launch_bounds__(kHT) void kInnerNodes(const uint32_t* __restrict__ child, uint32_t n, uint32_t below, const uint32_t* __restrict__ mptr,
+    const uint64_t* __restrict__ minv, const uint8_t* __restrict__ mst, uint32_t* __restrict__ cptr, uint64_t* __restrict__ cinv, uint8_t* __restrict__ cst,
+    uint32_t* __restrict__ bad)
 {
     const uint64_t i = (uint64_t)blockIdx.x * kHT + threadIdx.x;
     if (i >= (uint64_t)n * 8)
         return;
     const uint32_t ch = child[i];
+    if (ch != kEmptyChild && ch >= below) { // a child index outside the level below
+        atomicOr(bad, 1u);
+        cptr[i] = kEmptyPtr;
+        cinv[i] = ~0ull;
+        cst[i] = 0;
+        return;
+    }
     if (ch == kEmptyChild) {
         cptr[i] = kEmptyPtr;
         cinv[i] = ~0ull;
@@ -132,7 +141,7 @@ __global__ __launch_bounds__(kHT) void kSeeds(const uint32_t* __restrict__ cptr,
 // transformed copy is assembled in a per-thread LDS row (child c goes to slot childPos[id][c]);
 // private arrays are only indexed by unrolled loop counters.
 __global__ __launch_bounds__(kHT) void kCanonKeys(const DevTables* __restrict__ gt, const uint32_t* __restrict__ cptr, const uint8_t* __restrict__ normTab,
-    const uint8_t* __restrict__ cst, uint32_t n, uint32_t numIds, uint64_t* __restrict__ keys)
+    uint32_t normRows, const uint8_t* __restrict__ cst, uint32_t n, uint32_t numIds, uint64_t* __restrict__ keys, uint32_t* __restrict__ bad)
 {
     __shared__ LdsTables T;
     __shared__ uint64_t rows[kHT * 8];
@@ -147,6 +156,14 @@ __global__ __launch_bounds__(kHT) void kCanonKeys(const DevTables* __restrict__ 
     for (int c = 0; c < 8; ++c) {
         p[c] = cptr[(size_t)i * 8 + c];
         st[c] = cst[(size_t)i * 8 + c];
+        // Pointers index the normalisation table's rows and states its 48 columns; neither can be
+        // out of range when the previous level was consistent, and a violation is reported
+        // instead of read.
+        if ((p[c] != kEmptyPtr && p[c] >= normRows) || st[c] >= 48u) {
+            atomicOr(bad, 2u);
+            p[c] = kEmptyPtr;
+            st[c] = 0;
+        }
     }
     uint64_t best[8];
 #pragma unroll
@@ -413,7 +430,7 @@ struct Carve {
 } // namespace
 
 hipError_t hierarchicalDedupDevice(const Svdag& in, uint32_t midLevel, std::vector<uint32_t>& outMidInv, const tdag_config& cfg, bool staticOffsets,
-    hipStream_t st, void*& work, size_t& workBytes, TDag& out)
+    hipStream_t st, void*& work, size_t& workBytes, TDag& out, std::string* failure)
 {
     out = TDag {};
     out.resolution = in.resolution;
@@ -485,7 +502,7 @@ hipError_t hierarchicalDedupDevice(const Svdag& in, uint32_t midLevel, std::vect
         mid = B.get<uint32_t>(maxN);
         normTab = B.get<uint8_t>(std::max<size_t>(maxN, 256) * 48);
         invByHandle = B.get<uint64_t>(maxN);
-        flags = B.get<uint32_t>(4); // [0] scan total, [1] inconsistent-class flag, [2] offsets total
+        flags = B.get<uint32_t>(4); // [0] scan total, [1] inconsistent-class flag, [2] offsets total, [3] index checks
     };
     Carve sizing;
     carveAll(sizing);
@@ -529,6 +546,8 @@ hipError_t hierarchicalDedupDevice(const Svdag& in, uint32_t midLevel, std::vect
         }
     HD_TRY(hipMemcpyAsync(normTab, leafNorm.data(), leafNorm.size(), hipMemcpyHostToDevice, st));
     bool haveOffsets = false; // offsets of the previous level's output nodes are valid
+    uint32_t normRows = 256; // rows of normTab: the 256 leaf bytes, then the previous level's kept nodes
+    uint32_t prevN = 0; // items of the previous level (the range of this level's child indices)
     int cur = 0; // mapping buffers of the level just processed
     for (uint32_t level = 2; level <= std::max(2u, in.rootLevel); ++level) {
         const uint32_t n = (uint32_t)(level == 2 ? in.leaves.size() : in.nodes[level].size());
@@ -539,13 +558,14 @@ hipError_t hierarchicalDedupDevice(const Svdag& in, uint32_t midLevel, std::vect
                 hipLaunchKernelGGL(kLeafNodes, dim3(gridOf((uint64_t)n * 8)), dim3(kHT), 0, st, leaves, n, lptr, linv, lst, cptr, cinv, cst);
             } else {
                 HD_TRY(hipMemcpyAsync(child, in.nodes[level].data(), (size_t)n * 32, hipMemcpyHostToDevice, st));
-                hipLaunchKernelGGL(kInnerNodes, dim3(gridOf((uint64_t)n * 8)), dim3(kHT), 0, st, child, n, mptr[cur], minv[cur], mst[cur], cptr, cinv, cst);
+                hipLaunchKernelGGL(kInnerNodes, dim3(gridOf((uint64_t)n * 8)), dim3(kHT), 0, st, child, n, prevN, mptr[cur], minv[cur], mst[cur], cptr, cinv, cst,
+                    flags + 3);
             }
             const uint32_t* hashOf = (staticOffsets && level >= 4 && haveOffsets) ? offsets : nullptr;
             hipLaunchKernelGGL(kSeeds, dim3(gridOf(n)), dim3(kHT), 0, st, cptr, n, hashOf, seed, idx);
             HD_TRY(hipGetLastError());
             HD_TRY(radixSortPairsU64(seed, idx, n, 0, 64, sortScratch, st));
-            hipLaunchKernelGGL(kCanonKeys, dim3(gridOf(n)), dim3(kHT), 0, st, dT, cptr, normTab, cst, n, t.numIds, keys);
+            hipLaunchKernelGGL(kCanonKeys, dim3(gridOf(n)), dim3(kHT), 0, st, dT, cptr, normTab, normRows, cst, n, t.numIds, keys, flags + 3);
             uint32_t lcap = 64;
             while (lcap < 2 * (uint64_t)n)
                 lcap <<= 1;
@@ -561,14 +581,25 @@ hipError_t hierarchicalDedupDevice(const Svdag& in, uint32_t midLevel, std::vect
             hipLaunchKernelGGL(kOutput, dim3(gridOf(n)), dim3(kHT), 0, st, rep, handle, n, cptr, cst, level == 2 ? leaves : nullptr, nodeInv, outBuf, outBuf, sizes,
                 invByHandle);
             HD_TRY(hipGetLastError());
-            uint32_t hk[2] = { 0, 0 };
-            HD_TRY(hipMemcpyAsync(hk, flags, 8, hipMemcpyDeviceToHost, st));
+            uint32_t hk[4] = { 0, 0, 0, 0 };
+            HD_TRY(hipMemcpyAsync(hk, flags, 16, hipMemcpyDeviceToHost, st));
             HD_TRY(hipStreamSynchronize(st));
-            if (hk[1])
-                return hipErrorUnknown; // a non-kept node without a mapping transform: the keys rule it out
+            if (hk[3]) {
+                if (failure)
+                    *failure = "a child index or state outside its table at level " + std::to_string(level);
+                return hipErrorInvalidValue;
+            }
+            if (hk[1]) {
+                // A node the keys put in a class whose kept node no allowed transform maps it onto:
+                // the keys contradict the reference's equality (an internal invariant).
+                if (failure)
+                    *failure = "inconsistent canonical class at level " + std::to_string(level);
+                return hipErrorInvalidValue;
+            }
             nk = hk[0];
             // The next level's children point at these nk kept nodes.
             hipLaunchKernelGGL(kNormTable, dim3(gridOf((uint64_t)nk * 48)), dim3(kHT), 0, st, dT, invByHandle, nk, normTab);
+            normRows = nk;
             HD_TRY(hipGetLastError());
             if (level == 2) {
                 out.leaves.resize(nk);
@@ -592,6 +623,7 @@ hipError_t hierarchicalDedupDevice(const Svdag& in, uint32_t midLevel, std::vect
             outMidInv.clear();
         }
         haveOffsets = staticOffsets && level >= 3 && n > 0;
+        prevN = n;
         cur = nxt;
         if (level >= in.rootLevel)
             break;

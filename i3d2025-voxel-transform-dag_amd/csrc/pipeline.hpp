@@ -121,9 +121,11 @@ struct HierTables {
 void hierarchicalTables(const tdag_config& cfg, HierTables& out);
 
 // hierarchicalDedup computed on the device (same TDag and outMidInv, bit for bit) with work on
-// `stream`, in a workspace (work, workBytes) that is grown as needed and kept by the caller.
+// `stream`, in a workspace (work, workBytes) that is grown as needed and kept by the caller. A
+// failed internal check (an index outside its table, an inconsistent class) returns
+// hipErrorInvalidValue with a description in *failure.
 hipError_t hierarchicalDedupDevice(const Svdag& in, uint32_t midLevel, std::vector<uint32_t>& outMidInv, const tdag_config& cfg, bool staticOffsets,
-    hipStream_t stream, void*& work, size_t& workBytes, TDag& out);
+    hipStream_t stream, void*& work, size_t& workBytes, TDag& out, std::string* failure = nullptr);
 
 // .tdag byte stream (voxelizer/src/main.cpp:147-150).
 std::vector<uint8_t> serializeTdag(const TDag& t, const tdag_config& cfg);
