@@ -21,6 +21,7 @@ matcher) on this host.
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -111,9 +112,42 @@ def build_workload(ctx, resolution: int, seed: int, max_level: int, kind: int = 
         info["construct_ms"] = round(construct_ms, 1)
         info["construct_warm_ms"] = round((time.perf_counter() - tc) * 1e3, 1)
         info["tdag16"] = encode16_and_decode(ctx, tdag, bits, resolution)
+        info["device_translation"] = device_translation_stage(ctx, svdag, cfg, tdag.tdag_bytes())
     tdag.close()
     info["setup_s"] = round(time.time() - t0, 2)
     return cfg, levels, info
+
+
+def device_translation_stage(ctx, svdag, cfg, ref_bytes: bytes, replays: int = 5):
+    """The translation stage with every step on the device: one driver call with
+    TDAG_DEVICE_TRANSLATION (one count read-back per level), then the stage captured as a hipGraph
+    and replayed. Both results are compared with the host-rewrite driver's bytes."""
+    import numpy as np
+    from tdag_amd import pipeline as pl
+
+    ref = hashlib.sha256(ref_bytes).hexdigest()
+    tc = time.perf_counter()
+    t = pl.construct_transform_dag(ctx, svdag, cfg, device_translation=True)
+    call_ms = (time.perf_counter() - tc) * 1e3
+    same = hashlib.sha256(t.tdag_bytes()).hexdigest() == ref
+    t.close()
+    tc = time.perf_counter()
+    g = pl.BuildGraph(ctx, svdag, cfg)
+    create_ms = (time.perf_counter() - tc) * 1e3
+    try:
+        times = [g.run() for _ in range(replays)]
+        r = g.result()
+        same_graph = hashlib.sha256(r.tdag_bytes()).hexdigest() == ref
+        r.close()
+    finally:
+        g.close()
+    return {
+        "driver_call_ms": round(call_ms, 1),
+        "graph_create_ms": round(create_ms, 1),
+        "graph_replay_ms": round(float(np.median(times)), 3),
+        "graph_replay_ms_all": [round(x, 3) for x in times],
+        "bytes_equal_host_rewrite_driver": bool(same and same_graph),
+    }
 
 
 def encode16_and_decode(ctx, tdag, bits, resolution: int):
@@ -777,6 +811,7 @@ def main():
             **({"svdag_build": info["svdag_build"]} if info.get("svdag_build") else {}),
             **({"construct_transform_dag_ms": {"first_call": info["construct_ms"], "second_call": info["construct_warm_ms"]}} if "construct_ms" in info else {}),
             **({"tdag16": info["tdag16"]} if info.get("tdag16") else {}),
+            **({"device_translation_stage": info["device_translation"]} if info.get("device_translation") else {}),
             "levels": per_level,
             **({"host_buffers": host_buffers} if host_buffers else {}),
             "search_variants": {l: ctx.get_variant(l) for l in order if l >= 3},

@@ -14,6 +14,7 @@
 #include <thread>
 
 #include "context.hpp"
+#include "device_build.hpp"
 #include "errors.hpp"
 #include "pipeline.hpp"
 #include "primitives.hpp"
@@ -328,6 +329,67 @@ static tdag_status matchLevelOnDevice(tdag_context* ctx, const tdag_svdag* in, u
     return e == hipSuccess ? TDAG_OK : setHipErr(e, "driver readback");
 }
 
+// The context's matcher tuning per level, for the device translation stage.
+static void contextTuning(const tdag_context* ctx, tdag::LevelTuning* tune)
+{
+    for (uint32_t l = 0; l < 6; ++l) {
+        tune[l].variant = ctx->variant[l];
+        tune[l].firstBatch = ctx->firstBatch[l];
+        tune[l].batchGrowth = ctx->batchGrowth[l];
+        tune[l].bloomBits = ctx->bloomBits[l];
+        tune[l].childFrames = ctx->childFrames[l];
+    }
+}
+
+// The level maxT + 1 parents of the translation stage: the hierarchical pass's survivors as
+// TransformNodes over the input level below (transform_dag.cpp:165-170).
+static std::vector<std::array<uint64_t, 8>> midParents(const tdag::Svdag& dag, uint32_t maxT, const std::vector<uint32_t>& translationParents)
+{
+    std::vector<std::array<uint64_t, 8>> mid(translationParents.size());
+    for (size_t i = 0; i < translationParents.size(); ++i)
+        mid[i] = toTransformNode(dag.nodes[maxT + 1][translationParents[i]]);
+    return mid;
+}
+
+static tdag_status translationRecords(const tdag::DeviceTranslation& dt, uint32_t maxT, std::vector<tdag::LevelRecord>& records)
+{
+    for (uint32_t level = maxT; level >= 2; --level) {
+        tdag::LevelRecord rec;
+        rec.level = level;
+        rec.items = dt.n[level];
+        rec.unique = dt.nu[level];
+        rec.matched = dt.matched[level];
+        records.push_back(std::move(rec));
+        if (level == 2)
+            break;
+    }
+    return TDAG_OK;
+}
+
+// The translation stage with every step on the device (csrc/device_build.hip), run once.
+static tdag_status deviceTranslation(tdag_context* ctx, const tdag_svdag* in, const tdag_config& cfg, const std::vector<uint32_t>& translationParents,
+    tdag::TDag& t, std::vector<tdag::LevelRecord>& records)
+{
+    const uint32_t maxT = cfg.max_translation_level;
+    if (t.nodes[maxT + 1].size() != translationParents.size())
+        return setErr(TDAG_EINTERNAL, "mid level size mismatch");
+    if (tdag_status s = ensureDeviceCopy(ctx, in); s != TDAG_OK)
+        return s;
+    tdag::LevelTuning tune[6];
+    contextTuning(ctx, tune);
+    tdag::DeviceTranslation dt;
+    hipError_t e = dt.init(in->dev, in->s, midParents(in->s, maxT, translationParents), maxT, tune);
+    if (e == hipSuccess)
+        e = dt.runEager(tdag::capi::cfgBitsOf(&cfg), ctx->stream);
+    if (e == hipErrorInvalidValue)
+        return setErr(TDAG_EINVAL, "device translation stage: a child pointer or item outside its level");
+    if (e == hipSuccess)
+        e = dt.readResult(t, ctx->stream);
+    if (e != hipSuccess)
+        return setHipErr(e, "device translation stage");
+    return translationRecords(dt, maxT, records);
+}
+
 // Both drivers: constructTransformDAG (transform_dag.cpp:138-240) and, with staticForm,
 // constructStaticTransformDAG (transform_dag.cpp:242-390). The static driver makes the same
 // matcher calls on the same items: its hierarchical pass keeps the same nodes (only the order of
@@ -340,6 +402,10 @@ static tdag_status constructTransformDag(tdag_context* ctx, const tdag_svdag* in
         return setErr(TDAG_EINVAL, "null argument");
     if (cfg->translation && !ctx)
         return setErr(TDAG_EINVAL, "a context is required when translation is enabled");
+    if (flags & ~(TDAG_KEEP_LEVEL_INPUTS | TDAG_DEVICE_TRANSLATION))
+        return setErr(TDAG_EINVAL, "unknown flag");
+    if ((flags & TDAG_DEVICE_TRANSLATION) && ((flags & TDAG_KEEP_LEVEL_INPUTS) || learned || !ctx))
+        return setErr(TDAG_EINVAL, "the device translation stage needs a context and keeps no level inputs, and the learned driver has its own level 2");
     const tdag::Svdag& dag = in->s;
     if (dag.structureType != 1)
         return setErr(TDAG_EINVAL, "input must be a DAG (transform_dag.cpp:141)");
@@ -370,6 +436,16 @@ static tdag_status constructTransformDag(tdag_context* ctx, const tdag_svdag* in
         }
     } else {
         res->t = tdag::hierarchicalDedup(dag, maxT + 1, translationParents, *cfg, threads, staticForm);
+    }
+    if (cfg->translation && (flags & TDAG_DEVICE_TRANSLATION)) {
+        // Every step of the translation stage on the device; only counts come back per level.
+        tdag::TDag& t = res->t;
+        if (tdag_status s = deviceTranslation(ctx, in, *cfg, translationParents, t, res->records); s != TDAG_OK) {
+            delete res;
+            return s;
+        }
+        *out = res;
+        return TDAG_OK;
     }
     if (cfg->translation) {
         tdag::TDag& t = res->t;
@@ -456,6 +532,108 @@ static tdag_status constructTransformDag(tdag_context* ctx, const tdag_svdag* in
 tdag_status tdag_construct_transform_dag(tdag_context* ctx, const tdag_svdag* in, const tdag_config* cfg, uint32_t flags, tdag_tdag** out)
 {
     return constructTransformDag(ctx, in, cfg, flags, false, nullptr, 0, false, out);
+}
+
+} // extern "C"
+
+// A build's translation stage captured as one hipGraph (tdag_build_graph_*): the hierarchical
+// pass's result (the levels above the translated ones) and the device stage that replays the rest.
+struct tdag_build_graph {
+    tdag_context* ctx = nullptr;
+    tdag_config cfg {};
+    tdag::TDag base; // hierarchical result; its translated levels are replaced on readout
+    tdag::DeviceTranslation dt;
+    uint32_t cfgBits = 0;
+};
+
+extern "C" {
+
+tdag_status tdag_build_graph_create(tdag_context* ctx, const tdag_svdag* in, const tdag_config* cfg, tdag_build_graph** out)
+{
+    if (!ctx || !in || !cfg || !out)
+        return setErr(TDAG_EINVAL, "null argument");
+    *out = nullptr;
+    if (!cfg->translation)
+        return setErr(TDAG_EINVAL, "the build graph replays the translation stage: translation must be on");
+    const tdag::Svdag& dag = in->s;
+    if (dag.structureType != 1)
+        return setErr(TDAG_EINVAL, "input must be a DAG (transform_dag.cpp:141)");
+    const uint32_t maxT = cfg->max_translation_level;
+    if (maxT < 2 || maxT > 5 || maxT + 1 > dag.rootLevel)
+        return setErr(TDAG_EINVAL, "max_translation_level outside [2, min(5, root level - 1)]");
+    if (hipError_t e = hipSetDevice(ctx->device); e != hipSuccess)
+        return setHipErr(e, "hipSetDevice");
+    std::unique_ptr<tdag_build_graph> g(new tdag_build_graph());
+    g->ctx = ctx;
+    g->cfg = *cfg;
+    g->cfgBits = tdag::capi::cfgBitsOf(cfg);
+    std::vector<uint32_t> translationParents;
+    std::string failure;
+    if (hipError_t e = tdag::hierarchicalDedupDevice(dag, maxT + 1, translationParents, *cfg, false, ctx->stream, ctx->svdagScratch, ctx->svdagScratchBytes, g->base,
+            &failure);
+        e != hipSuccess)
+        return failure.empty() ? setHipErr(e, "device hierarchical pass") : setErr(TDAG_EINTERNAL, "device hierarchical pass: " + failure);
+    if (g->base.nodes[maxT + 1].size() != translationParents.size())
+        return setErr(TDAG_EINTERNAL, "mid level size mismatch");
+    if (tdag_status s = ensureDeviceCopy(ctx, in); s != TDAG_OK)
+        return s;
+    tdag::LevelTuning tune[6];
+    contextTuning(ctx, tune);
+    hipError_t e = g->dt.init(in->dev, dag, midParents(dag, maxT, translationParents), maxT, tune);
+    if (e == hipSuccess)
+        e = g->dt.runEager(g->cfgBits, ctx->stream);
+    if (e == hipErrorInvalidValue)
+        return setErr(TDAG_EINVAL, "device translation stage: a child pointer or item outside its level");
+    if (e == hipSuccess)
+        e = g->dt.capture(g->cfgBits, ctx->stream);
+    if (e != hipSuccess)
+        return setHipErr(e, "device translation stage capture");
+    *out = g.release();
+    return TDAG_OK;
+}
+
+tdag_status tdag_build_graph_run(tdag_build_graph* g, float* out_ms)
+{
+    if (!g)
+        return setErr(TDAG_EINVAL, "null argument");
+    tdag_context* ctx = g->ctx;
+    if (hipError_t e = hipSetDevice(ctx->device); e != hipSuccess)
+        return setHipErr(e, "hipSetDevice");
+    hipError_t e = hipEventRecord(ctx->ev[0], ctx->stream);
+    if (e == hipSuccess)
+        e = g->dt.replay(ctx->stream);
+    if (e == hipSuccess)
+        e = hipEventRecord(ctx->ev[3], ctx->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess)
+        return setHipErr(e, "build graph replay");
+    if (out_ms)
+        (void)hipEventElapsedTime(out_ms, ctx->ev[0], ctx->ev[3]);
+    return TDAG_OK;
+}
+
+tdag_status tdag_build_graph_result(tdag_build_graph* g, tdag_tdag** out)
+{
+    if (!g || !out)
+        return setErr(TDAG_EINVAL, "null argument");
+    std::unique_ptr<tdag_tdag> res(new tdag_tdag());
+    res->cfg = g->cfg;
+    res->t = g->base;
+    if (hipError_t e = g->dt.readResult(res->t, g->ctx->stream); e != hipSuccess)
+        return setHipErr(e, "build graph readout");
+    translationRecords(g->dt, g->cfg.max_translation_level, res->records);
+    *out = res.release();
+    return TDAG_OK;
+}
+
+tdag_status tdag_build_graph_destroy(tdag_build_graph* g)
+{
+    if (g) {
+        (void)hipSetDevice(g->ctx->device);
+        delete g;
+    }
+    return TDAG_OK;
 }
 
 tdag_status tdag_svdag_static_words(const tdag_svdag* h, uint32_t level, void* out, uint64_t cap, uint64_t* out_count)

@@ -23,6 +23,7 @@ _u32p = ctypes.POINTER(ctypes.c_uint32)
 _u8p = ctypes.POINTER(ctypes.c_uint8)
 
 KEEP_LEVEL_INPUTS = 1
+DEVICE_TRANSLATION = 2  # TDAG_DEVICE_TRANSLATION
 POINTER_TABLES = 1  # TDAG16_POINTER_TABLES
 HUFFMAN = 2  # TDAG16_HUFFMAN
 
@@ -103,6 +104,12 @@ def bind(L):
     L.tdag_svdag_from_view.restype = ctypes.c_int
     L.tdag_tdag_copy_level.argtypes = [vp, ctypes.c_uint32, _u64p, ctypes.c_uint64, _u64p]
     L.tdag_tdag_copy_level.restype = ctypes.c_int
+    L.tdag_build_graph_create.argtypes = [vp, vp, ctypes.POINTER(TdagConfig), ctypes.POINTER(vp)]
+    L.tdag_build_graph_run.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
+    L.tdag_build_graph_result.argtypes = [vp, ctypes.POINTER(vp)]
+    L.tdag_build_graph_destroy.argtypes = [vp]
+    for name in ("tdag_build_graph_create", "tdag_build_graph_run", "tdag_build_graph_result", "tdag_build_graph_destroy"):
+        getattr(L, name).restype = ctypes.c_int
     L.tdag_huffman_code_lengths.argtypes = [_u64p, ctypes.c_uint32, ctypes.c_uint32, _u32p]
     L.tdag_huffman_code_words.argtypes = [_u32p, ctypes.c_uint32, ctypes.c_int, _u64p]
     for name in ("tdag_scene_dense", "tdag_svdag_from_dense", "tdag_svdag_info", "tdag_svdag_copy_level", "tdag_svdag_flatten",
@@ -319,15 +326,51 @@ def construct_transform_dag_learned(ctx, svdag: Svdag, config: TransformDAGConfi
     return TDag(h, config)
 
 
-def construct_transform_dag(ctx, svdag: Svdag, config: TransformDAGConfig, keep_level_inputs: bool = False) -> TDag:
+def construct_transform_dag(ctx, svdag: Svdag, config: TransformDAGConfig, keep_level_inputs: bool = False, device_translation: bool = False) -> TDag:
     """constructTransformDAG on the GPU matcher. ``ctx`` may be None when translation is off
-    (the hierarchical pass runs on the host and no matcher call is made)."""
+    (the hierarchical pass runs on the host and no matcher call is made). With
+    ``device_translation`` every step of the translation stage runs on the device
+    (TDAG_DEVICE_TRANSLATION)."""
     cfg = config.to_c()
     h = ctypes.c_void_p()
-    check(lib().tdag_construct_transform_dag(ctx._ptr if ctx is not None else None, svdag.handle, ctypes.byref(cfg),
-                                             KEEP_LEVEL_INPUTS if keep_level_inputs else 0, ctypes.byref(h)),
+    flags = (KEEP_LEVEL_INPUTS if keep_level_inputs else 0) | (DEVICE_TRANSLATION if device_translation else 0)
+    check(lib().tdag_construct_transform_dag(ctx._ptr if ctx is not None else None, svdag.handle, ctypes.byref(cfg), flags, ctypes.byref(h)),
           "tdag_construct_transform_dag")
     return TDag(h, config)
+
+
+class BuildGraph:
+    """A build's translation stage captured as one hipGraph (tdag_build_graph_*)."""
+
+    def __init__(self, ctx, svdag: Svdag, config: TransformDAGConfig):
+        self._svdag = svdag  # the graph reads the SVDAG's device copy
+        self._ctx = ctx
+        self.config = config
+        cfg = config.to_c()
+        self._h = ctypes.c_void_p()
+        check(lib().tdag_build_graph_create(ctx._ptr, svdag.handle, ctypes.byref(cfg), ctypes.byref(self._h)), "tdag_build_graph_create")
+
+    def run(self) -> float:
+        """Replays the stage; returns its device time in milliseconds."""
+        ms = ctypes.c_float()
+        check(lib().tdag_build_graph_run(self._h, ctypes.byref(ms)), "tdag_build_graph_run")
+        return float(ms.value)
+
+    def result(self) -> TDag:
+        h = ctypes.c_void_p()
+        check(lib().tdag_build_graph_result(self._h, ctypes.byref(h)), "tdag_build_graph_result")
+        return TDag(h, self.config)
+
+    def close(self):
+        if self._h:
+            lib().tdag_build_graph_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class StaticSvdag:

@@ -334,6 +334,11 @@ typedef struct tdag_level_record {
 
 /* Flag of tdag_construct_transform_dag: keep each level's matcher input masks. */
 #define TDAG_KEEP_LEVEL_INPUTS 1u
+/* Flag of tdag_construct_transform_dag: run the translation stage with every step on the device
+ * (referenced-node marking and compaction, flattening, matching, emission of the unique nodes,
+ * rewriting of the parents' pointers; levels gated by stream order); only each level's counts come
+ * back to the host. Same result. Needs a context; not with TDAG_KEEP_LEVEL_INPUTS. */
+#define TDAG_DEVICE_TRANSLATION 2u
 
 /* Seeded synthetic scene as a dense bit grid (bit i = x + R*y + R*R*z, the VoxelGrid layout of
  * voxel_grid.h:70-73). kind 0: 200 sphere shells + 100 box shells; kind 1: kind 0 plus a
@@ -400,6 +405,20 @@ tdag_status tdag_construct_transform_dag(tdag_context* ctx, const tdag_svdag* in
  * tdag_construct_transform_dag where a query's minimum-key parent was not proposed. */
 tdag_status tdag_construct_transform_dag_learned(tdag_context* ctx, const tdag_svdag* in, const tdag_config* cfg, uint32_t flags, tdag_mlp* mlp, uint32_t k,
     tdag_tdag** out);
+/* A build's translation stage as one captured hipGraph. create runs the device hierarchical pass,
+ * then the device translation stage once (learning each level's sizes), then captures the whole
+ * stage (every level's marking, compaction, flattening, matching, emission and pointer rewrites;
+ * no host synchronisation inside) into a hipGraph on ctx's stream. run replays it (out_ms: device
+ * time of the replay); result returns the structure of the last run, the same as
+ * tdag_construct_transform_dag's. The SVDAG and the context must outlive the graph. No reference
+ * counterpart: the reference rebuilds its matcher inputs on the host between levels
+ * (transform_dag.cpp:165-233). */
+typedef struct tdag_build_graph tdag_build_graph;
+tdag_status tdag_build_graph_create(tdag_context* ctx, const tdag_svdag* in, const tdag_config* cfg, tdag_build_graph** out);
+tdag_status tdag_build_graph_run(tdag_build_graph* g, float* out_ms);
+tdag_status tdag_build_graph_result(tdag_build_graph* g, tdag_tdag** out);
+tdag_status tdag_build_graph_destroy(tdag_build_graph* g);
+
 /* The .tdag byte stream (voxelizer/src/main.cpp:144-151): pass buf = NULL to query the size; cfg
  * must be the config the structure was built with (TDAG_EINVAL otherwise). The
  * header's structureType is 0 (Tree), as in the reference, whose output structure is

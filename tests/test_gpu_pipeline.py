@@ -157,3 +157,36 @@ def test_edit_structure_view_bytes_match_oracle(oracle, gpu_ctx, kind, R, seed, 
     s = pl.Svdag.from_view(R, root, nodes, leaves)
     mine = pl.construct_transform_dag(gpu_ctx, s, tdag_amd.TransformDAGConfig(True, True, True, maxT)).tdag_bytes()
     assert mine == _oracle_bytes(oracle, bits, R, oracle.config(True, True, True, maxT))
+
+
+@pytest.mark.parametrize("kind,R,seed,maxT", [(0, 64, 1, 2), (0, 64, 2, 5), (1, 128, 3, 3), (2, 256, 5, 4), (1, 256, 7, 5)])
+def test_device_translation_stage_bytes_match_oracle(oracle, gpu_ctx, kind, R, seed, maxT):
+    """TDAG_DEVICE_TRANSLATION (marking, compaction, flattening, matching, emission and pointer
+    rewrites all on the device) and the captured build graph, replayed three times with other GPU
+    work in between: the same .tdag bytes as the oracle's driver every time."""
+    import torch
+
+    import tdag_amd
+    from tdag_amd import pipeline as pl
+
+    print(f"device translation kind={kind} R={R} seed={seed} maxT={maxT}", flush=True)
+    bits = pl.scene_dense(kind, R, seed)
+    svdag = pl.Svdag(bits, R)
+    cfg = tdag_amd.TransformDAGConfig(True, True, True, maxT)
+    ref = _oracle_bytes(oracle, bits, R, oracle.config(True, True, True, maxT))
+    t = pl.construct_transform_dag(gpu_ctx, svdag, cfg, device_translation=True)
+    assert t.tdag_bytes() == ref
+    host = pl.construct_transform_dag(gpu_ctx, svdag, cfg)
+    for level in range(2, maxT + 1):
+        a, b = t.record(level), host.record(level)
+        assert (a.items, a.unique, a.matched) == (b.items, b.unique, b.matched), level
+    g = pl.BuildGraph(gpu_ctx, svdag, cfg)
+    try:
+        assert g.result().tdag_bytes() == ref
+        for replay in range(3):
+            ms = g.run()
+            assert ms > 0
+            assert g.result().tdag_bytes() == ref, f"replay {replay}"
+            _ = torch.ones(1 << 20, device="cuda").sum().item()
+    finally:
+        g.close()

@@ -215,3 +215,19 @@ def test_edit_structure_view_is_validated(oracle, pl):
     tree = pl.Svdag.from_view(64, root, nodes, leaves, structure_type=0)
     with pytest.raises(tdag_amd.TdagError, match="EINVAL.*DAG"):
         pl.construct_transform_dag(None, tree, tdag_amd.TransformDAGConfig(True, True, False, 3))
+
+
+def test_device_translation_flag_validation(pl):
+    """TDAG_DEVICE_TRANSLATION needs a context (checked before any device work, so on the CPU)."""
+    import tdag_amd
+
+    bits = pl.scene_dense(0, 64, 1)
+    svdag = pl.Svdag(bits, 64)
+    with pytest.raises(tdag_amd.TdagError, match="EINVAL.*context"):
+        pl.construct_transform_dag(None, svdag, tdag_amd.TransformDAGConfig(True, True, True, 3), device_translation=True)
+    with pytest.raises(tdag_amd.TdagError, match="EINVAL.*context"):
+        pl.construct_transform_dag(None, svdag, tdag_amd.TransformDAGConfig(True, True, False, 3), device_translation=True)
+    cfg = tdag_amd.TransformDAGConfig(True, True, False, 3).to_c()
+    h = ctypes.c_void_p()
+    assert tdag_amd.lib().tdag_construct_transform_dag(None, svdag.handle, ctypes.byref(cfg), 4, ctypes.byref(h)) != 0
+    assert "unknown flag" in tdag_amd.last_error()
