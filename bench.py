@@ -419,6 +419,14 @@ def bench_combined(ctx, cfg, dev, order, levels, info, args, plain_ms: float):
     st = learned.match_level_learned_device(ctx, mlp, d["masks"], n2, cfg, 64, d_encoded=d_enc)  # standalone learned call: the reference
     torch.cuda.synchronize()
     matched2 = int(st.num_matched)
+    # The same network with bucketed retrieval (candidates: earlier leaves of the query's exact
+    # transform-invariant descriptor), measured beside it.
+    mlp.retrieval = learned.RETRIEVAL_BUCKETED
+    d_enc_b = torch.empty(max(1, n2), dtype=torch.int64, device="cuda")
+    learned.match_level_learned_device(ctx, mlp, d["masks"], n2, cfg, 64, d_encoded=d_enc_b)  # warm-up (workspace)
+    st_b = learned.match_level_learned_device(ctx, mlp, d["masks"], n2, cfg, 64, d_encoded=d_enc_b)
+    torch.cuda.synchronize()
+    mlp.retrieval = learned.RETRIEVAL_ALL
 
     def run_combined(concurrent: bool):
         jobs, outs = [], {}
@@ -503,6 +511,15 @@ def bench_combined(ctx, cfg, dev, order, levels, info, args, plain_ms: float):
             "never_below_exhaustive_key": never_below,
             "learned_matcher_ms": round(st.ms_total, 4),
             "ms_split": {"embed": round(st.ms_embed, 4), "propose": round(st.ms_propose, 4), "verify": round(st.ms_verify, 4), "fallback": round(st.ms_fallback, 4)},
+            "bucketed_retrieval": {
+                "recall_at_64": round(st_b.proposal_hits / matched2, 4) if matched2 else None,
+                "same_parent_as_exhaustive": round(int((d_enc_b[:n2] == ref_enc[:n2]).sum().item()) / n2, 4) if n2 else None,
+                "never_below_exhaustive_key": bool(np.all(d_enc_b[:n2].cpu().numpy().view(np.uint64) >= ref_enc[:n2].cpu().numpy().view(np.uint64))),
+                "matched": int(st_b.num_matched),
+                "learned_matcher_ms": round(st_b.ms_total, 4),
+                "ms_split": {"embed": round(st_b.ms_embed, 4), "propose": round(st_b.ms_propose, 4), "verify": round(st_b.ms_verify, 4),
+                             "fallback": round(st_b.ms_fallback, 4)},
+            },
         },
         "tdag_bytes": learned_bytes,
         "tdag_bytes_exhaustive": info["tdag_bytes"],
@@ -678,6 +695,17 @@ def secondary_config1(args):
         matches, elapsed, rec = time_plain(ctx, cfg, dev, order, args.steps, args.warmup)
         per_level = per_level_summary(levels, order, rec)
         dom = max(order, key=lambda l: per_level[l]["search_ms"])
+        # Whole builds on this scene: a warm host-rewrite driver call against the device stage.
+        from tdag_amd import pipeline as pl
+
+        svdag = info["svdag"]
+        pl.construct_transform_dag(ctx, svdag, cfg).close()
+        tc = time.perf_counter()
+        t = pl.construct_transform_dag(ctx, svdag, cfg)
+        warm_ms = (time.perf_counter() - tc) * 1e3
+        ref = t.tdag_bytes()
+        t.close()
+        stage = device_translation_stage(ctx, svdag, cfg, ref)
         return {
             "workload": workload_name(sub),
             "value": matches / elapsed,
@@ -685,6 +713,8 @@ def secondary_config1(args):
             "ms_per_step": elapsed * 1e3 / max(1, args.steps),
             "matches_per_step": matches // max(1, args.steps),
             "tdag_bytes": info["tdag_bytes"],
+            "construct_transform_dag_warm_ms": round(warm_ms, 1),
+            "device_translation_stage": stage,
             "levels": per_level,
             "tuning_source": source,
             "roofline": roofline(sub, dom, per_level[dom]["search_ms"], levels[dom]["items"], tuning, ctx.get_variant(dom)),

@@ -124,6 +124,40 @@ tdag_status tdag_mlp_topk(tdag_context* ctx, const uint16_t* d_emb, uint64_t n, 
     return TDAG_OK;
 }
 
+tdag_status tdag_mlp_set_retrieval(tdag_mlp* m, uint32_t mode)
+{
+    if (!m)
+        return fail(TDAG_EINVAL, "null argument");
+    if (mode != TDAG_RETRIEVAL_ALL && mode != TDAG_RETRIEVAL_BUCKETED)
+        return fail(TDAG_EINVAL, "unknown retrieval mode " + std::to_string(mode));
+    m->mlp.retrieval = mode;
+    return TDAG_OK;
+}
+
+tdag_status tdag_mlp_get_retrieval(const tdag_mlp* m, uint32_t* out_mode)
+{
+    if (!m || !out_mode)
+        return fail(TDAG_EINVAL, "null argument");
+    *out_mode = m->mlp.retrieval;
+    return TDAG_OK;
+}
+
+tdag_status tdag_mlp_topk_bucketed(tdag_context* ctx, const uint64_t* d_masks, const uint16_t* d_emb, uint64_t n, uint32_t k, uint32_t* d_cand)
+{
+    if (!ctx || (n > 0 && (!d_masks || !d_emb || !d_cand)))
+        return fail(TDAG_EINVAL, "null argument");
+    if (k == 0 || k > tdag::learn::kMaxProposals)
+        return fail(TDAG_EINVAL, "k must be in [1, 64]");
+    if (n >= (1ull << 31))
+        return fail(TDAG_EINVAL, "n must be below 2^31");
+    CAPI_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    if (tdag_status s = ensureBuffer(ctx->scratch, ctx->scratchBytes, tdag::learn::bucketWorkBytes(n), "retrieval workspace allocation"); s != TDAG_OK)
+        return s;
+    CAPI_HIP(tdag::learn::bucketPropose(d_masks, d_emb, (uint32_t)n, k, d_cand, ctx->scratch, ctx->stream), "bucketed retrieval");
+    CAPI_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return TDAG_OK;
+}
+
 tdag_status tdag_match_level_learned_device(tdag_context* ctx, tdag_mlp* mlp, uint32_t level, const uint64_t* d_masks, uint64_t n, const tdag_config* cfg, uint32_t k,
     uint64_t* d_encoded, uint64_t* d_mapping, uint32_t* d_unique, tdag_learned_stats* stats)
 {

@@ -5,7 +5,7 @@
 //    and its reverse, the three sorted: invariant under the 48 mirror/permutation transforms and
 //    unclipped translations), as bf16 rows of 80 (zero-padded).
 //  * MLP 80 -> 256 -> 256 -> 256 -> 128 (ReLU), L2-normalised. Every dense layer, forward and
-//    backward, is one bf16 MFMA GEMM (v_mfma_f32_16x16x16_bf16, fp32 accumulate, 64x64 tiles).
+//    backward, is one bf16 MFMA GEMM (v_mfma_f32_16x16x32_bf16, fp32 accumulate, 64x64 tiles).
 //  * Training: InfoNCE over (query, matched parent) pairs emitted by the exact matcher, Adam on
 //    fp32 master weights; the similarity/softmax part runs in fp32 on the vector units.
 //  * Matching: proposals = the k most similar earlier-ranked leaves (bf16 MFMA scores, per-query
@@ -33,8 +33,15 @@ namespace {
 
 using namespace dev;
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 mfma_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// gfx950's double-K bf16 MFMA: D (16 x 16, fp32) += A (16 x 32) B (32 x 16).
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(mfma_bf16x8, a), __builtin_bit_cast(mfma_bf16x8, b), c, 0, 0, 0);
+}
 
 #define LEARN_TRY(expr)                  \
     do {                                 \
@@ -82,16 +89,12 @@ __device__ __forceinline__ bool lexLess(const uint32_t* a, const uint32_t* b)
     return false;
 }
 
-__global__ __launch_bounds__(256) void kFeatures(const uint64_t* __restrict__ masks, uint32_t n, uint16_t* __restrict__ X)
+// The transform-invariant descriptor of a 4^3 mask: per axis, the occupancy histogram trimmed of
+// zero ends, left-aligned, and the smaller (lexicographically) of it and its reverse; the three
+// histograms sorted. Invariant under the 48 permutations and mirrors, and under translations that
+// lose no voxel.
+__device__ __forceinline__ void canonicalHists(uint64_t m, uint32_t h[3][4])
 {
-    const uint32_t r = blockIdx.x * 256u + threadIdx.x;
-    if (r >= n)
-        return;
-    const uint64_t m = masks[r];
-    uint16_t* row = X + (size_t)r * kIn;
-    for (int i = 0; i < 64; ++i)
-        row[i] = ((m >> i) & 1ull) ? 0x3F80u : 0u; // 1.0 or 0.0
-    uint32_t h[3][4];
     for (int a = 0; a < 3; ++a) {
         uint32_t raw[4];
         axisHist(m, a, raw);
@@ -101,7 +104,6 @@ __global__ __launch_bounds__(256) void kFeatures(const uint64_t* __restrict__ ma
                 lo = lo < v ? lo : v;
                 hi = v;
             }
-        // span trimmed of zero ends, left-aligned, forwards and reversed; keep the smaller
         uint32_t fwd[4] = { 0, 0, 0, 0 }, rev[4] = { 0, 0, 0, 0 };
         for (int v = lo; v <= hi; ++v) {
             fwd[v - lo] = raw[v];
@@ -123,6 +125,19 @@ __global__ __launch_bounds__(256) void kFeatures(const uint64_t* __restrict__ ma
     swapIf(0, 1);
     swapIf(1, 2);
     swapIf(0, 1);
+}
+
+__global__ __launch_bounds__(256) void kFeatures(const uint64_t* __restrict__ masks, uint32_t n, uint16_t* __restrict__ X)
+{
+    const uint32_t r = blockIdx.x * 256u + threadIdx.x;
+    if (r >= n)
+        return;
+    const uint64_t m = masks[r];
+    uint16_t* row = X + (size_t)r * kIn;
+    for (int i = 0; i < 64; ++i)
+        row[i] = ((m >> i) & 1ull) ? 0x3F80u : 0u; // 1.0 or 0.0
+    uint32_t h[3][4];
+    canonicalHists(m, h);
     for (int a = 0; a < 3; ++a)
         for (int i = 0; i < 4; ++i)
             row[64 + 4 * a + i] = f2bf((float)h[a][i] * (1.0f / 16.0f));
@@ -143,27 +158,27 @@ __global__ __launch_bounds__(256) void kGatherPairs(const uint64_t* __restrict__
 // ---------------------------------------------------------------------------------------
 // bf16 MFMA GEMM: C[M x N] (fp32) = opA[M x K] * opB[K x N].
 //   opA[m][k] = TA ? A[k * lda + m] : A[m * lda + k];  opB[k][n] = TB ? B[n * ldb + k] : B[k * ldb + n].
-// 256 threads per 64x64 tile of C; waves in 2x2, each 32x32 = 2x2 v_mfma_f32_16x16x16_bf16 tiles.
-// Each 16-deep K slice of both operands is staged in LDS (B transposed so a lane's four k values
+// 256 threads per 64x64 tile of C; waves in 2x2, each 32x32 = 2x2 v_mfma_f32_16x16x32_bf16 tiles.
+// Each 32-deep K slice of both operands is staged in LDS (B transposed so a lane's eight k values
 // are contiguous); out-of-range elements read as 0 and out-of-range C is not written.
-// MFMA 16x16x16 operand layout: lane l holds A[l % 16][4 (l / 16) + i] and B[4 (l / 16) + i][l % 16]
-// for i < 4, and receives D[4 (l / 16) + i][l % 16].
+// MFMA 16x16x32 operand layout: lane l holds A[l % 16][8 (l / 16) + i] and B[8 (l / 16) + i][l % 16]
+// for i < 8, and receives D[4 (l / 16) + i][l % 16] for i < 4.
 // ---------------------------------------------------------------------------------------
 constexpr int kTile = 64;
-constexpr int kKs = 16;
+constexpr int kKs = 32;
 constexpr int kLdsPitch = kKs + 4; // bf16 elements per staged row (padding against bank conflicts)
 
-// Stages one operand's 64 x 16 slice (rows r0.., k0..) into s[row][k]. RowFast: the row index
+// Stages one operand's 64 x 32 slice (rows r0.., k0..) into s[row][k]. RowFast: the row index
 // is contiguous in memory (element (row, k) at p[k * ld + row]); thread t takes row t % 64 and
-// k = 4 (t / 64) + i, so a wave reads 64 consecutive elements. Otherwise k is contiguous
-// (p[row * ld + k]) and thread t takes row t / 4 and k = 4 (t % 4) + i.
+// k = 8 (t / 64) + i, so a wave reads 64 consecutive elements. Otherwise k is contiguous
+// (p[row * ld + k]) and thread t takes row t / 4 and k = 8 (t % 4) + i.
 template <bool RowFast>
 __device__ __forceinline__ void stageSlice(const uint16_t* __restrict__ p, int ld, int r0, int rows, int k0, int kEnd, uint16_t (*sm)[kLdsPitch], int t)
 {
     const int row = RowFast ? (t & 63) : (t >> 2);
-    const int kb = RowFast ? 4 * (t >> 6) : 4 * (t & 3);
+    const int kb = RowFast ? 8 * (t >> 6) : 8 * (t & 3);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < 8; ++i) {
         const int r = r0 + row, k = k0 + kb + i;
         uint16_t v = 0;
         if (r < rows && k < kEnd)
@@ -194,17 +209,19 @@ __global__ __launch_bounds__(256) void kGemm(const uint16_t* __restrict__ A, int
         stageSlice<TA>(A, lda, m0, M, k0, kEnd, sA, t);
         stageSlice<!TB>(B, ldb, n0, N, k0, kEnd, sB, t);
         __syncthreads();
-        const int kq = 4 * (lane >> 4);
-        bf16x4 a[2], b[2];
+        const int kq = 8 * (lane >> 4);
+        bf16x8 a[2], b[2];
         for (int i = 0; i < 2; ++i) {
             const uint16_t* pa = &sA[wr * 32 + i * 16 + (lane & 15)][kq];
             const uint16_t* pb = &sB[wc * 32 + i * 16 + (lane & 15)][kq];
-            a[i] = bf16x4 { (short)pa[0], (short)pa[1], (short)pa[2], (short)pa[3] };
-            b[i] = bf16x4 { (short)pb[0], (short)pb[1], (short)pb[2], (short)pb[3] };
+            for (int e = 0; e < 8; ++e) {
+                a[i][e] = (short)pa[e];
+                b[i][e] = (short)pb[e];
+            }
         }
         for (int i = 0; i < 2; ++i)
             for (int j = 0; j < 2; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = mfma32(a[i], b[j], acc[i][j]);
         __syncthreads();
     }
     for (int i = 0; i < 2; ++i)
@@ -669,19 +686,16 @@ __device__ __forceinline__ uint64_t mergeReversed(uint64_t q, uint64_t lRev, uin
 }
 
 // Loads a lane's 32 embedding values of one row. The 128 dimensions are split as
-// k = 32 g + 4 s + i for lane group g = lane / 16, MFMA step s < 8 and element i < 4: both
-// operands use the same split, so the 8 steps sum the full dot product, and each lane reads 64
-// contiguous bytes per row.
-typedef short bf16x8 __attribute__((ext_vector_type(8)));
-__device__ __forceinline__ void loadFrag(const uint16_t* __restrict__ Ebf, uint32_t row, uint32_t g, bf16x4 (&f)[8])
+// k = 32 g + 8 s + i for lane group g = lane / 16, MFMA step s < 4 and element i < 8: both
+// operands use the same split, so the 4 steps of v_mfma_f32_16x16x32_bf16 (lane l supplies
+// elements 8 (l / 16) .. + 8 of the 32-deep slice for row or column l % 16) sum the full dot
+// product, and each lane reads 64 contiguous bytes per row.
+__device__ __forceinline__ void loadFrag(const uint16_t* __restrict__ Ebf, uint32_t row, uint32_t g, bf16x8 (&f)[4])
 {
     const bf16x8* p = reinterpret_cast<const bf16x8*>(Ebf + (size_t)row * kEmb + 32u * g);
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        const bf16x8 x = p[v];
-        f[2 * v] = x.lo;
-        f[2 * v + 1] = x.hi;
-    }
+    for (int v = 0; v < 4; ++v)
+        f[v] = p[v];
 }
 
 // Buffered survivors (lane i < count holds one, others 0) merged into a queue; the rare path of
@@ -717,7 +731,7 @@ __device__ __forceinline__ uint64_t readLane64(uint64_t v, int l)
 }
 
 // One block per (query block, chunk): 256 threads, waves in 2 x 2 over the 64 x 64 score tile,
-// each wave 32 x 32 = 2 x 2 MFMA 16x16x16 tiles. The wave's query fragments stay in registers;
+// each wave 32 x 32 = 2 x 2 MFMA 16x16x32 tiles. The wave's query fragments stay in registers;
 // the next candidate tile's fragments are loaded while the current one is filtered. Scores go to
 // LDS; wave w then owns queries 16 w + i, i < 16: their queues live in registers (lane = entry),
 // their thresholds (the queue's last key) and buffer fill counts are wave-uniform. Lane j takes
@@ -767,7 +781,7 @@ __global__ __launch_bounds__(256) void kTopKPartial(const uint16_t* __restrict__
         thrScore[i] = -INFINITY;
         cnt[i] = 0u;
     }
-    bf16x4 a[2][8], b[2][8];
+    bf16x8 a[2][4], b[2][4];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
         loadFrag(Ebf, min(q0 + wr * 32 + i * 16 + (lane & 15), n - 1), g, a[i]);
@@ -783,12 +797,12 @@ __global__ __launch_bounds__(256) void kTopKPartial(const uint16_t* __restrict__
             for (int j = 0; j < 2; ++j)
                 acc[i][j] = f32x4 { 0.f, 0.f, 0.f, 0.f };
 #pragma unroll
-        for (int st = 0; st < 8; ++st)
+        for (int st = 0; st < 4; ++st)
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
                 for (int j = 0; j < 2; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i][st], b[j][st], acc[i][j], 0, 0, 0);
+                    acc[i][j] = mfma32(a[i][st], b[j][st], acc[i][j]);
         // The shared thresholds (lane i < 16: query 16 w + i; the address is clamped so the load
         // is unconditional) are loaded before the next tile's prefetch: loads complete in order,
         // so waiting for them does not wait for the prefetch.
@@ -890,6 +904,84 @@ __global__ __launch_bounds__(256) void kNeighbourProposals(uint32_t n, uint32_t 
         return;
     const uint32_t q = (uint32_t)(i / k), s = (uint32_t)(i % k);
     cand[i] = s < q ? q - 1 - s : 0xFFFFFFFFu;
+}
+
+// Bucketed retrieval: the candidates of query q are the earlier leaves with q's exact descriptor
+// (canonicalHists, 12 entries of at most 16, packed 5 bits each: 60 bits), and among them the k
+// best under kTopKPartial's order (higher score, then lower rank). A match that loses no voxel to
+// the clipping translation has the query's descriptor, so a bucket holds every match of equal
+// popcount; matches that clip voxels are left to the fallback search. The work is the sum of the
+// squared bucket sizes instead of n^2 / 2.
+__device__ __forceinline__ uint64_t descriptorKey(uint64_t m)
+{
+    uint32_t h[3][4];
+    canonicalHists(m, h);
+    uint64_t key = 0;
+    for (int a = 0; a < 3; ++a)
+        for (int i = 0; i < 4; ++i)
+            key = (key << 5) | h[a][i];
+    return key;
+}
+
+__global__ __launch_bounds__(256) void kDescriptorKeys(const uint64_t* __restrict__ sortedMasks, uint32_t n, uint64_t* __restrict__ keys, uint32_t* __restrict__ vals)
+{
+    const uint32_t r = blockIdx.x * 256u + threadIdx.x;
+    if (r >= n)
+        return;
+    keys[r] = descriptorKey(sortedMasks[r]);
+    vals[r] = r;
+}
+
+// One wave per sorted position i (keys sorted, ties in rank order, so the bucket's earlier
+// leaves are positions [start, i) with start the bucket's first position). Lane j scores
+// candidate j of each chunk of 64 (fp32 dot product of the bf16 rows; the query row is in LDS),
+// and the chunk is sorted and merged into the wave's queue (lane = entry, descending).
+__global__ __launch_bounds__(256) void kTopKBucket(const uint16_t* __restrict__ Ebf, const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals, uint32_t n,
+    uint32_t k, uint32_t* __restrict__ cand)
+{
+    __shared__ float s_q[4][kEmb];
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t i = blockIdx.x * 4u + wave;
+    if (i >= n)
+        return; // the whole wave: the kernel has no block barrier
+    const uint32_t q = vals[i];
+    const uint64_t key = keys[i];
+    uint32_t lo = 0, hi = i; // first position of the bucket
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (keys[mid] < key)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    for (uint32_t d = lane; d < (uint32_t)kEmb; d += 64)
+        s_q[wave][d] = __uint_as_float((uint32_t)Ebf[(size_t)q * kEmb + d] << 16);
+    ldsWaveSync();
+    uint64_t queue = 0;
+    for (uint32_t j0 = lo; j0 < i; j0 += 64) {
+        const uint32_t j = j0 + lane;
+        uint64_t kk = 0;
+        if (j < i) {
+            const uint32_t c = vals[j];
+            const uint4* row = reinterpret_cast<const uint4*>(Ebf + (size_t)c * kEmb);
+            float acc = 0.f;
+#pragma unroll 4
+            for (int v = 0; v < kEmb / 8; ++v) {
+                const uint4 x = row[v];
+                const uint32_t w[4] = { x.x, x.y, x.z, x.w };
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    acc = fmaf(__uint_as_float(w[e] << 16), s_q[wave][8 * v + 2 * e], acc);
+                    acc = fmaf(__uint_as_float(w[e] & 0xFFFF0000u), s_q[wave][8 * v + 2 * e + 1], acc);
+                }
+            }
+            kk = scoreKey(acc, c);
+        }
+        const uint64_t sorted = bitonicSortDesc(kk, lane);
+        queue = mergeReversed(queue, shflXor64(sorted, 63), lane);
+    }
+    if (lane < k)
+        cand[(size_t)q * k + lane] = 0xFFFFFFFFu - (uint32_t)queue;
 }
 
 // Lowest / highest occupied coordinate of a non-empty 4^3 mask along each axis.
@@ -1145,12 +1237,34 @@ hipError_t topkPropose(const uint16_t* Ebf, uint32_t n, uint32_t k, uint32_t* ca
     return hipGetLastError();
 }
 
+size_t bucketWorkBytes(uint64_t n)
+{
+    return align256(n * 8) + align256(n * 4) + align256(radixScratchBytes(std::max<uint64_t>(n, 1), 8));
+}
+
+hipError_t bucketPropose(const uint64_t* sortedMasks, const uint16_t* Ebf, uint32_t n, uint32_t k, uint32_t* cand, void* work, hipStream_t s)
+{
+    if (n == 0)
+        return hipSuccess;
+    if (k == 0 || k > kMaxProposals)
+        return hipErrorInvalidValue;
+    uint64_t* keys = (uint64_t*)work;
+    uint32_t* vals = (uint32_t*)((char*)work + align256((size_t)n * 8));
+    void* sortScratch = (char*)vals + align256((size_t)n * 4);
+    hipLaunchKernelGGL(kDescriptorKeys, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, s, sortedMasks, n, keys, vals);
+    LEARN_TRY(hipGetLastError());
+    LEARN_TRY(radixSortPairsU64(keys, vals, n, 0, 60, sortScratch, s));
+    hipLaunchKernelGGL(kTopKBucket, dim3((unsigned)cdiv(n, 4)), dim3(256), 0, s, Ebf, keys, vals, n, k, cand);
+    return hipGetLastError();
+}
+
 size_t learnedWorkBytes(uint64_t n)
 {
     // proposals, flags, bf16 embeddings, counters, 48 images and boxes per leaf, the retrieval's
-    // per-chunk lists (the embedding workspace itself lives in the Mlp)
+    // workspace (per-chunk lists, or the bucket sort) (the embedding workspace itself lives in
+    // the Mlp)
     return align256(n * kMaxProposals * 4) + align256(n * 4) + align256((n + kQ) * kEmb * 2) + 256 + align256(n * 48 * 8) + align256(n * 48 * 2) +
-        align256(n * 8) + align256(n * kEmb * 4) + topkWorkBytes(n);
+        align256(n * 8) + align256(n * kEmb * 4) + std::max(topkWorkBytes(n), bucketWorkBytes(n));
 }
 
 hipError_t mlpPrepareEmbed(Mlp& mlp, uint64_t n)
@@ -1351,7 +1465,10 @@ hipError_t enqueueLearned2(const MatchPlan& p, Mlp* mlp, uint32_t k, const uint6
             hipLaunchKernelGGL(kCastBf, dim3((unsigned)cdiv((size_t)n * kEmb, 256)), dim3(256), 0, s, E, (size_t)n * kEmb, Ebf);
             LEARN_TRY(hipGetLastError());
             mark(1);
-            LEARN_TRY(topkPropose(Ebf, n, k, cand, topkWork, s));
+            if (mlp->retrieval == kRetrieveBucketed)
+                LEARN_TRY(bucketPropose(v.sortedMasks, Ebf, n, k, cand, topkWork, s));
+            else
+                LEARN_TRY(topkPropose(Ebf, n, k, cand, topkWork, s));
         } else {
             mark(1);
             hipLaunchKernelGGL(kNeighbourProposals, dim3((unsigned)cdiv((size_t)n * k, 256)), dim3(256), 0, s, n, k, cand);

@@ -34,7 +34,10 @@ constexpr size_t kParams = paramOffsetW(kLayers);
 // Device state of one network: fp32 master parameters (W_l row-major in x out, then b_l), their
 // bf16 copy used by the MFMA kernels, gradients and Adam moments, plus a workspace that grows
 // with the largest batch seen.
+constexpr uint32_t kRetrieveAll = 0; // TDAG_RETRIEVAL_ALL
+constexpr uint32_t kRetrieveBucketed = 1; // TDAG_RETRIEVAL_BUCKETED
 struct Mlp {
+    uint32_t retrieval = kRetrieveAll; // how the matcher retrieves its proposals
     float* w = nullptr;
     uint16_t* wbf = nullptr;
     float* g = nullptr;
@@ -88,6 +91,11 @@ const unsigned long long* learnedCounters(const MatchPlan& plan, void* d_work);
 // first, into cand[q * k ..] (0xFFFFFFFF where fewer exist). `work` holds topkWorkBytes(n) bytes.
 size_t topkWorkBytes(uint64_t n);
 hipError_t topkPropose(const uint16_t* Ebf, uint32_t n, uint32_t k, uint32_t* cand, void* work, hipStream_t stream);
+// Bucketed retrieval: as topkPropose, with the candidates of each leaf restricted to the earlier
+// leaves of its exact transform-invariant descriptor (sortedMasks: the n leaves in rank order).
+// `work` holds bucketWorkBytes(n) bytes.
+size_t bucketWorkBytes(uint64_t n);
+hipError_t bucketPropose(const uint64_t* sortedMasks, const uint16_t* Ebf, uint32_t n, uint32_t k, uint32_t* cand, void* work, hipStream_t stream);
 
 // Test hook: C = opA x opB with the bf16 MFMA kernel (row-major storage; TA/TB transpose).
 hipError_t gemmBf16(bool ta, bool tb, const uint16_t* A, int lda, const uint16_t* B, int ldb, float* C, int ldc, int M, int N, int K, hipStream_t stream);

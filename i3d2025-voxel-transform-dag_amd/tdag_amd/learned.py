@@ -37,6 +37,10 @@ class TdagLearnedStats(ctypes.Structure):
 _bound = False
 
 
+RETRIEVAL_ALL = 0  # TDAG_RETRIEVAL_ALL
+RETRIEVAL_BUCKETED = 1  # TDAG_RETRIEVAL_BUCKETED
+
+
 def _bind():
     global _bound, PARAMS
     if _bound:
@@ -53,9 +57,13 @@ def _bind():
     L.tdag_mlp_embed.argtypes = [vp, vp, u64, vp]
     L.tdag_gemm_bf16.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.tdag_mlp_topk.argtypes = [vp, vp, u64, u32, vp]
+    L.tdag_mlp_topk_bucketed.argtypes = [vp, vp, vp, u64, u32, vp]
+    L.tdag_mlp_set_retrieval.argtypes = [vp, u32]
+    L.tdag_mlp_get_retrieval.argtypes = [vp, ctypes.POINTER(u32)]
     L.tdag_match_level_learned_device.argtypes = [vp, vp, u32, vp, u64, ctypes.POINTER(TdagConfig), u32, vp, vp, vp, ctypes.POINTER(TdagLearnedStats)]
     for name in ("tdag_mlp_create", "tdag_mlp_destroy", "tdag_mlp_get_params", "tdag_mlp_set_params", "tdag_mlp_loss_grad", "tdag_mlp_train", "tdag_mlp_embed",
-                 "tdag_gemm_bf16", "tdag_mlp_topk", "tdag_match_level_learned_device"):
+                 "tdag_gemm_bf16", "tdag_mlp_topk", "tdag_mlp_topk_bucketed", "tdag_mlp_set_retrieval", "tdag_mlp_get_retrieval",
+                 "tdag_match_level_learned_device"):
         getattr(L, name).restype = ctypes.c_int
     PARAMS = int(L.tdag_mlp_param_count())
     _bound = True
@@ -123,6 +131,18 @@ class MLP:
     def embed(self, d_masks, n: int, d_out) -> None:
         check(lib().tdag_mlp_embed(self._ptr, _ptr(d_masks), int(n), _ptr(d_out)), "tdag_mlp_embed")
 
+    @property
+    def retrieval(self) -> int:
+        """RETRIEVAL_ALL (the k best of all earlier leaves) or RETRIEVAL_BUCKETED (the k best of the
+        earlier leaves with the query's transform-invariant descriptor)."""
+        mode = ctypes.c_uint32()
+        check(lib().tdag_mlp_get_retrieval(self._ptr, ctypes.byref(mode)), "tdag_mlp_get_retrieval")
+        return int(mode.value)
+
+    @retrieval.setter
+    def retrieval(self, mode: int) -> None:
+        check(lib().tdag_mlp_set_retrieval(self._ptr, int(mode)), "tdag_mlp_set_retrieval")
+
 
 def gemm_bf16(ctx, ta: bool, tb: bool, d_a, lda: int, d_b, ldb: int, d_c, ldc: int, m: int, n: int, k: int) -> None:
     L = _bind()
@@ -134,6 +154,14 @@ def topk(ctx, d_emb, n: int, k: int, d_cand) -> None:
     by dot product, ties to the lower row, best first (0xFFFFFFFF where fewer exist)."""
     L = _bind()
     check(L.tdag_mlp_topk(ctx._ptr, _ptr(d_emb), int(n), int(k), _ptr(d_cand)), "tdag_mlp_topk")
+
+
+def topk_bucketed(ctx, d_masks, d_emb, n: int, k: int, d_cand) -> None:
+    """Bucketed retrieval step: as topk, restricted to the earlier rows whose masks (d_masks, n
+    4^3 masks in rank order) have the query's transform-invariant descriptor; scores are fp32 sums
+    of the bf16 products in dimension order."""
+    L = _bind()
+    check(L.tdag_mlp_topk_bucketed(ctx._ptr, _ptr(d_masks), _ptr(d_emb), int(n), int(k), _ptr(d_cand)), "tdag_mlp_topk_bucketed")
 
 
 def match_level_learned_device(ctx, mlp: MLP | None, d_masks, n: int, config: TransformDAGConfig | None = None, k: int = 64, d_encoded=None,

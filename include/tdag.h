@@ -300,6 +300,20 @@ tdag_status tdag_gemm_bf16(tdag_context* ctx, int trans_a, int trans_b, const ui
  * "higher dot product first (bf16 MFMA, fp32 accumulate), then lower row", best first, into
  * d_cand (device, n x k; 0xFFFFFFFF where fewer than k earlier rows exist). */
 tdag_status tdag_mlp_topk(tdag_context* ctx, const uint16_t* d_emb, uint64_t n, uint32_t k, uint32_t* d_cand);
+/* How a network's matcher retrieves its proposals. ALL: the k best of all earlier leaves
+ * (tdag_mlp_topk). BUCKETED: the k best of the earlier leaves whose transform-invariant
+ * descriptor equals the query's (tdag_mlp_topk_bucketed): every match of equal popcount lies in
+ * the query's bucket, matches that clip voxels are left to the fallback search, and the work is
+ * the sum of the squared bucket sizes instead of n^2 / 2. No reference counterpart. */
+#define TDAG_RETRIEVAL_ALL 0u
+#define TDAG_RETRIEVAL_BUCKETED 1u
+tdag_status tdag_mlp_set_retrieval(tdag_mlp* m, uint32_t mode);
+tdag_status tdag_mlp_get_retrieval(const tdag_mlp* m, uint32_t* out_mode);
+/* The bucketed retrieval step, exposed for tests: d_masks (device, n 4^3 masks in rank order) and
+ * their bf16 embeddings d_emb (as tdag_mlp_topk); for each row the k (<= 64) best earlier rows
+ * with the same descriptor under "higher dot product first (fp32 sum of the bf16 products in
+ * dimension order), then lower row", best first, into d_cand (0xFFFFFFFF where fewer exist). */
+tdag_status tdag_mlp_topk_bucketed(tdag_context* ctx, const uint64_t* d_masks, const uint16_t* d_emb, uint64_t n, uint32_t k, uint32_t* d_cand);
 /* Level-2 matcher with proposals from `mlp` (NULL: the k leaves ranked just before each query,
  * i.e. popcount-neighbour proposals without a network). Outputs as tdag_match_level_device. */
 tdag_status tdag_match_level_learned_device(tdag_context* ctx, tdag_mlp* mlp, uint32_t level, const uint64_t* d_masks, uint64_t n, const tdag_config* cfg, uint32_t k,

@@ -357,3 +357,88 @@ def test_learned_driver_decodes_exactly(oracle, gpu_ctx):
     assert np.array_equal(tdag_decode.expand(d), tdag_decode.dense_to_cube(bits, R))
     assert len(mine) <= 1.05 * len(exact)
     mlp.close()
+
+
+def _bucket_masks(rng, n):
+    """4^3 masks with large descriptor buckets: single voxels and voxel pairs at random places
+    (a few descriptors shared by hundreds of masks) mixed with random masks."""
+    out = np.zeros(n, dtype=np.uint64)
+    for i in range(n):
+        u = rng.random()
+        if u < 0.4:
+            out[i] = np.uint64(1) << np.uint64(rng.integers(64))
+        elif u < 0.7:
+            a, b = rng.choice(64, 2, replace=False)
+            out[i] = (np.uint64(1) << np.uint64(a)) | (np.uint64(1) << np.uint64(b))
+        else:
+            bits = rng.random(64) < 0.3
+            out[i] = np.uint64(sum(1 << j for j in range(64) if bits[j]) or 1)
+    return out
+
+
+def _topk_bucketed_reference(masks, E, k):
+    """For each row q, the k rows c < q with q's descriptor (the oracle's) and the highest
+    <E_q, E_c>, ties to the lower row, best first. E holds small integers: exact products."""
+    n = len(masks)
+    desc = [mo.descriptor(int(m)).tobytes() for m in masks]
+    out = np.full((n, k), 0xFFFFFFFF, dtype=np.uint32)
+    Ef = E.astype(np.int64)
+    members = {}
+    for q in range(n):
+        prev = members.setdefault(desc[q], [])
+        if prev:
+            c = np.array(prev, dtype=np.int64)
+            s = Ef[c] @ Ef[q]
+            order = np.lexsort((c, -s))[:k]  # higher score first, then lower row
+            out[q, : len(order)] = c[order]
+        prev.append(q)
+    return out
+
+
+@pytest.mark.parametrize("n,k", [(1, 64), (70, 64), (3000, 64), (3000, 5)])
+def test_topk_bucketed_exact(gpu_ctx, n, k):
+    """Bucketed retrieval returns exactly the k best earlier rows of the query's descriptor bucket
+    (buckets of hundreds of rows span several 64-candidate chunks per query)."""
+    import torch
+
+    from tdag_amd import learned
+
+    rng = np.random.default_rng(n * 7 + k)
+    masks = _bucket_masks(rng, n)
+    E = (rng.random((n, 128)) < 0.08) * rng.choice([-1, 1], size=(n, 128))
+    Eb = _bf16_bits(E.astype(np.float32))
+    d_m = torch.from_numpy(masks.view(np.int64).copy()).cuda()
+    d_e = torch.from_numpy(Eb.view(np.int16).copy()).cuda()
+    d_c = torch.zeros(n * k, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    learned.topk_bucketed(gpu_ctx, d_m, d_e, n, k, d_c)
+    got = d_c.cpu().numpy().view(np.uint32).reshape(n, k)
+    ref = _topk_bucketed_reference(masks, E, k)
+    bad = np.nonzero((got != ref).any(axis=1))[0]
+    assert bad.size == 0, f"{bad.size} queries differ, first {bad[:5]}: got {got[bad[0]][:8]} want {ref[bad[0]][:8]}"
+
+
+def test_learned_matcher_bucketed_exact(oracle, gpu_ctx):
+    """The learned matcher with bucketed retrieval: every accepted match exact, never below the
+    exhaustive key, unmatched leaves unmatched; on leaves whose matches lose no voxel (one
+    popcount, copies translated without clipping) the buckets hold the parents, so the
+    proposals resolve most queries; and the results are deterministic."""
+    from tdag_amd import learned
+
+    masks, enc, q, p = _planted_pairs(oracle, 6, 400, 800)
+    mlp = learned.MLP(gpu_ctx, seed=4)
+    try:
+        mlp.retrieval = learned.RETRIEVAL_BUCKETED
+        assert mlp.retrieval == learned.RETRIEVAL_BUCKETED
+        enc_l, st = _run_learned(gpu_ctx, mlp, masks)
+        print(f"bucketed planted stats={st.as_dict()}", flush=True)
+        _check_learned_result(oracle, masks, enc_l, enc)
+        same = _fixed_popcount_set(oracle, 8, 1500, 1500)
+        enc_s = oracle.match_encoded(2, same, oracle.config(True, True, True))
+        runs = [_run_learned(gpu_ctx, mlp, same) for _ in range(2)]
+        print(f"bucketed fixed-popcount stats={runs[0][1].as_dict()}", flush=True)
+        _check_learned_result(oracle, same, runs[0][0], enc_s)
+        assert np.array_equal(runs[0][0], runs[1][0])
+        assert runs[0][1].proposal_hits >= 0.9 * runs[0][1].num_matched
+    finally:
+        mlp.close()
