@@ -183,6 +183,33 @@ __global__ __launch_bounds__(256) void kLayouts(uint64_t* __restrict__ layouts, 
         dst[i] = (Row)a[i];
 }
 
+// Shape bitmap of the children (the search's pre-filter, ShapeLds): one wave per item reads
+// its layout-0 rows, derives its occupancy along x, y and z, and sets the bits of its extent
+// triple in all six axis orders. Empty and unflagged items set nothing.
+template <int L>
+__global__ __launch_bounds__(64) void kShapeBits(const uint64_t* __restrict__ layouts, uint32_t n, uint32_t* __restrict__ bits, const uint32_t* __restrict__ childFlags)
+{
+    using G = Geo<L>;
+    using Row = typename G::Row;
+    const uint32_t r = blockIdx.x;
+    if (r >= n || (childFlags && childFlags[r] == 0u))
+        return;
+    const Row* g = reinterpret_cast<const Row*>(layouts) + (size_t)r * G::R2;
+    uint32_t ax = 0, ay = 0, az = 0;
+    for (int j = (int)threadIdx.x; j < G::R2; j += 64)
+        occupancyAdd((uint32_t)g[j], j, L, ax, ay, az);
+    ax = waveOr(ax);
+    ay = waveOr(ay);
+    az = waveOr(az);
+    if (threadIdx.x >= 6u || ax == 0u)
+        return;
+    const uint32_t e[3] = { spanOf(ax), spanOf(ay), spanOf(az) };
+    // Lane t < 6 inserts permutation t of (e0, e1, e2).
+    constexpr uint8_t kPerm[6][3] = { { 0, 1, 2 }, { 0, 2, 1 }, { 1, 0, 2 }, { 1, 2, 0 }, { 2, 0, 1 }, { 2, 1, 0 } };
+    const uint32_t t = threadIdx.x;
+    atomicOr(&bits[shapeWord(L, e[kPerm[t][1]], e[kPerm[t][2]])], 1u << e[kPerm[t][0]]);
+}
+
 // One wave per (item, frame): the hash of the whole grid in the pass frame of permutation code
 // q = blockIdx.y (1 frame: q = 0 only; 6 frames: every q) seen from each of its 8 corners goes
 // into the table and the Bloom filter. Pass a reads child axis sigma_a with sigma = q^-1, so the
@@ -269,7 +296,7 @@ __global__ __launch_bounds__(64) void kCornerTable(const uint64_t* __restrict__ 
 template <int L, int Waves, int Kind, bool XcdGrouped>
 __global__ __launch_bounds__(64 * Waves) void kSearch(const uint64_t* __restrict__ layouts, uint32_t n, const uint64_t* __restrict__ table, uint32_t logCap,
     const uint32_t* __restrict__ bloom, uint32_t logBloom, uint64_t* __restrict__ best, uint32_t cfgBits, unsigned long long* __restrict__ counters,
-    uint32_t rankBase, uint32_t numParents)
+    uint32_t rankBase, uint32_t numParents, const uint32_t* __restrict__ shapeBits)
 {
     using G = Geo<L>;
     constexpr int Buffers = Kind == kUnitShared ? 2 : 1;
@@ -278,6 +305,10 @@ __global__ __launch_bounds__(64 * Waves) void kSearch(const uint64_t* __restrict
     __shared__ uint32_t s_occ[3 * Waves];
     __shared__ uint32_t s_qMeta[Waves][kProbeQueue];
     __shared__ uint32_t s_qHash[Waves][kProbeQueue];
+    constexpr bool ShapeOn = shapeFits<L, Waves, Kind>();
+    __shared__ uint32_t s_pre[ShapeOn ? 2 * G::R * G::Pitch : 1];
+    __shared__ uint32_t s_n1[ShapeOn ? Waves * G::P * G::R : 1];
+    __shared__ uint32_t s_shape[ShapeOn ? shapeWords<L>() : 1];
     ProbeCounts cnt;
     const uint32_t Units = searchUnits(Kind, cfgBits);
     constexpr uint32_t Chunks = (uint32_t)(G::MaxWaves / Waves);
@@ -302,8 +333,36 @@ __global__ __launch_bounds__(64 * Waves) void kSearch(const uint64_t* __restrict
     Occupancy occ;
     if (!loadRowsOcc<L, 64 * Waves>(layouts, n, r, a0, s_rows, s_occ, occ))
         return;
-    searchUnit<L, Waves, Kind>(r, unit, chunk, (int)(threadIdx.x >> 6), s_rows, occ, s_planes[0], s_planes[Buffers - 1], s_qMeta[threadIdx.x >> 6], s_qHash[threadIdx.x >> 6], layouts,
-        n, table, logCap, bloom, logBloom, best, cfgBits, cnt);
+    const int wave = (int)(threadIdx.x >> 6);
+    // Shape pre-filter: only where the parent runs permutation code 0 alone (a0 = 0; rows
+    // (a1, a2) = (y, z) at a1 + Pitch * a2).
+    if constexpr (ShapeOn) {
+        const bool parentPerms = (cfgBits & kCfgAxis) != 0 && (cfgBits & kCfgChildFrames) == 0;
+        if (shapeBits != nullptr && !parentPerms) {
+            for (int i = (int)threadIdx.x; i < shapeWords<L>(); i += 64 * Waves)
+                s_shape[i] = shapeBits[i];
+            for (int i = (int)threadIdx.x; i < 2 * G::R; i += 64 * Waves) {
+                const int dy = i / G::R, a2 = i % G::R;
+                uint32_t acc = 0;
+                for (int k = 0; k < G::R; ++k) {
+                    const int a1 = dy ? G::R - 1 - k : k;
+                    acc |= (uint32_t)s_rows[a1 + G::Pitch * a2];
+                    s_pre[dy * G::R * G::Pitch + a1 * G::Pitch + a2] = acc;
+                }
+            }
+            __syncthreads();
+            ShapeLds sh;
+            sh.pre = s_pre;
+            sh.n1 = s_n1;
+            sh.bits = s_shape;
+            searchUnit<L, Waves, Kind, true>(r, unit, chunk, wave, s_rows, occ, s_planes[0], s_planes[Buffers - 1], s_qMeta[wave], s_qHash[wave], layouts, n,
+                table, logCap, bloom, logBloom, best, cfgBits, cnt, sh);
+            flushCounts(cnt, counters, cfgBits);
+            return;
+        }
+    }
+    searchUnit<L, Waves, Kind>(r, unit, chunk, wave, s_rows, occ, s_planes[0], s_planes[Buffers - 1], s_qMeta[wave], s_qHash[wave], layouts, n, table, logCap,
+        bloom, logBloom, best, cfgBits, cnt);
     flushCounts(cnt, counters, cfgBits);
 }
 
@@ -372,13 +431,13 @@ __global__ __launch_bounds__(256) void kFinalMapping(const uint64_t* __restrict_
 
 template <int L, int Waves, int Kind, bool XcdGrouped>
 hipError_t launchSearchShape(uint32_t parents, uint32_t r0, uint32_t n, const uint64_t* layouts, const uint64_t* table, uint32_t logCap, const uint32_t* bloom,
-    uint32_t logBloom, uint64_t* best, uint32_t cfgBits, unsigned long long* counters, hipStream_t stream)
+    uint32_t logBloom, uint64_t* best, uint32_t cfgBits, unsigned long long* counters, const uint32_t* shape, hipStream_t stream)
 {
     if constexpr (Waves >= 1 && Waves <= Geo<L>::MaxWaves && searchLds<L, Waves, Kind>() <= kLdsBytes) {
         const uint32_t perParent = searchUnits(Kind, cfgBits) * (uint32_t)(Geo<L>::MaxWaves / Waves);
         const dim3 grid(XcdGrouped ? xcdGridBlocks(parents, perParent) : parents * perParent);
         hipLaunchKernelGGL((kSearch<L, Waves, Kind, XcdGrouped>), grid, dim3(64 * Waves), 0, stream, layouts, n, table, logCap, bloom, logBloom, best, cfgBits,
-            counters, r0, parents);
+            counters, r0, parents, shape);
         return hipGetLastError();
     } else {
         return hipErrorInvalidValue;
@@ -390,11 +449,11 @@ hipError_t launchSearchShape(uint32_t parents, uint32_t r0, uint32_t n, const ui
 // 1: XCD-grouped).
 template <int L>
 hipError_t launchSearch(uint32_t variant, uint32_t n, uint32_t r0, uint32_t r1, const uint64_t* layouts, const uint64_t* table, uint32_t logCap, const uint32_t* bloom,
-    uint32_t logBloom, uint64_t* best, uint32_t cfgBits, unsigned long long* counters, hipStream_t stream)
+    uint32_t logBloom, uint64_t* best, uint32_t cfgBits, unsigned long long* counters, const uint32_t* shape, hipStream_t stream)
 {
     constexpr int M = Geo<L>::MaxWaves;
     const uint32_t p = r1 - r0;
-#define TDAG_SHAPE(W, K, X) launchSearchShape<L, W, K, X>(p, r0, n, layouts, table, logCap, bloom, logBloom, best, cfgBits, counters, stream)
+#define TDAG_SHAPE(W, K, X) launchSearchShape<L, W, K, X>(p, r0, n, layouts, table, logCap, bloom, logBloom, best, cfgBits, counters, shape, stream)
     switch (variant) {
     case 0:
         return TDAG_SHAPE(M, kUnitCombo, false);
@@ -525,6 +584,7 @@ MatchPlan planMatch(uint32_t level, uint64_t n, uint32_t variant, uint32_t first
     p.bloomBitsPerEntry = bloomBitsPerEntry ? bloomBitsPerEntry : defaultFilterBits(level);
     p.logBloom = std::min<uint32_t>(22, std::max<uint32_t>(6, log2Ceil((entries * p.bloomBitsPerEntry + 31) / 32)));
     p.offBloom = take((1ull << p.logBloom) * 4);
+    p.offShape = take(level >= 3 ? ((size_t)4 << (2 * level)) : 0);
     p.offParentTp = take(n * 8);
     p.offIsUnique = take(n * 4);
     p.offNewIdx = take(n * 4);
@@ -545,6 +605,7 @@ LevelView levelView(const MatchPlan& p, void* d_scratch)
     v.tabKeys = (uint64_t*)(base + p.offTabKeys);
     v.tabVals = (uint32_t*)(base + p.offTabVals);
     v.bloom = (uint32_t*)(base + p.offBloom);
+    v.shape = p.level >= 3 ? (uint32_t*)(base + p.offShape) : nullptr;
     v.counters = (unsigned long long*)(base + p.offCounters);
     v.logCap = p.logCap;
     v.logBloom = p.logBloom;
@@ -626,16 +687,22 @@ hipError_t enqueueBuildTable(const MatchPlan& p, void* d_scratch, hipStream_t st
         hipLaunchKernelGGL(kLayouts<3>, dim3((unsigned)ceilDiv((size_t)n * 2 * 8, 256)), dim3(256), 0, stream, v.sortedMasks, n);
         hipLaunchKernelGGL(kCornerTable<3>, dim3(n, p.childFrames), dim3(64), 0, stream, v.sortedMasks, n, v.tabKeys, p.logCap, v.bloom, p.logBloom,
             childFlags);
+        TDAG_TRY(fillU32(v.shape, shapeWords<3>(), 0u, stream));
+        hipLaunchKernelGGL(kShapeBits<3>, dim3(n), dim3(64), 0, stream, v.sortedMasks, n, v.shape, childFlags);
         break;
     case 4:
         hipLaunchKernelGGL(kLayouts<4>, dim3((unsigned)ceilDiv((size_t)n * 2 * 16, 256)), dim3(256), 0, stream, v.sortedMasks, n);
         hipLaunchKernelGGL(kCornerTable<4>, dim3(n, p.childFrames), dim3(64), 0, stream, v.sortedMasks, n, v.tabKeys, p.logCap, v.bloom, p.logBloom,
             childFlags);
+        TDAG_TRY(fillU32(v.shape, shapeWords<4>(), 0u, stream));
+        hipLaunchKernelGGL(kShapeBits<4>, dim3(n), dim3(64), 0, stream, v.sortedMasks, n, v.shape, childFlags);
         break;
     case 5:
         hipLaunchKernelGGL(kLayouts<5>, dim3((unsigned)ceilDiv((size_t)n * 2 * 32, 256)), dim3(256), 0, stream, v.sortedMasks, n);
         hipLaunchKernelGGL(kCornerTable<5>, dim3(n, p.childFrames), dim3(64), 0, stream, v.sortedMasks, n, v.tabKeys, p.logCap, v.bloom, p.logBloom,
             childFlags);
+        TDAG_TRY(fillU32(v.shape, shapeWords<5>(), 0u, stream));
+        hipLaunchKernelGGL(kShapeBits<5>, dim3(n), dim3(64), 0, stream, v.sortedMasks, n, v.shape, childFlags);
         break;
     default:
         return hipErrorInvalidValue;
@@ -668,13 +735,13 @@ hipError_t enqueueSearch(const MatchPlan& p, uint32_t cfgBitsIn, void* d_scratch
             TDAG_TRY(hipGetLastError());
             break;
         case 3:
-            TDAG_TRY(launchSearch<3>(p.variant, n, r0, r1, v.sortedMasks, v.tabKeys, p.logCap, v.bloom, p.logBloom, v.best, cfgBits, v.counters, stream));
+            TDAG_TRY(launchSearch<3>(p.variant, n, r0, r1, v.sortedMasks, v.tabKeys, p.logCap, v.bloom, p.logBloom, v.best, cfgBits, v.counters, v.shape, stream));
             break;
         case 4:
-            TDAG_TRY(launchSearch<4>(p.variant, n, r0, r1, v.sortedMasks, v.tabKeys, p.logCap, v.bloom, p.logBloom, v.best, cfgBits, v.counters, stream));
+            TDAG_TRY(launchSearch<4>(p.variant, n, r0, r1, v.sortedMasks, v.tabKeys, p.logCap, v.bloom, p.logBloom, v.best, cfgBits, v.counters, v.shape, stream));
             break;
         case 5:
-            TDAG_TRY(launchSearch<5>(p.variant, n, r0, r1, v.sortedMasks, v.tabKeys, p.logCap, v.bloom, p.logBloom, v.best, cfgBits, v.counters, stream));
+            TDAG_TRY(launchSearch<5>(p.variant, n, r0, r1, v.sortedMasks, v.tabKeys, p.logCap, v.bloom, p.logBloom, v.best, cfgBits, v.counters, v.shape, stream));
             break;
         default:
             return hipErrorInvalidValue;

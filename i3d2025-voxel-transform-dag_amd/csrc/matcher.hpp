@@ -44,7 +44,7 @@ struct MatchPlan {
     uint32_t childFrames; // levels 3..5: child-table permutation frames, 1 or 6 (kCfgChildFrames)
     uint64_t n;
     size_t offKeys, offOrder, offSortScratch, offSortedMasks, offBest;
-    size_t offTabKeys, offTabVals, offBloom, offParentTp, offIsUnique, offNewIdx, offScanScratch, offCounters;
+    size_t offTabKeys, offTabVals, offBloom, offShape, offParentTp, offIsUnique, offNewIdx, offScanScratch, offCounters;
     size_t scratchBytes;
 };
 
@@ -83,6 +83,7 @@ struct LevelView {
     uint64_t* tabKeys; // level 2: keys; levels >= 3: packed (hash << 32 | rank << 3 | corner)
     uint32_t* tabVals; // level 2 only
     uint32_t* bloom;
+    uint32_t* shape; // levels >= 3: the children's extent triples, bit e0 of word e1 + R * e2 (extents - 1)
     unsigned long long* counters;
     uint32_t logCap, logBloom;
     uint32_t childFrames;

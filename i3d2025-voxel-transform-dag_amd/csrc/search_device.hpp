@@ -408,6 +408,41 @@ __device__ __forceinline__ int emptyLead(uint32_t occ, uint32_t pos)
     return pos ? __builtin_clz(occ) - (32 - Geo<L>::R) : __builtin_ctz(occ);
 }
 
+// Shape pre-filter of the Bloom lookups (levels 3..5). A verified match makes the child equal
+// to the transformed content of the parent's anchored sub-box, so the sub-box content has the
+// child's extents (the span of occupied coordinates along each axis) up to an axis permutation.
+// The extents of every child in the table, inserted under all six axis orders, form a bitmap of
+// R^2 words of R bits (word e1 + R * e2, bit e0 for extents e - 1); a sub-box whose extent triple is not in it
+// cannot lead to a verified match, so its Bloom word is not loaded. The extents of a sub-box are
+// derived along the three passes: along A2 from the pass-2 states read by pass 3, along A1 from
+// per-slab masks of the non-empty row segments that pass 2 read, along A0 from the OR of the
+// rows over the box's A1 range (a per-block table) masked to its A0 range.
+template <int L>
+constexpr int shapeWords()
+{
+    return 1 << (2 * L);
+}
+
+// Word of extent pair (e1, e2) (each extent - 1 < R); bit e0 of it marks the triple.
+__host__ __device__ __forceinline__ uint32_t shapeWord(uint32_t L, uint32_t e1, uint32_t e2)
+{
+    return e1 | (e2 << L);
+}
+
+// Extent - 1 of the set bits of x != 0 (bare v_ffbh / v_ffbl: for x == 0 the value is
+// meaningless, and callers mask the key it enters).
+__device__ __forceinline__ uint32_t spanOf(uint32_t x)
+{
+    return 31u - (uint32_t)__builtin_clz(x) - (uint32_t)__builtin_ctz(x);
+}
+
+// LDS views of the shape pre-filter of one block (all null: the filter is off).
+struct ShapeLds {
+    const uint32_t* pre = nullptr; // [2][R * Pitch]: OR over a1 in [0, c1] (index 0) / [c1, R) (index 1) of row (a1, a2), at c1 * Pitch + a2
+    uint32_t* n1 = nullptr; // [Waves][P][R]: for plane w of a wave and line a2, bit k set iff pass-2 step k read a non-empty segment
+    const uint32_t* bits = nullptr; // shapeWords<L>() words
+};
+
 // Value of the first pass (along A0) at coordinate c0 of a row: the trimmed bit sequence of the
 // row segment read from the scan start toward c0 is the binary number formed by those bits, so
 // it needs no hashing. Positive (scan from R-1 down to c0): bits [c0, R-1] with bit R-1 most
@@ -426,8 +461,8 @@ __device__ __forceinline__ uint32_t rowPrefix(uint32_t row, int sh)
 // The first `lead` steps (a wave-uniform count, see emptyLead) read only empty rows: chunks of
 // R/4 steps inside them store the state 0 without hashing. All rows are read up front so the
 // chunk branches do not wait on LDS.
-template <int L, int A0, int A1, bool Pos0, bool Pos1>
-__device__ __forceinline__ void pass12(const typename Geo<L>::Row* s_rows, int c0, int u, uint32_t* plane, int lead)
+template <int L, int A0, int A1, bool Pos0, bool Pos1, bool Shape = false>
+__device__ __forceinline__ void pass12(const typename Geo<L>::Row* s_rows, int c0, int u, uint32_t* plane, int lead, uint32_t* n1Line = nullptr)
 {
     using G = Geo<L>;
     constexpr int R = G::R;
@@ -441,7 +476,7 @@ __device__ __forceinline__ void pass12(const typename Geo<L>::Row* s_rows, int c
 #pragma unroll
     for (int k = 0; k < R; ++k)
         v[k] = (uint32_t)rows[Step * (Pos1 ? (R - 1 - k) : k)];
-    uint32_t s = 0;
+    uint32_t s = 0, nz = 0;
 #pragma unroll
     for (int k0 = 0; k0 < R; k0 += C) {
         if (k0 + C <= lead) {
@@ -452,29 +487,40 @@ __device__ __forceinline__ void pass12(const typename Geo<L>::Row* s_rows, int c
         }
 #pragma unroll
         for (int k = k0; k < k0 + C; ++k) {
-            s = hashStep(s, rowPrefix<Pos0>(v[k], sh));
+            const uint32_t seg = rowPrefix<Pos0>(v[k], sh);
+            s = hashStep(s, seg);
             line[Pos1 ? (R - 1 - k) : k] = s;
+            if constexpr (Shape)
+                nz |= min(seg, 1u) << k;
         }
     }
+    if constexpr (Shape)
+        *n1Line = nz;
 }
 
-template <int L, int A0, int A1>
-__device__ __forceinline__ void pass12Dirs(uint32_t d, const typename Geo<L>::Row* s_rows, int c0, int u, uint32_t* plane, int lead)
+template <int L, int A0, int A1, bool Shape = false>
+__device__ __forceinline__ void pass12Dirs(uint32_t d, const typename Geo<L>::Row* s_rows, int c0, int u, uint32_t* plane, int lead, uint32_t* n1Line = nullptr)
 {
     switch (d & 3u) {
     case 0:
-        pass12<L, A0, A1, false, false>(s_rows, c0, u, plane, lead);
+        pass12<L, A0, A1, false, false, Shape>(s_rows, c0, u, plane, lead, n1Line);
         break;
     case 1:
-        pass12<L, A0, A1, true, false>(s_rows, c0, u, plane, lead);
+        pass12<L, A0, A1, true, false, Shape>(s_rows, c0, u, plane, lead, n1Line);
         break;
     case 2:
-        pass12<L, A0, A1, false, true>(s_rows, c0, u, plane, lead);
+        pass12<L, A0, A1, false, true, Shape>(s_rows, c0, u, plane, lead, n1Line);
         break;
     default:
-        pass12<L, A0, A1, true, true>(s_rows, c0, u, plane, lead);
+        pass12<L, A0, A1, true, true, Shape>(s_rows, c0, u, plane, lead, n1Line);
         break;
     }
+}
+
+// Bits [0, k] set (k < 32).
+__device__ __forceinline__ uint32_t lowMask(uint32_t k)
+{
+    return k >= 31u ? 0xFFFFFFFFu : (2u << k) - 1u;
 }
 
 // Passes 1 and 2 for permutation code pc (axes p0, p1) and direction bits d (bit 0: pass 1,
@@ -527,9 +573,14 @@ __device__ __forceinline__ uint32_t bloomWordOrZero(__amdgpu_buffer_rsrc_t rs, u
 // loaded without branches (empty voxels read 0 through an out-of-range buffer offset), one chunk
 // of R/4 steps at a time, and tested while the next chunk hashes. Chunks inside the first `lead` steps (wave-uniform; their
 // states are 0) are skipped: their voxels are empty, never pending, and out keeps its old words.
-template <int L, bool Pos2>
+// Shape (see ShapeLds): preLine = the pre table row of the lane's c1 for the combo's A1
+// direction, n1Plane = the n1 masks of the lane's plane, mask0 / mask1 = the box's range along A0
+// (coordinate bits) and A1 (pass-2 step bits); sub-boxes whose extents are not in the bitmap
+// load no filter word.
+template <int L, bool Pos2, bool Shape = false>
 __device__ __forceinline__ uint32_t pass3Filter(const uint32_t* in, uint32_t* out, const uint32_t* __restrict__ bloom, uint32_t logBloom, uint32_t cfgBits,
-    int lead, ProbeCounts& cnt)
+    int lead, ProbeCounts& cnt, const uint32_t* preLine = nullptr, const uint32_t* n1Plane = nullptr, const uint32_t* shapeBits = nullptr, uint32_t mask0 = 0u,
+    uint32_t mask1 = 0u)
 {
     constexpr int R = Geo<L>::R;
     constexpr int C = R / 4;
@@ -540,6 +591,7 @@ __device__ __forceinline__ uint32_t pass3Filter(const uint32_t* in, uint32_t* ou
     for (int k = 0; k < R; ++k)
         st[k] = in[Geo<L>::Pitch * (Pos2 ? (R - 1 - k) : k)];
     uint32_t s = 0, pending = 0;
+    uint32_t acc0 = 0, acc1 = 0, first2 = (uint32_t)R, last2 = 0; // shape state of the box so far
     uint32_t pf[C], pbw[C]; // the previous chunk's hashes and filter words (0: nothing passes)
 #pragma unroll
     for (int k = 0; k < C; ++k)
@@ -554,7 +606,20 @@ __device__ __forceinline__ uint32_t pass3Filter(const uint32_t* in, uint32_t* ou
             s = hashStep(s, st[k0 + k]);
             f[k] = finalHash(s);
             out[Geo<L>::Pitch * (Pos2 ? (R - 1 - k0 - k) : k0 + k)] = f[k];
-            bw[k] = bloomWordOrZero(rs, f[k], shift);
+            uint32_t fq = f[k];
+            if constexpr (Shape) {
+                const int a2 = Pos2 ? (R - 1 - k0 - k) : k0 + k;
+                acc0 |= preLine[a2];
+                acc1 |= n1Plane[a2];
+                const bool slab = st[k0 + k] != 0u;
+                first2 = min(first2, slab ? (uint32_t)(k0 + k) : (uint32_t)R);
+                last2 = slab ? (uint32_t)(k0 + k) : last2;
+                // Bit e0 of word (e1, e2), sign-extended to 0 or all ones (v_bfe_i32); the word
+                // index is masked so that empty boxes (meaningless spans, f == 0) stay in bounds.
+                const uint32_t word = shapeWord(L, spanOf(acc1 & mask1), last2 - first2) & (uint32_t)(shapeWords<L>() - 1);
+                fq &= (uint32_t)__builtin_amdgcn_sbfe((int)shapeBits[word], spanOf(acc0 & mask0), 1u);
+            }
+            bw[k] = bloomWordOrZero(rs, fq, shift);
         }
         if (k0 > 0) {
 #pragma unroll
@@ -789,13 +854,14 @@ __device__ __forceinline__ void probePending(uint32_t pending, ProbeList& list, 
 // Pass 3 in direction dz with the filter (pass-2 states read from `in`, final hashes written to
 // `out`, which is the lane's line of the wave's planes outWave), then the wave's filter passes go
 // to its probe list.
-template <int L>
+template <int L, bool Shape = false>
 __device__ __forceinline__ void lastPassProbe(uint32_t dz, int lead, const uint32_t* in, uint32_t* out, const uint32_t* outWave, ProbeList& list, int c0Base,
     const uint64_t* __restrict__ layouts, uint32_t n, const uint64_t* __restrict__ table, uint32_t logCap, const uint32_t* __restrict__ bloom,
-    uint32_t logBloom, uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t r, uint32_t pc, uint32_t d, ProbeCounts& cnt)
+    uint32_t logBloom, uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t r, uint32_t pc, uint32_t d, ProbeCounts& cnt, const uint32_t* preLine = nullptr,
+    const uint32_t* n1Plane = nullptr, const uint32_t* shapeBits = nullptr, uint32_t mask0 = 0u, uint32_t mask1 = 0u)
 {
-    const uint32_t pending = dz ? pass3Filter<L, true>(in, out, bloom, logBloom, cfgBits, lead, cnt)
-                                : pass3Filter<L, false>(in, out, bloom, logBloom, cfgBits, lead, cnt);
+    const uint32_t pending = dz ? pass3Filter<L, true, Shape>(in, out, bloom, logBloom, cfgBits, lead, cnt, preLine, n1Plane, shapeBits, mask0, mask1)
+                                : pass3Filter<L, false, Shape>(in, out, bloom, logBloom, cfgBits, lead, cnt, preLine, n1Plane, shapeBits, mask0, mask1);
     waveSync();
     probePending<L>(pending, list, outWave, c0Base, layouts, n, table, logCap, best, cfgBits, r, pc, d, cnt);
 }
@@ -814,11 +880,28 @@ __host__ __device__ __forceinline__ uint32_t searchUnits(int kind, uint32_t cfgB
 }
 
 // LDS bytes of one kSearch block.
+// LDS bytes of one kSearch block: rows, planes, occupancy and probe lists, plus the shape
+// pre-filter's tables where they still fit (shapeFits; the largest blocks run without it).
 template <int L, int Waves, int Kind>
-constexpr size_t searchLds()
+constexpr size_t searchLdsBase()
 {
     return Geo<L>::Rows * sizeof(typename Geo<L>::Row) + (Kind == kUnitShared ? 2 : 1) * (size_t)Waves * Geo<L>::P * Geo<L>::Slab * 4 + 3 * Waves * 4 +
         (size_t)Waves * kProbeQueue * 8;
+}
+template <int L, int Waves>
+constexpr size_t shapeLds()
+{
+    return (size_t)2 * Geo<L>::R * Geo<L>::Pitch * 4 + (size_t)Waves * Geo<L>::P * Geo<L>::R * 4 + (size_t)shapeWords<L>() * 4;
+}
+template <int L, int Waves, int Kind>
+constexpr bool shapeFits()
+{
+    return searchLdsBase<L, Waves, Kind>() + shapeLds<L, Waves>() <= kLdsBytes;
+}
+template <int L, int Waves, int Kind>
+constexpr size_t searchLds()
+{
+    return searchLdsBase<L, Waves, Kind>() + (shapeFits<L, Waves, Kind>() ? shapeLds<L, Waves>() : 0);
 }
 
 // Search of one unit of parent rank r (rows already in s_rows, occupancy occ) by `Waves` waves:
@@ -826,16 +909,19 @@ constexpr size_t searchLds()
 // for w < P. planes0 (and planes1 for kUnitShared) hold Waves * P * Slab words each; qMeta and
 // qHash are the wave's probe list (kProbeQueue entries each), walked when full and once at the
 // end of the unit. Scan steps over empty rows are skipped (emptyLead), and so is the whole unit
-// for a wave whose planes hold no voxel.
-template <int L, int Waves, int Kind>
+// for a wave whose planes hold no voxel. With Shape (only permutation code 0; `shape` filled by
+// the block), pass 3 loads no filter word for sub-boxes whose extents no child has.
+template <int L, int Waves, int Kind, bool Shape = false>
 __device__ __forceinline__ void searchUnit(uint32_t r, uint32_t unit, uint32_t chunk, int wave, const typename Geo<L>::Row* s_rows, const Occupancy& occ,
     uint32_t* planes0, uint32_t* planes1, uint32_t* qMeta, uint32_t* qHash, const uint64_t* __restrict__ layouts, uint32_t n, const uint64_t* __restrict__ table,
-    uint32_t logCap, const uint32_t* __restrict__ bloom, uint32_t logBloom, uint64_t* __restrict__ best, uint32_t cfgBits, ProbeCounts& cnt)
+    uint32_t logCap, const uint32_t* __restrict__ bloom, uint32_t logBloom, uint64_t* __restrict__ best, uint32_t cfgBits, ProbeCounts& cnt,
+    const ShapeLds& shape = ShapeLds {})
 {
     using G = Geo<L>;
     constexpr int R = G::R;
     // Parent-side permutations: all six, unless the table holds the children under all six
-    // (kCfgChildFrames) or permutations are off.
+    // (kCfgChildFrames) or permutations are off. The shape pre-filter runs only without them
+    // (permutation code 0: A0, A1, A2 = x, y, z).
     const bool parentPerms = (cfgBits & kCfgAxis) != 0 && (cfgBits & kCfgChildFrames) == 0;
     const int lane = (int)(threadIdx.x & 63u);
     const int u = lane & (R - 1), w = lane >> L;
@@ -854,12 +940,29 @@ __device__ __forceinline__ void searchUnit(uint32_t r, uint32_t unit, uint32_t c
     if (__ballot(segment != 0u) == 0ull)
         return;
     ProbeList list { qMeta, qHash, 0u };
+    // Shape state of the lane: its n1 slot (written by pass 2) and its plane's n1 masks (read by
+    // pass 3), and the box's A0 range (fixed per unit: dx and c0).
+    uint32_t* n1Line = nullptr;
+    const uint32_t* n1Plane = nullptr;
+    uint32_t mask0 = 0u;
+    if constexpr (Shape) {
+        n1Plane = shape.n1 + (wave * G::P + w) * R;
+        n1Line = const_cast<uint32_t*>(n1Plane) + u;
+        mask0 = dx ? (0xFFFFFFFFu << c0) : lowMask((uint32_t)c0);
+    }
+    // The pre table row of the lane's c1 and its A1 range in pass-2 steps, for A1 direction dy.
+    auto preLineOf = [&](uint32_t dy) { return Shape ? shape.pre + dy * (uint32_t)(R * G::Pitch) + (uint32_t)(u * G::Pitch) : nullptr; };
+    auto mask1Of = [&](uint32_t dy) { return Shape ? lowMask(dy ? (uint32_t)(R - 1 - u) : (uint32_t)u) : 0u; };
     if constexpr (Kind == kUnitCombo) {
         const uint32_t pc = unit >> 3, d = unit & 7u;
-        passes12<L>(pc, d, s_rows, c0, u, plane, emptyLead<L>(occ.axis(permAxis(pc, 1)), (d >> 1) & 1u));
+        const int lead2 = emptyLead<L>(occ.axis(permAxis(pc, 1)), (d >> 1) & 1u);
+        if constexpr (Shape)
+            pass12Dirs<L, 0, 1, true>(d, s_rows, c0, u, plane, lead2, n1Line);
+        else
+            passes12<L>(pc, d, s_rows, c0, u, plane, lead2);
         waveSync();
-        lastPassProbe<L>((d >> 2) & 1u, emptyLead<L>(occ.axis(permAxis(pc, 2)), (d >> 2) & 1u), line, line, planes0 + wave * G::P * G::Slab, list, c0Base,
-            layouts, n, table, logCap, bloom, logBloom, best, cfgBits, r, pc, d, cnt);
+        lastPassProbe<L, Shape>((d >> 2) & 1u, emptyLead<L>(occ.axis(permAxis(pc, 2)), (d >> 2) & 1u), line, line, planes0 + wave * G::P * G::Slab, list, c0Base,
+            layouts, n, table, logCap, bloom, logBloom, best, cfgBits, r, pc, d, cnt, preLineOf((d >> 1) & 1u), n1Plane, shape.bits, mask0, mask1Of((d >> 1) & 1u));
     } else {
 #pragma unroll 1
         for (uint32_t sel = 0; sel < 2; ++sel) {
@@ -870,9 +973,14 @@ __device__ __forceinline__ void searchUnit(uint32_t r, uint32_t unit, uint32_t c
 #pragma unroll 1
             for (uint32_t dy = 0; dy < 2; ++dy) {
                 const int lead2 = emptyLead<L>(occ1, dy);
+                const uint32_t* preLine = preLineOf(dy);
+                const uint32_t mask1 = mask1Of(dy);
                 if constexpr (Kind == kUnitShared) {
                     waveSync();
-                    passes12<L>(pc, dx | (dy << 1), s_rows, c0, u, plane, lead2);
+                    if constexpr (Shape)
+                        pass12Dirs<L, 0, 1, true>(dx | (dy << 1), s_rows, c0, u, plane, lead2, n1Line);
+                    else
+                        passes12<L>(pc, dx | (dy << 1), s_rows, c0, u, plane, lead2);
                     waveSync();
                 }
 #pragma unroll 1
@@ -880,11 +988,14 @@ __device__ __forceinline__ void searchUnit(uint32_t r, uint32_t unit, uint32_t c
                     const uint32_t d = dx | (dy << 1) | (dz << 2);
                     if constexpr (Kind == kUnitFamily) {
                         waveSync();
-                        passes12<L>(pc, d, s_rows, c0, u, plane, lead2);
+                        if constexpr (Shape)
+                            pass12Dirs<L, 0, 1, true>(d, s_rows, c0, u, plane, lead2, n1Line);
+                        else
+                            passes12<L>(pc, d, s_rows, c0, u, plane, lead2);
                         waveSync();
                     }
-                    lastPassProbe<L>(dz, emptyLead<L>(occ2, dz), line, outLine, outWave, list, c0Base, layouts, n, table, logCap, bloom, logBloom, best,
-                        cfgBits, r, pc, d, cnt);
+                    lastPassProbe<L, Shape>(dz, emptyLead<L>(occ2, dz), line, outLine, outWave, list, c0Base, layouts, n, table, logCap, bloom, logBloom, best,
+                        cfgBits, r, pc, d, cnt, preLine, n1Plane, shape.bits, mask0, mask1);
                 }
             }
         }
