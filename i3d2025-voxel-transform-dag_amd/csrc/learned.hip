@@ -264,7 +264,8 @@ hipError_t gemm(bool ta, bool tb, const uint16_t* A, int lda, const uint16_t* B,
     if (M <= 0 || N <= 0)
         return hipSuccess;
     const int tiles = (int)(cdiv(N, kTile) * cdiv(M, kTile));
-    const bool split = splitWork && tiles < 64 && K >= 32 * kKs * kSplitSlices / 4 && (size_t)M * N * kSplitSlices <= kSplitWorkFloats;
+    // Split K when few output tiles would leave the GPU idle and each slice keeps >= 128 of K.
+    const bool split = splitWork && tiles < 64 && K >= 128 * kSplitSlices && (size_t)M * N * kSplitSlices <= kSplitWorkFloats;
     const int slices = split ? kSplitSlices : 1;
     const int kChunk = split ? (int)(cdiv(cdiv(K, kSplitSlices), kKs) * kKs) : std::max(K, 1);
     float* out = split ? splitWork : C;

@@ -183,31 +183,42 @@ __global__ __launch_bounds__(256) void kLayouts(uint64_t* __restrict__ layouts, 
         dst[i] = (Row)a[i];
 }
 
-// Shape bitmap of the children (the search's pre-filter, ShapeLds): one wave per item reads
-// its layout-0 rows, derives its occupancy along x, y and z, and sets the bits of its extent
-// triple in all six axis orders. Empty and unflagged items set nothing.
+// Shape bitmap of the children (the search's pre-filter, ShapeLds): each wave takes items in
+// turn, reads an item's layout-0 rows, derives its occupancy along x, y and z, and sets the bits
+// of its extent triple in all six axis orders in the block's LDS copy, which is then ORed into
+// the global bitmap (one atomic per non-zero word and block, instead of six per item on a few
+// hundred words). Empty and unflagged items set nothing.
+constexpr uint32_t kShapeBlocks = 512;
 template <int L>
-__global__ __launch_bounds__(64) void kShapeBits(const uint64_t* __restrict__ layouts, uint32_t n, uint32_t* __restrict__ bits, const uint32_t* __restrict__ childFlags)
+__global__ __launch_bounds__(256) void kShapeBits(const uint64_t* __restrict__ layouts, uint32_t n, uint32_t* __restrict__ bits, const uint32_t* __restrict__ childFlags)
 {
     using G = Geo<L>;
     using Row = typename G::Row;
-    const uint32_t r = blockIdx.x;
-    if (r >= n || (childFlags && childFlags[r] == 0u))
-        return;
-    const Row* g = reinterpret_cast<const Row*>(layouts) + (size_t)r * G::R2;
-    uint32_t ax = 0, ay = 0, az = 0;
-    for (int j = (int)threadIdx.x; j < G::R2; j += 64)
-        occupancyAdd((uint32_t)g[j], j, L, ax, ay, az);
-    ax = waveOr(ax);
-    ay = waveOr(ay);
-    az = waveOr(az);
-    if (threadIdx.x >= 6u || ax == 0u)
-        return;
-    const uint32_t e[3] = { spanOf(ax), spanOf(ay), spanOf(az) };
-    // Lane t < 6 inserts permutation t of (e0, e1, e2).
+    __shared__ uint32_t s_bits[shapeWords<L>()];
+    for (int i = (int)threadIdx.x; i < shapeWords<L>(); i += 256)
+        s_bits[i] = 0u;
+    __syncthreads();
     constexpr uint8_t kPerm[6][3] = { { 0, 1, 2 }, { 0, 2, 1 }, { 1, 0, 2 }, { 1, 2, 0 }, { 2, 0, 1 }, { 2, 1, 0 } };
-    const uint32_t t = threadIdx.x;
-    atomicOr(&bits[shapeWord(L, e[kPerm[t][1]], e[kPerm[t][2]])], 1u << e[kPerm[t][0]]);
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint32_t r = blockIdx.x * 4u + (threadIdx.x >> 6); r < n; r += gridDim.x * 4u) {
+        if (childFlags && childFlags[r] == 0u)
+            continue;
+        const Row* g = reinterpret_cast<const Row*>(layouts) + (size_t)r * G::R2;
+        uint32_t ax = 0, ay = 0, az = 0;
+        for (int j = (int)lane; j < G::R2; j += 64)
+            occupancyAdd((uint32_t)g[j], j, L, ax, ay, az);
+        ax = waveOr(ax);
+        ay = waveOr(ay);
+        az = waveOr(az);
+        if (lane < 6u && ax != 0u) {
+            const uint32_t e[3] = { spanOf(ax), spanOf(ay), spanOf(az) };
+            atomicOr(&s_bits[shapeWord(L, e[kPerm[lane][1]], e[kPerm[lane][2]])], 1u << e[kPerm[lane][0]]);
+        }
+    }
+    __syncthreads();
+    for (int i = (int)threadIdx.x; i < shapeWords<L>(); i += 256)
+        if (s_bits[i] != 0u)
+            atomicOr(&bits[i], s_bits[i]);
 }
 
 // One wave per (item, frame): the hash of the whole grid in the pass frame of permutation code
@@ -688,21 +699,21 @@ hipError_t enqueueBuildTable(const MatchPlan& p, void* d_scratch, hipStream_t st
         hipLaunchKernelGGL(kCornerTable<3>, dim3(n, p.childFrames), dim3(64), 0, stream, v.sortedMasks, n, v.tabKeys, p.logCap, v.bloom, p.logBloom,
             childFlags);
         TDAG_TRY(fillU32(v.shape, shapeWords<3>(), 0u, stream));
-        hipLaunchKernelGGL(kShapeBits<3>, dim3(n), dim3(64), 0, stream, v.sortedMasks, n, v.shape, childFlags);
+        hipLaunchKernelGGL(kShapeBits<3>, dim3(std::min<uint32_t>(kShapeBlocks, (n + 3) / 4)), dim3(256), 0, stream, v.sortedMasks, n, v.shape, childFlags);
         break;
     case 4:
         hipLaunchKernelGGL(kLayouts<4>, dim3((unsigned)ceilDiv((size_t)n * 2 * 16, 256)), dim3(256), 0, stream, v.sortedMasks, n);
         hipLaunchKernelGGL(kCornerTable<4>, dim3(n, p.childFrames), dim3(64), 0, stream, v.sortedMasks, n, v.tabKeys, p.logCap, v.bloom, p.logBloom,
             childFlags);
         TDAG_TRY(fillU32(v.shape, shapeWords<4>(), 0u, stream));
-        hipLaunchKernelGGL(kShapeBits<4>, dim3(n), dim3(64), 0, stream, v.sortedMasks, n, v.shape, childFlags);
+        hipLaunchKernelGGL(kShapeBits<4>, dim3(std::min<uint32_t>(kShapeBlocks, (n + 3) / 4)), dim3(256), 0, stream, v.sortedMasks, n, v.shape, childFlags);
         break;
     case 5:
         hipLaunchKernelGGL(kLayouts<5>, dim3((unsigned)ceilDiv((size_t)n * 2 * 32, 256)), dim3(256), 0, stream, v.sortedMasks, n);
         hipLaunchKernelGGL(kCornerTable<5>, dim3(n, p.childFrames), dim3(64), 0, stream, v.sortedMasks, n, v.tabKeys, p.logCap, v.bloom, p.logBloom,
             childFlags);
         TDAG_TRY(fillU32(v.shape, shapeWords<5>(), 0u, stream));
-        hipLaunchKernelGGL(kShapeBits<5>, dim3(n), dim3(64), 0, stream, v.sortedMasks, n, v.shape, childFlags);
+        hipLaunchKernelGGL(kShapeBits<5>, dim3(std::min<uint32_t>(kShapeBlocks, (n + 3) / 4)), dim3(256), 0, stream, v.sortedMasks, n, v.shape, childFlags);
         break;
     default:
         return hipErrorInvalidValue;
