@@ -225,8 +225,10 @@ __global__ __launch_bounds__(256) void kShapeBits(const uint64_t* __restrict__ l
 // q = blockIdx.y (1 frame: q = 0 only; 6 frames: every q) seen from each of its 8 corners goes
 // into the table and the Bloom filter. Pass a reads child axis sigma_a with sigma = q^-1, so the
 // value equals what the search computes, at the voxel where every scan ends, for a parent whose
-// content is the child read in that frame. R lanes per corner run passes 1 and 2 along sigma_1
-// for one coordinate along sigma_2 each; the corner's first lane folds pass 3.
+// content is the child read in that frame. Passes 1 and 2 depend on the corner's first two
+// direction bits only, so they run for those 4 directions (R lanes each, one line along sigma_1
+// per coordinate along sigma_2) and leave their end states in LDS; then lane `corner` < 8 runs
+// pass 3 over them and inserts the corner's entry.
 template <int L>
 __global__ __launch_bounds__(64) void kCornerTable(const uint64_t* __restrict__ layouts, uint32_t n, uint64_t* __restrict__ table, uint32_t logCap,
     uint32_t* __restrict__ bloom, uint32_t logBloom, const uint32_t* __restrict__ childFlags)
@@ -235,6 +237,7 @@ __global__ __launch_bounds__(64) void kCornerTable(const uint64_t* __restrict__ 
     using Row = typename G::Row;
     constexpr int R = G::R;
     __shared__ Row s_rows[G::Rows];
+    __shared__ uint32_t s_p2[4][R]; // end states of passes 1-2 per (pos0, pos1) and line c2
     const uint32_t r = blockIdx.x;
     const uint32_t q = blockIdx.y;
     if (childFlags && childFlags[r] == 0u) // only flagged ranks enter the table as children
@@ -259,9 +262,11 @@ __global__ __launch_bounds__(64) void kCornerTable(const uint64_t* __restrict__ 
     // Rows of layout sigma_0 are indexed by the coordinates along the other two axes in ascending
     // axis order.
     const bool lowFirst = sigma[1] < sigma[2];
-    for (int cb = 0; cb < 8; cb += G::P) {
-        const uint32_t corner = (uint32_t)(cb + sub);
-        const bool pos0 = corner & 1u, pos1 = corner & 2u, pos2 = corner & 4u;
+    for (int db = 0; db < 4; db += G::P) {
+        const int dir = db + sub;
+        if (dir >= 4)
+            continue;
+        const bool pos0 = dir & 1, pos1 = dir & 2;
         const int c0 = pos0 ? 0 : R - 1;
         const int sh = pos0 ? c0 : 31 - c0;
         uint32_t s = 0;
@@ -270,23 +275,28 @@ __global__ __launch_bounds__(64) void kCornerTable(const uint64_t* __restrict__ 
             const uint32_t row = (uint32_t)s_rows[lowFirst ? c1 + G::Pitch * c2 : c2 + G::Pitch * c1];
             s = hashStep(s, pos0 ? rowPrefix<true>(row, sh) : rowPrefix<false>(row, sh));
         }
-        uint32_t s3 = 0;
-        for (int k = 0; k < R; ++k) {
-            const int z = pos2 ? (R - 1 - k) : k;
-            s3 = hashStep(s3, (uint32_t)__shfl((int)s, sub * R + z, 64));
-        }
-        const uint32_t h = finalHash(s3);
-        if (c2 == 0 && h != 0) {
-            atomicOr(&bloom[bloomWordNested(h, logBloom)], bloomBits(h));
-            const uint64_t packed = packEntry(h, r, q, corner);
-            uint32_t slot = homeSlot32(h, logCap);
-            for (;;) {
-                const unsigned long long prev = atomicCAS((unsigned long long*)&table[slot], 0ull, (unsigned long long)packed);
-                if (prev == 0ull)
-                    break;
-                slot = (slot + 1) & capMask;
-            }
-        }
+        s_p2[dir][c2] = s;
+    }
+    __syncthreads();
+    if (lane >= 8)
+        return;
+    const uint32_t corner = (uint32_t)lane;
+    const int dir = (int)(corner & 3u);
+    const bool pos2 = corner & 4u;
+    uint32_t s3 = 0;
+    for (int k = 0; k < R; ++k)
+        s3 = hashStep(s3, s_p2[dir][pos2 ? (R - 1 - k) : k]);
+    const uint32_t h = finalHash(s3);
+    if (h == 0)
+        return;
+    atomicOr(&bloom[bloomWordNested(h, logBloom)], bloomBits(h));
+    const uint64_t packed = packEntry(h, r, q, corner);
+    uint32_t slot = homeSlot32(h, logCap);
+    for (;;) {
+        const unsigned long long prev = atomicCAS((unsigned long long*)&table[slot], 0ull, (unsigned long long)packed);
+        if (prev == 0ull)
+            break;
+        slot = (slot + 1) & capMask;
     }
 }
 

@@ -2,20 +2,45 @@
 
 config 1: the 1024^3 scene; the driver's .tdag and every level's keys equal the CPU oracle's
 (through the digests it wrote, tests/golden/config1_digest.json).
-config 2: the 1024^3 generator at 2048^3; the batched form (one persistent kernel for all levels,
-device work queue, captured hipGraph) must equal the per-level launches bit for bit on every
-level's full input.
+config 2: the 1024^3 generator at 2048^3 with the committed autotuner picks; both batched forms
+(one persistent kernel for all levels with a device work queue, and concurrent per-level branches;
+captured hipGraphs) must equal the per-level launches bit for bit on every level's full input.
 config 3: the learned proposal on the 2048^3 scene, trained on three 1024^3 scenes.
 config 4: the 4096^3 city: batched forms, the combined form (batched + learned + fallback with a
 fine-tuned network) and the .tdag16 decode of the whole volume.
 """
+import json
+import os
+
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
 
-def test_config2_batched_bit_exact_2048(gpu_ctx):
+def _tuned_context(name: str):
+    """A context with the committed autotuner output configs/<name> applied (search variant, batch
+    schedule, filter size and child frames per level)."""
+    import tdag_amd
+
+    tuning = json.load(open(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "configs", name)))
+    ctx = tdag_amd.Context(0)
+    for level_s, tt in tuning["levels"].items():
+        level = int(level_s)
+        ctx.set_variant(level, int(tt["variant"]))
+        ctx.set_batching(level, int(tt["first_batch"]), int(tt["growth"]))
+        ctx.set_filter_bits(level, int(tt["filter_bits"]))
+        if level >= 3:
+            ctx.set_child_frames(level, int(tt["child_frames"]))
+    return ctx
+
+
+def test_config2_batched_bit_exact_2048():
+    """Config 2 at full size (2048^3 scene, seed 0) with the autotuner's picks: the committed
+    per-level tuning of the workload, then the batched call's own autotune of its tile grab and
+    phases. Both batched forms (one persistent kernel, and concurrent per-level branches), each
+    captured as a hipGraph, equal the plain per-level launches key for key and mapping for
+    mapping."""
     import torch
 
     import tdag_amd
@@ -23,37 +48,48 @@ def test_config2_batched_bit_exact_2048(gpu_ctx):
 
     R = 2048
     cfg = tdag_amd.TransformDAGConfig(True, True, True, 5)
-    print("config 2: building the 2048^3 scene", flush=True)
-    bits = pl.scene_dense(1, R, 0)
-    t = pl.construct_transform_dag(gpu_ctx, pl.Svdag(bits, R), cfg, keep_level_inputs=True)
-    del bits
-    jobs, ref = [], {}
-    for level in (5, 4, 3, 2):
-        m = np.ascontiguousarray(t.level_masks(level))
-        n = len(m)
-        d_m = torch.from_numpy(m.view(np.int64).reshape(-1)).cuda()
-        enc = torch.empty(max(1, n), dtype=torch.int64, device="cuda")
-        mp = torch.empty(max(1, n), dtype=torch.int64, device="cuda")
-        torch.cuda.synchronize()
-        _, st = gpu_ctx.match_level_device(level, d_m, n, cfg, d_encoded=enc, d_mapping=mp)
-        ref[level] = (enc, mp, n, int(st.num_matched))
-        jobs.append({"level": level, "n": n, "d_masks": d_m, "d_encoded": torch.empty_like(enc), "d_mapping": torch.empty_like(mp),
-                     "d_unique": torch.empty(max(1, n), dtype=torch.int32, device="cuda")})
-        print(f"level {level}: {n} items, {st.num_matched} matched", flush=True)
-    torch.cuda.synchronize()
-    b = gpu_ctx.batch(jobs, cfg, use_graph=True)
+    ctx = _tuned_context("tuning_scene_2048_seed0_maxT5.json")
     try:
-        stats, ms = b.run()
+        print("config 2: building the 2048^3 scene", flush=True)
+        bits = pl.scene_dense(1, R, 0)
+        t = pl.construct_transform_dag(ctx, pl.Svdag(bits, R), cfg, keep_level_inputs=True)
+        del bits
+        ref, masks = {}, {}
+        for level in (5, 4, 3, 2):
+            m = np.ascontiguousarray(t.level_masks(level))
+            n = len(m)
+            d_m = torch.from_numpy(m.view(np.int64).reshape(-1)).cuda()
+            enc = torch.empty(max(1, n), dtype=torch.int64, device="cuda")
+            mp = torch.empty(max(1, n), dtype=torch.int64, device="cuda")
+            torch.cuda.synchronize()
+            _, st = ctx.match_level_device(level, d_m, n, cfg, d_encoded=enc, d_mapping=mp)
+            ref[level] = (enc, mp, n, int(st.num_matched))
+            masks[level] = d_m
+            print(f"level {level}: {n} items, {st.num_matched} matched", flush=True)
+        t.close()
+        for concurrent in (False, True):
+            jobs = []
+            for level in (5, 4, 3, 2):
+                enc, mp, n, _ = ref[level]
+                jobs.append({"level": level, "n": n, "d_masks": masks[level], "d_encoded": torch.empty_like(enc), "d_mapping": torch.empty_like(mp),
+                             "d_unique": torch.empty(max(1, n), dtype=torch.int32, device="cuda")})
+            torch.cuda.synchronize()
+            b = ctx.batch(jobs, cfg, use_graph=True, concurrent=concurrent)
+            try:
+                if not concurrent:
+                    b.autotune(1)
+                stats, ms = b.run()
+            finally:
+                b.close()
+            torch.cuda.synchronize()
+            print(f"batched call (concurrent={concurrent}) {ms:.2f} ms", flush=True)
+            for j, s in zip(jobs, stats):
+                enc, mp, n, matched = ref[j["level"]]
+                assert torch.equal(j["d_encoded"][:n], enc[:n]), f"level {j['level']}: keys differ (concurrent={concurrent})"
+                assert torch.equal(j["d_mapping"][:n], mp[:n]), f"level {j['level']}: mapping differs (concurrent={concurrent})"
+                assert int(s.num_matched) == matched
     finally:
-        b.close()
-    torch.cuda.synchronize()
-    print(f"batched call {ms:.2f} ms", flush=True)
-    for j, s in zip(jobs, stats):
-        enc, mp, n, matched = ref[j["level"]]
-        assert torch.equal(j["d_encoded"][:n], enc[:n]), f"level {j['level']}: keys differ"
-        assert torch.equal(j["d_mapping"][:n], mp[:n]), f"level {j['level']}: mapping differs"
-        assert int(s.num_matched) == matched
-    t.close()
+        ctx.close()
 
 
 def _sha(a) -> str:
@@ -199,16 +235,8 @@ def test_config4_city_4096(oracle, gpu_ctx):
     from tdag_amd import pipeline as pl
 
     R = 4096
-    tuning = json.load(open(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "configs", "tuning_city_4096_seed0_maxT5.json")))
-    ctx = tdag_amd.Context(0)
+    ctx = _tuned_context("tuning_city_4096_seed0_maxT5.json")
     try:
-        for level_s, tt in tuning["levels"].items():
-            level = int(level_s)
-            ctx.set_variant(level, int(tt["variant"]))
-            ctx.set_batching(level, int(tt["first_batch"]), int(tt["growth"]))
-            ctx.set_filter_bits(level, int(tt["filter_bits"]))
-            if level >= 3:
-                ctx.set_child_frames(level, int(tt["child_frames"]))
         cfg = tdag_amd.TransformDAGConfig(True, True, True, 5)
         print("config 4: building the 4096^3 city", flush=True)
         bits = pl.scene_dense(2, R, 0)
