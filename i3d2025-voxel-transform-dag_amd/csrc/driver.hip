@@ -542,8 +542,16 @@ struct tdag_build_graph {
     tdag_context* ctx = nullptr;
     tdag_config cfg {};
     tdag::TDag base; // hierarchical result; its translated levels are replaced on readout
+    tdag::DeviceSvdag dev {}; // the graph's own device copy of the input levels it reads
+    bool devOwned = false;
     tdag::DeviceTranslation dt;
     uint32_t cfgBits = 0;
+    ~tdag_build_graph()
+    {
+        dt.release();
+        if (devOwned)
+            tdag::freeDeviceSvdag(dev);
+    }
 };
 
 extern "C" {
@@ -575,11 +583,15 @@ tdag_status tdag_build_graph_create(tdag_context* ctx, const tdag_svdag* in, con
         return failure.empty() ? setHipErr(e, "device hierarchical pass") : setErr(TDAG_EINTERNAL, "device hierarchical pass: " + failure);
     if (g->base.nodes[maxT + 1].size() != translationParents.size())
         return setErr(TDAG_EINTERNAL, "mid level size mismatch");
-    if (tdag_status s = ensureDeviceCopy(ctx, in); s != TDAG_OK)
-        return s;
+    // The graph reads its own copy of the SVDAG's levels, so it does not depend on the handle.
+    if (hipError_t e = tdag::uploadSvdag(dag, 5, g->dev); e != hipSuccess) {
+        tdag::freeDeviceSvdag(g->dev);
+        return setHipErr(e, "SVDAG upload");
+    }
+    g->devOwned = true;
     tdag::LevelTuning tune[6];
     contextTuning(ctx, tune);
-    hipError_t e = g->dt.init(in->dev, dag, midParents(dag, maxT, translationParents), maxT, tune);
+    hipError_t e = g->dt.init(g->dev, dag, midParents(dag, maxT, translationParents), maxT, tune);
     if (e == hipSuccess)
         e = g->dt.runEager(g->cfgBits, ctx->stream);
     if (e == hipErrorInvalidValue)

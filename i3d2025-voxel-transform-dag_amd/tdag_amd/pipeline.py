@@ -343,8 +343,7 @@ class BuildGraph:
     """A build's translation stage captured as one hipGraph (tdag_build_graph_*)."""
 
     def __init__(self, ctx, svdag: Svdag, config: TransformDAGConfig):
-        self._svdag = svdag  # the graph reads the SVDAG's device copy
-        self._ctx = ctx
+        self._ctx = ctx  # the graph runs on the context's stream (it keeps its own SVDAG copy)
         self.config = config
         cfg = config.to_c()
         self._h = ctypes.c_void_p()

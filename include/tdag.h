@@ -424,7 +424,8 @@ tdag_status tdag_construct_transform_dag_learned(tdag_context* ctx, const tdag_s
  * stage (every level's marking, compaction, flattening, matching, emission and pointer rewrites;
  * no host synchronisation inside) into a hipGraph on ctx's stream. run replays it (out_ms: device
  * time of the replay); result returns the structure of the last run, the same as
- * tdag_construct_transform_dag's. The SVDAG and the context must outlive the graph. No reference
+ * tdag_construct_transform_dag's. The graph keeps its own device copy of the SVDAG's levels (the
+ * handle may be freed after create); the context must outlive the graph. No reference
  * counterpart: the reference rebuilds its matcher inputs on the host between levels
  * (transform_dag.cpp:165-233). */
 typedef struct tdag_build_graph tdag_build_graph;

@@ -163,7 +163,8 @@ def test_edit_structure_view_bytes_match_oracle(oracle, gpu_ctx, kind, R, seed, 
 def test_device_translation_stage_bytes_match_oracle(oracle, gpu_ctx, kind, R, seed, maxT):
     """TDAG_DEVICE_TRANSLATION (marking, compaction, flattening, matching, emission and pointer
     rewrites all on the device) and the captured build graph, replayed three times with other GPU
-    work in between: the same .tdag bytes as the oracle's driver every time."""
+    work in between and after the SVDAG handle was freed: the same .tdag bytes as the oracle's
+    driver every time."""
     import torch
 
     import tdag_amd
@@ -181,6 +182,7 @@ def test_device_translation_stage_bytes_match_oracle(oracle, gpu_ctx, kind, R, s
         a, b = t.record(level), host.record(level)
         assert (a.items, a.unique, a.matched) == (b.items, b.unique, b.matched), level
     g = pl.BuildGraph(gpu_ctx, svdag, cfg)
+    svdag.close()  # the graph keeps its own device copy of the input levels
     try:
         assert g.result().tdag_bytes() == ref
         for replay in range(3):
