@@ -10,11 +10,13 @@
 // done by all blocks together. Tiles are ordered heaviest level first inside a phase.
 //
 // Tile shapes (four waves per block):
-//   level 5: one (permutation, direction) combo on 8 of the 32 planes (192 tiles per parent, 32
-//            when the table holds the children in all six permutation frames)
-//   level 4: one (permutation, direction) combo on all 16 planes      (48 tiles per parent)
-//   level 3: four wave items, each a family sharing passes 1 and 2 on all 8 planes (6 per parent)
+//   level 5: one family of (direction) combos of one parent on 8 of the 32 planes (8 tiles per
+//            parent when the table holds the children in all six permutation frames, else 24)
+//   level 4: one family on all 16 planes                                       (2 or 6 per parent)
+//   level 3: four wave items, each a family sharing passes 1 and 2 on all 8 planes (2 or 6 per parent)
 //   level 2: four parents, one per wave (each with its work list in its LDS slice)
+// Levels 3..5 run the shape pre-filter of the Bloom lookups (the children's extent triples in
+// LDS) wherever the per-level kernels do: when the parent searches permutation code 0 alone.
 // Results are bit-identical to the per-level launches: the search computes the same exhaustive
 // minimum, whatever the schedule.
 #include <hip/hip_runtime.h>
@@ -44,14 +46,31 @@ constexpr size_t rowsBytes()
 
 constexpr size_t alignUp16(size_t v) { return (v + 15) / 16 * 16; }
 
-// Per-wave LDS of a level-3 wave item: rows, then two plane buffers.
-constexpr size_t kL3RowsBytes = alignUp16(rowsBytes<3>());
-constexpr size_t kL3WaveBytes = kL3RowsBytes + 2 * (size_t)Geo<3>::P * Geo<3>::Slab * 4;
+// LDS of a block tile of level 4 or 5 (byte offsets): the rows of the family's first-pass axis,
+// the four waves' planes, then the shape pre-filter's pre tables, n1 masks and extent bitmap.
+template <int L>
+struct BlockTileLds {
+    static constexpr size_t planes = alignUp16(rowsBytes<L>());
+    static constexpr size_t pre = planes + (size_t)kBatchWaves * Geo<L>::P * Geo<L>::Slab * 4;
+    static constexpr size_t n1 = pre + (size_t)shapePreWords<L>() * 4;
+    static constexpr size_t bits = n1 + (size_t)shapeN1Words<L>(kBatchWaves) * 4;
+    static constexpr size_t bytes = bits + (size_t)shapeWords<L>() * 4;
+};
 
-constexpr size_t kL5Bytes = alignUp16(rowsBytes<5>()) + (size_t)kBatchWaves * Geo<5>::P * Geo<5>::Slab * 4;
-constexpr size_t kL4Bytes = alignUp16(rowsBytes<4>()) + (size_t)kBatchWaves * Geo<4>::P * Geo<4>::Slab * 4;
+// LDS slice of one wave of a level-3 wave item: rows, two plane buffers, then the shape
+// pre-filter's tables for that wave alone.
+struct Wave3Lds {
+    static constexpr size_t planes0 = alignUp16(rowsBytes<3>());
+    static constexpr size_t planes1 = planes0 + (size_t)Geo<3>::P * Geo<3>::Slab * 4;
+    static constexpr size_t pre = planes1 + (size_t)Geo<3>::P * Geo<3>::Slab * 4;
+    static constexpr size_t n1 = pre + (size_t)shapePreWords<3>() * 4;
+    static constexpr size_t bits = n1 + (size_t)shapeN1Words<3>(1) * 4;
+    static constexpr size_t bytes = alignUp16(bits + (size_t)shapeWords<3>() * 4);
+};
+constexpr size_t kL3WaveBytes = Wave3Lds::bytes;
+
 constexpr size_t kL3Bytes = kBatchWaves * kL3WaveBytes;
-constexpr size_t kBatchLds = std::max(kL5Bytes, std::max(kL4Bytes, kL3Bytes));
+constexpr size_t kBatchLds = std::max(BlockTileLds<5>::bytes, std::max(BlockTileLds<4>::bytes, kL3Bytes));
 static_assert(kBatchLds <= kLdsBytes, "batched tile LDS");
 
 constexpr uint32_t kL5Chunks = Geo<5>::MaxWaves / kBatchWaves; // plane chunks of a level-5 tile
@@ -69,9 +88,9 @@ uint32_t tilesFor(const MatchPlan& p, uint32_t parents)
     const uint32_t cfg = p.childFrames == 6 ? kCfgChildFrames : 0u;
     switch (p.level) {
     case 5:
-        return parents * searchUnits(kUnitCombo, cfg) * kL5Chunks;
+        return parents * searchUnits(kUnitFamily, cfg) * kL5Chunks;
     case 4:
-        return parents * searchUnits(kUnitCombo, cfg);
+        return parents * searchUnits(kUnitFamily, cfg);
     case 3:
         return (parents * searchUnits(kUnitShared, cfg) + kBatchWaves - 1) / kBatchWaves;
     default:
@@ -79,22 +98,45 @@ uint32_t tilesFor(const MatchPlan& p, uint32_t parents)
     }
 }
 
-// Block-level tile of levels 4 and 5: all four waves search one unit of parent r.
-template <int L, int Kind>
+// Whether a level's search runs the shape pre-filter: the parent searches permutation code 0
+// alone (the table holds the children in all six frames, or permutations are off).
+__device__ __forceinline__ bool shapeOn(const LevelView& v, uint32_t cfgBits)
+{
+    return v.shape != nullptr && !((cfgBits & kCfgAxis) != 0 && (cfgBits & kCfgChildFrames) == 0);
+}
+
+// Block tile of levels 4 and 5: all four waves search family `unit` of parent r on plane chunk
+// `chunk`.
+template <int L>
 __device__ __forceinline__ void blockTile(const LevelView& v, uint32_t r, uint32_t unit, uint32_t chunk, unsigned char* s_mem, uint32_t* s_occ, uint32_t* qMeta, uint32_t* qHash,
     uint32_t cfgBits)
 {
     using G = Geo<L>;
-    if (!(cfgBits & kCfgAxis) && (Kind == kUnitCombo ? (unit >> 3) : (unit >> 1)) != 0)
+    using Lds = BlockTileLds<L>;
+    if (!(cfgBits & kCfgAxis) && (unit >> 1) != 0)
         return; // only permutation code 0 (p0 == 0) is allowed
     auto* s_rows = reinterpret_cast<typename G::Row*>(s_mem);
-    uint32_t* planes = reinterpret_cast<uint32_t*>(s_mem + alignUp16(rowsBytes<L>()));
+    uint32_t* planes = reinterpret_cast<uint32_t*>(s_mem + Lds::planes);
     Occupancy occ;
-    if (!loadRowsOcc<L, kBatchThreads>(v.sortedMasks, v.n, r, Kind == kUnitCombo ? (unit >> 3) >> 1 : unit >> 1, s_rows, s_occ, occ))
+    if (!loadRowsOcc<L, kBatchThreads>(v.sortedMasks, v.n, r, unit >> 1, s_rows, s_occ, occ))
         return;
     ProbeCounts cnt;
-    searchUnit<L, kBatchWaves, Kind>(r, unit, chunk, (int)(threadIdx.x >> 6), s_rows, occ, planes, planes, qMeta, qHash, v.sortedMasks, v.n, v.tabKeys, v.logCap, v.bloom, v.logBloom, v.best,
-        cfgBits, cnt);
+    const int wave = (int)(threadX() >> 6);
+    if (shapeOn(v, cfgBits)) {
+        uint32_t* s_pre = reinterpret_cast<uint32_t*>(s_mem + Lds::pre);
+        uint32_t* s_bits = reinterpret_cast<uint32_t*>(s_mem + Lds::bits);
+        buildShapeTables<L>(s_rows, v.shape, s_bits, s_pre, (int)threadX(), kBatchThreads);
+        __syncthreads();
+        ShapeLds sh;
+        sh.pre = s_pre;
+        sh.n1 = reinterpret_cast<uint32_t*>(s_mem + Lds::n1);
+        sh.bits = s_bits;
+        searchUnit<L, kBatchWaves, kUnitFamily, true>(r, unit, chunk, wave, s_rows, occ, planes, planes, qMeta, qHash, v.sortedMasks, v.n, v.tabKeys, v.logCap, v.bloom,
+            v.logBloom, v.best, cfgBits, cnt, sh);
+    } else {
+        searchUnit<L, kBatchWaves, kUnitFamily>(r, unit, chunk, wave, s_rows, occ, planes, planes, qMeta, qHash, v.sortedMasks, v.n, v.tabKeys, v.logCap, v.bloom, v.logBloom,
+            v.best, cfgBits, cnt);
+    }
     flushCounts(cnt, v.counters, cfgBits);
 }
 
@@ -104,15 +146,29 @@ __device__ __forceinline__ void waveItem3(const LevelView& v, uint32_t r, uint32
     using G = Geo<3>;
     if (!(cfgBits & kCfgAxis) && (unit >> 1) != 0)
         return;
-    unsigned char* base = s_mem + (threadIdx.x >> 6) * kL3WaveBytes;
+    unsigned char* base = s_mem + (threadX() >> 6) * kL3WaveBytes;
     auto* s_rows = reinterpret_cast<G::Row*>(base);
-    uint32_t* planes0 = reinterpret_cast<uint32_t*>(base + kL3RowsBytes);
-    uint32_t* planes1 = planes0 + G::P * G::Slab;
+    uint32_t* planes0 = reinterpret_cast<uint32_t*>(base + Wave3Lds::planes0);
+    uint32_t* planes1 = reinterpret_cast<uint32_t*>(base + Wave3Lds::planes1);
     Occupancy occ;
     if (!loadRowsWave<3>(v.sortedMasks, v.n, r, unit >> 1, s_rows, occ))
         return;
     ProbeCounts cnt;
-    searchUnit<3, 1, kUnitShared>(r, unit, 0, 0, s_rows, occ, planes0, planes1, qMeta, qHash, v.sortedMasks, v.n, v.tabKeys, v.logCap, v.bloom, v.logBloom, v.best, cfgBits, cnt);
+    if (shapeOn(v, cfgBits)) {
+        uint32_t* s_pre = reinterpret_cast<uint32_t*>(base + Wave3Lds::pre);
+        uint32_t* s_bits = reinterpret_cast<uint32_t*>(base + Wave3Lds::bits);
+        buildShapeTables<3>(s_rows, v.shape, s_bits, s_pre, (int)(threadX() & 63u), 64);
+        waveSync();
+        ShapeLds sh;
+        sh.pre = s_pre;
+        sh.n1 = reinterpret_cast<uint32_t*>(base + Wave3Lds::n1);
+        sh.bits = s_bits;
+        searchUnit<3, 1, kUnitShared, true>(r, unit, 0, 0, s_rows, occ, planes0, planes1, qMeta, qHash, v.sortedMasks, v.n, v.tabKeys, v.logCap, v.bloom, v.logBloom, v.best,
+            cfgBits, cnt, sh);
+    } else {
+        searchUnit<3, 1, kUnitShared>(r, unit, 0, 0, s_rows, occ, planes0, planes1, qMeta, qHash, v.sortedMasks, v.n, v.tabKeys, v.logCap, v.bloom, v.logBloom, v.best, cfgBits,
+            cnt);
+    }
     flushCounts(cnt, v.counters, cfgBits);
 }
 
@@ -120,7 +176,7 @@ __device__ __forceinline__ void waveItem3(const LevelView& v, uint32_t r, uint32
 static_assert(kWalkQueue2 * (8 + 2) <= kL3WaveBytes, "level-2 work list fits a wave's LDS slice");
 __device__ __forceinline__ void waveItem2(const LevelView& v, uint32_t r, unsigned char* s_mem, uint32_t cfgBits)
 {
-    unsigned char* base = s_mem + (threadIdx.x >> 6) * kL3WaveBytes;
+    unsigned char* base = s_mem + (threadX() >> 6) * kL3WaveBytes;
     uint64_t* qKey = reinterpret_cast<uint64_t*>(base);
     uint16_t* qMeta = reinterpret_cast<uint16_t*>(base + kWalkQueue2 * 8);
     ProbeCounts cnt;
@@ -185,7 +241,9 @@ __global__ __launch_bounds__(kBatchThreads, 3) void kBatched(const BatchDesc* __
             if (threadIdx.x == 0)
                 s_tile = atomicAdd(&sync->queue[p], grab);
             __syncthreads();
-            const uint32_t t0 = s_tile;
+            // Uniform over the block: in a scalar register, the tile's level view, parent and
+            // unit stay scalar too (their pointers in SGPRs, the descriptor read with scalar loads).
+            const uint32_t t0 = __builtin_amdgcn_readfirstlane(s_tile);
             __syncthreads();
             if (t0 >= total)
                 break;
@@ -202,14 +260,14 @@ __global__ __launch_bounds__(kBatchThreads, 3) void kBatched(const BatchDesc* __
                 const uint32_t lvCfg = levelCfgBits(v, cfgBits);
                 switch (v.level) {
                 case 5: {
-                    const uint32_t per = searchUnits(kUnitCombo, lvCfg) * kL5Chunks;
+                    const uint32_t per = searchUnits(kUnitFamily, lvCfg) * kL5Chunks;
                     const uint32_t m = k % per;
-                    blockTile<5, kUnitCombo>(v, pl.r0 + k / per, m / kL5Chunks, m % kL5Chunks, s_mem, s_occ, s_qMeta[wave], s_qHash[wave], lvCfg);
+                    blockTile<5>(v, pl.r0 + k / per, m / kL5Chunks, m % kL5Chunks, s_mem, s_occ, s_qMeta[wave], s_qHash[wave], lvCfg);
                     break;
                 }
                 case 4: {
-                    const uint32_t per = searchUnits(kUnitCombo, lvCfg);
-                    blockTile<4, kUnitCombo>(v, pl.r0 + k / per, k % per, 0, s_mem, s_occ, s_qMeta[wave], s_qHash[wave], lvCfg);
+                    const uint32_t per = searchUnits(kUnitFamily, lvCfg);
+                    blockTile<4>(v, pl.r0 + k / per, k % per, 0, s_mem, s_occ, s_qMeta[wave], s_qHash[wave], lvCfg);
                     break;
                 }
                 case 3: {

@@ -212,7 +212,7 @@ __global__ __launch_bounds__(256) void kShapeBits(const uint64_t* __restrict__ l
         az = waveOr(az);
         if (lane < 6u && ax != 0u) {
             const uint32_t e[3] = { spanOf(ax), spanOf(ay), spanOf(az) };
-            atomicOr(&s_bits[shapeWord(L, e[kPerm[lane][1]], e[kPerm[lane][2]])], 1u << e[kPerm[lane][0]]);
+            atomicOr(&s_bits[shapeWord(L, e[kPerm[lane][1]], e[kPerm[lane][2]])], 1u << shapeBit(e[kPerm[lane][0]]));
         }
     }
     __syncthreads();
@@ -360,17 +360,7 @@ __global__ __launch_bounds__(64 * Waves) void kSearch(const uint64_t* __restrict
     if constexpr (ShapeOn) {
         const bool parentPerms = (cfgBits & kCfgAxis) != 0 && (cfgBits & kCfgChildFrames) == 0;
         if (shapeBits != nullptr && !parentPerms) {
-            for (int i = (int)threadIdx.x; i < shapeWords<L>(); i += 64 * Waves)
-                s_shape[i] = shapeBits[i];
-            for (int i = (int)threadIdx.x; i < 2 * G::R; i += 64 * Waves) {
-                const int dy = i / G::R, a2 = i % G::R;
-                uint32_t acc = 0;
-                for (int k = 0; k < G::R; ++k) {
-                    const int a1 = dy ? G::R - 1 - k : k;
-                    acc |= (uint32_t)s_rows[a1 + G::Pitch * a2];
-                    s_pre[dy * G::R * G::Pitch + a1 * G::Pitch + a2] = acc;
-                }
-            }
+            buildShapeTables<L>(s_rows, shapeBits, s_shape, s_pre, (int)threadIdx.x, 64 * Waves);
             __syncthreads();
             ShapeLds sh;
             sh.pre = s_pre;

@@ -83,7 +83,7 @@ struct LevelView {
     uint64_t* tabKeys; // level 2: keys; levels >= 3: packed (hash << 32 | rank << 3 | corner)
     uint32_t* tabVals; // level 2 only
     uint32_t* bloom;
-    uint32_t* shape; // levels >= 3: the children's extent triples, bit e0 of word e1 + R * e2 (extents - 1)
+    uint32_t* shape; // levels >= 3: the children's extent triples (spans e = extent - 1), bit shapeBit(e0) of word shapeWord(e1, e2)
     unsigned long long* counters;
     uint32_t logCap, logBloom;
     uint32_t childFrames;

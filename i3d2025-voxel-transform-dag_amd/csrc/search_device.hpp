@@ -21,6 +21,16 @@ __device__ __forceinline__ uint64_t bestHint(const uint64_t* p)
     return *p;
 }
 
+// threadIdx.x through an empty asm: to the compiler a new value at every call, so values derived
+// from the lane are computed where they are used. In the persistent batched kernel they would
+// otherwise be hoisted out of its tile loop and stay live in registers across every tile.
+__device__ __forceinline__ uint32_t threadX()
+{
+    uint32_t t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
+}
+
 __device__ __forceinline__ bool configAllows(uint32_t cfgBits, uint32_t permCode, uint32_t sym, int tx, int ty, int tz)
 {
     if (!(cfgBits & kCfgAxis) && permCode != 0)
@@ -93,7 +103,7 @@ __device__ __forceinline__ void flushCounts(const ProbeCounts& c, unsigned long 
     if (!(cfgBits & kCfgCount))
         return;
     const uint32_t v[6] = { waveSum(c.probes), waveSum(c.filterPass), waveSum(c.keyHits), waveSum(c.candidates), waveSum(c.verifications), waveSum(c.verified) };
-    if ((threadIdx.x & 63u) == 0)
+    if ((threadX() & 63u) == 0)
         for (int i = 0; i < 6; ++i)
             if (v[i])
                 atomicAdd(&counters[2 + i], (unsigned long long)v[i]);
@@ -122,7 +132,7 @@ __device__ __forceinline__ void walkQueue2(const uint64_t* qKey, const uint16_t*
     const uint64_t* __restrict__ sortedMasks, const uint64_t* __restrict__ tabKeys, const uint32_t* __restrict__ tabVals, uint32_t logCap,
     uint64_t* __restrict__ best, uint32_t cfgBits, ProbeCounts& cnt)
 {
-    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t lane = threadX() & 63u;
     const uint32_t capMask = (1u << logCap) - 1u;
     for (uint32_t base = 0; base < count; base += 64) {
         if (base + lane >= count)
@@ -175,7 +185,7 @@ __device__ __forceinline__ void search2ParentQueued(uint32_t r, const uint64_t* 
     const uint32_t* __restrict__ tabVals, uint32_t logCap, const uint32_t* __restrict__ bloom, uint32_t logBloom, uint64_t* __restrict__ best, uint32_t cfgBits,
     uint64_t* qKey, uint16_t* qMeta, ProbeCounts& cnt)
 {
-    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t lane = threadX() & 63u;
     const uint64_t P = sortedMasks[r];
     if (P == 0)
         return;
@@ -293,7 +303,7 @@ __device__ __forceinline__ bool loadRows(const uint64_t* __restrict__ layouts, u
     using Row = typename G::Row;
     const Row* g = reinterpret_cast<const Row*>(layouts);
     int any = 0;
-    for (int i = (int)threadIdx.x; i < numLayouts * G::R2; i += Threads) {
+    for (int i = (int)threadX(); i < numLayouts * G::R2; i += Threads) {
         const int lay = i >> (2 * L), j = i & (G::R2 - 1);
         const Row v = g[((size_t)lay * n + r) * G::R2 + j];
         s_rows[lay * G::Rows + j + (j >> L)] = v;
@@ -342,7 +352,7 @@ __device__ __forceinline__ bool loadRowsOcc(const uint64_t* __restrict__ layouts
     constexpr int Waves = Threads / 64;
     const Row* g = reinterpret_cast<const Row*>(layouts) + ((size_t)a0 * n + r) * G::R2;
     uint32_t acc0 = 0, accLo = 0, accHi = 0;
-    for (int j = (int)threadIdx.x; j < G::R2; j += Threads) {
+    for (int j = (int)threadX(); j < G::R2; j += Threads) {
         const Row v = g[j];
         s_rows[j + (j >> L)] = v;
         occupancyAdd((uint32_t)v, j, L, acc0, accLo, accHi);
@@ -350,8 +360,8 @@ __device__ __forceinline__ bool loadRowsOcc(const uint64_t* __restrict__ layouts
     acc0 = waveOr(acc0);
     accLo = waveOr(accLo);
     accHi = waveOr(accHi);
-    const int wave = (int)(threadIdx.x >> 6);
-    if ((threadIdx.x & 63u) == 0) {
+    const int wave = (int)(threadX() >> 6);
+    if ((threadX() & 63u) == 0) {
         s_occ[3 * wave + 0] = acc0;
         s_occ[3 * wave + 1] = accLo;
         s_occ[3 * wave + 2] = accHi;
@@ -387,7 +397,7 @@ __device__ __forceinline__ bool loadRowsWave(const uint64_t* __restrict__ layout
     using Row = typename G::Row;
     const Row* g = reinterpret_cast<const Row*>(layouts) + ((size_t)a0 * n + r) * G::R2;
     uint32_t acc0 = 0, accLo = 0, accHi = 0;
-    for (int j = (int)(threadIdx.x & 63u); j < G::R2; j += 64) {
+    for (int j = (int)(threadX() & 63u); j < G::R2; j += 64) {
         const Row v = g[j];
         s_rows[j + (j >> L)] = v;
         occupancyAdd((uint32_t)v, j, L, acc0, accLo, accHi);
@@ -412,7 +422,7 @@ __device__ __forceinline__ int emptyLead(uint32_t occ, uint32_t pos)
 // to the transformed content of the parent's anchored sub-box, so the sub-box content has the
 // child's extents (the span of occupied coordinates along each axis) up to an axis permutation.
 // The extents of every child in the table, inserted under all six axis orders, form a bitmap of
-// R^2 words of R bits (word e1 + R * e2, bit e0 for extents e - 1); a sub-box whose extent triple is not in it
+// R^2 words of R bits (the bit of spans e = extent - 1 at shapeWord and shapeBit); a sub-box whose extent triple is not in it
 // cannot lead to a verified match, so its Bloom word is not loaded. The extents of a sub-box are
 // derived along the three passes: along A2 from the pass-2 states read by pass 3, along A1 from
 // per-slab masks of the non-empty row segments that pass 2 read, along A0 from the OR of the
@@ -423,10 +433,17 @@ constexpr int shapeWords()
     return 1 << (2 * L);
 }
 
-// Word of extent pair (e1, e2) (each extent - 1 < R); bit e0 of it marks the triple.
+// Positions of a triple (e0, e1, e2) of spans (extent - 1, each < R) in the bitmap: bit 31 - e0
+// of word R*R - 1 - (e1 + R*e2). The complemented positions let the search form them from
+// leading plus trailing zero counts: for the occupancy x != 0 of an axis, 31 - span(x) =
+// clz(x) + ctz(x), so no subtraction is needed.
 __host__ __device__ __forceinline__ uint32_t shapeWord(uint32_t L, uint32_t e1, uint32_t e2)
 {
-    return e1 | (e2 << L);
+    return (1u << (2 * L)) - 1u - (e1 + (e2 << L));
+}
+__host__ __device__ __forceinline__ uint32_t shapeBit(uint32_t e0)
+{
+    return 31u - e0;
 }
 
 // Extent - 1 of the set bits of x != 0 (bare v_ffbh / v_ffbl: for x == 0 the value is
@@ -436,12 +453,52 @@ __device__ __forceinline__ uint32_t spanOf(uint32_t x)
     return 31u - (uint32_t)__builtin_clz(x) - (uint32_t)__builtin_ctz(x);
 }
 
+// 31 - spanOf(x), the complemented span, for x != 0 (meaningless for x == 0, as spanOf).
+__device__ __forceinline__ uint32_t spanCompl(uint32_t x)
+{
+    return (uint32_t)__builtin_clz(x) + (uint32_t)__builtin_ctz(x);
+}
+
 // LDS views of the shape pre-filter of one block (all null: the filter is off).
 struct ShapeLds {
     const uint32_t* pre = nullptr; // [2][R * Pitch]: OR over a1 in [0, c1] (index 0) / [c1, R) (index 1) of row (a1, a2), at c1 * Pitch + a2
     uint32_t* n1 = nullptr; // [Waves][P][R]: for plane w of a wave and line a2, bit k set iff pass-2 step k read a non-empty segment
     const uint32_t* bits = nullptr; // shapeWords<L>() words
 };
+
+// LDS words of the pre tables and of the n1 masks of `waves` waves.
+template <int L>
+constexpr int shapePreWords()
+{
+    return 2 * Geo<L>::R * Geo<L>::Pitch;
+}
+template <int L>
+constexpr int shapeN1Words(int waves)
+{
+    return waves * Geo<L>::P * Geo<L>::R;
+}
+
+// Fills the shape pre-filter's per-parent LDS: the extent bitmap of the level (copied from HBM)
+// and the pre tables of the parent whose layout-0 rows (rows along x, (a1, a2) = (y, z)) are in
+// s_rows. Threads tid = 0..threads-1 of the caller share the work; the caller synchronises
+// before the tables are read.
+template <int L>
+__device__ __forceinline__ void buildShapeTables(const typename Geo<L>::Row* s_rows, const uint32_t* __restrict__ shapeBits, uint32_t* s_shape, uint32_t* s_pre,
+    int tid, int threads)
+{
+    using G = Geo<L>;
+    for (int i = tid; i < shapeWords<L>(); i += threads)
+        s_shape[i] = shapeBits[i];
+    for (int i = tid; i < 2 * G::R; i += threads) {
+        const int dy = i / G::R, a2 = i % G::R;
+        uint32_t acc = 0;
+        for (int k = 0; k < G::R; ++k) {
+            const int a1 = dy ? G::R - 1 - k : k;
+            acc |= (uint32_t)s_rows[a1 + G::Pitch * a2];
+            s_pre[dy * G::R * G::Pitch + a1 * G::Pitch + a2] = acc;
+        }
+    }
+}
 
 // Value of the first pass (along A0) at coordinate c0 of a row: the trimmed bit sequence of the
 // row segment read from the scan start toward c0 is the binary number formed by those bits, so
@@ -591,7 +648,12 @@ __device__ __forceinline__ uint32_t pass3Filter(const uint32_t* in, uint32_t* ou
     for (int k = 0; k < R; ++k)
         st[k] = in[Geo<L>::Pitch * (Pos2 ? (R - 1 - k) : k)];
     uint32_t s = 0, pending = 0;
-    uint32_t acc0 = 0, acc1 = 0, first2 = (uint32_t)R, last2 = 0; // shape state of the box so far
+    // Shape state of the box so far: its occupancy along A0 and along A1 (pass-2 steps), both
+    // masked to the box's range; 4R times its first non-empty slab along A2; and 4 times the
+    // bitmap word index of the box without the A1 term, set at every non-empty slab (the span
+    // along A2 ends there): word = clz(occ1) + ctz(occ1) + R*first2 + R*R - 32 - R*last2 (see
+    // shapeWord).
+    uint32_t occ0 = 0, occ1 = 0, first4 = 4u * R * R, word4 = 0;
     uint32_t pf[C], pbw[C]; // the previous chunk's hashes and filter words (0: nothing passes)
 #pragma unroll
     for (int k = 0; k < C; ++k)
@@ -608,16 +670,19 @@ __device__ __forceinline__ uint32_t pass3Filter(const uint32_t* in, uint32_t* ou
             out[Geo<L>::Pitch * (Pos2 ? (R - 1 - k0 - k) : k0 + k)] = f[k];
             uint32_t fq = f[k];
             if constexpr (Shape) {
-                const int a2 = Pos2 ? (R - 1 - k0 - k) : k0 + k;
-                acc0 |= preLine[a2];
-                acc1 |= n1Plane[a2];
-                const bool slab = st[k0 + k] != 0u;
-                first2 = min(first2, slab ? (uint32_t)(k0 + k) : (uint32_t)R);
-                last2 = slab ? (uint32_t)(k0 + k) : last2;
-                // Bit e0 of word (e1, e2), sign-extended to 0 or all ones (v_bfe_i32); the word
-                // index is masked so that empty boxes (meaningless spans, f == 0) stay in bounds.
-                const uint32_t word = shapeWord(L, spanOf(acc1 & mask1), last2 - first2) & (uint32_t)(shapeWords<L>() - 1);
-                fq &= (uint32_t)__builtin_amdgcn_sbfe((int)shapeBits[word], spanOf(acc0 & mask0), 1u);
+                const int kk = k0 + k;
+                const int a2 = Pos2 ? (R - 1 - kk) : kk;
+                occ0 |= preLine[a2] & mask0;
+                occ1 |= n1Plane[a2] & mask1;
+                const bool slab = st[kk] != 0u;
+                first4 = min(first4, slab ? 4u * (uint32_t)(R * kk) : 4u * R * R);
+                word4 = slab ? first4 + 4u * (uint32_t)(R * R - 32 - R * kk) : word4;
+                // Bit 31 - e0 of the box's word, sign-extended to 0 or all ones (v_bfe_i32); the
+                // byte offset is masked so that empty boxes (meaningless spans, f == 0) stay in
+                // bounds.
+                const uint32_t off = ((spanCompl(occ1) << 2) + word4) & (uint32_t)(4 * shapeWords<L>() - 4);
+                const uint32_t wordBits = *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(shapeBits) + off);
+                fq &= (uint32_t)__builtin_amdgcn_sbfe((int)wordBits, spanCompl(occ0), 1u);
             }
             bw[k] = bloomWordOrZero(rs, fq, shift);
         }
@@ -718,7 +783,7 @@ __device__ __forceinline__ void probeQueue(const ProbeList& list, int c0Base, co
 {
     using G = Geo<L>;
     constexpr int R = G::R;
-    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t lane = threadX() & 63u;
     const uint32_t count = list.count;
     const uint32_t capMask = (1u << logCap) - 1u;
     for (uint32_t base = 0; base < count; base += 64) {
@@ -831,7 +896,7 @@ __device__ __forceinline__ void probePending(uint32_t pending, ProbeList& list, 
     ProbeCounts& cnt)
 {
     using G = Geo<L>;
-    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t lane = threadX() & 63u;
     const uint32_t u = lane & (G::R - 1), w = lane >> L;
     for (;;) {
         const uint64_t has = __ballot(pending != 0u);
@@ -923,7 +988,7 @@ __device__ __forceinline__ void searchUnit(uint32_t r, uint32_t unit, uint32_t c
     // (kCfgChildFrames) or permutations are off. The shape pre-filter runs only without them
     // (permutation code 0: A0, A1, A2 = x, y, z).
     const bool parentPerms = (cfgBits & kCfgAxis) != 0 && (cfgBits & kCfgChildFrames) == 0;
-    const int lane = (int)(threadIdx.x & 63u);
+    const int lane = (int)(threadX() & 63u);
     const int u = lane & (R - 1), w = lane >> L;
     const int c0Base = ((int)chunk * Waves + wave) * G::P;
     const int c0 = c0Base + w;
