@@ -54,7 +54,7 @@ struct BlockTileLds {
     static constexpr size_t pre = planes + (size_t)kBatchWaves * Geo<L>::P * Geo<L>::Slab * 4;
     static constexpr size_t n1 = pre + (size_t)shapePreWords<L>() * 4;
     static constexpr size_t bits = n1 + (size_t)shapeN1Words<L>(kBatchWaves) * 4;
-    static constexpr size_t bytes = bits + (size_t)shapeWords<L>() * 4;
+    static constexpr size_t bytes = bits + (size_t)shapeLdsWords<L>() * 4;
 };
 
 // LDS slice of one wave of a level-3 wave item: rows, two plane buffers, then the shape
@@ -65,7 +65,7 @@ struct Wave3Lds {
     static constexpr size_t pre = planes1 + (size_t)Geo<3>::P * Geo<3>::Slab * 4;
     static constexpr size_t n1 = pre + (size_t)shapePreWords<3>() * 4;
     static constexpr size_t bits = n1 + (size_t)shapeN1Words<3>(1) * 4;
-    static constexpr size_t bytes = alignUp16(bits + (size_t)shapeWords<3>() * 4);
+    static constexpr size_t bytes = alignUp16(bits + (size_t)shapeLdsWords<3>() * 4);
 };
 constexpr size_t kL3WaveBytes = Wave3Lds::bytes;
 

@@ -191,13 +191,9 @@ TDAG_HD uint32_t bloomWord(uint32_t f, uint32_t logWords) { return f >> (32 - lo
 TDAG_HD uint32_t bloomBits(uint32_t f) { return (1u << (f & 31u)) | (1u << ((f >> 5) & 31u)); }
 // 1 when both bits of f are set in the filter word bw.
 TDAG_HD uint32_t bloomHit(uint32_t bw, uint32_t f) { return (bw >> (f & 31u)) & (bw >> ((f >> 5) & 31u)) & 1u; }
-// Word of a level-3..5 nested hash. Word 0 is never set (its keys share word 1), so the hash 0 of
-// an empty region always misses the filter and needs no test of its own.
-TDAG_HD uint32_t bloomWordNested(uint32_t f, uint32_t logWords)
-{
-    const uint32_t w = f >> (32 - logWords);
-    return w > 1u ? w : 1u;
-}
+// Word of a level-3..5 nested hash: its top bits. (The search loads no word for the hash 0 of an
+// empty region: its buffer offset is out of range.)
+TDAG_HD uint32_t bloomWordNested(uint32_t f, uint32_t logWords) { return f >> (32 - logWords); }
 TDAG_HD uint32_t bloomMix64(uint64_t key) { return (uint32_t)((key * 0xD6E8FEB86659FD93ull) >> 32); }
 
 } // namespace tdag

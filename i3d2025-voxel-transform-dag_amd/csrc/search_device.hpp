@@ -432,6 +432,13 @@ constexpr int shapeWords()
 {
     return 1 << (2 * L);
 }
+// LDS words of a block's copy of the bitmap: the R^2 words, then one word that stays 0 (an
+// empty sub-box reads it, so it never passes).
+template <int L>
+constexpr int shapeLdsWords()
+{
+    return shapeWords<L>() + 1;
+}
 
 // Positions of a triple (e0, e1, e2) of spans (extent - 1, each < R) in the bitmap: bit 31 - e0
 // of word R*R - 1 - (e1 + R*e2). The complemented positions let the search form them from
@@ -453,17 +460,21 @@ __device__ __forceinline__ uint32_t spanOf(uint32_t x)
     return 31u - (uint32_t)__builtin_clz(x) - (uint32_t)__builtin_ctz(x);
 }
 
-// 31 - spanOf(x), the complemented span, for x != 0 (meaningless for x == 0, as spanOf).
+// 31 - spanOf(x), the complemented span, for x != 0; 0xFFFFFFFE for x == 0 (v_ffbh_u32 and
+// v_ffbl_b32 both return 0xFFFFFFFF for 0, a defined result here, unlike __builtin_clz(0)).
 __device__ __forceinline__ uint32_t spanCompl(uint32_t x)
 {
-    return (uint32_t)__builtin_clz(x) + (uint32_t)__builtin_ctz(x);
+    uint32_t hi, lo;
+    asm("v_ffbh_u32 %0, %1" : "=v"(hi) : "v"(x));
+    asm("v_ffbl_b32 %0, %1" : "=v"(lo) : "v"(x));
+    return hi + lo;
 }
 
 // LDS views of the shape pre-filter of one block (all null: the filter is off).
 struct ShapeLds {
     const uint32_t* pre = nullptr; // [2][R * Pitch]: OR over a1 in [0, c1] (index 0) / [c1, R) (index 1) of row (a1, a2), at c1 * Pitch + a2
     uint32_t* n1 = nullptr; // [Waves][P][R]: for plane w of a wave and line a2, bit k set iff pass-2 step k read a non-empty segment
-    const uint32_t* bits = nullptr; // shapeWords<L>() words
+    const uint32_t* bits = nullptr; // shapeLdsWords<L>() words
 };
 
 // LDS words of the pre tables and of the n1 masks of `waves` waves.
@@ -489,6 +500,8 @@ __device__ __forceinline__ void buildShapeTables(const typename Geo<L>::Row* s_r
     using G = Geo<L>;
     for (int i = tid; i < shapeWords<L>(); i += threads)
         s_shape[i] = shapeBits[i];
+    if (tid == 0)
+        s_shape[shapeWords<L>()] = 0u;
     for (int i = tid; i < 2 * G::R; i += threads) {
         const int dy = i / G::R, a2 = i % G::R;
         uint32_t acc = 0;
@@ -510,6 +523,15 @@ template <bool Pos>
 __device__ __forceinline__ uint32_t rowPrefix(uint32_t row, int sh)
 {
     return Pos ? (row >> sh) : (__brev(row) >> sh);
+}
+
+// 1 if x != 0, else 0, as one v_min_u32 (written as a compare, the compiler selects between
+// constants, and the shifted ones of an unrolled sweep end up in registers of their own).
+__device__ __forceinline__ uint32_t nonZeroBit(uint32_t x)
+{
+    uint32_t b;
+    asm("v_min_u32 %0, 1, %1" : "=v"(b) : "v"(x));
+    return b;
 }
 
 // Passes 1 and 2 of the nested hash on plane c0 (the coordinate along the first pass axis A0):
@@ -548,7 +570,7 @@ __device__ __forceinline__ void pass12(const typename Geo<L>::Row* s_rows, int c
             s = hashStep(s, seg);
             line[Pos1 ? (R - 1 - k) : k] = s;
             if constexpr (Shape)
-                nz |= min(seg, 1u) << k;
+                nz |= nonZeroBit(seg) << k;
         }
     }
     if constexpr (Shape)
@@ -614,13 +636,22 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t bloomRsrc(const uint32_t* bloo
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(bloom), (short)0, (int)(4u << logBloom), 0x00020000);
 }
 
-// Filter word of nested hash f (its top bits, word 0 folded into word 1), or 0 for f == 0: an
-// empty region's hash is never in the table, and its lane then issues no memory request (the
-// offset is out of range), which keeps the filter's cache-line traffic to the non-empty voxels.
+// Filter word of nested hash f (bloomWordNested), or 0 for f == 0: an empty region's hash is
+// never in the table, and its lane then issues no memory request (the offset is out of range),
+// which keeps the filter's cache-line traffic to the non-empty voxels.
 __device__ __forceinline__ uint32_t bloomWordOrZero(__amdgpu_buffer_rsrc_t rs, uint32_t f, uint32_t shift)
 {
-    const uint32_t w = f >> shift;
-    const uint32_t off = f != 0u ? (w > 1u ? w : 1u) * 4u : 0xFFFFFFF0u;
+    const uint32_t off = f != 0u ? (f >> shift) * 4u : 0xFFFFFFF0u;
+    return __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0);
+}
+
+// Filter word of nested hash f where m (0 or all ones) says whether the lane loads it: the byte
+// offset (f >> shift) * 4 with bit 31 set when m == 0, out of range (no memory request, 0
+// returned). Offsets are below 2^24, so one bitwise select of three operands forms it: the
+// offset bits from f >> (shift - 2), bit 31 and the two low bits from ~m.
+__device__ __forceinline__ uint32_t bloomWordMasked(__amdgpu_buffer_rsrc_t rs, uint32_t f, uint32_t shift, uint32_t m)
+{
+    const uint32_t off = ((f >> (shift - 2u)) & 0x7FFFFFFCu) | (~m & 0x80000003u);
     return __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0);
 }
 
@@ -628,8 +659,9 @@ __device__ __forceinline__ uint32_t bloomWordOrZero(__amdgpu_buffer_rsrc_t rs, u
 // with the Bloom filter: stores the final hash of every voxel at out[Pitch*c2] (out may equal in)
 // and returns the bit set of voxels c2 that are non-empty and pass the filter. Filter words are
 // loaded without branches (empty voxels read 0 through an out-of-range buffer offset), one chunk
-// of R/4 steps at a time, and tested while the next chunk hashes. Chunks inside the first `lead` steps (wave-uniform; their
-// states are 0) are skipped: their voxels are empty, never pending, and out keeps its old words.
+// of R/4 steps at a time, and tested while the next chunk hashes. Chunks inside the first `lead`
+// steps (wave-uniform; their states are 0) are skipped: their voxels are empty, never pending,
+// and out keeps its old words.
 // Shape (see ShapeLds): preLine = the pre table row of the lane's c1 for the combo's A1
 // direction, n1Plane = the n1 masks of the lane's plane, mask0 / mask1 = the box's range along A0
 // (coordinate bits) and A1 (pass-2 step bits); sub-boxes whose extents are not in the bitmap
@@ -652,8 +684,12 @@ __device__ __forceinline__ uint32_t pass3Filter(const uint32_t* in, uint32_t* ou
     // masked to the box's range; 4R times its first non-empty slab along A2; and 4 times the
     // bitmap word index of the box without the A1 term, set at every non-empty slab (the span
     // along A2 ends there): word = clz(occ1) + ctz(occ1) + R*first2 + R*R - 32 - R*last2 (see
-    // shapeWord).
-    uint32_t occ0 = 0, occ1 = 0, first4 = 4u * R * R, word4 = 0;
+    // shapeWord). Before the first non-empty slab occ1 == 0 (its spanCompl is 0xFFFFFFFE), and
+    // word4 = 8 + 4R^2 makes the byte offset 4R^2: an empty box reads the bitmap's zero word and
+    // never passes. A slab is non-empty exactly when some row segment of it in the box's A1
+    // range is, so occ1 != 0 from then on and the offset stays below 4R^2. A box that passes is
+    // thus non-empty (f != 0), and the filter load takes the pass bit alone (bloomWordMasked).
+    uint32_t occ0 = 0, occ1 = 0, first4 = 4u * R * R, word4 = 8u + 4u * R * R;
     uint32_t pf[C], pbw[C]; // the previous chunk's hashes and filter words (0: nothing passes)
 #pragma unroll
     for (int k = 0; k < C; ++k)
@@ -668,23 +704,24 @@ __device__ __forceinline__ uint32_t pass3Filter(const uint32_t* in, uint32_t* ou
             s = hashStep(s, st[k0 + k]);
             f[k] = finalHash(s);
             out[Geo<L>::Pitch * (Pos2 ? (R - 1 - k0 - k) : k0 + k)] = f[k];
-            uint32_t fq = f[k];
             if constexpr (Shape) {
                 const int kk = k0 + k;
                 const int a2 = Pos2 ? (R - 1 - kk) : kk;
                 occ0 |= preLine[a2] & mask0;
                 occ1 |= n1Plane[a2] & mask1;
+                // Selects between registers only (a select between two constants would keep one
+                // of them in a register of its own for every step of the unrolled sweep).
                 const bool slab = st[kk] != 0u;
-                first4 = min(first4, slab ? 4u * (uint32_t)(R * kk) : 4u * R * R);
+                first4 = slab ? min(first4, 4u * (uint32_t)(R * kk)) : first4;
                 word4 = slab ? first4 + 4u * (uint32_t)(R * R - 32 - R * kk) : word4;
-                // Bit 31 - e0 of the box's word, sign-extended to 0 or all ones (v_bfe_i32); the
-                // byte offset is masked so that empty boxes (meaningless spans, f == 0) stay in
-                // bounds.
-                const uint32_t off = ((spanCompl(occ1) << 2) + word4) & (uint32_t)(4 * shapeWords<L>() - 4);
+                // Bit 31 - e0 of the box's word, sign-extended to 0 or all ones (v_bfe_i32).
+                const uint32_t off = (spanCompl(occ1) << 2) + word4;
                 const uint32_t wordBits = *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(shapeBits) + off);
-                fq &= (uint32_t)__builtin_amdgcn_sbfe((int)wordBits, spanCompl(occ0), 1u);
+                const uint32_t pass = (uint32_t)__builtin_amdgcn_sbfe((int)wordBits, spanCompl(occ0), 1u);
+                bw[k] = bloomWordMasked(rs, f[k], shift, pass);
+            } else {
+                bw[k] = bloomWordOrZero(rs, f[k], shift);
             }
-            bw[k] = bloomWordOrZero(rs, fq, shift);
         }
         if (k0 > 0) {
 #pragma unroll
@@ -956,7 +993,7 @@ constexpr size_t searchLdsBase()
 template <int L, int Waves>
 constexpr size_t shapeLds()
 {
-    return (size_t)2 * Geo<L>::R * Geo<L>::Pitch * 4 + (size_t)Waves * Geo<L>::P * Geo<L>::R * 4 + (size_t)shapeWords<L>() * 4;
+    return (size_t)2 * Geo<L>::R * Geo<L>::Pitch * 4 + (size_t)Waves * Geo<L>::P * Geo<L>::R * 4 + (size_t)shapeLdsWords<L>() * 4;
 }
 template <int L, int Waves, int Kind>
 constexpr bool shapeFits()
