@@ -322,7 +322,8 @@ def bench_batched(ctx, cfg, dev, order, args):
         torch.cuda.synchronize()
         b = ctx.batch(jobs, cfg, use_graph=True, concurrent=concurrent)
         try:
-            grab = b.autotune(1)
+            b.autotune(1)
+            grab = b.tuning()
             for _ in range(max(1, args.warmup)):
                 b.run()
             t0 = time.perf_counter()
@@ -352,7 +353,8 @@ def bench_batched(ctx, cfg, dev, order, args):
         "form": "one persistent kernel for all levels (device work queue, phases = popcount buckets of parents), hipGraph replay",
         "ms_per_step": round(elapsed * 1e3 / max(1, args.steps), 4),
         "matches_per_s": matches / elapsed if elapsed > 0 else None,
-        "tiles_per_pop": grab,
+        "queue_tuning": {"tiles_per_pop": grab[0], "split_every": grab[1],
+                         "note": "autotuned per run: tiles per queue pop, then one block in split_every takes the lighter levels' tiles first (0: none)"},
         "bit_exact_vs_per_level": exact_vs_per_level(outs),
     }
     outs, _, matches, elapsed = run_form(True)

@@ -74,6 +74,7 @@ constexpr size_t kBatchLds = std::max(BlockTileLds<5>::bytes, std::max(BlockTile
 static_assert(kBatchLds <= kLdsBytes, "batched tile LDS");
 
 constexpr uint32_t kL5Chunks = Geo<5>::MaxWaves / kBatchWaves; // plane chunks of a level-5 tile
+constexpr uint32_t kNoTile = 0xFFFFFFFFu; // a queue pop that found both queues drained
 static_assert(Geo<4>::MaxWaves == kBatchWaves, "a level-4 tile covers all planes");
 static_assert(Geo<3>::MaxWaves == 1, "a level-3 wave item covers all planes");
 
@@ -236,18 +237,43 @@ __global__ __launch_bounds__(kBatchThreads, 3) void kBatched(const BatchDesc* __
     const uint64_t gstride = (uint64_t)gridDim.x * kBatchThreads;
     for (uint32_t p = 0; p < numPhases; ++p) {
         const uint32_t total = desc->totalTiles[p];
+        const uint32_t split = desc->splitTile[p];
         const uint32_t grab = desc->grab;
+        // Queue A holds tiles [0, split) (the heaviest level), queue B tiles [split, total). A
+        // block takes from its preferred queue while it has tiles, then from the other one; the
+        // phase ends for the block when both are drained.
+        const bool preferB = split < total && desc->splitEvery != 0u && blockIdx.x % desc->splitEvery == desc->splitEvery - 1u;
+        bool doneA = split == 0u, doneB = split >= total;
         for (;;) {
-            if (threadIdx.x == 0)
-                s_tile = atomicAdd(&sync->queue[p], grab);
+            if (threadIdx.x == 0) {
+                uint32_t t = kNoTile;
+                for (int attempt = 0; attempt < 2 && t == kNoTile; ++attempt) {
+                    if ((attempt == 0) == preferB) {
+                        if (!doneB) {
+                            const uint32_t x = split + atomicAdd(&sync->queueB[p], grab);
+                            if (x < total)
+                                t = x;
+                            else
+                                doneB = true;
+                        }
+                    } else if (!doneA) {
+                        const uint32_t x = atomicAdd(&sync->queue[p], grab);
+                        if (x < split)
+                            t = x;
+                        else
+                            doneA = true;
+                    }
+                }
+                s_tile = t;
+            }
             __syncthreads();
             // Uniform over the block: in a scalar register, the tile's level view, parent and
             // unit stay scalar too (their pointers in SGPRs, the descriptor read with scalar loads).
             const uint32_t t0 = __builtin_amdgcn_readfirstlane(s_tile);
             __syncthreads();
-            if (t0 >= total)
+            if (t0 == kNoTile)
                 break;
-            const uint32_t tEnd = min(total, t0 + grab);
+            const uint32_t tEnd = min(t0 < split ? split : total, t0 + grab);
             for (uint32_t t = t0; t < tEnd; ++t) {
                 if (t != t0)
                     __syncthreads(); // the previous tile's waves are done with the LDS
@@ -365,10 +391,11 @@ hipError_t planBatch(uint32_t count, const uint32_t* levels, const uint64_t* ns,
     return hipSuccess;
 }
 
-void fillBatchDesc(const BatchPlan& b, void* d_scratch, uint32_t cfgBits, uint32_t grab, BatchDesc* desc)
+void fillBatchDesc(const BatchPlan& b, void* d_scratch, uint32_t cfgBits, uint32_t grab, BatchDesc* desc, uint32_t splitEvery)
 {
     *desc = BatchDesc {};
     desc->grab = std::max<uint32_t>(1u, grab);
+    desc->splitEvery = splitEvery;
     desc->numLevels = b.count;
     desc->numPhases = b.numPhases;
     desc->cfgBits = cfgBits;
@@ -392,6 +419,7 @@ void fillBatchDesc(const BatchPlan& b, void* d_scratch, uint32_t cfgBits, uint32
             base += pl.tiles;
         }
         desc->totalTiles[p] = base;
+        desc->splitTile[p] = b.count > 1 ? desc->ph[p][1].tileBase : base;
     }
 }
 

@@ -12,6 +12,8 @@ namespace tdag {
 
 constexpr uint32_t kBatchMaxLevels = 4;
 constexpr uint32_t kBatchMaxPhases = 32;
+// Default share of blocks that start on the lighter levels' queue (one in kBatchSplitEvery).
+constexpr uint32_t kBatchSplitEvery = 3;
 
 // Work of one level in one phase: parents [r0, r1) as tiles [tileBase, tileBase + tiles) of the
 // phase's queue; after the phase, children whose best key is below pruneKey leave the table
@@ -26,15 +28,21 @@ struct BatchPhaseLevel {
 struct BatchDesc {
     uint32_t numLevels, numPhases, cfgBits;
     uint32_t grab; // consecutive tiles a block takes per queue pop (tuned)
+    // Every splitEvery-th block (0: none) takes its tiles from the phase's second queue first:
+    // tiles [splitTile, totalTiles) of the lighter levels, so those run beside the heaviest
+    // level's tiles [0, splitTile) through the whole phase instead of after them.
+    uint32_t splitEvery, pad;
     LevelView lv[kBatchMaxLevels];
     uint32_t totalTiles[kBatchMaxPhases];
+    uint32_t splitTile[kBatchMaxPhases];
     BatchPhaseLevel ph[kBatchMaxPhases][kBatchMaxLevels];
 };
 
 // Queue heads, grid-barrier state and the error flag of one batched launch (reset by the
 // enqueue, in-graph, with a fill kernel).
 struct BatchSync {
-    uint32_t queue[kBatchMaxPhases];
+    uint32_t queue[kBatchMaxPhases]; // heads of the heaviest level's tiles
+    uint32_t queueB[kBatchMaxPhases]; // heads of the other levels' tiles, counted from splitTile
     uint32_t arrive, generation, error, pad;
 };
 
@@ -55,8 +63,9 @@ struct BatchPlan {
 hipError_t planBatch(uint32_t count, const uint32_t* levels, const uint64_t* ns, const uint32_t* firstBatch, const uint32_t* bloomBits, const uint32_t* childFrames,
     int device, BatchPlan* out);
 
-// Writes the BatchDesc for scratch base `d_scratch` into host memory `desc`.
-void fillBatchDesc(const BatchPlan& plan, void* d_scratch, uint32_t cfgBits, uint32_t grab, BatchDesc* desc);
+// Writes the BatchDesc for scratch base `d_scratch` into host memory `desc` (splitEvery: see
+// BatchDesc).
+void fillBatchDesc(const BatchPlan& plan, void* d_scratch, uint32_t cfgBits, uint32_t grab, BatchDesc* desc, uint32_t splitEvery = kBatchSplitEvery);
 
 // Enqueues prepare (all levels), the persistent search and finish (all levels) on `stream`. The
 // BatchDesc must already be at d_scratch + plan.descOff. Array arguments are indexed by the

@@ -90,6 +90,7 @@ struct tdag_batch {
     uint64_t spinTicks = 0;
     uint32_t cfgBits = 0;
     uint32_t grab = 1;
+    uint32_t splitEvery = tdag::kBatchSplitEvery;
     // Concurrent mode: every level's own tuned matcher (the per-level plan and kernels) on its own
     // stream, forked from and joined back into the context's stream (graph branches when captured).
     bool concurrent = false;
@@ -552,7 +553,7 @@ namespace {
 tdag_status uploadBatchDesc(tdag_batch* b)
 {
     tdag::BatchDesc desc;
-    tdag::fillBatchDesc(b->plan, b->scratch, b->cfgBits, b->grab, &desc);
+    tdag::fillBatchDesc(b->plan, b->scratch, b->cfgBits, b->grab, &desc, b->splitEvery);
     CAPI_HIP(hipMemcpy((char*)b->scratch + b->plan.descOff, &desc, sizeof(desc), hipMemcpyHostToDevice), "batch descriptor upload");
     return TDAG_OK;
 }
@@ -856,31 +857,80 @@ tdag_status tdag_batch_autotune(tdag_batch* b, uint32_t repeats, uint32_t* out_g
         return TDAG_OK;
     }
     repeats = std::max<uint32_t>(1, repeats);
-    const uint32_t cands[5] = { 1, 2, 4, 8, 16 };
-    uint32_t best = b->grab;
-    float bestMs = -1.f;
-    for (uint32_t c : cands) {
-        b->grab = c;
+    // Mean time of `repeats` runs after a warm-up with the batch's current tuning.
+    auto timeRuns = [&](float* outMs) -> tdag_status {
         if (tdag_status s = uploadBatchDesc(b); s != TDAG_OK)
             return s;
         float total = 0.f;
-        for (uint32_t rep = 0; rep <= repeats; ++rep) { // the first run is a warm-up
+        for (uint32_t rep = 0; rep <= repeats; ++rep) {
             float ms = 0.f;
             if (tdag_status s = tdag_batch_run(b, nullptr, &ms); s != TDAG_OK)
                 return s;
             if (rep > 0)
                 total += ms;
         }
-        if (bestMs < 0.f || total / repeats < bestMs) {
-            bestMs = total / repeats;
-            best = c;
+        *outMs = total / repeats;
+        return TDAG_OK;
+    };
+    // Tiles per pop first, then the share of blocks that start on the lighter levels' queue.
+    const uint32_t grabs[5] = { 1, 2, 4, 8, 16 };
+    uint32_t bestGrab = b->grab;
+    float bestMs = -1.f;
+    for (uint32_t c : grabs) {
+        b->grab = c;
+        float ms = 0.f;
+        if (tdag_status s = timeRuns(&ms); s != TDAG_OK)
+            return s;
+        if (bestMs < 0.f || ms < bestMs) {
+            bestMs = ms;
+            bestGrab = c;
         }
     }
-    b->grab = best;
+    b->grab = bestGrab;
+    const uint32_t splits[5] = { 0, 2, 3, 4, 6 };
+    uint32_t bestSplit = b->splitEvery;
+    bestMs = -1.f;
+    for (uint32_t c : splits) {
+        b->splitEvery = c;
+        float ms = 0.f;
+        if (tdag_status s = timeRuns(&ms); s != TDAG_OK)
+            return s;
+        if (bestMs < 0.f || ms < bestMs) {
+            bestMs = ms;
+            bestSplit = c;
+        }
+    }
+    b->splitEvery = bestSplit;
     if (tdag_status s = uploadBatchDesc(b); s != TDAG_OK)
         return s;
     if (out_grab)
-        *out_grab = best;
+        *out_grab = bestGrab;
+    return TDAG_OK;
+}
+
+tdag_status tdag_batch_set_tuning(tdag_batch* b, uint32_t grab, uint32_t split_every)
+{
+    if (!b)
+        return fail(TDAG_EINVAL, "batch is null");
+    if (grab == 0 || grab > 64)
+        return fail(TDAG_EINVAL, "tiles per pop must be 1..64");
+    if (split_every == 1 || split_every > 64)
+        return fail(TDAG_EINVAL, "split_every must be 0 (off) or 2..64");
+    b->grab = grab;
+    b->splitEvery = split_every;
+    if (b->concurrent || b->plan.count == 0)
+        return TDAG_OK;
+    return uploadBatchDesc(b);
+}
+
+tdag_status tdag_batch_get_tuning(const tdag_batch* b, uint32_t* out_grab, uint32_t* out_split_every)
+{
+    if (!b)
+        return fail(TDAG_EINVAL, "batch is null");
+    if (out_grab)
+        *out_grab = b->grab;
+    if (out_split_every)
+        *out_split_every = b->splitEvery;
     return TDAG_OK;
 }
 

@@ -147,8 +147,10 @@ def lib() -> ctypes.CDLL:
     L.tdag_batch_create_learned.argtypes = [ctypes.c_void_p, ctypes.POINTER(TdagLevelJob), ctypes.c_uint32, ctypes.POINTER(TdagConfig), ctypes.c_uint32,
                                             ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)]
     L.tdag_batch_learned_stats.argtypes = [ctypes.c_void_p, _u64p, _u64p]
+    L.tdag_batch_set_tuning.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]
+    L.tdag_batch_get_tuning.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]
     for name in ("tdag_batch_create", "tdag_batch_run", "tdag_batch_destroy", "tdag_batch_autotune", "tdag_match_levels_batched_device",
-                 "tdag_batch_create_learned", "tdag_batch_learned_stats"):
+                 "tdag_batch_create_learned", "tdag_batch_learned_stats", "tdag_batch_set_tuning", "tdag_batch_get_tuning"):
         getattr(L, name).restype = ctypes.c_int
     for name in ("tdag_context_create", "tdag_context_destroy", "tdag_match_level_device", "tdag_match_level_device_noalloc", "tdag_match_level_subset_device", "tdag_match_level", "tdag_last_stats",
                  "tdag_context_set_variant", "tdag_context_get_variant", "tdag_autotune_level", "tdag_context_set_counters",
@@ -235,10 +237,20 @@ class Batch:
         return list(stats), float(ms.value)
 
     def autotune(self, repeats: int = 1) -> int:
-        """Picks the queue granularity (tiles per pop); returns it."""
+        """Tunes the work queue (tiles per pop, then the blocks that start on the lighter levels'
+        queue); returns the tiles per pop."""
         g = ctypes.c_uint32()
         check(lib().tdag_batch_autotune(self._ptr, int(repeats), ctypes.byref(g)), "tdag_batch_autotune")
         return int(g.value)
+
+    def set_tuning(self, grab: int, split_every: int):
+        check(lib().tdag_batch_set_tuning(self._ptr, int(grab), int(split_every)), "tdag_batch_set_tuning")
+
+    def tuning(self):
+        """(tiles per pop, split_every) of the persistent kernel's queue."""
+        g, sp = ctypes.c_uint32(), ctypes.c_uint32()
+        check(lib().tdag_batch_get_tuning(self._ptr, ctypes.byref(g), ctypes.byref(sp)), "tdag_batch_get_tuning")
+        return int(g.value), int(sp.value)
 
     def close(self):
         if self._ptr:

@@ -232,10 +232,17 @@ tdag_status tdag_batch_create(tdag_context* ctx, const tdag_level_job* jobs, uin
  * (count entries, job order, may be NULL) receives per-level results; ms_total and ms_search
  * are the whole call's time. Fails with TDAG_EINTERNAL if a grid barrier timed out. */
 tdag_status tdag_batch_run(tdag_batch* b, tdag_match_stats* out_stats, float* out_ms);
-/* Tunes the batch's queue granularity (consecutive tiles per pop: 1, 2, 4, 8, 16) by running it
- * (1 warm-up + `repeats` timed runs each) and keeps the fastest; the captured graph stays valid
- * (it reads the descriptor at run time). A concurrent batch has no queue: *out_grab = 0. */
+/* Tunes the persistent kernel's work queue by running the batch (1 warm-up + `repeats` timed
+ * runs per candidate) and keeps the fastest: first the tiles a block takes per pop (1, 2, 4, 8,
+ * 16), then the share of blocks that take the lighter levels' tiles first, beside the heaviest
+ * level's (one block in 0 = none, 2, 3, 4 or 6). The captured graph stays valid (it reads the
+ * descriptor at run time). A concurrent batch has no queue: *out_grab = 0. */
 tdag_status tdag_batch_autotune(tdag_batch* b, uint32_t repeats, uint32_t* out_grab);
+/* Sets or reads the persistent kernel's queue tuning: tiles per pop (1..64) and split_every
+ * (every split_every-th block takes the lighter levels' tiles first; 0 = off, else 2..64).
+ * No reference counterpart. */
+tdag_status tdag_batch_set_tuning(tdag_batch* b, uint32_t grab, uint32_t split_every);
+tdag_status tdag_batch_get_tuning(const tdag_batch* b, uint32_t* out_grab, uint32_t* out_split_every);
 tdag_status tdag_batch_destroy(tdag_batch* b);
 /* The batched form with the learned proposal at level 2 (the combined form): the batch's level-2
  * job runs the learned matcher (network `mlp` of the same context, k <= 64 proposals per query,

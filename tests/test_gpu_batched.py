@@ -111,8 +111,18 @@ def test_batched_graph_replays(oracle, gpu_ctx, concurrent):
     try:
         grab = b.autotune(1)  # rewrites the descriptor the graph reads; replays must stay exact
         assert grab in ((0,) if concurrent else (1, 2, 4, 8, 16))
+        if not concurrent:
+            assert b.tuning()[0] == grab and b.tuning()[1] in (0, 2, 3, 4, 6)
+        for bad in ((0, 3), (1, 1), (65, 0)):
+            with pytest.raises(RuntimeError):
+                b.set_tuning(*bad)
+        # queue tunings the replays run with: tiles per pop, split_every (the persistent form)
+        settings = [None, (1, 0), (4, 2), (2, 6)]
         for rep in range(4):
-            print(f"replay {rep} concurrent={concurrent}", flush=True)
+            if settings[rep] is not None:
+                b.set_tuning(*settings[rep])
+                assert b.tuning() == settings[rep]
+            print(f"replay {rep} concurrent={concurrent} tuning={b.tuning()}", flush=True)
             for d in bufs.values():
                 d["enc"].fill_(-1)
                 d["map"].fill_(-1)
