@@ -150,23 +150,23 @@ TDAG_HD uint64_t translate4(uint64_t m, int tx, int ty, int tz)
 // rule as the reference's passes (find_translation.cpp:213-217): the state stays 0 while only
 // zeros were seen and is never 0 afterwards, so a hash is 0 exactly for an empty sub-box.
 // Per step: rotate, xor, odd multiply (a bijection of the rotated state for a fixed input),
-// then max(x, min(s | v, 1)): 0 before anything was seen, >= 1 afterwards (only the product 0
-// is merged into 1). The final value of the last pass goes through finalHash before it is used
-// as a table or filter key.
+// then max(x, min(s, 1)): 0 before anything was seen, >= 1 afterwards (only the product 0 is
+// merged into 1). The product is 0 exactly when v == rotl(s, 7), and then s and v are both 0 or
+// both non-zero, so min(s, 1) says whether anything was seen. The final value of the last pass
+// goes through finalHash before it is used as a table or filter key.
 // ---------------------------------------------------------------------------------------
 TDAG_HD uint32_t rotl32(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
 
 TDAG_HD uint32_t hashStep(uint32_t s, uint32_t v)
 {
     const uint32_t x = (rotl32(s, 7) ^ v) * 0x9E3779B1u;
-    const uint32_t t = s | v;
 #if defined(__HIP_DEVICE_COMPILE__)
-    // min(t, 1) as one v_min_u32: written as a compare and a select, it costs two instructions
+    // min(s, 1) as one v_min_u32: written as a compare and a select, it costs two instructions
     // and a VCC hazard in the unrolled sweeps.
     uint32_t seen;
-    asm("v_min_u32 %0, 1, %1" : "=v"(seen) : "v"(t));
+    asm("v_min_u32 %0, 1, %1" : "=v"(seen) : "v"(s));
 #else
-    const uint32_t seen = t < 1u ? t : 1u;
+    const uint32_t seen = s < 1u ? s : 1u;
 #endif
     return x > seen ? x : seen;
 }
