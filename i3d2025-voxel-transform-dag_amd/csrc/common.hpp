@@ -189,8 +189,17 @@ TDAG_HD uint32_t homeSlot32(uint32_t h, uint32_t logCap)
 // positions its low ten bits (disjoint for filters up to 2^22 words).
 TDAG_HD uint32_t bloomWord(uint32_t f, uint32_t logWords) { return f >> (32 - logWords); }
 TDAG_HD uint32_t bloomBits(uint32_t f) { return (1u << (f & 31u)) | (1u << ((f >> 5) & 31u)); }
-// 1 when both bits of f are set in the filter word bw.
-TDAG_HD uint32_t bloomHit(uint32_t bw, uint32_t f) { return (bw >> (f & 31u)) & (bw >> ((f >> 5) & 31u)) & 1u; }
+// 1 when both bits of f are set in the filter word bw. On the device: two bit-field extracts
+// (v_bfe_u32 reads its offset from the low five bits of the operand), the second of width 0 or 1
+// (the first bit), so a clear first bit yields 0 at once.
+TDAG_HD uint32_t bloomHit(uint32_t bw, uint32_t f)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_ubfe(bw, f, __builtin_amdgcn_ubfe(bw, f >> 5, 1u));
+#else
+    return (bw >> (f & 31u)) & (bw >> ((f >> 5) & 31u)) & 1u;
+#endif
+}
 // Word of a level-3..5 nested hash: its top bits. (The search loads no word for the hash 0 of an
 // empty region: its buffer offset is out of range.)
 TDAG_HD uint32_t bloomWordNested(uint32_t f, uint32_t logWords) { return f >> (32 - logWords); }

@@ -651,8 +651,19 @@ __device__ __forceinline__ uint32_t bloomWordOrZero(__amdgpu_buffer_rsrc_t rs, u
 // offset bits from f >> (shift - 2), bit 31 and the two low bits from ~m.
 __device__ __forceinline__ uint32_t bloomWordMasked(__amdgpu_buffer_rsrc_t rs, uint32_t f, uint32_t shift, uint32_t m)
 {
-    const uint32_t off = ((f >> (shift - 2u)) & 0x7FFFFFFCu) | (~m & 0x80000003u);
+    // v_bitop3_b32 with truth table 0xB1 over (a, b, c) = (f >> (shift - 2), m, mask): c ? a : ~b.
+    const uint32_t off = __builtin_amdgcn_bitop3_b32(f >> (shift - 2u), m, 0x7FFFFFFCu, 0xB1);
     return __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0);
+}
+
+// acc | (x << k) kept as a chain of v_lshl_or_b32: the empty asm stops the compiler from
+// regrouping an unrolled sweep's ORs into shifts merged three at a time (one instruction more per
+// two bits).
+__device__ __forceinline__ uint32_t lshlOr(uint32_t x, int k, uint32_t acc)
+{
+    uint32_t r = (x << k) | acc;
+    asm volatile("" : "+v"(r));
+    return r;
 }
 
 // Pass 3 along the lane's line (c1 = u) of its plane over the pass-2 states in[Pitch*c2], fused
@@ -726,7 +737,7 @@ __device__ __forceinline__ uint32_t pass3Filter(const uint32_t* in, uint32_t* ou
         if (k0 > 0) {
 #pragma unroll
             for (int k = 0; k < C; ++k)
-                pending |= bloomHit(pbw[k], pf[k]) << (Pos2 ? (R - 1 - (k0 - C) - k) : (k0 - C) + k);
+                pending = lshlOr(bloomHit(pbw[k], pf[k]), Pos2 ? (R - 1 - (k0 - C) - k) : (k0 - C) + k, pending);
         }
 #pragma unroll
         for (int k = 0; k < C; ++k) {
@@ -736,7 +747,7 @@ __device__ __forceinline__ uint32_t pass3Filter(const uint32_t* in, uint32_t* ou
     }
 #pragma unroll
     for (int k = 0; k < C; ++k)
-        pending |= bloomHit(pbw[k], pf[k]) << (Pos2 ? (C - 1 - k) : (R - C) + k);
+        pending = lshlOr(bloomHit(pbw[k], pf[k]), Pos2 ? (C - 1 - k) : (R - C) + k, pending);
     if (cfgBits & kCfgCount) {
         uint32_t acc = 0, nonEmpty = 0;
 #pragma unroll
