@@ -17,6 +17,7 @@
 //    best[c] with a 64-bit atomicMin. The self-match seeds best[c], so only ranks < c probe.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
 #include <vector>
 
 #include "common.hpp"
@@ -188,7 +189,7 @@ __global__ __launch_bounds__(256) void kLayouts(uint64_t* __restrict__ layouts, 
 // of its extent triple in all six axis orders in the block's LDS copy, which is then ORed into
 // the global bitmap (one atomic per non-zero word and block, instead of six per item on a few
 // hundred words). Empty and unflagged items set nothing.
-constexpr uint32_t kShapeBlocks = 512;
+constexpr uint32_t kShapeBlocks = 2048; // 8192 waves: one pass of resident waves, a few items each
 template <int L>
 __global__ __launch_bounds__(256) void kShapeBits(const uint64_t* __restrict__ layouts, uint32_t n, uint32_t* __restrict__ bits, const uint32_t* __restrict__ childFlags)
 {
@@ -221,83 +222,151 @@ __global__ __launch_bounds__(256) void kShapeBits(const uint64_t* __restrict__ l
             atomicOr(&bits[i], s_bits[i]);
 }
 
-// One wave per (item, frame): the hash of the whole grid in the pass frame of permutation code
-// q = blockIdx.y (1 frame: q = 0 only; 6 frames: every q) seen from each of its 8 corners goes
-// into the table and the Bloom filter. Pass a reads child axis sigma_a with sigma = q^-1, so the
-// value equals what the search computes, at the voxel where every scan ends, for a parent whose
-// content is the child read in that frame. Passes 1 and 2 depend on the corner's first two
-// direction bits only, so they run for those 4 directions (R lanes each, one line along sigma_1
-// per coordinate along sigma_2) and leave their end states in LDS; then lane `corner` < 8 runs
-// pass 3 over them and inserts the corner's entry.
+// Corner table of levels 3..5: for every item and permutation code q < F (1 frame: q = 0 only;
+// 6 frames: every q), the hash of the whole grid in the pass frame of q seen from each of its 8
+// corners goes into the table and the Bloom filter. Pass a reads child axis sigma_a with
+// sigma = q^-1, so the value equals what the search computes, at the voxel where every scan
+// ends, for a parent whose content is the child read in that frame. Passes 1 and 2 depend on the
+// corner's first two direction bits only, so they run once per (direction pos0 | pos1 << 1, line
+// c2 along sigma_2) and leave their end states in LDS; pass 3 then runs per corner over them.
+
+// Stages layout `a` of item r into the LDS rows (row j at j + j / R) with the 64 lanes of one
+// wave, every lane's loads issued before any is used; a chunk of rows (4 at level 5, 8 below)
+// stays within one line of R rows. Returns whether this lane saw a non-zero row.
 template <int L>
-__global__ __launch_bounds__(64) void kCornerTable(const uint64_t* __restrict__ layouts, uint32_t n, uint64_t* __restrict__ table, uint32_t logCap,
-    uint32_t* __restrict__ bloom, uint32_t logBloom, const uint32_t* __restrict__ childFlags)
+__device__ __forceinline__ int stageRows(const uint64_t* __restrict__ layouts, uint32_t n, uint32_t r, uint32_t a, typename Geo<L>::Row* s_rows, uint32_t lane)
 {
     using G = Geo<L>;
     using Row = typename G::Row;
+    using Chunk = std::conditional_t<L == 3, uint2, uint4>;
+    constexpr int ChunkRows = (int)(sizeof(Chunk) / sizeof(Row));
+    constexpr int Chunks = G::R2 / ChunkRows;
+    constexpr int PerLane = (Chunks + 63) / 64;
+    static_assert(G::R % ChunkRows == 0, "a chunk stays within one line of rows");
+    const Chunk* g = reinterpret_cast<const Chunk*>(layouts) + ((size_t)a * n + r) * Chunks;
+    Chunk c[PerLane];
+#pragma unroll
+    for (int i = 0; i < PerLane; ++i) {
+        const uint32_t k = lane + 64u * i;
+        c[i] = (Chunks % 64 == 0 || k < (uint32_t)Chunks) ? g[k] : Chunk {};
+    }
+    int any = 0;
+#pragma unroll
+    for (int i = 0; i < PerLane; ++i) {
+        const uint32_t k = lane + 64u * i;
+        if (Chunks % 64 != 0 && k >= (uint32_t)Chunks)
+            break;
+        const uint32_t j = k * ChunkRows;
+        Row rows[ChunkRows];
+        __builtin_memcpy(rows, &c[i], sizeof(Chunk));
+#pragma unroll
+        for (int e = 0; e < ChunkRows; ++e) {
+            s_rows[j + (j >> L) + e] = rows[e];
+            any |= (rows[e] != 0);
+        }
+    }
+    return any;
+}
+
+// Passes 1 and 2 of frame q for direction bits `dir` (pos0 | pos1 << 1) along line c2, over the
+// LDS rows of layout sigma_0 (`rows`): the rows of layout sigma_0 are indexed by the coordinates
+// along the other two axes in ascending axis order.
+template <int L>
+__device__ __forceinline__ uint32_t cornerPass12(const typename Geo<L>::Row* rows, uint32_t q, uint32_t dir, int c2)
+{
+    using G = Geo<L>;
     constexpr int R = G::R;
-    __shared__ Row s_rows[G::Rows];
-    __shared__ uint32_t s_p2[4][R]; // end states of passes 1-2 per (pos0, pos1) and line c2
-    const uint32_t r = blockIdx.x;
-    const uint32_t q = blockIdx.y;
-    if (childFlags && childFlags[r] == 0u) // only flagged ranks enter the table as children
-        return;
     // sigma = q^-1: pass a reads the child axis b with permAxis(q, b) == a.
     uint32_t sigma[3];
 #pragma unroll
     for (uint32_t b = 0; b < 3; ++b)
         sigma[permAxis(q, b)] = b;
-    const Row* g = reinterpret_cast<const Row*>(layouts) + ((size_t)sigma[0] * n + r) * G::R2;
-    int any = 0;
-    for (int j = (int)threadIdx.x; j < G::R2; j += 64) {
-        const Row v = g[j];
-        s_rows[j + (j >> L)] = v;
-        any |= (v != 0);
-    }
-    if (__syncthreads_or(any) == 0)
-        return;
-    const uint32_t capMask = (1u << logCap) - 1u;
-    const int lane = (int)threadIdx.x;
-    const int c2 = lane & (R - 1), sub = lane >> L;
-    // Rows of layout sigma_0 are indexed by the coordinates along the other two axes in ascending
-    // axis order.
     const bool lowFirst = sigma[1] < sigma[2];
-    for (int db = 0; db < 4; db += G::P) {
-        const int dir = db + sub;
-        if (dir >= 4)
-            continue;
-        const bool pos0 = dir & 1, pos1 = dir & 2;
-        const int c0 = pos0 ? 0 : R - 1;
-        const int sh = pos0 ? c0 : 31 - c0;
-        uint32_t s = 0;
-        for (int k = 0; k < R; ++k) {
-            const int c1 = pos1 ? (R - 1 - k) : k;
-            const uint32_t row = (uint32_t)s_rows[lowFirst ? c1 + G::Pitch * c2 : c2 + G::Pitch * c1];
-            s = hashStep(s, pos0 ? rowPrefix<true>(row, sh) : rowPrefix<false>(row, sh));
-        }
-        s_p2[dir][c2] = s;
+    const bool pos0 = dir & 1u, pos1 = dir & 2u;
+    const int c0 = pos0 ? 0 : R - 1;
+    const int sh = pos0 ? c0 : 31 - c0;
+    uint32_t s = 0;
+    for (int k = 0; k < R; ++k) {
+        const int c1 = pos1 ? (R - 1 - k) : k;
+        const uint32_t row = (uint32_t)rows[lowFirst ? c1 + G::Pitch * c2 : c2 + G::Pitch * c1];
+        s = hashStep(s, pos0 ? rowPrefix<true>(row, sh) : rowPrefix<false>(row, sh));
     }
-    __syncthreads();
-    if (lane >= 8)
-        return;
-    const uint32_t corner = (uint32_t)lane;
-    const int dir = (int)(corner & 3u);
+    return s;
+}
+
+// sigma_0 of frame q (the child axis its first pass reads).
+__device__ __forceinline__ uint32_t firstPassAxis(uint32_t q)
+{
+    return permAxis(q, 0) == 0u ? 0u : (permAxis(q, 1) == 0u ? 1u : 2u);
+}
+
+// Pass 3 of (frame q, corner) over the pass-2 end states `p2` (R per direction), then the Bloom
+// bits and the table entry. The t-th entry of a home slot starts probing at home + t: slots
+// home .. home + t - 1 are the starts of the entries before it, so each is taken by the time
+// the walks end and the probe run from home stays unbroken. The entries of one hash (up to 48:
+// the frames and corners of children that are transforms of one another) thus take about one
+// attempt each instead of walking past all the earlier ones, and they sit in the run in the
+// order of their tickets.
+template <int L>
+__device__ __forceinline__ void cornerInsert(const uint32_t (*p2)[Geo<L>::R], uint32_t r, uint32_t q, uint32_t corner, uint64_t* __restrict__ table,
+    uint32_t logCap, uint32_t* __restrict__ bloom, uint32_t logBloom, uint32_t* __restrict__ tickets)
+{
+    constexpr int R = Geo<L>::R;
+    const uint32_t dir = corner & 3u;
     const bool pos2 = corner & 4u;
     uint32_t s3 = 0;
     for (int k = 0; k < R; ++k)
-        s3 = hashStep(s3, s_p2[dir][pos2 ? (R - 1 - k) : k]);
+        s3 = hashStep(s3, p2[dir][pos2 ? (R - 1 - k) : k]);
     const uint32_t h = finalHash(s3);
     if (h == 0)
         return;
     atomicOr(&bloom[bloomWordNested(h, logBloom)], bloomBits(h));
     const uint64_t packed = packEntry(h, r, q, corner);
-    uint32_t slot = homeSlot32(h, logCap);
+    const uint32_t capMask = (1u << logCap) - 1u;
+    const uint32_t home = homeSlot32(h, logCap);
+    uint32_t slot = (home + atomicAdd(&tickets[home], 1u)) & capMask;
     for (;;) {
         const unsigned long long prev = atomicCAS((unsigned long long*)&table[slot], 0ull, (unsigned long long)packed);
         if (prev == 0ull)
             break;
         slot = (slot + 1) & capMask;
     }
+}
+
+// One wave per (item, frame q = blockIdx.y): passes 1-2 run as 4 directions x R lines over the
+// 64 lanes, then lane `corner` < 8 runs pass 3 and inserts the corner's entry.
+// Frame-major dispatch hands the entries of one hash their tickets mostly in ascending frame,
+// the order of their keys for parents read in frame 0, so a search verifies the best one first
+// and skips the rest.
+template <int L>
+__global__ __launch_bounds__(64) void kCornerTable(const uint64_t* __restrict__ layouts, uint32_t n, uint64_t* __restrict__ table, uint32_t logCap,
+    uint32_t* __restrict__ bloom, uint32_t logBloom, const uint32_t* __restrict__ childFlags, uint32_t* __restrict__ tickets)
+{
+    using G = Geo<L>;
+    using Row = typename G::Row;
+    constexpr int R = G::R;
+    __shared__ Row s_rows[G::Rows];
+    __shared__ uint32_t s_p2[4][R]; // end states of passes 1-2 per (pos0 | pos1 << 1) and line c2
+    const uint32_t r = blockIdx.x;
+    const uint32_t q = blockIdx.y;
+    if (childFlags && childFlags[r] == 0u) // only flagged ranks enter the table as children
+        return;
+    const uint32_t lane = threadIdx.x;
+    if (__syncthreads_or(stageRows<L>(layouts, n, r, firstPassAxis(q), s_rows, lane)) == 0)
+        return;
+    for (uint32_t t = lane; t < 4u * R; t += 64)
+        s_p2[t >> L][t & (R - 1)] = cornerPass12<L>(s_rows, q, t >> L, (int)(t & (R - 1)));
+    __syncthreads();
+    if (lane < 8)
+        cornerInsert<L>(s_p2, r, q, lane, table, logCap, bloom, logBloom, tickets);
+}
+
+template <int L>
+void launchCornerTable(const MatchPlan& p, const LevelView& v, const uint32_t* childFlags, hipStream_t stream)
+{
+    const uint32_t n = (uint32_t)p.n;
+    hipLaunchKernelGGL(kCornerTable<L>, dim3(n, p.childFrames), dim3(64), 0, stream, v.sortedMasks, n, v.tabKeys, p.logCap, v.bloom, p.logBloom, childFlags,
+        v.tickets);
 }
 
 // Search kernel for levels 3..5. One block per (parent rank, unit, plane chunk) with `Waves`
@@ -591,6 +660,7 @@ MatchPlan planMatch(uint32_t level, uint64_t n, uint32_t variant, uint32_t first
     p.offBest = take(n * 8);
     p.offTabKeys = take(cap * 8);
     p.offTabVals = take(level == 2 ? cap * 4 : 0);
+    p.offTickets = take(level >= 3 ? cap * 4 : 0); // entries per home slot while the table is built
     // Filter words = entries * bitsPerEntry / 32 (rounded up to a power of two).
     p.bloomBitsPerEntry = bloomBitsPerEntry ? bloomBitsPerEntry : defaultFilterBits(level);
     p.logBloom = std::min<uint32_t>(22, std::max<uint32_t>(6, log2Ceil((entries * p.bloomBitsPerEntry + 31) / 32)));
@@ -615,6 +685,7 @@ LevelView levelView(const MatchPlan& p, void* d_scratch)
     v.best = (uint64_t*)(base + p.offBest);
     v.tabKeys = (uint64_t*)(base + p.offTabKeys);
     v.tabVals = (uint32_t*)(base + p.offTabVals);
+    v.tickets = p.level >= 3 ? (uint32_t*)(base + p.offTickets) : nullptr;
     v.bloom = (uint32_t*)(base + p.offBloom);
     v.shape = p.level >= 3 ? (uint32_t*)(base + p.offShape) : nullptr;
     v.counters = (unsigned long long*)(base + p.offCounters);
@@ -696,22 +767,22 @@ hipError_t enqueueBuildTable(const MatchPlan& p, void* d_scratch, hipStream_t st
         break;
     case 3:
         hipLaunchKernelGGL(kLayouts<3>, dim3((unsigned)ceilDiv((size_t)n * 2 * 8, 256)), dim3(256), 0, stream, v.sortedMasks, n);
-        hipLaunchKernelGGL(kCornerTable<3>, dim3(n, p.childFrames), dim3(64), 0, stream, v.sortedMasks, n, v.tabKeys, p.logCap, v.bloom, p.logBloom,
-            childFlags);
+        TDAG_TRY(fillU32(v.tickets, cap, 0u, stream));
+        launchCornerTable<3>(p, v, childFlags, stream);
         TDAG_TRY(fillU32(v.shape, shapeWords<3>(), 0u, stream));
         hipLaunchKernelGGL(kShapeBits<3>, dim3(std::min<uint32_t>(kShapeBlocks, (n + 3) / 4)), dim3(256), 0, stream, v.sortedMasks, n, v.shape, childFlags);
         break;
     case 4:
         hipLaunchKernelGGL(kLayouts<4>, dim3((unsigned)ceilDiv((size_t)n * 2 * 16, 256)), dim3(256), 0, stream, v.sortedMasks, n);
-        hipLaunchKernelGGL(kCornerTable<4>, dim3(n, p.childFrames), dim3(64), 0, stream, v.sortedMasks, n, v.tabKeys, p.logCap, v.bloom, p.logBloom,
-            childFlags);
+        TDAG_TRY(fillU32(v.tickets, cap, 0u, stream));
+        launchCornerTable<4>(p, v, childFlags, stream);
         TDAG_TRY(fillU32(v.shape, shapeWords<4>(), 0u, stream));
         hipLaunchKernelGGL(kShapeBits<4>, dim3(std::min<uint32_t>(kShapeBlocks, (n + 3) / 4)), dim3(256), 0, stream, v.sortedMasks, n, v.shape, childFlags);
         break;
     case 5:
         hipLaunchKernelGGL(kLayouts<5>, dim3((unsigned)ceilDiv((size_t)n * 2 * 32, 256)), dim3(256), 0, stream, v.sortedMasks, n);
-        hipLaunchKernelGGL(kCornerTable<5>, dim3(n, p.childFrames), dim3(64), 0, stream, v.sortedMasks, n, v.tabKeys, p.logCap, v.bloom, p.logBloom,
-            childFlags);
+        TDAG_TRY(fillU32(v.tickets, cap, 0u, stream));
+        launchCornerTable<5>(p, v, childFlags, stream);
         TDAG_TRY(fillU32(v.shape, shapeWords<5>(), 0u, stream));
         hipLaunchKernelGGL(kShapeBits<5>, dim3(std::min<uint32_t>(kShapeBlocks, (n + 3) / 4)), dim3(256), 0, stream, v.sortedMasks, n, v.shape, childFlags);
         break;

@@ -44,7 +44,7 @@ struct MatchPlan {
     uint32_t childFrames; // levels 3..5: child-table permutation frames, 1 or 6 (kCfgChildFrames)
     uint64_t n;
     size_t offKeys, offOrder, offSortScratch, offSortedMasks, offBest;
-    size_t offTabKeys, offTabVals, offBloom, offShape, offParentTp, offIsUnique, offNewIdx, offScanScratch, offCounters;
+    size_t offTabKeys, offTabVals, offTickets, offBloom, offShape, offParentTp, offIsUnique, offNewIdx, offScanScratch, offCounters;
     size_t scratchBytes;
 };
 
@@ -80,8 +80,9 @@ struct LevelView {
     uint32_t level, n;
     uint64_t* sortedMasks; // rank-ordered masks; levels >= 3 followed by the two transposed layouts
     uint64_t* best;
-    uint64_t* tabKeys; // level 2: keys; levels >= 3: packed (hash << 32 | rank << 3 | corner)
+    uint64_t* tabKeys; // level 2: keys; levels >= 3: packed (hash << 32 | rank << 6 | perm << 3 | corner)
     uint32_t* tabVals; // level 2 only
+    uint32_t* tickets; // levels >= 3: entries started per home slot while the table is built
     uint32_t* bloom;
     uint32_t* shape; // levels >= 3: the children's extent triples (spans e = extent - 1), bit shapeBit(e0) of word shapeWord(e1, e2)
     unsigned long long* counters;
