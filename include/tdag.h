@@ -112,9 +112,10 @@ tdag_status tdag_match_level(tdag_context* ctx, uint32_t level, const uint64_t* 
  * tdag_match_level_device with the context's tuning; the call allocates nothing and does not
  * synchronise. Its scratch is one fixed array of TDAG_NOALLOC_SCRATCH_BYTES per GPU, held from
  * the first context created on that GPU until the process ends; a level / item count whose
- * scratch needs more returns TDAG_EINVAL (at default tuning about 100 K items fit at levels 2
- * and 3, 17 K at level 4 and 4.6 K at level 5). Every call on one GPU shares that array, so calls that overlap on
- * one GPU (any contexts, any streams) race; consecutive calls are independent.
+ * scratch needs more returns TDAG_EINVAL (at default tuning about 131 K items fit at level 2,
+ * 65 K at level 3, 10.9 K at level 4 and 4.4 K at level 5). Every call on one GPU shares that
+ * array, so calls that overlap on one GPU (any contexts, any streams) race; consecutive calls
+ * are independent.
  *   d_counts  optional device output, 2 uint64: verified matches, number of unique items
  *   stream    hipStream_t (NULL = the context's stream)
  */
