@@ -179,10 +179,11 @@ TDAG_HD uint32_t homeSlot64(uint64_t key, uint32_t logCap)
 {
     return (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> (64 - logCap));
 }
-TDAG_HD uint32_t homeSlot32(uint32_t h, uint32_t logCap)
-{
-    return (uint32_t)(((uint64_t)h * 0x9E3779B97F4A7C15ull) >> (64 - logCap));
-}
+// Home slot of a 32-bit nested hash: its top bits, as the filter word (bloomWordNested) is, so
+// the table's slots run in the order of the filter words (the prune rebuilds the filter from a
+// sweep of the table with few atomics). finalHash leaves the well-mixed top half of the last
+// product in the top bits.
+TDAG_HD uint32_t homeSlot32(uint32_t h, uint32_t logCap) { return h >> (32 - logCap); }
 
 // Blocked Bloom filter over the child table keys: one 32-bit word per key, two bits set in it.
 // `f` is a well-mixed 32-bit value of the key; the word index uses its top bits and the two bit

@@ -1120,23 +1120,37 @@ __device__ __forceinline__ void searchUnit(uint32_t r, uint32_t unit, uint32_t c
 // below the smallest key a parent >= r1 can produce is final. Its table entries become
 // tombstones and the Bloom filter is rebuilt from the live entries, so later batches neither
 // probe nor load best[] for it. (The reference prunes its candidate ranges between search
-// batches for the same reason, find_translation.cpp:887-915.) Slots [start, cap) step `stride`.
+// batches for the same reason, find_translation.cpp:887-915.) Slots [start, cap) step `stride`;
+// start and stride are multiples of 64 per wave (lane l takes slot base + l), so a wave's lanes
+// hold 64 consecutive slots. Home slots and filter words are both top bits of the hash, so those
+// slots hold few distinct filter words: the wave ORs the bits of equal words together (a scan
+// over the lanes whose combining steps join equal words only) and the last lane of each run of
+// equal words sets them with one atomic.
 __device__ __forceinline__ void prunePacked(uint64_t* __restrict__ table, uint64_t cap, const uint64_t* __restrict__ best, uint64_t minKey, uint32_t* __restrict__ bloom,
     uint32_t logBloom, uint64_t start, uint64_t stride)
 {
+    const int lane = (int)(threadX() & 63u);
     for (uint64_t s = start; s < cap; s += stride) {
         const uint64_t e = table[s];
-        if (e == 0)
-            continue;
         const uint32_t c = (uint32_t)e >> 6;
-        if (c == kPackedDead)
-            continue;
-        if (best[c] < minKey) {
+        const bool entry = e != 0ull && c != kPackedDead;
+        const bool dead = entry && best[c] < minKey;
+        if (dead)
             table[s] = (e & 0xFFFFFFFF0000003Full) | ((uint64_t)kPackedDead << 6);
-            continue;
-        }
         const uint32_t h = (uint32_t)(e >> 32);
-        atomicOr(&bloom[bloomWordNested(h, logBloom)], bloomBits(h));
+        const bool live = entry && !dead;
+        const uint32_t word = live ? bloomWordNested(h, logBloom) : 0xFFFFFFFFu;
+        uint32_t bits = live ? bloomBits(h) : 0u;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t w = (uint32_t)__shfl_up((int)word, off, 64);
+            const uint32_t b = (uint32_t)__shfl_up((int)bits, off, 64);
+            if (lane >= off && w == word)
+                bits |= b;
+        }
+        const uint32_t next = (uint32_t)__shfl_down((int)word, 1, 64);
+        if (live && (lane == 63 || next != word))
+            atomicOr(&bloom[word], bits);
     }
 }
 
