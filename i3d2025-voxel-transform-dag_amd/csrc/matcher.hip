@@ -157,31 +157,57 @@ __device__ __forceinline__ void transposeBits(uint32_t (&a)[R])
 }
 
 // Transposed bit layouts 1 and 2 of every item (layout 0 is the sorted masks themselves). One
-// thread per (item, layout, slab): layout 1's rows at z = slab are the transpose of layout 0's
-// rows (y, z = slab) for y < R; layout 2's rows at y = slab are the transpose of layout 0's rows
-// (y = slab, z) for z < R.
+// thread per (layout, item, slab), layout slowest so a wave makes one layout: layout 1's rows at
+// z = slab are the transpose of layout 0's rows (y, z = slab) for y < R (R consecutive rows,
+// read as 16-byte or 8-byte vectors); layout 2's rows at y = slab are the transpose of layout
+// 0's rows (y = slab, z) for z < R (one row per z; the lanes of consecutive slabs read
+// consecutive rows). Both write R consecutive rows as vectors.
 template <int L>
 __global__ __launch_bounds__(256) void kLayouts(uint64_t* __restrict__ layouts, uint32_t n)
 {
     using G = Geo<L>;
     using Row = typename G::Row;
     constexpr int R = G::R;
+    using Vec = std::conditional_t<L == 3, uint2, uint4>;
+    constexpr int PerVec = (int)(sizeof(Vec) / sizeof(Row));
+    static_assert(R % PerVec == 0, "a slab's rows are whole vectors");
     const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (t >= (size_t)n * 2 * R)
         return;
-    const uint32_t r = (uint32_t)(t / (2 * R));
-    const int lay = 1 + (int)((t / R) & 1u);
-    const int slab = (int)(t % R);
+    const int lay = t < (size_t)n * R ? 1 : 2;
+    const size_t tl = lay == 1 ? t : t - (size_t)n * R;
+    const uint32_t r = (uint32_t)(tl / R);
+    const int slab = (int)(tl % R);
     const Row* src = reinterpret_cast<const Row*>(layouts) + (size_t)r * G::R2;
     uint32_t a[R];
+    if (lay == 1) {
+        const Vec* v = reinterpret_cast<const Vec*>(src + R * slab);
 #pragma unroll
-    for (int i = 0; i < R; ++i)
-        a[i] = (uint32_t)src[lay == 1 ? i + R * slab : slab + R * i];
+        for (int i = 0; i < R / PerVec; ++i) {
+            const Vec x = v[i];
+            Row rows[PerVec];
+            __builtin_memcpy(rows, &x, sizeof(Vec));
+#pragma unroll
+            for (int e = 0; e < PerVec; ++e)
+                a[i * PerVec + e] = (uint32_t)rows[e];
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+            a[i] = (uint32_t)src[slab + R * i];
+    }
     transposeBits<R>(a);
-    Row* dst = reinterpret_cast<Row*>(layouts) + ((size_t)lay * n + r) * G::R2 + (size_t)R * slab;
+    Vec* dst = reinterpret_cast<Vec*>(reinterpret_cast<Row*>(layouts) + ((size_t)lay * n + r) * G::R2 + (size_t)R * slab);
 #pragma unroll
-    for (int i = 0; i < R; ++i)
-        dst[i] = (Row)a[i];
+    for (int i = 0; i < R / PerVec; ++i) {
+        Row rows[PerVec];
+#pragma unroll
+        for (int e = 0; e < PerVec; ++e)
+            rows[e] = (Row)a[i * PerVec + e];
+        Vec x;
+        __builtin_memcpy(&x, rows, sizeof(Vec));
+        dst[i] = x;
+    }
 }
 
 // Shape bitmap of the children (the search's pre-filter, ShapeLds): each wave takes items in
@@ -300,16 +326,17 @@ __device__ __forceinline__ uint32_t firstPassAxis(uint32_t q)
     return permAxis(q, 0) == 0u ? 0u : (permAxis(q, 1) == 0u ? 1u : 2u);
 }
 
-// Pass 3 of (frame q, corner) over the pass-2 end states `p2` (R per direction), then the Bloom
-// bits and the table entry. The t-th entry of a home slot starts probing at home + t: slots
-// home .. home + t - 1 are the starts of the entries before it, so each is taken by the time
-// the walks end and the probe run from home stays unbroken. The entries of one hash (up to 48:
+// Pass 3 of (frame q, corner) over the pass-2 end states `p2` (R per direction), then the table
+// entry (the Bloom filter is set afterwards from a sweep of the table, see launchCornerTable).
+// The t-th entry of a home slot starts probing at home + t: slots home .. home + t - 1 are the
+// starts of the entries before it, so each is taken by the time the walks end and the probe run
+// from home stays unbroken. The entries of one hash (up to 48:
 // the frames and corners of children that are transforms of one another) thus take about one
 // attempt each instead of walking past all the earlier ones, and they sit in the run in the
 // order of their tickets.
 template <int L>
 __device__ __forceinline__ void cornerInsert(const uint32_t (*p2)[Geo<L>::R], uint32_t r, uint32_t q, uint32_t corner, uint64_t* __restrict__ table,
-    uint32_t logCap, uint32_t* __restrict__ bloom, uint32_t logBloom, uint32_t* __restrict__ tickets)
+    uint32_t logCap, uint32_t* __restrict__ tickets)
 {
     constexpr int R = Geo<L>::R;
     const uint32_t dir = corner & 3u;
@@ -320,7 +347,6 @@ __device__ __forceinline__ void cornerInsert(const uint32_t (*p2)[Geo<L>::R], ui
     const uint32_t h = finalHash(s3);
     if (h == 0)
         return;
-    atomicOr(&bloom[bloomWordNested(h, logBloom)], bloomBits(h));
     const uint64_t packed = packEntry(h, r, q, corner);
     const uint32_t capMask = (1u << logCap) - 1u;
     const uint32_t home = homeSlot32(h, logCap);
@@ -340,7 +366,7 @@ __device__ __forceinline__ void cornerInsert(const uint32_t (*p2)[Geo<L>::R], ui
 // and skips the rest.
 template <int L>
 __global__ __launch_bounds__(64) void kCornerTable(const uint64_t* __restrict__ layouts, uint32_t n, uint64_t* __restrict__ table, uint32_t logCap,
-    uint32_t* __restrict__ bloom, uint32_t logBloom, const uint32_t* __restrict__ childFlags, uint32_t* __restrict__ tickets)
+    const uint32_t* __restrict__ childFlags, uint32_t* __restrict__ tickets)
 {
     using G = Geo<L>;
     using Row = typename G::Row;
@@ -358,15 +384,7 @@ __global__ __launch_bounds__(64) void kCornerTable(const uint64_t* __restrict__ 
         s_p2[t >> L][t & (R - 1)] = cornerPass12<L>(s_rows, q, t >> L, (int)(t & (R - 1)));
     __syncthreads();
     if (lane < 8)
-        cornerInsert<L>(s_p2, r, q, lane, table, logCap, bloom, logBloom, tickets);
-}
-
-template <int L>
-void launchCornerTable(const MatchPlan& p, const LevelView& v, const uint32_t* childFlags, hipStream_t stream)
-{
-    const uint32_t n = (uint32_t)p.n;
-    hipLaunchKernelGGL(kCornerTable<L>, dim3(n, p.childFrames), dim3(64), 0, stream, v.sortedMasks, n, v.tabKeys, p.logCap, v.bloom, p.logBloom, childFlags,
-        v.tickets);
+        cornerInsert<L>(s_p2, r, q, lane, table, logCap, tickets);
 }
 
 // Search kernel for levels 3..5. One block per (parent rank, unit, plane chunk) with `Waves`
@@ -457,6 +475,24 @@ __global__ __launch_bounds__(256) void kPruneSplit(const uint64_t* __restrict__ 
 {
     pruneSplit(tabKeys, tabVals, cap, best, minKey, bloom, logBloom, (uint64_t)blockIdx.x * 256 + threadIdx.x, (uint64_t)gridDim.x * 256);
 }
+
+unsigned pruneBlocks(uint64_t cap)
+{
+    return (unsigned)std::min<uint64_t>(ceilDiv(cap, 256), 8192);
+}
+
+// The table's entries, then its Bloom filter: a prune pass with key 0 keeps every entry and sets
+// the filter bits of all of them, a few atomics per wave instead of one per entry inserted (the
+// table's slots run in filter-word order; the filter was cleared by the caller).
+template <int L>
+void launchCornerTable(const MatchPlan& p, const LevelView& v, const uint32_t* childFlags, hipStream_t stream)
+{
+    const uint32_t n = (uint32_t)p.n;
+    hipLaunchKernelGGL(kCornerTable<L>, dim3(n, p.childFrames), dim3(64), 0, stream, v.sortedMasks, n, v.tabKeys, p.logCap, childFlags, v.tickets);
+    const uint64_t cap = 1ull << p.logCap;
+    hipLaunchKernelGGL(kPrunePacked, dim3(pruneBlocks(cap)), dim3(256), 0, stream, v.tabKeys, cap, v.best, 0ull, v.bloom, p.logBloom);
+}
+
 
 // ---------------------------------------------------------------------------------------
 // Post-processing (find_translation.cpp:923-997).
@@ -831,12 +867,12 @@ hipError_t enqueueSearch(const MatchPlan& p, uint32_t cfgBitsIn, void* d_scratch
         r0 = r1;
         if (r0 < n) {
             const uint64_t minKey = pruneKey(p.level, r0);
-            const unsigned pruneBlocks = (unsigned)std::min<uint64_t>(ceilDiv(cap, 256), 8192);
+            const unsigned blocks = pruneBlocks(cap);
             TDAG_TRY(fillU32(v.bloom, 1ull << p.logBloom, 0u, stream));
             if (p.level == 2)
-                hipLaunchKernelGGL(kPruneSplit, dim3(pruneBlocks), dim3(256), 0, stream, v.tabKeys, v.tabVals, cap, v.best, minKey, v.bloom, p.logBloom);
+                hipLaunchKernelGGL(kPruneSplit, dim3(blocks), dim3(256), 0, stream, v.tabKeys, v.tabVals, cap, v.best, minKey, v.bloom, p.logBloom);
             else
-                hipLaunchKernelGGL(kPrunePacked, dim3(pruneBlocks), dim3(256), 0, stream, v.tabKeys, cap, v.best, minKey, v.bloom, p.logBloom);
+                hipLaunchKernelGGL(kPrunePacked, dim3(blocks), dim3(256), 0, stream, v.tabKeys, cap, v.best, minKey, v.bloom, p.logBloom);
             TDAG_TRY(hipGetLastError());
         }
     }
