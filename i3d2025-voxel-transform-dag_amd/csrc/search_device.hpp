@@ -815,6 +815,9 @@ __device__ __forceinline__ bool verifyWave(const uint64_t* __restrict__ layouts,
 // combo (pc, d), and hash[i] = its final nested hash (the planes are reused by the next combo).
 // `count` is uniform over the wave.
 constexpr uint32_t kProbeQueue = 64;
+// Table slots a walk reads per round trip (consecutive, issued together).
+constexpr int kWalkSlots = 4;
+
 struct ProbeList {
     uint32_t* meta;
     uint32_t* hash;
@@ -854,54 +857,65 @@ __device__ __forceinline__ void probeQueue(const ProbeList& list, int c0Base, co
             uint32_t hitC = 0;
             uint64_t hitKey = 0;
             while (active) {
-                const uint64_t e = table[slot];
-                if (e == 0) {
-                    active = false;
+                // kWalkSlots consecutive slots per round trip: the probe run of one hash is
+                // contiguous, and at level 3 it often holds several entries.
+                uint64_t ev[kWalkSlots];
+#pragma unroll
+                for (int k = 0; k < kWalkSlots; ++k)
+                    ev[k] = table[(slot + (uint32_t)k) & capMask];
+#pragma unroll
+                for (int k = 0; k < kWalkSlots; ++k) {
+                    const uint64_t e = ev[k];
+                    if (e == 0) {
+                        active = false;
+                        break;
+                    }
+                    slot = (slot + 1) & capMask;
+                    if ((uint32_t)(e >> 32) != h)
+                        continue;
+                    ++cnt.keyHits;
+                    const uint32_t cc = (uint32_t)e >> 6;
+                    const uint32_t q = ((uint32_t)e >> 3) & 7u, kc = (uint32_t)e & 7u;
+                    if (cc <= r || cc == kPackedDead)
+                        continue;
+                    // Per pass index a: mirror bit and translation of the anchored match
+                    // (find_translation.cpp:835-837); the voxel's coordinates along the three pass
+                    // axes are (c0, u, c2).
+                    const int voxel[3] = { c0, u, c2 };
+                    const uint32_t sa = d ^ kc;
+                    int ta[3];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        const int anchor = ((d >> a) & 1u) ? 0 : R - 1;
+                        ta[a] = ((sa >> a) & 1u) ? voxel[a] - anchor : anchor - voxel[a];
+                    }
+                    // Pass a reads parent axis permAxis(pc, a) and child axis sigma_a; child axis b
+                    // is read by pass a = permAxis(q, b). Output axis b of the transform thus takes
+                    // parent axis permAxis(pc, a), mirror bit sa_a and translation ta_a.
+                    uint32_t pb[3], sym = 0;
+                    int tt[3];
+#pragma unroll
+                    for (int bo = 0; bo < 3; ++bo) {
+                        const uint32_t a = permAxis(q, bo);
+                        pb[bo] = permAxis(pc, a);
+                        sym |= ((sa >> a) & 1u) << bo;
+                        tt[bo] = a == 0 ? ta[0] : (a == 1 ? ta[1] : ta[2]);
+                    }
+                    const uint32_t tc = permCode(pb[0], pb[1], pb[2]);
+                    if (!configAllows(cfgBits, tc, sym, tt[0], tt[1], tt[2]))
+                        continue;
+                    ++cnt.candidates;
+                    const uint64_t k64 = encodeKey(r, tc, sym, tt[0], tt[1], tt[2], L);
+                    if (k64 >= bestHint(&best[cc]))
+                        continue;
+                    ++cnt.verifications;
+                    found = true;
+                    hitC = cc;
+                    hitKey = k64;
                     break;
                 }
-                slot = (slot + 1) & capMask;
-                if ((uint32_t)(e >> 32) != h)
-                    continue;
-                ++cnt.keyHits;
-                const uint32_t cc = (uint32_t)e >> 6;
-                const uint32_t q = ((uint32_t)e >> 3) & 7u, kc = (uint32_t)e & 7u;
-                if (cc <= r || cc == kPackedDead)
-                    continue;
-                // Per pass index a: mirror bit and translation of the anchored match
-                // (find_translation.cpp:835-837); the voxel's coordinates along the three pass
-                // axes are (c0, u, c2).
-                const int voxel[3] = { c0, u, c2 };
-                const uint32_t sa = d ^ kc;
-                int ta[3];
-#pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    const int anchor = ((d >> a) & 1u) ? 0 : R - 1;
-                    ta[a] = ((sa >> a) & 1u) ? voxel[a] - anchor : anchor - voxel[a];
-                }
-                // Pass a reads parent axis permAxis(pc, a) and child axis sigma_a; child axis b
-                // is read by pass a = permAxis(q, b). Output axis b of the transform thus takes
-                // parent axis permAxis(pc, a), mirror bit sa_a and translation ta_a.
-                uint32_t pb[3], sym = 0;
-                int tt[3];
-#pragma unroll
-                for (int bo = 0; bo < 3; ++bo) {
-                    const uint32_t a = permAxis(q, bo);
-                    pb[bo] = permAxis(pc, a);
-                    sym |= ((sa >> a) & 1u) << bo;
-                    tt[bo] = a == 0 ? ta[0] : (a == 1 ? ta[1] : ta[2]);
-                }
-                const uint32_t tc = permCode(pb[0], pb[1], pb[2]);
-                if (!configAllows(cfgBits, tc, sym, tt[0], tt[1], tt[2]))
-                    continue;
-                ++cnt.candidates;
-                const uint64_t k64 = encodeKey(r, tc, sym, tt[0], tt[1], tt[2], L);
-                if (k64 >= bestHint(&best[cc]))
-                    continue;
-                ++cnt.verifications;
-                found = true;
-                hitC = cc;
-                hitKey = k64;
-                break;
+                if (found)
+                    break;
             }
             uint64_t stopped = __ballot(found);
             if (stopped == 0)
