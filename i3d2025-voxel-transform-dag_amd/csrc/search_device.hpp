@@ -125,6 +125,12 @@ __device__ __forceinline__ void waveSyncLds()
 // Capacity of the level-2 work list of one wave (entries of 8 + 2 bytes in LDS).
 constexpr uint32_t kWalkQueue2 = 128;
 
+// Table slots a walk reads per round trip (consecutive, issued together): the probe run of one
+// key is contiguous and often holds several entries.
+constexpr int kWalkSlots = 4;
+// The same at level 2, whose kernel runs 8 waves per SIMD within 64 VGPRs.
+constexpr int kWalkSlots2 = 2;
+
 // Table walks of `count` queued level-2 keys of parent r under permutation pc (its permuted mask
 // Q): lane j walks entry base + j, so the walks of all directions run side by side. Entry meta:
 // source lane (anchored voxel) | direction << 6.
@@ -141,36 +147,49 @@ __device__ __forceinline__ void walkQueue2(const uint64_t* qKey, const uint16_t*
         const uint32_t meta = qMeta[base + lane];
         const uint32_t src = meta & 63u, d = meta >> 6;
         const int vx = (int)(src & 3u), vy = (int)((src >> 2) & 3u), vz = (int)(src >> 4);
+        // kWalkSlots2 consecutive slots (keys and values) per round trip.
         uint32_t s = homeSlot64(key, logCap);
-        uint64_t k = tabKeys[s];
-        while (k != 0) {
-            if (k == key) {
+        bool walking = true;
+        while (walking) {
+            uint64_t kv[kWalkSlots2];
+            uint32_t vv[kWalkSlots2];
+#pragma unroll
+            for (int j = 0; j < kWalkSlots2; ++j) {
+                kv[j] = tabKeys[(s + (uint32_t)j) & capMask];
+                vv[j] = tabVals[(s + (uint32_t)j) & capMask];
+            }
+#pragma unroll
+            for (int j = 0; j < kWalkSlots2; ++j) {
+                if (kv[j] == 0) {
+                    walking = false;
+                    break;
+                }
+                if (kv[j] != key)
+                    continue;
                 ++cnt.keyHits;
-                const uint32_t e = tabVals[s];
-                const uint32_t c = e >> 3;
-                const uint32_t kc = e & 7u;
-                if (c > r && c != kDeadRank) {
-                    const uint32_t sym = d ^ kc;
-                    const int tx = (kc & 1u) ? -vx : vx;
-                    const int ty = (kc & 2u) ? -vy : vy;
-                    const int tz = (kc & 4u) ? -vz : vz;
-                    if (configAllows(cfgBits, pc, sym, tx, ty, tz)) {
-                        ++cnt.candidates;
-                        const uint64_t k64 = encodeKey(r, pc, sym, tx, ty, tz, 2);
-                        if (k64 < bestHint(&best[c])) {
-                            ++cnt.verifications;
-                            // Bitwise verification of the derived transform.
-                            const uint64_t T = translate4(mirror4(Q, sym), tx, ty, tz);
-                            if (T == sortedMasks[c]) {
-                                ++cnt.verified;
-                                atomicMin((unsigned long long*)&best[c], (unsigned long long)k64);
-                            }
-                        }
-                    }
+                const uint32_t c = vv[j] >> 3;
+                const uint32_t kc = vv[j] & 7u;
+                if (c <= r || c == kDeadRank)
+                    continue;
+                const uint32_t sym = d ^ kc;
+                const int tx = (kc & 1u) ? -vx : vx;
+                const int ty = (kc & 2u) ? -vy : vy;
+                const int tz = (kc & 4u) ? -vz : vz;
+                if (!configAllows(cfgBits, pc, sym, tx, ty, tz))
+                    continue;
+                ++cnt.candidates;
+                const uint64_t k64 = encodeKey(r, pc, sym, tx, ty, tz, 2);
+                if (k64 >= bestHint(&best[c]))
+                    continue;
+                ++cnt.verifications;
+                // Bitwise verification of the derived transform.
+                const uint64_t T = translate4(mirror4(Q, sym), tx, ty, tz);
+                if (T == sortedMasks[c]) {
+                    ++cnt.verified;
+                    atomicMin((unsigned long long*)&best[c], (unsigned long long)k64);
                 }
             }
-            s = (s + 1) & capMask;
-            k = tabKeys[s];
+            s = (s + (uint32_t)kWalkSlots2) & capMask;
         }
     }
 }
@@ -815,9 +834,6 @@ __device__ __forceinline__ bool verifyWave(const uint64_t* __restrict__ layouts,
 // combo (pc, d), and hash[i] = its final nested hash (the planes are reused by the next combo).
 // `count` is uniform over the wave.
 constexpr uint32_t kProbeQueue = 64;
-// Table slots a walk reads per round trip (consecutive, issued together).
-constexpr int kWalkSlots = 4;
-
 struct ProbeList {
     uint32_t* meta;
     uint32_t* hash;
