@@ -52,7 +52,7 @@ template <int L>
 struct BlockTileLds {
     static constexpr size_t planes = alignUp16(rowsBytes<L>());
     static constexpr size_t pre = planes + (size_t)kBatchWaves * Geo<L>::P * Geo<L>::Slab * 4;
-    static constexpr size_t n1 = pre + (size_t)shapePreWords<L>() * 4;
+    static constexpr size_t n1 = alignUp16(pre + (size_t)shapePreBytes<L>());
     static constexpr size_t bits = n1 + (size_t)shapeN1Words<L>(kBatchWaves) * 4;
     static constexpr size_t bytes = bits + (size_t)shapeLdsWords<L>() * 4;
 };
@@ -63,7 +63,7 @@ struct Wave3Lds {
     static constexpr size_t planes0 = alignUp16(rowsBytes<3>());
     static constexpr size_t planes1 = planes0 + (size_t)Geo<3>::P * Geo<3>::Slab * 4;
     static constexpr size_t pre = planes1 + (size_t)Geo<3>::P * Geo<3>::Slab * 4;
-    static constexpr size_t n1 = pre + (size_t)shapePreWords<3>() * 4;
+    static constexpr size_t n1 = alignUp16(pre + (size_t)shapePreBytes<3>());
     static constexpr size_t bits = n1 + (size_t)shapeN1Words<3>(1) * 4;
     static constexpr size_t bytes = alignUp16(bits + (size_t)shapeLdsWords<3>() * 4);
 };
@@ -124,7 +124,7 @@ __device__ __forceinline__ void blockTile(const LevelView& v, uint32_t r, uint32
     ProbeCounts cnt;
     const int wave = (int)(threadX() >> 6);
     if (shapeOn(v, cfgBits)) {
-        uint32_t* s_pre = reinterpret_cast<uint32_t*>(s_mem + Lds::pre);
+        auto* s_pre = reinterpret_cast<typename G::Row*>(s_mem + Lds::pre);
         uint32_t* s_bits = reinterpret_cast<uint32_t*>(s_mem + Lds::bits);
         buildShapeTables<L>(s_rows, v.shape, s_bits, s_pre, (int)threadX(), kBatchThreads);
         __syncthreads();
@@ -156,7 +156,7 @@ __device__ __forceinline__ void waveItem3(const LevelView& v, uint32_t r, uint32
         return;
     ProbeCounts cnt;
     if (shapeOn(v, cfgBits)) {
-        uint32_t* s_pre = reinterpret_cast<uint32_t*>(base + Wave3Lds::pre);
+        auto* s_pre = reinterpret_cast<G::Row*>(base + Wave3Lds::pre);
         uint32_t* s_bits = reinterpret_cast<uint32_t*>(base + Wave3Lds::bits);
         buildShapeTables<3>(s_rows, v.shape, s_bits, s_pre, (int)(threadX() & 63u), 64);
         waveSync();

@@ -414,7 +414,7 @@ __global__ __launch_bounds__(64 * Waves) void kSearch(const uint64_t* __restrict
     __shared__ uint32_t s_qMeta[Waves][kProbeQueue];
     __shared__ uint32_t s_qHash[Waves][kProbeQueue];
     constexpr bool ShapeOn = shapeFits<L, Waves, Kind>();
-    __shared__ uint32_t s_pre[ShapeOn ? 2 * G::R * G::Pitch : 1];
+    __shared__ typename G::Row s_pre[ShapeOn ? 2 * G::R * G::Pitch : 1];
     __shared__ uint32_t s_n1[ShapeOn ? Waves * G::P * G::R : 1];
     __shared__ uint32_t s_shape[ShapeOn ? shapeLdsWords<L>() : 1];
     ProbeCounts cnt;

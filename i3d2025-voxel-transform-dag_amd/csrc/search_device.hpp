@@ -491,16 +491,17 @@ __device__ __forceinline__ uint32_t spanCompl(uint32_t x)
 
 // LDS views of the shape pre-filter of one block (all null: the filter is off).
 struct ShapeLds {
-    const uint32_t* pre = nullptr; // [2][R * Pitch]: OR over a1 in [0, c1] (index 0) / [c1, R) (index 1) of row (a1, a2), at c1 * Pitch + a2
+    const void* pre = nullptr; // Geo<L>::Row [2][R * Pitch]: OR over a1 in [0, c1] (index 0) / [c1, R) (index 1) of row (a1, a2), at c1 * Pitch + a2
     uint32_t* n1 = nullptr; // [Waves][P][R]: for plane w of a wave and line a2, bit k set iff pass-2 step k read a non-empty segment
     const uint32_t* bits = nullptr; // shapeLdsWords<L>() words
 };
 
-// LDS words of the pre tables and of the n1 masks of `waves` waves.
+// LDS bytes of the pre tables (in the row type: an OR of rows fits it, so level 4 needs half and
+// level 3 a quarter of 32-bit entries) and words of the n1 masks of `waves` waves.
 template <int L>
-constexpr int shapePreWords()
+constexpr int shapePreBytes()
 {
-    return 2 * Geo<L>::R * Geo<L>::Pitch;
+    return 2 * Geo<L>::R * Geo<L>::Pitch * (int)sizeof(typename Geo<L>::Row);
 }
 template <int L>
 constexpr int shapeN1Words(int waves)
@@ -513,7 +514,7 @@ constexpr int shapeN1Words(int waves)
 // s_rows. Threads tid = 0..threads-1 of the caller share the work; the caller synchronises
 // before the tables are read.
 template <int L>
-__device__ __forceinline__ void buildShapeTables(const typename Geo<L>::Row* s_rows, const uint32_t* __restrict__ shapeBits, uint32_t* s_shape, uint32_t* s_pre,
+__device__ __forceinline__ void buildShapeTables(const typename Geo<L>::Row* s_rows, const uint32_t* __restrict__ shapeBits, uint32_t* s_shape, typename Geo<L>::Row* s_pre,
     int tid, int threads)
 {
     using G = Geo<L>;
@@ -527,7 +528,7 @@ __device__ __forceinline__ void buildShapeTables(const typename Geo<L>::Row* s_r
         for (int k = 0; k < G::R; ++k) {
             const int a1 = dy ? G::R - 1 - k : k;
             acc |= (uint32_t)s_rows[a1 + G::Pitch * a2];
-            s_pre[dy * G::R * G::Pitch + a1 * G::Pitch + a2] = acc;
+            s_pre[dy * G::R * G::Pitch + a1 * G::Pitch + a2] = (typename G::Row)acc;
         }
     }
 }
@@ -698,7 +699,7 @@ __device__ __forceinline__ uint32_t lshlOr(uint32_t x, int k, uint32_t acc)
 // load no filter word.
 template <int L, bool Pos2, bool Shape = false>
 __device__ __forceinline__ uint32_t pass3Filter(const uint32_t* in, uint32_t* out, const uint32_t* __restrict__ bloom, uint32_t logBloom, uint32_t cfgBits,
-    int lead, ProbeCounts& cnt, const uint32_t* preLine = nullptr, const uint32_t* n1Plane = nullptr, const uint32_t* shapeBits = nullptr, uint32_t mask0 = 0u,
+    int lead, ProbeCounts& cnt, const typename Geo<L>::Row* preLine = nullptr, const uint32_t* n1Plane = nullptr, const uint32_t* shapeBits = nullptr, uint32_t mask0 = 0u,
     uint32_t mask1 = 0u)
 {
     constexpr int R = Geo<L>::R;
@@ -737,7 +738,7 @@ __device__ __forceinline__ uint32_t pass3Filter(const uint32_t* in, uint32_t* ou
             if constexpr (Shape) {
                 const int kk = k0 + k;
                 const int a2 = Pos2 ? (R - 1 - kk) : kk;
-                occ0 |= preLine[a2] & mask0;
+                occ0 |= (uint32_t)preLine[a2] & mask0;
                 occ1 |= n1Plane[a2] & mask1;
                 // Selects between registers only (a select between two constants would keep one
                 // of them in a register of its own for every step of the unrolled sweep).
@@ -1000,7 +1001,7 @@ __device__ __forceinline__ void probePending(uint32_t pending, ProbeList& list, 
 template <int L, bool Shape = false>
 __device__ __forceinline__ void lastPassProbe(uint32_t dz, int lead, const uint32_t* in, uint32_t* out, const uint32_t* outWave, ProbeList& list, int c0Base,
     const uint64_t* __restrict__ layouts, uint32_t n, const uint64_t* __restrict__ table, uint32_t logCap, const uint32_t* __restrict__ bloom,
-    uint32_t logBloom, uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t r, uint32_t pc, uint32_t d, ProbeCounts& cnt, const uint32_t* preLine = nullptr,
+    uint32_t logBloom, uint64_t* __restrict__ best, uint32_t cfgBits, uint32_t r, uint32_t pc, uint32_t d, ProbeCounts& cnt, const typename Geo<L>::Row* preLine = nullptr,
     const uint32_t* n1Plane = nullptr, const uint32_t* shapeBits = nullptr, uint32_t mask0 = 0u, uint32_t mask1 = 0u)
 {
     const uint32_t pending = dz ? pass3Filter<L, true, Shape>(in, out, bloom, logBloom, cfgBits, lead, cnt, preLine, n1Plane, shapeBits, mask0, mask1)
@@ -1034,7 +1035,7 @@ constexpr size_t searchLdsBase()
 template <int L, int Waves>
 constexpr size_t shapeLds()
 {
-    return (size_t)2 * Geo<L>::R * Geo<L>::Pitch * 4 + (size_t)Waves * Geo<L>::P * Geo<L>::R * 4 + (size_t)shapeLdsWords<L>() * 4;
+    return (size_t)shapePreBytes<L>() + (size_t)Waves * Geo<L>::P * Geo<L>::R * 4 + (size_t)shapeLdsWords<L>() * 4;
 }
 template <int L, int Waves, int Kind>
 constexpr bool shapeFits()
@@ -1094,7 +1095,8 @@ __device__ __forceinline__ void searchUnit(uint32_t r, uint32_t unit, uint32_t c
         mask0 = dx ? (0xFFFFFFFFu << c0) : lowMask((uint32_t)c0);
     }
     // The pre table row of the lane's c1 and its A1 range in pass-2 steps, for A1 direction dy.
-    auto preLineOf = [&](uint32_t dy) { return Shape ? shape.pre + dy * (uint32_t)(R * G::Pitch) + (uint32_t)(u * G::Pitch) : nullptr; };
+    const auto* pre = static_cast<const typename G::Row*>(shape.pre);
+    auto preLineOf = [&](uint32_t dy) { return Shape ? pre + dy * (uint32_t)(R * G::Pitch) + (uint32_t)(u * G::Pitch) : nullptr; };
     auto mask1Of = [&](uint32_t dy) { return Shape ? lowMask(dy ? (uint32_t)(R - 1 - u) : (uint32_t)u) : 0u; };
     if constexpr (Kind == kUnitCombo) {
         const uint32_t pc = unit >> 3, d = unit & 7u;
@@ -1116,7 +1118,7 @@ __device__ __forceinline__ void searchUnit(uint32_t r, uint32_t unit, uint32_t c
 #pragma unroll 1
             for (uint32_t dy = 0; dy < 2; ++dy) {
                 const int lead2 = emptyLead<L>(occ1, dy);
-                const uint32_t* preLine = preLineOf(dy);
+                const typename G::Row* preLine = preLineOf(dy);
                 const uint32_t mask1 = mask1Of(dy);
                 if constexpr (Kind == kUnitShared) {
                     waveSync();
