@@ -880,6 +880,17 @@ __device__ __forceinline__ void probeQueue(const ProbeList& list, int c0Base, co
 #pragma unroll
                 for (int k = 0; k < kWalkSlots; ++k)
                     ev[k] = table[(slot + (uint32_t)k) & capMask];
+                // Levels 4 and 5: best[] of every key hit among them, the loads issued together (a
+                // hit past the end of the run loads a value that is never used). At level 3 the
+                // registers would cost a wave per SIMD.
+                uint64_t bv[kWalkSlots];
+                if constexpr (L >= 4) {
+#pragma unroll
+                    for (int k = 0; k < kWalkSlots; ++k) {
+                        const uint32_t ck = (uint32_t)ev[k] >> 6;
+                        bv[k] = ((uint32_t)(ev[k] >> 32) == h && ck > r && ck < n) ? bestHint(&best[ck]) : 0ull;
+                    }
+                }
 #pragma unroll
                 for (int k = 0; k < kWalkSlots; ++k) {
                     const uint64_t e = ev[k];
@@ -923,7 +934,7 @@ __device__ __forceinline__ void probeQueue(const ProbeList& list, int c0Base, co
                         continue;
                     ++cnt.candidates;
                     const uint64_t k64 = encodeKey(r, tc, sym, tt[0], tt[1], tt[2], L);
-                    if (k64 >= bestHint(&best[cc]))
+                    if (k64 >= (L >= 4 ? bv[k] : bestHint(&best[cc])))
                         continue;
                     ++cnt.verifications;
                     found = true;
