@@ -883,7 +883,7 @@ __device__ __forceinline__ void probeQueue(const ProbeList& list, int c0Base, co
                 // Levels 4 and 5: best[] of every key hit among them, the loads issued together (a
                 // hit past the end of the run loads a value that is never used). At level 3 the
                 // registers would cost a wave per SIMD.
-                uint64_t bv[kWalkSlots];
+                uint64_t bv[kWalkSlots] = {};
                 if constexpr (L >= 4) {
 #pragma unroll
                     for (int k = 0; k < kWalkSlots; ++k) {
@@ -934,7 +934,12 @@ __device__ __forceinline__ void probeQueue(const ProbeList& list, int c0Base, co
                         continue;
                     ++cnt.candidates;
                     const uint64_t k64 = encodeKey(r, tc, sym, tt[0], tt[1], tt[2], L);
-                    if (k64 >= (L >= 4 ? bv[k] : bestHint(&best[cc])))
+                    uint64_t bestNow;
+                    if constexpr (L >= 4)
+                        bestNow = bv[k];
+                    else
+                        bestNow = bestHint(&best[cc]);
+                    if (k64 >= bestNow)
                         continue;
                     ++cnt.verifications;
                     found = true;
