@@ -468,8 +468,28 @@ tdag_status tdag_autotune_level(tdag_context* ctx, uint32_t level, const uint64_
                 bestBits = bitsCands[bi];
             }
         }
+        // 4. The batch growth (each batch this many times the previous one), jointly with the
+        //    first batch around the one found: fewer, larger batches prune the table less often
+        //    (a prune sweeps the whole table) but leave more settled children in it for longer.
+        uint32_t bestGrowth = 2;
+        const uint32_t growthCands[5] = { 3, 4, 6, 8, 12 };
+        const uint32_t firstAround[3] = { std::max<uint32_t>(1u, bestFirst / 2), bestFirst, (uint32_t)std::min<uint64_t>(n, 2ull * bestFirst) };
+        for (uint32_t gi = 0; gi < 5; ++gi) {
+            for (uint32_t fi = 0; fi < 3; ++fi) {
+                float avg = 0.f;
+                if (tdag_status s = timePlan(ctx, tdag::planMatch(level, n, (uint32_t)best, firstAround[fi], growthCands[gi], bestBits, bestFrames), d_masks, cfgBits, 0,
+                        repeats, &avg);
+                    s != TDAG_OK)
+                    return s;
+                if (avg < bestMs) {
+                    bestMs = avg;
+                    bestGrowth = growthCands[gi];
+                    bestFirst = firstAround[fi];
+                }
+            }
+        }
         ctx->firstBatch[level] = bestFirst;
-        ctx->batchGrowth[level] = 2;
+        ctx->batchGrowth[level] = bestGrowth;
         ctx->bloomBits[level] = bestBits;
         if (out_ms)
             out_ms[kVariantSlots] = bestMs;

@@ -380,7 +380,8 @@ class Context:
     def autotune(self, level: int, d_masks, n: int, config: TransformDAGConfig | None = None, repeats: int = 2):
         """Times the whole matcher call for every valid search variant (levels 3..5: with 1 and 6
         child frames), then four first-batch sizes with each of the three fastest variants, then
-        four filter sizes, on device masks, and keeps the fastest for this context; returns
+        four filter sizes, then batch growths 3 to 12 around the first batch found, on device
+        masks, and keeps the fastest for this context; returns
         (best_variant, first_batch, {variant: call ms}, tuned call ms)."""
         cfg = (config or TransformDAGConfig()).to_c()
         ms = (ctypes.c_float * (VARIANT_SLOTS + 1))()

@@ -183,9 +183,11 @@ tdag_status tdag_context_get_child_frames(const tdag_context* ctx, uint32_t leve
  * default schedule (after one untimed warm-up run each; at levels 3..5 for both child-frame
  * counts, keeping the one with the fastest variant), then tries four first-batch sizes
  * (n/256, n/64, n/16, n) with each of the three fastest variants and keeps the fastest pair,
- * then filter sizes of 4, 8, 16 and 32 bits per table entry; the winners become this context's
- * settings for `level`. Every trial times the whole matcher call (preparation, search,
- * post-processing), since the frame count and the filter size also change the table build.
+ * then filter sizes of 4, 8, 16 and 32 bits per table entry, then batch growths of 3, 4, 6, 8
+ * and 12 (each batch that many times the previous one; the default is 2) with the first batch
+ * halved, kept and doubled; the winners become this context's settings for `level`. Every trial
+ * times the whole matcher call (preparation, search, post-processing), since the frame count and
+ * the filter size also change the table build.
  * out_ms (TDAG_VARIANT_SLOTS + 1 floats, may be NULL) receives the average call time per variant
  * at the default schedule (-1 for invalid or unused slots) followed by the tuned time; also
  * writes the winning variant and first batch. No reference counterpart. */
