@@ -476,9 +476,15 @@ __global__ __launch_bounds__(256) void kPruneSplit(const uint64_t* __restrict__ 
     pruneSplit(tabKeys, tabVals, cap, best, minKey, bloom, logBloom, (uint64_t)blockIdx.x * 256 + threadIdx.x, (uint64_t)gridDim.x * 256);
 }
 
+// Blocks of 256 threads for a sweep of cap slots, at most 8192 (the sweeps stride over the rest):
+// one slot per thread for the split table, four for the packed one.
 unsigned pruneBlocks(uint64_t cap)
 {
     return (unsigned)std::min<uint64_t>(ceilDiv(cap, 256), 8192);
+}
+unsigned pruneBlocksPacked(uint64_t cap)
+{
+    return (unsigned)std::min<uint64_t>(ceilDiv(cap, 1024), 8192);
 }
 
 // The table's entries, then its Bloom filter: a prune pass with key 0 keeps every entry and sets
@@ -490,7 +496,7 @@ void launchCornerTable(const MatchPlan& p, const LevelView& v, const uint32_t* c
     const uint32_t n = (uint32_t)p.n;
     hipLaunchKernelGGL(kCornerTable<L>, dim3(n, p.childFrames), dim3(64), 0, stream, v.sortedMasks, n, v.tabKeys, p.logCap, childFlags, v.tickets);
     const uint64_t cap = 1ull << p.logCap;
-    hipLaunchKernelGGL(kPrunePacked, dim3(pruneBlocks(cap)), dim3(256), 0, stream, v.tabKeys, cap, v.best, 0ull, v.bloom, p.logBloom);
+    hipLaunchKernelGGL(kPrunePacked, dim3(pruneBlocksPacked(cap)), dim3(256), 0, stream, v.tabKeys, cap, v.best, 0ull, v.bloom, p.logBloom);
 }
 
 
@@ -867,12 +873,11 @@ hipError_t enqueueSearch(const MatchPlan& p, uint32_t cfgBitsIn, void* d_scratch
         r0 = r1;
         if (r0 < n) {
             const uint64_t minKey = pruneKey(p.level, r0);
-            const unsigned blocks = pruneBlocks(cap);
             TDAG_TRY(fillU32(v.bloom, 1ull << p.logBloom, 0u, stream));
             if (p.level == 2)
-                hipLaunchKernelGGL(kPruneSplit, dim3(blocks), dim3(256), 0, stream, v.tabKeys, v.tabVals, cap, v.best, minKey, v.bloom, p.logBloom);
+                hipLaunchKernelGGL(kPruneSplit, dim3(pruneBlocks(cap)), dim3(256), 0, stream, v.tabKeys, v.tabVals, cap, v.best, minKey, v.bloom, p.logBloom);
             else
-                hipLaunchKernelGGL(kPrunePacked, dim3(blocks), dim3(256), 0, stream, v.tabKeys, cap, v.best, minKey, v.bloom, p.logBloom);
+                hipLaunchKernelGGL(kPrunePacked, dim3(pruneBlocksPacked(cap)), dim3(256), 0, stream, v.tabKeys, cap, v.best, minKey, v.bloom, p.logBloom);
             TDAG_TRY(hipGetLastError());
         }
     }

@@ -1174,21 +1174,61 @@ __device__ __forceinline__ void searchUnit(uint32_t r, uint32_t unit, uint32_t c
 // slots hold few distinct filter words: the wave ORs the bits of equal words together (a scan
 // over the lanes whose combining steps join equal words only) and the last lane of each run of
 // equal words sets them with one atomic.
+// One sweep of the packed table: entries whose child settled below minKey become tombstones (the
+// hash stays, so walks still step over them), and the Bloom filter (cleared by the caller) gets
+// the bits of every live entry. Each lane takes 4 consecutive slots (two 16-byte loads), so a
+// wave covers 256 slots per step with four times the bytes in flight of one slot per lane. The
+// slots run in filter-word order: a lane ORs its slots' bits per run of equal words, runs that
+// end inside the lane go out at once, and the lane's last run joins a segmented OR scan across
+// the wave, one atomic per run. A run split over two lanes costs one extra atomic, no lost bits:
+// the scan only ever adds bits of the same word. minKey 0 settles nothing, so that sweep reads
+// no best[]. start and stride count 4-slot groups; stride is a multiple of 64 and start lies in
+// lane (start & 63) of its wave, so whole waves enter each step and the shuffles see every lane.
 __device__ __forceinline__ void prunePacked(uint64_t* __restrict__ table, uint64_t cap, const uint64_t* __restrict__ best, uint64_t minKey, uint32_t* __restrict__ bloom,
     uint32_t logBloom, uint64_t start, uint64_t stride)
 {
+    constexpr uint32_t kNone = 0xFFFFFFFFu; // no word: filter words stay below 2^22
     const int lane = (int)(threadX() & 63u);
-    for (uint64_t s = start; s < cap; s += stride) {
-        const uint64_t e = table[s];
-        const uint32_t c = (uint32_t)e >> 6;
-        const bool entry = e != 0ull && c != kPackedDead;
-        const bool dead = entry && best[c] < minKey;
-        if (dead)
-            table[s] = (e & 0xFFFFFFFF0000003Full) | ((uint64_t)kPackedDead << 6);
-        const uint32_t h = (uint32_t)(e >> 32);
-        const bool live = entry && !dead;
-        const uint32_t word = live ? bloomWordNested(h, logBloom) : 0xFFFFFFFFu;
-        uint32_t bits = live ? bloomBits(h) : 0u;
+    const uint64_t groups = cap >> 2; // cap is a power of two >= 64
+    for (uint64_t g = start; (g & ~63ull) < groups; g += stride) {
+        uint64_t e[4] = {0ull, 0ull, 0ull, 0ull};
+        if (g < groups) {
+            const ulonglong2* quad = reinterpret_cast<const ulonglong2*>(table + 4 * g);
+            const ulonglong2 lo = quad[0];
+            const ulonglong2 hi = quad[1];
+            e[0] = lo.x;
+            e[1] = lo.y;
+            e[2] = hi.x;
+            e[3] = hi.y;
+        }
+        bool live[4];
+        uint64_t bk[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t c = (uint32_t)e[j] >> 6;
+            live[j] = e[j] != 0ull && c != kPackedDead;
+            bk[j] = (live[j] && minKey != 0ull) ? best[c] : ~0ull;
+        }
+        uint32_t word = kNone;
+        uint32_t bits = 0u;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (live[j] && bk[j] < minKey) {
+                table[4 * g + j] = (e[j] & 0xFFFFFFFF0000003Full) | ((uint64_t)kPackedDead << 6);
+                live[j] = false;
+            }
+            if (live[j]) {
+                const uint32_t h = (uint32_t)(e[j] >> 32);
+                const uint32_t w = bloomWordNested(h, logBloom);
+                if (w != word) {
+                    if (word != kNone)
+                        atomicOr(&bloom[word], bits);
+                    word = w;
+                    bits = 0u;
+                }
+                bits |= bloomBits(h);
+            }
+        }
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
             const uint32_t w = (uint32_t)__shfl_up((int)word, off, 64);
@@ -1197,7 +1237,7 @@ __device__ __forceinline__ void prunePacked(uint64_t* __restrict__ table, uint64
                 bits |= b;
         }
         const uint32_t next = (uint32_t)__shfl_down((int)word, 1, 64);
-        if (live && (lane == 63 || next != word))
+        if (word != kNone && (lane == 63 || next != word))
             atomicOr(&bloom[word], bits);
     }
 }
