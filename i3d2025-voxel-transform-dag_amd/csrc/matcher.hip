@@ -214,8 +214,12 @@ __global__ __launch_bounds__(256) void kLayouts(uint64_t* __restrict__ layouts, 
 // turn, reads an item's layout-0 rows, derives its occupancy along x, y and z, and sets the bits
 // of its extent triple in all six axis orders in the block's LDS copy, which is then ORed into
 // the global bitmap (one atomic per non-zero word and block, instead of six per item on a few
-// hundred words). Empty and unflagged items set nothing.
-constexpr uint32_t kShapeBlocks = 2048; // 8192 waves: one pass of resident waves, a few items each
+// hundred words). Empty and unflagged items set nothing. The flush's atomics all land on those
+// few hundred words, so each block costs a fixed share of time: level 5 (4 KiB per item) wants
+// the waves, levels 4 and 3 fewer blocks (city, us per call at 512 / 1024 / 2048 / 4096 blocks:
+// level 5 155 / 89 / 67 / 75, level 4 54 / 42 / 46 / 63, level 3 30 / 31 / 45 / 77).
+template <int L>
+constexpr uint32_t kShapeBlocks = L == 5 ? 2048 : 1024;
 template <int L>
 __global__ __launch_bounds__(256) void kShapeBits(const uint64_t* __restrict__ layouts, uint32_t n, uint32_t* __restrict__ bits, const uint32_t* __restrict__ childFlags)
 {
@@ -812,21 +816,21 @@ hipError_t enqueueBuildTable(const MatchPlan& p, void* d_scratch, hipStream_t st
         TDAG_TRY(fillU32(v.tickets, cap, 0u, stream));
         launchCornerTable<3>(p, v, childFlags, stream);
         TDAG_TRY(fillU32(v.shape, shapeWords<3>(), 0u, stream));
-        hipLaunchKernelGGL(kShapeBits<3>, dim3(std::min<uint32_t>(kShapeBlocks, (n + 3) / 4)), dim3(256), 0, stream, v.sortedMasks, n, v.shape, childFlags);
+        hipLaunchKernelGGL(kShapeBits<3>, dim3(std::min<uint32_t>(kShapeBlocks<3>, (n + 3) / 4)), dim3(256), 0, stream, v.sortedMasks, n, v.shape, childFlags);
         break;
     case 4:
         hipLaunchKernelGGL(kLayouts<4>, dim3((unsigned)ceilDiv((size_t)n * 2 * 16, 256)), dim3(256), 0, stream, v.sortedMasks, n);
         TDAG_TRY(fillU32(v.tickets, cap, 0u, stream));
         launchCornerTable<4>(p, v, childFlags, stream);
         TDAG_TRY(fillU32(v.shape, shapeWords<4>(), 0u, stream));
-        hipLaunchKernelGGL(kShapeBits<4>, dim3(std::min<uint32_t>(kShapeBlocks, (n + 3) / 4)), dim3(256), 0, stream, v.sortedMasks, n, v.shape, childFlags);
+        hipLaunchKernelGGL(kShapeBits<4>, dim3(std::min<uint32_t>(kShapeBlocks<4>, (n + 3) / 4)), dim3(256), 0, stream, v.sortedMasks, n, v.shape, childFlags);
         break;
     case 5:
         hipLaunchKernelGGL(kLayouts<5>, dim3((unsigned)ceilDiv((size_t)n * 2 * 32, 256)), dim3(256), 0, stream, v.sortedMasks, n);
         TDAG_TRY(fillU32(v.tickets, cap, 0u, stream));
         launchCornerTable<5>(p, v, childFlags, stream);
         TDAG_TRY(fillU32(v.shape, shapeWords<5>(), 0u, stream));
-        hipLaunchKernelGGL(kShapeBits<5>, dim3(std::min<uint32_t>(kShapeBlocks, (n + 3) / 4)), dim3(256), 0, stream, v.sortedMasks, n, v.shape, childFlags);
+        hipLaunchKernelGGL(kShapeBits<5>, dim3(std::min<uint32_t>(kShapeBlocks<5>, (n + 3) / 4)), dim3(256), 0, stream, v.sortedMasks, n, v.shape, childFlags);
         break;
     default:
         return hipErrorInvalidValue;
